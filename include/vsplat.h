@@ -1,0 +1,199 @@
+/* vsplat: an MI355X-native differentiable Gaussian-splat tile rasterizer.
+ *
+ * The C interface of the trainer's rasterizer path. It replaces the reference's
+ * `class Trainer` (src/train/trainer.hpp:89-270) behind plain pointers and
+ * sizes; include/vsplat_trainer.hpp wraps it back into that class shape with
+ * the reference's exception types.
+ *
+ * Conventions: every function returning int returns VSPLAT_OK (0) or a
+ * negative status; vsplat_last_error() then holds a message for the calling
+ * thread. Host arrays are borrowed for the duration of the call unless noted.
+ * A trainer is used by one thread at a time; each call that returns results is
+ * synchronous, and the vsplat_enqueue_* calls only queue work on the trainer's
+ * stream (vsplat_synchronize waits for it).
+ */
+#ifndef VSPLAT_H
+#define VSPLAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Statuses, and the reference exception each one stands for. */
+#define VSPLAT_OK 0
+#define VSPLAT_RUNTIME_ERROR (-1)    /* std::runtime_error: empty cloud, bad image size, target/camera mismatch (trainer.cpp:174, 276, 571) */
+#define VSPLAT_BUDGET_EXCEEDED (-2)  /* splat::BudgetExceeded, recoverable (trainer.hpp:46-49, trainer.cpp:359) */
+#define VSPLAT_INVALID_ARGUMENT (-3) /* std::invalid_argument (trainer.cpp:802), and null or malformed arguments */
+#define VSPLAT_LOGIC_ERROR (-4)      /* std::logic_error: gradients before compute_gradients (trainer.cpp:853) */
+#define VSPLAT_DEVICE_ERROR (-5)     /* a HIP call failed, or no GPU */
+#define VSPLAT_UNSUPPORTED (-6)      /* an option this build does not implement (TrainerOptions::positionNoise, poseGradients) */
+
+/* Trainer::kBytesPerTileEntry (trainer.hpp:194): entry keys, values and their sort copies. */
+#define VSPLAT_BYTES_PER_TILE_ENTRY 16u
+
+typedef struct vsplat_trainer vsplat_trainer;
+
+/* GaussianCloud (src/splat/gaussian_cloud.hpp:21-38): raw parameters, arrays of
+ * count*3, count*3, count*4, count and count*3*(degree+1)^2 floats. */
+typedef struct {
+  const float *positions;
+  const float *scales;    /* log standard deviations */
+  const float *rotations; /* quaternion w x y z, not necessarily normalised */
+  const float *opacities; /* logit */
+  const float *sh;        /* [Gaussian][coefficient][rgb] */
+  uint32_t count;
+  int32_t sh_degree; /* 0..3 */
+} vsplat_cloud;
+
+/* Camera (src/render/camera.hpp:9-18): view is world->camera, column-major as glm stores it. */
+typedef struct {
+  float view[16];
+  float position[3];
+  float fx, fy, cx, cy;
+  int32_t width, height;
+} vsplat_camera;
+
+/* TrainerOptions (src/train/trainer.hpp:24-42). */
+typedef struct {
+  float lambda_ssim;
+  float background[3];
+  float near_plane;
+  int32_t force_fixed_point; /* 64-bit fixed-point sums, deterministic; else float atomics */
+  int32_t selective_adam;
+  float opacity_reg, scale_reg;
+  float position_noise;   /* must be 0: VSPLAT_UNSUPPORTED otherwise */
+  int32_t mip_filter_2d;
+  int32_t pose_gradients; /* must be 0: VSPLAT_UNSUPPORTED otherwise */
+  uint64_t memory_budget; /* bytes; UINT64_MAX for none */
+} vsplat_trainer_options;
+
+/* LearningRates (src/train/trainer.hpp:51-61). trust_region must be 0 (first-order Adam only). */
+typedef struct {
+  float position, scale, rotation, opacity, sh, sh_rest_scale, trust_region;
+} vsplat_learning_rates;
+
+/* StepStats (src/train/trainer.hpp:63-73). The exposure and pose gradients stay zero. */
+typedef struct {
+  float loss, l1, ssim;
+  uint32_t visible_splats;
+  uint32_t tile_entries;
+  float exposure_gradient[12];
+  float pose_gradient[6];
+} vsplat_step_stats;
+
+/* One stage of Trainer::timings() (trainer.hpp:186): average ms per timed iteration. */
+typedef struct {
+  char name[32];
+  double milliseconds;
+} vsplat_timing;
+
+/* The defaults of TrainerOptions and LearningRates. */
+void vsplat_default_options(vsplat_trainer_options *options);
+void vsplat_default_rates(vsplat_learning_rates *rates);
+
+/* Trainer::Trainer (trainer.hpp:91): copies the cloud to the GPU. options may be NULL. */
+int vsplat_trainer_create(const vsplat_cloud *cloud, const vsplat_trainer_options *options, vsplat_trainer **out);
+/* Trainer::~Trainer (trainer.hpp:92). NULL is ignored. */
+void vsplat_trainer_destroy(vsplat_trainer *trainer);
+
+/* Trainer::render (trainer.hpp:98): width*height*4 floats, rgb and log transmittance. */
+int vsplat_render(vsplat_trainer *trainer, const vsplat_camera *camera, float *out_rgba);
+/* Trainer::compute_gradients (trainer.hpp:104): forward, loss, backward; the
+ * gradients stay on the GPU for vsplat_get_gradients and vsplat_apply_gradients.
+ * target is RGBA8, width*height*4 bytes. stats may be NULL. */
+int vsplat_compute_gradients(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *target, size_t target_bytes,
+                             vsplat_step_stats *stats);
+/* Trainer::collect_statistics (trainer.hpp:106): densification statistics only. */
+int vsplat_collect_statistics(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *target, size_t target_bytes,
+                              vsplat_step_stats *stats);
+/* Trainer::apply_gradients (trainer.hpp:121): one Adam step from the last compute_gradients. */
+int vsplat_apply_gradients(vsplat_trainer *trainer, const vsplat_learning_rates *rates);
+/* Trainer::step (trainer.hpp:125-126): compute_gradients and the Adam step in one
+ * pass. next_target (may be NULL) is the following step's image: it is copied
+ * to staging memory on a worker thread and must stay valid until the next
+ * vsplat_step or vsplat_compute_gradients. */
+int vsplat_step(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *target, size_t target_bytes,
+                const vsplat_learning_rates *rates, const uint8_t *next_target, size_t next_target_bytes, vsplat_step_stats *stats);
+/* Trainer::prefetch_target (trainer.hpp:129). The same borrowing rule as next_target. */
+int vsplat_prefetch_target(vsplat_trainer *trainer, const uint8_t *target, size_t target_bytes);
+
+/* Trainer::gradients (trainer.hpp:141-144): fp32 per parameter, laid out as the
+ * cloud. VSPLAT_LOGIC_ERROR before any compute_gradients. Any pointer may be NULL. */
+int vsplat_get_gradients(vsplat_trainer *trainer, float *positions, float *scales, float *rotations, float *opacities, float *sh);
+/* Trainer::cloud (trainer.hpp:131), into caller arrays sized as the cloud. */
+int vsplat_get_cloud(vsplat_trainer *trainer, float *positions, float *scales, float *rotations, float *opacities, float *sh);
+/* Trainer::densify_stats (trainer.hpp:176): 8 floats per Gaussian, the
+ * DensifyStat of train_types.glsl:18-21 (gradient sum, visits, largest radius,
+ * contribution, absolute gradient sum, marked pixels, marked loss, edge score). */
+int vsplat_get_densify_stats(vsplat_trainer *trainer, float *out);
+/* Trainer::reset_densify_stats (trainer.hpp:178). */
+int vsplat_reset_densify_stats(vsplat_trainer *trainer);
+
+/* Trainer::set_active_sh_degree (trainer.hpp:151), clamped to [0, degree]. */
+int vsplat_set_active_sh_degree(vsplat_trainer *trainer, int32_t degree);
+/* Trainer::set_filter3d (trainer.hpp:168): one value per Gaussian, else VSPLAT_INVALID_ARGUMENT. */
+int vsplat_set_filter3d(vsplat_trainer *trainer, const float *filter, size_t count);
+/* Trainer::set_sh_dropout (trainer.hpp:171). */
+int vsplat_set_sh_dropout(vsplat_trainer *trainer, float probability, int32_t retained_degree, uint32_t seed);
+
+/* Trainer::parameters (trainer.hpp:137): the device buffer of group 0 positions,
+ * 1 scales, 2 rotations, 3 opacities, 4 SH (packed as half_sh_words describes).
+ * Valid until the trainer is destroyed. */
+int vsplat_parameters(vsplat_trainer *trainer, uint32_t group, void **device_pointer, uint64_t *bytes);
+
+/* Trainer accessors (trainer.hpp:138-146, 187-193). */
+uint32_t vsplat_gaussian_count(const vsplat_trainer *trainer);
+int32_t vsplat_sh_degree(const vsplat_trainer *trainer);
+int32_t vsplat_active_sh_degree(const vsplat_trainer *trainer);
+int32_t vsplat_uses_fixed_point(const vsplat_trainer *trainer);
+uint64_t vsplat_gpu_bytes(const vsplat_trainer *trainer);
+uint64_t vsplat_bytes_per_gaussian(const vsplat_trainer *trainer);
+uint64_t vsplat_staging_bytes(const vsplat_trainer *trainer);
+/* Trainer::timings (trainer.hpp:186): up to capacity stages; *count gets how many exist. */
+int vsplat_timings(const vsplat_trainer *trainer, vsplat_timing *out, uint32_t capacity, uint32_t *count);
+
+/* The message of the calling thread's last failed call. */
+const char *vsplat_last_error(void);
+
+/* ---- Device-resident path: no reference counterpart; for callers that keep
+ * targets in GPU memory and do not want a host round trip per step. ---- */
+
+/* Queues compute_gradients (mode 1) or step (mode 2, rates required) against a
+ * target already in device memory (width*height*4 bytes, on this trainer's GPU). */
+int vsplat_enqueue(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *device_target, int32_t mode,
+                   const vsplat_learning_rates *rates);
+/* Waits for queued work; stats (may be NULL) of the last queued step. Returns
+ * VSPLAT_RUNTIME_ERROR if any queued view needed more tile entries than were
+ * reserved: that step computed nothing and changed nothing. vsplat_compute_gradients
+ * with the view, or vsplat_reserve_entries, grows the reservation. */
+int vsplat_synchronize(vsplat_trainer *trainer, vsplat_step_stats *stats);
+/* The trainer's HIP stream (a hipStream_t). */
+void *vsplat_stream(vsplat_trainer *trainer);
+/* Reserves room for this many tile entries now (subject to the memory budget). */
+int vsplat_reserve_entries(vsplat_trainer *trainer, uint64_t entries);
+
+/* ---- Introspection, for parity tests: no reference counterpart. ---- */
+
+/* Copies one of the trainer's intermediate device buffers, as the last call
+ * left it, to host memory: "splats" (48 bytes per Gaussian), "keys", "values"
+ * (depth-sorted, 4 bytes per Gaussian), "coverage", "entry_keys",
+ * "entry_values" (4 bytes per tile entry), "ranges" (8 bytes per tile),
+ * "image" (16 bytes per pixel: rgb, log T), "stops", "adjoint" (16 per pixel),
+ * "fields" (36 per pixel), "tile_totals" (8 per tile), "raw" (14 int64 per
+ * Gaussian with fixed point, else 14 float), "counters" (4 words). *bytes
+ * gets the size in use; at most `capacity` bytes are copied. */
+int vsplat_inspect(vsplat_trainer *trainer, const char *buffer, void *out, uint64_t capacity, uint64_t *bytes);
+
+/* Runs one of the device's elementwise functions over n floats (device
+ * arithmetic check): 0 exp, 1 log, 2 sqrt, 3 reciprocal (1/x), 4 float to half
+ * and back. in and out are host arrays. */
+int vsplat_math_probe(int32_t function, const float *in, float *out, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VSPLAT_H */

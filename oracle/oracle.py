@@ -1,0 +1,401 @@
+"""Test infrastructure: ctypes access to the two CPU oracles (liboracle.so).
+
+Only the tests, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+this module; the product never does. Build with ``make -C oracle``.
+
+Oracle A (``render``, ``loss``, ``gradients``, ``contributions``) is the
+double-precision restatement of the reference's src/train/reference.cpp.
+Oracle B (``Model``) is the single-precision model of the GPU pipeline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIB = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class CCloud(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("scales", C.c_void_p), ("rotations", C.c_void_p), ("opacities", C.c_void_p),
+                ("sh", C.c_void_p), ("count", C.c_uint32), ("sh_degree", C.c_int32)]
+
+
+class CCamera(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("position", C.c_float * 3), ("fx", C.c_float), ("fy", C.c_float),
+                ("cx", C.c_float), ("cy", C.c_float), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class COptions(C.Structure):
+    _fields_ = [("background", C.c_float * 3), ("near_plane", C.c_float), ("no_cutoffs", C.c_int32),
+                ("opacity_reg", C.c_float), ("scale_reg", C.c_float), ("mip_filter_2d", C.c_int32),
+                ("filter3d", C.c_void_p), ("filter3d_count", C.c_uint32)]
+
+
+class OBCamera(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("position", C.c_float * 3), ("near_plane", C.c_float), ("fx", C.c_float),
+                ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float), ("tan_half_x", C.c_float),
+                ("tan_half_y", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("count", C.c_uint32),
+                ("sh_degree", C.c_uint32), ("sh_words", C.c_uint32), ("sh_coefficients", C.c_uint32)]
+
+
+class OBPreOptions(C.Structure):
+    _fields_ = [("mip_filter_2d", C.c_int32), ("sh_dropout", C.c_float), ("sh_retained", C.c_uint32),
+                ("sh_seed", C.c_uint32)]
+
+
+class OBChainOptions(C.Structure):
+    _fields_ = [("opacity_reg", C.c_float), ("scale_reg", C.c_float), ("mip_filter_2d", C.c_int32),
+                ("sh_dropout", C.c_float), ("sh_retained", C.c_uint32), ("sh_seed", C.c_uint32),
+                ("mode", C.c_uint32), ("flags", C.c_uint32), ("rates", C.c_float * 5), ("rest_scale", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float), ("correction1", C.c_float),
+                ("correction2", C.c_float), ("seed", C.c_uint32), ("sh_floats", C.c_uint32)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            raise ImportError(f"{path} is not built: run make -C oracle")
+        _LIB = C.CDLL(str(path))
+        L = _LIB
+        vp = C.c_void_p
+        L.oa_render.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp]
+        L.oa_contributions.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.oa_loss.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp]
+        L.oa_gradients.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.ob_preprocess.argtypes = [vp] * 11
+        L.ob_stable_sort.argtypes = [vp, vp, C.c_uint32]
+        L.ob_bin.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, C.c_uint64, vp]
+        L.ob_tile_ranges.argtypes = [vp, C.c_uint64, C.c_uint32, vp]
+        L.ob_forward.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]
+        L.ob_loss.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_float, vp, vp, vp]
+        L.ob_backward_fixed.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_float, C.c_uint32, vp]
+        L.ob_backward_float.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_uint32, vp]
+        L.ob_fixed_to_float.argtypes = [vp, C.c_uint64, C.c_float, vp]
+        L.ob_chain.argtypes = [vp] * 21
+        L.ob_sh_step.argtypes = [vp] * 5
+        L.ob_exp_array.argtypes = [vp, vp, C.c_uint64]
+        L.ob_log_array.argtypes = [vp, vp, C.c_uint64]
+        L.ob_exp_array.restype = None
+        L.ob_log_array.restype = None
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(code, what):
+    if code != 0:
+        raise RuntimeError(f"{what} failed with {code}")
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+# ------------------------------------------------------------------- Oracle A
+
+@dataclass
+class RefOptions:
+    """ReferenceOptions (src/train/reference.hpp:18-33)."""
+
+    background: tuple = (0.0, 0.0, 0.0)
+    near_plane: float = 0.01
+    no_cutoffs: bool = False
+    opacity_reg: float = 0.0
+    scale_reg: float = 0.0
+    mip_filter_2d: bool = False
+    filter3d: np.ndarray | None = None
+
+
+class _Held:
+    """Keeps the arrays a C struct points into alive."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def keep(self, a):
+        self.arrays.append(a)
+        return a
+
+
+def _ccloud(cloud, held):
+    arrs = [held.keep(np.ascontiguousarray(a, np.float32)) for a in
+            (cloud.positions, cloud.scales, cloud.rotations, cloud.opacities, cloud.sh)]
+    return CCloud(*[_p(a) for a in arrs], cloud.count, cloud.sh_degree)
+
+
+def _ccamera(cam):
+    c = CCamera()
+    c.view[:] = [float(v) for v in cam.view_column_major()]
+    c.position[:] = [float(v) for v in np.asarray(cam.position, np.float32)]
+    c.fx, c.fy, c.cx, c.cy = cam.fx, cam.fy, cam.cx, cam.cy
+    c.width, c.height = cam.width, cam.height
+    return c
+
+
+def _coptions(o, held):
+    o = o or RefOptions()
+    c = COptions()
+    c.background[:] = list(o.background)
+    c.near_plane = o.near_plane
+    c.no_cutoffs = int(o.no_cutoffs)
+    c.opacity_reg = o.opacity_reg
+    c.scale_reg = o.scale_reg
+    c.mip_filter_2d = int(o.mip_filter_2d)
+    if o.filter3d is not None:
+        f = held.keep(np.ascontiguousarray(o.filter3d, np.float32))
+        c.filter3d = _p(f)
+        c.filter3d_count = f.size
+    return c
+
+
+@dataclass
+class RefImage:
+    width: int
+    height: int
+    rgb: np.ndarray  # (P, 3) float64
+    log_t: np.ndarray  # (P,)
+    stop: np.ndarray  # (P,) uint64
+
+
+def render(cloud, cam, options=None, threads=None) -> RefImage:
+    held = _Held()
+    P = cam.width * cam.height
+    rgb = np.zeros((P, 3)); log_t = np.zeros(P); stop = np.zeros(P, np.uint64)
+    cc, ca, co = _ccloud(cloud, held), _ccamera(cam), _coptions(options, held)
+    _check(lib().oa_render(C.byref(cc), C.byref(ca), C.byref(co), threads or default_threads(), _p(rgb), _p(log_t), _p(stop)),
+           "oa_render")
+    return RefImage(cam.width, cam.height, rgb, log_t, stop)
+
+
+def contributions(cloud, cam, options=None, threads=None) -> np.ndarray:
+    held = _Held()
+    out = np.zeros(cloud.count)
+    cc, ca, co = _ccloud(cloud, held), _ccamera(cam), _coptions(options, held)
+    _check(lib().oa_contributions(C.byref(cc), C.byref(ca), C.byref(co), threads or default_threads(), _p(out)),
+           "oa_contributions")
+    return out
+
+
+@dataclass
+class RefLoss:
+    loss: float
+    l1: float
+    ssim: float
+    adjoint: np.ndarray  # (P, 3)
+
+
+def loss(image: RefImage, target: np.ndarray, lam: float, threads=None) -> RefLoss:
+    rgb = np.ascontiguousarray(image.rgb, np.float64)
+    tgt = np.ascontiguousarray(target, np.uint8).reshape(-1)
+    adj = np.zeros_like(rgb)
+    stats = np.zeros(3)
+    _check(lib().oa_loss(_p(rgb), _p(tgt), image.width, image.height, lam, threads or default_threads(), _p(adj),
+                         _p(stats)), "oa_loss")
+    return RefLoss(float(stats[0]), float(stats[1]), float(stats[2]), adj)
+
+
+@dataclass
+class RefGradients:
+    positions: np.ndarray
+    scales: np.ndarray
+    rotations: np.ndarray
+    opacities: np.ndarray
+    sh: np.ndarray
+    absolute_centre: np.ndarray
+
+
+def gradients(cloud, cam, adjoint, options=None, threads=None) -> RefGradients:
+    held = _Held()
+    n = cloud.count
+    out = RefGradients(np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 4)), np.zeros(n),
+                       np.zeros(cloud.sh.shape), np.zeros((n, 2)))
+    adj = held.keep(np.ascontiguousarray(adjoint, np.float64))
+    cc, ca, co = _ccloud(cloud, held), _ccamera(cam), _coptions(options, held)
+    _check(lib().oa_gradients(C.byref(cc), C.byref(ca), _p(adj), C.byref(co), threads or default_threads(),
+                              _p(out.positions), _p(out.scales), _p(out.rotations), _p(out.opacities), _p(out.sh),
+                              _p(out.absolute_centre)), "oa_gradients")
+    return out
+
+
+# ------------------------------------------------------------------- Oracle B
+
+def exp32(x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().ob_exp_array(_p(x), _p(y), x.size)
+    return y
+
+
+def log32(x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().ob_log_array(_p(x), _p(y), x.size)
+    return y
+
+
+def ob_camera(cam, count, sh_degree, active_degree=None, near_plane=0.01):
+    """The per-view constants block, as the product's host code fills it."""
+    from vsplat.fixtures import half_sh_words  # noqa: WPS433 (tests put the package on sys.path)
+    active = sh_degree if active_degree is None else active_degree
+    c = OBCamera()
+    c.view[:] = [float(v) for v in cam.view_column_major()]
+    c.position[:] = [float(v) for v in np.asarray(cam.position, np.float32)]
+    c.near_plane = near_plane
+    c.fx, c.fy, c.cx, c.cy = cam.fx, cam.fy, cam.cx, cam.cy
+    c.tan_half_x = float(np.float32(0.5) * np.float32(cam.width) / np.float32(cam.fx))
+    c.tan_half_y = float(np.float32(0.5) * np.float32(cam.height) / np.float32(cam.fy))
+    c.width, c.height = cam.width, cam.height
+    c.count = count
+    c.sh_degree = active
+    c.sh_words = half_sh_words((sh_degree + 1) ** 2)
+    c.sh_coefficients = (active + 1) ** 2
+    return c
+
+
+@dataclass
+class ModelResult:
+    splats: np.ndarray  # (N, 12)
+    keys: np.ndarray  # depth-sorted keys
+    values: np.ndarray
+    coverage: np.ndarray
+    visible: int
+    entries: int
+    entry_keys: np.ndarray
+    entry_values: np.ndarray
+    ranges: np.ndarray  # (T, 2)
+    image: np.ndarray | None = None  # (P, 4)
+    stops: np.ndarray | None = None
+    fields: np.ndarray | None = None
+    adjoint: np.ndarray | None = None
+    tile_totals: np.ndarray | None = None
+    raw_fixed: np.ndarray | None = None
+    raw: np.ndarray | None = None
+
+
+class Model:
+    """Oracle B over one cloud and camera: each stage as the GPU runs it."""
+
+    def __init__(self, cloud, cam, *, near_plane=0.01, mip_filter_2d=False, filter3d=None, background=(0, 0, 0),
+                 active_degree=None, sh_dropout=(0.0, 0, 0)):
+        from vsplat.fixtures import pack_half_sh
+        self.cloud = cloud
+        self.cam = cam
+        self.n = cloud.count
+        self.words = pack_half_sh(cloud.sh)
+        self.camera = ob_camera(cam, self.n, cloud.sh_degree, active_degree, near_plane)
+        self.pre = OBPreOptions(int(mip_filter_2d), float(sh_dropout[0]), int(sh_dropout[1]), int(sh_dropout[2]))
+        self.mip = mip_filter_2d
+        self.filter3d = None if filter3d is None else np.ascontiguousarray(filter3d, np.float32)
+        self.background = np.asarray(background, np.float32)
+        self.grid_w = (cam.width + 15) // 16
+        self.grid_h = (cam.height + 15) // 16
+        self.tiles = self.grid_w * self.grid_h
+
+    def fixed_scale(self):
+        return float(np.float32(3.0) * np.float32(self.cam.width) * np.float32(self.cam.height) * np.float32(64.0))
+
+    def forward(self) -> ModelResult:
+        L = lib()
+        n = self.n
+        c = self.cloud
+        pos = np.ascontiguousarray(c.positions, np.float32); sc = np.ascontiguousarray(c.scales, np.float32)
+        rot = np.ascontiguousarray(c.rotations, np.float32); op = np.ascontiguousarray(c.opacities, np.float32)
+        splats = np.zeros((n, 12), np.float32); keys = np.zeros(n, np.uint32); values = np.zeros(n, np.uint32)
+        _check(L.ob_preprocess(_p(pos), _p(sc), _p(rot), _p(op), _p(self.words), _p(self.filter3d), C.byref(self.camera),
+                               C.byref(self.pre), _p(splats), _p(keys), _p(values)), "ob_preprocess")
+        _check(L.ob_stable_sort(_p(keys), _p(values), n), "ob_stable_sort")
+        counts = np.zeros(2, np.uint64)
+        coverage = np.zeros(n, np.uint32)
+        _check(L.ob_bin(_p(keys), _p(values), _p(splats), n, self.grid_w, self.grid_h, _p(coverage), None, None, 0,
+                        _p(counts)), "ob_bin count")
+        E = int(counts[1])
+        ek = np.zeros(max(E, 1), np.uint32); ev = np.zeros(max(E, 1), np.uint32)
+        _check(L.ob_bin(_p(keys), _p(values), _p(splats), n, self.grid_w, self.grid_h, None, _p(ek), _p(ev), E,
+                        _p(counts)), "ob_bin emit")
+        ek, ev = ek[:E].copy(), ev[:E].copy()
+        _check(L.ob_stable_sort(_p(ek), _p(ev), E), "ob_stable_sort entries")
+        ranges = np.zeros((self.tiles, 2), np.uint32)
+        _check(L.ob_tile_ranges(_p(ek), E, self.tiles, _p(ranges)), "ob_tile_ranges")
+        P = self.cam.width * self.cam.height
+        image = np.zeros((P, 4), np.float32); stops = np.zeros(P, np.uint32)
+        _check(L.ob_forward(_p(splats), _p(ev), _p(ranges), self.cam.width, self.cam.height, _p(self.background),
+                            _p(image), _p(stops)), "ob_forward")
+        return ModelResult(splats, keys, values, coverage, int(counts[0]), E, ek, ev, ranges, image, stops)
+
+    def loss(self, r: ModelResult, target, lam):
+        P = self.cam.width * self.cam.height
+        tgt = np.ascontiguousarray(target, np.uint8).reshape(-1)
+        r.fields = np.zeros((P, 9), np.float32)
+        r.adjoint = np.zeros((P, 4), np.float32)
+        r.tile_totals = np.zeros((self.tiles, 2), np.float32)
+        _check(lib().ob_loss(_p(r.image), _p(tgt), self.cam.width, self.cam.height, float(lam), _p(r.fields),
+                             _p(r.adjoint), _p(r.tile_totals)), "ob_loss")
+        return r
+
+    def backward_fixed(self, r: ModelResult, adjoint=None):
+        adj = r.adjoint if adjoint is None else np.ascontiguousarray(adjoint, np.float32)
+        r.raw_fixed = np.zeros((self.n, 14), np.int64)
+        _check(lib().ob_backward_fixed(_p(r.splats), _p(r.entry_values), _p(r.ranges), self.cam.width, self.cam.height,
+                                       _p(self.background), _p(r.image), _p(adj), _p(r.stops), self.fixed_scale(),
+                                       self.n, _p(r.raw_fixed)), "ob_backward_fixed")
+        r.raw = np.zeros((self.n, 14), np.float32)
+        _check(lib().ob_fixed_to_float(_p(r.raw_fixed), self.n * 14, self.fixed_scale(), _p(r.raw)), "ob_fixed_to_float")
+        return r
+
+    def backward_float(self, r: ModelResult, adjoint=None):
+        adj = r.adjoint if adjoint is None else np.ascontiguousarray(adjoint, np.float32)
+        r.raw = np.zeros((self.n, 14), np.float32)
+        _check(lib().ob_backward_float(_p(r.splats), _p(r.entry_values), _p(r.ranges), self.cam.width, self.cam.height,
+                                       _p(self.background), _p(r.image), _p(adj), _p(r.stops), self.n, _p(r.raw)),
+               "ob_backward_float")
+        return r
+
+    def chain_options(self, mode=1, flags=1, opacity_reg=0.0, scale_reg=0.0, rates=(0, 0, 0, 0, 0), rest_scale=0.05,
+                      step=1, sh_dropout=(0.0, 0, 0)):
+        o = OBChainOptions()
+        o.opacity_reg = float(np.float32(opacity_reg) / np.float32(self.n))
+        o.scale_reg = float(np.float32(scale_reg) / np.float32(self.n))
+        o.mip_filter_2d = int(self.mip)
+        o.sh_dropout, o.sh_retained, o.sh_seed = float(sh_dropout[0]), int(sh_dropout[1]), int(sh_dropout[2])
+        o.mode, o.flags = mode, flags
+        o.rates[:] = [float(v) for v in rates]
+        o.rest_scale = rest_scale
+        o.beta1, o.beta2, o.epsilon = 0.9, 0.999, 1e-15
+        o.correction1 = float(np.float32(1.0) - np.float32(0.9) ** np.float32(step))
+        o.correction2 = float(np.float32(1.0) - np.float32(0.999) ** np.float32(step))
+        o.seed = step
+        o.sh_floats = 3 * self.cloud.coefficients
+        return o
+
+    def gradients(self, r: ModelResult, *, opacity_reg=0.0, scale_reg=0.0, stats=None):
+        """Chain rule in gradients mode: (positions, scales, rotations, opacities, sh) as float32 arrays."""
+        n = self.n
+        c = self.cloud
+        pos = np.ascontiguousarray(c.positions, np.float32).copy(); sc = np.ascontiguousarray(c.scales, np.float32).copy()
+        rot = np.ascontiguousarray(c.rotations, np.float32).copy(); op = np.ascontiguousarray(c.opacities, np.float32).copy()
+        words = self.words.copy()
+        raw = r.raw.copy()
+        st = np.zeros((n, 8), np.float32) if stats is None else stats
+        states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
+        g = [np.zeros(k, np.float32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
+        opt = self.chain_options(1, 1, opacity_reg, scale_reg)
+        _check(lib().ob_chain(_p(pos), _p(sc), _p(rot), _p(op), _p(words), _p(self.filter3d), C.byref(self.camera), _p(raw),
+                              _p(r.splats), _p(st), *[_p(s) for s in states], *[_p(x) for x in g], C.byref(opt)),
+               "ob_chain")
+        return g, st

@@ -1,0 +1,141 @@
+"""GPU parity: the HIP path through the C interface against Oracle B (the FP32
+model of the GPU algorithm), stage by stage and bit for bit, and against Oracle A
+(the FP64 restatement of the reference's src/train/reference.cpp) within the
+reference's own tolerances (tests/test_train.cpp:218-293, 353-403).
+
+Bit-exact: device exp/log; projected splats; depth-sorted keys and values; tile
+coverage, entries (after the stable tile sort) and ranges; the image, log
+transmittance and stop of every pixel; the SSIM fields and the adjoint; the
+14 fixed-point sums per Gaussian; and the gradients the chain rule derives from them.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vsplat import fixtures as F
+import vsplat
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    if name == "ref60":  # test_train.cpp:256-293
+        return F.random_cloud(60, 1, 13), F.train_camera(72, 56), F.smooth_target(72, 56, 11)
+    if name == "ref300":  # test_train.cpp:218-254
+        return F.random_cloud(300, 1, 7), F.train_camera(100, 75), F.smooth_target(100, 75, 5)
+    if name == "stop400":  # test_train.cpp:353-403: nearly opaque, most pixels stop early
+        c = F.random_cloud(400, 1, 17)
+        c.opacities[:] = 5.0
+        return c, F.train_camera(80, 60), F.smooth_target(80, 60, 19)
+    if name == "deg3":
+        return F.random_cloud(300, 3, 7), F.train_camera(72, 56), F.smooth_target(72, 56, 17)
+    if name == "config0":
+        w = F.config(0)
+        return w.cloud, w.cameras[0], w.targets[0]
+    if name == "config1_small":  # the 1280x720 workload's distributions at 20k Gaussians
+        w = F.config(1, count=20_000)
+        return w.cloud, w.cameras[0], w.targets[0]
+    raise KeyError(name)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def test_device_math_is_bit_exact():
+    rng = np.random.default_rng(1)
+    words = rng.integers(0, 2 ** 32, 1 << 20, dtype=np.uint64).astype(np.uint32)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, 1.17549435e-38, 88.7, 89.0, -87.3, -103.9, -104.1,
+                        1.0, 0.99, 0.5, 1.0 / 255.0], np.float32)
+    x = np.concatenate([words.view(np.float32), special])
+    small = np.concatenate([rng.uniform(-20, 5, 1 << 18).astype(np.float32), rng.uniform(0, 1, 1 << 18).astype(np.float32)])
+    for xs in (x, small):
+        for fn, ref in ((0, O.exp32), (1, O.log32)):
+            got, want = vsplat.math_probe(fn, xs), ref(xs)
+            same = (_bits(got) == _bits(want)) | (np.isnan(got) & np.isnan(want))
+            assert same.all(), f"function {fn}: {np.count_nonzero(~same)} differ, first at {xs[~same][:4]}"
+    pos = np.abs(x[np.isfinite(x)])
+    assert (_bits(vsplat.math_probe(2, pos)) == _bits(np.sqrt(pos))).all()
+    nz = x[np.isfinite(x) & (x != 0)]
+    with np.errstate(over="ignore", divide="ignore"):
+        assert (_bits(vsplat.math_probe(3, nz)) == _bits(np.float32(1.0) / nz)).all()
+    fin = x[np.isfinite(x)]
+    assert (_bits(vsplat.math_probe(4, fin)) == _bits(fin.astype(np.float16).astype(np.float32))).all()
+
+
+def _run_both(name, fixed=True, lam=0.2):
+    cloud, cam, target = _case(name)
+    model = O.Model(cloud, cam)
+    r = model.forward()
+    model.loss(r, target, lam)
+    if fixed:
+        model.backward_fixed(r)
+    else:
+        model.backward_float(r)
+    g_model, st_model = model.gradients(r)
+    t = vsplat.Trainer(cloud, vsplat.TrainerOptions(lambda_ssim=lam, force_fixed_point=fixed))
+    stats = t.compute_gradients(cam, target)
+    return cloud, cam, target, model, r, g_model, st_model, t, stats
+
+
+@pytest.mark.parametrize("name", ["ref60", "ref300", "stop400", "deg3", "config0", "config1_small"])
+def test_pipeline_bit_exact_fixed_point(name):
+    cloud, cam, target, model, r, g_model, st_model, t, stats = _run_both(name, fixed=True)
+    print(f"{name}: visible {r.visible} entries {r.entries}")
+    assert stats.visible_splats == r.visible
+    assert stats.tile_entries == r.entries
+    np.testing.assert_array_equal(_bits(t.inspect("keys")), _bits(r.keys))
+    np.testing.assert_array_equal(t.inspect("values"), r.values)
+    visible = r.values[r.keys != 0xFFFFFFFF]
+    np.testing.assert_array_equal(_bits(t.inspect("splats")[visible]), _bits(r.splats[visible]))
+    np.testing.assert_array_equal(t.inspect("entry_keys"), r.entry_keys)
+    np.testing.assert_array_equal(t.inspect("entry_values"), r.entry_values)
+    np.testing.assert_array_equal(t.inspect("ranges"), r.ranges)
+    np.testing.assert_array_equal(_bits(t.inspect("image")), _bits(r.image))
+    np.testing.assert_array_equal(t.inspect("stops"), r.stops)
+    np.testing.assert_array_equal(_bits(t.inspect("fields")), _bits(r.fields))
+    np.testing.assert_array_equal(_bits(t.inspect("adjoint")), _bits(r.adjoint))
+    np.testing.assert_array_equal(_bits(t.inspect("tile_totals")), _bits(r.tile_totals))
+    np.testing.assert_array_equal(t.inspect("raw"), r.raw_fixed)
+    g = t.gradients()
+    for got, want in zip((g.positions, g.scales, g.rotations, g.opacities, g.sh), g_model):
+        np.testing.assert_array_equal(_bits(got.reshape(-1)), _bits(want))
+    np.testing.assert_array_equal(_bits(t.densify_stats()), _bits(st_model))
+
+
+def _check_close(a, b, what, min_cos, max_ratio):
+    """check_close of tests/test_train.cpp:38-53: direction and length."""
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    assert (a @ a) > 0 and (b @ b) > 0, what
+    cos = (a @ b) / np.sqrt((a @ a) * (b @ b))
+    ratio = np.sqrt((a @ a) / (b @ b))
+    print(f"  {what:10s} cosine {cos:.6f} ratio {ratio:.5f}")
+    assert cos > min_cos, what
+    assert abs(ratio - 1) < max_ratio, what
+
+
+@pytest.mark.parametrize("name", ["ref60", "stop400", "deg3"])
+@pytest.mark.parametrize("fixed", [False, True])
+def test_against_fp64_reference(name, fixed):
+    """tests/test_train.cpp:256-293 and 353-403: forward <= 1e-3, log T <= 1e-2,
+    loss/L1/SSIM <= 1e-3; gradients cosine > 0.999 and length within 2% with
+    float sums, > 0.97 and 10% with fixed point (the reference's bars)."""
+    cloud, cam, target = _case(name)
+    ref = O.render(cloud, cam)
+    ref_loss = O.loss(ref, target, 0.2)
+    ref_g = O.gradients(cloud, cam, ref_loss.adjoint)
+    t = vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=fixed))
+    image = t.render(cam)
+    assert np.abs(image[:, :3] - ref.rgb).max() < 1e-3
+    assert np.abs(image[:, 3] - ref.log_t).max() < 1e-2
+    stats = t.compute_gradients(cam, target)
+    assert abs(stats.loss - ref_loss.loss) < 1e-3
+    assert abs(stats.l1 - ref_loss.l1) < 1e-3
+    assert abs(stats.ssim - ref_loss.ssim) < 1e-3
+    g = t.gradients()
+    cos, ratio = (0.97, 0.1) if fixed else (0.999, 0.02)
+    for got, want, what in ((g.positions, ref_g.positions, "positions"), (g.scales, ref_g.scales, "scales"),
+                            (g.rotations, ref_g.rotations, "rotations"), (g.opacities, ref_g.opacities, "opacities"),
+                            (g.sh, ref_g.sh, "sh")):
+        _check_close(got, want, what, cos, ratio)

@@ -1,0 +1,208 @@
+// The C entry points of include/vsplat.h: argument checks, then the Trainer,
+// with every exception turned into a status and a per-thread message.
+#include "trainer.hpp"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+struct vsplat_trainer {
+  vsplat::Trainer impl;
+  vsplat_trainer(const vsplat_cloud &c, const vsplat_trainer_options &o) : impl(c, o) {}
+};
+
+namespace {
+
+thread_local std::string lastError;
+
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    return VSPLAT_OK;
+  } catch (const vsplat::Error &e) {
+    lastError = e.what();
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    lastError = "out of host memory";
+    return VSPLAT_RUNTIME_ERROR;
+  } catch (const std::exception &e) {
+    lastError = e.what();
+    return VSPLAT_RUNTIME_ERROR;
+  } catch (...) {
+    lastError = "unknown error";
+    return VSPLAT_RUNTIME_ERROR;
+  }
+}
+
+int invalid(const char *what) {
+  lastError = what;
+  return VSPLAT_INVALID_ARGUMENT;
+}
+
+} // namespace
+
+extern "C" {
+
+void vsplat_default_options(vsplat_trainer_options *o) {
+  if (o == nullptr) return;
+  std::memset(o, 0, sizeof(*o));
+  o->lambda_ssim = 0.2f; // trainer.hpp:25-41
+  o->near_plane = 0.01f;
+  o->selective_adam = 1;
+  o->memory_budget = ~uint64_t{0};
+}
+
+void vsplat_default_rates(vsplat_learning_rates *r) {
+  if (r == nullptr) return;
+  *r = vsplat_learning_rates{1.6e-4f, 0.005f, 0.001f, 0.05f, 0.025f, 0.05f, 0.0f}; // trainer.hpp:51-61
+}
+
+int vsplat_trainer_create(const vsplat_cloud *cloud, const vsplat_trainer_options *options, vsplat_trainer **out) {
+  if (cloud == nullptr || out == nullptr) return invalid("null cloud or output");
+  *out = nullptr;
+  vsplat_trainer_options defaults;
+  vsplat_default_options(&defaults);
+  return guarded([&] { *out = new vsplat_trainer(*cloud, options != nullptr ? *options : defaults); });
+}
+
+void vsplat_trainer_destroy(vsplat_trainer *t) { delete t; }
+
+int vsplat_render(vsplat_trainer *t, const vsplat_camera *camera, float *out) {
+  if (t == nullptr || camera == nullptr || out == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.render(*camera, out); });
+}
+
+int vsplat_compute_gradients(vsplat_trainer *t, const vsplat_camera *camera, const uint8_t *target, size_t bytes, vsplat_step_stats *stats) {
+  if (t == nullptr || camera == nullptr) return invalid("null argument");
+  return guarded([&] {
+    const vsplat_step_stats s = t->impl.run(*camera, target, bytes, vsplat::kModeGradients, nullptr, nullptr, 0);
+    if (stats) *stats = s;
+  });
+}
+
+int vsplat_collect_statistics(vsplat_trainer *t, const vsplat_camera *camera, const uint8_t *target, size_t bytes, vsplat_step_stats *stats) {
+  if (t == nullptr || camera == nullptr) return invalid("null argument");
+  return guarded([&] {
+    const vsplat_step_stats s = t->impl.run(*camera, target, bytes, vsplat::kModeStatistics, nullptr, nullptr, 0);
+    if (stats) *stats = s;
+  });
+}
+
+int vsplat_apply_gradients(vsplat_trainer *t, const vsplat_learning_rates *rates) {
+  if (t == nullptr || rates == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.apply_gradients(*rates); });
+}
+
+int vsplat_step(vsplat_trainer *t, const vsplat_camera *camera, const uint8_t *target, size_t bytes, const vsplat_learning_rates *rates,
+                const uint8_t *next, size_t nextBytes, vsplat_step_stats *stats) {
+  if (t == nullptr || camera == nullptr || rates == nullptr) return invalid("null argument");
+  return guarded([&] {
+    const vsplat_step_stats s = t->impl.run(*camera, target, bytes, vsplat::kModeStep, rates, next, nextBytes);
+    if (stats) *stats = s;
+  });
+}
+
+int vsplat_prefetch_target(vsplat_trainer *t, const uint8_t *target, size_t bytes) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.prefetch_target(target, bytes); });
+}
+
+int vsplat_get_gradients(vsplat_trainer *t, float *positions, float *scales, float *rotations, float *opacities, float *sh) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.gradients(positions, scales, rotations, opacities, sh); });
+}
+
+int vsplat_get_cloud(vsplat_trainer *t, float *positions, float *scales, float *rotations, float *opacities, float *sh) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.cloud(positions, scales, rotations, opacities, sh); });
+}
+
+int vsplat_get_densify_stats(vsplat_trainer *t, float *out) {
+  if (t == nullptr || out == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.densify_stats(out); });
+}
+
+int vsplat_reset_densify_stats(vsplat_trainer *t) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.reset_densify_stats(); });
+}
+
+int vsplat_set_active_sh_degree(vsplat_trainer *t, int32_t degree) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.set_active_sh_degree(degree); });
+}
+
+int vsplat_set_filter3d(vsplat_trainer *t, const float *filter, size_t count) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.set_filter3d(filter, count); });
+}
+
+int vsplat_set_sh_dropout(vsplat_trainer *t, float probability, int32_t retained, uint32_t seed) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.set_sh_dropout(probability, retained, seed); });
+}
+
+int vsplat_parameters(vsplat_trainer *t, uint32_t group, void **pointer, uint64_t *bytes) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.parameters(group, pointer, bytes); });
+}
+
+uint32_t vsplat_gaussian_count(const vsplat_trainer *t) { return t != nullptr ? t->impl.count() : 0u; }
+int32_t vsplat_sh_degree(const vsplat_trainer *t) { return t != nullptr ? t->impl.sh_degree() : -1; }
+int32_t vsplat_active_sh_degree(const vsplat_trainer *t) { return t != nullptr ? t->impl.active_sh_degree() : -1; }
+int32_t vsplat_uses_fixed_point(const vsplat_trainer *t) { return t != nullptr && t->impl.fixed_point() ? 1 : 0; }
+uint64_t vsplat_gpu_bytes(const vsplat_trainer *t) { return t != nullptr ? t->impl.gpu_bytes() : 0u; }
+uint64_t vsplat_bytes_per_gaussian(const vsplat_trainer *t) { return t != nullptr ? t->impl.bytes_per_gaussian() : 0u; }
+uint64_t vsplat_staging_bytes(const vsplat_trainer *t) { return t != nullptr ? t->impl.staging_bytes() : 0u; }
+
+int vsplat_timings(const vsplat_trainer *t, vsplat_timing *out, uint32_t capacity, uint32_t *count) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] {
+    const auto stages = t->impl.timings();
+    if (count) *count = static_cast<uint32_t>(stages.size());
+    for (uint32_t i = 0; i < capacity && i < stages.size() && out != nullptr; ++i) {
+      std::memset(out[i].name, 0, sizeof(out[i].name));
+      std::strncpy(out[i].name, stages[i].first.c_str(), sizeof(out[i].name) - 1);
+      out[i].milliseconds = stages[i].second;
+    }
+  });
+}
+
+const char *vsplat_last_error(void) { return lastError.c_str(); }
+
+int vsplat_enqueue(vsplat_trainer *t, const vsplat_camera *camera, const uint8_t *deviceTarget, int32_t mode,
+                   const vsplat_learning_rates *rates) {
+  if (t == nullptr || camera == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.enqueue(*camera, deviceTarget, static_cast<uint32_t>(mode), rates); });
+}
+
+int vsplat_synchronize(vsplat_trainer *t, vsplat_step_stats *stats) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] {
+    const vsplat_step_stats s = t->impl.synchronize();
+    if (stats) *stats = s;
+  });
+}
+
+void *vsplat_stream(vsplat_trainer *t) { return t != nullptr ? static_cast<void *>(t->impl.stream()) : nullptr; }
+
+int vsplat_reserve_entries(vsplat_trainer *t, uint64_t entries) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.reserve_entries(entries); });
+}
+
+} // extern "C"
+
+namespace vsplat {
+void math_probe(int function, const float *in, float *out, uint64_t n);
+}
+
+extern "C" int vsplat_inspect(vsplat_trainer *t, const char *buffer, void *out, uint64_t capacity, uint64_t *bytes) {
+  if (t == nullptr || buffer == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.inspect(buffer, out, capacity, bytes); });
+}
+
+extern "C" int vsplat_math_probe(int32_t function, const float *in, float *out, uint64_t n) {
+  if ((in == nullptr || out == nullptr) && n > 0) return invalid("null argument");
+  return guarded([&] { vsplat::math_probe(function, in, out, n); });
+}

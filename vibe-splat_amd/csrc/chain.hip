@@ -1,0 +1,420 @@
+// The chain rule from the backward pass's 14 screen-space sums per Gaussian to
+// the raw parameters, and what becomes of the result (train_gradient.comp:105-315):
+// the densification statistics only, the gradients stored (one float per
+// parameter), or Adam's step taken at once with bf16 moments. The SH
+// coefficients step in a pass of their own, one thread per packed word
+// (train_sh_step.comp:19-70), so that neighbouring threads touch neighbouring memory.
+//
+// The fixed-point sums are converted here (train_fixed_to_float.comp:9-14):
+// int64 to double (exact), divided by the scale in double, rounded to float once.
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace vsplat {
+
+namespace {
+
+// bf16 moments (bf16.glsl:10-29).
+__device__ __forceinline__ float bf16_load(uint32_t word, uint32_t slot) { return as_float(slot == 0 ? word << 16 : word & 0xFFFF0000u); }
+__device__ __forceinline__ uint32_t bf16_round(float x) {
+  const uint32_t bits = as_uint(x);
+  return (bits + 0x7FFFu + ((bits >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t bf16_stochastic(float x, uint32_t index, uint32_t seed) {
+  const uint32_t state = (index ^ (seed * 0x9E3779B9u)) * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (as_uint(x) + (((word >> 22u) ^ word) & 0xFFFFu)) >> 16;
+}
+__device__ __forceinline__ uint32_t bf16_store(uint32_t word, uint32_t slot, uint32_t bits) {
+  return slot == 0 ? (word & 0xFFFF0000u) | bits : (word & 0xFFFFu) | (bits << 16);
+}
+
+// Adam (train_step.glsl:388-399, first order): the step uses both moments
+// before they are rounded; m is rounded to nearest, v at random.
+__device__ __forceinline__ void step_parameter(const ChainParams &o, float &value, uint32_t &state, float g, float rate, uint32_t index) {
+  const float m = fma32(o.beta1, bf16_load(state, 0), (1.0f - o.beta1) * g);
+  const float v = fma32(o.beta2, bf16_load(state, 1), ((1.0f - o.beta2) * g) * g);
+  value = value - (rate * (m / o.correction1)) / (sqrtf(v / o.correction2) + o.epsilon);
+  state = bf16_store(bf16_store(state, 0, bf16_round(m)), 1, bf16_stochastic(v, index, o.seed));
+}
+
+// Half floats rounded at random in proportion to the distance (train_step.glsl:409-419).
+__device__ __forceinline__ uint32_t half_stochastic(float x, uint32_t index, uint32_t seed) {
+  const uint32_t nearest = float_to_half(x);
+  const float rounded = half_to_float(nearest);
+  if (rounded == x || fabsf(rounded) == __builtin_inff()) return nearest;
+  const uint32_t other = fabsf(x) > fabsf(rounded) ? nearest + 1u : nearest - 1u;
+  const float gap = half_to_float(other) - rounded;
+  return train_uniform(index, seed) < (x - rounded) / gap ? other : nearest;
+}
+
+// d basis / d direction (train_types.glsl:126-153).
+__device__ __forceinline__ void sh_basis_gradient(float x, float y, float z, uint32_t degree, float g[16][3]) {
+  const float C1 = 0.4886025119029199f, A = 1.0925484305920792f, K = 0.31539156525252005f, L = 0.5462742152960396f;
+  const float C = -0.5900435899266435f, E = 2.890611442640554f, F = -0.4570457994644658f, G = 0.3731763325901154f,
+              H = 1.445305721320277f;
+  const float xx = x * x, yy = y * y, zz = z * z;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0f;
+  if (degree > 0) {
+    g[1][1] = -C1;
+    g[2][2] = C1;
+    g[3][0] = -C1;
+  }
+  if (degree > 1) {
+    g[4][0] = A * y;
+    g[4][1] = A * x;
+    g[5][1] = -A * z;
+    g[5][2] = -A * y;
+    g[6][0] = K * (-2.0f * x);
+    g[6][1] = K * (-2.0f * y);
+    g[6][2] = K * (4.0f * z);
+    g[7][0] = -A * z;
+    g[7][2] = -A * x;
+    g[8][0] = L * (2.0f * x);
+    g[8][1] = L * (-2.0f * y);
+  }
+  if (degree > 2) {
+    g[9][0] = C * (6.0f * x * y);
+    g[9][1] = C * (3.0f * xx - 3.0f * yy);
+    g[10][0] = E * (y * z);
+    g[10][1] = E * (x * z);
+    g[10][2] = E * (x * y);
+    g[11][0] = F * (-2.0f * x * y);
+    g[11][1] = F * (4.0f * zz - xx - 3.0f * yy);
+    g[11][2] = F * (8.0f * y * z);
+    g[12][0] = G * (-6.0f * x * z);
+    g[12][1] = G * (-6.0f * y * z);
+    g[12][2] = G * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+    g[13][0] = F * (4.0f * zz - 3.0f * xx - yy);
+    g[13][1] = F * (-2.0f * x * y);
+    g[13][2] = F * (8.0f * x * z);
+    g[14][0] = H * (2.0f * x * z);
+    g[14][1] = H * (-2.0f * y * z);
+    g[14][2] = H * (xx - yy);
+    g[15][0] = C * (3.0f * xx - 3.0f * yy);
+    g[15][1] = C * (-6.0f * x * y);
+  }
+}
+
+// What becomes of gradient g of parameter `at` (train_gradient.comp:57-72).
+__device__ __forceinline__ void emit(const ChainParams &o, float *values, uint32_t *state, float *grad, uint32_t at, float g, float rate,
+                                     uint32_t group) {
+  if (o.mode == kModeGradients) {
+    grad[at] = g;
+  } else if (g != 0.0f || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0)) {
+    float v = values[at];
+    uint32_t s = state[at];
+    step_parameter(o, v, s, g, rate, at * 5u + group);
+    values[at] = v;
+    state[at] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.count) return;
+  if (counters[kEntriesUsed] != counters[kEntries]) return; // the view did not fit: nothing was computed
+  float g[kRawPerSplat];
+  if (o.fixedPoint) {
+    const long long *raw = reinterpret_cast<const long long *>(B.rawFixed) + static_cast<size_t>(i) * kRawPerSplat;
+    const double scale = static_cast<double>(o.fixedScale);
+#pragma unroll
+    for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = static_cast<float>(static_cast<double>(raw[k]) / scale);
+  } else {
+    const float *raw = B.rawFloat + static_cast<size_t>(i) * kRawPerSplat;
+#pragma unroll
+    for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = raw[k];
+  }
+  bool touched = false;
+#pragma unroll
+  for (uint32_t k = 0; k < kRawPerSplat; ++k) touched = touched || g[k] != 0.0f;
+  const bool zeros = o.mode == kModeGradients || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0);
+  const bool penalties = o.opacityReg > 0.0f || o.scaleReg > 0.0f;
+  const float W = static_cast<float>(c.width), H = static_cast<float>(c.height);
+  float *shPass = B.shPass + static_cast<size_t>(i) * kShPassFloats;
+
+  if ((touched && (o.flags & kFlagStatistics) != 0) && B.stats != nullptr) {
+    // DensifyStat (train_gradient.comp:94-101), per half image rather than per pixel.
+    float4 *st = reinterpret_cast<float4 *>(B.stats + static_cast<size_t>(i) * 8);
+    float4 a = st[0], b = st[1];
+    const float ax = (g[1] * 0.5f) * W, ay = (g[2] * 0.5f) * H;
+    a.x = a.x + sqrtf(fma32(ay, ay, ax * ax));
+    a.y = a.y + 1.0f;
+    a.z = maxf(a.z, B.splats[i].radius);
+    a.w = fma32((g[9] * 3.0f) * W, H, a.w);
+    const float bx = (g[10] * 0.5f) * W, by = (g[11] * 0.5f) * H;
+    b.x = b.x + sqrtf(fma32(by, by, bx * bx));
+    st[0] = a;
+    st[1] = b;
+  }
+  if (o.mode == kModeStatistics || (!touched && !zeros && !penalties)) {
+    if (o.mode == kModeStep) shPass[9] = as_float(0u); // nothing for the SH pass
+    return;
+  }
+  const float logScale[3] = {B.scales[i * 3], B.scales[i * 3 + 1], B.scales[i * 3 + 2]};
+  const float opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
+  const float s2[3] = {vs_exp(2.0f * logScale[0]), vs_exp(2.0f * logScale[1]), vs_exp(2.0f * logScale[2])};
+  if (!touched) {
+    // Nothing from the pixels; the penalties still apply.
+    const float op = penalties ? o.opacityReg * opacity * (1.0f - opacity) : 0.0f;
+    for (uint32_t a = 0; a < 3; ++a) {
+      emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, 0.0f, o.rates[0], 0);
+      emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, penalties ? o.scaleReg * vs_exp(logScale[a]) : 0.0f, o.rates[1], 1);
+    }
+    for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, 0.0f, o.rates[2], 2);
+    emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, op, o.rates[3], 3);
+    if (o.mode == kModeGradients) {
+      for (uint32_t j = 0; j < o.shFloats; ++j) B.gradSh[static_cast<size_t>(i) * o.shFloats + j] = 0.0f;
+    } else {
+      shPass[9] = as_float(0u);
+    }
+    return;
+  }
+
+  const float px = B.positions[i * 3], py = B.positions[i * 3 + 1], pz = B.positions[i * 3 + 2];
+  float qw = B.rotations[i * 4], qx = B.rotations[i * 4 + 1], qy = B.rotations[i * 4 + 2], qz = B.rotations[i * 4 + 3];
+  const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
+  if (qlen > 0.0f) {
+    qw = qw / qlen;
+    qx = qx / qlen;
+    qy = qy / qlen;
+    qz = qz / qlen;
+  } else {
+    qw = 1.0f;
+    qx = qy = qz = 0.0f;
+  }
+  float tx, ty, tz;
+  to_camera(c, px, py, pz, tx, ty, tz);
+  const float iz = 1.0f / tz, iz2 = iz * iz;
+  const float gc0 = g[1], gc1 = g[2];
+  float G00 = g[3], G01 = g[4], G11 = g[5];
+  float k2 = 1.0f;
+  float R[3][3];
+  quat_to_mat(qw, qx, qy, qz, R);
+  const float f = B.filter3d != nullptr ? B.filter3d[i] : 0.0f;
+  // Sigma = R D R^T + f I; X = Wv Sigma Wv^T with Wv the view rotation.
+  float S[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S[r][k] = dot3(R[r][0] * s2[0], R[r][1] * s2[1], R[r][2] * s2[2], R[k][0], R[k][1], R[k][2]);
+  S[0][0] = S[0][0] + f;
+  S[1][1] = S[1][1] + f;
+  S[2][2] = S[2][2] + f;
+  const float *V = c.view;
+  float Y[3][3], X[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Y[r][k] = dot3(V[0 * 4 + r], V[1 * 4 + r], V[2 * 4 + r], S[0][k], S[1][k], S[2][k]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) X[r][k] = dot3(Y[r][0], Y[r][1], Y[r][2], V[0 * 4 + k], V[1 * 4 + k], V[2 * 4 + k]);
+  const float k3 = filter_factor(s2[0], s2[1], s2[2], f);
+  const float opacity3 = opacity * k3;
+  const float limX = 1.3f * c.tanHalfX, limY = 1.3f * c.tanHalfY;
+  const float rx = tx * iz, ry = ty * iz;
+  const bool clampedX = fabsf(rx) > limX, clampedY = fabsf(ry) > limY;
+  const float ux = clampf(rx, -limX, limX), uy = clampf(ry, -limY, limY);
+  const float J00 = c.fx * iz, J02 = -(c.fx * ux) * iz, J11 = c.fy * iz, J12 = -(c.fy * uy) * iz;
+  float T[2][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    T[0][k] = fma32(J02, V[k * 4 + 2], J00 * V[k * 4 + 0]);
+    T[1][k] = fma32(J12, V[k * 4 + 2], J11 * V[k * 4 + 1]);
+  }
+  float XJ[3][2];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    XJ[r][0] = fma32(X[r][2], J02, X[r][0] * J00);
+    XJ[r][1] = fma32(X[r][2], J12, X[r][1] * J11);
+  }
+  if (o.mipFilter2d) {
+    // The 2D filter's opacity factor k = sqrt(det C / det(C + 0.1 I)) moves with C too.
+    const float C00 = fma32(J02, XJ[2][0], J00 * XJ[0][0]);
+    const float C01 = fma32(J02, XJ[2][1], J00 * XJ[0][1]);
+    const float C10 = fma32(J12, XJ[2][0], J11 * XJ[1][0]);
+    const float C11 = fma32(J12, XJ[2][1], J11 * XJ[1][1]);
+    const float F00 = C00 + 0.1f, F11 = C11 + 0.1f;
+    const float detC = fma32(C00, C11, -(C01 * C10)), detF = fma32(F00, F11, -(C01 * C10));
+    if (detC > 0.0f && detF > 0.0f) {
+      const float k = sqrtf(detC / detF);
+      const float m = ((g[0] * opacity3) * 0.5f) * k;
+      G00 = fma32(m, C11 / detC - F11 / detF, G00);
+      G01 = fma32(m, -C01 / detC + C01 / detF, G01);
+      G11 = fma32(m, C00 / detC - F00 / detF, G11);
+      k2 = k;
+    } else {
+      k2 = 0.0f;
+    }
+  }
+  // Position: the centre's (unclamped) Jacobian, and tr(G dJ X J^T) for dJ along x, y, z.
+  const float Jc02 = -(c.fx * tx) * iz2, Jc12 = -(c.fy * ty) * iz2;
+  float gT0 = J00 * gc0, gT1 = J11 * gc1, gT2 = fma32(Jc12, gc1, Jc02 * gc0);
+  const float ax = clampedX ? 0.0f : -c.fx * iz2, ay = clampedY ? 0.0f : -c.fy * iz2;
+  const float zx = (c.fx * ux) * iz2, zy = (c.fy * uy) * iz2;
+  const float az0 = -c.fx * iz2, az1 = -c.fy * iz2, az2 = clampedX ? zx : 2.0f * zx, az3 = clampedY ? zy : 2.0f * zy;
+  const float trX = ax * fma32(G01, XJ[2][1], G00 * XJ[2][0]);
+  const float trY = ay * fma32(G11, XJ[2][1], G01 * XJ[2][0]);
+  const float N00 = fma32(az2, XJ[2][0], az0 * XJ[0][0]), N01 = fma32(az2, XJ[2][1], az0 * XJ[0][1]);
+  const float N10 = fma32(az3, XJ[2][0], az1 * XJ[1][0]), N11 = fma32(az3, XJ[2][1], az1 * XJ[1][1]);
+  const float trZ = fma32(G11, N11, fma32(G01, N10, fma32(G01, N01, G00 * N00)));
+  gT0 = fma32(2.0f, trX, gT0);
+  gT1 = fma32(2.0f, trY, gT1);
+  gT2 = fma32(2.0f, trZ, gT2);
+  float gP[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gP[k] = dot3(V[k * 4 + 0], V[k * 4 + 1], V[k * 4 + 2], gT0, gT1, gT2);
+
+  // The 3D covariance: G3 = T^T G T, dL = tr(G3 dSigma).
+  float GT[2][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    GT[0][k] = fma32(G01, T[1][k], G00 * T[0][k]);
+    GT[1][k] = fma32(G11, T[1][k], G01 * T[0][k]);
+  }
+  float G3[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) G3[r][k] = fma32(T[1][r], GT[1][k], T[0][r] * GT[0][k]);
+  float dScale[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float v[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) v[r] = dot3(G3[r][0], G3[r][1], G3[r][2], R[0][a], R[1][a], R[2][a]);
+    const float quad = dot3(R[0][a], R[1][a], R[2][a], v[0], v[1], v[2]);
+    const float filterTerm = ((((g[0] * k2) * opacity) * k3) * f) / (s2[a] + f);
+    dScale[a] = fma32(o.scaleReg, vs_exp(logScale[a]), fma32(2.0f * s2[a], quad, filterTerm));
+  }
+  // Rotation: P = G3 R D, dL/dq = 2 <2P, dR/dq> written out per component.
+  float P[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P[r][k] = dot3(G3[r][0], G3[r][1], G3[r][2], R[0][k], R[1][k], R[2][k]) * s2[k];
+  const float w = qw, x = qx, y = qy, z = qz;
+  const float sw = fma32(x, P[2][1] - P[1][2], fma32(y, P[0][2] - P[2][0], z * (P[1][0] - P[0][1])));
+  const float sx = fma32(w, P[2][1] - P[1][2], fma32(z, P[0][2] + P[2][0], y * (P[0][1] + P[1][0]))) - 2.0f * x * (P[1][1] + P[2][2]);
+  const float sy = fma32(w, P[0][2] - P[2][0], fma32(z, P[1][2] + P[2][1], x * (P[0][1] + P[1][0]))) - 2.0f * y * (P[0][0] + P[2][2]);
+  const float sz = fma32(w, P[1][0] - P[0][1], fma32(y, P[1][2] + P[2][1], x * (P[0][2] + P[2][0]))) - 2.0f * z * (P[0][0] + P[1][1]);
+  const float gq[4] = {4.0f * sw, 4.0f * sx, 4.0f * sy, 4.0f * sz};
+  const float qd = fma32(z, gq[3], fma32(y, gq[2], fma32(x, gq[1], w * gq[0])));
+  const float qv[4] = {w, x, y, z};
+  const float denom = qlen > 1e-12f ? qlen : 1e-12f;
+  float tangent[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) tangent[k] = fma32(-qv[k], qd, gq[k]) / denom;
+
+  const float dOpacity = (fma32(g[0] * k2, k3, o.opacityReg) * opacity) * (1.0f - opacity);
+
+  // Colour, clamped at zero, and the view direction's part of the position gradient.
+  const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
+  const float dist = sqrtf(fma32(ddz, ddz, fma32(ddy, ddy, ddx * ddx)));
+  const float dir[3] = {ddx / dist, ddy / dist, ddz / dist};
+  float basis[16];
+  sh_basis(dir[0], dir[1], dir[2], c.shDegree, basis);
+  const uint32_t used = sh_coefficients_used(i, c.shCoefficients, o.shDropout, o.shRetained, o.shSeed);
+  const uint32_t base = i * c.shWords;
+  float colour[3] = {0.5f, 0.5f, 0.5f};
+  for (uint32_t k = 0; k < used; ++k) {
+#pragma unroll
+    for (uint32_t ch = 0; ch < 3; ++ch) colour[ch] = fma32(basis[k], sh_value(B.shWords, base, k * 3 + ch), colour[ch]);
+  }
+  const float gcu[3] = {g[6] * (colour[0] > 0.0f ? 1.0f : 0.0f), g[7] * (colour[1] > 0.0f ? 1.0f : 0.0f),
+                        g[8] * (colour[2] > 0.0f ? 1.0f : 0.0f)};
+  if (c.shDegree > 0) {
+    float bg[16][3];
+    sh_basis_gradient(dir[0], dir[1], dir[2], c.shDegree, bg);
+    float gDir[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 1; k < used; ++k) {
+      const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value(B.shWords, base, k * 3), sh_value(B.shWords, base, k * 3 + 1),
+                           sh_value(B.shWords, base, k * 3 + 2));
+#pragma unroll
+      for (int a = 0; a < 3; ++a) gDir[a] = fma32(d, bg[k][a], gDir[a]);
+    }
+    const float dd = dot3(dir[0], dir[1], dir[2], gDir[0], gDir[1], gDir[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gP[a] = gP[a] + fma32(-dir[a], dd, gDir[a]) / dist;
+  }
+
+  // Everything above read the parameters before any of them changes.
+  for (uint32_t a = 0; a < 3; ++a) {
+    emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, gP[a], o.rates[0], 0);
+    emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, dScale[a], o.rates[1], 1);
+  }
+  for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, tangent[a], o.rates[2], 2);
+  emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, dOpacity, o.rates[3], 3);
+  if (o.mode == kModeGradients) {
+    const uint32_t stored = o.shFloats / 3;
+    float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
+    for (uint32_t k = 0; k < stored; ++k)
+#pragma unroll
+      for (uint32_t ch = 0; ch < 3; ++ch) out[k * 3 + ch] = k < used ? basis[k] * gcu[ch] : 0.0f;
+  } else {
+    // For the SH pass: the colour gradient, the direction and the coefficients used.
+    shPass[0] = gcu[0];
+    shPass[1] = gcu[1];
+    shPass[2] = gcu[2];
+    shPass[3] = dir[0];
+    shPass[4] = dir[1];
+    shPass[5] = dir[2];
+    shPass[9] = as_float(used);
+  }
+}
+
+__global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shWords, uint32_t *__restrict__ stateSh,
+                                                      const float *__restrict__ shPass, ViewParams c, ChainParams o,
+                                                      const uint32_t *counters) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t words = c.shWords;
+  if (w >= c.count * words) return;
+  if (counters[kEntriesUsed] != counters[kEntries]) return;
+  const uint32_t i = w / words, q = w - i * words;
+  const float *g = shPass + static_cast<size_t>(i) * kShPassFloats;
+  const uint32_t used = as_uint(g[9]);
+  const bool zeros = o.mode == kModeStep && (o.flags & kFlagSelective) == 0;
+  if (used == 0 && !zeros) return;
+  float basis[16];
+  sh_basis(g[3], g[4], g[5], c.shDegree, basis);
+  const uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
+  const uint32_t last = q < 3 ? q : (first + 1 < o.shFloats - 1 ? first + 1 : o.shFloats - 1);
+  uint32_t word = shWords[w];
+  bool changed = false;
+  for (uint32_t j = first; j <= last; ++j) {
+    const uint32_t k = j / 3, ch = j - 3 * k;
+    const float gv = k < used ? basis[k] * g[ch] : 0.0f;
+    if (gv == 0.0f && !zeros) continue;
+    const uint32_t s = i * o.shFloats + j;
+    uint32_t state = stateSh[s];
+    float value = j < 3 ? as_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+    step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.restScale, s * 5u + 4u);
+    stateSh[s] = state;
+    if (j < 3) {
+      word = as_uint(value);
+    } else {
+      const uint32_t shift = 16u * ((j - 3) & 1u);
+      word = (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, o.seed ^ 0x5DEECE66u) << shift);
+    }
+    changed = true;
+  }
+  if (changed) shWords[w] = word;
+}
+
+} // namespace
+
+void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
+                  const uint32_t *counters) {
+  if (view.count == 0) return;
+  hipLaunchKernelGGL(chain_kernel, dim3((view.count + 255) / 256), dim3(256), 0, stream, buffers, view, params, counters);
+  if (params.mode == kModeStep) {
+    const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
+    hipLaunchKernelGGL(sh_step_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
+                       buffers.stateSh, buffers.shPass, view, params, counters);
+  }
+}
+
+} // namespace vsplat

@@ -1,0 +1,78 @@
+// Host-side launchers of the rasterizer's kernels, one translation unit per
+// stage. Every launcher only enqueues on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace vsplat {
+
+// What the chain-rule pass does with the gradients (train_step.glsl:346-353).
+constexpr uint32_t kModeStatistics = 0, kModeGradients = 1, kModeStep = 2;
+constexpr uint32_t kModeRender = 0xFFFFFFFFu; // forward only (Trainer::render)
+constexpr uint32_t kFlagStatistics = 1, kFlagSelective = 4;
+constexpr uint32_t kShPassFloats = 10; // per Gaussian, what the SH step needs from the chain rule
+
+struct ChainParams {
+  float opacityReg, scaleReg; // per Gaussian: the option over the count
+  uint32_t mipFilter2d;
+  float shDropout;
+  uint32_t shRetained, shSeed;
+  uint32_t mode, flags;
+  float rates[5]; // position, scale, rotation, opacity, SH
+  float restScale, beta1, beta2, epsilon, correction1, correction2;
+  uint32_t seed, shFloats;
+  uint32_t fixedPoint;
+  float fixedScale;
+};
+
+struct ChainBuffers {
+  float *positions, *scales, *rotations, *opacities;
+  uint32_t *shWords;
+  const float *filter3d;
+  const Splat *splats;
+  const int64_t *rawFixed;
+  const float *rawFloat;
+  float *shPass;
+  float *stats; // 8 per Gaussian, or null
+  uint32_t *statePos, *stateScale, *stateRot, *stateOpacity, *stateSh;
+  float *gradPos, *gradScale, *gradRot, *gradOpacity, *gradSh;
+};
+
+void launch_project(hipStream_t stream, const float *positions, const float *scales, const float *rotations, const float *opacities,
+                    const uint32_t *shWords, const float *filter3d, const ViewParams &view, bool mipFilter2d, float shDropout,
+                    uint32_t shRetained, uint32_t shSeed, Splat *splats, uint32_t *keys, uint32_t *values);
+
+size_t scan_scratch_words(uint64_t count);
+void launch_exclusive_scan(hipStream_t stream, uint32_t *data, uint32_t count, uint32_t *scratch, uint32_t *total);
+size_t radix_sort_scratch_words(uint64_t capacity);
+void launch_radix_sort(hipStream_t stream, uint32_t *keys, uint32_t *values, uint32_t *tmpKeys, uint32_t *tmpValues,
+                       const uint32_t *countDev, uint32_t capacity, uint32_t bits, uint32_t *scratch);
+void launch_copy_u32(hipStream_t stream, const uint32_t *src, uint32_t *dst, const uint32_t *countDev, uint32_t capacity);
+void launch_fill_u32(hipStream_t stream, uint32_t *dst, uint64_t count, uint32_t value);
+
+void launch_bin_count(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
+                      uint32_t gridW, uint32_t gridH, uint32_t *coverage, uint32_t *counters);
+void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity);
+void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
+                     uint32_t gridW, uint32_t gridH, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
+                     uint32_t *entryValues);
+void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges);
+
+void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
+                    uint32_t height, const float background[3], float4 *image, uint32_t *stops);
+void launch_backward(hipStream_t stream, bool fixedPoint, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges,
+                     uint32_t width, uint32_t height, const float background[3], const float4 *image, const float4 *adjoint,
+                     const uint32_t *stops, float fixedScale, int64_t *rawFixed, float *rawFloat);
+
+void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
+                 float *fields, float4 *adjoint, float2 *tileTotals, double *sums);
+
+void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
+                  const uint32_t *counters);
+
+} // namespace vsplat

@@ -1,0 +1,229 @@
+// Loss and its derivative per pixel, L = (1 - lambda) L1 + lambda (1 - SSIM)
+// (train_ssim.comp:42-123, train_loss.comp:52-113). The 11x11 Gaussian window
+// (sigma 1.5) is applied separably: a horizontal pass over the tile's 16
+// columns and 26 rows (5 rows of halo above and below) into LDS, then a
+// vertical pass out of it. The window is cut at the image border and
+// renormalised there, as in the reference.
+//
+// Pass 1 writes per pixel the three fields P = dS/dmu - 2 mu dS/dvar - mu_y dS/dcov,
+// Q = dS/dvar, R = dS/dcov (each over the window's weight), 9 floats; pass 2
+// applies the same window to them to get dSSIM/dimage, and adds the L1 term.
+// Each pass also leaves its tile's total (SSIM, L1), summed in a fixed tree.
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace vsplat {
+
+namespace {
+
+__constant__ float kWeights[11] = {0.00102838f, 0.00759876f, 0.03600077f, 0.10936069f, 0.21300554f, 0.26601172f,
+                                   0.21300554f, 0.10936069f, 0.03600077f, 0.00759876f, 0.00102838f};
+constexpr int kHalo = 5;
+constexpr int kRows = 16 + 2 * kHalo;
+constexpr int kHalfPixels = kRows * 16;
+
+__device__ __forceinline__ float window_norm(int at, int size) {
+  float norm = 0.0f;
+  const int lo = at - 5 > 0 ? at - 5 : 0, hi = at + 5 < size - 1 ? at + 5 : size - 1;
+  for (int i = lo; i <= hi; ++i) norm = norm + kWeights[i - at + 5];
+  return norm;
+}
+
+__device__ __forceinline__ float unorm8(uint32_t v) { return static_cast<float>(v) * (1.0f / 255.0f); }
+
+// The tile's total of one value per thread: xor butterfly within each wave
+// (lane l plus lane l ^ m, m = 32 ... 1), then the four waves in order.
+__device__ __forceinline__ float tile_sum(float v, float *waveTotals) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, 64);
+  if ((threadIdx.x & 63u) == 0) waveTotals[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const float total = ((waveTotals[0] + waveTotals[1]) + waveTotals[2]) + waveTotals[3];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ image, const uchar4 *__restrict__ target, uint32_t width,
+                                                   uint32_t height, uint32_t gridW, float *__restrict__ fields,
+                                                   float2 *__restrict__ tileTotals) {
+  __shared__ float h[15][kHalfPixels];
+  __shared__ float waveTotals[4];
+  const int W = static_cast<int>(width), H = static_cast<int>(height);
+  const uint32_t tile = blockIdx.x;
+  const int x0 = static_cast<int>(tile % gridW) * 16, y0 = static_cast<int>(tile / gridW) * 16;
+  for (int i = static_cast<int>(threadIdx.x); i < kHalfPixels; i += 256) {
+    const int row = i / 16, col = i % 16;
+    const int y = y0 + row - kHalo, x = x0 + col;
+    float acc[15];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) acc[q] = 0.0f;
+    if (y >= 0 && y < H && x < W) {
+      float norm = 0.0f;
+      const int u0 = x - 5 > 0 ? x - 5 : 0, u1 = x + 5 < W - 1 ? x + 5 : W - 1;
+      for (int u = u0; u <= u1; ++u) {
+        const float w = kWeights[u - x + 5];
+        const size_t at = static_cast<size_t>(y) * W + u;
+        const float4 a4 = image[at];
+        const uchar4 b4 = target[at];
+        const float a[3] = {a4.x, a4.y, a4.z};
+        const float b[3] = {unorm8(b4.x), unorm8(b4.y), unorm8(b4.z)};
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          const float wa = w * a[ch], wb = w * b[ch];
+          acc[ch] = acc[ch] + wa;
+          acc[3 + ch] = acc[3 + ch] + wb;
+          acc[6 + ch] = fma32(wa, a[ch], acc[6 + ch]);
+          acc[9 + ch] = fma32(wb, b[ch], acc[9 + ch]);
+          acc[12 + ch] = fma32(wa, b[ch], acc[12 + ch]);
+        }
+        norm = norm + w;
+      }
+#pragma unroll
+      for (int q = 0; q < 15; ++q) acc[q] = acc[q] / norm;
+    }
+#pragma unroll
+    for (int q = 0; q < 15; ++q) h[q][i] = acc[q];
+  }
+  __syncthreads();
+
+  const int lx = static_cast<int>(threadIdx.x & 15u), px = x0 + lx, py = y0 + static_cast<int>(threadIdx.x >> 4);
+  float own = 0.0f;
+  if (px < W && py < H) {
+    const float rowNorm = window_norm(py, H);
+    float acc[15];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) acc[q] = 0.0f;
+    const int v0 = py - 5 > 0 ? py - 5 : 0, v1 = py + 5 < H - 1 ? py + 5 : H - 1;
+    for (int v = v0; v <= v1; ++v) {
+      const float w = kWeights[v - py + 5];
+      const int idx = (v - y0 + kHalo) * 16 + lx;
+#pragma unroll
+      for (int q = 0; q < 15; ++q) acc[q] = fma32(w, h[q][idx], acc[q]);
+    }
+    const float norm = rowNorm * window_norm(px, W);
+    const size_t at = static_cast<size_t>(py) * W + px;
+    const float C1 = 0.0001f, C2 = 0.0009f;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const float mx = acc[ch] / rowNorm, my = acc[3 + ch] / rowNorm;
+      const float vx = fma32(-mx, mx, acc[6 + ch] / rowNorm), vy = fma32(-my, my, acc[9 + ch] / rowNorm),
+                  cxy = fma32(-mx, my, acc[12 + ch] / rowNorm);
+      const float n1 = fma32(2.0f * mx, my, C1), n2 = fma32(2.0f, cxy, C2);
+      const float d1 = fma32(mx, mx, fma32(my, my, C1)), d2 = (vx + vy) + C2;
+      const float dd = d1 * d2;
+      const float s = (n1 * n2) / dd;
+      own = ch == 0 ? s : own + s;
+      // n1 and n2 are cancelled by hand: n2 can be zero where the derivative is not.
+      const float dMu = ((2.0f * my) * n2) / dd - ((2.0f * s) * mx) / d1;
+      const float dVar = -s / d2;
+      const float dCov = (2.0f * n1) / dd;
+      fields[at * 9 + ch] = fma32(-my, dCov, fma32(-2.0f * mx, dVar, dMu)) / norm;
+      fields[at * 9 + 3 + ch] = dVar / norm;
+      fields[at * 9 + 6 + ch] = dCov / norm;
+    }
+  }
+  const float total = tile_sum(own, waveTotals);
+  if (threadIdx.x == 0) tileTotals[tile].x = total;
+}
+
+__global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ image, const uchar4 *__restrict__ target, uint32_t width,
+                                                   uint32_t height, uint32_t gridW, float lambda, const float *__restrict__ fields,
+                                                   float4 *__restrict__ adjoint, float2 *__restrict__ tileTotals) {
+  __shared__ float h[9][kHalfPixels];
+  __shared__ float waveTotals[4];
+  const int W = static_cast<int>(width), H = static_cast<int>(height);
+  const uint32_t tile = blockIdx.x;
+  const int x0 = static_cast<int>(tile % gridW) * 16, y0 = static_cast<int>(tile / gridW) * 16;
+  for (int i = static_cast<int>(threadIdx.x); i < kHalfPixels; i += 256) {
+    const int row = i / 16, col = i % 16;
+    const int y = y0 + row - kHalo, x = x0 + col;
+    float acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
+    if (y >= 0 && y < H && x < W) {
+      const int u0 = x - 5 > 0 ? x - 5 : 0, u1 = x + 5 < W - 1 ? x + 5 : W - 1;
+      for (int u = u0; u <= u1; ++u) {
+        const float w = kWeights[u - x + 5];
+        const float *f = fields + (static_cast<size_t>(y) * W + u) * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] = fma32(w, f[q], acc[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) h[q][i] = acc[q];
+  }
+  __syncthreads();
+
+  const int lx = static_cast<int>(threadIdx.x & 15u), px = x0 + lx, py = y0 + static_cast<int>(threadIdx.x >> 4);
+  float own = 0.0f;
+  if (px < W && py < H) {
+    float acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
+    const int v0 = py - 5 > 0 ? py - 5 : 0, v1 = py + 5 < H - 1 ? py + 5 : H - 1;
+    for (int v = v0; v <= v1; ++v) {
+      const float w = kWeights[v - py + 5];
+      const int idx = (v - y0 + kHalo) * 16 + lx;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] = fma32(w, h[q][idx], acc[q]);
+    }
+    const size_t at = static_cast<size_t>(py) * W + px;
+    const float perValue = 1.0f / (3.0f * static_cast<float>(W) * static_cast<float>(H));
+    const float keep = 1.0f - lambda;
+    const float4 x4 = image[at];
+    const uchar4 y4 = target[at];
+    const float xs[3] = {x4.x, x4.y, x4.z};
+    const float ys[3] = {unorm8(y4.x), unorm8(y4.y), unorm8(y4.z)};
+    float out[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const float d = xs[ch] - ys[ch];
+      own = ch == 0 ? fabsf(d) : own + fabsf(d);
+      const float sgn = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+      const float inner = fma32(ys[ch], acc[6 + ch], fma32(2.0f * xs[ch], acc[3 + ch], acc[ch]));
+      out[ch] = fma32(-lambda, inner, keep * sgn) * perValue;
+    }
+    adjoint[at] = make_float4(out[0], out[1], out[2], 0.0f);
+  }
+  const float total = tile_sum(own, waveTotals);
+  if (threadIdx.x == 0) tileTotals[tile].y = total;
+}
+
+// Sums the tiles' totals in double, in a fixed order: thread t adds tiles t,
+// t + 256, ..., then a tree over the threads. sums[0] SSIM, sums[1] L1.
+__global__ __launch_bounds__(256) void total_kernel(const float2 *__restrict__ tileTotals, uint32_t tiles, double *__restrict__ sums) {
+  __shared__ double s[2][256];
+  double ssim = 0.0, l1 = 0.0;
+  for (uint32_t t = threadIdx.x; t < tiles; t += 256) {
+    ssim += static_cast<double>(tileTotals[t].x);
+    l1 += static_cast<double>(tileTotals[t].y);
+  }
+  s[0][threadIdx.x] = ssim;
+  s[1][threadIdx.x] = l1;
+  __syncthreads();
+  for (uint32_t half = 128; half > 0; half >>= 1) {
+    if (threadIdx.x < half) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + half];
+      s[1][threadIdx.x] += s[1][threadIdx.x + half];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[0] = s[0][0];
+    sums[1] = s[1][0];
+  }
+}
+
+} // namespace
+
+void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
+                 float *fields, float4 *adjoint, float2 *tileTotals, double *sums) {
+  const uint32_t gridW = (width + 15) / 16, tiles = gridW * ((height + 15) / 16);
+  const uchar4 *t = reinterpret_cast<const uchar4 *>(target);
+  hipLaunchKernelGGL(ssim_kernel, dim3(tiles), dim3(256), 0, stream, image, t, width, height, gridW, fields, tileTotals);
+  hipLaunchKernelGGL(loss_kernel, dim3(tiles), dim3(256), 0, stream, image, t, width, height, gridW, lambda, fields, adjoint,
+                     tileTotals);
+  hipLaunchKernelGGL(total_kernel, dim3(1), dim3(256), 0, stream, tileTotals, tiles, sums);
+}
+
+} // namespace vsplat

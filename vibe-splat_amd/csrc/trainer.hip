@@ -1,0 +1,654 @@
+// The trainer's host side (src/train/trainer.cpp:157-863 for what it does; the
+// layout is this project's): buffers sized for the view, the passes queued on
+// one HIP stream, one wait per step for the counts and loss totals.
+//
+// The reference reads the tile-entry count back between two submissions to
+// size the entry buffers (trainer.cpp:328-375). Here the whole step is queued
+// at once against the entry capacity already reserved; the binning stages
+// leave the count on the device, every later kernel reads it from there, and
+// a view that does not fit sets a flag that makes the rest of the step a no-op
+// (nothing is written to the parameters). The host then grows the reservation,
+// within the memory budget, and runs the step again. In steady state that is
+// one synchronisation per step and no host work between kernels.
+#include "trainer.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace vsplat {
+
+namespace {
+
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw Error(VSPLAT_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+uint32_t half_words(uint32_t coefficients) { return 3u + (3u * (coefficients - 1u) + 1u) / 2u; }
+
+// Float to IEEE binary16, round to nearest even (as the device's float_to_half).
+uint16_t float_to_half(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u, a = u & 0x7FFFFFFFu;
+  if (a >= 0x7F800000u) return static_cast<uint16_t>(sign | 0x7C00u | (a > 0x7F800000u ? 0x200u | ((a >> 13) & 0x3FFu) : 0u));
+  if (a >= 0x477FF000u) return static_cast<uint16_t>(sign | 0x7C00u);
+  if (a < 0x38800000u) {
+    float v;
+    std::memcpy(&v, &a, 4);
+    return static_cast<uint16_t>(sign | static_cast<uint32_t>(std::nearbyint(v * 16777216.0f)));
+  }
+  return static_cast<uint16_t>(sign | ((a + 0x0FFFu + ((a >> 13) & 1u) - 0x38000000u) >> 13));
+}
+
+float half_to_float(uint32_t h) {
+  const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  uint32_t bits;
+  if (e == 0) {
+    const float v = static_cast<float>(m) * 5.9604644775390625e-08f;
+    return sign ? -v : v;
+  }
+  bits = e == 31 ? (sign | 0x7F800000u | (m << 13)) : (sign | ((e + 112u) << 23) | (m << 13));
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+} // namespace
+
+void DeviceArray::ensure(size_t bytes) {
+  if (bytes <= bytes_ && ptr_ != nullptr) return;
+  release();
+  check(hipMalloc(&ptr_, std::max<size_t>(bytes, 256)), "hipMalloc");
+  bytes_ = std::max<size_t>(bytes, 256);
+}
+void DeviceArray::release() {
+  if (ptr_ != nullptr) (void)hipFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+void PinnedArray::ensure(size_t bytes) {
+  if (bytes <= bytes_ && ptr_ != nullptr) return;
+  release();
+  check(hipHostMalloc(&ptr_, std::max<size_t>(bytes, 256), hipHostMallocDefault), "hipHostMalloc");
+  bytes_ = std::max<size_t>(bytes, 256);
+}
+void PinnedArray::release() {
+  if (ptr_ != nullptr) (void)hipHostFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+
+Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) : options_(options) {
+  if (c.count == 0) throw Error(VSPLAT_RUNTIME_ERROR, "nothing to train: the cloud is empty");
+  if (c.sh_degree < 0 || c.sh_degree > 3) throw Error(VSPLAT_INVALID_ARGUMENT, "the SH degree must be 0 to 3");
+  if (c.positions == nullptr || c.scales == nullptr || c.rotations == nullptr || c.opacities == nullptr || c.sh == nullptr) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, "the cloud has a null array");
+  }
+  if (options.position_noise != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "position noise is not supported by this build");
+  if (options.pose_gradients != 0) throw Error(VSPLAT_UNSUPPORTED, "pose gradients are not supported by this build");
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) throw Error(VSPLAT_DEVICE_ERROR, "no GPU is visible");
+  count_ = c.count;
+  shDegree_ = activeShDegree_ = c.sh_degree;
+  const uint32_t coefficients = static_cast<uint32_t>((c.sh_degree + 1) * (c.sh_degree + 1));
+  shFloats_ = 3u * coefficients;
+  shWords_ = half_words(coefficients);
+  // Float atomics are the default on this GPU, as on any with them (trainer.cpp:181).
+  fixedPoint_ = options.force_fixed_point != 0;
+  check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto &e : events_) check(hipEventCreate(&e), "hipEventCreate");
+
+  const size_t n = count_;
+  positions_.ensure(n * 12);
+  scales_.ensure(n * 12);
+  rotations_.ensure(n * 16);
+  opacities_.ensure(n * 4);
+  shWordsBuf_.ensure(n * shWords_ * 4);
+  check(hipMemcpy(positions_.as<float>(), c.positions, n * 12, hipMemcpyHostToDevice), "upload positions");
+  check(hipMemcpy(scales_.as<float>(), c.scales, n * 12, hipMemcpyHostToDevice), "upload scales");
+  check(hipMemcpy(rotations_.as<float>(), c.rotations, n * 16, hipMemcpyHostToDevice), "upload rotations");
+  check(hipMemcpy(opacities_.as<float>(), c.opacities, n * 4, hipMemcpyHostToDevice), "upload opacities");
+  // pack_half_sh (src/splat/gaussian_cloud.cpp:537-550): DC as floats, the rest as halves.
+  std::vector<uint32_t> words(n * shWords_, 0u);
+  for (size_t i = 0; i < n; ++i) {
+    const float *in = c.sh + i * shFloats_;
+    uint32_t *to = words.data() + i * shWords_;
+    for (uint32_t j = 0; j < 3; ++j) std::memcpy(&to[j], &in[j], 4);
+    for (uint32_t j = 3; j < shFloats_; ++j) to[3 + (j - 3) / 2] |= static_cast<uint32_t>(float_to_half(in[j])) << (16 * ((j - 3) & 1));
+  }
+  check(hipMemcpy(shWordsBuf_.as<uint32_t>(), words.data(), words.size() * 4, hipMemcpyHostToDevice), "upload SH");
+
+  const size_t stateFloats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
+  for (int g = 0; g < 5; ++g) {
+    state_[g].ensure(stateFloats[g] * 4);
+    launch_fill_u32(stream_, state_[g].as<uint32_t>(), stateFloats[g], 0u);
+  }
+  splats_.ensure(n * sizeof(Splat));
+  keys_.ensure(n * 4);
+  values_.ensure(n * 4);
+  tmpKeys_.ensure(n * 4);
+  tmpValues_.ensure(n * 4);
+  coverage_.ensure(n * 4);
+  if (fixedPoint_) {
+    rawFixed_.ensure(n * kRawPerSplat * 8);
+    launch_fill_u32(stream_, rawFixed_.as<uint32_t>(), n * kRawPerSplat * 2, 0u);
+  } else {
+    rawFloat_.ensure(n * kRawPerSplat * 4);
+    launch_fill_u32(stream_, rawFloat_.as<uint32_t>(), n * kRawPerSplat, 0u);
+  }
+  shPass_.ensure(n * kShPassFloats * 4);
+  launch_fill_u32(stream_, shPass_.as<uint32_t>(), n * kShPassFloats, 0u);
+  stats_.ensure(n * 32);
+  launch_fill_u32(stream_, stats_.as<uint32_t>(), n * 8, 0u);
+  filter3d_.ensure(n * 4);
+  launch_fill_u32(stream_, filter3d_.as<uint32_t>(), n, 0u);
+  scanScratch_.ensure(scan_scratch_words(n) * 4);
+  sortScratch_.ensure(radix_sort_scratch_words(n) * 4);
+  counters_.ensure(kCounterWords * 4);
+  launch_fill_u32(stream_, counters_.as<uint32_t>(), kCounterWords, 0u);
+  sums_.ensure(2 * sizeof(double));
+  readback_.ensure(kCounterWords * 4 + 2 * sizeof(double));
+  check(hipStreamSynchronize(stream_), "trainer set-up");
+}
+
+Trainer::~Trainer() {
+  {
+    std::lock_guard<std::mutex> lock(prefetchMutex_);
+    stopWorker_ = true;
+  }
+  prefetchChanged_.notify_all();
+  if (prefetchWorker_.joinable()) prefetchWorker_.join();
+  if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
+  for (auto &e : events_)
+    if (e != nullptr) (void)hipEventDestroy(e);
+  if (stream_ != nullptr) (void)hipStreamDestroy(stream_);
+}
+
+ViewParams Trainer::view_params(const vsplat_camera &cam) const {
+  ViewParams v{};
+  std::memcpy(v.view, cam.view, sizeof(v.view));
+  v.position[0] = cam.position[0];
+  v.position[1] = cam.position[1];
+  v.position[2] = cam.position[2];
+  v.nearPlane = options_.near_plane;
+  v.fx = cam.fx;
+  v.fy = cam.fy;
+  v.cx = cam.cx;
+  v.cy = cam.cy;
+  // upload_camera (trainer.cpp:316-317), in float.
+  v.tanHalfX = 0.5f * static_cast<float>(cam.width) / cam.fx;
+  v.tanHalfY = 0.5f * static_cast<float>(cam.height) / cam.fy;
+  v.width = static_cast<uint32_t>(cam.width);
+  v.height = static_cast<uint32_t>(cam.height);
+  v.count = count_;
+  v.shDegree = static_cast<uint32_t>(activeShDegree_);
+  v.shWords = shWords_;
+  v.shCoefficients = static_cast<uint32_t>((activeShDegree_ + 1) * (activeShDegree_ + 1));
+  return v;
+}
+
+void Trainer::ensure_size(int width, int height) {
+  if (width <= 0 || height <= 0) throw Error(VSPLAT_RUNTIME_ERROR, "bad image size");
+  if (width == width_ && height == height_) return;
+  const uint64_t gridW = div_up(static_cast<uint64_t>(width), kTileSize), gridH = div_up(static_cast<uint64_t>(height), kTileSize);
+  if (gridW * gridH > (1u << 24)) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too large for the tile index");
+  width_ = width;
+  height_ = height;
+  gridW_ = static_cast<uint32_t>(gridW);
+  gridH_ = static_cast<uint32_t>(gridH);
+  tiles_ = gridW_ * gridH_;
+  tileBits_ = 1;
+  while ((uint64_t{1} << tileBits_) < tiles_) ++tileBits_;
+  const size_t pixels = static_cast<size_t>(width) * static_cast<size_t>(height);
+  image_.ensure(pixels * 16);
+  stops_.ensure(pixels * 4);
+  adjoint_.ensure(pixels * 16);
+  fields_.ensure(pixels * 36);
+  target_.ensure(pixels * 4);
+  ranges_.ensure(static_cast<size_t>(tiles_) * 8);
+  tileTotals_.ensure(static_cast<size_t>(tiles_) * 8);
+}
+
+void Trainer::grow_entries(uint64_t entries) {
+  if (entries <= entryCapacity_) return;
+  if (entries > 0xFFFFFFFFull) throw Error(VSPLAT_BUDGET_EXCEEDED, std::to_string(entries) + " tile entries exceed the 32-bit index");
+  const uint64_t budget = options_.memory_budget;
+  const uint64_t current = entryCapacity_ * VSPLAT_BYTES_PER_TILE_ENTRY;
+  const uint64_t others = gpu_bytes() - current;
+  // Checked before allocating: running out of memory is not recoverable, this is (trainer.cpp:354-360).
+  if (others + entries * VSPLAT_BYTES_PER_TILE_ENTRY > budget) {
+    throw Error(VSPLAT_BUDGET_EXCEEDED, std::to_string(entries) + " tile entries do not fit the memory budget");
+  }
+  const uint64_t wanted = std::min<uint64_t>(entries + entries / 2 + 1024, 0xFFFFFFFFull);
+  const uint64_t affordable = budget == ~uint64_t{0} ? wanted : (budget - others) / VSPLAT_BYTES_PER_TILE_ENTRY;
+  const uint64_t capacity = std::max(std::min(wanted, affordable), entries);
+  check(hipStreamSynchronize(stream_), "grow entries");
+  for (DeviceArray *a : {&entryKeys_, &entryValues_, &tmpEntryKeys_, &tmpEntryValues_}) {
+    a->release();
+    a->ensure(capacity * 4);
+  }
+  sortScratch_.ensure(std::max(radix_sort_scratch_words(capacity), radix_sort_scratch_words(count_)) * 4);
+  entryCapacity_ = capacity;
+}
+
+void Trainer::reserve_entries(uint64_t entries) { grow_entries(entries); }
+
+void Trainer::mark(const char *stage) {
+  if (marks_.size() + 1 > static_cast<size_t>(kMaxMarks)) return;
+  marks_.push_back(stage);
+  check(hipEventRecord(events_[marks_.size()], stream_), "hipEventRecord");
+}
+
+void Trainer::finish_timing() {
+  for (size_t m = 0; m < marks_.size(); ++m) {
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, events_[m], events_[m + 1]), "hipEventElapsedTime");
+    auto it = std::find_if(stageTotals_.begin(), stageTotals_.end(), [&](const auto &s) { return s.first == marks_[m]; });
+    if (it == stageTotals_.end()) {
+      stageTotals_.emplace_back(marks_[m], static_cast<double>(ms));
+    } else {
+      it->second += static_cast<double>(ms);
+    }
+  }
+  marks_.clear();
+}
+
+std::vector<std::pair<std::string, double>> Trainer::timings() const {
+  auto out = stageTotals_;
+  for (auto &s : out) s.second /= static_cast<double>(std::max<uint64_t>(timedIterations_, 1));
+  return out;
+}
+
+void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates,
+                             bool timed) {
+  hipStream_t s = stream_;
+  marks_.clear();
+  if (timed) check(hipEventRecord(events_[0], s), "hipEventRecord");
+  uint32_t *counters = counters_.as<uint32_t>();
+  launch_fill_u32(s, counters, 3, 0u); // not the sticky overflow word
+  launch_project(s, positions_.as<float>(), scales_.as<float>(), rotations_.as<float>(), opacities_.as<float>(),
+                 shWordsBuf_.as<uint32_t>(), filterSet_ ? filter3d_.as<float>() : nullptr, v, options_.mip_filter_2d != 0, dropoutNow_,
+                 shRetained_, shSeed_, splats_.as<Splat>(), keys_.as<uint32_t>(), values_.as<uint32_t>());
+  if (timed) mark("project");
+  launch_radix_sort(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tmpKeys_.as<uint32_t>(), tmpValues_.as<uint32_t>(), nullptr, count_,
+                    32, sortScratch_.as<uint32_t>());
+  if (timed) mark("depth sort");
+  launch_bin_count(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), count_, gridW_, gridH_, coverage_.as<uint32_t>(),
+                   counters);
+  launch_exclusive_scan(s, coverage_.as<uint32_t>(), count_, scanScratch_.as<uint32_t>(), counters + kEntries);
+  launch_bin_check(s, counters, static_cast<uint32_t>(entryCapacity_));
+  if (timed) mark("count tiles");
+  launch_fill_u32(s, ranges_.as<uint32_t>(), static_cast<uint64_t>(tiles_) * 2, 0u);
+  if (entryCapacity_ > 0) {
+    launch_bin_emit(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), count_, gridW_, gridH_, coverage_.as<uint32_t>(),
+                    counters, entryKeys_.as<uint32_t>(), entryValues_.as<uint32_t>());
+    launch_radix_sort(s, entryKeys_.as<uint32_t>(), entryValues_.as<uint32_t>(), tmpEntryKeys_.as<uint32_t>(), tmpEntryValues_.as<uint32_t>(),
+                      counters + kEntriesUsed, static_cast<uint32_t>(entryCapacity_), tileBits_, sortScratch_.as<uint32_t>());
+    launch_tile_ranges(s, entryKeys_.as<uint32_t>(), counters, static_cast<uint32_t>(entryCapacity_), ranges_.as<uint2>());
+  }
+  if (timed) mark("tile lists");
+  launch_forward(s, splats_.as<Splat>(), entryValues_.as<uint32_t>(), ranges_.as<uint2>(), v.width, v.height, options_.background,
+                 image_.as<float4>(), stops_.as<uint32_t>());
+  if (timed) mark("composite");
+  if (mode == kModeRender) return;
+  launch_loss(s, image_.as<float4>(), deviceTarget, v.width, v.height, options_.lambda_ssim, fields_.as<float>(), adjoint_.as<float4>(),
+              tileTotals_.as<float2>(), sums_.as<double>());
+  if (fixedPoint_) {
+    launch_fill_u32(s, rawFixed_.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat * 2, 0u);
+  } else {
+    launch_fill_u32(s, rawFloat_.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat, 0u);
+  }
+  if (timed) mark("loss");
+  // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
+  // a resolution of 1/64 (trainer.cpp:586-591).
+  const float fixedScale = 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f;
+  launch_backward(s, fixedPoint_, splats_.as<Splat>(), entryValues_.as<uint32_t>(), ranges_.as<uint2>(), v.width, v.height,
+                  options_.background, image_.as<float4>(), adjoint_.as<float4>(), stops_.as<uint32_t>(), fixedScale, rawFixed_.as<int64_t>(),
+                  rawFloat_.as<float>());
+  if (timed) mark("backward");
+  enqueue_chain(v, mode, rates);
+  if (timed) mark(mode == kModeStep ? "chain rule and step" : "chain rule");
+}
+
+void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_learning_rates *rates) {
+  ChainParams p{};
+  p.opacityReg = options_.opacity_reg / static_cast<float>(count_);
+  p.scaleReg = options_.scale_reg / static_cast<float>(count_);
+  p.mipFilter2d = options_.mip_filter_2d ? 1u : 0u;
+  p.shDropout = dropoutNow_;
+  p.shRetained = shRetained_;
+  p.shSeed = shSeed_;
+  p.mode = mode;
+  p.flags = (options_.selective_adam ? kFlagSelective : 0u) | (statisticsNext_ ? kFlagStatistics : 0u);
+  p.beta1 = 0.9f;
+  p.beta2 = 0.999f;
+  p.epsilon = 1e-15f; // trainer.cpp:23
+  if (mode == kModeStep) {
+    const uint32_t step = adamStep_ + 1;
+    p.rates[0] = rates->position;
+    p.rates[1] = rates->scale;
+    p.rates[2] = rates->rotation;
+    p.rates[3] = rates->opacity;
+    p.rates[4] = rates->sh;
+    p.restScale = rates->sh_rest_scale;
+    p.correction1 = 1.0f - std::pow(0.9f, static_cast<float>(step)); // trainer.cpp:467-468
+    p.correction2 = 1.0f - std::pow(0.999f, static_cast<float>(step));
+    p.seed = step;
+  }
+  p.shFloats = shFloats_;
+  p.fixedPoint = fixedPoint_ ? 1u : 0u;
+  p.fixedScale = 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f;
+  if (mode == kModeGradients) {
+    const size_t n = count_;
+    const size_t floats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
+    for (int g = 0; g < 5; ++g) grad_[g].ensure(floats[g] * 4);
+  }
+  ChainBuffers b{};
+  b.positions = positions_.as<float>();
+  b.scales = scales_.as<float>();
+  b.rotations = rotations_.as<float>();
+  b.opacities = opacities_.as<float>();
+  b.shWords = shWordsBuf_.as<uint32_t>();
+  b.filter3d = filterSet_ ? filter3d_.as<float>() : nullptr;
+  b.splats = splats_.as<Splat>();
+  b.rawFixed = rawFixed_.as<int64_t>();
+  b.rawFloat = rawFloat_.as<float>();
+  b.shPass = shPass_.as<float>();
+  b.stats = stats_.as<float>();
+  b.statePos = state_[0].as<uint32_t>();
+  b.stateScale = state_[1].as<uint32_t>();
+  b.stateRot = state_[2].as<uint32_t>();
+  b.stateOpacity = state_[3].as<uint32_t>();
+  b.stateSh = state_[4].as<uint32_t>();
+  b.gradPos = grad_[0].as<float>();
+  b.gradScale = grad_[1].as<float>();
+  b.gradRot = grad_[2].as<float>();
+  b.gradOpacity = grad_[3].as<float>();
+  b.gradSh = grad_[4].as<float>();
+  launch_chain(stream_, b, v, p, counters_.as<uint32_t>());
+}
+
+vsplat_step_stats Trainer::read_stats(bool &overflow) {
+  uint8_t *host = readback_.as<uint8_t>();
+  check(hipMemcpyAsync(host, counters_.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToHost, stream_), "read counters");
+  check(hipMemcpyAsync(host + kCounterWords * 4, sums_.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToHost, stream_), "read sums");
+  check(hipStreamSynchronize(stream_), "step");
+  uint32_t counters[kCounterWords];
+  double sums[2];
+  std::memcpy(counters, host, sizeof(counters));
+  std::memcpy(sums, host + kCounterWords * 4, sizeof(sums));
+  overflow = counters[kOverflow] != 0;
+  vsplat_step_stats st{};
+  // trainer.cpp:606-620: the per-tile totals summed in double, then in float as the reference.
+  const double perValue = 1.0 / (3.0 * static_cast<double>(width_) * static_cast<double>(height_));
+  st.l1 = static_cast<float>(sums[1] * perValue);
+  st.ssim = static_cast<float>(sums[0] * perValue);
+  st.loss = (1.0f - options_.lambda_ssim) * st.l1 + options_.lambda_ssim * (1.0f - st.ssim);
+  st.visible_splats = counters[kVisible];
+  st.tile_entries = counters[kEntries];
+  return st;
+}
+
+void Trainer::wait_prefetch() {
+  std::unique_lock<std::mutex> lock(prefetchMutex_);
+  prefetchChanged_.wait(lock, [&] { return !prefetchPending_; });
+}
+
+void Trainer::prefetch_target(const uint8_t *target, size_t bytes) {
+  if (target == nullptr || bytes == 0) return;
+  wait_prefetch();
+  next_.ensure(bytes);
+  if (!prefetchWorker_.joinable()) {
+    prefetchWorker_ = std::thread([this] {
+      std::unique_lock<std::mutex> lock(prefetchMutex_);
+      for (;;) {
+        prefetchChanged_.wait(lock, [&] { return prefetchPending_ || stopWorker_; });
+        if (stopWorker_) return;
+        std::memcpy(next_.as<uint8_t>(), prefetched_, prefetchedBytes_);
+        prefetchPending_ = false;
+        prefetchChanged_.notify_all();
+      }
+    });
+  }
+  {
+    std::lock_guard<std::mutex> lock(prefetchMutex_);
+    prefetched_ = target;
+    prefetchedBytes_ = bytes;
+    prefetchPending_ = true;
+  }
+  prefetchChanged_.notify_all();
+}
+
+void Trainer::stage_target(const uint8_t *target, size_t bytes) {
+  if (prefetched_ == target && prefetchedBytes_ == bytes) {
+    // Started during the previous step (trainer.cpp:572-576).
+    wait_prefetch();
+    staging_.swap(next_);
+    prefetched_ = nullptr;
+  } else {
+    staging_.ensure(bytes);
+    std::memcpy(staging_.as<uint8_t>(), target, bytes);
+  }
+  check(hipMemcpyAsync(target_.as<uint8_t>(), staging_.as<uint8_t>(), bytes, hipMemcpyHostToDevice, stream_), "upload target");
+}
+
+vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *target, size_t bytes, uint32_t mode,
+                               const vsplat_learning_rates *rates, const uint8_t *next, size_t nextBytes) {
+  ensure_size(camera.width, camera.height);
+  const size_t expected = static_cast<size_t>(width_) * static_cast<size_t>(height_) * 4;
+  if (target == nullptr || bytes != expected) throw Error(VSPLAT_RUNTIME_ERROR, "the target image does not match the camera");
+  if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
+  if (rates != nullptr && rates->trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
+  // Dropout trains; scoring and statistics see the whole model (trainer.cpp:569).
+  dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
+  stage_target(target, bytes);
+  if (next != nullptr && nextBytes > 0) prefetch_target(next, nextBytes);
+  const ViewParams v = view_params(camera);
+  statisticsNext_ = true; // run() always adds to the densification statistics (trainer.cpp:600)
+  vsplat_step_stats stats{};
+  for (int attempt = 0;; ++attempt) {
+    enqueue_passes(v, target_.as<uint8_t>(), mode, rates, true);
+    bool overflow = false;
+    stats = read_stats(overflow);
+    if (!overflow) break;
+    marks_.clear();
+    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
+    if (attempt > 0 && stats.tile_entries <= entryCapacity_) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+    grow_entries(stats.tile_entries);
+  }
+  finish_timing();
+  ++timedIterations_;
+  if (mode == kModeGradients) haveGradients_ = true;
+  if (mode == kModeStep) ++adamStep_;
+  lastView_ = v;
+  return stats;
+}
+
+void Trainer::render(const vsplat_camera &camera, float *out) {
+  ensure_size(camera.width, camera.height);
+  dropoutNow_ = 0.0f; // reports and evaluation see the whole model
+  const ViewParams v = view_params(camera);
+  for (int attempt = 0;; ++attempt) {
+    enqueue_passes(v, nullptr, kModeRender, nullptr, false);
+    bool overflow = false;
+    const vsplat_step_stats stats = read_stats(overflow);
+    if (!overflow) break;
+    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
+    if (attempt > 0 && stats.tile_entries <= entryCapacity_) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+    grow_entries(stats.tile_entries);
+  }
+  const size_t pixels = static_cast<size_t>(width_) * static_cast<size_t>(height_);
+  check(hipMemcpy(out, image_.as<float4>(), pixels * 16, hipMemcpyDeviceToHost), "download image");
+}
+
+void Trainer::apply_gradients(const vsplat_learning_rates &rates) {
+  if (width_ == 0) throw Error(VSPLAT_LOGIC_ERROR, "no gradients: compute_gradients() comes first");
+  if (rates.trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
+  dropoutNow_ = 0.0f;
+  statisticsNext_ = false;
+  marks_.clear();
+  check(hipEventRecord(events_[0], stream_), "hipEventRecord");
+  enqueue_chain(lastView_, kModeStep, &rates);
+  mark("chain rule and step");
+  check(hipStreamSynchronize(stream_), "apply gradients");
+  finish_timing();
+  ++adamStep_;
+}
+
+void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates) {
+  if (mode != kModeGradients && mode != kModeStep) throw Error(VSPLAT_INVALID_ARGUMENT, "mode must be 1 (gradients) or 2 (step)");
+  if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
+  if (deviceTarget == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no target");
+  ensure_size(camera.width, camera.height);
+  dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
+  statisticsNext_ = true;
+  const ViewParams v = view_params(camera);
+  enqueue_passes(v, deviceTarget, mode, rates, false);
+  if (mode == kModeGradients) haveGradients_ = true;
+  if (mode == kModeStep) ++adamStep_;
+  lastView_ = v;
+}
+
+vsplat_step_stats Trainer::synchronize() {
+  bool overflow = false;
+  const vsplat_step_stats stats = read_stats(overflow);
+  if (overflow) {
+    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
+    check(hipStreamSynchronize(stream_), "clear overflow");
+    throw Error(VSPLAT_RUNTIME_ERROR, "a queued view needed more tile entries than were reserved; its step was skipped");
+  }
+  return stats;
+}
+
+void Trainer::gradients(float *positions, float *scales, float *rotations, float *opacities, float *sh) const {
+  if (!haveGradients_) throw Error(VSPLAT_LOGIC_ERROR, "no gradients: compute_gradients() stores them");
+  const size_t n = count_;
+  float *out[5] = {positions, scales, rotations, opacities, sh};
+  const size_t floats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
+  for (int g = 0; g < 5; ++g) {
+    if (out[g] != nullptr) check(hipMemcpy(out[g], grad_[g].as<float>(), floats[g] * 4, hipMemcpyDeviceToHost), "download gradients");
+  }
+}
+
+void Trainer::cloud(float *positions, float *scales, float *rotations, float *opacities, float *sh) const {
+  const size_t n = count_;
+  check(hipStreamSynchronize(stream_), "cloud");
+  if (positions) check(hipMemcpy(positions, positions_.as<float>(), n * 12, hipMemcpyDeviceToHost), "download positions");
+  if (scales) check(hipMemcpy(scales, scales_.as<float>(), n * 12, hipMemcpyDeviceToHost), "download scales");
+  if (rotations) check(hipMemcpy(rotations, rotations_.as<float>(), n * 16, hipMemcpyDeviceToHost), "download rotations");
+  if (opacities) check(hipMemcpy(opacities, opacities_.as<float>(), n * 4, hipMemcpyDeviceToHost), "download opacities");
+  if (sh) {
+    std::vector<uint32_t> words(n * shWords_);
+    check(hipMemcpy(words.data(), shWordsBuf_.as<uint32_t>(), words.size() * 4, hipMemcpyDeviceToHost), "download SH");
+    for (size_t i = 0; i < n; ++i) {
+      const uint32_t *in = words.data() + i * shWords_;
+      float *to = sh + i * shFloats_;
+      for (uint32_t j = 0; j < 3; ++j) std::memcpy(&to[j], &in[j], 4);
+      for (uint32_t j = 3; j < shFloats_; ++j) to[j] = half_to_float((in[3 + (j - 3) / 2] >> (16 * ((j - 3) & 1))) & 0xFFFFu);
+    }
+  }
+}
+
+void Trainer::densify_stats(float *out) const {
+  check(hipStreamSynchronize(stream_), "densify stats");
+  check(hipMemcpy(out, stats_.as<float>(), static_cast<size_t>(count_) * 32, hipMemcpyDeviceToHost), "download stats");
+}
+
+void Trainer::reset_densify_stats() {
+  launch_fill_u32(stream_, stats_.as<uint32_t>(), static_cast<uint64_t>(count_) * 8, 0u);
+  check(hipStreamSynchronize(stream_), "reset stats");
+}
+
+void Trainer::set_active_sh_degree(int degree) { activeShDegree_ = std::clamp(degree, 0, shDegree_); }
+
+void Trainer::set_filter3d(const float *filter, size_t count) {
+  if (filter == nullptr || count != count_) throw Error(VSPLAT_INVALID_ARGUMENT, "the filter must have one value per Gaussian");
+  check(hipMemcpy(filter3d_.as<float>(), filter, count * 4, hipMemcpyHostToDevice), "upload filter");
+  filterSet_ = true;
+}
+
+void Trainer::set_sh_dropout(float probability, int retained, uint32_t seed) {
+  shDropout_ = std::clamp(probability, 0.0f, 1.0f);
+  shRetained_ = static_cast<uint32_t>(std::clamp(retained, 0, 3));
+  shSeed_ = seed;
+}
+
+void Trainer::parameters(uint32_t group, void **pointer, uint64_t *bytes) const {
+  const DeviceArray *g[5] = {&positions_, &scales_, &rotations_, &opacities_, &shWordsBuf_};
+  if (group >= 5) throw Error(VSPLAT_INVALID_ARGUMENT, "parameter groups are 0 to 4");
+  const size_t n = count_;
+  const size_t size[5] = {n * 12, n * 12, n * 16, n * 4, n * shWords_ * 4};
+  if (pointer) *pointer = g[group]->as<void>();
+  if (bytes) *bytes = size[group];
+}
+
+uint64_t Trainer::bytes_per_gaussian() const {
+  const uint64_t floats = 3 + 3 + 4 + 1 + shFloats_;
+  // Values (SH as packed words) and one optimiser word per parameter.
+  const uint64_t parameters = (floats - shFloats_ + shWords_) * 4 + floats * 4;
+  // Splat, sort keys and values with copies, coverage, filter, sums, SH-pass data, statistics.
+  const uint64_t raw = fixedPoint_ ? kRawPerSplat * 8 : kRawPerSplat * 4;
+  return parameters + sizeof(Splat) + 4 * 6 + raw + kShPassFloats * 4 + 32;
+}
+
+uint64_t Trainer::gpu_bytes() const {
+  uint64_t total = bytes_per_gaussian() * count_;
+  for (const DeviceArray *a : {&image_, &stops_, &adjoint_, &fields_, &target_, &ranges_, &tileTotals_, &entryKeys_, &entryValues_,
+                               &tmpEntryKeys_, &tmpEntryValues_}) {
+    total += a->bytes();
+  }
+  for (const DeviceArray &a : grad_) total += a.bytes();
+  return total;
+}
+
+} // namespace vsplat
+
+namespace vsplat {
+
+void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uint64_t *bytes) const {
+  const uint64_t n = count_, pixels = static_cast<uint64_t>(width_) * static_cast<uint64_t>(height_);
+  uint64_t entries = 0;
+  if (name == "entry_keys" || name == "entry_values") {
+    uint32_t counters[kCounterWords];
+    check(hipStreamSynchronize(stream_), "inspect");
+    check(hipMemcpy(counters, counters_.as<uint32_t>(), sizeof(counters), hipMemcpyDeviceToHost), "inspect counters");
+    entries = counters[kEntriesUsed];
+  }
+  struct Item {
+    const char *name;
+    const DeviceArray *array;
+    uint64_t size;
+  };
+  const Item items[] = {
+      {"splats", &splats_, n * sizeof(Splat)},
+      {"keys", &keys_, n * 4},
+      {"values", &values_, n * 4},
+      {"coverage", &coverage_, n * 4},
+      {"entry_keys", &entryKeys_, entries * 4},
+      {"entry_values", &entryValues_, entries * 4},
+      {"ranges", &ranges_, static_cast<uint64_t>(tiles_) * 8},
+      {"image", &image_, pixels * 16},
+      {"stops", &stops_, pixels * 4},
+      {"adjoint", &adjoint_, pixels * 16},
+      {"fields", &fields_, pixels * 36},
+      {"tile_totals", &tileTotals_, static_cast<uint64_t>(tiles_) * 8},
+      {"raw", fixedPoint_ ? &rawFixed_ : &rawFloat_, n * kRawPerSplat * (fixedPoint_ ? 8u : 4u)},
+      {"counters", &counters_, kCounterWords * 4},
+  };
+  for (const Item &item : items) {
+    if (name != item.name) continue;
+    if (bytes != nullptr) *bytes = item.size;
+    const uint64_t copy = std::min(item.size, capacity);
+    if (out != nullptr && copy > 0) {
+      check(hipStreamSynchronize(stream_), "inspect");
+      check(hipMemcpy(out, item.array->as<void>(), copy, hipMemcpyDeviceToHost), "inspect copy");
+    }
+    return;
+  }
+  throw Error(VSPLAT_INVALID_ARGUMENT, "no buffer named " + name);
+}
+
+} // namespace vsplat

@@ -1,0 +1,160 @@
+// The trainer's host side: device buffers, the order of passes, and the
+// reads back of counts and loss totals (the reference's Trainer,
+// src/train/trainer.cpp:157-863, re-laid out for HIP).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "../../include/vsplat.h"
+#include "kernels.hpp"
+
+namespace vsplat {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string &what) : std::runtime_error(what), status(s) {}
+};
+
+// One hipMalloc'd array; grows by reallocation, contents are not kept.
+class DeviceArray {
+public:
+  DeviceArray() = default;
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  ~DeviceArray() { release(); }
+  void ensure(size_t bytes);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(ptr_); }
+  size_t bytes() const { return bytes_; }
+
+private:
+  void *ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class PinnedArray {
+public:
+  PinnedArray() = default;
+  PinnedArray(const PinnedArray &) = delete;
+  PinnedArray &operator=(const PinnedArray &) = delete;
+  ~PinnedArray() { release(); }
+  void ensure(size_t bytes);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(ptr_); }
+  size_t bytes() const { return bytes_; }
+  void swap(PinnedArray &other) {
+    std::swap(ptr_, other.ptr_);
+    std::swap(bytes_, other.bytes_);
+  }
+
+private:
+  void *ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class Trainer {
+public:
+  Trainer(const vsplat_cloud &cloud, const vsplat_trainer_options &options);
+  ~Trainer();
+  Trainer(const Trainer &) = delete;
+  Trainer &operator=(const Trainer &) = delete;
+
+  void render(const vsplat_camera &camera, float *out);
+  vsplat_step_stats run(const vsplat_camera &camera, const uint8_t *target, size_t bytes, uint32_t mode,
+                        const vsplat_learning_rates *rates, const uint8_t *next, size_t nextBytes);
+  void apply_gradients(const vsplat_learning_rates &rates);
+  void prefetch_target(const uint8_t *target, size_t bytes);
+
+  void enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates);
+  vsplat_step_stats synchronize();
+  void reserve_entries(uint64_t entries);
+
+  void gradients(float *positions, float *scales, float *rotations, float *opacities, float *sh) const;
+  void cloud(float *positions, float *scales, float *rotations, float *opacities, float *sh) const;
+  void densify_stats(float *out) const;
+  void reset_densify_stats();
+  void set_active_sh_degree(int degree);
+  void set_filter3d(const float *filter, size_t count);
+  void set_sh_dropout(float probability, int retained, uint32_t seed);
+  void parameters(uint32_t group, void **pointer, uint64_t *bytes) const;
+  std::vector<std::pair<std::string, double>> timings() const;
+  void inspect(const std::string &buffer, void *out, uint64_t capacity, uint64_t *bytes) const;
+
+  uint32_t count() const { return count_; }
+  int sh_degree() const { return shDegree_; }
+  int active_sh_degree() const { return activeShDegree_; }
+  bool fixed_point() const { return fixedPoint_; }
+  uint64_t gpu_bytes() const;
+  uint64_t bytes_per_gaussian() const;
+  uint64_t staging_bytes() const { return staging_.bytes(); }
+  hipStream_t stream() const { return stream_; }
+
+private:
+  ViewParams view_params(const vsplat_camera &camera) const;
+  void ensure_size(int width, int height);
+  void grow_entries(uint64_t entries);
+  void stage_target(const uint8_t *target, size_t bytes);
+  void enqueue_passes(const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates,
+                      bool timed);
+  void enqueue_chain(const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates);
+  void mark(const char *stage);
+  void finish_timing();
+  void wait_prefetch();
+  vsplat_step_stats read_stats(bool &overflow);
+
+  vsplat_trainer_options options_;
+  uint32_t count_ = 0;
+  int shDegree_ = 0, activeShDegree_ = 0;
+  uint32_t shFloats_ = 3, shWords_ = 3;
+  bool fixedPoint_ = true;
+  uint32_t adamStep_ = 0;
+  float shDropout_ = 0.0f, dropoutNow_ = 0.0f;
+  uint32_t shRetained_ = 0, shSeed_ = 0;
+  int width_ = 0, height_ = 0;
+  uint32_t gridW_ = 0, gridH_ = 0, tiles_ = 0, tileBits_ = 0;
+  uint64_t entryCapacity_ = 0;
+  bool haveGradients_ = false;
+  bool statisticsNext_ = true; // the next chain pass adds to the densification statistics
+  ViewParams lastView_{};
+
+  hipStream_t stream_ = nullptr;
+  // Parameters and optimiser state, in GaussianCloud's layout; SH packed as halves.
+  DeviceArray positions_, scales_, rotations_, opacities_, shWordsBuf_;
+  DeviceArray state_[5], grad_[5];
+  // Per Gaussian.
+  DeviceArray splats_, keys_, values_, tmpKeys_, tmpValues_, coverage_, rawFixed_, rawFloat_, shPass_, stats_, filter3d_;
+  bool filterSet_ = false;
+  // Per tile entry.
+  DeviceArray entryKeys_, entryValues_, tmpEntryKeys_, tmpEntryValues_;
+  // Per pixel and per tile.
+  DeviceArray image_, stops_, adjoint_, fields_, target_, ranges_, tileTotals_;
+  // Scratch and counters.
+  DeviceArray sortScratch_, scanScratch_, counters_, sums_;
+  PinnedArray staging_, next_, readback_;
+
+  // Prefetch worker (trainer.cpp:539-564): copies the next step's target into pinned memory.
+  std::mutex prefetchMutex_;
+  std::condition_variable prefetchChanged_;
+  std::thread prefetchWorker_;
+  bool prefetchPending_ = false, stopWorker_ = false;
+  const uint8_t *prefetched_ = nullptr;
+  size_t prefetchedBytes_ = 0;
+
+  // Stage timing with events (trainer.cpp:201-240).
+  static constexpr int kMaxMarks = 12;
+  hipEvent_t events_[kMaxMarks + 1] = {};
+  std::vector<const char *> marks_;
+  std::vector<std::pair<std::string, double>> stageTotals_;
+  uint64_t timedIterations_ = 0;
+};
+
+} // namespace vsplat

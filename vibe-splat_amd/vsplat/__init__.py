@@ -1,0 +1,418 @@
+"""vsplat: Python access to libvsplat.so, the MI355X differentiable Gaussian-splat
+tile rasterizer, through its C interface (include/vsplat.h).
+
+``Trainer`` mirrors the reference's ``class Trainer`` (src/train/trainer.hpp:89-270):
+``render``, ``compute_gradients``, ``step``, ``apply_gradients``, ``gradients``,
+``cloud`` and the rest, with the reference's errors as Python exceptions
+(``RuntimeError``, ``BudgetExceeded``, ``ValueError`` for std::invalid_argument,
+``LogicError`` for std::logic_error). There is no fallback: without the built
+library or a GPU, constructing a Trainer raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import fixtures  # noqa: F401  (re-exported)
+from .fixtures import Camera, Cloud
+
+LIBRARY = Path(__file__).resolve().parent / "libvsplat.so"
+
+OK, RUNTIME_ERROR, BUDGET_EXCEEDED, INVALID_ARGUMENT, LOGIC_ERROR, DEVICE_ERROR, UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
+BYTES_PER_TILE_ENTRY = 16
+MODE_STATISTICS, MODE_GRADIENTS, MODE_STEP = 0, 1, 2
+
+EXPORTS = [
+    "vsplat_default_options", "vsplat_default_rates", "vsplat_trainer_create", "vsplat_trainer_destroy", "vsplat_render",
+    "vsplat_compute_gradients", "vsplat_collect_statistics", "vsplat_apply_gradients", "vsplat_step",
+    "vsplat_prefetch_target", "vsplat_get_gradients", "vsplat_get_cloud", "vsplat_get_densify_stats",
+    "vsplat_reset_densify_stats", "vsplat_set_active_sh_degree", "vsplat_set_filter3d", "vsplat_set_sh_dropout",
+    "vsplat_parameters", "vsplat_gaussian_count", "vsplat_sh_degree", "vsplat_active_sh_degree",
+    "vsplat_uses_fixed_point", "vsplat_gpu_bytes", "vsplat_bytes_per_gaussian", "vsplat_staging_bytes",
+    "vsplat_timings", "vsplat_last_error", "vsplat_enqueue", "vsplat_synchronize", "vsplat_stream",
+    "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe",
+]
+
+
+class BudgetExceeded(RuntimeError):
+    """splat::BudgetExceeded (src/train/trainer.hpp:46-49): recoverable."""
+
+
+class LogicError(Exception):
+    """std::logic_error."""
+
+
+class Unsupported(RuntimeError):
+    """An option this build does not implement."""
+
+
+class DeviceError(RuntimeError):
+    """A HIP call failed, or there is no GPU."""
+
+
+class CCloud(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("scales", C.c_void_p), ("rotations", C.c_void_p), ("opacities", C.c_void_p),
+                ("sh", C.c_void_p), ("count", C.c_uint32), ("sh_degree", C.c_int32)]
+
+
+class CCamera(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("position", C.c_float * 3), ("fx", C.c_float), ("fy", C.c_float),
+                ("cx", C.c_float), ("cy", C.c_float), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class COptions(C.Structure):
+    _fields_ = [("lambda_ssim", C.c_float), ("background", C.c_float * 3), ("near_plane", C.c_float),
+                ("force_fixed_point", C.c_int32), ("selective_adam", C.c_int32), ("opacity_reg", C.c_float),
+                ("scale_reg", C.c_float), ("position_noise", C.c_float), ("mip_filter_2d", C.c_int32),
+                ("pose_gradients", C.c_int32), ("memory_budget", C.c_uint64)]
+
+
+class CRates(C.Structure):
+    _fields_ = [("position", C.c_float), ("scale", C.c_float), ("rotation", C.c_float), ("opacity", C.c_float),
+                ("sh", C.c_float), ("sh_rest_scale", C.c_float), ("trust_region", C.c_float)]
+
+
+class CStats(C.Structure):
+    _fields_ = [("loss", C.c_float), ("l1", C.c_float), ("ssim", C.c_float), ("visible_splats", C.c_uint32),
+                ("tile_entries", C.c_uint32), ("exposure_gradient", C.c_float * 12), ("pose_gradient", C.c_float * 6)]
+
+
+class CTiming(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("milliseconds", C.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises ImportError when it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not LIBRARY.exists():
+            raise ImportError(f"{LIBRARY} is not built (run __graft_entry__.build() or make -C vibe-splat_amd)")
+        L = C.CDLL(str(LIBRARY))
+        vp = C.c_void_p
+        L.vsplat_default_options.argtypes = [vp]
+        L.vsplat_default_options.restype = None
+        L.vsplat_default_rates.argtypes = [vp]
+        L.vsplat_default_rates.restype = None
+        L.vsplat_trainer_create.argtypes = [vp, vp, vp]
+        L.vsplat_trainer_destroy.argtypes = [vp]
+        L.vsplat_trainer_destroy.restype = None
+        L.vsplat_render.argtypes = [vp, vp, vp]
+        L.vsplat_compute_gradients.argtypes = [vp, vp, vp, C.c_size_t, vp]
+        L.vsplat_collect_statistics.argtypes = [vp, vp, vp, C.c_size_t, vp]
+        L.vsplat_apply_gradients.argtypes = [vp, vp]
+        L.vsplat_step.argtypes = [vp, vp, vp, C.c_size_t, vp, vp, C.c_size_t, vp]
+        L.vsplat_prefetch_target.argtypes = [vp, vp, C.c_size_t]
+        L.vsplat_get_gradients.argtypes = [vp] * 6
+        L.vsplat_get_cloud.argtypes = [vp] * 6
+        L.vsplat_get_densify_stats.argtypes = [vp, vp]
+        L.vsplat_reset_densify_stats.argtypes = [vp]
+        L.vsplat_set_active_sh_degree.argtypes = [vp, C.c_int32]
+        L.vsplat_set_filter3d.argtypes = [vp, vp, C.c_size_t]
+        L.vsplat_set_sh_dropout.argtypes = [vp, C.c_float, C.c_int32, C.c_uint32]
+        L.vsplat_parameters.argtypes = [vp, C.c_uint32, vp, vp]
+        for name, rt in (("vsplat_gaussian_count", C.c_uint32), ("vsplat_sh_degree", C.c_int32),
+                         ("vsplat_active_sh_degree", C.c_int32), ("vsplat_uses_fixed_point", C.c_int32),
+                         ("vsplat_gpu_bytes", C.c_uint64), ("vsplat_bytes_per_gaussian", C.c_uint64),
+                         ("vsplat_staging_bytes", C.c_uint64)):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = rt
+        L.vsplat_timings.argtypes = [vp, vp, C.c_uint32, vp]
+        L.vsplat_last_error.argtypes = []
+        L.vsplat_last_error.restype = C.c_char_p
+        L.vsplat_enqueue.argtypes = [vp, vp, vp, C.c_int32, vp]
+        L.vsplat_synchronize.argtypes = [vp, vp]
+        L.vsplat_stream.argtypes = [vp]
+        L.vsplat_stream.restype = vp
+        L.vsplat_reserve_entries.argtypes = [vp, C.c_uint64]
+        L.vsplat_inspect.argtypes = [vp, C.c_char_p, vp, C.c_uint64, vp]
+        L.vsplat_math_probe.argtypes = [C.c_int32, vp, vp, C.c_uint64]
+        _LIB = L
+    return _LIB
+
+
+def _raise(code: int, what: str):
+    if code == OK:
+        return
+    message = f"{what}: {lib().vsplat_last_error().decode(errors='replace')}"
+    if code == BUDGET_EXCEEDED:
+        raise BudgetExceeded(message)
+    if code == INVALID_ARGUMENT:
+        raise ValueError(message)
+    if code == LOGIC_ERROR:
+        raise LogicError(message)
+    if code == UNSUPPORTED:
+        raise Unsupported(message)
+    if code == DEVICE_ERROR:
+        raise DeviceError(message)
+    raise RuntimeError(message)
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class TrainerOptions:
+    """TrainerOptions (src/train/trainer.hpp:24-42)."""
+
+    lambda_ssim: float = 0.2
+    background: tuple = (0.0, 0.0, 0.0)
+    near_plane: float = 0.01
+    force_fixed_point: bool = False
+    selective_adam: bool = True
+    opacity_reg: float = 0.0
+    scale_reg: float = 0.0
+    position_noise: float = 0.0
+    mip_filter_2d: bool = False
+    pose_gradients: bool = False
+    memory_budget: int = 2 ** 64 - 1
+
+    def to_c(self) -> COptions:
+        o = COptions()
+        o.lambda_ssim = self.lambda_ssim
+        o.background[:] = [float(v) for v in self.background]
+        o.near_plane = self.near_plane
+        o.force_fixed_point = int(self.force_fixed_point)
+        o.selective_adam = int(self.selective_adam)
+        o.opacity_reg, o.scale_reg, o.position_noise = self.opacity_reg, self.scale_reg, self.position_noise
+        o.mip_filter_2d = int(self.mip_filter_2d)
+        o.pose_gradients = int(self.pose_gradients)
+        o.memory_budget = int(self.memory_budget)
+        return o
+
+
+@dataclass
+class LearningRates:
+    """LearningRates (src/train/trainer.hpp:51-61)."""
+
+    position: float = 1.6e-4
+    scale: float = 0.005
+    rotation: float = 0.001
+    opacity: float = 0.05
+    sh: float = 0.025
+    sh_rest_scale: float = 0.05
+    trust_region: float = 0.0
+
+    def to_c(self) -> CRates:
+        return CRates(self.position, self.scale, self.rotation, self.opacity, self.sh, self.sh_rest_scale, self.trust_region)
+
+
+@dataclass
+class StepStats:
+    """StepStats (src/train/trainer.hpp:63-73)."""
+
+    loss: float
+    l1: float
+    ssim: float
+    visible_splats: int
+    tile_entries: int
+
+    @staticmethod
+    def of(c: CStats) -> "StepStats":
+        return StepStats(c.loss, c.l1, c.ssim, c.visible_splats, c.tile_entries)
+
+
+@dataclass
+class Gradients:
+    positions: np.ndarray
+    scales: np.ndarray
+    rotations: np.ndarray
+    opacities: np.ndarray
+    sh: np.ndarray
+
+
+def camera_to_c(cam: Camera) -> CCamera:
+    c = CCamera()
+    c.view[:] = [float(v) for v in cam.view_column_major()]
+    c.position[:] = [float(v) for v in np.asarray(cam.position, np.float32)]
+    c.fx, c.fy, c.cx, c.cy = cam.fx, cam.fy, cam.cx, cam.cy
+    c.width, c.height = cam.width, cam.height
+    return c
+
+
+INSPECT_TYPES = {"splats": (np.float32, 12), "keys": (np.uint32, 1), "values": (np.uint32, 1), "coverage": (np.uint32, 1),
+                 "entry_keys": (np.uint32, 1), "entry_values": (np.uint32, 1), "ranges": (np.uint32, 2),
+                 "image": (np.float32, 4), "stops": (np.uint32, 1), "adjoint": (np.float32, 4), "fields": (np.float32, 9),
+                 "tile_totals": (np.float32, 2), "counters": (np.uint32, 1)}
+
+
+class Trainer:
+    """The reference's Trainer over the C interface (src/train/trainer.hpp:89-270)."""
+
+    def __init__(self, cloud: Cloud, options: TrainerOptions | None = None):
+        self._lib = lib()
+        self.options = options or TrainerOptions()
+        self._shape = cloud
+        arrays = [np.ascontiguousarray(a, np.float32) for a in
+                  (cloud.positions, cloud.scales, cloud.rotations, cloud.opacities, cloud.sh)]
+        cc = CCloud(*[_p(a) for a in arrays], cloud.count, cloud.sh_degree)
+        handle = C.c_void_p()
+        opts = self.options.to_c()
+        _raise(self._lib.vsplat_trainer_create(C.byref(cc), C.byref(opts), C.byref(handle)), "Trainer")
+        self._h = handle
+        self.count = cloud.count
+        self.sh_degree = cloud.sh_degree
+        self._next = None  # keeps a prefetched target alive until the next step
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vsplat_trainer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 (interpreter shutdown)
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- the per-step surface
+    def render(self, camera: Camera) -> np.ndarray:
+        out = np.zeros((camera.height * camera.width, 4), np.float32)
+        cam = camera_to_c(camera)
+        _raise(self._lib.vsplat_render(self._h, C.byref(cam), _p(out)), "render")
+        return out
+
+    def compute_gradients(self, camera: Camera, target: np.ndarray) -> StepStats:
+        t = np.ascontiguousarray(target, np.uint8)
+        cam, st = camera_to_c(camera), CStats()
+        _raise(self._lib.vsplat_compute_gradients(self._h, C.byref(cam), _p(t), t.nbytes, C.byref(st)), "compute_gradients")
+        return StepStats.of(st)
+
+    def collect_statistics(self, camera: Camera, target: np.ndarray) -> StepStats:
+        t = np.ascontiguousarray(target, np.uint8)
+        cam, st = camera_to_c(camera), CStats()
+        _raise(self._lib.vsplat_collect_statistics(self._h, C.byref(cam), _p(t), t.nbytes, C.byref(st)), "collect_statistics")
+        return StepStats.of(st)
+
+    def step(self, camera: Camera, target: np.ndarray, rates: LearningRates, next_target: np.ndarray | None = None) -> StepStats:
+        t = np.ascontiguousarray(target, np.uint8)
+        n = None if next_target is None else np.ascontiguousarray(next_target, np.uint8)
+        cam, st, r = camera_to_c(camera), CStats(), rates.to_c()
+        code = self._lib.vsplat_step(self._h, C.byref(cam), _p(t), t.nbytes, C.byref(r), _p(n), 0 if n is None else n.nbytes,
+                                     C.byref(st))
+        self._next = n
+        _raise(code, "step")
+        return StepStats.of(st)
+
+    def prefetch_target(self, target: np.ndarray):
+        t = np.ascontiguousarray(target, np.uint8)
+        self._next = t
+        _raise(self._lib.vsplat_prefetch_target(self._h, _p(t), t.nbytes), "prefetch_target")
+
+    def apply_gradients(self, rates: LearningRates):
+        r = rates.to_c()
+        _raise(self._lib.vsplat_apply_gradients(self._h, C.byref(r)), "apply_gradients")
+
+    # --- results
+    def gradients(self) -> Gradients:
+        n = self.count
+        coefficients = (self.sh_degree + 1) ** 2
+        g = Gradients(np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32),
+                      np.zeros(n, np.float32), np.zeros((n, coefficients, 3), np.float32))
+        _raise(self._lib.vsplat_get_gradients(self._h, _p(g.positions), _p(g.scales), _p(g.rotations), _p(g.opacities), _p(g.sh)),
+               "gradients")
+        return g
+
+    def cloud(self) -> Cloud:
+        c = Cloud.zeros(self.count, self.sh_degree)
+        _raise(self._lib.vsplat_get_cloud(self._h, _p(c.positions), _p(c.scales), _p(c.rotations), _p(c.opacities), _p(c.sh)),
+               "cloud")
+        return c
+
+    def densify_stats(self) -> np.ndarray:
+        out = np.zeros((self.count, 8), np.float32)
+        _raise(self._lib.vsplat_get_densify_stats(self._h, _p(out)), "densify_stats")
+        return out
+
+    def reset_densify_stats(self):
+        _raise(self._lib.vsplat_reset_densify_stats(self._h), "reset_densify_stats")
+
+    # --- settings
+    def set_active_sh_degree(self, degree: int):
+        _raise(self._lib.vsplat_set_active_sh_degree(self._h, degree), "set_active_sh_degree")
+
+    def set_filter3d(self, values: np.ndarray):
+        f = np.ascontiguousarray(values, np.float32)
+        _raise(self._lib.vsplat_set_filter3d(self._h, _p(f), f.size), "set_filter3d")
+
+    def set_sh_dropout(self, probability: float, retained_degree: int, seed: int):
+        _raise(self._lib.vsplat_set_sh_dropout(self._h, probability, retained_degree, seed), "set_sh_dropout")
+
+    def parameters(self, group: int):
+        ptr, size = C.c_void_p(), C.c_uint64()
+        _raise(self._lib.vsplat_parameters(self._h, group, C.byref(ptr), C.byref(size)), "parameters")
+        return ptr.value, size.value
+
+    # --- accessors
+    def uses_fixed_point(self) -> bool:
+        return bool(self._lib.vsplat_uses_fixed_point(self._h))
+
+    def active_sh_degree(self) -> int:
+        return int(self._lib.vsplat_active_sh_degree(self._h))
+
+    def gaussian_count(self) -> int:
+        return int(self._lib.vsplat_gaussian_count(self._h))
+
+    def gpu_bytes(self) -> int:
+        return int(self._lib.vsplat_gpu_bytes(self._h))
+
+    def bytes_per_gaussian(self) -> int:
+        return int(self._lib.vsplat_bytes_per_gaussian(self._h))
+
+    def staging_bytes(self) -> int:
+        return int(self._lib.vsplat_staging_bytes(self._h))
+
+    def timings(self) -> list:
+        items = (CTiming * 16)()
+        count = C.c_uint32()
+        _raise(self._lib.vsplat_timings(self._h, items, 16, C.byref(count)), "timings")
+        return [(items[i].name.decode(), items[i].milliseconds) for i in range(min(count.value, 16))]
+
+    # --- device-resident path
+    def enqueue(self, camera: Camera, device_target_ptr: int, mode: int, rates: LearningRates | None = None):
+        cam = camera_to_c(camera)
+        r = None if rates is None else rates.to_c()
+        _raise(self._lib.vsplat_enqueue(self._h, C.byref(cam), C.c_void_p(device_target_ptr), mode,
+                                        None if r is None else C.byref(r)), "enqueue")
+
+    def synchronize(self) -> StepStats:
+        st = CStats()
+        _raise(self._lib.vsplat_synchronize(self._h, C.byref(st)), "synchronize")
+        return StepStats.of(st)
+
+    def stream(self) -> int:
+        return int(self._lib.vsplat_stream(self._h) or 0)
+
+    def reserve_entries(self, entries: int):
+        _raise(self._lib.vsplat_reserve_entries(self._h, entries), "reserve_entries")
+
+    def inspect(self, name: str) -> np.ndarray:
+        size = C.c_uint64()
+        _raise(self._lib.vsplat_inspect(self._h, name.encode(), None, 0, C.byref(size)), "inspect")
+        if name == "raw":
+            dtype, width = (np.int64, 14) if self.uses_fixed_point() else (np.float32, 14)
+        else:
+            dtype, width = INSPECT_TYPES[name]
+        out = np.zeros(size.value // np.dtype(dtype).itemsize, dtype)
+        _raise(self._lib.vsplat_inspect(self._h, name.encode(), _p(out), out.nbytes, C.byref(size)), "inspect")
+        return out.reshape(-1, width) if width > 1 else out
+
+
+def math_probe(function: int, x: np.ndarray) -> np.ndarray:
+    """The device's exp (0), log (1), sqrt (2), reciprocal (3) or half round trip (4) of x."""
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(a)
+    _raise(lib().vsplat_math_probe(function, _p(a), _p(out), a.size), "math_probe")
+    return out
