@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""fwd+bwd ms/iter of the differentiable tile rasterizer on one MI355X.
+
+Workload (the 1M-Gaussian, 1280x720 configuration the metric is quoted on):
+1,000,000 Gaussians, SH degree 3 (48 coefficients N(0, 0.3)), means U[-1,1]^3,
+log-scales N(-4, 0.5), normalised N(0,1) quaternions, opacity logit N(0,1); one
+1280x720 camera at (0,0,-3) looking at the origin, 60 degree vertical field of
+view; loss 0.8 L1 + 0.2 D-SSIM against a smooth synthetic target; 64-bit
+fixed-point backward. Inputs are generated from seeded explicit RNG maths
+(vsplat.fixtures) and are resident in GPU memory before the timed region.
+
+A step is one pass of the hot path: projection and SH, depth sort, tile
+binning and the tile sort, compositing, L1+SSIM and its adjoint, the backward
+pass and the chain rule to fp32 gradients in GPU memory, queued on the
+trainer's stream with no host synchronisation inside (vsplat_enqueue).
+W untimed steps, then K timed ones bracketed by synchronisation; per-stage
+times come from HIP events recorded on the trainer's stream.
+
+Prints one JSON line. --gpus above 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import dataclasses
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "vibe-splat_amd"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+import vsplat  # noqa: E402
+from vsplat import fixtures as F  # noqa: E402
+
+METRIC = "fwd+bwd ms/iter, 1M Gaussians @1280x720; batched-bf16 vs plain fp32; % HBM peak"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Hip:
+    """The few HIP runtime calls the benchmark needs for a device-resident target."""
+
+    def __init__(self):
+        vsplat.lib()  # loads libamdhip64 as a dependency
+        self.lib = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [C.c_void_p, C.c_size_t]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+
+    def upload(self, array: np.ndarray) -> int:
+        a = np.ascontiguousarray(array)
+        ptr = C.c_void_p()
+        if self.lib.hipMalloc(C.byref(ptr), a.nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        if self.lib.hipMemcpy(ptr, C.c_void_p(a.ctypes.data), a.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        return ptr.value
+
+    def free(self, ptr: int):
+        self.lib.hipFree(C.c_void_p(ptr))
+
+
+def touched_entries(ranges: np.ndarray, stops: np.ndarray, width: int, height: int) -> int:
+    """Ep = sum over tiles of (range end - the lowest pixel stop): the entries the
+    forward and backward passes must visit."""
+    gw, gh = (width + 15) // 16, (height + 15) // 16
+    s = stops.reshape(height, width)
+    pad = np.full((gh * 16, gw * 16), np.iinfo(np.uint32).max, np.uint64)
+    pad[:height, :width] = s
+    low = pad.reshape(gh, 16, gw, 16).min(axis=(1, 3)).reshape(-1)
+    end = ranges[:, 1].astype(np.int64)
+    low = np.minimum(low.astype(np.int64), end)
+    return int((end - low).sum())
+
+
+def algorithmic_bytes(N, V, E, Ep, P, T, Bp):
+    """Compulsory HBM traffic of one iteration: each stage reads its logical
+    inputs once and writes its outputs once (sort passes, histograms, LDS
+    staging and halo re-reads not counted)."""
+    stages = {
+        "project": N * Bp + 56 * V,
+        "depth sort": 16 * V,
+        "binning": 56 * V + 8 * E,
+        "tile sort": 16 * E,
+        "ranges": 4 * E + 8 * T,
+        "forward": 52 * Ep + 8 * T + 20 * P,
+        "loss": 132 * P,
+        "backward": 52 * Ep + 8 * T + 24 * P + 56 * V,
+        "chain rule": 56 * V + N * Bp + 236 * N,
+    }
+    return sum(stages.values()), stages
+
+
+def cpu_baseline(workload, threads: int) -> dict:
+    """The reference's CPU path (its src/train/reference.cpp, as the FP64 oracle
+    restates it): forward, loss and backward with the chain rule, on `threads`
+    host threads. Timed on a band of 90 of the 720 rows (the same per-pixel
+    work as the full view) and on a 2-row band (mostly the per-Gaussian costs:
+    projection and sort, and the chain rule); the full view is the 2-row time
+    plus the per-row difference times the remaining 718 rows."""
+    from oracle import oracle as O
+
+    cam, target = workload.cameras[0], workload.targets[0]
+    H = cam.height
+
+    def run(rows):
+        top = (H - rows) // 2
+        c = dataclasses.replace(cam, height=rows, cy=float(np.float32(cam.cy) - np.float32(top)))
+        t0 = time.perf_counter()
+        img = O.render(workload.cloud, c, threads=threads)
+        ls = O.loss(img, np.ascontiguousarray(target[top:top + rows]), workload.lambda_ssim, threads=threads)
+        O.gradients(workload.cloud, c, ls.adjoint, threads=threads)
+        return time.perf_counter() - t0
+
+    band = 90
+    t_fixed = run(2)
+    t_band = run(band)
+    per_row = (t_band - t_fixed) / (band - 2)
+    full = t_fixed + per_row * (H - 2)
+    return {"value": round(full * 1000.0, 1), "unit": "ms/iter", "cores": threads, "kind": "port",
+            "sample": f"FP64 restatement of reference.cpp, fwd+loss+bwd+chain rule on {threads} threads: "
+                      f"a {band}-row band ({t_band:.2f} s) and a 2-row band ({t_fixed:.2f} s) of the 1280x720 view, "
+                      f"extrapolated linearly in rows to 720"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--count", type=int, default=None, help="Gaussians (default: the workload's 1,000,000)")
+    ap.add_argument("--mode", choices=["gradients", "step"], default="gradients")
+    ap.add_argument("--float-atomics", action="store_true", help="float-atomic backward instead of fixed point")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU view split is a design note, not code)", file=sys.stderr)
+        sys.exit(2)
+
+    log("generating the workload")
+    w = F.config(1, count=args.count)
+    cam, target = w.cameras[0], w.targets[0]
+    N, P = w.cloud.count, cam.width * cam.height
+    T = ((cam.width + 15) // 16) * ((cam.height + 15) // 16)
+    fixed = not args.float_atomics
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed))
+    hip = Hip()
+    d_target = hip.upload(target)
+    rates = vsplat.LearningRates()
+    mode = vsplat.MODE_GRADIENTS if args.mode == "gradients" else vsplat.MODE_STEP
+
+    log("sizing: one synchronous step")
+    first = trainer.compute_gradients(cam, target)
+    log(f"  loss {first.loss:.6f} visible {first.visible_splats} entries {first.tile_entries}")
+    for _ in range(args.warmup):
+        trainer.enqueue(cam, d_target, mode, rates if mode == vsplat.MODE_STEP else None)
+    trainer.synchronize()
+    trainer.reset_timings()
+
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        trainer.enqueue(cam, d_target, mode, rates if mode == vsplat.MODE_STEP else None)
+    stats = trainer.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1000.0 / args.steps
+    stages = dict(trainer.timings())
+    log("  " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
+
+    V, E = stats.visible_splats, stats.tile_entries
+    Ep = touched_entries(trainer.inspect("ranges"), trainer.inspect("stops"), cam.width, cam.height)
+    coefficients = (w.cloud.sh_degree + 1) ** 2
+    Bp = (3 + 3 + 4 + 1) * 4 + F.half_sh_words(coefficients) * 4  # as stored: SH DC fp32, rest fp16
+    total, per_stage = algorithmic_bytes(N, V, E, Ep, P, T, Bp)
+    dominant = max(("composite", "forward"), ("backward", "backward"), key=lambda s: stages.get(s[0], 0.0))
+    kernel_ms = stages[dominant[0]]
+    achieved = per_stage[dominant[1]] / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    traffic_file = ROOT / "profiles" / f"{dominant[1]}_kernel_traffic_c1.json"
+    if traffic_file.exists():
+        traffic = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": METRIC,
+        "value": round(ms, 4),
+        "unit": "ms/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "1M Gaussians SH3, one 1280x720 view, 0.8 L1 + 0.2 D-SSIM, "
+                               + ("fixed-point" if fixed else "float-atomic") + " backward, " + args.mode,
+                   "gaussians": N, "width": cam.width, "height": cam.height, "sh_degree": w.cloud.sh_degree,
+                   "form": "plain fp32, single view"},
+        "roofline": {"bound": "hbm", "kernel": dominant[1], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "algorithmic_bytes": int(per_stage[dominant[1]]), "kernel_ms": round(kernel_ms, 4)},
+        "iteration": {"algorithmic_bytes": int(total), "hbm_frac": round(total / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                      "V": V, "E": E, "Ep": Ep, "P": P, "T": T, "Bp": Bp, "loss": round(float(stats.loss), 6)},
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"CPU baseline on {threads} threads")
+        result["cpu_baseline"] = cpu_baseline(w, threads)
+    hip.free(d_target)
+    trainer.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

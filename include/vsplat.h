@@ -155,6 +155,9 @@ uint64_t vsplat_staging_bytes(const vsplat_trainer *trainer);
 /* Trainer::timings (trainer.hpp:186): up to capacity stages; *count gets how many exist. */
 int vsplat_timings(const vsplat_trainer *trainer, vsplat_timing *out, uint32_t capacity, uint32_t *count);
 
+/* Forgets the stage times gathered so far (waits for queued work first). */
+int vsplat_reset_timings(vsplat_trainer *trainer);
+
 /* The message of the calling thread's last failed call. */
 const char *vsplat_last_error(void);
 

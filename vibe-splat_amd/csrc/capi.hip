@@ -206,3 +206,8 @@ extern "C" int vsplat_math_probe(int32_t function, const float *in, float *out, 
   if ((in == nullptr || out == nullptr) && n > 0) return invalid("null argument");
   return guarded([&] { vsplat::math_probe(function, in, out, n); });
 }
+
+extern "C" int vsplat_reset_timings(vsplat_trainer *t) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.reset_timings(); });
+}

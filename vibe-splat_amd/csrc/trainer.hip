@@ -99,7 +99,9 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   // Float atomics are the default on this GPU, as on any with them (trainer.cpp:181).
   fixedPoint_ = options.force_fixed_point != 0;
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-  for (auto &e : events_) check(hipEventCreate(&e), "hipEventCreate");
+  slots_.resize(kTimingSlots);
+  for (auto &slot : slots_)
+    for (auto &e : slot.events) check(hipEventCreate(&e), "hipEventCreate");
 
   const size_t n = count_;
   positions_.ensure(n * 12);
@@ -162,8 +164,9 @@ Trainer::~Trainer() {
   prefetchChanged_.notify_all();
   if (prefetchWorker_.joinable()) prefetchWorker_.join();
   if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
-  for (auto &e : events_)
-    if (e != nullptr) (void)hipEventDestroy(e);
+  for (auto &slot : slots_)
+    for (auto &e : slot.events)
+      if (e != nullptr) (void)hipEventDestroy(e);
   if (stream_ != nullptr) (void)hipStreamDestroy(stream_);
 }
 
@@ -236,24 +239,60 @@ void Trainer::grow_entries(uint64_t entries) {
 
 void Trainer::reserve_entries(uint64_t entries) { grow_entries(entries); }
 
-void Trainer::mark(const char *stage) {
-  if (marks_.size() + 1 > static_cast<size_t>(kMaxMarks)) return;
-  marks_.push_back(stage);
-  check(hipEventRecord(events_[marks_.size()], stream_), "hipEventRecord");
+void Trainer::begin_timing(bool iteration) {
+  if (slotPending_ == slots_.size()) {
+    check(hipStreamSynchronize(stream_), "timing");
+    drain_timing();
+  }
+  current_ = &slots_[slotHead_ % slots_.size()];
+  current_->marks.clear();
+  current_->iteration = iteration;
+  check(hipEventRecord(current_->events[0], stream_), "hipEventRecord");
+  ++slotHead_;
+  ++slotPending_;
 }
 
-void Trainer::finish_timing() {
-  for (size_t m = 0; m < marks_.size(); ++m) {
-    float ms = 0.0f;
-    check(hipEventElapsedTime(&ms, events_[m], events_[m + 1]), "hipEventElapsedTime");
-    auto it = std::find_if(stageTotals_.begin(), stageTotals_.end(), [&](const auto &s) { return s.first == marks_[m]; });
-    if (it == stageTotals_.end()) {
-      stageTotals_.emplace_back(marks_[m], static_cast<double>(ms));
-    } else {
-      it->second += static_cast<double>(ms);
-    }
+void Trainer::mark(const char *stage) {
+  if (current_ == nullptr || current_->marks.size() + 1 > static_cast<size_t>(kMaxMarks)) return;
+  current_->marks.push_back(stage);
+  check(hipEventRecord(current_->events[current_->marks.size()], stream_), "hipEventRecord");
+}
+
+// Forgets the last step's events (a step that ran again).
+void Trainer::cancel_timing() {
+  if (current_ != nullptr && slotPending_ > 0) {
+    --slotHead_;
+    --slotPending_;
   }
-  marks_.clear();
+  current_ = nullptr;
+}
+
+// After the stream has passed every pending slot: add their stage times up.
+void Trainer::drain_timing() {
+  for (size_t k = slotHead_ - slotPending_; k < slotHead_; ++k) {
+    TimingSlot &slot = slots_[k % slots_.size()];
+    for (size_t m = 0; m < slot.marks.size(); ++m) {
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, slot.events[m], slot.events[m + 1]), "hipEventElapsedTime");
+      auto it = std::find_if(stageTotals_.begin(), stageTotals_.end(), [&](const auto &s) { return s.first == slot.marks[m]; });
+      if (it == stageTotals_.end()) {
+        stageTotals_.emplace_back(slot.marks[m], static_cast<double>(ms));
+      } else {
+        it->second += static_cast<double>(ms);
+      }
+    }
+    if (slot.iteration) ++timedIterations_;
+  }
+  slotPending_ = 0;
+  current_ = nullptr;
+}
+
+void Trainer::reset_timings() {
+  check(hipStreamSynchronize(stream_), "reset timings");
+  slotPending_ = 0;
+  current_ = nullptr;
+  stageTotals_.clear();
+  timedIterations_ = 0;
 }
 
 std::vector<std::pair<std::string, double>> Trainer::timings() const {
@@ -265,22 +304,22 @@ std::vector<std::pair<std::string, double>> Trainer::timings() const {
 void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates,
                              bool timed) {
   hipStream_t s = stream_;
-  marks_.clear();
-  if (timed) check(hipEventRecord(events_[0], s), "hipEventRecord");
+  current_ = nullptr;
+  if (timed) begin_timing(true);
   uint32_t *counters = counters_.as<uint32_t>();
   launch_fill_u32(s, counters, 3, 0u); // not the sticky overflow word
   launch_project(s, positions_.as<float>(), scales_.as<float>(), rotations_.as<float>(), opacities_.as<float>(),
                  shWordsBuf_.as<uint32_t>(), filterSet_ ? filter3d_.as<float>() : nullptr, v, options_.mip_filter_2d != 0, dropoutNow_,
                  shRetained_, shSeed_, splats_.as<Splat>(), keys_.as<uint32_t>(), values_.as<uint32_t>());
-  if (timed) mark("project");
+  mark("project");
   launch_radix_sort(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tmpKeys_.as<uint32_t>(), tmpValues_.as<uint32_t>(), nullptr, count_,
                     32, sortScratch_.as<uint32_t>());
-  if (timed) mark("depth sort");
+  mark("depth sort");
   launch_bin_count(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), count_, gridW_, gridH_, coverage_.as<uint32_t>(),
                    counters);
   launch_exclusive_scan(s, coverage_.as<uint32_t>(), count_, scanScratch_.as<uint32_t>(), counters + kEntries);
   launch_bin_check(s, counters, static_cast<uint32_t>(entryCapacity_));
-  if (timed) mark("count tiles");
+  mark("count tiles");
   launch_fill_u32(s, ranges_.as<uint32_t>(), static_cast<uint64_t>(tiles_) * 2, 0u);
   if (entryCapacity_ > 0) {
     launch_bin_emit(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), count_, gridW_, gridH_, coverage_.as<uint32_t>(),
@@ -289,10 +328,10 @@ void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, u
                       counters + kEntriesUsed, static_cast<uint32_t>(entryCapacity_), tileBits_, sortScratch_.as<uint32_t>());
     launch_tile_ranges(s, entryKeys_.as<uint32_t>(), counters, static_cast<uint32_t>(entryCapacity_), ranges_.as<uint2>());
   }
-  if (timed) mark("tile lists");
+  mark("tile lists");
   launch_forward(s, splats_.as<Splat>(), entryValues_.as<uint32_t>(), ranges_.as<uint2>(), v.width, v.height, options_.background,
                  image_.as<float4>(), stops_.as<uint32_t>());
-  if (timed) mark("composite");
+  mark("composite");
   if (mode == kModeRender) return;
   launch_loss(s, image_.as<float4>(), deviceTarget, v.width, v.height, options_.lambda_ssim, fields_.as<float>(), adjoint_.as<float4>(),
               tileTotals_.as<float2>(), sums_.as<double>());
@@ -301,16 +340,16 @@ void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, u
   } else {
     launch_fill_u32(s, rawFloat_.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat, 0u);
   }
-  if (timed) mark("loss");
+  mark("loss");
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
   // a resolution of 1/64 (trainer.cpp:586-591).
   const float fixedScale = 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f;
   launch_backward(s, fixedPoint_, splats_.as<Splat>(), entryValues_.as<uint32_t>(), ranges_.as<uint2>(), v.width, v.height,
                   options_.background, image_.as<float4>(), adjoint_.as<float4>(), stops_.as<uint32_t>(), fixedScale, rawFixed_.as<int64_t>(),
                   rawFloat_.as<float>());
-  if (timed) mark("backward");
+  mark("backward");
   enqueue_chain(v, mode, rates);
-  if (timed) mark(mode == kModeStep ? "chain rule and step" : "chain rule");
+  mark(mode == kModeStep ? "chain rule and step" : "chain rule");
 }
 
 void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_learning_rates *rates) {
@@ -454,13 +493,12 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
     bool overflow = false;
     stats = read_stats(overflow);
     if (!overflow) break;
-    marks_.clear();
+    cancel_timing();
     launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
     if (attempt > 0 && stats.tile_entries <= entryCapacity_) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
     grow_entries(stats.tile_entries);
   }
-  finish_timing();
-  ++timedIterations_;
+  drain_timing();
   if (mode == kModeGradients) haveGradients_ = true;
   if (mode == kModeStep) ++adamStep_;
   lastView_ = v;
@@ -489,12 +527,11 @@ void Trainer::apply_gradients(const vsplat_learning_rates &rates) {
   if (rates.trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
   dropoutNow_ = 0.0f;
   statisticsNext_ = false;
-  marks_.clear();
-  check(hipEventRecord(events_[0], stream_), "hipEventRecord");
+  begin_timing(false);
   enqueue_chain(lastView_, kModeStep, &rates);
   mark("chain rule and step");
   check(hipStreamSynchronize(stream_), "apply gradients");
-  finish_timing();
+  drain_timing();
   ++adamStep_;
 }
 
@@ -506,7 +543,7 @@ void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, 
   dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
   statisticsNext_ = true;
   const ViewParams v = view_params(camera);
-  enqueue_passes(v, deviceTarget, mode, rates, false);
+  enqueue_passes(v, deviceTarget, mode, rates, timeQueued_);
   if (mode == kModeGradients) haveGradients_ = true;
   if (mode == kModeStep) ++adamStep_;
   lastView_ = v;
@@ -515,6 +552,7 @@ void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, 
 vsplat_step_stats Trainer::synchronize() {
   bool overflow = false;
   const vsplat_step_stats stats = read_stats(overflow);
+  drain_timing();
   if (overflow) {
     launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
     check(hipStreamSynchronize(stream_), "clear overflow");

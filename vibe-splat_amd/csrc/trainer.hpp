@@ -87,6 +87,7 @@ public:
   void set_sh_dropout(float probability, int retained, uint32_t seed);
   void parameters(uint32_t group, void **pointer, uint64_t *bytes) const;
   std::vector<std::pair<std::string, double>> timings() const;
+  void reset_timings();
   void inspect(const std::string &buffer, void *out, uint64_t capacity, uint64_t *bytes) const;
 
   uint32_t count() const { return count_; }
@@ -106,8 +107,10 @@ private:
   void enqueue_passes(const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates,
                       bool timed);
   void enqueue_chain(const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates);
+  void begin_timing(bool iteration);
   void mark(const char *stage);
-  void finish_timing();
+  void cancel_timing();
+  void drain_timing();
   void wait_prefetch();
   vsplat_step_stats read_stats(bool &overflow);
 
@@ -149,12 +152,21 @@ private:
   const uint8_t *prefetched_ = nullptr;
   size_t prefetchedBytes_ = 0;
 
-  // Stage timing with events (trainer.cpp:201-240).
+  // Stage timing with events (trainer.cpp:201-240): one set of events per
+  // queued step, in a ring, read once the stream has passed them.
   static constexpr int kMaxMarks = 12;
-  hipEvent_t events_[kMaxMarks + 1] = {};
-  std::vector<const char *> marks_;
+  static constexpr size_t kTimingSlots = 64;
+  struct TimingSlot {
+    hipEvent_t events[kMaxMarks + 1] = {};
+    std::vector<const char *> marks;
+    bool iteration = true; // counts as a training iteration in the averages
+  };
+  std::vector<TimingSlot> slots_;
+  size_t slotHead_ = 0, slotPending_ = 0;
+  TimingSlot *current_ = nullptr;
   std::vector<std::pair<std::string, double>> stageTotals_;
   uint64_t timedIterations_ = 0;
+  bool timeQueued_ = true; // the enqueue path records stage events too
 };
 
 } // namespace vsplat

@@ -33,7 +33,7 @@ EXPORTS = [
     "vsplat_parameters", "vsplat_gaussian_count", "vsplat_sh_degree", "vsplat_active_sh_degree",
     "vsplat_uses_fixed_point", "vsplat_gpu_bytes", "vsplat_bytes_per_gaussian", "vsplat_staging_bytes",
     "vsplat_timings", "vsplat_last_error", "vsplat_enqueue", "vsplat_synchronize", "vsplat_stream",
-    "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe",
+    "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
 ]
 
 
@@ -132,6 +132,7 @@ def lib():
         L.vsplat_reserve_entries.argtypes = [vp, C.c_uint64]
         L.vsplat_inspect.argtypes = [vp, C.c_char_p, vp, C.c_uint64, vp]
         L.vsplat_math_probe.argtypes = [C.c_int32, vp, vp, C.c_uint64]
+        L.vsplat_reset_timings.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -379,6 +380,9 @@ class Trainer:
         count = C.c_uint32()
         _raise(self._lib.vsplat_timings(self._h, items, 16, C.byref(count)), "timings")
         return [(items[i].name.decode(), items[i].milliseconds) for i in range(min(count.value, 16))]
+
+    def reset_timings(self):
+        _raise(self._lib.vsplat_reset_timings(self._h), "reset_timings")
 
     # --- device-resident path
     def enqueue(self, camera: Camera, device_target_ptr: int, mode: int, rates: LearningRates | None = None):
