@@ -192,7 +192,8 @@ int vsplat_inspect(vsplat_trainer *trainer, const char *buffer, void *out, uint6
 
 /* Runs one of the device's elementwise functions over n floats (device
  * arithmetic check): 0 exp, 1 log, 2 sqrt, 3 reciprocal (1/x), 4 float to half
- * and back. in and out are host arrays. */
+ * and back, 5 the half in the low 16 bits of the input's bits to float. in and
+ * out are host arrays. */
 int vsplat_math_probe(int32_t function, const float *in, float *out, uint64_t n);
 
 #ifdef __cplusplus

@@ -198,7 +198,13 @@ void quat_to_mat(float w, float x, float y, float z, float R[3][3]) {
 float dot3(float a0, float a1, float a2, float b0, float b1, float b2) { return fma(a2, b2, fma(a1, b1, a0 * b0)); }
 
 // The opacity factor of the 3D filter, sqrt(prod s2 / (s2 + f)) (train_preprocess.comp:62-63).
+// With no filter each ratio s2 / (s2 + 0) is exactly 1 for a finite positive s2
+// and NaN for 0 or infinity, which is what the shortcut returns.
 float filter_factor(float s2x, float s2y, float s2z, float f) {
+  if (f == 0.0f) {
+    const bool finite = s2x > 0.0f && s2x < INFINITY && s2y > 0.0f && s2y < INFINITY && s2z > 0.0f && s2z < INFINITY;
+    return finite ? 1.0f : NAN;
+  }
   return std::sqrt((s2x / (s2x + f)) * (s2y / (s2y + f)) * (s2z / (s2z + f)));
 }
 
@@ -240,27 +246,28 @@ void splat_tiles(const Splat &s, std::uint32_t gridW, std::uint32_t gridH, std::
   t[3] = static_cast<std::uint32_t>(clampf(std::floor((s.cy + s.ey + 1.0f) * 0.0625f), 0.0f, lastY));
 }
 
-float edge_min(const Splat &s, float x, float ylo, float yhi, bool vertical) {
-  // q(x, y) = qx x^2 + 2 qy x y + qz y^2 along a line, least at the clamped stationary point.
+// q(x, y) = qx x^2 + 2 qy x y + qz y^2 along a line, least at the clamped
+// stationary point; iqx, iqz are 1/qx and 1/qz.
+float edge_min(const Splat &s, float iqx, float iqz, float x, float ylo, float yhi, bool vertical) {
   if (vertical) {
-    const float y = clampf((-s.qy * x) / s.qz, ylo, yhi);
+    const float y = clampf((-s.qy * x) * iqz, ylo, yhi);
     return fma(s.qz * y, y, fma(2.0f * s.qy * x, y, s.qx * x * x));
   }
   const float v = x; // the line y = v, parameter u along x
-  const float u = clampf((-s.qy * v) / s.qx, ylo, yhi);
+  const float u = clampf((-s.qy * v) * iqx, ylo, yhi);
   return fma(s.qz * v, v, fma(2.0f * s.qy * u, v, s.qx * u * u));
 }
 
 // train_types.glsl:56-77
-bool reaches_tile(const Splat &s, float reach, std::uint32_t tx, std::uint32_t ty) {
+bool reaches_tile(const Splat &s, float reach, float iqx, float iqz, std::uint32_t tx, std::uint32_t ty) {
   const float lowx = (static_cast<float>(tx) * 16.0f + 0.5f) - s.cx, lowy = (static_cast<float>(ty) * 16.0f + 0.5f) - s.cy;
   const float highx = lowx + 15.0f, highy = lowy + 15.0f;
   if (lowx <= 0.0f && lowy <= 0.0f && highx >= 0.0f && highy >= 0.0f) return true;
   float least = 1e30f;
-  least = minf(least, edge_min(s, lowx, lowy, highy, true));
-  least = minf(least, edge_min(s, lowy, lowx, highx, false));
-  least = minf(least, edge_min(s, highx, lowy, highy, true));
-  least = minf(least, edge_min(s, highy, lowx, highx, false));
+  least = minf(least, edge_min(s, iqx, iqz, lowx, lowy, highy, true));
+  least = minf(least, edge_min(s, iqx, iqz, lowy, lowx, highx, false));
+  least = minf(least, edge_min(s, iqx, iqz, highx, lowy, highy, true));
+  least = minf(least, edge_min(s, iqx, iqz, highy, lowx, highx, false));
   return least <= fma(reach, 1.001f, 1e-3f);
 }
 
@@ -378,10 +385,11 @@ extern "C" int ob_preprocess(const float *positions, const float *scales, const 
     float qw = rotations[i * 4], qx = rotations[i * 4 + 1], qy = rotations[i * 4 + 2], qz = rotations[i * 4 + 3];
     const float qlen = std::sqrt(fma(qz, qz, fma(qy, qy, fma(qx, qx, qw * qw))));
     if (!(qlen >= 1e-4f)) continue;
-    qw = qw / qlen;
-    qx = qx / qlen;
-    qy = qy / qlen;
-    qz = qz / qlen;
+    const float iq = 1.0f / qlen;
+    qw = qw * iq;
+    qx = qx * iq;
+    qy = qy * iq;
+    qz = qz * iq;
     const float sx = fexp(scales[i * 3]), sy = fexp(scales[i * 3 + 1]), sz = fexp(scales[i * 3 + 2]);
     float R[3][3];
     quat_to_mat(qw, qx, qy, qz, R);
@@ -429,7 +437,8 @@ extern "C" int ob_preprocess(const float *positions, const float *scales, const 
     const float dx = px - c.position[0], dy = py - c.position[1], dz = pz - c.position[2];
     const float dl = std::sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
     float basis[16];
-    sh_basis(dx / dl, dy / dl, dz / dl, c.sh_degree, basis);
+    const float idl = 1.0f / dl;
+    sh_basis(dx * idl, dy * idl, dz * idl, c.sh_degree, basis);
     const std::uint32_t used = sh_coefficients_used(i, c.sh_coefficients, opt->sh_dropout, opt->sh_retained, opt->sh_seed);
     const std::uint32_t base = i * c.sh_words;
     float col[3] = {0.5f, 0.5f, 0.5f};
@@ -439,9 +448,10 @@ extern "C" int ob_preprocess(const float *positions, const float *scales, const 
     const float reach = reach_of(opacityUsed);
     const float ex = std::sqrt(reach * a), ey = std::sqrt(reach * cc);
     Splat out;
-    out.qx = cc / det;
-    out.qy = -b / det;
-    out.qz = a / det;
+    const float id = 1.0f / det;
+    out.qx = cc * id;
+    out.qy = -b * id;
+    out.qz = a * id;
     out.o = opacityUsed;
     out.r = maxf(col[0], 0.0f);
     out.g = maxf(col[1], 0.0f);
@@ -481,12 +491,12 @@ extern "C" int ob_bin(const uint32_t *keys, const uint32_t *values, const float 
     std::uint32_t n = 0;
     if (keys[r] != kInvisible) {
       const Splat s = load_splat(splats, values[r]);
-      const float reach = reach_of(s.o);
+      const float reach = reach_of(s.o), iqx = 1.0f / s.qx, iqz = 1.0f / s.qz;
       std::uint32_t t[4];
       splat_tiles(s, gridW, gridH, t);
       for (std::uint32_t y = t[1]; y <= t[3]; ++y) {
         for (std::uint32_t x = t[0]; x <= t[2]; ++x) {
-          if (!reaches_tile(s, reach, x, y)) continue;
+          if (!reaches_tile(s, reach, iqx, iqz, x, y)) continue;
           if (entryKeys != nullptr) {
             if (total + n >= capacity) return -2;
             entryKeys[total + n] = y * gridW + x;
@@ -825,9 +835,11 @@ extern "C" int ob_backward_float(const float *splats, const uint32_t *entryValue
   return 0;
 }
 
-// train_fixed_to_float.comp:9-14, through double so that the conversion is exact then rounded once.
+// train_fixed_to_float.comp:9-14, through double: the int64 converts exactly, is scaled by the
+// reciprocal of the scale in double, and is rounded to float once.
 extern "C" int ob_fixed_to_float(const int64_t *raw, uint64_t n, float fixedScale, float *out) {
-  for (std::uint64_t i = 0; i < n; ++i) out[i] = static_cast<float>(static_cast<double>(raw[i]) / static_cast<double>(fixedScale));
+  const double inverse = 1.0 / static_cast<double>(fixedScale);
+  for (std::uint64_t i = 0; i < n; ++i) out[i] = static_cast<float>(static_cast<double>(raw[i]) * inverse);
   return 0;
 }
 
@@ -895,10 +907,11 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
     float qw = rotations[i * 4], qx = rotations[i * 4 + 1], qy = rotations[i * 4 + 2], qz = rotations[i * 4 + 3];
     const float qlen = std::sqrt(fma(qz, qz, fma(qy, qy, fma(qx, qx, qw * qw))));
     if (qlen > 0.0f) {
-      qw = qw / qlen;
-      qx = qx / qlen;
-      qy = qy / qlen;
-      qz = qz / qlen;
+      const float iq = 1.0f / qlen;
+      qw = qw * iq;
+      qx = qx * iq;
+      qy = qy * iq;
+      qz = qz * iq;
     } else {
       qw = 1.0f;
       qx = qy = qz = 0.0f;
@@ -988,7 +1001,7 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
       float v[3];
       for (int r = 0; r < 3; ++r) v[r] = dot3(G3[r][0], G3[r][1], G3[r][2], R[0][a], R[1][a], R[2][a]);
       const float quad = dot3(R[0][a], R[1][a], R[2][a], v[0], v[1], v[2]);
-      const float filterTerm = ((((g[0] * k2) * opacity) * k3) * f) / (s2[a] + f);
+      const float filterTerm = f != 0.0f ? ((((g[0] * k2) * opacity) * k3) * f) / (s2[a] + f) : 0.0f;
       dScale[a] = fma(o.scale_reg, fexp(logScale[a]), fma(2.0f * s2[a], quad, filterTerm));
     }
     // Rotation: P = G3 R D (half of the reference's P), dL/dq = 2 <P, dR/dq>.
@@ -1005,7 +1018,8 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
     const float qv[4] = {w, x, y, z};
     const float denom = qlen > 1e-12f ? qlen : 1e-12f;
     float tangent[4];
-    for (int k = 0; k < 4; ++k) tangent[k] = fma(-qv[k], qd, gq[k]) / denom;
+    const float idenom = 1.0f / denom;
+    for (int k = 0; k < 4; ++k) tangent[k] = fma(-qv[k], qd, gq[k]) * idenom;
 
     const float dOpacity = ((fma(g[0] * k2, k3, o.opacity_reg) * opacity) * (1.0f - opacity));
 
@@ -1013,7 +1027,8 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
     const float gColour[3] = {g[6], g[7], g[8]};
     const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
     const float dist = std::sqrt(fma(ddz, ddz, fma(ddy, ddy, ddx * ddx)));
-    const float dir[3] = {ddx / dist, ddy / dist, ddz / dist};
+    const float idist = 1.0f / dist;
+    const float dir[3] = {ddx * idist, ddy * idist, ddz * idist};
     float basis[16];
     sh_basis(dir[0], dir[1], dir[2], c.sh_degree, basis);
     const std::uint32_t used = sh_coefficients_used(i, c.sh_coefficients, o.sh_dropout, o.sh_retained, o.sh_seed);
@@ -1032,7 +1047,7 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
         for (int a = 0; a < 3; ++a) gDir[a] = fma(d, bg[k][a], gDir[a]);
       }
       const float dd = dot3(dir[0], dir[1], dir[2], gDir[0], gDir[1], gDir[2]);
-      for (int a = 0; a < 3; ++a) gP[a] = gP[a] + fma(-dir[a], dd, gDir[a]) / dist;
+      for (int a = 0; a < 3; ++a) gP[a] = gP[a] + fma(-dir[a], dd, gDir[a]) * idist;
     }
 
     for (std::uint32_t a = 0; a < 3; ++a) {

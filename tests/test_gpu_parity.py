@@ -60,7 +60,12 @@ def test_device_math_is_bit_exact():
     with np.errstate(over="ignore", divide="ignore"):
         assert (_bits(vsplat.math_probe(3, nz)) == _bits(np.float32(1.0) / nz)).all()
     fin = x[np.isfinite(x)]
-    assert (_bits(vsplat.math_probe(4, fin)) == _bits(fin.astype(np.float16).astype(np.float32))).all()
+    with np.errstate(over="ignore"):
+        assert (_bits(vsplat.math_probe(4, fin)) == _bits(fin.astype(np.float16).astype(np.float32))).all()
+    halves = np.arange(65536, dtype=np.uint32)
+    got = vsplat.math_probe(5, halves.view(np.float32))
+    want = halves.astype(np.uint16).view(np.float16).astype(np.float32)
+    assert ((_bits(got) == _bits(want)) | (np.isnan(got) & np.isnan(want))).all()
 
 
 def _run_both(name, fixed=True, lam=0.2):

@@ -20,26 +20,27 @@ __device__ __forceinline__ void splat_tiles(const Splat &s, uint32_t gridW, uint
 
 // The least of q = qx x^2 + 2 qy x y + qz y^2 along one edge of the tile's
 // square of pixel centres, at the clamped stationary point.
-__device__ __forceinline__ float edge_min_vertical(const Splat &s, float x, float ylo, float yhi) {
-  const float y = clampf((-s.qy * x) / s.qz, ylo, yhi);
+__device__ __forceinline__ float edge_min_vertical(const Splat &s, float iqz, float x, float ylo, float yhi) {
+  const float y = clampf((-s.qy * x) * iqz, ylo, yhi);
   return fma32(s.qz * y, y, fma32(2.0f * s.qy * x, y, s.qx * x * x));
 }
-__device__ __forceinline__ float edge_min_horizontal(const Splat &s, float v, float xlo, float xhi) {
-  const float u = clampf((-s.qy * v) / s.qx, xlo, xhi);
+__device__ __forceinline__ float edge_min_horizontal(const Splat &s, float iqx, float v, float xlo, float xhi) {
+  const float u = clampf((-s.qy * v) * iqx, xlo, xhi);
   return fma32(s.qz * v, v, fma32(2.0f * s.qy * u, v, s.qx * u * u));
 }
 
 // Whether the splat's 1/255 ellipse reaches a pixel centre of tile (tx, ty)
 // (train_types.glsl:56-77), with the same slack so rounding never drops a pixel.
-__device__ __forceinline__ bool reaches_tile(const Splat &s, float reach, uint32_t tx, uint32_t ty) {
+// iqx and iqz are 1/qx and 1/qz, computed once per splat.
+__device__ __forceinline__ bool reaches_tile(const Splat &s, float reach, float iqx, float iqz, uint32_t tx, uint32_t ty) {
   const float lowx = (static_cast<float>(tx) * 16.0f + 0.5f) - s.cx, lowy = (static_cast<float>(ty) * 16.0f + 0.5f) - s.cy;
   const float highx = lowx + 15.0f, highy = lowy + 15.0f;
   if (lowx <= 0.0f && lowy <= 0.0f && highx >= 0.0f && highy >= 0.0f) return true;
   float least = 1e30f;
-  least = minf(least, edge_min_vertical(s, lowx, lowy, highy));
-  least = minf(least, edge_min_horizontal(s, lowy, lowx, highx));
-  least = minf(least, edge_min_vertical(s, highx, lowy, highy));
-  least = minf(least, edge_min_horizontal(s, highy, lowx, highx));
+  least = minf(least, edge_min_vertical(s, iqz, lowx, lowy, highy));
+  least = minf(least, edge_min_horizontal(s, iqx, lowy, lowx, highx));
+  least = minf(least, edge_min_vertical(s, iqz, highx, lowy, highy));
+  least = minf(least, edge_min_horizontal(s, iqx, highy, lowx, highx));
   return least <= fma32(reach, 1.001f, 1e-3f);
 }
 
@@ -53,11 +54,11 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const uint32_t *__restri
     if (keys[r] != kInvisible) {
       visible = true;
       const Splat s = load_splat(splats, values[r]);
-      const float reach = reach_of(s.o);
+      const float reach = reach_of(s.o), iqx = 1.0f / s.qx, iqz = 1.0f / s.qz;
       uint32_t t[4];
       splat_tiles(s, gridW, gridH, t);
       for (uint32_t y = t[1]; y <= t[3]; ++y)
-        for (uint32_t x = t[0]; x <= t[2]; ++x) n += reaches_tile(s, reach, x, y) ? 1u : 0u;
+        for (uint32_t x = t[0]; x <= t[2]; ++x) n += reaches_tile(s, reach, iqx, iqz, x, y) ? 1u : 0u;
     }
     coverage[r] = n;
   }
@@ -85,13 +86,13 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
   if (r >= count || counters[kEntriesUsed] != counters[kEntries] || keys[r] == kInvisible) return;
   const uint32_t splat = values[r];
   const Splat s = load_splat(splats, splat);
-  const float reach = reach_of(s.o);
+  const float reach = reach_of(s.o), iqx = 1.0f / s.qx, iqz = 1.0f / s.qz;
   uint32_t t[4];
   splat_tiles(s, gridW, gridH, t);
   uint32_t slot = offsets[r];
   for (uint32_t y = t[1]; y <= t[3]; ++y) {
     for (uint32_t x = t[0]; x <= t[2]; ++x) {
-      if (!reaches_tile(s, reach, x, y)) continue;
+      if (!reaches_tile(s, reach, iqx, iqz, x, y)) continue;
       entryKeys[slot] = y * gridW + x;
       entryValues[slot] = splat;
       ++slot;

@@ -6,7 +6,7 @@
 // (train_sh_step.comp:19-70), so that neighbouring threads touch neighbouring memory.
 //
 // The fixed-point sums are converted here (train_fixed_to_float.comp:9-14):
-// int64 to double (exact), divided by the scale in double, rounded to float once.
+// int64 to double (exact), times the scale's reciprocal in double, rounded to float once.
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -111,6 +111,7 @@ __device__ __forceinline__ void emit(const ChainParams &o, float *values, uint32
   }
 }
 
+template <int Degree>
 __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.count) return;
@@ -118,9 +119,9 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   float g[kRawPerSplat];
   if (o.fixedPoint) {
     const long long *raw = reinterpret_cast<const long long *>(B.rawFixed) + static_cast<size_t>(i) * kRawPerSplat;
-    const double scale = static_cast<double>(o.fixedScale);
+    const double inverse = 1.0 / static_cast<double>(o.fixedScale);
 #pragma unroll
-    for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = static_cast<float>(static_cast<double>(raw[k]) / scale);
+    for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = static_cast<float>(static_cast<double>(raw[k]) * inverse);
   } else {
     const float *raw = B.rawFloat + static_cast<size_t>(i) * kRawPerSplat;
 #pragma unroll
@@ -176,10 +177,11 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   float qw = B.rotations[i * 4], qx = B.rotations[i * 4 + 1], qy = B.rotations[i * 4 + 2], qz = B.rotations[i * 4 + 3];
   const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
   if (qlen > 0.0f) {
-    qw = qw / qlen;
-    qx = qx / qlen;
-    qy = qy / qlen;
-    qz = qz / qlen;
+    const float iq = 1.0f / qlen;
+    qw = qw * iq;
+    qx = qx * iq;
+    qy = qy * iq;
+    qz = qz * iq;
   } else {
     qw = 1.0f;
     qx = qy = qz = 0.0f;
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
 #pragma unroll
     for (int r = 0; r < 3; ++r) v[r] = dot3(G3[r][0], G3[r][1], G3[r][2], R[0][a], R[1][a], R[2][a]);
     const float quad = dot3(R[0][a], R[1][a], R[2][a], v[0], v[1], v[2]);
-    const float filterTerm = ((((g[0] * k2) * opacity) * k3) * f) / (s2[a] + f);
+    const float filterTerm = f != 0.0f ? ((((g[0] * k2) * opacity) * k3) * f) / (s2[a] + f) : 0.0f;
     dScale[a] = fma32(o.scaleReg, vs_exp(logScale[a]), fma32(2.0f * s2[a], quad, filterTerm));
   }
   // Rotation: P = G3 R D, dL/dq = 2 <2P, dR/dq> written out per component.
@@ -305,40 +307,48 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   const float qd = fma32(z, gq[3], fma32(y, gq[2], fma32(x, gq[1], w * gq[0])));
   const float qv[4] = {w, x, y, z};
   const float denom = qlen > 1e-12f ? qlen : 1e-12f;
+  const float idenom = 1.0f / denom;
   float tangent[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) tangent[k] = fma32(-qv[k], qd, gq[k]) / denom;
+  for (int k = 0; k < 4; ++k) tangent[k] = fma32(-qv[k], qd, gq[k]) * idenom;
 
   const float dOpacity = (fma32(g[0] * k2, k3, o.opacityReg) * opacity) * (1.0f - opacity);
 
   // Colour, clamped at zero, and the view direction's part of the position gradient.
   const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
   const float dist = sqrtf(fma32(ddz, ddz, fma32(ddy, ddy, ddx * ddx)));
-  const float dir[3] = {ddx / dist, ddy / dist, ddz / dist};
+  const float idist = 1.0f / dist;
+  const float dir[3] = {ddx * idist, ddy * idist, ddz * idist};
+  constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
   float basis[16];
-  sh_basis(dir[0], dir[1], dir[2], c.shDegree, basis);
+  sh_basis(dir[0], dir[1], dir[2], Degree, basis);
   const uint32_t used = sh_coefficients_used(i, c.shCoefficients, o.shDropout, o.shRetained, o.shSeed);
   const uint32_t base = i * c.shWords;
+  float coefficient[kCoefficients][3];
   float colour[3] = {0.5f, 0.5f, 0.5f};
-  for (uint32_t k = 0; k < used; ++k) {
 #pragma unroll
-    for (uint32_t ch = 0; ch < 3; ++ch) colour[ch] = fma32(basis[k], sh_value(B.shWords, base, k * 3 + ch), colour[ch]);
+  for (uint32_t k = 0; k < kCoefficients; ++k) {
+#pragma unroll
+    for (uint32_t ch = 0; ch < 3; ++ch) {
+      coefficient[k][ch] = sh_value(B.shWords, base, k * 3 + ch);
+      colour[ch] = k < used ? fma32(basis[k], coefficient[k][ch], colour[ch]) : colour[ch];
+    }
   }
   const float gcu[3] = {g[6] * (colour[0] > 0.0f ? 1.0f : 0.0f), g[7] * (colour[1] > 0.0f ? 1.0f : 0.0f),
                         g[8] * (colour[2] > 0.0f ? 1.0f : 0.0f)};
-  if (c.shDegree > 0) {
+  if (Degree > 0) {
     float bg[16][3];
-    sh_basis_gradient(dir[0], dir[1], dir[2], c.shDegree, bg);
+    sh_basis_gradient(dir[0], dir[1], dir[2], Degree, bg);
     float gDir[3] = {0.0f, 0.0f, 0.0f};
-    for (uint32_t k = 1; k < used; ++k) {
-      const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value(B.shWords, base, k * 3), sh_value(B.shWords, base, k * 3 + 1),
-                           sh_value(B.shWords, base, k * 3 + 2));
 #pragma unroll
-      for (int a = 0; a < 3; ++a) gDir[a] = fma32(d, bg[k][a], gDir[a]);
+    for (uint32_t k = 1; k < kCoefficients; ++k) {
+      const float d = dot3(gcu[0], gcu[1], gcu[2], coefficient[k][0], coefficient[k][1], coefficient[k][2]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) gDir[a] = k < used ? fma32(d, bg[k][a], gDir[a]) : gDir[a];
     }
     const float dd = dot3(dir[0], dir[1], dir[2], gDir[0], gDir[1], gDir[2]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) gP[a] = gP[a] + fma32(-dir[a], dd, gDir[a]) / dist;
+    for (int a = 0; a < 3; ++a) gP[a] = gP[a] + fma32(-dir[a], dd, gDir[a]) * idist;
   }
 
   // Everything above read the parameters before any of them changes.
@@ -349,11 +359,14 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, tangent[a], o.rates[2], 2);
   emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, dOpacity, o.rates[3], 3);
   if (o.mode == kModeGradients) {
+    // Coefficients above the active degree get zero (they are stored, not trained).
     const uint32_t stored = o.shFloats / 3;
     float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
-    for (uint32_t k = 0; k < stored; ++k)
+#pragma unroll
+    for (uint32_t k = 0; k < kCoefficients; ++k)
 #pragma unroll
       for (uint32_t ch = 0; ch < 3; ++ch) out[k * 3 + ch] = k < used ? basis[k] * gcu[ch] : 0.0f;
+    for (uint32_t j = kCoefficients * 3; j < stored * 3; ++j) out[j] = 0.0f;
   } else {
     // For the SH pass: the colour gradient, the direction and the coefficients used.
     shPass[0] = gcu[0];
@@ -409,7 +422,13 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
                   const uint32_t *counters) {
   if (view.count == 0) return;
-  hipLaunchKernelGGL(chain_kernel, dim3((view.count + 255) / 256), dim3(256), 0, stream, buffers, view, params, counters);
+  const dim3 grid((view.count + 255) / 256), block(256);
+  switch (view.shDegree) {
+  case 0: hipLaunchKernelGGL(chain_kernel<0>, grid, block, 0, stream, buffers, view, params, counters); break;
+  case 1: hipLaunchKernelGGL(chain_kernel<1>, grid, block, 0, stream, buffers, view, params, counters); break;
+  case 2: hipLaunchKernelGGL(chain_kernel<2>, grid, block, 0, stream, buffers, view, params, counters); break;
+  default: hipLaunchKernelGGL(chain_kernel<3>, grid, block, 0, stream, buffers, view, params, counters); break;
+  }
   if (params.mode == kModeStep) {
     const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
     hipLaunchKernelGGL(sh_step_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,

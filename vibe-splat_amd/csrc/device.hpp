@@ -109,15 +109,10 @@ __device__ __forceinline__ float vs_log(float x) {
   return fma32(ef, 0x1.62e4p-1f, fma32(ef, 0x1.7f7d1cp-20f, lp));
 }
 
-// IEEE binary16 to float.
+// IEEE binary16 to float: the hardware conversion, which is exact (every half,
+// denormals included, is a float).
 __device__ __forceinline__ float half_to_float(uint32_t h) {
-  const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
-  if (e == 0) {
-    const float v = static_cast<float>(m) * 0x1p-24f;
-    return sign ? -v : v;
-  }
-  if (e == 31) return as_float(sign | 0x7F800000u | (m << 13));
-  return as_float(sign | ((e + 112u) << 23) | (m << 13));
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h)));
 }
 
 // Float to binary16, round to nearest even; overflow to infinity.
@@ -167,7 +162,14 @@ __device__ __forceinline__ float dot3(float a0, float a1, float a2, float b0, fl
 }
 
 // The 3D smoothing filter's opacity factor sqrt(prod s2 / (s2 + f)) (train_preprocess.comp:62-63).
+// With no filter each ratio s2 / (s2 + 0) is exactly 1 for a finite positive s2
+// and NaN for 0 or infinity, which the shortcut returns without dividing.
 __device__ __forceinline__ float filter_factor(float s2x, float s2y, float s2z, float f) {
+  if (f == 0.0f) {
+    const float inf = __builtin_inff();
+    const bool finite = s2x > 0.0f && s2x < inf && s2y > 0.0f && s2y < inf && s2z > 0.0f && s2z < inf;
+    return finite ? 1.0f : __builtin_nanf("");
+  }
   return sqrtf((s2x / (s2x + f)) * (s2y / (s2y + f)) * (s2z / (s2z + f)));
 }
 

@@ -16,7 +16,8 @@ __global__ void math_probe_kernel(int function, const float *__restrict__ in, fl
   case 1: y = vs_log(x); break;
   case 2: y = sqrtf(x); break;
   case 3: y = 1.0f / x; break;
-  default: y = half_to_float(float_to_half(x)); break;
+  case 4: y = half_to_float(float_to_half(x)); break;
+  default: y = half_to_float(as_uint(x) & 0xFFFFu); break;
   }
   out[i] = y;
 }
@@ -28,7 +29,7 @@ void check(hipError_t e, const char *what) {
 } // namespace
 
 void math_probe(int function, const float *in, float *out, uint64_t n) {
-  if (function < 0 || function > 4) throw Error(VSPLAT_INVALID_ARGUMENT, "math probe functions are 0 to 4");
+  if (function < 0 || function > 5) throw Error(VSPLAT_INVALID_ARGUMENT, "math probe functions are 0 to 5");
   if (n == 0) return;
   DeviceArray a, b;
   a.ensure(n * 4);

@@ -13,6 +13,7 @@ struct ProjectOptions {
   uint32_t shRetained, shSeed;
 };
 
+template <int Degree>
 __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ positions, const float *__restrict__ scales,
                                                       const float *__restrict__ rotations, const float *__restrict__ opacities,
                                                       const uint32_t *__restrict__ shWords, const float *__restrict__ filter3d,
@@ -32,10 +33,11 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   float qw = rotations[i * 4], qx = rotations[i * 4 + 1], qy = rotations[i * 4 + 2], qz = rotations[i * 4 + 3];
   const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
   if (!(qlen >= 1e-4f)) return;
-  qw = qw / qlen;
-  qx = qx / qlen;
-  qy = qy / qlen;
-  qz = qz / qlen;
+  const float iq = 1.0f / qlen;
+  qw = qw * iq;
+  qx = qx * iq;
+  qy = qy * iq;
+  qz = qz * iq;
   const float sx = vs_exp(scales[i * 3]), sy = vs_exp(scales[i * 3 + 1]), sz = vs_exp(scales[i * 3 + 2]);
   float R[3][3];
   quat_to_mat(qw, qx, qy, qz, R);
@@ -95,24 +97,32 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
       centerY - radius > static_cast<float>(c.height)) {
     return;
   }
-  // Colour from SH along the view direction.
+  // Colour from SH along the view direction; the active degree is a template
+  // argument so that the coefficient loop unrolls. Dropped coefficients are
+  // skipped by selection, which leaves the sum exactly as a shorter loop would.
   const float dx = px - c.position[0], dy = py - c.position[1], dz = pz - c.position[2];
   const float dl = sqrtf(fma32(dz, dz, fma32(dy, dy, dx * dx)));
+  const float idl = 1.0f / dl;
   float basis[16];
-  sh_basis(dx / dl, dy / dl, dz / dl, c.shDegree, basis);
+  sh_basis(dx * idl, dy * idl, dz * idl, Degree, basis);
   const uint32_t used = sh_coefficients_used(i, c.shCoefficients, opt.shDropout, opt.shRetained, opt.shSeed);
   const uint32_t base = i * c.shWords;
   float col[3] = {0.5f, 0.5f, 0.5f};
-  for (uint32_t k = 0; k < used; ++k) {
 #pragma unroll
-    for (uint32_t ch = 0; ch < 3; ++ch) col[ch] = fma32(basis[k], sh_value(shWords, base, k * 3 + ch), col[ch]);
+  for (uint32_t k = 0; k < (Degree + 1) * (Degree + 1); ++k) {
+#pragma unroll
+    for (uint32_t ch = 0; ch < 3; ++ch) {
+      const float v = sh_value(shWords, base, k * 3 + ch);
+      col[ch] = k < used ? fma32(basis[k], v, col[ch]) : col[ch];
+    }
   }
   if (!(fabsf(col[0]) < 1e30f && fabsf(col[1]) < 1e30f && fabsf(col[2]) < 1e30f)) return;
   // The extent of the 1/255 ellipse, which is what binning and the pixel loops use.
   const float reach = reach_of(opacityUsed);
   const float ex = sqrtf(reach * a), ey = sqrtf(reach * cc);
   float4 *out = reinterpret_cast<float4 *>(splats + i);
-  out[0] = make_float4(cc / det, -b / det, a / det, opacityUsed);
+  const float id = 1.0f / det;
+  out[0] = make_float4(cc * id, -b * id, a * id, opacityUsed);
   out[1] = make_float4(maxf(col[0], 0.0f), maxf(col[1], 0.0f), maxf(col[2], 0.0f), radius);
   out[2] = make_float4(centerX, centerY, minf(ex, radius), minf(ey, radius));
   keys[i] = 0xFFFFFFFFu - as_uint(tz);
@@ -123,8 +133,13 @@ void launch_project(hipStream_t stream, const float *positions, const float *sca
                     uint32_t shRetained, uint32_t shSeed, Splat *splats, uint32_t *keys, uint32_t *values) {
   if (view.count == 0) return;
   const ProjectOptions opt{mipFilter2d ? 1u : 0u, shDropout, shRetained, shSeed};
-  hipLaunchKernelGGL(project_kernel, dim3((view.count + 255) / 256), dim3(256), 0, stream, positions, scales, rotations, opacities,
-                     shWords, filter3d, view, opt, splats, keys, values);
+  const dim3 grid((view.count + 255) / 256), block(256);
+  switch (view.shDegree) {
+  case 0: hipLaunchKernelGGL(project_kernel<0>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
+  case 1: hipLaunchKernelGGL(project_kernel<1>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
+  case 2: hipLaunchKernelGGL(project_kernel<2>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
+  default: hipLaunchKernelGGL(project_kernel<3>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
+  }
 }
 
 } // namespace vsplat

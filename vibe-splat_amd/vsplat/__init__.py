@@ -415,7 +415,7 @@ class Trainer:
 
 
 def math_probe(function: int, x: np.ndarray) -> np.ndarray:
-    """The device's exp (0), log (1), sqrt (2), reciprocal (3) or half round trip (4) of x."""
+    """The device's exp (0), log (1), sqrt (2), reciprocal (3), half round trip (4) or half bits to float (5)."""
     a = np.ascontiguousarray(x, np.float32)
     out = np.empty_like(a)
     _raise(lib().vsplat_math_probe(function, _p(a), _p(out), a.size), "math_probe")
