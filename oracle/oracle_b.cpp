@@ -314,11 +314,13 @@ std::int32_t quantise(float v, float scale) {
   return static_cast<std::int32_t>(std::rint(s));
 }
 
-// The wave's reduction tree: lane l and lane l ^ m are added, for m = 32, 16, 8, 4, 2, 1 in turn.
+// The backward pass's reduction tree over a wave: lane l and lane l ^ m are
+// added, for m = 32, 16, 15, 7, 2, 1 in turn (the partners of the GPU's
+// permlane swaps, row mirror, half-row mirror and quad permutations).
 float wave_sum(float lanes[64]) {
   float a[64];
   std::copy(lanes, lanes + 64, a);
-  for (int m = 32; m >= 1; m >>= 1) {
+  for (int m : {32, 16, 15, 7, 2, 1}) {
     float b[64];
     for (int l = 0; l < 64; ++l) b[l] = a[l] + a[l ^ m];
     std::copy(b, b + 64, a);
@@ -584,15 +586,19 @@ float window_norm(int at, int size) {
 
 float unorm8(std::uint8_t v) { return static_cast<float>(v) * (1.0f / 255.0f); }
 
-// The sum over a tile's 256 pixels (zero outside the image): a wave_sum per row
-// group of 4, then the four in order.
+// The sum over a tile's 256 pixels (zero outside the image): per wave of 64,
+// lane l plus lane l ^ m for m = 32, 16, 8, 4, 2, 1, then the four waves in order.
 float tile_sum(const float values[256]) {
   float total = 0.0f;
   for (int w = 0; w < 4; ++w) {
-    float lanes[64];
-    std::copy(values + w * 64, values + w * 64 + 64, lanes);
-    const float s = wave_sum(lanes);
-    total = w == 0 ? s : total + s;
+    float a[64];
+    std::copy(values + w * 64, values + w * 64 + 64, a);
+    for (int m = 32; m >= 1; m >>= 1) {
+      float b[64];
+      for (int l = 0; l < 64; ++l) b[l] = a[l] + a[l ^ m];
+      std::copy(b, b + 64, a);
+    }
+    total = w == 0 ? a[0] : total + a[0];
   }
   return total;
 }

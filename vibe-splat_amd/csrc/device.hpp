@@ -109,6 +109,42 @@ __device__ __forceinline__ float vs_log(float x) {
   return fma32(ef, 0x1.62e4p-1f, fma32(ef, 0x1.7f7d1cp-20f, lp));
 }
 
+// vs_exp on [-87, 87] without its special cases: the argument is clamped into
+// that range, where 2^n is a normal float and one multiplication applies it
+// exactly as vs_exp's two do. Equal to vs_exp(x) for every x in [-87, 87].
+__device__ __forceinline__ float exp_finite(float x) {
+  x = x < -87.0f ? -87.0f : (x > 87.0f ? 87.0f : x);
+  const float n = __builtin_rintf(x * 0x1.715476p+0f);
+  float r = fma32(n, -0x1.62e4p-1f, x);
+  r = fma32(n, -0x1.7f7d1cp-20f, r);
+  float q = fma32(0x1.6d752p-10f, r, 0x1.120b66p-7f);
+  q = fma32(q, r, 0x1.5554b8p-5f);
+  q = fma32(q, r, 0x1.5554dcp-3f);
+  q = fma32(q, r, 0x1p-1f);
+  const float p = fma32(r * r, q, r) + 1.0f;
+  return p * as_float(static_cast<uint32_t>(static_cast<int>(n) + 127) << 23);
+}
+
+// vs_log for a positive normal finite x, without its special cases; equal to
+// vs_log(x) there. The pixel loops take it of 1 - alpha, in [0.01, 1].
+__device__ __forceinline__ float log_normal(float x) {
+  const uint32_t u = as_uint(x) - 0x3f3504f3u;
+  const int e = static_cast<int32_t>(u) >> 23;
+  const float m = as_float((u & 0x007fffffu) + 0x3f3504f3u);
+  const float f = m - 1.0f;
+  float q = fma32(-0x1.4a4f98p-4f, f, 0x1.088dd0p-3f);
+  q = fma32(q, f, -0x1.0bba2ep-3f);
+  q = fma32(q, f, 0x1.2245e4p-3f);
+  q = fma32(q, f, -0x1.548f6ap-3f);
+  q = fma32(q, f, 0x1.99a7bap-3f);
+  q = fma32(q, f, -0x1.0001f8p-2f);
+  q = fma32(q, f, 0x1.555548p-2f);
+  q = fma32(q, f, -0x1.fffffep-2f);
+  const float lp = fma32(f * f, q, f);
+  const float ef = static_cast<float>(e);
+  return fma32(ef, 0x1.62e4p-1f, fma32(ef, 0x1.7f7d1cp-20f, lp));
+}
+
 // IEEE binary16 to float: the hardware conversion, which is exact (every half,
 // denormals included, is a float).
 __device__ __forceinline__ float half_to_float(uint32_t h) {

@@ -1,15 +1,24 @@
 // Per-tile compositing and its derivative (train_forward.comp:38-100,
 // train_backward.glsl:78-158). One workgroup of 256 threads per 16x16 tile,
 // one thread per pixel; the tile's splats are staged through LDS in batches
-// of 256 with their pixel box already cut to the tile, and every thread works
-// on its own pixel in registers.
+// of 256, each with the tile's pixels it can touch already worked out.
+//
+// Staging: for each splat, a 16-bit column mask and a 16-bit row mask of the
+// pixels inside both its extent box (splat_pixels, train_types.glsl:43-48) and
+// its radius square (the first test of splat_at): a pixel is a candidate iff
+// its column and row bits are set. The masks are computed with the same float
+// operations as the per-pixel tests they replace, so the set is the same.
+// A wave skips a splat none of its lanes is a candidate for; otherwise every
+// lane evaluates the Gaussian without branches and the result is selected.
 //
 // Backward: wave w of a tile holds pixel rows 4w..4w+3. For each splat the 14
-// per-pixel sums are reduced over the wave by a transposed butterfly (each
-// xor step halves the values a lane carries, so 16 slots cost 15 shuffles),
-// after which lane 4j holds total j; the totals go to global memory with one
-// atomic each, as 64-bit fixed point (quantised per wave total, so the integer
-// sums are the same in any order) or as float atomics.
+// per-pixel sums are reduced over the wave by a transposed butterfly in which
+// each step halves the values a lane carries: partner lanes l ^ 32 and l ^ 16
+// through the permlane swaps, l ^ 15 and l ^ 7 through DPP row and half-row
+// mirrors, then l ^ 2 and l ^ 1 through quad permutations. Lane 4j ends with
+// total j, which goes to global memory with one atomic: 64-bit fixed point
+// (each wave total quantised, so the integer sums are the same in any order)
+// or float.
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -17,32 +26,14 @@ namespace vsplat {
 
 namespace {
 
-// A splat as the pixel loops use it: the extent is replaced by the pixel box
-// within this tile.
 struct Staged {
   float qx, qy, qz, o;
-  float r, g, b, radius;
+  float r, g, b, pad0;
   float cx, cy;
-  uint32_t box, pad;
+  uint32_t mask; // columns in bits 0-15, rows in bits 16-31, tile-local
+  uint32_t pad1;
 };
 static_assert(sizeof(Staged) == 48, "48-byte staged splat");
-
-__device__ __forceinline__ Staged stage(const Splat &s, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1) {
-  return Staged{s.qx, s.qy, s.qz, s.o, s.r, s.g, s.b, s.radius, s.cx, s.cy, splat_box(s, x0, y0, x1, y1), 0u};
-}
-
-// splat_at (train_types.glsl:81-94); e = exp(power) is kept for the backward pass.
-__device__ __forceinline__ bool splat_at(const Staged &s, uint32_t x, uint32_t y, float &dx, float &dy, float &e, float &alpha) {
-  dx = (static_cast<float>(x) + 0.5f) - s.cx;
-  dy = (static_cast<float>(y) + 0.5f) - s.cy;
-  if (fabsf(dx) > s.radius || fabsf(dy) > s.radius) return false;
-  const float power = fma32(-0.5f, fma32(s.qx * dx, dx, (s.qz * dy) * dy), -((s.qy * dx) * dy));
-  if (power > 0.0f) return false;
-  e = vs_exp(power);
-  const float a = s.o * e;
-  alpha = a < 0.99f ? a : 0.99f;
-  return alpha >= kMinAlpha;
-}
 
 struct TileGeometry {
   uint32_t x0, y0, x1, y1;
@@ -51,6 +42,33 @@ __device__ __forceinline__ TileGeometry tile_geometry(uint32_t tile, uint32_t gr
   const uint32_t tx = tile % gridW, ty = tile / gridW;
   const uint32_t x0 = tx * kTileSize, y0 = ty * kTileSize;
   return {x0, y0, x0 + kTileSize < width ? x0 + kTileSize : width, y0 + kTileSize < height ? y0 + kTileSize : height};
+}
+
+// The pixels of the tile a splat can touch: inside the extent box and within
+// the radius (|d| <= r on both axes, d from the pixel centre).
+__device__ __forceinline__ Staged stage(const Splat &s, const TileGeometry &geo) {
+  const uint32_t box = splat_box(s, geo.x0, geo.y0, geo.x1, geo.y1);
+  uint32_t mask = 0;
+  if (box != 0u) {
+    const uint32_t bx0 = box & 0xFFu, by0 = (box >> 8) & 0xFFu, bx1 = (box >> 16) & 0xFFu, by1 = box >> 24;
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) {
+      const float dx = (static_cast<float>(geo.x0 + k) + 0.5f) - s.cx;
+      const float dy = (static_cast<float>(geo.y0 + k) + 0.5f) - s.cy;
+      const bool col = k >= bx0 && k < bx1 && !(fabsf(dx) > s.radius);
+      const bool row = k >= by0 && k < by1 && !(fabsf(dy) > s.radius);
+      mask |= (col ? 1u : 0u) << k;
+      mask |= (row ? 1u : 0u) << (16 + k);
+    }
+  }
+  return Staged{s.qx, s.qy, s.qz, s.o, s.r, s.g, s.b, 0.0f, s.cx, s.cy, mask, 0u};
+}
+
+__device__ __forceinline__ bool covers(uint32_t mask, uint32_t lx, uint32_t ly) { return ((mask >> lx) & (mask >> (16u + ly)) & 1u) != 0u; }
+
+// The Gaussian's exponent at offset (dx, dy) (splat_at, train_types.glsl:88).
+__device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
+  return fma32(-0.5f, fma32(s.qx * dx, dx, (s.qz * dy) * dy), -((s.qy * dx) * dy));
 }
 
 __global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
@@ -63,6 +81,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ 
   const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
+  const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   const uint2 range = ranges[tile];
   float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, logT = 0.0f;
   uint32_t stop = range.x;
@@ -72,26 +91,36 @@ __global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ 
     end -= count;
     // Also the barrier that keeps the batch from being overwritten while read.
     if (__syncthreads_count(!done) == 0) break;
-    if (threadIdx.x < count) batch[threadIdx.x] = stage(load_splat(splats, entryValues[end + threadIdx.x]), geo.x0, geo.y0, geo.x1, geo.y1);
+    if (threadIdx.x < count) batch[threadIdx.x] = stage(load_splat(splats, entryValues[end + threadIdx.x]), geo);
     __syncthreads();
-    if (done) continue;
     for (uint32_t k = count; k-- > 0;) {
       const Staged s = batch[k];
-      if (!in_box(s.box, lx, ly)) continue;
-      float dx, dy, e, alpha;
-      if (!splat_at(s, x, y, dx, dy, e, alpha)) continue;
-      const float logRest = vs_log(1.0f - alpha);
-      if (logT + logRest < kLogMinTransmittance) {
-        stop = end + k + 1;
-        done = true;
-        break;
+      const bool candidate = !done && covers(s.mask, lx, ly);
+      if (!__any(candidate)) {
+        if (__all(done)) break;
+        continue;
       }
-      const float w = T * alpha;
-      r = fma32(w, s.r, r);
-      g = fma32(w, s.g, g);
-      b = fma32(w, s.b, b);
-      T = T * (1.0f - alpha);
-      logT = logT + logRest;
+      const float dx = px - s.cx, dy = py - s.cy;
+      const float power = power_at(s, dx, dy);
+      // Below -87 alpha is far under 1/255 either way, so the clamp in exp_finite decides nothing.
+      const float e = exp_finite(power);
+      const float a = s.o * e;
+      const float alpha = a < 0.99f ? a : 0.99f;
+      const bool ok = candidate && !(power > 0.0f) && alpha >= kMinAlpha;
+      const float logRest = log_normal(1.0f - alpha);
+      if (ok) {
+        if (logT + logRest < kLogMinTransmittance) {
+          stop = end + k + 1;
+          done = true;
+        } else {
+          const float w = T * alpha;
+          r = fma32(w, s.r, r);
+          g = fma32(w, s.g, g);
+          b = fma32(w, s.b, b);
+          T = T * (1.0f - alpha);
+          logT = logT + logRest;
+        }
+      }
     }
   }
   if (inside) {
@@ -109,15 +138,45 @@ __device__ __forceinline__ int32_t quantise(float v, float scale) {
   return static_cast<int32_t>(__builtin_rintf(s));
 }
 
-template <int Count>
-__device__ __forceinline__ void halve(float (&v)[16], uint32_t lane, int offset) {
-  const bool upper = (lane & static_cast<uint32_t>(offset)) != 0;
+__device__ __forceinline__ float dpp_row_mirror(float v) { return as_float(__builtin_amdgcn_update_dpp(0, static_cast<int>(as_uint(v)), 0x140, 0xF, 0xF, false)); }
+__device__ __forceinline__ float dpp_half_mirror(float v) { return as_float(__builtin_amdgcn_update_dpp(0, static_cast<int>(as_uint(v)), 0x141, 0xF, 0xF, false)); }
+__device__ __forceinline__ float dpp_xor2(float v) { return as_float(__builtin_amdgcn_update_dpp(0, static_cast<int>(as_uint(v)), 0x4E, 0xF, 0xF, false)); }
+__device__ __forceinline__ float dpp_xor1(float v) { return as_float(__builtin_amdgcn_update_dpp(0, static_cast<int>(as_uint(v)), 0xB1, 0xF, 0xF, false)); }
+
+// Sixteen values per lane to one total per lane group: afterwards lane l holds
+// the wave's total of value (l >> 2) & 15 in v[0].
+__device__ __forceinline__ void reduce_wave(float (&v)[16], uint32_t lane) {
+  // Partner l ^ 32: lanes 0-31 keep values 0-7, lanes 32-63 values 8-15.
 #pragma unroll
-  for (int i = 0; i < Count / 2; ++i) {
-    const float send = upper ? v[i] : v[i + Count / 2];
-    const float keep = upper ? v[i + Count / 2] : v[i];
-    v[i] = keep + __shfl_xor(send, offset, 64);
+  for (int i = 0; i < 8; ++i) {
+    const auto p = __builtin_amdgcn_permlane32_swap(as_uint(v[i]), as_uint(v[i + 8]), false, false);
+    v[i] = as_float(p[0]) + as_float(p[1]);
   }
+  // Partner l ^ 16: rows 0 and 2 keep the lower four, rows 1 and 3 the upper.
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const auto p = __builtin_amdgcn_permlane16_swap(as_uint(v[i]), as_uint(v[i + 4]), false, false);
+    v[i] = as_float(p[0]) + as_float(p[1]);
+  }
+  // Partner l ^ 15 (row mirror): lanes with bit 3 clear keep the lower two.
+  {
+    const bool upper = (lane & 8u) != 0u;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float send = upper ? v[i] : v[i + 2];
+      const float keep = upper ? v[i + 2] : v[i];
+      v[i] = keep + dpp_row_mirror(send);
+    }
+  }
+  // Partner l ^ 7 (half-row mirror): lanes with bit 2 clear keep the lower one.
+  {
+    const bool upper = (lane & 4u) != 0u;
+    const float send = upper ? v[0] : v[1];
+    const float keep = upper ? v[1] : v[0];
+    v[0] = keep + dpp_half_mirror(send);
+  }
+  v[0] = v[0] + dpp_xor2(v[0]);
+  v[0] = v[0] + dpp_xor1(v[0]);
 }
 
 template <bool Fixed>
@@ -135,9 +194,10 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
   const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4, lane = threadIdx.x & 63u;
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
+  const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   const uint32_t at = inside ? y * width + x : 0u;
   const float logTotal = inside ? image[at].w : 0.0f;
-  float4 adj = inside ? adjoint[at] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 adj = inside ? adjoint[at] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float perPixel = 1.0f / (3.0f * static_cast<float>(width) * static_cast<float>(height));
   const uint2 range = ranges[tile];
   const uint32_t stop = inside ? stops[at] : range.y;
@@ -154,54 +214,54 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
     if (threadIdx.x < count) {
       const uint32_t index = entryValues[start + threadIdx.x];
       batchIndex[threadIdx.x] = index;
-      batch[threadIdx.x] = stage(load_splat(splats, index), geo.x0, geo.y0, geo.x1, geo.y1);
+      batch[threadIdx.x] = stage(load_splat(splats, index), geo);
     }
     __syncthreads();
     for (uint32_t k = 0; k < count; ++k) {
       const Staged s = batch[k];
+      const bool candidate = inside && start + k >= stop && covers(s.mask, lx, ly);
+      if (!__any(candidate)) continue;
+      const float dx = px - s.cx, dy = py - s.cy;
+      const float power = power_at(s, dx, dy);
+      const float e = exp_finite(power);
+      const float a = s.o * e;
+      const float alpha = a < 0.99f ? a : 0.99f;
+      const bool ok = candidate && !(power > 0.0f) && alpha >= kMinAlpha;
+      if (!__any(ok)) continue;
+      const float logRest = log_normal(1.0f - alpha);
+      const float T = exp_finite((logTotal - logBehind) - logRest);
+      const float Ta = T * alpha;
+      // At the 0.99 clamp alpha no longer depends on the parameters.
+      const bool free = ok && a < 0.99f;
+      const float dAlpha = T * fma32(adj.z, s.b - Bb, fma32(adj.y, s.g - Bg, adj.x * (s.r - Br)));
+      const float dPower = dAlpha * alpha;
+      const float vx = fma32(s.qy, dy, s.qx * dx), vy = fma32(s.qz, dy, s.qy * dx);
+      const float g1 = dPower * vx, g2 = dPower * vy;
+      const float hp = 0.5f * dPower;
       float v[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) v[i] = 0.0f;
-      bool active = false;
-      float dx, dy, e, alpha;
-      if (inside && start + k >= stop && in_box(s.box, lx, ly) && splat_at(s, x, y, dx, dy, e, alpha)) {
-        active = true;
-        const float logRest = vs_log(1.0f - alpha);
-        const float T = vs_exp((logTotal - logBehind) - logRest);
-        const float Ta = T * alpha;
-        v[6] = Ta * adj.x;
-        v[7] = Ta * adj.y;
-        v[8] = Ta * adj.z;
-        v[9] = Ta * perPixel;
-        // At the 0.99 clamp alpha no longer depends on the parameters.
-        if (s.o * e < 0.99f) {
-          const float dAlpha = T * fma32(adj.z, s.b - Bb, fma32(adj.y, s.g - Bg, adj.x * (s.r - Br)));
-          v[0] = dAlpha * e;
-          const float dPower = dAlpha * alpha;
-          const float vx = fma32(s.qy, dy, s.qx * dx), vy = fma32(s.qz, dy, s.qy * dx);
-          v[1] = dPower * vx;
-          v[2] = dPower * vy;
-          v[10] = fabsf(v[1]);
-          v[11] = fabsf(v[2]);
-          const float hp = 0.5f * dPower;
-          v[3] = (hp * vx) * vx;
-          v[4] = (hp * vx) * vy;
-          v[5] = (hp * vy) * vy;
-        }
+      v[0] = free ? dAlpha * e : 0.0f;
+      v[1] = free ? g1 : 0.0f;
+      v[2] = free ? g2 : 0.0f;
+      v[3] = free ? (hp * vx) * vx : 0.0f;
+      v[4] = free ? (hp * vx) * vy : 0.0f;
+      v[5] = free ? (hp * vy) * vy : 0.0f;
+      v[6] = ok ? Ta * adj.x : 0.0f;
+      v[7] = ok ? Ta * adj.y : 0.0f;
+      v[8] = ok ? Ta * adj.z : 0.0f;
+      v[9] = ok ? Ta * perPixel : 0.0f;
+      v[10] = free ? fabsf(g1) : 0.0f;
+      v[11] = free ? fabsf(g2) : 0.0f;
+      v[12] = v[13] = v[14] = v[15] = 0.0f;
+      if (ok) {
         Br = fma32(alpha, s.r - Br, Br);
         Bg = fma32(alpha, s.g - Bg, Bg);
         Bb = fma32(alpha, s.b - Bb, Bb);
         logBehind = logBehind + logRest;
       }
-      if (!__any(active)) continue;
-      halve<16>(v, lane, 32);
-      halve<8>(v, lane, 16);
-      halve<4>(v, lane, 8);
-      halve<2>(v, lane, 4);
-      v[0] += __shfl_xor(v[0], 2, 64);
-      v[0] += __shfl_xor(v[0], 1, 64);
+      reduce_wave(v, lane);
+      // Values 12 and 13 of the kRawPerSplat slots are always zero here.
       const uint32_t value = lane >> 2;
-      if ((lane & 3u) == 0 && value < kRawPerSplat) {
+      if ((lane & 3u) == 0 && value < 12u) {
         const size_t slot = static_cast<size_t>(batchIndex[k]) * kRawPerSplat + value;
         if constexpr (Fixed) {
           const int32_t q = quantise(v[0], fixedScale);
