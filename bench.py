@@ -21,7 +21,6 @@ Prints one JSON line. --gpus above 1 is refused: this build runs on one GPU.
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import dataclasses
 import json
 import os
@@ -45,29 +44,6 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-class Hip:
-    """The few HIP runtime calls the benchmark needs for a device-resident target."""
-
-    def __init__(self):
-        vsplat.lib()  # loads libamdhip64 as a dependency
-        self.lib = C.CDLL("/opt/rocm/lib/libamdhip64.so")
-        self.lib.hipMalloc.argtypes = [C.c_void_p, C.c_size_t]
-        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.lib.hipFree.argtypes = [C.c_void_p]
-
-    def upload(self, array: np.ndarray) -> int:
-        a = np.ascontiguousarray(array)
-        ptr = C.c_void_p()
-        if self.lib.hipMalloc(C.byref(ptr), a.nbytes) != 0:
-            raise RuntimeError("hipMalloc failed")
-        if self.lib.hipMemcpy(ptr, C.c_void_p(a.ctypes.data), a.nbytes, 1) != 0:
-            raise RuntimeError("hipMemcpy failed")
-        return ptr.value
-
-    def free(self, ptr: int):
-        self.lib.hipFree(C.c_void_p(ptr))
 
 
 def touched_entries(ranges: np.ndarray, stops: np.ndarray, width: int, height: int) -> int:
@@ -156,8 +132,8 @@ def main():
     T = ((cam.width + 15) // 16) * ((cam.height + 15) // 16)
     fixed = not args.float_atomics
     trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed))
-    hip = Hip()
-    d_target = hip.upload(target)
+    device_target = vsplat.DeviceBuffer(target)
+    d_target = device_target.ptr
     rates = vsplat.LearningRates()
     mode = vsplat.MODE_GRADIENTS if args.mode == "gradients" else vsplat.MODE_STEP
 
@@ -169,15 +145,22 @@ def main():
     trainer.synchronize()
     trainer.reset_timings()
 
+    def timed(stage_events: bool):
+        trainer.set_stage_timing(stage_events)
+        trainer.reset_timings()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            trainer.enqueue(cam, d_target, mode, rates if mode == vsplat.MODE_STEP else None)
+        st = trainer.synchronize()
+        return st, (time.perf_counter() - t0) * 1000.0 / args.steps
+
     log(f"timing {args.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        trainer.enqueue(cam, d_target, mode, rates if mode == vsplat.MODE_STEP else None)
-    stats = trainer.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = elapsed * 1000.0 / args.steps
+    stats, ms = timed(False)
+    log(f"  {ms:.4f} ms per step")
+    log(f"timing {args.steps} steps with per-stage HIP events")
+    _, ms_staged = timed(True)
     stages = dict(trainer.timings())
-    log("  " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
+    log(f"  {ms_staged:.4f} ms per step: " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
 
     V, E = stats.visible_splats, stats.tile_entries
     Ep = touched_entries(trainer.inspect("ranges"), trainer.inspect("stops"), cam.width, cam.height)
@@ -187,10 +170,14 @@ def main():
     dominant = max(("composite", "forward"), ("backward", "backward"), key=lambda s: stages.get(s[0], 0.0))
     kernel_ms = stages[dominant[0]]
     achieved = per_stage[dominant[1]] / (kernel_ms * 1e-3) / 1e9
+    # HBM bytes per launch of that kernel, from the FETCH_SIZE / WRITE_SIZE counter runs
+    # on this workload (profiles/), when this run is that workload.
     traffic = None
-    traffic_file = ROOT / "profiles" / f"{dominant[1]}_kernel_traffic_c1.json"
-    if traffic_file.exists():
-        traffic = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
+    traffic_file = ROOT / "profiles" / "traffic_1M_1280x720_fixed_gradients.json"
+    kernel_name = {"forward": "forward_kernel", "backward": "backward_kernel<true>"}[dominant[1]]
+    if traffic_file.exists() and fixed and N == 1_000_000 and args.mode == "gradients":
+        entry = json.loads(traffic_file.read_text())["kernels"].get(kernel_name)
+        traffic = None if entry is None else entry["hbm_bytes_per_launch"]
 
     result = {
         "metric": METRIC,
@@ -214,13 +201,14 @@ def main():
                      "algorithmic_bytes": int(per_stage[dominant[1]]), "kernel_ms": round(kernel_ms, 4)},
         "iteration": {"algorithmic_bytes": int(total), "hbm_frac": round(total / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                       "V": V, "E": E, "Ep": Ep, "P": P, "T": T, "Bp": Bp, "loss": round(float(stats.loss), 6)},
+        "ms_per_step_with_stage_events": round(ms_staged, 4),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline on {threads} threads")
         result["cpu_baseline"] = cpu_baseline(w, threads)
-    hip.free(d_target)
+    device_target.free()
     trainer.close()
     print(json.dumps(result), flush=True)
 

@@ -157,6 +157,9 @@ int vsplat_timings(const vsplat_trainer *trainer, vsplat_timing *out, uint32_t c
 
 /* Forgets the stage times gathered so far (waits for queued work first). */
 int vsplat_reset_timings(vsplat_trainer *trainer);
+/* Whether steps queued with vsplat_enqueue record per-stage events (on by
+ * default); the synchronous calls always do. No reference counterpart. */
+int vsplat_set_stage_timing(vsplat_trainer *trainer, int32_t enabled);
 
 /* The message of the calling thread's last failed call. */
 const char *vsplat_last_error(void);
@@ -169,9 +172,11 @@ const char *vsplat_last_error(void);
 int vsplat_enqueue(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *device_target, int32_t mode,
                    const vsplat_learning_rates *rates);
 /* Waits for queued work; stats (may be NULL) of the last queued step. Returns
- * VSPLAT_RUNTIME_ERROR if any queued view needed more tile entries than were
- * reserved: that step computed nothing and changed nothing. vsplat_compute_gradients
- * with the view, or vsplat_reserve_entries, grows the reservation. */
+ * VSPLAT_RUNTIME_ERROR if a queued view needed more tile entries than were
+ * reserved: that step and every step queued after it computed nothing and
+ * changed nothing (Adam's step count included), while the steps before it
+ * completed. vsplat_compute_gradients with the view, or vsplat_reserve_entries,
+ * grows the reservation. */
 int vsplat_synchronize(vsplat_trainer *trainer, vsplat_step_stats *stats);
 /* The trainer's HIP stream (a hipStream_t). */
 void *vsplat_stream(vsplat_trainer *trainer);

@@ -399,3 +399,24 @@ class Model:
                               _p(r.splats), _p(st), *[_p(s) for s in states], *[_p(x) for x in g], C.byref(opt)),
                "ob_chain")
         return g, st
+
+    def step(self, r: ModelResult, rates, *, rest_scale=0.05, step=1, selective=True, opacity_reg=0.0, scale_reg=0.0):
+        """Chain rule in step mode, then SH's part of Adam (train_gradient.comp,
+        train_sh_step.comp), from zero optimiser state: the parameters after one
+        Adam step, as (positions, scales, rotations, opacities, sh_words)."""
+        n = self.n
+        c = self.cloud
+        pos = np.ascontiguousarray(c.positions, np.float32).reshape(-1).copy()
+        sc = np.ascontiguousarray(c.scales, np.float32).reshape(-1).copy()
+        rot = np.ascontiguousarray(c.rotations, np.float32).reshape(-1).copy()
+        op = np.ascontiguousarray(c.opacities, np.float32).copy()
+        words = self.words.copy()
+        raw = r.raw.copy()
+        st = np.zeros((n, 8), np.float32)
+        states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
+        opt = self.chain_options(2, 4 if selective else 0, opacity_reg, scale_reg, rates=rates, rest_scale=rest_scale, step=step)
+        _check(lib().ob_chain(_p(pos), _p(sc), _p(rot), _p(op), _p(words), _p(self.filter3d), C.byref(self.camera), _p(raw),
+                              _p(r.splats), _p(st), *[_p(s) for s in states], None, None, None, None, None, C.byref(opt)),
+               "ob_chain")
+        _check(lib().ob_sh_step(_p(words), _p(states[4]), _p(raw), C.byref(self.camera), C.byref(opt)), "ob_sh_step")
+        return pos, sc, rot, op, words

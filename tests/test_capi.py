@@ -96,3 +96,46 @@ def test_last_error_is_per_thread():
     t.start()
     t.join()
     assert b"empty" in mine and seen == [b""]
+
+
+WRAPPER_PROGRAM = r"""
+#include "vsplat_trainer.hpp"
+#include <cstdio>
+#include <stdexcept>
+int main() {
+  vsplat::GaussianCloud empty;
+  try { vsplat::Trainer t(empty); return 1; } catch (const std::runtime_error &e) { std::printf("empty:%s\n", e.what()); }
+  vsplat::GaussianCloud c;
+  c.shDegree = 0;
+  c.positions = {0.f, 0.f, 0.f};
+  c.scales = {-3.f, -3.f, -3.f};
+  c.rotations = {1.f, 0.f, 0.f, 0.f};
+  c.opacities = {0.f};
+  c.sh = {0.f, 0.f};  // one float short
+  try { vsplat::Trainer t(c); return 2; } catch (const std::runtime_error &e) { std::printf("sizes:%s\n", e.what()); }
+  c.sh.push_back(0.f);
+  vsplat::TrainerOptions o;
+  o.poseGradients = true;
+  try { vsplat::Trainer t(c, o); return 3; } catch (const vsplat::Unsupported &e) { std::printf("unsupported\n"); }
+  static_assert(vsplat::Trainer::kBytesPerTileEntry == 16, "");
+  vsplat::LearningRates r;
+  std::printf("rates:%g %g\n", r.position, r.shRestScale);
+  return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("std", ["c++17", "c++20"])
+def test_cpp_wrapper_maps_statuses_to_reference_exceptions(tmp_path, std):
+    """include/vsplat_trainer.hpp compiles as C++17 and C++20, links against the
+    library, and turns the statuses the C interface returns before any device
+    call into the reference's exception types."""
+    import subprocess
+    src = tmp_path / "wrapper.cpp"
+    src.write_text(WRAPPER_PROGRAM)
+    exe = tmp_path / "wrapper"
+    libdir = Path(vsplat.__file__).resolve().parent
+    subprocess.run(["g++", f"-std={std}", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
+                    f"-L{libdir}", "-lvsplat", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert "empty:" in out and "sizes:" in out and "unsupported" in out and "rates:0.00016 0.05" in out

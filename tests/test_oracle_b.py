@@ -209,3 +209,55 @@ def test_first_adam_step_moves_by_the_rate():
             expected = before[i, j] - math.copysign(rate, gsh[i, j])
             tol = 2e-5 if j < 3 else max(2e-5, abs(expected) / 512)
             assert abs(sh[i, j] - expected) <= tol
+
+
+def _bf16_neighbours(x):
+    """The bf16 values either side of each float32 (truncation and one bf16 ulp
+    away from zero), as float32."""
+    bits = np.asarray(x, np.float32).view(np.uint32)
+    low = (bits & 0xFFFF0000).view(np.float32)
+    high = ((bits & 0xFFFF0000) + 0x10000).astype(np.uint32).view(np.float32)
+    return low, high
+
+
+def test_adam_moments_round_to_bf16_as_stated():
+    """train_step.glsl:49-60 and bf16.glsl:10-23 on the model: after the first step
+    the first moment is (1 - b1) g rounded to the nearest bf16 (ties to even), and
+    the second moment is ((1 - b2) g) g rounded to one of its two bf16 neighbours
+    (stochastic rounding). Checked by distance in float64, not by the bit formula."""
+    import ctypes as C
+    cloud, cam, target = F.random_cloud(8, 1, 5), F.train_camera(40, 32), F.smooth_target(40, 32, 2)
+    m = O.Model(cloud, cam)
+    r = m.forward()
+    m.loss(r, target, 0.2)
+    m.backward_float(r)
+    g, _ = m.gradients(r)
+    n = cloud.count
+    pos = cloud.positions.copy().reshape(-1); sc = cloud.scales.copy().reshape(-1)
+    rot = cloud.rotations.copy().reshape(-1); op = cloud.opacities.copy()
+    words = m.words.copy()
+    states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 12)]
+    stats = np.zeros((n, 8), np.float32)
+    opt = m.chain_options(mode=2, flags=4, rates=(1.6e-4, 0.005, 0.001, 0.05, 0.02), step=1)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    assert O.lib().ob_chain(p(pos), p(sc), p(rot), p(op), p(words), None, C.byref(m.camera), p(r.raw.copy()), p(r.splats),
+                            p(stats), *[p(s) for s in states], None, None, None, None, None, C.byref(opt)) == 0
+    one = np.float32(1)
+    checked = 0
+    for grad, state in zip(g[:4], states[:4]):
+        grad = np.asarray(grad, np.float32).reshape(-1)
+        touched = grad != 0
+        m1 = ((one - np.float32(0.9)) * grad)[touched]
+        v1 = (((one - np.float32(0.999)) * grad) * grad)[touched]
+        got_m = ((state[touched] & 0xFFFF) << 16).astype(np.uint32).view(np.float32)
+        got_v = (state[touched] & 0xFFFF0000).astype(np.uint32).view(np.float32)
+        lo, hi = _bf16_neighbours(m1)
+        dlo, dhi = np.abs(lo.astype(np.float64) - m1), np.abs(hi.astype(np.float64) - m1)
+        even_lo = ((lo.view(np.uint32) >> 16) & 1) == 0
+        nearest = np.where((dlo < dhi) | ((dlo == dhi) & even_lo), lo, hi)
+        assert np.array_equal(got_m.view(np.uint32), nearest.view(np.uint32))
+        lo, hi = _bf16_neighbours(v1)
+        assert np.all((got_v == lo) | (got_v == hi))
+        assert np.all(state[~touched] == 0)  # selective Adam: untouched state stays zero
+        checked += int(touched.sum())
+    assert checked > 20

@@ -67,12 +67,15 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const uint32_t *__restri
   if ((threadIdx.x & 63u) == 0 && mask != 0) atomicAdd(&counters[kVisible], static_cast<uint32_t>(__popcll(mask)));
 }
 
-// Whether the entries fit: the later kernels only list and sort what fits.
-__global__ void bin_check_kernel(uint32_t *counters, uint32_t capacity) {
+// Whether the entries fit. A view that does not fit sets the sticky overflow
+// word; from then on every queued pass lists nothing and its chain rule does
+// nothing, so the passes that ran are exactly those before the first overflow.
+__global__ void bin_check_kernel(uint32_t *counters, uint32_t capacity, uint32_t isStep) {
   const uint32_t entries = counters[kEntries];
-  if (entries > capacity) {
+  if (counters[kOverflow] != 0 || entries > capacity) {
     counters[kEntriesUsed] = 0;
     counters[kOverflow] = 1;
+    counters[kSkipped] += isStep;
   } else {
     counters[kEntriesUsed] = entries;
   }
@@ -83,7 +86,7 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
                                                        uint32_t gridH, const uint32_t *__restrict__ offsets, const uint32_t *counters,
                                                        uint32_t *__restrict__ entryKeys, uint32_t *__restrict__ entryValues) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= count || counters[kEntriesUsed] != counters[kEntries] || keys[r] == kInvisible) return;
+  if (r >= count || counters[kOverflow] != 0 || keys[r] == kInvisible) return;
   const uint32_t splat = values[r];
   const Splat s = load_splat(splats, splat);
   const float reach = reach_of(s.o), iqx = 1.0f / s.qx, iqz = 1.0f / s.qz;
@@ -120,8 +123,8 @@ void launch_bin_count(hipStream_t stream, const uint32_t *keys, const uint32_t *
                      coverage, counters);
 }
 
-void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity) {
-  hipLaunchKernelGGL(bin_check_kernel, dim3(1), dim3(1), 0, stream, counters, capacity);
+void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity, bool isStep) {
+  hipLaunchKernelGGL(bin_check_kernel, dim3(1), dim3(1), 0, stream, counters, capacity, isStep ? 1u : 0u);
 }
 
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,

@@ -207,6 +207,11 @@ extern "C" int vsplat_math_probe(int32_t function, const float *in, float *out, 
   return guarded([&] { vsplat::math_probe(function, in, out, n); });
 }
 
+extern "C" int vsplat_set_stage_timing(vsplat_trainer *t, int32_t enabled) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.set_queued_timing(enabled != 0); });
+}
+
 extern "C" int vsplat_reset_timings(vsplat_trainer *t) {
   if (t == nullptr) return invalid("null trainer");
   return guarded([&] { t->impl.reset_timings(); });

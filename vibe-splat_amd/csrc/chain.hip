@@ -115,7 +115,7 @@ template <int Degree>
 __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.count) return;
-  if (counters[kEntriesUsed] != counters[kEntries]) return; // the view did not fit: nothing was computed
+  if (counters[kOverflow] != 0) return; // a view did not fit: this pass computed nothing
   float g[kRawPerSplat];
   if (o.fixedPoint) {
     const long long *raw = reinterpret_cast<const long long *>(B.rawFixed) + static_cast<size_t>(i) * kRawPerSplat;
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t words = c.shWords;
   if (w >= c.count * words) return;
-  if (counters[kEntriesUsed] != counters[kEntries]) return;
+  if (counters[kOverflow] != 0) return;
   const uint32_t i = w / words, q = w - i * words;
   const float *g = shPass + static_cast<size_t>(i) * kShPassFloats;
   const uint32_t used = as_uint(g[9]);

@@ -50,8 +50,9 @@ enum Counter : uint32_t {
   kVisible = 0,      // splats in view
   kEntries = 1,      // tile entries this view needs
   kEntriesUsed = 2,  // entries actually listed: kEntries, or 0 if they did not fit
-  kOverflow = 3,     // sticky: a view needed more entries than the buffers hold
-  kCounterWords = 4,
+  kOverflow = 3,     // sticky: a view needed more entries than the buffers hold; every later pass is skipped until the host clears it
+  kSkipped = 4,      // step-mode passes skipped that way since the host last cleared it
+  kCounterWords = 5,
 };
 
 __device__ __forceinline__ float fma32(float a, float b, float c) { return __builtin_fmaf(a, b, c); }

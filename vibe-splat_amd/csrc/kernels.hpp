@@ -57,7 +57,7 @@ void launch_fill_u32(hipStream_t stream, uint32_t *dst, uint64_t count, uint32_t
 
 void launch_bin_count(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
                       uint32_t gridW, uint32_t gridH, uint32_t *coverage, uint32_t *counters);
-void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity);
+void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity, bool isStep);
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
                      uint32_t gridW, uint32_t gridH, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
                      uint32_t *entryValues);

@@ -318,7 +318,7 @@ void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, u
   launch_bin_count(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), count_, gridW_, gridH_, coverage_.as<uint32_t>(),
                    counters);
   launch_exclusive_scan(s, coverage_.as<uint32_t>(), count_, scanScratch_.as<uint32_t>(), counters + kEntries);
-  launch_bin_check(s, counters, static_cast<uint32_t>(entryCapacity_));
+  launch_bin_check(s, counters, static_cast<uint32_t>(entryCapacity_), mode == kModeStep);
   mark("count tiles");
   launch_fill_u32(s, ranges_.as<uint32_t>(), static_cast<uint64_t>(tiles_) * 2, 0u);
   if (entryCapacity_ > 0) {
@@ -410,7 +410,7 @@ void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_lea
   launch_chain(stream_, b, v, p, counters_.as<uint32_t>());
 }
 
-vsplat_step_stats Trainer::read_stats(bool &overflow) {
+vsplat_step_stats Trainer::read_stats(bool &overflow, uint32_t *skipped) {
   uint8_t *host = readback_.as<uint8_t>();
   check(hipMemcpyAsync(host, counters_.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToHost, stream_), "read counters");
   check(hipMemcpyAsync(host + kCounterWords * 4, sums_.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToHost, stream_), "read sums");
@@ -420,6 +420,7 @@ vsplat_step_stats Trainer::read_stats(bool &overflow) {
   std::memcpy(counters, host, sizeof(counters));
   std::memcpy(sums, host + kCounterWords * 4, sizeof(sums));
   overflow = counters[kOverflow] != 0;
+  if (skipped != nullptr) *skipped = counters[kSkipped];
   vsplat_step_stats st{};
   // trainer.cpp:606-620: the per-tile totals summed in double, then in float as the reference.
   const double perValue = 1.0 / (3.0 * static_cast<double>(width_) * static_cast<double>(height_));
@@ -494,7 +495,7 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
     stats = read_stats(overflow);
     if (!overflow) break;
     cancel_timing();
-    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
+    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
     if (attempt > 0 && stats.tile_entries <= entryCapacity_) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
     grow_entries(stats.tile_entries);
   }
@@ -514,7 +515,7 @@ void Trainer::render(const vsplat_camera &camera, float *out) {
     bool overflow = false;
     const vsplat_step_stats stats = read_stats(overflow);
     if (!overflow) break;
-    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
+    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
     if (attempt > 0 && stats.tile_entries <= entryCapacity_) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
     grow_entries(stats.tile_entries);
   }
@@ -551,12 +552,15 @@ void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, 
 
 vsplat_step_stats Trainer::synchronize() {
   bool overflow = false;
-  const vsplat_step_stats stats = read_stats(overflow);
+  uint32_t skipped = 0;
+  const vsplat_step_stats stats = read_stats(overflow, &skipped);
   drain_timing();
   if (overflow) {
-    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 1, 0u);
+    // Adam's step count (and so its bias corrections) covers only the steps that ran.
+    adamStep_ -= skipped < adamStep_ ? skipped : adamStep_;
+    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
     check(hipStreamSynchronize(stream_), "clear overflow");
-    throw Error(VSPLAT_RUNTIME_ERROR, "a queued view needed more tile entries than were reserved; its step was skipped");
+    throw Error(VSPLAT_RUNTIME_ERROR, "a queued view needed more tile entries than were reserved; it and the views queued after it were skipped");
   }
   return stats;
 }

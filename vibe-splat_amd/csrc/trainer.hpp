@@ -88,6 +88,7 @@ public:
   void parameters(uint32_t group, void **pointer, uint64_t *bytes) const;
   std::vector<std::pair<std::string, double>> timings() const;
   void reset_timings();
+  void set_queued_timing(bool on) { timeQueued_ = on; }
   void inspect(const std::string &buffer, void *out, uint64_t capacity, uint64_t *bytes) const;
 
   uint32_t count() const { return count_; }
@@ -112,7 +113,7 @@ private:
   void cancel_timing();
   void drain_timing();
   void wait_prefetch();
-  vsplat_step_stats read_stats(bool &overflow);
+  vsplat_step_stats read_stats(bool &overflow, uint32_t *skipped = nullptr);
 
   vsplat_trainer_options options_;
   uint32_t count_ = 0;

@@ -34,6 +34,7 @@ EXPORTS = [
     "vsplat_uses_fixed_point", "vsplat_gpu_bytes", "vsplat_bytes_per_gaussian", "vsplat_staging_bytes",
     "vsplat_timings", "vsplat_last_error", "vsplat_enqueue", "vsplat_synchronize", "vsplat_stream",
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
+    "vsplat_set_stage_timing",
 ]
 
 
@@ -133,6 +134,7 @@ def lib():
         L.vsplat_inspect.argtypes = [vp, C.c_char_p, vp, C.c_uint64, vp]
         L.vsplat_math_probe.argtypes = [C.c_int32, vp, vp, C.c_uint64]
         L.vsplat_reset_timings.argtypes = [vp]
+        L.vsplat_set_stage_timing.argtypes = [vp, C.c_int32]
         _LIB = L
     return _LIB
 
@@ -384,6 +386,9 @@ class Trainer:
     def reset_timings(self):
         _raise(self._lib.vsplat_reset_timings(self._h), "reset_timings")
 
+    def set_stage_timing(self, enabled: bool):
+        _raise(self._lib.vsplat_set_stage_timing(self._h, int(bool(enabled))), "set_stage_timing")
+
     # --- device-resident path
     def enqueue(self, camera: Camera, device_target_ptr: int, mode: int, rates: LearningRates | None = None):
         cam = camera_to_c(camera)
@@ -412,6 +417,54 @@ class Trainer:
         out = np.zeros(size.value // np.dtype(dtype).itemsize, dtype)
         _raise(self._lib.vsplat_inspect(self._h, name.encode(), _p(out), out.nbytes, C.byref(size)), "inspect")
         return out.reshape(-1, width) if width > 1 else out
+
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        lib()  # the HIP runtime comes in as the library's dependency
+        H = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        H.hipMalloc.argtypes = [C.c_void_p, C.c_size_t]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipFree.argtypes = [C.c_void_p]
+        _HIP = H
+    return _HIP
+
+
+class DeviceBuffer:
+    """GPU memory holding a copy of a host array, for the device-resident calls
+    (Trainer.enqueue takes its .ptr as the target)."""
+
+    def __init__(self, array: np.ndarray):
+        a = np.ascontiguousarray(array)
+        hip = _hip()
+        ptr = C.c_void_p()
+        if hip.hipMalloc(C.byref(ptr), max(a.nbytes, 1)) != 0:
+            raise DeviceError("hipMalloc failed")
+        self.ptr, self.nbytes = ptr.value, a.nbytes
+        if hip.hipMemcpy(ptr, C.c_void_p(a.ctypes.data), a.nbytes, 1) != 0:  # host to device
+            self.free()
+            raise DeviceError("hipMemcpy failed")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            _hip().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001 (interpreter shutdown)
+            pass
 
 
 def math_probe(function: int, x: np.ndarray) -> np.ndarray:
