@@ -1,70 +1,22 @@
-// Tile binning: how many 16x16 tiles each depth-sorted splat reaches, the
-// (tile, splat) entries in depth order, and each tile's run of entries after
-// the stable sort by tile (train_bin_count.comp, train_bin_emit.comp,
-// train_tile_ranges.comp).
+// Tile lists: the (tile, splat) entries in depth order at the offsets the scan
+// of the tile counts gave (train_bin_emit.comp:18-37), and each tile's run of
+// entries after the stable sort by tile (train_tile_ranges.comp:11-22). The
+// counts themselves come from the projection (project.hip).
 #include "device.hpp"
 #include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace vsplat {
 
 namespace {
 
-// Inclusive tile box of a splat on a gridW x gridH grid (train_types.glsl:34-39).
-__device__ __forceinline__ void splat_tiles(const Splat &s, uint32_t gridW, uint32_t gridH, uint32_t t[4]) {
-  const float lastX = static_cast<float>(gridW) - 1.0f, lastY = static_cast<float>(gridH) - 1.0f;
-  t[0] = static_cast<uint32_t>(clampf(floorf((s.cx - s.ex - 1.0f) * 0.0625f), 0.0f, lastX));
-  t[1] = static_cast<uint32_t>(clampf(floorf((s.cy - s.ey - 1.0f) * 0.0625f), 0.0f, lastY));
-  t[2] = static_cast<uint32_t>(clampf(floorf((s.cx + s.ex + 1.0f) * 0.0625f), 0.0f, lastX));
-  t[3] = static_cast<uint32_t>(clampf(floorf((s.cy + s.ey + 1.0f) * 0.0625f), 0.0f, lastY));
-}
-
-// The least of q = qx x^2 + 2 qy x y + qz y^2 along one edge of the tile's
-// square of pixel centres, at the clamped stationary point.
-__device__ __forceinline__ float edge_min_vertical(const Splat &s, float iqz, float x, float ylo, float yhi) {
-  const float y = clampf((-s.qy * x) * iqz, ylo, yhi);
-  return fma32(s.qz * y, y, fma32(2.0f * s.qy * x, y, s.qx * x * x));
-}
-__device__ __forceinline__ float edge_min_horizontal(const Splat &s, float iqx, float v, float xlo, float xhi) {
-  const float u = clampf((-s.qy * v) * iqx, xlo, xhi);
-  return fma32(s.qz * v, v, fma32(2.0f * s.qy * u, v, s.qx * u * u));
-}
-
-// Whether the splat's 1/255 ellipse reaches a pixel centre of tile (tx, ty)
-// (train_types.glsl:56-77), with the same slack so rounding never drops a pixel.
-// iqx and iqz are 1/qx and 1/qz, computed once per splat.
-__device__ __forceinline__ bool reaches_tile(const Splat &s, float reach, float iqx, float iqz, uint32_t tx, uint32_t ty) {
-  const float lowx = (static_cast<float>(tx) * 16.0f + 0.5f) - s.cx, lowy = (static_cast<float>(ty) * 16.0f + 0.5f) - s.cy;
-  const float highx = lowx + 15.0f, highy = lowy + 15.0f;
-  if (lowx <= 0.0f && lowy <= 0.0f && highx >= 0.0f && highy >= 0.0f) return true;
-  float least = 1e30f;
-  least = minf(least, edge_min_vertical(s, iqz, lowx, lowy, highy));
-  least = minf(least, edge_min_horizontal(s, iqx, lowy, lowx, highx));
-  least = minf(least, edge_min_vertical(s, iqz, highx, lowy, highy));
-  least = minf(least, edge_min_horizontal(s, iqx, highy, lowx, highx));
-  return least <= fma32(reach, 1.001f, 1e-3f);
-}
-
-__global__ __launch_bounds__(256) void bin_count_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
-                                                        const Splat *__restrict__ splats, uint32_t count, uint32_t gridW,
-                                                        uint32_t gridH, uint32_t *__restrict__ coverage, uint32_t *counters) {
+// The tile count of each depth-sorted splat, gathered from Gaussian order (the
+// projection counted them), ready for the exclusive scan that gives the offsets.
+__global__ __launch_bounds__(256) void gather_counts_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
+                                                            const uint32_t *__restrict__ tileCount, uint32_t count,
+                                                            uint32_t *__restrict__ coverage) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  bool visible = false;
-  if (r < count) {
-    uint32_t n = 0;
-    if (keys[r] != kInvisible) {
-      visible = true;
-      const Splat s = load_splat(splats, values[r]);
-      const float reach = reach_of(s.o), iqx = 1.0f / s.qx, iqz = 1.0f / s.qz;
-      uint32_t t[4];
-      splat_tiles(s, gridW, gridH, t);
-      for (uint32_t y = t[1]; y <= t[3]; ++y)
-        for (uint32_t x = t[0]; x <= t[2]; ++x) n += reaches_tile(s, reach, iqx, iqz, x, y) ? 1u : 0u;
-    }
-    coverage[r] = n;
-  }
-  // One atomic per wave for the visible count.
-  const uint64_t mask = __ballot(visible);
-  if ((threadIdx.x & 63u) == 0 && mask != 0) atomicAdd(&counters[kVisible], static_cast<uint32_t>(__popcll(mask)));
+  if (r < count) coverage[r] = keys[r] == kInvisible ? 0u : tileCount[values[r]];
 }
 
 // Whether the entries fit. A view that does not fit sets the sticky overflow
@@ -81,25 +33,70 @@ __global__ void bin_check_kernel(uint32_t *counters, uint32_t capacity, uint32_t
   }
 }
 
+// A wave takes 64 consecutive depth-sorted splats, whose entries fill one
+// contiguous run of the list starting at the first one's offset. It walks their
+// candidate tiles 64 at a time (tiles.hpp); the reached ones of a round are
+// numbered by a ballot prefix count, so each round's stores are contiguous.
+// Small splats read the outcome from the bits the projection kept; large ones
+// repeat the test on the stored splat.
 __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
-                                                       const Splat *__restrict__ splats, uint32_t count, uint32_t gridW,
-                                                       uint32_t gridH, const uint32_t *__restrict__ offsets, const uint32_t *counters,
-                                                       uint32_t *__restrict__ entryKeys, uint32_t *__restrict__ entryValues) {
+                                                       const Splat *__restrict__ splats, const TileRecord *__restrict__ records,
+                                                       uint32_t count, uint32_t gridW, const uint32_t *__restrict__ offsets,
+                                                       const uint32_t *counters, uint32_t *__restrict__ entryKeys,
+                                                       uint32_t *__restrict__ entryValues) {
+  __shared__ TileTest tests[256];
+  __shared__ uint32_t boxes[256][3]; // tx0, ty0, width
+  __shared__ uint64_t masks[256];
+  __shared__ uint32_t gaussian[256];
+  __shared__ uint32_t ends[256];
+  if (counters[kOverflow] != 0) return; // the same for every thread of the launch
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= count || counters[kOverflow] != 0 || keys[r] == kInvisible) return;
-  const uint32_t splat = values[r];
-  const Splat s = load_splat(splats, splat);
-  const float reach = reach_of(s.o), iqx = 1.0f / s.qx, iqz = 1.0f / s.qz;
-  uint32_t t[4];
-  splat_tiles(s, gridW, gridH, t);
-  uint32_t slot = offsets[r];
-  for (uint32_t y = t[1]; y <= t[3]; ++y) {
-    for (uint32_t x = t[0]; x <= t[2]; ++x) {
-      if (!reaches_tile(s, reach, iqx, iqz, x, y)) continue;
-      entryKeys[slot] = y * gridW + x;
-      entryValues[slot] = splat;
-      ++slot;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x & ~63u;
+  uint32_t candidates = 0;
+  if (r < count && keys[r] != kInvisible) {
+    const uint32_t i = values[r];
+    const TileRecord rec = records[i];
+    const uint32_t width = rec.size & 0xFFFFu, height = rec.size >> 16;
+    candidates = width * height;
+    boxes[threadIdx.x][0] = rec.origin & 0xFFFFu;
+    boxes[threadIdx.x][1] = rec.origin >> 16;
+    boxes[threadIdx.x][2] = width;
+    masks[threadIdx.x] = static_cast<uint64_t>(rec.maskLo) | (static_cast<uint64_t>(rec.maskHi) << 32);
+    gaussian[threadIdx.x] = i;
+    if (candidates > kMaskTiles) {
+      const Splat s = load_splat(splats, i);
+      tests[threadIdx.x] = tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy);
     }
+  }
+  const uint32_t end = wave_inclusive_sum(candidates, lane);
+  ends[threadIdx.x] = end;
+  const uint32_t total = __shfl(end, 63, 64);
+  __syncthreads();
+  if (total == 0) return; // nothing in this wave; no barrier follows
+  uint32_t slot = offsets[r - lane]; // the wave's first splat is in range when it has candidates
+  const uint32_t limit = counters[kEntriesUsed];
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t j = base + lane;
+    bool hit = false;
+    uint32_t owner = wave, tx = 0, ty = 0;
+    if (j < total) {
+      owner = wave + owner_of(ends + wave, j);
+      const uint32_t local = j - (owner == wave ? 0u : ends[owner - 1]);
+      const uint32_t w = boxes[owner][2];
+      const uint32_t row = local / w;
+      tx = boxes[owner][0] + (local - row * w);
+      ty = boxes[owner][1] + row;
+      const uint32_t size = owner == wave ? ends[owner] : ends[owner] - ends[owner - 1];
+      hit = size <= kMaskTiles ? ((masks[owner] >> local) & 1ull) != 0 : reaches_tile(tests[owner], tx, ty);
+    }
+    const uint64_t round = __ballot(hit);
+    const uint32_t at = slot + static_cast<uint32_t>(__popcll(round & ((1ull << lane) - 1ull)));
+    // The hits are the ones the projection counted, so at < limit; the test only guards the buffer.
+    if (hit && at < limit) {
+      entryKeys[at] = ty * gridW + tx;
+      entryValues[at] = gaussian[owner];
+    }
+    slot += static_cast<uint32_t>(__popcll(round));
   }
 }
 
@@ -116,22 +113,21 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *__rest
 
 } // namespace
 
-void launch_bin_count(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
-                      uint32_t gridW, uint32_t gridH, uint32_t *coverage, uint32_t *counters) {
+void launch_gather_counts(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
+                          uint32_t *coverage) {
   if (count == 0) return;
-  hipLaunchKernelGGL(bin_count_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, splats, count, gridW, gridH,
-                     coverage, counters);
+  hipLaunchKernelGGL(gather_counts_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, tileCount, count, coverage);
 }
 
 void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity, bool isStep) {
   hipLaunchKernelGGL(bin_check_kernel, dim3(1), dim3(1), 0, stream, counters, capacity, isStep ? 1u : 0u);
 }
 
-void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
-                     uint32_t gridW, uint32_t gridH, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
+void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
+                     uint32_t count, uint32_t gridW, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
                      uint32_t *entryValues) {
   if (count == 0) return;
-  hipLaunchKernelGGL(bin_emit_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, splats, count, gridW, gridH,
+  hipLaunchKernelGGL(bin_emit_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, splats, records, count, gridW,
                      offsets, counters, entryKeys, entryValues);
 }
 

@@ -43,9 +43,14 @@ struct ChainBuffers {
   float *gradPos, *gradScale, *gradRot, *gradOpacity, *gradSh;
 };
 
+struct TileRecord;
+
+// Projection and tile coverage: splats, depth keys and values (the identity),
+// and per Gaussian the tile count and TileRecord; adds the visible count.
 void launch_project(hipStream_t stream, const float *positions, const float *scales, const float *rotations, const float *opacities,
                     const uint32_t *shWords, const float *filter3d, const ViewParams &view, bool mipFilter2d, float shDropout,
-                    uint32_t shRetained, uint32_t shSeed, Splat *splats, uint32_t *keys, uint32_t *values);
+                    uint32_t shRetained, uint32_t shSeed, uint32_t gridW, uint32_t gridH, Splat *splats, uint32_t *keys,
+                    uint32_t *values, uint32_t *tileCount, TileRecord *records, uint32_t *counters);
 
 size_t scan_scratch_words(uint64_t count);
 void launch_exclusive_scan(hipStream_t stream, uint32_t *data, uint32_t count, uint32_t *scratch, uint32_t *total);
@@ -55,11 +60,11 @@ void launch_radix_sort(hipStream_t stream, uint32_t *keys, uint32_t *values, uin
 void launch_copy_u32(hipStream_t stream, const uint32_t *src, uint32_t *dst, const uint32_t *countDev, uint32_t capacity);
 void launch_fill_u32(hipStream_t stream, uint32_t *dst, uint64_t count, uint32_t value);
 
-void launch_bin_count(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
-                      uint32_t gridW, uint32_t gridH, uint32_t *coverage, uint32_t *counters);
+void launch_gather_counts(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
+                          uint32_t *coverage);
 void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity, bool isStep);
-void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, uint32_t count,
-                     uint32_t gridW, uint32_t gridH, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
+void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
+                     uint32_t count, uint32_t gridW, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
                      uint32_t *entryValues);
 void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges);
 

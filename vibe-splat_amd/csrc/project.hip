@@ -1,9 +1,14 @@
 // Projection: raw parameters to screen-space splats and depth keys
-// (train_preprocess.comp:31-134). One thread per Gaussian; the parameter
-// arrays are read as the GaussianCloud layout keeps them (xyz, xyz, wxyz, o),
-// contiguous across a wave.
+// (train_preprocess.comp:31-134), and each splat's tile coverage
+// (train_bin_count.comp:18-37), in one pass. One thread per Gaussian; the
+// parameter arrays are read as the GaussianCloud layout keeps them (xyz, xyz,
+// wxyz, o), contiguous across a wave. The coverage is counted here, in
+// Gaussian order and while the splat is still in registers, by the wave's
+// shared walk over candidate tiles (tiles.hpp); the bits it finds are kept for
+// the list pass.
 #include "device.hpp"
 #include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace vsplat {
 
@@ -13,26 +18,22 @@ struct ProjectOptions {
   uint32_t shRetained, shSeed;
 };
 
+// One Gaussian: false if it is culled, else its splat and depth key.
 template <int Degree>
-__global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ positions, const float *__restrict__ scales,
-                                                      const float *__restrict__ rotations, const float *__restrict__ opacities,
-                                                      const uint32_t *__restrict__ shWords, const float *__restrict__ filter3d,
-                                                      ViewParams c, ProjectOptions opt, Splat *__restrict__ splats,
-                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ values) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.count) return;
-  keys[i] = kInvisible;
-  values[i] = i;
+__device__ __forceinline__ bool project_one(uint32_t i, const float *__restrict__ positions, const float *__restrict__ scales,
+                                            const float *__restrict__ rotations, const float *__restrict__ opacities,
+                                            const uint32_t *__restrict__ shWords, const float *__restrict__ filter3d, const ViewParams &c,
+                                            const ProjectOptions &opt, Splat &out, uint32_t &key) {
   const float px = positions[i * 3], py = positions[i * 3 + 1], pz = positions[i * 3 + 2];
   float tx, ty, tz;
   to_camera(c, px, py, pz, tx, ty, tz);
   // Written so that NaN fails every test, as the reference does.
-  if (!(tz >= c.nearPlane)) return;
+  if (!(tz >= c.nearPlane)) return false;
   const float opacity = 1.0f / (1.0f + vs_exp(-opacities[i]));
-  if (!(opacity >= kMinAlpha)) return;
+  if (!(opacity >= kMinAlpha)) return false;
   float qw = rotations[i * 4], qx = rotations[i * 4 + 1], qy = rotations[i * 4 + 2], qz = rotations[i * 4 + 3];
   const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
-  if (!(qlen >= 1e-4f)) return;
+  if (!(qlen >= 1e-4f)) return false;
   const float iq = 1.0f / qlen;
   qw = qw * iq;
   qx = qx * iq;
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   S[1][1] = S[1][1] + f;
   S[2][2] = S[2][2] + f;
   const float opacity3 = opacity * filter_factor(sx * sx, sy * sy, sz * sz, f);
-  if (!(opacity3 >= kMinAlpha)) return;
+  if (!(opacity3 >= kMinAlpha)) return false;
 
   // EWA: J, clamped to 1.3x the frustum, times the view rotation.
   const float iz = 1.0f / tz;
@@ -81,12 +82,12 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   const float dilation = opt.mipFilter2d ? 0.1f : 0.3f;
   const float a = a0 + dilation, cc = c0 + dilation;
   const float det = fma32(a, cc, -(b * b));
-  if (!(det > 1e-6f)) return;
+  if (!(det > 1e-6f)) return false;
   float opacityUsed = opacity3;
   if (opt.mipFilter2d) {
     const float rawDet = fma32(a0, c0, -(b * b));
     opacityUsed = rawDet > 0.0f ? opacity3 * sqrtf(rawDet / det) : 0.0f;
-    if (!(opacityUsed >= kMinAlpha)) return;
+    if (!(opacityUsed >= kMinAlpha)) return false;
   }
   const float mid = 0.5f * (a + cc);
   const float disc = fma32(mid, mid, -det);
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   const float centerX = fma32(c.fx, tx * iz, c.cx), centerY = fma32(c.fy, ty * iz, c.cy);
   if (centerX + radius < 0.0f || centerY + radius < 0.0f || centerX - radius > static_cast<float>(c.width) ||
       centerY - radius > static_cast<float>(c.height)) {
-    return;
+    return false;
   }
   // Colour from SH along the view direction; the active degree is a template
   // argument so that the coefficient loop unrolls. Dropped coefficients are
@@ -116,30 +117,113 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
       col[ch] = k < used ? fma32(basis[k], v, col[ch]) : col[ch];
     }
   }
-  if (!(fabsf(col[0]) < 1e30f && fabsf(col[1]) < 1e30f && fabsf(col[2]) < 1e30f)) return;
+  if (!(fabsf(col[0]) < 1e30f && fabsf(col[1]) < 1e30f && fabsf(col[2]) < 1e30f)) return false;
   // The extent of the 1/255 ellipse, which is what binning and the pixel loops use.
   const float reach = reach_of(opacityUsed);
   const float ex = sqrtf(reach * a), ey = sqrtf(reach * cc);
-  float4 *out = reinterpret_cast<float4 *>(splats + i);
   const float id = 1.0f / det;
-  out[0] = make_float4(cc * id, -b * id, a * id, opacityUsed);
-  out[1] = make_float4(maxf(col[0], 0.0f), maxf(col[1], 0.0f), maxf(col[2], 0.0f), radius);
-  out[2] = make_float4(centerX, centerY, minf(ex, radius), minf(ey, radius));
-  keys[i] = 0xFFFFFFFFu - as_uint(tz);
+  out = Splat{cc * id, -b * id, a * id, opacityUsed, maxf(col[0], 0.0f), maxf(col[1], 0.0f), maxf(col[2], 0.0f), radius,
+              centerX, centerY, minf(ex, radius), minf(ey, radius)};
+  key = 0xFFFFFFFFu - as_uint(tz);
+  return true;
+}
+
+template <int Degree>
+__global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ positions, const float *__restrict__ scales,
+                                                      const float *__restrict__ rotations, const float *__restrict__ opacities,
+                                                      const uint32_t *__restrict__ shWords, const float *__restrict__ filter3d,
+                                                      ViewParams c, ProjectOptions opt, uint32_t gridW, uint32_t gridH,
+                                                      Splat *__restrict__ splats, uint32_t *__restrict__ keys,
+                                                      uint32_t *__restrict__ values, uint32_t *__restrict__ tileCount,
+                                                      TileRecord *__restrict__ records, uint32_t *counters) {
+  __shared__ TileTest tests[256];
+  __shared__ uint32_t boxes[256][3]; // tx0, ty0, width
+  __shared__ uint32_t ends[256];     // running candidate totals, per wave
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x & ~63u;
+  Splat s{};
+  uint32_t key = kInvisible;
+  const bool visible = i < c.count && project_one<Degree>(i, positions, scales, rotations, opacities, shWords, filter3d, c, opt, s, key);
+  if (i < c.count) {
+    keys[i] = key;
+    values[i] = i;
+    if (visible) {
+      float4 *out = reinterpret_cast<float4 *>(splats + i);
+      out[0] = make_float4(s.qx, s.qy, s.qz, s.o);
+      out[1] = make_float4(s.r, s.g, s.b, s.radius);
+      out[2] = make_float4(s.cx, s.cy, s.ex, s.ey);
+    }
+  }
+  // One atomic per wave for the visible count.
+  const uint64_t seen = __ballot(visible);
+  if (lane == 0 && seen != 0) atomicAdd(&counters[kVisible], static_cast<uint32_t>(__popcll(seen)));
+
+  // Candidate tiles: the bounding box of each visible splat.
+  uint32_t t[4] = {0, 0, 0, 0};
+  uint32_t candidates = 0;
+  if (visible) {
+    splat_tiles(s.cx, s.cy, s.ex, s.ey, gridW, gridH, t);
+    candidates = (t[2] - t[0] + 1) * (t[3] - t[1] + 1);
+    tests[threadIdx.x] = tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy);
+  }
+  const uint32_t width = t[2] - t[0] + 1;
+  boxes[threadIdx.x][0] = t[0];
+  boxes[threadIdx.x][1] = t[1];
+  boxes[threadIdx.x][2] = width;
+  const uint32_t end = wave_inclusive_sum(candidates, lane);
+  ends[threadIdx.x] = end;
+  const uint32_t start = end - candidates;
+  const uint32_t total = __shfl(end, 63, 64);
+  __syncthreads();
+
+  uint32_t reached = 0;
+  uint64_t bits = 0;
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t j = base + lane;
+    bool hit = false;
+    if (j < total) {
+      const uint32_t owner = wave + owner_of(ends + wave, j);
+      const uint32_t first = owner == wave ? 0u : ends[owner - 1];
+      const uint32_t local = j - first;
+      const uint32_t w = boxes[owner][2];
+      const uint32_t row = local / w;
+      hit = reaches_tile(tests[owner], boxes[owner][0] + (local - row * w), boxes[owner][1] + row);
+    }
+    const uint64_t round = __ballot(hit);
+    // This lane's own candidates within the round: lanes start - base .. start + candidates - base.
+    if (candidates != 0 && start < base + 64 && start + candidates > base) {
+      const uint32_t lo = start > base ? start - base : 0u;
+      const uint32_t hi = start + candidates - base < 64u ? start + candidates - base : 64u;
+      const uint64_t mine = round & bit_range(lo, hi);
+      reached += static_cast<uint32_t>(__popcll(mine));
+      if (candidates <= kMaskTiles) bits |= (mine >> lo) << (base + lo - start);
+    }
+  }
+  if (i < c.count) {
+    tileCount[i] = reached;
+    const uint32_t height = t[3] - t[1] + 1;
+    records[i] = TileRecord{static_cast<uint32_t>(bits), static_cast<uint32_t>(bits >> 32), t[0] | (t[1] << 16),
+                            visible ? (width | (height << 16)) : 0u};
+  }
 }
 
 void launch_project(hipStream_t stream, const float *positions, const float *scales, const float *rotations, const float *opacities,
                     const uint32_t *shWords, const float *filter3d, const ViewParams &view, bool mipFilter2d, float shDropout,
-                    uint32_t shRetained, uint32_t shSeed, Splat *splats, uint32_t *keys, uint32_t *values) {
+                    uint32_t shRetained, uint32_t shSeed, uint32_t gridW, uint32_t gridH, Splat *splats, uint32_t *keys,
+                    uint32_t *values, uint32_t *tileCount, TileRecord *records, uint32_t *counters) {
   if (view.count == 0) return;
   const ProjectOptions opt{mipFilter2d ? 1u : 0u, shDropout, shRetained, shSeed};
   const dim3 grid((view.count + 255) / 256), block(256);
+#define VSPLAT_PROJECT(D)                                                                                                          \
+  hipLaunchKernelGGL(project_kernel<D>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view,  \
+                     opt, gridW, gridH, splats, keys, values, tileCount, records, counters)
   switch (view.shDegree) {
-  case 0: hipLaunchKernelGGL(project_kernel<0>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
-  case 1: hipLaunchKernelGGL(project_kernel<1>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
-  case 2: hipLaunchKernelGGL(project_kernel<2>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
-  default: hipLaunchKernelGGL(project_kernel<3>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view, opt, splats, keys, values); break;
+  case 0: VSPLAT_PROJECT(0); break;
+  case 1: VSPLAT_PROJECT(1); break;
+  case 2: VSPLAT_PROJECT(2); break;
+  default: VSPLAT_PROJECT(3); break;
   }
+#undef VSPLAT_PROJECT
 }
 
 } // namespace vsplat

@@ -11,6 +11,7 @@
 // within the memory budget, and runs the step again. In steady state that is
 // one synchronisation per step and no host work between kernels.
 #include "trainer.hpp"
+#include "tiles.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -134,6 +135,8 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   tmpKeys_.ensure(n * 4);
   tmpValues_.ensure(n * 4);
   coverage_.ensure(n * 4);
+  tileCount_.ensure(n * 4);
+  tileRecords_.ensure(n * sizeof(TileRecord));
   if (fixedPoint_) {
     rawFixed_.ensure(n * kRawPerSplat * 8);
     launch_fill_u32(stream_, rawFixed_.as<uint32_t>(), n * kRawPerSplat * 2, 0u);
@@ -198,6 +201,8 @@ void Trainer::ensure_size(int width, int height) {
   if (width == width_ && height == height_) return;
   const uint64_t gridW = div_up(static_cast<uint64_t>(width), kTileSize), gridH = div_up(static_cast<uint64_t>(height), kTileSize);
   if (gridW * gridH > (1u << 24)) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too large for the tile index");
+  // Tile coordinates are kept as 16-bit pairs per splat.
+  if (gridW > 0xFFFFu || gridH > 0xFFFFu) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too wide or too tall");
   width_ = width;
   height_ = height;
   gridW_ = static_cast<uint32_t>(gridW);
@@ -310,20 +315,20 @@ void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, u
   launch_fill_u32(s, counters, 3, 0u); // not the sticky overflow word
   launch_project(s, positions_.as<float>(), scales_.as<float>(), rotations_.as<float>(), opacities_.as<float>(),
                  shWordsBuf_.as<uint32_t>(), filterSet_ ? filter3d_.as<float>() : nullptr, v, options_.mip_filter_2d != 0, dropoutNow_,
-                 shRetained_, shSeed_, splats_.as<Splat>(), keys_.as<uint32_t>(), values_.as<uint32_t>());
+                 shRetained_, shSeed_, gridW_, gridH_, splats_.as<Splat>(), keys_.as<uint32_t>(), values_.as<uint32_t>(),
+                 tileCount_.as<uint32_t>(), tileRecords_.as<TileRecord>(), counters);
   mark("project");
   launch_radix_sort(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tmpKeys_.as<uint32_t>(), tmpValues_.as<uint32_t>(), nullptr, count_,
                     32, sortScratch_.as<uint32_t>());
   mark("depth sort");
-  launch_bin_count(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), count_, gridW_, gridH_, coverage_.as<uint32_t>(),
-                   counters);
+  launch_gather_counts(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tileCount_.as<uint32_t>(), count_, coverage_.as<uint32_t>());
   launch_exclusive_scan(s, coverage_.as<uint32_t>(), count_, scanScratch_.as<uint32_t>(), counters + kEntries);
   launch_bin_check(s, counters, static_cast<uint32_t>(entryCapacity_), mode == kModeStep);
   mark("count tiles");
   launch_fill_u32(s, ranges_.as<uint32_t>(), static_cast<uint64_t>(tiles_) * 2, 0u);
   if (entryCapacity_ > 0) {
-    launch_bin_emit(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), count_, gridW_, gridH_, coverage_.as<uint32_t>(),
-                    counters, entryKeys_.as<uint32_t>(), entryValues_.as<uint32_t>());
+    launch_bin_emit(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), tileRecords_.as<TileRecord>(), count_, gridW_,
+                    coverage_.as<uint32_t>(), counters, entryKeys_.as<uint32_t>(), entryValues_.as<uint32_t>());
     launch_radix_sort(s, entryKeys_.as<uint32_t>(), entryValues_.as<uint32_t>(), tmpEntryKeys_.as<uint32_t>(), tmpEntryValues_.as<uint32_t>(),
                       counters + kEntriesUsed, static_cast<uint32_t>(entryCapacity_), tileBits_, sortScratch_.as<uint32_t>());
     launch_tile_ranges(s, entryKeys_.as<uint32_t>(), counters, static_cast<uint32_t>(entryCapacity_), ranges_.as<uint2>());
@@ -631,9 +636,10 @@ uint64_t Trainer::bytes_per_gaussian() const {
   const uint64_t floats = 3 + 3 + 4 + 1 + shFloats_;
   // Values (SH as packed words) and one optimiser word per parameter.
   const uint64_t parameters = (floats - shFloats_ + shWords_) * 4 + floats * 4;
-  // Splat, sort keys and values with copies, coverage, filter, sums, SH-pass data, statistics.
+  // Splat, sort keys and values with copies, coverage, tile count, filter, tile
+  // record, sums, SH-pass data, statistics.
   const uint64_t raw = fixedPoint_ ? kRawPerSplat * 8 : kRawPerSplat * 4;
-  return parameters + sizeof(Splat) + 4 * 6 + raw + kShPassFloats * 4 + 32;
+  return parameters + sizeof(Splat) + 4 * 7 + sizeof(TileRecord) + raw + kShPassFloats * 4 + 32;
 }
 
 uint64_t Trainer::gpu_bytes() const {

@@ -135,7 +135,7 @@ private:
   DeviceArray positions_, scales_, rotations_, opacities_, shWordsBuf_;
   DeviceArray state_[5], grad_[5];
   // Per Gaussian.
-  DeviceArray splats_, keys_, values_, tmpKeys_, tmpValues_, coverage_, rawFixed_, rawFloat_, shPass_, stats_, filter3d_;
+  DeviceArray splats_, keys_, values_, tmpKeys_, tmpValues_, coverage_, tileCount_, tileRecords_, rawFixed_, rawFloat_, shPass_, stats_, filter3d_;
   bool filterSet_ = false;
   // Per tile entry.
   DeviceArray entryKeys_, entryValues_, tmpEntryKeys_, tmpEntryValues_;
