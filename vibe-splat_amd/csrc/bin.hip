@@ -4,32 +4,89 @@
 // counts themselves come from the projection (project.hip).
 #include "device.hpp"
 #include "kernels.hpp"
+#include "lookback.hpp"
 #include "tiles.hpp"
 
 namespace vsplat {
 
 namespace {
 
-// The tile count of each depth-sorted splat, gathered from Gaussian order (the
-// projection counted them), ready for the exclusive scan that gives the offsets.
-__global__ __launch_bounds__(256) void gather_counts_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
-                                                            const uint32_t *__restrict__ tileCount, uint32_t count,
-                                                            uint32_t *__restrict__ coverage) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < count) coverage[r] = keys[r] == kInvisible ? 0u : tileCount[values[r]];
-}
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanPerThread = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanPerThread; // 4096 splats per workgroup
 
-// Whether the entries fit. A view that does not fit sets the sticky overflow
-// word; from then on every queued pass lists nothing and its chain rule does
-// nothing, so the passes that ran are exactly those before the first overflow.
-__global__ void bin_check_kernel(uint32_t *counters, uint32_t capacity, uint32_t isStep) {
-  const uint32_t entries = counters[kEntries];
-  if (counters[kOverflow] != 0 || entries > capacity) {
-    counters[kEntriesUsed] = 0;
-    counters[kOverflow] = 1;
-    counters[kSkipped] += isStep;
-  } else {
-    counters[kEntriesUsed] = entries;
+// The list offsets: the exclusive sum, in depth order, of the tile counts the
+// projection left in Gaussian order (train_bin_count.comp's counts, scanned as
+// scan_blocks.comp does), in one launch by decoupled look-back
+// (lookback.hpp). The last tile also stores the total and decides whether the
+// entries fit: the later kernels only list and sort what fits. A view that does
+// not fit sets the sticky overflow word; from then on every queued pass lists
+// nothing and its chain rule does nothing, so the passes that ran are exactly
+// those before the first overflow.
+//
+// control: [0] the ticket, [1] the launch number, which tags this launch's
+// status words (one 64-bit word per tile, after the first two 64-bit words),
+// so they never need clearing: each launch rewrites every tile's word, and a
+// word left by the previous launch carries the previous number.
+__global__ __launch_bounds__(kScanThreads) void coverage_scan_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
+                                                                     const uint32_t *__restrict__ tileCount, uint32_t count,
+                                                                     uint32_t *__restrict__ offsets, uint32_t *counters, uint32_t capacity,
+                                                                     uint32_t isStep, uint32_t *control) {
+  __shared__ uint32_t waveTotals[kScanThreads / 64];
+  __shared__ uint32_t ticketShared, launchShared, earlierShared;
+  uint64_t *status = reinterpret_cast<uint64_t *>(control) + 2;
+  if (threadIdx.x == 0) {
+    launchShared = __hip_atomic_load(&control[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // The launch number is read before the ticket is taken (release), so the
+    // last ticket can advance it for the next launch.
+    ticketShared = __hip_atomic_fetch_add(&control[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const uint32_t tile = ticketShared;
+  const uint32_t tag = (launchShared & 0x3FFFFFFFu) << 2;
+  const uint32_t first = tile * kScanTile + threadIdx.x * kScanPerThread;
+  uint32_t v[kScanPerThread];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPerThread; ++k) {
+    const uint32_t r = first + k;
+    v[k] = r < count && keys[r] != kInvisible ? tileCount[values[r]] : 0u;
+    sum += v[k];
+  }
+  uint32_t exclusive;
+  const uint32_t tileTotal = block_exclusive_sum(sum, exclusive, waveTotals);
+  if (threadIdx.x < 64) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t earlier = 0;
+    if (tile == 0) {
+      if (lane == 0) status_publish(status, tag | kTagInclusive, tileTotal);
+    } else {
+      if (lane == 0) status_publish(status + tile, tag | kTagAggregate, tileTotal);
+      earlier = look_back_window(status, tile, tag, lane, &counters[kFault]);
+      if (lane == 0) status_publish(status + tile, tag | kTagInclusive, earlier + tileTotal);
+    }
+    if (lane == 0) earlierShared = earlier;
+  }
+  __syncthreads();
+  uint32_t running = earlierShared + exclusive;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPerThread; ++k) {
+    if (first + k < count) offsets[first + k] = running;
+    running += v[k];
+  }
+  if (tile == gridDim.x - 1 && threadIdx.x == 0) {
+    const uint32_t entries = earlierShared + tileTotal;
+    counters[kEntries] = entries;
+    if (counters[kOverflow] != 0 || entries > capacity) {
+      counters[kEntriesUsed] = 0;
+      counters[kOverflow] = 1;
+      counters[kSkipped] += isStep;
+    } else {
+      counters[kEntriesUsed] = entries;
+    }
+    // Every workgroup has its ticket and its launch number: ready the next launch.
+    control[0] = 0;
+    __hip_atomic_store(&control[1], launchShared + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -44,57 +101,59 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
                                                        uint32_t count, uint32_t gridW, const uint32_t *__restrict__ offsets,
                                                        const uint32_t *counters, uint32_t *__restrict__ entryKeys,
                                                        uint32_t *__restrict__ entryValues) {
-  __shared__ TileTest tests[256];
-  __shared__ uint32_t boxes[256][3]; // tx0, ty0, width
-  __shared__ uint64_t masks[256];
-  __shared__ uint32_t gaussian[256];
-  __shared__ uint32_t ends[256];
+  __shared__ WalkOwner owners[256]; // per wave: its splats with candidates, packed
+  __shared__ uint32_t flags[256];    // per wave: the walk's round marks
   if (counters[kOverflow] != 0) return; // the same for every thread of the launch
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x & ~63u;
-  uint32_t candidates = 0;
+  uint32_t candidates = 0, i = 0;
+  TileRecord rec{};
   if (r < count && keys[r] != kInvisible) {
-    const uint32_t i = values[r];
-    const TileRecord rec = records[i];
-    const uint32_t width = rec.size & 0xFFFFu, height = rec.size >> 16;
-    candidates = width * height;
-    boxes[threadIdx.x][0] = rec.origin & 0xFFFFu;
-    boxes[threadIdx.x][1] = rec.origin >> 16;
-    boxes[threadIdx.x][2] = width;
-    masks[threadIdx.x] = static_cast<uint64_t>(rec.maskLo) | (static_cast<uint64_t>(rec.maskHi) << 32);
-    gaussian[threadIdx.x] = i;
+    i = values[r];
+    rec = records[i];
+    candidates = (rec.size & 0xFFFFu) * (rec.size >> 16);
+  }
+  const bool owner = candidates != 0;
+  const uint64_t owning = __ballot(owner);
+  const uint32_t end = wave_inclusive_sum(candidates, lane);
+  const uint32_t total = __shfl(end, 63, 64);
+  if (total == 0) return; // nothing in this wave; no barrier follows
+  const uint32_t start = end - candidates;
+  if (owner) {
+    const uint32_t width = rec.size & 0xFFFFu;
+    TileTest test{};
     if (candidates > kMaskTiles) {
       const Splat s = load_splat(splats, i);
-      tests[threadIdx.x] = tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy);
+      test = tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy);
     }
+    const uint32_t k = static_cast<uint32_t>(__popcll(owning & ((1ull << lane) - 1ull)));
+    owners[wave + k] = WalkOwner{test, rec.origin & 0xFFFFu, rec.origin >> 16, width, start, 1.0f / static_cast<float>(width), i,
+                                 rec.maskLo, rec.maskHi};
   }
-  const uint32_t end = wave_inclusive_sum(candidates, lane);
-  ends[threadIdx.x] = end;
-  const uint32_t total = __shfl(end, 63, 64);
-  __syncthreads();
-  if (total == 0) return; // nothing in this wave; no barrier follows
+  __builtin_amdgcn_wave_barrier();
   uint32_t slot = offsets[r - lane]; // the wave's first splat is in range when it has candidates
   const uint32_t limit = counters[kEntriesUsed];
+  uint32_t before = 0;
   for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t k = walk_round(flags + wave, lane, base, owner, start, before);
     const uint32_t j = base + lane;
     bool hit = false;
-    uint32_t owner = wave, tx = 0, ty = 0;
+    uint32_t tx = 0, ty = 0, gaussian = 0;
     if (j < total) {
-      owner = wave + owner_of(ends + wave, j);
-      const uint32_t local = j - (owner == wave ? 0u : ends[owner - 1]);
-      const uint32_t w = boxes[owner][2];
-      const uint32_t row = local / w;
-      tx = boxes[owner][0] + (local - row * w);
-      ty = boxes[owner][1] + row;
-      const uint32_t size = owner == wave ? ends[owner] : ends[owner] - ends[owner - 1];
-      hit = size <= kMaskTiles ? ((masks[owner] >> local) & 1ull) != 0 : reaches_tile(tests[owner], tx, ty);
+      const WalkOwner &o = owners[wave + k];
+      const uint32_t local = j - o.start;
+      box_tile(o, local, tx, ty);
+      const uint32_t size = (k + 1 < static_cast<uint32_t>(__popcll(owning)) ? owners[wave + k + 1].start : total) - o.start;
+      const uint64_t mask = static_cast<uint64_t>(o.maskLo) | (static_cast<uint64_t>(o.maskHi) << 32);
+      hit = size <= kMaskTiles ? ((mask >> local) & 1ull) != 0 : reaches_tile(o.test, tx, ty);
+      gaussian = o.gaussian;
     }
     const uint64_t round = __ballot(hit);
     const uint32_t at = slot + static_cast<uint32_t>(__popcll(round & ((1ull << lane) - 1ull)));
     // The hits are the ones the projection counted, so at < limit; the test only guards the buffer.
     if (hit && at < limit) {
       entryKeys[at] = ty * gridW + tx;
-      entryValues[at] = gaussian[owner];
+      entryValues[at] = gaussian;
     }
     slot += static_cast<uint32_t>(__popcll(round));
   }
@@ -113,14 +172,13 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *__rest
 
 } // namespace
 
-void launch_gather_counts(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
-                          uint32_t *coverage) {
-  if (count == 0) return;
-  hipLaunchKernelGGL(gather_counts_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, tileCount, count, coverage);
-}
+size_t coverage_scan_words(uint32_t count) { return 4 + 2 * static_cast<size_t>((count + kScanTile - 1) / kScanTile); }
 
-void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity, bool isStep) {
-  hipLaunchKernelGGL(bin_check_kernel, dim3(1), dim3(1), 0, stream, counters, capacity, isStep ? 1u : 0u);
+void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
+                          uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *control) {
+  const uint32_t tiles = count == 0 ? 1 : (count + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(coverage_scan_kernel, dim3(tiles), dim3(kScanThreads), 0, stream, keys, values, tileCount, count, offsets,
+                     counters, capacity, isStep ? 1u : 0u, control);
 }
 
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,

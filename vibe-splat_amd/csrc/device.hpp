@@ -52,7 +52,8 @@ enum Counter : uint32_t {
   kEntriesUsed = 2,  // entries actually listed: kEntries, or 0 if they did not fit
   kOverflow = 3,     // sticky: a view needed more entries than the buffers hold; every later pass is skipped until the host clears it
   kSkipped = 4,      // step-mode passes skipped that way since the host last cleared it
-  kCounterWords = 5,
+  kFault = 5,        // a kernel gave up a wait that should have ended (see lookback.hpp)
+  kCounterWords = 6,
 };
 
 __device__ __forceinline__ float fma32(float a, float b, float c) { return __builtin_fmaf(a, b, c); }

@@ -52,17 +52,18 @@ void launch_project(hipStream_t stream, const float *positions, const float *sca
                     uint32_t shRetained, uint32_t shSeed, uint32_t gridW, uint32_t gridH, Splat *splats, uint32_t *keys,
                     uint32_t *values, uint32_t *tileCount, TileRecord *records, uint32_t *counters);
 
-size_t scan_scratch_words(uint64_t count);
-void launch_exclusive_scan(hipStream_t stream, uint32_t *data, uint32_t count, uint32_t *scratch, uint32_t *total);
+// Scratch for launch_radix_sort over up to capacity keys.
 size_t radix_sort_scratch_words(uint64_t capacity);
 void launch_radix_sort(hipStream_t stream, uint32_t *keys, uint32_t *values, uint32_t *tmpKeys, uint32_t *tmpValues,
                        const uint32_t *countDev, uint32_t capacity, uint32_t bits, uint32_t *scratch);
 void launch_copy_u32(hipStream_t stream, const uint32_t *src, uint32_t *dst, const uint32_t *countDev, uint32_t capacity);
 void launch_fill_u32(hipStream_t stream, uint32_t *dst, uint64_t count, uint32_t value);
 
-void launch_gather_counts(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
-                          uint32_t *coverage);
-void launch_bin_check(hipStream_t stream, uint32_t *counters, uint32_t capacity, bool isStep);
+// Scratch (zeroed once when allocated) for launch_coverage_scan over count splats.
+size_t coverage_scan_words(uint32_t count);
+// Offsets of each depth-sorted splat's entries, the entry count and the capacity check.
+void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
+                          uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *control);
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
                      uint32_t count, uint32_t gridW, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
                      uint32_t *entryValues);

@@ -136,9 +136,8 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
                                                       Splat *__restrict__ splats, uint32_t *__restrict__ keys,
                                                       uint32_t *__restrict__ values, uint32_t *__restrict__ tileCount,
                                                       TileRecord *__restrict__ records, uint32_t *counters) {
-  __shared__ TileTest tests[256];
-  __shared__ uint32_t boxes[256][3]; // tx0, ty0, width
-  __shared__ uint32_t ends[256];     // running candidate totals, per wave
+  __shared__ WalkOwner owners[256]; // per wave: its splats with candidates, packed
+  __shared__ uint32_t flags[256];    // per wave: the walk's round marks
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x & ~63u;
   Splat s{};
@@ -164,30 +163,29 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   if (visible) {
     splat_tiles(s.cx, s.cy, s.ex, s.ey, gridW, gridH, t);
     candidates = (t[2] - t[0] + 1) * (t[3] - t[1] + 1);
-    tests[threadIdx.x] = tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy);
   }
   const uint32_t width = t[2] - t[0] + 1;
-  boxes[threadIdx.x][0] = t[0];
-  boxes[threadIdx.x][1] = t[1];
-  boxes[threadIdx.x][2] = width;
   const uint32_t end = wave_inclusive_sum(candidates, lane);
-  ends[threadIdx.x] = end;
   const uint32_t start = end - candidates;
   const uint32_t total = __shfl(end, 63, 64);
-  __syncthreads();
+  if (visible) {
+    const uint32_t k = static_cast<uint32_t>(__popcll(seen & ((1ull << lane) - 1ull)));
+    owners[wave + k] = WalkOwner{tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy), t[0], t[1], width, start,
+                                 1.0f / static_cast<float>(width), i, 0u, 0u};
+  }
+  __builtin_amdgcn_wave_barrier();
 
-  uint32_t reached = 0;
+  uint32_t reached = 0, before = 0;
   uint64_t bits = 0;
   for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t slot = walk_round(flags + wave, lane, base, visible, start, before);
     const uint32_t j = base + lane;
     bool hit = false;
     if (j < total) {
-      const uint32_t owner = wave + owner_of(ends + wave, j);
-      const uint32_t first = owner == wave ? 0u : ends[owner - 1];
-      const uint32_t local = j - first;
-      const uint32_t w = boxes[owner][2];
-      const uint32_t row = local / w;
-      hit = reaches_tile(tests[owner], boxes[owner][0] + (local - row * w), boxes[owner][1] + row);
+      const WalkOwner &o = owners[wave + slot];
+      uint32_t tx, ty;
+      box_tile(o, j - o.start, tx, ty);
+      hit = reaches_tile(o.test, tx, ty);
     }
     const uint64_t round = __ballot(hit);
     // This lane's own candidates within the round: lanes start - base .. start + candidates - base.

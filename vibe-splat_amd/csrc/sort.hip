@@ -1,101 +1,46 @@
-// Exclusive scan and stable LSD radix sort, the rasterizer's two integer
-// primitives (the reference's radix_histogram.comp, radix_scatter.comp,
-// scan_blocks.comp and scan_add.comp, driven by src/core/radix_sort.cpp:58-96).
+// Stable LSD radix sort, the rasterizer's depth sort and tile sort (the
+// reference's radix_histogram.comp, radix_scatter.comp and scan_blocks.comp,
+// driven by src/core/radix_sort.cpp:58-96). Per digit pass, three launches:
 //
-// The sort ranks keys with wave64 ballots instead of the reference's eight
-// barrier-separated 1-bit splits per round: for each digit bit a lane's
-// __ballot, and-ed together, gives the mask of lanes in its wave holding the
-// same digit; its rank is the popcount of that mask below it. Waves, rounds and
-// blocks are visited in index order, so the sort is stable, as the tile lists
-// need (test_radix_sort.cpp:15-60 checks the reference's against std::stable_sort).
+//   histogram  per tile of 2048 keys, how many hold each digit, stored
+//              digit-major (hist[digit * tiles + tile]);
+//   scan       one workgroup per digit: the exclusive sum over tiles in place,
+//              and the digit's total;
+//   scatter    per tile: where each digit starts (the exclusive sum of the
+//              digit totals), plus the tile's offset, plus each key's rank
+//              among the tile's keys of its digit.
+//
+// Ranking uses wave64 ballots instead of the reference's barrier-separated
+// 1-bit splits: for each digit bit a lane's ballot, and-ed together, gives the
+// lanes of its wave holding the same digit, and its rank among them is the
+// popcount below it. Each wave ranks 512 consecutive keys of the tile in
+// order, keeping its running digit counts in LDS, so a tile needs two
+// barriers and the sort is stable, as the tile lists need
+// (test_radix_sort.cpp:15-60 checks the reference's against std::stable_sort).
 //
 // Counts may live in device memory (the tile entries are only known on the
-// device): grids are sized for the capacity and blocks past the count exit.
+// device): grids are sized for the capacity and tiles past the count exit.
 #include "device.hpp"
 #include "kernels.hpp"
-
-#include <vector>
+#include "lookback.hpp"
 
 namespace vsplat {
 
 namespace {
 
-constexpr uint32_t kScanThreads = 256;
-constexpr uint32_t kScanPerThread = 16;
-constexpr uint32_t kScanBlock = kScanThreads * kScanPerThread; // 4096 elements per block
-
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortRounds = 8; // keys per thread
-constexpr uint32_t kSortTile = kSortThreads * kSortRounds;
+constexpr uint32_t kSortPerThread = 8;
+constexpr uint32_t kSortTile = kSortThreads * kSortPerThread; // 2048 keys
+constexpr uint32_t kWaveKeys = kSortTile / (kSortThreads / 64);  // 512 consecutive keys per wave
+constexpr uint32_t kScanThreads = 1024;
 
 __device__ __forceinline__ uint32_t count_of(const uint32_t *countDev, uint32_t countHost) {
   return countDev != nullptr ? *countDev : countHost;
 }
 
-// Block-wide exclusive sum of one value per thread; returns the total.
-__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t &exclusive, uint32_t *waveTotals) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  if (lane == 63) waveTotals[wave] = x;
-  __syncthreads();
-  uint32_t before = 0, total = 0;
-  for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
-    const uint32_t t = waveTotals[w];
-    if (w < wave) before += t;
-    total += t;
-  }
-  exclusive = before + x - v;
-  __syncthreads();
-  return total;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t *__restrict__ data, uint32_t count,
-                                                                   uint32_t *__restrict__ partials) {
-  __shared__ uint32_t waveTotals[kScanThreads / 64];
-  const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
-  uint32_t sum = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < kScanPerThread; ++k) sum += base + k < count ? data[base + k] : 0u;
-  uint32_t excl;
-  const uint32_t total = block_exclusive_sum(sum, excl, waveTotals);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
-}
-
-// In-place exclusive scan of a block's 4096 elements plus an offset per block.
-__global__ __launch_bounds__(kScanThreads) void scan_block_kernel(uint32_t *__restrict__ data, uint32_t count,
-                                                                  const uint32_t *__restrict__ offsets, uint32_t *__restrict__ total) {
-  __shared__ uint32_t waveTotals[kScanThreads / 64];
-  const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
-  uint32_t v[kScanPerThread];
-  uint32_t sum = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < kScanPerThread; ++k) {
-    v[k] = base + k < count ? data[base + k] : 0u;
-    sum += v[k];
-  }
-  uint32_t excl;
-  const uint32_t blockTotal = block_exclusive_sum(sum, excl, waveTotals);
-  uint32_t running = excl + (offsets != nullptr ? offsets[blockIdx.x] : 0u);
-#pragma unroll
-  for (uint32_t k = 0; k < kScanPerThread; ++k) {
-    if (base + k < count) data[base + k] = running;
-    running += v[k];
-  }
-  if (total != nullptr && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-    *total = (offsets != nullptr ? offsets[blockIdx.x] : 0u) + blockTotal;
-  }
-}
-
-// Per block, how many of its keys fall in each digit; digit-major so that one
-// exclusive scan gives every block its scatter base (radix_histogram.comp:15-30).
 template <uint32_t Bits>
 __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(const uint32_t *__restrict__ keys, const uint32_t *countDev,
-                                                                       uint32_t countHost, uint32_t shift, uint32_t numBlocks,
+                                                                       uint32_t countHost, uint32_t shift, uint32_t tiles,
                                                                        uint32_t *__restrict__ hist) {
   constexpr uint32_t kRadix = 1u << Bits;
   __shared__ uint32_t counts[kRadix];
@@ -103,135 +48,149 @@ __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(const uin
   __syncthreads();
   const uint32_t count = count_of(countDev, countHost);
   const uint32_t base = blockIdx.x * kSortTile;
+  if (base < count) {
 #pragma unroll
-  for (uint32_t r = 0; r < kSortRounds; ++r) {
-    const uint32_t i = base + r * kSortThreads + threadIdx.x;
-    if (i < count) atomicAdd(&counts[(keys[i] >> shift) & (kRadix - 1u)], 1u);
+    for (uint32_t r = 0; r < kSortPerThread; ++r) {
+      const uint32_t i = base + r * kSortThreads + threadIdx.x;
+      if (i < count) atomicAdd(&counts[(keys[i] >> shift) & (kRadix - 1u)], 1u);
+    }
   }
   __syncthreads();
-  for (uint32_t d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d * numBlocks + blockIdx.x] = counts[d];
+  for (uint32_t d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d * tiles + blockIdx.x] = counts[d];
 }
 
-// Stable scatter (radix_scatter.comp:33-85), ranks from wave ballots.
+// One workgroup per digit: the exclusive sum of that digit's counts over the
+// tiles, in place, and the digit's total.
+__global__ __launch_bounds__(kScanThreads) void radix_scan_kernel(uint32_t *__restrict__ hist, uint32_t tiles,
+                                                                  uint32_t *__restrict__ totals) {
+  __shared__ uint32_t waveTotals[kScanThreads / 64];
+  uint32_t *row = hist + static_cast<size_t>(blockIdx.x) * tiles;
+  uint32_t carry = 0;
+  for (uint32_t start = 0; start < tiles; start += kScanThreads) {
+    const uint32_t t = start + threadIdx.x;
+    const uint32_t v = t < tiles ? row[t] : 0u;
+    uint32_t exclusive;
+    const uint32_t sum = block_exclusive_sum(v, exclusive, waveTotals);
+    if (t < tiles) row[t] = carry + exclusive;
+    carry += sum;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
 template <uint32_t Bits>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
                                                                      uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                                      const uint32_t *countDev, uint32_t countHost, uint32_t shift,
-                                                                     uint32_t numBlocks, const uint32_t *__restrict__ offsets) {
+                                                                     uint32_t tiles, const uint32_t *__restrict__ hist,
+                                                                     const uint32_t *__restrict__ totals) {
   constexpr uint32_t kRadix = 1u << Bits;
   constexpr uint32_t kWaves = kSortThreads / 64;
-  __shared__ uint32_t base[kRadix];
+  __shared__ uint32_t digitBase[kRadix];
   __shared__ uint32_t waveCount[kWaves][kRadix];
+  __shared__ uint32_t waveTotals[kWaves];
   const uint32_t count = count_of(countDev, countHost);
   const uint32_t tileStart = blockIdx.x * kSortTile;
-  if (tileStart >= count) return;
-  for (uint32_t d = threadIdx.x; d < kRadix; d += kSortThreads) base[d] = offsets[d * numBlocks + blockIdx.x];
+  if (tileStart >= count) return; // the same for the whole workgroup
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+
+  // Where this tile's keys of each digit go: the digit's start plus the tile's offset.
+  const uint32_t total = threadIdx.x < kRadix ? totals[threadIdx.x] : 0u;
+  uint32_t start;
+  block_exclusive_sum(total, start, waveTotals);
+  if (threadIdx.x < kRadix) digitBase[threadIdx.x] = start + hist[threadIdx.x * tiles + blockIdx.x];
+  for (uint32_t x = threadIdx.x; x < kWaves * kRadix; x += kSortThreads) (&waveCount[0][0])[x] = 0;
+  __syncthreads();
+
+  // Rank within the wave's 512 keys, in order.
   const uint64_t below = (uint64_t{1} << lane) - 1u;
-  for (uint32_t r = 0; r < kSortRounds; ++r) {
-    for (uint32_t d = threadIdx.x; d < kRadix * kWaves; d += kSortThreads) (&waveCount[0][0])[d] = 0;
-    __syncthreads();
-    const uint32_t i = tileStart + r * kSortThreads + threadIdx.x;
-    const bool valid = i < count;
-    const uint32_t key = valid ? keysIn[i] : 0u;
-    const uint32_t val = valid ? valsIn[i] : 0u;
-    const uint32_t digit = (key >> shift) & (kRadix - 1u);
+  uint32_t key[kSortPerThread], val[kSortPerThread], rank[kSortPerThread];
+  const uint32_t first = tileStart + wave * kWaveKeys + lane;
+#pragma unroll
+  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+    const uint32_t i = first + r * 64;
+    key[r] = i < count ? keysIn[i] : 0u;
+    val[r] = i < count ? valsIn[i] : 0u;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+    const bool valid = first + r * 64 < count;
+    const uint32_t digit = (key[r] >> shift) & (kRadix - 1u);
     uint64_t same = __ballot(valid);
 #pragma unroll
     for (uint32_t b = 0; b < Bits; ++b) {
       const uint64_t ones = __ballot((digit >> b) & 1u);
       same &= ((digit >> b) & 1u) ? ones : ~ones;
     }
-    const uint32_t rank = __popcll(same & below);
-    if (valid && rank == 0) waveCount[wave][digit] = __popcll(same);
-    __syncthreads();
-    // Per digit: where each wave's run starts, after the earlier waves and rounds.
-    for (uint32_t d = threadIdx.x; d < kRadix; d += kSortThreads) {
-      uint32_t s = base[d];
+    const uint32_t lower = static_cast<uint32_t>(__popcll(same & below));
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t prior = waveCount[wave][digit];
+    rank[r] = prior + lower;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && lower == 0) waveCount[wave][digit] = prior + static_cast<uint32_t>(__popcll(same));
+  }
+  __syncthreads();
+  // Per digit: each wave's offset within the tile.
+  for (uint32_t d = threadIdx.x; d < kRadix; d += kSortThreads) {
+    uint32_t running = digitBase[d];
 #pragma unroll
-      for (uint32_t w = 0; w < kWaves; ++w) {
-        const uint32_t c = waveCount[w][d];
-        waveCount[w][d] = s;
-        s += c;
-      }
-      base[d] = s;
+    for (uint32_t w = 0; w < kWaves; ++w) {
+      const uint32_t c = waveCount[w][d];
+      waveCount[w][d] = running;
+      running += c;
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t dst = waveCount[wave][digit] + rank;
-      keysOut[dst] = key;
-      valsOut[dst] = val;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+    if (first + r * 64 < count) {
+      const uint32_t dst = waveCount[wave][(key[r] >> shift) & (kRadix - 1u)] + rank[r];
+      keysOut[dst] = key[r];
+      valsOut[dst] = val[r];
     }
-    __syncthreads();
   }
 }
 
 uint32_t div_up(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
+template <uint32_t Bits>
+void sort_pass(hipStream_t stream, const uint32_t *kIn, const uint32_t *vIn, uint32_t *kOut, uint32_t *vOut, const uint32_t *countDev,
+               uint32_t capacity, uint32_t shift, uint32_t *scratch) {
+  constexpr uint32_t kRadix = 1u << Bits;
+  const uint32_t tiles = div_up(capacity, kSortTile);
+  uint32_t *totals = scratch;
+  uint32_t *hist = scratch + 256;
+  hipLaunchKernelGGL((radix_histogram_kernel<Bits>), dim3(tiles), dim3(kSortThreads), 0, stream, kIn, countDev, capacity, shift, tiles,
+                     hist);
+  hipLaunchKernelGGL(radix_scan_kernel, dim3(kRadix), dim3(kScanThreads), 0, stream, hist, tiles, totals);
+  hipLaunchKernelGGL((radix_scatter_kernel<Bits>), dim3(tiles), dim3(kSortThreads), 0, stream, kIn, vIn, kOut, vOut, countDev, capacity,
+                     shift, tiles, hist, totals);
+}
+
 } // namespace
 
-size_t scan_scratch_words(uint64_t count) {
-  size_t words = 0;
-  while (count > kScanBlock) {
-    count = div_up(count, kScanBlock);
-    words += count + 64; // each level's partials, padded
-  }
-  return words + 64;
-}
+size_t radix_sort_scratch_words(uint64_t capacity) { return 256 + static_cast<size_t>(div_up(capacity, kSortTile)) * 256u; }
 
-void launch_exclusive_scan(hipStream_t stream, uint32_t *data, uint32_t count, uint32_t *scratch, uint32_t *total) {
-  if (count == 0) return;
-  if (count <= kScanBlock) {
-    hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(kScanThreads), 0, stream, data, count, nullptr, total);
-    return;
-  }
-  const uint32_t blocks = div_up(count, kScanBlock);
-  uint32_t *partials = scratch;
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(blocks), dim3(kScanThreads), 0, stream, data, count, partials);
-  launch_exclusive_scan(stream, partials, blocks, scratch + blocks + 64, nullptr);
-  hipLaunchKernelGGL(scan_block_kernel, dim3(blocks), dim3(kScanThreads), 0, stream, data, count, partials, total);
-}
-
-// The histogram (256 digits per block at most) and its scan's partials.
-size_t radix_sort_scratch_words(uint64_t capacity) {
-  const uint64_t hist = static_cast<uint64_t>(div_up(capacity, kSortTile)) * 256u;
-  return hist + 64 + scan_scratch_words(hist);
-}
-
-template <uint32_t Bits>
-static void sort_pass(hipStream_t stream, const uint32_t *kIn, const uint32_t *vIn, uint32_t *kOut, uint32_t *vOut,
-                      const uint32_t *countDev, uint32_t capacity, uint32_t shift, uint32_t *hist, uint32_t *scanScratch) {
-  const uint32_t blocks = div_up(capacity, kSortTile);
-  hipLaunchKernelGGL((radix_histogram_kernel<Bits>), dim3(blocks), dim3(kSortThreads), 0, stream, kIn, countDev, capacity, shift,
-                     blocks, hist);
-  launch_exclusive_scan(stream, hist, blocks << Bits, scanScratch, nullptr);
-  hipLaunchKernelGGL((radix_scatter_kernel<Bits>), dim3(blocks), dim3(kSortThreads), 0, stream, kIn, vIn, kOut, vOut, countDev,
-                     capacity, shift, blocks, hist);
-}
-
-// Sorts (keys, values) stably by the low `bits` bits of the keys. The result
-// ends in keys/values; tmpKeys/tmpValues are ping-pong buffers. countDev, when
-// given, holds the element count (at most capacity).
+// Sorts (keys, values) stably by the low `bits` bits of the keys (bits <= 32),
+// in ceil(bits / 8) passes of equal width. The result ends in keys/values;
+// tmpKeys/tmpValues are the other half of each pass. countDev, when given,
+// holds the element count (at most capacity).
 void launch_radix_sort(hipStream_t stream, uint32_t *keys, uint32_t *values, uint32_t *tmpKeys, uint32_t *tmpValues,
                        const uint32_t *countDev, uint32_t capacity, uint32_t bits, uint32_t *scratch) {
   if (capacity <= 1 || bits == 0) return;
   const uint32_t passes = div_up(bits, 8);
   const uint32_t digit = div_up(bits, passes); // even split, at most 8 bits a pass
-  const uint32_t blocks = div_up(capacity, kSortTile);
-  uint32_t *hist = scratch;
-  uint32_t *scanScratch = scratch + static_cast<size_t>(blocks) * 256u + 64;
   uint32_t *srcK = keys, *srcV = values, *dstK = tmpKeys, *dstV = tmpValues;
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t shift = p * digit;
     switch (digit) {
-    case 1: sort_pass<1>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
-    case 2: sort_pass<2>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
-    case 3: sort_pass<3>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
-    case 4: sort_pass<4>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
-    case 5: sort_pass<5>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
-    case 6: sort_pass<6>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
-    case 7: sort_pass<7>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
-    default: sort_pass<8>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, hist, scanScratch); break;
+    case 1: sort_pass<1>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
+    case 2: sort_pass<2>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
+    case 3: sort_pass<3>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
+    case 4: sort_pass<4>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
+    case 5: sort_pass<5>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
+    case 6: sort_pass<6>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
+    case 7: sort_pass<7>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
+    default: sort_pass<8>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
     }
     uint32_t *t = srcK;
     srcK = dstK;

@@ -70,13 +70,53 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane
   return v;
 }
 
-// The lane whose candidates contain candidate j: the number of lanes whose
-// running total (inclusive, in LDS) is at most j. Requires j < ends[63].
-__device__ __forceinline__ uint32_t owner_of(const uint32_t *ends, uint32_t j) {
-  uint32_t lo = 0;
-#pragma unroll
-  for (uint32_t step = 32; step > 0; step >>= 1) lo += ends[lo + step - 1] <= j ? step : 0u;
-  return lo;
+// A splat as the lanes testing its candidates read it from LDS. The splats
+// with candidates are packed in lane order (slot k is the k-th such lane), so
+// their candidate ranges follow each other without gaps.
+struct WalkOwner {
+  TileTest test;
+  uint32_t tx0, ty0, width, start; // the box, row-major, and its first candidate in the wave's list
+  float invWidth;                  // 1 / width, rounded
+  uint32_t gaussian;
+  uint32_t maskLo, maskHi;
+};
+static_assert(sizeof(WalkOwner) == 64, "64-byte walk owner");
+
+// Candidate `local` of a box `width` tiles wide: its tile. The quotient from
+// the rounded reciprocal is at most a few units off and is corrected exactly.
+__device__ __forceinline__ void box_tile(const WalkOwner &o, uint32_t local, uint32_t &tx, uint32_t &ty) {
+  int q = static_cast<int>(static_cast<float>(local) * o.invWidth);
+  int r = static_cast<int>(local) - static_cast<int>(__umul24(static_cast<uint32_t>(q), o.width));
+  while (r < 0) {
+    --q;
+    r += static_cast<int>(o.width);
+  }
+  while (r >= static_cast<int>(o.width)) {
+    ++q;
+    r -= static_cast<int>(o.width);
+  }
+  tx = o.tx0 + static_cast<uint32_t>(r);
+  ty = o.ty0 + static_cast<uint32_t>(q);
+}
+
+// One round of a wave's walk: candidates base .. base + 63 of the wave's list.
+// `flags` is the wave's 64-word scratch, `before` the number of owners whose
+// candidates start before base (advanced here). Returns the slot of the owner
+// of candidate base + lane (meaningful when that candidate exists).
+//
+// An owner starting in the round marks its position; a ballot of the marks
+// gives, by a popcount below each lane, how many owners start at or before it.
+__device__ __forceinline__ uint32_t walk_round(uint32_t *flags, uint32_t lane, uint32_t base, bool owner, uint32_t start,
+                                               uint32_t &before) {
+  __builtin_amdgcn_wave_barrier();
+  flags[lane] = 0u;
+  if (owner && start >= base && start - base < 64u) flags[start - base] = 1u;
+  __builtin_amdgcn_wave_barrier();
+  const uint64_t starts = __ballot(flags[lane] != 0u);
+  const uint64_t upTo = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+  const uint32_t slot = before + static_cast<uint32_t>(__popcll(starts & upTo)) - 1u;
+  before += static_cast<uint32_t>(__popcll(starts));
+  return slot;
 }
 
 // Bits lo..hi-1 of a 64-bit mask (0 <= lo <= hi <= 64).

@@ -150,7 +150,8 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   launch_fill_u32(stream_, stats_.as<uint32_t>(), n * 8, 0u);
   filter3d_.ensure(n * 4);
   launch_fill_u32(stream_, filter3d_.as<uint32_t>(), n, 0u);
-  scanScratch_.ensure(scan_scratch_words(n) * 4);
+  scanScratch_.ensure(coverage_scan_words(static_cast<uint32_t>(n)) * 4);
+  launch_fill_u32(stream_, scanScratch_.as<uint32_t>(), scanScratch_.bytes() / 4, 0u);
   sortScratch_.ensure(radix_sort_scratch_words(n) * 4);
   counters_.ensure(kCounterWords * 4);
   launch_fill_u32(stream_, counters_.as<uint32_t>(), kCounterWords, 0u);
@@ -321,9 +322,8 @@ void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, u
   launch_radix_sort(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tmpKeys_.as<uint32_t>(), tmpValues_.as<uint32_t>(), nullptr, count_,
                     32, sortScratch_.as<uint32_t>());
   mark("depth sort");
-  launch_gather_counts(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tileCount_.as<uint32_t>(), count_, coverage_.as<uint32_t>());
-  launch_exclusive_scan(s, coverage_.as<uint32_t>(), count_, scanScratch_.as<uint32_t>(), counters + kEntries);
-  launch_bin_check(s, counters, static_cast<uint32_t>(entryCapacity_), mode == kModeStep);
+  launch_coverage_scan(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tileCount_.as<uint32_t>(), count_, coverage_.as<uint32_t>(),
+                       counters, static_cast<uint32_t>(entryCapacity_), mode == kModeStep, scanScratch_.as<uint32_t>());
   mark("count tiles");
   launch_fill_u32(s, ranges_.as<uint32_t>(), static_cast<uint64_t>(tiles_) * 2, 0u);
   if (entryCapacity_ > 0) {
@@ -424,6 +424,10 @@ vsplat_step_stats Trainer::read_stats(bool &overflow, uint32_t *skipped) {
   double sums[2];
   std::memcpy(counters, host, sizeof(counters));
   std::memcpy(sums, host + kCounterWords * 4, sizeof(sums));
+  if (counters[kFault] != 0) {
+    launch_fill_u32(stream_, counters_.as<uint32_t>() + kFault, 1, 0u);
+    throw Error(VSPLAT_DEVICE_ERROR, "a prefix-sum wait on the GPU timed out; the step's results are not valid");
+  }
   overflow = counters[kOverflow] != 0;
   if (skipped != nullptr) *skipped = counters[kSkipped];
   vsplat_step_stats st{};
