@@ -46,13 +46,20 @@ __global__ __launch_bounds__(kScanThreads) void coverage_scan_kernel(const uint3
   const uint32_t tag = (launchShared & 0x3FFFFFFFu) << 2;
   const uint32_t first = tile * kScanTile + threadIdx.x * kScanPerThread;
   uint32_t v[kScanPerThread];
-  uint32_t sum = 0;
+  uint32_t sum = 0, seen = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kScanPerThread; ++k) {
     const uint32_t r = first + k;
-    v[k] = r < count && keys[r] != kInvisible ? tileCount[values[r]] : 0u;
+    const bool visible = r < count && keys[r] != kInvisible;
+    v[k] = visible ? tileCount[values[r]] : 0u;
     sum += v[k];
+    seen += visible ? 1u : 0u;
   }
+  // The visible count, one atomic per workgroup (a same-address atomic per
+  // wave, over a million splats, would serialise).
+  uint32_t unused;
+  const uint32_t visibleTotal = block_exclusive_sum(seen, unused, waveTotals);
+  if (threadIdx.x == 0 && visibleTotal != 0) atomicAdd(&counters[kVisible], visibleTotal);
   uint32_t exclusive;
   const uint32_t tileTotal = block_exclusive_sum(sum, exclusive, waveTotals);
   if (threadIdx.x < 64) {

@@ -1,16 +1,23 @@
 // Projection: raw parameters to screen-space splats and depth keys
 // (train_preprocess.comp:31-134), and each splat's tile coverage
-// (train_bin_count.comp:18-37), in one pass. One thread per Gaussian; the
-// parameter arrays are read as the GaussianCloud layout keeps them (xyz, xyz,
-// wxyz, o), contiguous across a wave. The coverage is counted here, in
-// Gaussian order and while the splat is still in registers, by the wave's
-// shared walk over candidate tiles (tiles.hpp); the bits it finds are kept for
-// the list pass.
+// (train_bin_count.comp:18-37), in one pass. One thread per Gaussian. A
+// workgroup first issues every load it needs: each thread its Gaussian's
+// position, scales, rotation and opacity (contiguous across a wave), and the
+// workgroup its 256 Gaussians' packed SH words as one contiguous run of
+// 16-byte loads into LDS, so that the 104-byte SH records are not read lane by
+// lane. The coverage is then counted in Gaussian order while the splat is
+// still in registers, by the wave's shared walk over candidate tiles
+// (tiles.hpp); the bits it finds are kept for the list pass.
 #include "device.hpp"
 #include "kernels.hpp"
 #include "tiles.hpp"
 
 namespace vsplat {
+
+constexpr uint32_t kProjectThreads = 256;
+constexpr uint32_t kMaxShWords = 26; // degree 3: three floats, then 45 halves in 23 words
+constexpr uint32_t kStageVectors = (kProjectThreads * kMaxShWords / 4 + kProjectThreads - 1) / kProjectThreads;
+static_assert(kProjectThreads * kMaxShWords * 4 >= kProjectThreads * 64, "the stage holds the walk's owners");
 
 struct ProjectOptions {
   uint32_t mipFilter2d;
@@ -18,20 +25,28 @@ struct ProjectOptions {
   uint32_t shRetained, shSeed;
 };
 
-// One Gaussian: false if it is culled, else its splat and depth key.
+// A Gaussian's raw parameters as read from the cloud's arrays.
+struct GaussianIn {
+  float px, py, pz;        // position
+  float ls[3];             // log standard deviations
+  float qw, qx, qy, qz;    // rotation, not normalised
+  float logit;             // opacity
+  float filter;            // Mip-Splatting 3D filter, or 0
+};
+
+// One Gaussian: false if it is culled, else its splat and depth key. `sh` is
+// this Gaussian's packed SH words.
 template <int Degree>
-__device__ __forceinline__ bool project_one(uint32_t i, const float *__restrict__ positions, const float *__restrict__ scales,
-                                            const float *__restrict__ rotations, const float *__restrict__ opacities,
-                                            const uint32_t *__restrict__ shWords, const float *__restrict__ filter3d, const ViewParams &c,
+__device__ __forceinline__ bool project_one(uint32_t i, const GaussianIn &g, const uint32_t *sh, const ViewParams &c,
                                             const ProjectOptions &opt, Splat &out, uint32_t &key) {
-  const float px = positions[i * 3], py = positions[i * 3 + 1], pz = positions[i * 3 + 2];
+  const float px = g.px, py = g.py, pz = g.pz;
   float tx, ty, tz;
   to_camera(c, px, py, pz, tx, ty, tz);
   // Written so that NaN fails every test, as the reference does.
   if (!(tz >= c.nearPlane)) return false;
-  const float opacity = 1.0f / (1.0f + vs_exp(-opacities[i]));
+  const float opacity = 1.0f / (1.0f + vs_exp(-g.logit));
   if (!(opacity >= kMinAlpha)) return false;
-  float qw = rotations[i * 4], qx = rotations[i * 4 + 1], qy = rotations[i * 4 + 2], qz = rotations[i * 4 + 3];
+  float qw = g.qw, qx = g.qx, qy = g.qy, qz = g.qz;
   const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
   if (!(qlen >= 1e-4f)) return false;
   const float iq = 1.0f / qlen;
@@ -39,7 +54,7 @@ __device__ __forceinline__ bool project_one(uint32_t i, const float *__restrict_
   qx = qx * iq;
   qy = qy * iq;
   qz = qz * iq;
-  const float sx = vs_exp(scales[i * 3]), sy = vs_exp(scales[i * 3 + 1]), sz = vs_exp(scales[i * 3 + 2]);
+  const float sx = vs_exp(g.ls[0]), sy = vs_exp(g.ls[1]), sz = vs_exp(g.ls[2]);
   float R[3][3];
   quat_to_mat(qw, qx, qy, qz, R);
   const float s[3] = {sx, sy, sz};
@@ -48,7 +63,7 @@ __device__ __forceinline__ bool project_one(uint32_t i, const float *__restrict_
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int k = 0; k < 3; ++k) M[r][k] = R[r][k] * s[k];
-  const float f = filter3d != nullptr ? filter3d[i] : 0.0f;
+  const float f = g.filter;
   float S[3][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
@@ -107,13 +122,12 @@ __device__ __forceinline__ bool project_one(uint32_t i, const float *__restrict_
   float basis[16];
   sh_basis(dx * idl, dy * idl, dz * idl, Degree, basis);
   const uint32_t used = sh_coefficients_used(i, c.shCoefficients, opt.shDropout, opt.shRetained, opt.shSeed);
-  const uint32_t base = i * c.shWords;
   float col[3] = {0.5f, 0.5f, 0.5f};
 #pragma unroll
   for (uint32_t k = 0; k < (Degree + 1) * (Degree + 1); ++k) {
 #pragma unroll
     for (uint32_t ch = 0; ch < 3; ++ch) {
-      const float v = sh_value(shWords, base, k * 3 + ch);
+      const float v = sh_value(sh, 0, k * 3 + ch);
       col[ch] = k < used ? fma32(basis[k], v, col[ch]) : col[ch];
     }
   }
@@ -136,13 +150,56 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
                                                       Splat *__restrict__ splats, uint32_t *__restrict__ keys,
                                                       uint32_t *__restrict__ values, uint32_t *__restrict__ tileCount,
                                                       TileRecord *__restrict__ records, uint32_t *counters) {
-  __shared__ WalkOwner owners[256]; // per wave: its splats with candidates, packed
-  __shared__ uint32_t flags[256];    // per wave: the walk's round marks
+  // The block's SH words, staged with coalesced loads; afterwards the walk's owners.
+  __shared__ uint32_t stage[kProjectThreads * kMaxShWords];
+  __shared__ uint32_t flags[kProjectThreads]; // per wave: the walk's round marks
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x & ~63u;
+  const uint32_t first = blockIdx.x * kProjectThreads;
+  const uint32_t inBlock = c.count - first < kProjectThreads ? c.count - first : kProjectThreads;
+
+  // Every load is issued before any of them is needed: the parameters of this
+  // thread's Gaussian, and this thread's share of the block's SH words.
+  GaussianIn g{};
+  if (i < c.count) {
+    g.px = positions[i * 3];
+    g.py = positions[i * 3 + 1];
+    g.pz = positions[i * 3 + 2];
+    g.ls[0] = scales[i * 3];
+    g.ls[1] = scales[i * 3 + 1];
+    g.ls[2] = scales[i * 3 + 2];
+    const float4 q = reinterpret_cast<const float4 *>(rotations)[i];
+    g.qw = q.x;
+    g.qx = q.y;
+    g.qy = q.z;
+    g.qz = q.w;
+    g.logit = opacities[i];
+    g.filter = filter3d != nullptr ? filter3d[i] : 0.0f;
+  }
+  {
+    // 256 Gaussians' words start 16-byte aligned (256 * 4 bytes is a multiple of 16).
+    const uint32_t words = inBlock * c.shWords;
+    const uint4 *src = reinterpret_cast<const uint4 *>(shWords + static_cast<size_t>(first) * c.shWords);
+    uint4 part[kStageVectors];
+#pragma unroll
+    for (uint32_t k = 0; k < kStageVectors; ++k) {
+      const uint32_t v = k * kProjectThreads + threadIdx.x;
+      part[k] = v < words / 4 ? src[v] : make_uint4(0, 0, 0, 0);
+    }
+    const uint32_t tail = (words / 4) * 4 + threadIdx.x; // the last words, fewer than four
+    const uint32_t last = tail < words ? shWords[static_cast<size_t>(first) * c.shWords + tail] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kStageVectors; ++k) {
+      const uint32_t v = k * kProjectThreads + threadIdx.x;
+      if (v < words / 4) reinterpret_cast<uint4 *>(stage)[v] = part[k];
+    }
+    if (tail < words) stage[tail] = last;
+  }
+  __syncthreads();
+
   Splat s{};
   uint32_t key = kInvisible;
-  const bool visible = i < c.count && project_one<Degree>(i, positions, scales, rotations, opacities, shWords, filter3d, c, opt, s, key);
+  const bool visible = i < c.count && project_one<Degree>(i, g, stage + threadIdx.x * c.shWords, c, opt, s, key);
   if (i < c.count) {
     keys[i] = key;
     values[i] = i;
@@ -153,9 +210,9 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
       out[2] = make_float4(s.cx, s.cy, s.ex, s.ey);
     }
   }
-  // One atomic per wave for the visible count.
+  __syncthreads(); // the SH words are read; the stage now holds the walk's owners
+  WalkOwner *owners = reinterpret_cast<WalkOwner *>(stage);
   const uint64_t seen = __ballot(visible);
-  if (lane == 0 && seen != 0) atomicAdd(&counters[kVisible], static_cast<uint32_t>(__popcll(seen)));
 
   // Candidate tiles: the bounding box of each visible splat.
   uint32_t t[4] = {0, 0, 0, 0};
