@@ -51,7 +51,8 @@ __device__ __forceinline__ Staged stage(const Splat &s, const TileGeometry &geo)
   uint32_t mask = 0;
   if (box != 0u) {
     const uint32_t bx0 = box & 0xFFu, by0 = (box >> 8) & 0xFFu, bx1 = (box >> 16) & 0xFFu, by1 = box >> 24;
-#pragma unroll
+    // Kept as a loop: unrolled, its temporaries would set the kernel's register count.
+#pragma unroll 1
     for (uint32_t k = 0; k < 16; ++k) {
       const float dx = (static_cast<float>(geo.x0 + k) + 0.5f) - s.cx;
       const float dy = (static_cast<float>(geo.y0 + k) + 0.5f) - s.cy;
