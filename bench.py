@@ -109,6 +109,49 @@ def cpu_baseline(workload, threads: int) -> dict:
                       f"extrapolated linearly in rows to 720"}
 
 
+def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
+    """The batched form on the eight-view ring (1M Gaussians, SH 3, views of
+    640x480 to 1920x1080 on a ring of radius 3), fixed point, with Adam: each
+    view timed as its own step (vsplat_enqueue), then the eight as one batched
+    iteration (vsplat_enqueue_views: gradients summed over the views, one Adam
+    step). Targets are in GPU memory; no stage events in either timed loop."""
+    w = F.config(3, count=count)
+    rates = vsplat.LearningRates()
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True))
+    trainer.set_stage_timing(False)
+    buffers = [vsplat.DeviceBuffer(t) for t in w.targets]
+    ptrs = [b.ptr for b in buffers]
+    try:
+        trainer.step_views(w.cameras, w.targets, rates)  # sizes every buffer and the entry reservation
+        single = []
+        for cam, ptr in zip(w.cameras, ptrs):
+            for _ in range(warmup):
+                trainer.enqueue(cam, ptr, vsplat.MODE_STEP, rates)
+            trainer.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                trainer.enqueue(cam, ptr, vsplat.MODE_STEP, rates)
+            trainer.synchronize()
+            single.append((time.perf_counter() - t0) * 1000.0 / steps)
+        for _ in range(warmup):
+            trainer.enqueue_views(w.cameras, ptrs, vsplat.MODE_STEP, rates)
+            trainer.synchronize_views()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            trainer.enqueue_views(w.cameras, ptrs, vsplat.MODE_STEP, rates)
+        trainer.synchronize_views()
+        batch = (time.perf_counter() - t0) * 1000.0 / steps
+    finally:
+        for b in buffers:
+            b.free()
+        trainer.close()
+    views = len(w.cameras)
+    return {"workload": w.description, "views": views, "sizes": [[c.width, c.height] for c in w.cameras],
+            "single_view_ms": [round(x, 4) for x in single], "single_views_total_ms": round(sum(single), 4),
+            "batched_ms": round(batch, 4), "batched_ms_per_view": round(batch / views, 4),
+            "speedup_per_view": round(sum(single) / batch, 4), "sh_storage": "DC fp32, rest fp16 (the reference's trainer layout)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +162,7 @@ def main():
     ap.add_argument("--float-atomics", action="store_true", help="float-atomic backward instead of fixed point")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--no-batched", action="store_true", help="skip the eight-view batched measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
@@ -204,6 +248,10 @@ def main():
         "ms_per_step_with_stage_events": round(ms_staged, 4),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
     }
+    if not args.no_batched:
+        log("batched form: the eight-view ring")
+        result["batched"] = batched_ring(max(5, args.steps // 2), 2, args.count)
+        log(f"  per view {result['batched']['batched_ms_per_view']} ms batched, speedup {result['batched']['speedup_per_view']}")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline on {threads} threads")

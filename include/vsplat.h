@@ -183,6 +183,29 @@ void *vsplat_stream(vsplat_trainer *trainer);
 /* Reserves room for this many tile entries now (subject to the memory budget). */
 int vsplat_reserve_entries(vsplat_trainer *trainer, uint64_t entries);
 
+/* ---- The batched step: no reference counterpart (the reference trains one
+ * view per iteration). For callers that train on several views at once. ---- */
+
+/* One training iteration over n views: each view's forward, loss and backward,
+ * the gradients summed over the views in view order (each view's objective
+ * includes the penalties, and adds to the densification statistics), then for
+ * mode 2 one Adam step from the sum (one step count, one bias correction); mode 1
+ * keeps the sum for vsplat_get_gradients. targets[k] is view k's RGBA8 image in
+ * host memory, bytes[k] its size. stats (may be NULL) gets n entries. A view that
+ * does not fit the tile-entry reservation grows it and the batch runs again, as
+ * vsplat_step does. */
+int vsplat_step_views(vsplat_trainer *trainer, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets,
+                      const size_t *bytes, int32_t mode, const vsplat_learning_rates *rates, vsplat_step_stats *stats);
+/* The same queued against images already in GPU memory (device_targets[k],
+ * width*height*4 bytes each). */
+int vsplat_enqueue_views(vsplat_trainer *trainer, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *device_targets,
+                         int32_t mode, const vsplat_learning_rates *rates);
+/* Waits for a batch queued with vsplat_enqueue_views: up to capacity views'
+ * stats into stats (may be NULL), *count (may be NULL) the batch's view count.
+ * VSPLAT_RUNTIME_ERROR, as vsplat_synchronize, if a view did not fit: the batch
+ * changed nothing. */
+int vsplat_synchronize_views(vsplat_trainer *trainer, vsplat_step_stats *stats, uint32_t capacity, uint32_t *count);
+
 /* ---- Introspection, for parity tests: no reference counterpart. ---- */
 
 /* Copies one of the trainer's intermediate device buffers, as the last call

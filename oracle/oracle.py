@@ -87,6 +87,7 @@ def lib():
         L.ob_fixed_to_float.argtypes = [vp, C.c_uint64, C.c_float, vp]
         L.ob_chain.argtypes = [vp] * 21
         L.ob_sh_step.argtypes = [vp] * 5
+        L.ob_adam.argtypes = [vp] * 15 + [C.c_uint32, C.c_uint32, vp]
         L.ob_exp_array.argtypes = [vp, vp, C.c_uint64]
         L.ob_log_array.argtypes = [vp, vp, C.c_uint64]
         L.ob_exp_array.restype = None
@@ -420,3 +421,29 @@ class Model:
                "ob_chain")
         _check(lib().ob_sh_step(_p(words), _p(states[4]), _p(raw), C.byref(self.camera), C.byref(opt)), "ob_sh_step")
         return pos, sc, rot, op, words
+
+
+def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, selective=True):
+    """The batched step's Adam (ob_adam) from zero optimiser state: the cloud's
+    parameters (SH as packed words) after one step with the summed gradients
+    grads = (positions, scales, rotations, opacities, sh) as flat float32 arrays."""
+    n = cloud.count
+    pos = np.ascontiguousarray(cloud.positions, np.float32).reshape(-1).copy()
+    sc = np.ascontiguousarray(cloud.scales, np.float32).reshape(-1).copy()
+    rot = np.ascontiguousarray(cloud.rotations, np.float32).reshape(-1).copy()
+    op = np.ascontiguousarray(cloud.opacities, np.float32).copy()
+    w = np.ascontiguousarray(words, np.uint32).copy()
+    states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * cloud.coefficients)]
+    g = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in grads]
+    opt = OBChainOptions()
+    opt.mode, opt.flags = 2, 4 if selective else 0
+    opt.rates[:] = [float(v) for v in rates]
+    opt.rest_scale = rest_scale
+    opt.beta1, opt.beta2, opt.epsilon = 0.9, 0.999, 1e-15
+    opt.correction1 = float(np.float32(1.0) - np.float32(0.9) ** np.float32(step))
+    opt.correction2 = float(np.float32(1.0) - np.float32(0.999) ** np.float32(step))
+    opt.seed = step
+    opt.sh_floats = 3 * cloud.coefficients
+    _check(lib().ob_adam(_p(pos), _p(sc), _p(rot), _p(op), _p(w), *[_p(s) for s in states], *[_p(x) for x in g], n, w.shape[1],
+                         C.byref(opt)), "ob_adam")
+    return pos, sc, rot, op, w

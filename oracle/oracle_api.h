@@ -144,6 +144,12 @@ int ob_chain(float *positions, float *scales, float *rotations, float *opacities
 int ob_sh_step(uint32_t *sh_words, uint32_t *state_sh, const float *raw, const ob_camera *cam,
                const ob_chain_options *opt);
 
+/* One Adam step of every parameter from summed gradients (the batched step). */
+int ob_adam(float *positions, float *scales, float *rotations, float *opacities, uint32_t *sh_words, uint32_t *state_pos,
+            uint32_t *state_scale, uint32_t *state_rot, uint32_t *state_opacity, uint32_t *state_sh, const float *g_pos,
+            const float *g_scale, const float *g_rot, const float *g_opacity, const float *g_sh, uint32_t count,
+            uint32_t sh_words_per, const ob_chain_options *opt);
+
 /* The exp and log the model uses, for the math tests. */
 void ob_exp_array(const float *x, float *y, uint64_t n);
 void ob_log_array(const float *x, float *y, uint64_t n);

@@ -637,36 +637,39 @@ extern "C" int ob_loss(const float *image, const uint8_t *target, uint32_t width
         }
         norm = norm + w;
       }
-      for (int q = 0; q < 15; ++q) hsum[(static_cast<std::size_t>(y) * W + x) * 15 + q] = acc[q] / norm;
+      const float inorm = 1.0f / norm;
+      for (int q = 0; q < 15; ++q) hsum[(static_cast<std::size_t>(y) * W + x) * 15 + q] = acc[q] * inorm;
     }
   }
   std::vector<float> ownSsim(pixels);
   for (int py = 0; py < H; ++py) {
     const float rowNorm = window_norm(py, H);
+    const float irow = 1.0f / rowNorm;
     for (int px = 0; px < W; ++px) {
       float acc[15] = {};
       for (int v = std::max(py - 5, 0); v <= std::min(py + 5, H - 1); ++v) {
         const float w = kWeights[v - py + 5];
         for (int q = 0; q < 15; ++q) acc[q] = fma(w, hsum[(static_cast<std::size_t>(v) * W + px) * 15 + q], acc[q]);
       }
-      const float norm = rowNorm * window_norm(px, W);
+      const float inorm = 1.0f / (rowNorm * window_norm(px, W));
       const std::size_t at = static_cast<std::size_t>(py) * W + px;
       float own = 0.0f;
       for (int ch = 0; ch < 3; ++ch) {
-        const float mx = acc[ch] / rowNorm, my = acc[3 + ch] / rowNorm;
-        const float vx = fma(-mx, mx, acc[6 + ch] / rowNorm), vy = fma(-my, my, acc[9 + ch] / rowNorm),
-                    cxy = fma(-mx, my, acc[12 + ch] / rowNorm);
+        const float mx = acc[ch] * irow, my = acc[3 + ch] * irow;
+        const float vx = fma(-mx, mx, acc[6 + ch] * irow), vy = fma(-my, my, acc[9 + ch] * irow),
+                    cxy = fma(-mx, my, acc[12 + ch] * irow);
         const float n1 = fma(2.0f * mx, my, C1), n2 = fma(2.0f, cxy, C2);
         const float d1 = fma(mx, mx, fma(my, my, C1)), d2 = (vx + vy) + C2;
         const float dd = d1 * d2;
-        const float s = (n1 * n2) / dd;
+        const float idd = 1.0f / dd, id1 = 1.0f / d1, id2 = 1.0f / d2;
+        const float s = (n1 * n2) * idd;
         own = ch == 0 ? s : own + s;
-        const float dMu = ((2.0f * my) * n2) / dd - ((2.0f * s) * mx) / d1;
-        const float dVar = -s / d2;
-        const float dCov = (2.0f * n1) / dd;
-        fields[at * 9 + ch] = fma(-my, dCov, fma(-2.0f * mx, dVar, dMu)) / norm;
-        fields[at * 9 + 3 + ch] = dVar / norm;
-        fields[at * 9 + 6 + ch] = dCov / norm;
+        const float dMu = ((2.0f * my) * n2) * idd - ((2.0f * s) * mx) * id1;
+        const float dVar = -s * id2;
+        const float dCov = (2.0f * n1) * idd;
+        fields[at * 9 + ch] = fma(-my, dCov, fma(-2.0f * mx, dVar, dMu)) * inorm;
+        fields[at * 9 + 3 + ch] = dVar * inorm;
+        fields[at * 9 + 6 + ch] = dCov * inorm;
       }
       ownSsim[at] = own;
     }
@@ -1112,6 +1115,57 @@ extern "C" int ob_sh_step(uint32_t *shWords, uint32_t *stateSh, const float *raw
       std::uint32_t state = stateSh[s];
       float value = j < 3 ? bits_to_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
       step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.rest_scale, s * 5u + 4u);
+      stateSh[s] = state;
+      if (j < 3) {
+        word = float_to_bits(value);
+      } else {
+        const std::uint32_t shift = 16u * ((j - 3) & 1u);
+        word = (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, o.seed ^ 0x5DEECE66u) << shift);
+      }
+      changed = true;
+    }
+    if (changed) shWords[w] = word;
+  }
+  return 0;
+}
+
+// The batched step's Adam from summed gradients: every non-SH parameter with
+// the chain rule's rule and rounding stream (index at * 5 + group), then SH
+// per packed word as ob_sh_step does, with the gradient read from g_sh.
+extern "C" int ob_adam(float *positions, float *scales, float *rotations, float *opacities, uint32_t *shWords, uint32_t *statePos,
+                       uint32_t *stateScale, uint32_t *stateRot, uint32_t *stateOpacity, uint32_t *stateSh, const float *gPos,
+                       const float *gScale, const float *gRot, const float *gOpacity, const float *gSh, uint32_t count,
+                       uint32_t shWordsPer, const ob_chain_options *opt) {
+  const ob_chain_options &o = *opt;
+  const bool zeros = (o.flags & kFlagSelective) == 0;
+  const auto step = [&](float *values, std::uint32_t *state, const float *grad, std::uint32_t n, float rate, std::uint32_t group) {
+    for (std::uint32_t at = 0; at < n; ++at) {
+      const float gv = grad[at];
+      if (gv == 0.0f && !zeros) continue;
+      float v = values[at];
+      std::uint32_t st = state[at];
+      step_parameter(o, v, st, gv, rate, at * 5u + group);
+      values[at] = v;
+      state[at] = st;
+    }
+  };
+  step(positions, statePos, gPos, count * 3, o.rates[0], 0);
+  step(scales, stateScale, gScale, count * 3, o.rates[1], 1);
+  step(rotations, stateRot, gRot, count * 4, o.rates[2], 2);
+  step(opacities, stateOpacity, gOpacity, count, o.rates[3], 3);
+  for (std::uint32_t w = 0; w < count * shWordsPer; ++w) {
+    const std::uint32_t i = w / shWordsPer, q = w - i * shWordsPer;
+    const std::uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
+    const std::uint32_t last = q < 3 ? q : std::min(first + 1, o.sh_floats - 1);
+    std::uint32_t word = shWords[w];
+    bool changed = false;
+    for (std::uint32_t j = first; j <= last; ++j) {
+      const std::uint32_t s = i * o.sh_floats + j;
+      const float gv = gSh[s];
+      if (gv == 0.0f && !zeros) continue;
+      std::uint32_t state = stateSh[s];
+      float value = j < 3 ? bits_to_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+      step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.rest_scale, s * 5u + 4u);
       stateSh[s] = state;
       if (j < 3) {
         word = float_to_bits(value);

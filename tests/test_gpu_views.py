@@ -1,0 +1,120 @@
+"""The batched step (several views per iteration; no reference counterpart) on
+the GPU against Oracle B:
+
+* gradients summed over views of different sizes, bit for bit: each view's
+  gradients from the FP32 model, added in view order in float32; the
+  densification statistics likewise; each view's visible count, entries and
+  loss;
+* a batch of one view steps exactly as vsplat_step does;
+* a batch of two views steps as Oracle B's Adam from the summed gradients;
+* the queued form with device targets gives the same result.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vsplat import fixtures as F
+import vsplat
+
+pytestmark = pytest.mark.gpu
+
+RATES = vsplat.LearningRates(position=1.6e-4, scale=0.005, rotation=0.001, opacity=0.05, sh=0.025, sh_rest_scale=0.05)
+RATE_TUPLE = (RATES.position, RATES.scale, RATES.rotation, RATES.opacity, RATES.sh)
+SIZES = [(64, 48), (96, 54), (80, 60)]
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def _ring(count=2000, degree=3, seed=77):
+    """A small version of the eight-view ring: three views of different sizes."""
+    cloud = F.synthetic_cloud(count, degree, seed, sh_sigma=0.3)
+    up = (0.0, -1.0, 0.0)
+    cams, targets = [], []
+    for k, (w, h) in enumerate(SIZES):
+        th = 2.0 * math.pi * k / 8.0
+        cams.append(F.look_at((3.0 * math.sin(th), 0.0, -3.0 * math.cos(th)), (0, 0, 0), up, 60.0, w, h))
+        targets.append(F.smooth_target_explicit(w, h, seed * 16 + k))
+    return cloud, cams, targets
+
+
+def _model_sum(cloud, cams, targets):
+    """Oracle B per view (fixed point), gradients and statistics summed in view order."""
+    total, stats, per_view = None, None, []
+    for cam, target in zip(cams, targets):
+        m = O.Model(cloud, cam)
+        r = m.forward()
+        m.loss(r, target, 0.2)
+        m.backward_fixed(r)
+        st = np.zeros((cloud.count, 8), np.float32) if stats is None else stats
+        g, stats = m.gradients(r, stats=st)
+        g = [np.asarray(x, np.float32).reshape(-1) for x in g]
+        total = [x.copy() for x in g] if total is None else [a + b for a, b in zip(total, g)]
+        tiles = r.tile_totals.astype(np.float64).sum(0) / (3.0 * cam.width * cam.height)
+        per_view.append((r.visible, r.entries, float(tiles[1]), float(tiles[0])))
+    return total, stats, per_view, m.words
+
+
+def test_gradients_summed_over_views_bit_exact():
+    cloud, cams, targets = _ring()
+    want, want_stats, per_view, _ = _model_sum(cloud, cams, targets)
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+        stats = t.step_views(cams, targets)
+        g = t.gradients()
+        dens = t.densify_stats()
+    for got, w in zip((g.positions, g.scales, g.rotations, g.opacities, g.sh), want):
+        np.testing.assert_array_equal(_bits(got.reshape(-1)), _bits(w))
+    np.testing.assert_array_equal(_bits(dens), _bits(want_stats))
+    assert len(stats) == len(cams)
+    for s, (visible, entries, l1, ssim) in zip(stats, per_view):
+        assert s.visible_splats == visible and s.tile_entries == entries
+        assert abs(s.l1 - l1) < 1e-6 and abs(s.ssim - ssim) < 1e-6
+
+
+def test_batch_of_one_steps_as_step_does():
+    cloud, cams, targets = _ring()
+    options = vsplat.TrainerOptions(force_fixed_point=True)
+    with vsplat.Trainer(cloud, options) as a, vsplat.Trainer(cloud, options) as b:
+        a.step(cams[1], targets[1], RATES)
+        b.step_views([cams[1]], [targets[1]], RATES)
+        ca, cb = a.cloud(), b.cloud()
+    for x, y in zip((ca.positions, ca.scales, ca.rotations, ca.opacities, ca.sh),
+                    (cb.positions, cb.scales, cb.rotations, cb.opacities, cb.sh)):
+        np.testing.assert_array_equal(_bits(x), _bits(y))
+
+
+def test_batch_of_two_steps_as_the_model():
+    cloud, cams, targets = _ring()
+    want_g, _, _, words = _model_sum(cloud, cams[:2], targets[:2])
+    pos, sc, rot, op, w = O.adam_from_gradients(cloud, words, want_g, RATE_TUPLE, rest_scale=RATES.sh_rest_scale)
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+        t.step_views(cams[:2], targets[:2], RATES)
+        c = t.cloud()
+    np.testing.assert_array_equal(_bits(c.positions.reshape(-1)), _bits(pos))
+    np.testing.assert_array_equal(_bits(c.scales.reshape(-1)), _bits(sc))
+    np.testing.assert_array_equal(_bits(c.rotations.reshape(-1)), _bits(rot))
+    np.testing.assert_array_equal(_bits(c.opacities), _bits(op))
+    sh = F.unpack_half_sh(w, (cloud.sh_degree + 1) ** 2).reshape(c.sh.shape)
+    np.testing.assert_array_equal(_bits(c.sh), _bits(sh))
+
+
+def test_queued_batch_matches_the_synchronous_one():
+    cloud, cams, targets = _ring()
+    options = vsplat.TrainerOptions(force_fixed_point=True)
+    buffers = [vsplat.DeviceBuffer(x) for x in targets]
+    try:
+        with vsplat.Trainer(cloud, options) as a, vsplat.Trainer(cloud, options) as b:
+            a.step_views(cams, targets, RATES)
+            b.reserve_entries(1 << 16)
+            b.enqueue_views(cams, [d.ptr for d in buffers], vsplat.MODE_STEP, RATES)
+            stats = b.synchronize_views()
+            assert len(stats) == len(cams)
+            ca, cb = a.cloud(), b.cloud()
+    finally:
+        for d in buffers:
+            d.free()
+    for x, y in zip((ca.positions, ca.opacities, ca.sh), (cb.positions, cb.opacities, cb.sh)):
+        np.testing.assert_array_equal(_bits(x), _bits(y))

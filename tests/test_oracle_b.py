@@ -261,3 +261,21 @@ def test_adam_moments_round_to_bf16_as_stated():
         assert np.all(state[~touched] == 0)  # selective Adam: untouched state stays zero
         checked += int(touched.sum())
     assert checked > 20
+
+
+def test_adam_from_gradients_equals_the_chain_rules_step():
+    """The batched step's Adam (ob_adam, from gradient arrays) and the chain
+    rule's own step (ob_chain in step mode with ob_sh_step) move a view's
+    parameters identically: same rule, same rounding stream."""
+    cloud, cam, target = _case("ref60")
+    m = O.Model(cloud, cam)
+    r = m.forward()
+    m.loss(r, target, 0.2)
+    m.backward_fixed(r)
+    g, _ = m.gradients(r)
+    rates = (1.6e-4, 0.005, 0.001, 0.05, 0.025)
+    want = m.step(r, rates)
+    got = O.adam_from_gradients(cloud, m.words, g, rates)
+    for a, b in zip(got, want):
+        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    assert not np.array_equal(got[3], cloud.opacities)

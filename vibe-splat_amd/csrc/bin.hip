@@ -97,6 +97,13 @@ __global__ __launch_bounds__(kScanThreads) void coverage_scan_kernel(const uint3
   }
 }
 
+// A batch's views each have their own counters; the first view's gather the
+// sticky words of all (a view that did not fit, a wait that gave up).
+__global__ void merge_flags_kernel(uint32_t *status, const uint32_t *counters) {
+  status[kOverflow] |= counters[kOverflow];
+  status[kFault] |= counters[kFault];
+}
+
 // A wave takes 64 consecutive depth-sorted splats, whose entries fill one
 // contiguous run of the list starting at the first one's offset. It walks their
 // candidate tiles 64 at a time (tiles.hpp); the reached ones of a round are
@@ -186,6 +193,10 @@ void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32
   const uint32_t tiles = count == 0 ? 1 : (count + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(coverage_scan_kernel, dim3(tiles), dim3(kScanThreads), 0, stream, keys, values, tileCount, count, offsets,
                      counters, capacity, isStep ? 1u : 0u, control);
+}
+
+void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters) {
+  hipLaunchKernelGGL(merge_flags_kernel, dim3(1), dim3(1), 0, stream, status, counters);
 }
 
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,

@@ -184,6 +184,31 @@ int vsplat_synchronize(vsplat_trainer *t, vsplat_step_stats *stats) {
   });
 }
 
+int vsplat_step_views(vsplat_trainer *t, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, const size_t *bytes,
+                      int32_t mode, const vsplat_learning_rates *rates, vsplat_step_stats *stats) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.step_views(n, cameras, targets, bytes, static_cast<uint32_t>(mode), rates, stats); });
+}
+
+int vsplat_enqueue_views(vsplat_trainer *t, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *device_targets, int32_t mode,
+                         const vsplat_learning_rates *rates) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.enqueue_views(n, cameras, device_targets, static_cast<uint32_t>(mode), rates); });
+}
+
+int vsplat_synchronize_views(vsplat_trainer *t, vsplat_step_stats *stats, uint32_t capacity, uint32_t *count) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] {
+    bool overflow = false;
+    const uint32_t n = t->impl.synchronize_views(stats, capacity, overflow);
+    if (count) *count = n;
+    if (overflow) {
+      throw vsplat::Error(VSPLAT_RUNTIME_ERROR,
+                          "a queued view needed more tile entries than were reserved; its batch and the steps queued after it were skipped");
+    }
+  });
+}
+
 void *vsplat_stream(vsplat_trainer *t) { return t != nullptr ? static_cast<void *>(t->impl.stream()) : nullptr; }
 
 int vsplat_reserve_entries(vsplat_trainer *t, uint64_t entries) {

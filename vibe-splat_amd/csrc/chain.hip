@@ -101,13 +101,39 @@ __device__ __forceinline__ void sh_basis_gradient(float x, float y, float z, uin
 __device__ __forceinline__ void emit(const ChainParams &o, float *values, uint32_t *state, float *grad, uint32_t at, float g, float rate,
                                      uint32_t group) {
   if (o.mode == kModeGradients) {
-    grad[at] = g;
+    grad[at] = (o.flags & kFlagAccumulate) != 0 ? grad[at] + g : g;
   } else if (g != 0.0f || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0)) {
     float v = values[at];
     uint32_t s = state[at];
     step_parameter(o, v, s, g, rate, at * 5u + group);
     values[at] = v;
     state[at] = s;
+  }
+}
+
+// Row k of sh_basis_gradient: the same values, for one basis function (k >= 1).
+__device__ __forceinline__ void sh_basis_gradient_term(uint32_t k, float x, float y, float z, float g[3]) {
+  const float C1 = 0.4886025119029199f, A = 1.0925484305920792f, K = 0.31539156525252005f, L = 0.5462742152960396f;
+  const float C = -0.5900435899266435f, E = 2.890611442640554f, F = -0.4570457994644658f, G = 0.3731763325901154f,
+              H = 1.445305721320277f;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  g[0] = g[1] = g[2] = 0.0f;
+  switch (k) {
+  case 1: g[1] = -C1; break;
+  case 2: g[2] = C1; break;
+  case 3: g[0] = -C1; break;
+  case 4: g[0] = A * y; g[1] = A * x; break;
+  case 5: g[1] = -A * z; g[2] = -A * y; break;
+  case 6: g[0] = K * (-2.0f * x); g[1] = K * (-2.0f * y); g[2] = K * (4.0f * z); break;
+  case 7: g[0] = -A * z; g[2] = -A * x; break;
+  case 8: g[0] = L * (2.0f * x); g[1] = L * (-2.0f * y); break;
+  case 9: g[0] = C * (6.0f * x * y); g[1] = C * (3.0f * xx - 3.0f * yy); break;
+  case 10: g[0] = E * (y * z); g[1] = E * (x * z); g[2] = E * (x * y); break;
+  case 11: g[0] = F * (-2.0f * x * y); g[1] = F * (4.0f * zz - xx - 3.0f * yy); g[2] = F * (8.0f * y * z); break;
+  case 12: g[0] = G * (-6.0f * x * z); g[1] = G * (-6.0f * y * z); g[2] = G * (6.0f * zz - 3.0f * xx - 3.0f * yy); break;
+  case 13: g[0] = F * (4.0f * zz - 3.0f * xx - yy); g[1] = F * (-2.0f * x * y); g[2] = F * (8.0f * x * z); break;
+  case 14: g[0] = H * (2.0f * x * z); g[1] = H * (-2.0f * y * z); g[2] = H * (xx - yy); break;
+  default: g[0] = C * (3.0f * xx - 3.0f * yy); g[1] = C * (-6.0f * x * y); break;
   }
 }
 
@@ -166,7 +192,9 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
     for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, 0.0f, o.rates[2], 2);
     emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, op, o.rates[3], 3);
     if (o.mode == kModeGradients) {
-      for (uint32_t j = 0; j < o.shFloats; ++j) B.gradSh[static_cast<size_t>(i) * o.shFloats + j] = 0.0f;
+      float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
+      const bool add = (o.flags & kFlagAccumulate) != 0;
+      for (uint32_t j = 0; j < o.shFloats; ++j) out[j] = add ? out[j] + 0.0f : 0.0f;
     } else {
       shPass[9] = as_float(0u);
     }
@@ -174,6 +202,73 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   }
 
   const float px = B.positions[i * 3], py = B.positions[i * 3 + 1], pz = B.positions[i * 3 + 2];
+  // Colour, clamped at zero, and the view direction's part of the position
+  // gradient, first: the SH work's temporaries are dead before the geometry's
+  // begin, which keeps the register count down. Only reads so far.
+  const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
+  const float dist = sqrtf(fma32(ddz, ddz, fma32(ddy, ddy, ddx * ddx)));
+  const float idist = 1.0f / dist;
+  const float dir[3] = {ddx * idist, ddy * idist, ddz * idist};
+  constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
+  float basis[16];
+  sh_basis(dir[0], dir[1], dir[2], Degree, basis);
+  const uint32_t used = sh_coefficients_used(i, c.shCoefficients, o.shDropout, o.shRetained, o.shSeed);
+  // The coefficients are read where they are used (colour, then the direction
+  // term) rather than held, which keeps the kernel's register count down.
+  const uint32_t base = i * c.shWords;
+  float colour[3] = {0.5f, 0.5f, 0.5f};
+#pragma unroll
+  for (uint32_t k = 0; k < kCoefficients; ++k) {
+#pragma unroll
+    for (uint32_t ch = 0; ch < 3; ++ch) {
+      const float v = sh_value(B.shWords, base, k * 3 + ch);
+      colour[ch] = k < used ? fma32(basis[k], v, colour[ch]) : colour[ch];
+    }
+  }
+  const float gcu[3] = {g[6] * (colour[0] > 0.0f ? 1.0f : 0.0f), g[7] * (colour[1] > 0.0f ? 1.0f : 0.0f),
+                        g[8] * (colour[2] > 0.0f ? 1.0f : 0.0f)};
+  float dirTerm[3] = {0.0f, 0.0f, 0.0f};
+  if (Degree > 0) {
+    float gDir[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (uint32_t k = 1; k < kCoefficients; ++k) {
+      const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value(B.shWords, base, k * 3), sh_value(B.shWords, base, k * 3 + 1),
+                           sh_value(B.shWords, base, k * 3 + 2));
+      float bg[3];
+      sh_basis_gradient_term(k, dir[0], dir[1], dir[2], bg);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) gDir[a] = k < used ? fma32(d, bg[a], gDir[a]) : gDir[a];
+    }
+    const float dd = dot3(dir[0], dir[1], dir[2], gDir[0], gDir[1], gDir[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dirTerm[a] = fma32(-dir[a], dd, gDir[a]) * idist;
+  }
+
+  if (o.mode == kModeGradients) {
+    // Coefficients above the active degree get zero (they are stored, not trained).
+    const uint32_t stored = o.shFloats / 3;
+    float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
+#pragma unroll
+    for (uint32_t k = 0; k < kCoefficients; ++k)
+#pragma unroll
+      for (uint32_t ch = 0; ch < 3; ++ch) {
+        const float v = k < used ? basis[k] * gcu[ch] : 0.0f;
+        out[k * 3 + ch] = (o.flags & kFlagAccumulate) != 0 ? out[k * 3 + ch] + v : v;
+      }
+    for (uint32_t j = kCoefficients * 3; j < stored * 3; ++j) out[j] = (o.flags & kFlagAccumulate) != 0 ? out[j] + 0.0f : 0.0f;
+  } else {
+    // For the SH pass: the colour gradient, the direction and the coefficients used.
+    shPass[0] = gcu[0];
+    shPass[1] = gcu[1];
+    shPass[2] = gcu[2];
+    shPass[3] = dir[0];
+    shPass[4] = dir[1];
+    shPass[5] = dir[2];
+    shPass[9] = as_float(used);
+  }
+
+  // Keep the scheduler from hoisting the geometry's work into the SH section.
+  __builtin_amdgcn_sched_barrier(0);
   float qw = B.rotations[i * 4], qx = B.rotations[i * 4 + 1], qy = B.rotations[i * 4 + 2], qz = B.rotations[i * 4 + 3];
   const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
   if (qlen > 0.0f) {
@@ -314,41 +409,9 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
 
   const float dOpacity = (fma32(g[0] * k2, k3, o.opacityReg) * opacity) * (1.0f - opacity);
 
-  // Colour, clamped at zero, and the view direction's part of the position gradient.
-  const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
-  const float dist = sqrtf(fma32(ddz, ddz, fma32(ddy, ddy, ddx * ddx)));
-  const float idist = 1.0f / dist;
-  const float dir[3] = {ddx * idist, ddy * idist, ddz * idist};
-  constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
-  float basis[16];
-  sh_basis(dir[0], dir[1], dir[2], Degree, basis);
-  const uint32_t used = sh_coefficients_used(i, c.shCoefficients, o.shDropout, o.shRetained, o.shSeed);
-  const uint32_t base = i * c.shWords;
-  float coefficient[kCoefficients][3];
-  float colour[3] = {0.5f, 0.5f, 0.5f};
-#pragma unroll
-  for (uint32_t k = 0; k < kCoefficients; ++k) {
-#pragma unroll
-    for (uint32_t ch = 0; ch < 3; ++ch) {
-      coefficient[k][ch] = sh_value(B.shWords, base, k * 3 + ch);
-      colour[ch] = k < used ? fma32(basis[k], coefficient[k][ch], colour[ch]) : colour[ch];
-    }
-  }
-  const float gcu[3] = {g[6] * (colour[0] > 0.0f ? 1.0f : 0.0f), g[7] * (colour[1] > 0.0f ? 1.0f : 0.0f),
-                        g[8] * (colour[2] > 0.0f ? 1.0f : 0.0f)};
   if (Degree > 0) {
-    float bg[16][3];
-    sh_basis_gradient(dir[0], dir[1], dir[2], Degree, bg);
-    float gDir[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (uint32_t k = 1; k < kCoefficients; ++k) {
-      const float d = dot3(gcu[0], gcu[1], gcu[2], coefficient[k][0], coefficient[k][1], coefficient[k][2]);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) gDir[a] = k < used ? fma32(d, bg[k][a], gDir[a]) : gDir[a];
-    }
-    const float dd = dot3(dir[0], dir[1], dir[2], gDir[0], gDir[1], gDir[2]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) gP[a] = gP[a] + fma32(-dir[a], dd, gDir[a]) * idist;
+    for (int a = 0; a < 3; ++a) gP[a] = gP[a] + dirTerm[a];
   }
 
   // Everything above read the parameters before any of them changes.
@@ -358,25 +421,6 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   }
   for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, tangent[a], o.rates[2], 2);
   emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, dOpacity, o.rates[3], 3);
-  if (o.mode == kModeGradients) {
-    // Coefficients above the active degree get zero (they are stored, not trained).
-    const uint32_t stored = o.shFloats / 3;
-    float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
-#pragma unroll
-    for (uint32_t k = 0; k < kCoefficients; ++k)
-#pragma unroll
-      for (uint32_t ch = 0; ch < 3; ++ch) out[k * 3 + ch] = k < used ? basis[k] * gcu[ch] : 0.0f;
-    for (uint32_t j = kCoefficients * 3; j < stored * 3; ++j) out[j] = 0.0f;
-  } else {
-    // For the SH pass: the colour gradient, the direction and the coefficients used.
-    shPass[0] = gcu[0];
-    shPass[1] = gcu[1];
-    shPass[2] = gcu[2];
-    shPass[3] = dir[0];
-    shPass[4] = dir[1];
-    shPass[5] = dir[2];
-    shPass[9] = as_float(used);
-  }
 }
 
 __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shWords, uint32_t *__restrict__ stateSh,
@@ -417,6 +461,62 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
   if (changed) shWords[w] = word;
 }
 
+// One Adam step from summed gradients (the batched step): thread t takes
+// parameter t of positions, scales, rotations and opacities in turn, with the
+// same rule and the same rounding stream (index at * 5 + group) as the chain
+// rule's own step, so a batch of one view steps exactly as the chain rule does.
+__global__ __launch_bounds__(256) void adam_kernel(ChainBuffers B, ViewParams c, ChainParams o, uint32_t *counters) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (counters[kOverflow] != 0) {
+    if (t == 0) counters[kSkipped] += 1; // this step did not happen: the host rolls its count back
+    return;
+  }
+  const uint32_t n = c.count;
+  if (t < 3 * n) {
+    emit(o, B.positions, B.statePos, nullptr, t, B.gradPos[t], o.rates[0], 0);
+  } else if (t < 6 * n) {
+    emit(o, B.scales, B.stateScale, nullptr, t - 3 * n, B.gradScale[t - 3 * n], o.rates[1], 1);
+  } else if (t < 10 * n) {
+    emit(o, B.rotations, B.stateRot, nullptr, t - 6 * n, B.gradRot[t - 6 * n], o.rates[2], 2);
+  } else if (t < 11 * n) {
+    emit(o, B.opacities, B.stateOpacity, nullptr, t - 10 * n, B.gradOpacity[t - 10 * n], o.rates[3], 3);
+  }
+}
+
+// The SH part of that step, per packed word (as sh_step_kernel, with the
+// gradient read from the summed SH gradients).
+__global__ __launch_bounds__(256) void sh_adam_kernel(uint32_t *__restrict__ shWords, uint32_t *__restrict__ stateSh,
+                                                      const float *__restrict__ gradSh, ViewParams c, ChainParams o,
+                                                      const uint32_t *counters) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t words = c.shWords;
+  if (w >= c.count * words) return;
+  if (counters[kOverflow] != 0) return;
+  const uint32_t i = w / words, q = w - i * words;
+  const bool zeros = (o.flags & kFlagSelective) == 0;
+  const uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
+  const uint32_t last = q < 3 ? q : (first + 1 < o.shFloats - 1 ? first + 1 : o.shFloats - 1);
+  uint32_t word = shWords[w];
+  bool changed = false;
+  for (uint32_t j = first; j <= last; ++j) {
+    const uint32_t s = i * o.shFloats + j;
+    const float gv = gradSh[s];
+    if (gv == 0.0f && !zeros) continue;
+    uint32_t state = stateSh[s];
+    float value = j < 3 ? as_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+    step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.restScale, s * 5u + 4u);
+    stateSh[s] = state;
+    if (j < 3) {
+      word = as_uint(value);
+    } else {
+      const uint32_t shift = 16u * ((j - 3) & 1u);
+      word = (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, o.seed ^ 0x5DEECE66u) << shift);
+    }
+    changed = true;
+  }
+  if (changed) shWords[w] = word;
+}
+
 } // namespace
 
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
@@ -434,6 +534,17 @@ void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewPar
     hipLaunchKernelGGL(sh_step_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
                        buffers.stateSh, buffers.shPass, view, params, counters);
   }
+}
+
+void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
+                 uint32_t *counters) {
+  if (view.count == 0) return;
+  const uint64_t parameters = static_cast<uint64_t>(view.count) * 11u;
+  hipLaunchKernelGGL(adam_kernel, dim3(static_cast<uint32_t>((parameters + 255) / 256)), dim3(256), 0, stream, buffers, view, params,
+                     counters);
+  const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
+  hipLaunchKernelGGL(sh_adam_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
+                     buffers.stateSh, buffers.gradSh, view, params, counters);
 }
 
 } // namespace vsplat

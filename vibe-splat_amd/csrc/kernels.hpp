@@ -15,6 +15,7 @@ namespace vsplat {
 constexpr uint32_t kModeStatistics = 0, kModeGradients = 1, kModeStep = 2;
 constexpr uint32_t kModeRender = 0xFFFFFFFFu; // forward only (Trainer::render)
 constexpr uint32_t kFlagStatistics = 1, kFlagSelective = 4;
+constexpr uint32_t kFlagAccumulate = 8; // gradients mode: add to the gradient buffers instead of writing them
 constexpr uint32_t kShPassFloats = 10; // per Gaussian, what the SH step needs from the chain rule
 
 struct ChainParams {
@@ -64,6 +65,8 @@ size_t coverage_scan_words(uint32_t count);
 // Offsets of each depth-sorted splat's entries, the entry count and the capacity check.
 void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
                           uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *control);
+// status[overflow] and status[fault] |= counters' (a batch's views).
+void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters);
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
                      uint32_t count, uint32_t gridW, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
                      uint32_t *entryValues);
@@ -78,6 +81,10 @@ void launch_backward(hipStream_t stream, bool fixedPoint, const Splat *splats, c
 void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
                  float *fields, float4 *adjoint, float2 *tileTotals, double *sums);
 
+// One Adam step of every parameter from the summed gradients in buffers.grad*
+// (the batched step); params.mode is kModeStep.
+void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
+                 uint32_t *counters);
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
                   const uint32_t *counters);
 

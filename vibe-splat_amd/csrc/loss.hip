@@ -78,8 +78,9 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
         }
         norm = norm + w;
       }
+      const float inorm = 1.0f / norm;
 #pragma unroll
-      for (int q = 0; q < 15; ++q) acc[q] = acc[q] / norm;
+      for (int q = 0; q < 15; ++q) acc[q] = acc[q] * inorm;
     }
 #pragma unroll
     for (int q = 0; q < 15; ++q) h[q][i] = acc[q];
@@ -90,6 +91,7 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
   float own = 0.0f;
   if (px < W && py < H) {
     const float rowNorm = window_norm(py, H);
+    const float irow = 1.0f / rowNorm;
     float acc[15];
 #pragma unroll
     for (int q = 0; q < 15; ++q) acc[q] = 0.0f;
@@ -100,26 +102,28 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
 #pragma unroll
       for (int q = 0; q < 15; ++q) acc[q] = fma32(w, h[q][idx], acc[q]);
     }
-    const float norm = rowNorm * window_norm(px, W);
+    const float inorm = 1.0f / (rowNorm * window_norm(px, W));
     const size_t at = static_cast<size_t>(py) * W + px;
     const float C1 = 0.0001f, C2 = 0.0009f;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-      const float mx = acc[ch] / rowNorm, my = acc[3 + ch] / rowNorm;
-      const float vx = fma32(-mx, mx, acc[6 + ch] / rowNorm), vy = fma32(-my, my, acc[9 + ch] / rowNorm),
-                  cxy = fma32(-mx, my, acc[12 + ch] / rowNorm);
+      const float mx = acc[ch] * irow, my = acc[3 + ch] * irow;
+      const float vx = fma32(-mx, mx, acc[6 + ch] * irow), vy = fma32(-my, my, acc[9 + ch] * irow),
+                  cxy = fma32(-mx, my, acc[12 + ch] * irow);
       const float n1 = fma32(2.0f * mx, my, C1), n2 = fma32(2.0f, cxy, C2);
       const float d1 = fma32(mx, mx, fma32(my, my, C1)), d2 = (vx + vy) + C2;
       const float dd = d1 * d2;
-      const float s = (n1 * n2) / dd;
+      // One reciprocal per denominator instead of a division per use.
+      const float idd = 1.0f / dd, id1 = 1.0f / d1, id2 = 1.0f / d2;
+      const float s = (n1 * n2) * idd;
       own = ch == 0 ? s : own + s;
       // n1 and n2 are cancelled by hand: n2 can be zero where the derivative is not.
-      const float dMu = ((2.0f * my) * n2) / dd - ((2.0f * s) * mx) / d1;
-      const float dVar = -s / d2;
-      const float dCov = (2.0f * n1) / dd;
-      fields[at * 9 + ch] = fma32(-my, dCov, fma32(-2.0f * mx, dVar, dMu)) / norm;
-      fields[at * 9 + 3 + ch] = dVar / norm;
-      fields[at * 9 + 6 + ch] = dCov / norm;
+      const float dMu = ((2.0f * my) * n2) * idd - ((2.0f * s) * mx) * id1;
+      const float dVar = -s * id2;
+      const float dCov = (2.0f * n1) * idd;
+      fields[at * 9 + ch] = fma32(-my, dCov, fma32(-2.0f * mx, dVar, dMu)) * inorm;
+      fields[at * 9 + 3 + ch] = dVar * inorm;
+      fields[at * 9 + 6 + ch] = dCov * inorm;
     }
   }
   const float total = tile_sum(own, waveTotals);

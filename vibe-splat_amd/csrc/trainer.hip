@@ -129,34 +129,15 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
     state_[g].ensure(stateFloats[g] * 4);
     launch_fill_u32(stream_, state_[g].as<uint32_t>(), stateFloats[g], 0u);
   }
-  splats_.ensure(n * sizeof(Splat));
-  keys_.ensure(n * 4);
-  values_.ensure(n * 4);
-  tmpKeys_.ensure(n * 4);
-  tmpValues_.ensure(n * 4);
-  coverage_.ensure(n * 4);
-  tileCount_.ensure(n * 4);
-  tileRecords_.ensure(n * sizeof(TileRecord));
-  if (fixedPoint_) {
-    rawFixed_.ensure(n * kRawPerSplat * 8);
-    launch_fill_u32(stream_, rawFixed_.as<uint32_t>(), n * kRawPerSplat * 2, 0u);
-  } else {
-    rawFloat_.ensure(n * kRawPerSplat * 4);
-    launch_fill_u32(stream_, rawFloat_.as<uint32_t>(), n * kRawPerSplat, 0u);
-  }
   shPass_.ensure(n * kShPassFloats * 4);
   launch_fill_u32(stream_, shPass_.as<uint32_t>(), n * kShPassFloats, 0u);
   stats_.ensure(n * 32);
   launch_fill_u32(stream_, stats_.as<uint32_t>(), n * 8, 0u);
   filter3d_.ensure(n * 4);
   launch_fill_u32(stream_, filter3d_.as<uint32_t>(), n, 0u);
-  scanScratch_.ensure(coverage_scan_words(static_cast<uint32_t>(n)) * 4);
-  launch_fill_u32(stream_, scanScratch_.as<uint32_t>(), scanScratch_.bytes() / 4, 0u);
-  sortScratch_.ensure(radix_sort_scratch_words(n) * 4);
-  counters_.ensure(kCounterWords * 4);
-  launch_fill_u32(stream_, counters_.as<uint32_t>(), kCounterWords, 0u);
-  sums_.ensure(2 * sizeof(double));
   readback_.ensure(kCounterWords * 4 + 2 * sizeof(double));
+  check(hipEventCreateWithFlags(&batchStart_, hipEventDisableTiming), "hipEventCreate");
+  view_buffers(0);
   check(hipStreamSynchronize(stream_), "trainer set-up");
 }
 
@@ -168,10 +149,60 @@ Trainer::~Trainer() {
   prefetchChanged_.notify_all();
   if (prefetchWorker_.joinable()) prefetchWorker_.join();
   if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
+  for (auto &v : views_) {
+    if (v->ownsStream) {
+      (void)hipStreamSynchronize(v->stream);
+      (void)hipStreamDestroy(v->stream);
+    }
+    if (v->done != nullptr) (void)hipEventDestroy(v->done);
+  }
+  views_.clear();
+  if (batchStart_ != nullptr) (void)hipEventDestroy(batchStart_);
   for (auto &slot : slots_)
     for (auto &e : slot.events)
       if (e != nullptr) (void)hipEventDestroy(e);
   if (stream_ != nullptr) (void)hipStreamDestroy(stream_);
+}
+
+// The k-th view's buffers, sized for the cloud on first use; the first runs on
+// the trainer's stream, the others each on a stream of their own.
+ViewBuffers &Trainer::view_buffers(size_t k) {
+  while (views_.size() <= k) {
+    auto v = std::make_unique<ViewBuffers>();
+    if (views_.empty()) {
+      v->stream = stream_;
+    } else {
+      check(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking), "hipStreamCreate");
+      v->ownsStream = true;
+    }
+    check(hipEventCreateWithFlags(&v->done, hipEventDisableTiming), "hipEventCreate");
+    const size_t n = count_;
+    v->splats.ensure(n * sizeof(Splat));
+    v->keys.ensure(n * 4);
+    v->values.ensure(n * 4);
+    v->tmpKeys.ensure(n * 4);
+    v->tmpValues.ensure(n * 4);
+    v->coverage.ensure(n * 4);
+    v->tileCount.ensure(n * 4);
+    v->tileRecords.ensure(n * sizeof(TileRecord));
+    if (fixedPoint_) {
+      v->rawFixed.ensure(n * kRawPerSplat * 8);
+      launch_fill_u32(v->stream, v->rawFixed.as<uint32_t>(), n * kRawPerSplat * 2, 0u);
+    } else {
+      v->rawFloat.ensure(n * kRawPerSplat * 4);
+      launch_fill_u32(v->stream, v->rawFloat.as<uint32_t>(), n * kRawPerSplat, 0u);
+    }
+    v->scanScratch.ensure(coverage_scan_words(static_cast<uint32_t>(n)) * 4);
+    launch_fill_u32(v->stream, v->scanScratch.as<uint32_t>(), v->scanScratch.bytes() / 4, 0u);
+    v->sortScratch.ensure(radix_sort_scratch_words(n) * 4);
+    v->counters.ensure(kCounterWords * 4);
+    launch_fill_u32(v->stream, v->counters.as<uint32_t>(), kCounterWords, 0u);
+    v->sums.ensure(2 * sizeof(double));
+    check(hipStreamSynchronize(v->stream), "view buffers");
+    views_.push_back(std::move(v));
+    if (reservedEntries_ > 0) grow_entries(*views_.back(), reservedEntries_);
+  }
+  return *views_[k];
 }
 
 ViewParams Trainer::view_params(const vsplat_camera &cam) const {
@@ -197,35 +228,34 @@ ViewParams Trainer::view_params(const vsplat_camera &cam) const {
   return v;
 }
 
-void Trainer::ensure_size(int width, int height) {
+void Trainer::ensure_size(ViewBuffers &b, int width, int height) {
   if (width <= 0 || height <= 0) throw Error(VSPLAT_RUNTIME_ERROR, "bad image size");
-  if (width == width_ && height == height_) return;
+  if (width == b.width && height == b.height) return;
   const uint64_t gridW = div_up(static_cast<uint64_t>(width), kTileSize), gridH = div_up(static_cast<uint64_t>(height), kTileSize);
   if (gridW * gridH > (1u << 24)) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too large for the tile index");
   // Tile coordinates are kept as 16-bit pairs per splat.
   if (gridW > 0xFFFFu || gridH > 0xFFFFu) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too wide or too tall");
-  width_ = width;
-  height_ = height;
-  gridW_ = static_cast<uint32_t>(gridW);
-  gridH_ = static_cast<uint32_t>(gridH);
-  tiles_ = gridW_ * gridH_;
-  tileBits_ = 1;
-  while ((uint64_t{1} << tileBits_) < tiles_) ++tileBits_;
+  b.width = width;
+  b.height = height;
+  b.gridW = static_cast<uint32_t>(gridW);
+  b.gridH = static_cast<uint32_t>(gridH);
+  b.tiles = b.gridW * b.gridH;
+  b.tileBits = 1;
+  while ((uint64_t{1} << b.tileBits) < b.tiles) ++b.tileBits;
   const size_t pixels = static_cast<size_t>(width) * static_cast<size_t>(height);
-  image_.ensure(pixels * 16);
-  stops_.ensure(pixels * 4);
-  adjoint_.ensure(pixels * 16);
-  fields_.ensure(pixels * 36);
-  target_.ensure(pixels * 4);
-  ranges_.ensure(static_cast<size_t>(tiles_) * 8);
-  tileTotals_.ensure(static_cast<size_t>(tiles_) * 8);
+  b.image.ensure(pixels * 16);
+  b.stops.ensure(pixels * 4);
+  b.adjoint.ensure(pixels * 16);
+  b.fields.ensure(pixels * 36);
+  b.ranges.ensure(static_cast<size_t>(b.tiles) * 8);
+  b.tileTotals.ensure(static_cast<size_t>(b.tiles) * 8);
 }
 
-void Trainer::grow_entries(uint64_t entries) {
-  if (entries <= entryCapacity_) return;
+void Trainer::grow_entries(ViewBuffers &b, uint64_t entries) {
+  if (entries <= b.entryCapacity) return;
   if (entries > 0xFFFFFFFFull) throw Error(VSPLAT_BUDGET_EXCEEDED, std::to_string(entries) + " tile entries exceed the 32-bit index");
   const uint64_t budget = options_.memory_budget;
-  const uint64_t current = entryCapacity_ * VSPLAT_BYTES_PER_TILE_ENTRY;
+  const uint64_t current = b.entryCapacity * VSPLAT_BYTES_PER_TILE_ENTRY;
   const uint64_t others = gpu_bytes() - current;
   // Checked before allocating: running out of memory is not recoverable, this is (trainer.cpp:354-360).
   if (others + entries * VSPLAT_BYTES_PER_TILE_ENTRY > budget) {
@@ -234,16 +264,19 @@ void Trainer::grow_entries(uint64_t entries) {
   const uint64_t wanted = std::min<uint64_t>(entries + entries / 2 + 1024, 0xFFFFFFFFull);
   const uint64_t affordable = budget == ~uint64_t{0} ? wanted : (budget - others) / VSPLAT_BYTES_PER_TILE_ENTRY;
   const uint64_t capacity = std::max(std::min(wanted, affordable), entries);
-  check(hipStreamSynchronize(stream_), "grow entries");
-  for (DeviceArray *a : {&entryKeys_, &entryValues_, &tmpEntryKeys_, &tmpEntryValues_}) {
+  check(hipStreamSynchronize(b.stream), "grow entries");
+  for (DeviceArray *a : {&b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues}) {
     a->release();
     a->ensure(capacity * 4);
   }
-  sortScratch_.ensure(std::max(radix_sort_scratch_words(capacity), radix_sort_scratch_words(count_)) * 4);
-  entryCapacity_ = capacity;
+  b.sortScratch.ensure(std::max(radix_sort_scratch_words(capacity), radix_sort_scratch_words(count_)) * 4);
+  b.entryCapacity = capacity;
 }
 
-void Trainer::reserve_entries(uint64_t entries) { grow_entries(entries); }
+void Trainer::reserve_entries(uint64_t entries) {
+  reservedEntries_ = std::max(reservedEntries_, entries);
+  for (auto &v : views_) grow_entries(*v, entries);
+}
 
 void Trainer::begin_timing(bool iteration) {
   if (slotPending_ == slots_.size()) {
@@ -307,57 +340,58 @@ std::vector<std::pair<std::string, double>> Trainer::timings() const {
   return out;
 }
 
-void Trainer::enqueue_passes(const ViewParams &v, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates,
-                             bool timed) {
-  hipStream_t s = stream_;
+void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &v, const uint8_t *deviceTarget, uint32_t mode,
+                             const vsplat_learning_rates *rates, bool timed, bool chain) {
+  hipStream_t s = b.stream;
   current_ = nullptr;
   if (timed) begin_timing(true);
-  uint32_t *counters = counters_.as<uint32_t>();
+  uint32_t *counters = b.counters.as<uint32_t>();
   launch_fill_u32(s, counters, 3, 0u); // not the sticky overflow word
   launch_project(s, positions_.as<float>(), scales_.as<float>(), rotations_.as<float>(), opacities_.as<float>(),
                  shWordsBuf_.as<uint32_t>(), filterSet_ ? filter3d_.as<float>() : nullptr, v, options_.mip_filter_2d != 0, dropoutNow_,
-                 shRetained_, shSeed_, gridW_, gridH_, splats_.as<Splat>(), keys_.as<uint32_t>(), values_.as<uint32_t>(),
-                 tileCount_.as<uint32_t>(), tileRecords_.as<TileRecord>(), counters);
+                 shRetained_, shSeed_, b.gridW, b.gridH, b.splats.as<Splat>(), b.keys.as<uint32_t>(), b.values.as<uint32_t>(),
+                 b.tileCount.as<uint32_t>(), b.tileRecords.as<TileRecord>(), counters);
   mark("project");
-  launch_radix_sort(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tmpKeys_.as<uint32_t>(), tmpValues_.as<uint32_t>(), nullptr, count_,
-                    32, sortScratch_.as<uint32_t>());
+  launch_radix_sort(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.tmpKeys.as<uint32_t>(), b.tmpValues.as<uint32_t>(), nullptr, count_,
+                    32, b.sortScratch.as<uint32_t>());
   mark("depth sort");
-  launch_coverage_scan(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), tileCount_.as<uint32_t>(), count_, coverage_.as<uint32_t>(),
-                       counters, static_cast<uint32_t>(entryCapacity_), mode == kModeStep, scanScratch_.as<uint32_t>());
+  launch_coverage_scan(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.tileCount.as<uint32_t>(), count_, b.coverage.as<uint32_t>(),
+                       counters, static_cast<uint32_t>(b.entryCapacity), mode == kModeStep, b.scanScratch.as<uint32_t>());
   mark("count tiles");
-  launch_fill_u32(s, ranges_.as<uint32_t>(), static_cast<uint64_t>(tiles_) * 2, 0u);
-  if (entryCapacity_ > 0) {
-    launch_bin_emit(s, keys_.as<uint32_t>(), values_.as<uint32_t>(), splats_.as<Splat>(), tileRecords_.as<TileRecord>(), count_, gridW_,
-                    coverage_.as<uint32_t>(), counters, entryKeys_.as<uint32_t>(), entryValues_.as<uint32_t>());
-    launch_radix_sort(s, entryKeys_.as<uint32_t>(), entryValues_.as<uint32_t>(), tmpEntryKeys_.as<uint32_t>(), tmpEntryValues_.as<uint32_t>(),
-                      counters + kEntriesUsed, static_cast<uint32_t>(entryCapacity_), tileBits_, sortScratch_.as<uint32_t>());
-    launch_tile_ranges(s, entryKeys_.as<uint32_t>(), counters, static_cast<uint32_t>(entryCapacity_), ranges_.as<uint2>());
+  launch_fill_u32(s, b.ranges.as<uint32_t>(), static_cast<uint64_t>(b.tiles) * 2, 0u);
+  if (b.entryCapacity > 0) {
+    launch_bin_emit(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.splats.as<Splat>(), b.tileRecords.as<TileRecord>(), count_, b.gridW,
+                    b.coverage.as<uint32_t>(), counters, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>());
+    launch_radix_sort(s, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tmpEntryKeys.as<uint32_t>(), b.tmpEntryValues.as<uint32_t>(),
+                      counters + kEntriesUsed, static_cast<uint32_t>(b.entryCapacity), b.tileBits, b.sortScratch.as<uint32_t>());
+    launch_tile_ranges(s, b.entryKeys.as<uint32_t>(), counters, static_cast<uint32_t>(b.entryCapacity), b.ranges.as<uint2>());
   }
   mark("tile lists");
-  launch_forward(s, splats_.as<Splat>(), entryValues_.as<uint32_t>(), ranges_.as<uint2>(), v.width, v.height, options_.background,
-                 image_.as<float4>(), stops_.as<uint32_t>());
+  launch_forward(s, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height, options_.background,
+                 b.image.as<float4>(), b.stops.as<uint32_t>());
   mark("composite");
   if (mode == kModeRender) return;
-  launch_loss(s, image_.as<float4>(), deviceTarget, v.width, v.height, options_.lambda_ssim, fields_.as<float>(), adjoint_.as<float4>(),
-              tileTotals_.as<float2>(), sums_.as<double>());
+  launch_loss(s, b.image.as<float4>(), deviceTarget, v.width, v.height, options_.lambda_ssim, b.fields.as<float>(), b.adjoint.as<float4>(),
+              b.tileTotals.as<float2>(), b.sums.as<double>());
   if (fixedPoint_) {
-    launch_fill_u32(s, rawFixed_.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat * 2, 0u);
+    launch_fill_u32(s, b.rawFixed.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat * 2, 0u);
   } else {
-    launch_fill_u32(s, rawFloat_.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat, 0u);
+    launch_fill_u32(s, b.rawFloat.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat, 0u);
   }
   mark("loss");
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
   // a resolution of 1/64 (trainer.cpp:586-591).
   const float fixedScale = 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f;
-  launch_backward(s, fixedPoint_, splats_.as<Splat>(), entryValues_.as<uint32_t>(), ranges_.as<uint2>(), v.width, v.height,
-                  options_.background, image_.as<float4>(), adjoint_.as<float4>(), stops_.as<uint32_t>(), fixedScale, rawFixed_.as<int64_t>(),
-                  rawFloat_.as<float>());
+  launch_backward(s, fixedPoint_, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height,
+                  options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
+                  b.rawFloat.as<float>());
   mark("backward");
-  enqueue_chain(v, mode, rates);
+  if (!chain) return;
+  enqueue_chain(b, v, mode, rates, s, counters);
   mark(mode == kModeStep ? "chain rule and step" : "chain rule");
 }
 
-void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_learning_rates *rates) {
+ChainParams Trainer::chain_params(const ViewParams &v, uint32_t mode, const vsplat_learning_rates *rates) const {
   ChainParams p{};
   p.opacityReg = options_.opacity_reg / static_cast<float>(count_);
   p.scaleReg = options_.scale_reg / static_cast<float>(count_);
@@ -366,7 +400,8 @@ void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_lea
   p.shRetained = shRetained_;
   p.shSeed = shSeed_;
   p.mode = mode;
-  p.flags = (options_.selective_adam ? kFlagSelective : 0u) | (statisticsNext_ ? kFlagStatistics : 0u);
+  p.flags = (options_.selective_adam ? kFlagSelective : 0u) | (statisticsNext_ ? kFlagStatistics : 0u) |
+            (accumulateNext_ ? kFlagAccumulate : 0u);
   p.beta1 = 0.9f;
   p.beta2 = 0.999f;
   p.epsilon = 1e-15f; // trainer.cpp:23
@@ -385,11 +420,16 @@ void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_lea
   p.shFloats = shFloats_;
   p.fixedPoint = fixedPoint_ ? 1u : 0u;
   p.fixedScale = 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f;
-  if (mode == kModeGradients) {
-    const size_t n = count_;
-    const size_t floats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
-    for (int g = 0; g < 5; ++g) grad_[g].ensure(floats[g] * 4);
-  }
+  return p;
+}
+
+void Trainer::ensure_gradients() {
+  const size_t n = count_;
+  const size_t floats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
+  for (int g = 0; g < 5; ++g) grad_[g].ensure(floats[g] * 4);
+}
+
+ChainBuffers Trainer::chain_buffers(const ViewBuffers &v) const {
   ChainBuffers b{};
   b.positions = positions_.as<float>();
   b.scales = scales_.as<float>();
@@ -397,9 +437,9 @@ void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_lea
   b.opacities = opacities_.as<float>();
   b.shWords = shWordsBuf_.as<uint32_t>();
   b.filter3d = filterSet_ ? filter3d_.as<float>() : nullptr;
-  b.splats = splats_.as<Splat>();
-  b.rawFixed = rawFixed_.as<int64_t>();
-  b.rawFloat = rawFloat_.as<float>();
+  b.splats = v.splats.as<Splat>();
+  b.rawFixed = v.rawFixed.as<int64_t>();
+  b.rawFloat = v.rawFloat.as<float>();
   b.shPass = shPass_.as<float>();
   b.stats = stats_.as<float>();
   b.statePos = state_[0].as<uint32_t>();
@@ -412,27 +452,38 @@ void Trainer::enqueue_chain(const ViewParams &v, uint32_t mode, const vsplat_lea
   b.gradRot = grad_[2].as<float>();
   b.gradOpacity = grad_[3].as<float>();
   b.gradSh = grad_[4].as<float>();
-  launch_chain(stream_, b, v, p, counters_.as<uint32_t>());
+  return b;
+}
+
+void Trainer::enqueue_chain(const ViewBuffers &b, const ViewParams &v, uint32_t mode, const vsplat_learning_rates *rates,
+                            hipStream_t stream, uint32_t *counters) {
+  if (mode == kModeGradients) ensure_gradients();
+  launch_chain(stream, chain_buffers(b), v, chain_params(v, mode, rates), counters);
 }
 
 vsplat_step_stats Trainer::read_stats(bool &overflow, uint32_t *skipped) {
+  ViewBuffers &b = *views_[0];
   uint8_t *host = readback_.as<uint8_t>();
-  check(hipMemcpyAsync(host, counters_.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToHost, stream_), "read counters");
-  check(hipMemcpyAsync(host + kCounterWords * 4, sums_.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToHost, stream_), "read sums");
+  check(hipMemcpyAsync(host, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToHost, stream_), "read counters");
+  check(hipMemcpyAsync(host + kCounterWords * 4, b.sums.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToHost, stream_), "read sums");
   check(hipStreamSynchronize(stream_), "step");
   uint32_t counters[kCounterWords];
   double sums[2];
   std::memcpy(counters, host, sizeof(counters));
   std::memcpy(sums, host + kCounterWords * 4, sizeof(sums));
   if (counters[kFault] != 0) {
-    launch_fill_u32(stream_, counters_.as<uint32_t>() + kFault, 1, 0u);
+    launch_fill_u32(stream_, b.counters.as<uint32_t>() + kFault, 1, 0u);
     throw Error(VSPLAT_DEVICE_ERROR, "a prefix-sum wait on the GPU timed out; the step's results are not valid");
   }
   overflow = counters[kOverflow] != 0;
   if (skipped != nullptr) *skipped = counters[kSkipped];
+  return make_stats(counters, sums, b.width, b.height);
+}
+
+vsplat_step_stats Trainer::make_stats(const uint32_t *counters, const double *sums, int width, int height) const {
   vsplat_step_stats st{};
   // trainer.cpp:606-620: the per-tile totals summed in double, then in float as the reference.
-  const double perValue = 1.0 / (3.0 * static_cast<double>(width_) * static_cast<double>(height_));
+  const double perValue = 1.0 / (3.0 * static_cast<double>(width) * static_cast<double>(height));
   st.l1 = static_cast<float>(sums[1] * perValue);
   st.ssim = static_cast<float>(sums[0] * perValue);
   st.loss = (1.0f - options_.lambda_ssim) * st.l1 + options_.lambda_ssim * (1.0f - st.ssim);
@@ -481,13 +532,15 @@ void Trainer::stage_target(const uint8_t *target, size_t bytes) {
     staging_.ensure(bytes);
     std::memcpy(staging_.as<uint8_t>(), target, bytes);
   }
+  target_.ensure(bytes);
   check(hipMemcpyAsync(target_.as<uint8_t>(), staging_.as<uint8_t>(), bytes, hipMemcpyHostToDevice, stream_), "upload target");
 }
 
 vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *target, size_t bytes, uint32_t mode,
                                const vsplat_learning_rates *rates, const uint8_t *next, size_t nextBytes) {
-  ensure_size(camera.width, camera.height);
-  const size_t expected = static_cast<size_t>(width_) * static_cast<size_t>(height_) * 4;
+  ViewBuffers &b = *views_[0];
+  ensure_size(b, camera.width, camera.height);
+  const size_t expected = static_cast<size_t>(b.width) * static_cast<size_t>(b.height) * 4;
   if (target == nullptr || bytes != expected) throw Error(VSPLAT_RUNTIME_ERROR, "the target image does not match the camera");
   if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
   if (rates != nullptr && rates->trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
@@ -499,14 +552,14 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
   statisticsNext_ = true; // run() always adds to the densification statistics (trainer.cpp:600)
   vsplat_step_stats stats{};
   for (int attempt = 0;; ++attempt) {
-    enqueue_passes(v, target_.as<uint8_t>(), mode, rates, true);
+    enqueue_passes(b, v, target_.as<uint8_t>(), mode, rates, true);
     bool overflow = false;
     stats = read_stats(overflow);
     if (!overflow) break;
     cancel_timing();
-    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
-    if (attempt > 0 && stats.tile_entries <= entryCapacity_) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
-    grow_entries(stats.tile_entries);
+    launch_fill_u32(stream_, b.counters.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
+    if (attempt > 0 && stats.tile_entries <= b.entryCapacity) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+    grow_entries(b, stats.tile_entries);
   }
   drain_timing();
   if (mode == kModeGradients) haveGradients_ = true;
@@ -516,30 +569,33 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
 }
 
 void Trainer::render(const vsplat_camera &camera, float *out) {
-  ensure_size(camera.width, camera.height);
+  ViewBuffers &b = *views_[0];
+  ensure_size(b, camera.width, camera.height);
   dropoutNow_ = 0.0f; // reports and evaluation see the whole model
   const ViewParams v = view_params(camera);
   for (int attempt = 0;; ++attempt) {
-    enqueue_passes(v, nullptr, kModeRender, nullptr, false);
+    enqueue_passes(b, v, nullptr, kModeRender, nullptr, false);
     bool overflow = false;
     const vsplat_step_stats stats = read_stats(overflow);
     if (!overflow) break;
-    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
-    if (attempt > 0 && stats.tile_entries <= entryCapacity_) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
-    grow_entries(stats.tile_entries);
+    launch_fill_u32(stream_, b.counters.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
+    if (attempt > 0 && stats.tile_entries <= b.entryCapacity) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+    grow_entries(b, stats.tile_entries);
   }
-  const size_t pixels = static_cast<size_t>(width_) * static_cast<size_t>(height_);
-  check(hipMemcpy(out, image_.as<float4>(), pixels * 16, hipMemcpyDeviceToHost), "download image");
+  const size_t pixels = static_cast<size_t>(b.width) * static_cast<size_t>(b.height);
+  check(hipMemcpy(out, b.image.as<float4>(), pixels * 16, hipMemcpyDeviceToHost), "download image");
 }
 
 void Trainer::apply_gradients(const vsplat_learning_rates &rates) {
-  if (width_ == 0) throw Error(VSPLAT_LOGIC_ERROR, "no gradients: compute_gradients() comes first");
+  if (!haveGradients_) throw Error(VSPLAT_LOGIC_ERROR, "no gradients: compute_gradients() comes first");
   if (rates.trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
   dropoutNow_ = 0.0f;
   statisticsNext_ = false;
   begin_timing(false);
-  enqueue_chain(lastView_, kModeStep, &rates);
-  mark("chain rule and step");
+  // Adam from the stored gradients: the same rule and rounding as the chain
+  // rule's own step, and it also serves a batch's summed gradients.
+  launch_adam(stream_, chain_buffers(*views_[0]), lastView_, chain_params(lastView_, kModeStep, &rates), views_[0]->counters.as<uint32_t>());
+  mark("adam step");
   check(hipStreamSynchronize(stream_), "apply gradients");
   drain_timing();
   ++adamStep_;
@@ -549,11 +605,12 @@ void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, 
   if (mode != kModeGradients && mode != kModeStep) throw Error(VSPLAT_INVALID_ARGUMENT, "mode must be 1 (gradients) or 2 (step)");
   if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
   if (deviceTarget == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no target");
-  ensure_size(camera.width, camera.height);
+  ViewBuffers &b = *views_[0];
+  ensure_size(b, camera.width, camera.height);
   dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
   statisticsNext_ = true;
   const ViewParams v = view_params(camera);
-  enqueue_passes(v, deviceTarget, mode, rates, timeQueued_);
+  enqueue_passes(b, v, deviceTarget, mode, rates, timeQueued_);
   if (mode == kModeGradients) haveGradients_ = true;
   if (mode == kModeStep) ++adamStep_;
   lastView_ = v;
@@ -567,11 +624,129 @@ vsplat_step_stats Trainer::synchronize() {
   if (overflow) {
     // Adam's step count (and so its bias corrections) covers only the steps that ran.
     adamStep_ -= skipped < adamStep_ ? skipped : adamStep_;
-    launch_fill_u32(stream_, counters_.as<uint32_t>() + kOverflow, 2, 0u); // overflow and skipped
-    check(hipStreamSynchronize(stream_), "clear overflow");
+    clear_overflow();
     throw Error(VSPLAT_RUNTIME_ERROR, "a queued view needed more tile entries than were reserved; it and the views queued after it were skipped");
   }
   return stats;
+}
+
+// Clears every view's sticky overflow word and skipped count (after a failed
+// queue has been reported).
+void Trainer::clear_overflow() {
+  for (auto &v : views_) launch_fill_u32(stream_, v->counters.as<uint32_t>() + kOverflow, 2, 0u);
+  check(hipStreamSynchronize(stream_), "clear overflow");
+}
+
+// The batched step (no reference counterpart): every view's passes up to its
+// backward pass, each on its own buffers and stream, side by side; then on the
+// trainer's stream, once all are through, each view's chain rule in view order
+// adding its gradients to the sums (the first view writes them), and one Adam
+// step from the sums. A batch is one training iteration: one Adam step count,
+// one bias correction. A view that does not fit skips the whole batch: its
+// overflow word is or-ed into the first view's, which the chain rule and the
+// Adam step check.
+void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                            const vsplat_learning_rates *rates) {
+  if (n == 0 || cameras == nullptr || targets == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a batch needs views and targets");
+  if (mode != kModeGradients && mode != kModeStep) throw Error(VSPLAT_INVALID_ARGUMENT, "mode must be 1 (gradients) or 2 (step)");
+  if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
+  if (rates != nullptr && rates->trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
+  for (uint32_t k = 0; k < n; ++k) {
+    if (targets[k] == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no target for a view");
+    ensure_size(view_buffers(k), cameras[k].width, cameras[k].height);
+  }
+  ensure_gradients();
+  viewStats_.ensure(static_cast<size_t>(n) * kViewStatsBytes);
+  queuedViewSizes_.clear();
+  dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
+  statisticsNext_ = true;
+  current_ = nullptr; // no stage events in a batch
+  // The views start after everything already queued on the trainer's stream
+  // (the previous step's update of the parameters).
+  check(hipEventRecord(batchStart_, stream_), "hipEventRecord");
+  std::vector<ViewParams> params(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    ViewBuffers &b = *views_[k];
+    if (b.stream != stream_) check(hipStreamWaitEvent(b.stream, batchStart_, 0), "hipStreamWaitEvent");
+    params[k] = view_params(cameras[k]);
+    enqueue_passes(b, params[k], targets[k], kModeGradients, nullptr, false, false);
+    uint8_t *slot = viewStats_.as<uint8_t>() + static_cast<size_t>(k) * kViewStatsBytes;
+    check(hipMemcpyAsync(slot, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToDevice, b.stream), "view counters");
+    check(hipMemcpyAsync(slot + kCounterWords * 4, b.sums.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToDevice, b.stream),
+          "view sums");
+    check(hipEventRecord(b.done, b.stream), "hipEventRecord");
+    queuedViewSizes_.emplace_back(cameras[k].width, cameras[k].height);
+  }
+  uint32_t *status = views_[0]->counters.as<uint32_t>();
+  for (uint32_t k = 1; k < n; ++k) {
+    check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
+    launch_merge_flags(stream_, status, views_[k]->counters.as<uint32_t>());
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    accumulateNext_ = k > 0;
+    enqueue_chain(*views_[k], params[k], kModeGradients, nullptr, stream_, status);
+  }
+  accumulateNext_ = false;
+  haveGradients_ = true;
+  lastView_ = params[n - 1];
+  if (mode == kModeStep) {
+    launch_adam(stream_, chain_buffers(*views_[0]), lastView_, chain_params(lastView_, kModeStep, rates), status);
+    ++adamStep_;
+  }
+}
+
+uint32_t Trainer::synchronize_views(vsplat_step_stats *stats, uint32_t capacity, bool &overflow) {
+  uint32_t skipped = 0;
+  (void)read_stats(overflow, &skipped);
+  drain_timing();
+  const uint32_t n = static_cast<uint32_t>(queuedViewSizes_.size());
+  std::vector<uint8_t> host(static_cast<size_t>(n) * kViewStatsBytes);
+  if (n > 0) check(hipMemcpy(host.data(), viewStats_.as<uint8_t>(), host.size(), hipMemcpyDeviceToHost), "read view stats");
+  for (uint32_t k = 0; k < n && k < capacity && stats != nullptr; ++k) {
+    uint32_t counters[kCounterWords];
+    double sums[2];
+    std::memcpy(counters, host.data() + static_cast<size_t>(k) * kViewStatsBytes, sizeof(counters));
+    std::memcpy(sums, host.data() + static_cast<size_t>(k) * kViewStatsBytes + kCounterWords * 4, sizeof(sums));
+    stats[k] = make_stats(counters, sums, queuedViewSizes_[k].first, queuedViewSizes_[k].second);
+  }
+  if (overflow) {
+    adamStep_ -= skipped < adamStep_ ? skipped : adamStep_;
+    clear_overflow();
+  }
+  return n;
+}
+
+uint32_t Trainer::step_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, const size_t *bytes, uint32_t mode,
+                             const vsplat_learning_rates *rates, vsplat_step_stats *stats) {
+  if (n == 0 || cameras == nullptr || targets == nullptr || bytes == nullptr) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, "a batch needs views and targets");
+  }
+  while (viewTargets_.size() < n) viewTargets_.push_back(std::make_unique<DeviceArray>());
+  std::vector<const uint8_t *> device(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    const size_t expected = static_cast<size_t>(cameras[k].width) * static_cast<size_t>(cameras[k].height) * 4;
+    if (cameras[k].width <= 0 || cameras[k].height <= 0 || targets[k] == nullptr || bytes[k] != expected) {
+      throw Error(VSPLAT_RUNTIME_ERROR, "a target image does not match its camera");
+    }
+    viewTargets_[k]->ensure(bytes[k]);
+    check(hipMemcpyAsync(viewTargets_[k]->as<uint8_t>(), targets[k], bytes[k], hipMemcpyHostToDevice, stream_), "upload target");
+    device[k] = viewTargets_[k]->as<uint8_t>();
+  }
+  std::vector<vsplat_step_stats> all(n);
+  for (int attempt = 0;; ++attempt) {
+    enqueue_views(n, cameras, device.data(), mode, rates);
+    bool overflow = false;
+    synchronize_views(all.data(), n, overflow);
+    if (!overflow) break;
+    bool grew = false;
+    for (uint32_t k = 0; k < n; ++k) {
+      grew = grew || all[k].tile_entries > views_[k]->entryCapacity;
+      grow_entries(*views_[k], all[k].tile_entries);
+    }
+    if (attempt > 0 && !grew) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+  }
+  if (stats != nullptr) std::copy(all.begin(), all.end(), stats);
+  return n;
 }
 
 void Trainer::gradients(float *positions, float *scales, float *rotations, float *opacities, float *sh) const {
@@ -647,10 +822,19 @@ uint64_t Trainer::bytes_per_gaussian() const {
 }
 
 uint64_t Trainer::gpu_bytes() const {
-  uint64_t total = bytes_per_gaussian() * count_;
-  for (const DeviceArray *a : {&image_, &stops_, &adjoint_, &fields_, &target_, &ranges_, &tileTotals_, &entryKeys_, &entryValues_,
-                               &tmpEntryKeys_, &tmpEntryValues_}) {
-    total += a->bytes();
+  // One view's per-Gaussian share is in bytes_per_gaussian; the others' are added as allocated.
+  uint64_t total = bytes_per_gaussian() * count_ + target_.bytes() + viewStats_.bytes();
+  for (size_t k = 0; k < views_.size(); ++k) {
+    const ViewBuffers &b = *views_[k];
+    for (const DeviceArray *a : {&b.image, &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.entryKeys, &b.entryValues,
+                                 &b.tmpEntryKeys, &b.tmpEntryValues}) {
+      total += a->bytes();
+    }
+    if (k == 0) continue;
+    for (const DeviceArray *a : {&b.splats, &b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount, &b.tileRecords,
+                                 &b.rawFixed, &b.rawFloat, &b.sortScratch, &b.scanScratch}) {
+      total += a->bytes();
+    }
   }
   for (const DeviceArray &a : grad_) total += a.bytes();
   return total;
@@ -661,12 +845,13 @@ uint64_t Trainer::gpu_bytes() const {
 namespace vsplat {
 
 void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uint64_t *bytes) const {
-  const uint64_t n = count_, pixels = static_cast<uint64_t>(width_) * static_cast<uint64_t>(height_);
+  const ViewBuffers &b = *views_[0];
+  const uint64_t n = count_, pixels = static_cast<uint64_t>(b.width) * static_cast<uint64_t>(b.height);
   uint64_t entries = 0;
   if (name == "entry_keys" || name == "entry_values") {
     uint32_t counters[kCounterWords];
     check(hipStreamSynchronize(stream_), "inspect");
-    check(hipMemcpy(counters, counters_.as<uint32_t>(), sizeof(counters), hipMemcpyDeviceToHost), "inspect counters");
+    check(hipMemcpy(counters, b.counters.as<uint32_t>(), sizeof(counters), hipMemcpyDeviceToHost), "inspect counters");
     entries = counters[kEntriesUsed];
   }
   struct Item {
@@ -675,20 +860,20 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
     uint64_t size;
   };
   const Item items[] = {
-      {"splats", &splats_, n * sizeof(Splat)},
-      {"keys", &keys_, n * 4},
-      {"values", &values_, n * 4},
-      {"coverage", &coverage_, n * 4},
-      {"entry_keys", &entryKeys_, entries * 4},
-      {"entry_values", &entryValues_, entries * 4},
-      {"ranges", &ranges_, static_cast<uint64_t>(tiles_) * 8},
-      {"image", &image_, pixels * 16},
-      {"stops", &stops_, pixels * 4},
-      {"adjoint", &adjoint_, pixels * 16},
-      {"fields", &fields_, pixels * 36},
-      {"tile_totals", &tileTotals_, static_cast<uint64_t>(tiles_) * 8},
-      {"raw", fixedPoint_ ? &rawFixed_ : &rawFloat_, n * kRawPerSplat * (fixedPoint_ ? 8u : 4u)},
-      {"counters", &counters_, kCounterWords * 4},
+      {"splats", &b.splats, n * sizeof(Splat)},
+      {"keys", &b.keys, n * 4},
+      {"values", &b.values, n * 4},
+      {"coverage", &b.coverage, n * 4},
+      {"entry_keys", &b.entryKeys, entries * 4},
+      {"entry_values", &b.entryValues, entries * 4},
+      {"ranges", &b.ranges, static_cast<uint64_t>(b.tiles) * 8},
+      {"image", &b.image, pixels * 16},
+      {"stops", &b.stops, pixels * 4},
+      {"adjoint", &b.adjoint, pixels * 16},
+      {"fields", &b.fields, pixels * 36},
+      {"tile_totals", &b.tileTotals, static_cast<uint64_t>(b.tiles) * 8},
+      {"raw", fixedPoint_ ? &b.rawFixed : &b.rawFloat, n * kRawPerSplat * (fixedPoint_ ? 8u : 4u)},
+      {"counters", &b.counters, kCounterWords * 4},
   };
   for (const Item &item : items) {
     if (name != item.name) continue;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <condition_variable>
+#include <memory>
 #include <cstdint>
 #include <mutex>
 #include <stdexcept>
@@ -61,6 +62,30 @@ private:
   size_t bytes_ = 0;
 };
 
+// Everything one view's passes write: the per-Gaussian, per-pixel and
+// per-entry buffers, the counters, and the stream the passes run on. The
+// single-view calls use the first; a batch uses one per view, so that the
+// views' passes can run side by side.
+struct ViewBuffers {
+  ViewBuffers() = default;
+  ViewBuffers(const ViewBuffers &) = delete;
+  ViewBuffers &operator=(const ViewBuffers &) = delete;
+  hipStream_t stream = nullptr;
+  bool ownsStream = false;
+  hipEvent_t done = nullptr; // recorded on `stream` after the view's backward pass
+  int width = 0, height = 0;
+  uint32_t gridW = 0, gridH = 0, tiles = 0, tileBits = 0;
+  uint64_t entryCapacity = 0;
+  // Per Gaussian.
+  DeviceArray splats, keys, values, tmpKeys, tmpValues, coverage, tileCount, tileRecords, rawFixed, rawFloat;
+  // Per tile entry.
+  DeviceArray entryKeys, entryValues, tmpEntryKeys, tmpEntryValues;
+  // Per pixel and per tile.
+  DeviceArray image, stops, adjoint, fields, ranges, tileTotals;
+  // Scratch and counters.
+  DeviceArray sortScratch, scanScratch, counters, sums;
+};
+
 class Trainer {
 public:
   Trainer(const vsplat_cloud &cloud, const vsplat_trainer_options &options);
@@ -76,6 +101,13 @@ public:
 
   void enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates);
   vsplat_step_stats synchronize();
+  // The batched step: n views' gradients summed, then (mode step) one Adam step.
+  void enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                     const vsplat_learning_rates *rates);
+  // Waits; each queued view's stats into stats[0 .. min(n, capacity)); returns n.
+  uint32_t synchronize_views(vsplat_step_stats *stats, uint32_t capacity, bool &overflow);
+  uint32_t step_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, const size_t *bytes, uint32_t mode,
+                      const vsplat_learning_rates *rates, vsplat_step_stats *stats);
   void reserve_entries(uint64_t entries);
 
   void gradients(float *positions, float *scales, float *rotations, float *opacities, float *sh) const;
@@ -102,12 +134,20 @@ public:
 
 private:
   ViewParams view_params(const vsplat_camera &camera) const;
-  void ensure_size(int width, int height);
-  void grow_entries(uint64_t entries);
+  ViewBuffers &view_buffers(size_t k); // the k-th view's buffers, made on first use
+  void ensure_size(ViewBuffers &b, int width, int height);
+  void grow_entries(ViewBuffers &b, uint64_t entries);
   void stage_target(const uint8_t *target, size_t bytes);
-  void enqueue_passes(const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates,
-                      bool timed);
-  void enqueue_chain(const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates);
+  // The passes up to the backward pass on b's stream, then (chain) the chain rule.
+  void enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode,
+                      const vsplat_learning_rates *rates, bool timed, bool chain = true);
+  void enqueue_chain(const ViewBuffers &b, const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates,
+                     hipStream_t stream, uint32_t *counters);
+  ChainParams chain_params(const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates) const;
+  ChainBuffers chain_buffers(const ViewBuffers &b) const;
+  void ensure_gradients();
+  void clear_overflow();
+  vsplat_step_stats make_stats(const uint32_t *counters, const double *sums, int width, int height) const;
   void begin_timing(bool iteration);
   void mark(const char *stage);
   void cancel_timing();
@@ -123,26 +163,27 @@ private:
   uint32_t adamStep_ = 0;
   float shDropout_ = 0.0f, dropoutNow_ = 0.0f;
   uint32_t shRetained_ = 0, shSeed_ = 0;
-  int width_ = 0, height_ = 0;
-  uint32_t gridW_ = 0, gridH_ = 0, tiles_ = 0, tileBits_ = 0;
-  uint64_t entryCapacity_ = 0;
   bool haveGradients_ = false;
   bool statisticsNext_ = true; // the next chain pass adds to the densification statistics
+  bool accumulateNext_ = false; // the next chain pass (gradients mode) adds to the gradient sums
+  // Per queued view of a batch: its counters and loss sums, copied on the device.
+  static constexpr size_t kViewStatsBytes = 48;
+  std::vector<std::pair<int, int>> queuedViewSizes_;
   ViewParams lastView_{};
 
   hipStream_t stream_ = nullptr;
   // Parameters and optimiser state, in GaussianCloud's layout; SH packed as halves.
   DeviceArray positions_, scales_, rotations_, opacities_, shWordsBuf_;
   DeviceArray state_[5], grad_[5];
-  // Per Gaussian.
-  DeviceArray splats_, keys_, values_, tmpKeys_, tmpValues_, coverage_, tileCount_, tileRecords_, rawFixed_, rawFloat_, shPass_, stats_, filter3d_;
+  // Per Gaussian, shared by the views.
+  DeviceArray shPass_, stats_, filter3d_;
   bool filterSet_ = false;
-  // Per tile entry.
-  DeviceArray entryKeys_, entryValues_, tmpEntryKeys_, tmpEntryValues_;
-  // Per pixel and per tile.
-  DeviceArray image_, stops_, adjoint_, fields_, target_, ranges_, tileTotals_;
-  // Scratch and counters.
-  DeviceArray sortScratch_, scanScratch_, counters_, sums_;
+  // The views' buffers; the first is the single-view calls' and runs on stream_.
+  std::vector<std::unique_ptr<ViewBuffers>> views_;
+  hipEvent_t batchStart_ = nullptr;
+  uint64_t reservedEntries_ = 0; // what reserve_entries asked for, for views made later
+  DeviceArray target_, viewStats_;
+  std::vector<std::unique_ptr<DeviceArray>> viewTargets_; // step_views' targets, one per view
   PinnedArray staging_, next_, readback_;
 
   // Prefetch worker (trainer.cpp:539-564): copies the next step's target into pinned memory.

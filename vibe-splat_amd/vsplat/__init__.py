@@ -34,7 +34,7 @@ EXPORTS = [
     "vsplat_uses_fixed_point", "vsplat_gpu_bytes", "vsplat_bytes_per_gaussian", "vsplat_staging_bytes",
     "vsplat_timings", "vsplat_last_error", "vsplat_enqueue", "vsplat_synchronize", "vsplat_stream",
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
-    "vsplat_set_stage_timing",
+    "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
 ]
 
 
@@ -135,6 +135,9 @@ def lib():
         L.vsplat_math_probe.argtypes = [C.c_int32, vp, vp, C.c_uint64]
         L.vsplat_reset_timings.argtypes = [vp]
         L.vsplat_set_stage_timing.argtypes = [vp, C.c_int32]
+        L.vsplat_step_views.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_int32, vp, vp]
+        L.vsplat_enqueue_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_int32, vp]
+        L.vsplat_synchronize_views.argtypes = [vp, vp, C.c_uint32, vp]
         _LIB = L
     return _LIB
 
@@ -388,6 +391,37 @@ class Trainer:
 
     def set_stage_timing(self, enabled: bool):
         _raise(self._lib.vsplat_set_stage_timing(self._h, int(bool(enabled))), "set_stage_timing")
+
+    # --- the batched step (no reference counterpart)
+    def step_views(self, cameras: list, targets: list, rates: LearningRates | None = None) -> list:
+        """One iteration over several views: gradients summed in view order, then
+        one Adam step when rates are given (else the sum stays for gradients())."""
+        n = len(cameras)
+        if n != len(targets):
+            raise ValueError("one target per camera")
+        cams = (CCamera * n)(*[camera_to_c(c) for c in cameras])
+        arrays = [np.ascontiguousarray(t, np.uint8) for t in targets]
+        ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in arrays])
+        sizes = (C.c_size_t * n)(*[a.nbytes for a in arrays])
+        stats = (CStats * n)()
+        mode = MODE_STEP if rates is not None else MODE_GRADIENTS
+        r = None if rates is None else rates.to_c()
+        _raise(self._lib.vsplat_step_views(self._h, n, cams, ptrs, sizes, mode, None if r is None else C.byref(r), stats),
+               "step_views")
+        return [StepStats.of(s) for s in stats]
+
+    def enqueue_views(self, cameras: list, device_target_ptrs: list, mode: int, rates: LearningRates | None = None):
+        n = len(cameras)
+        cams = (CCamera * n)(*[camera_to_c(c) for c in cameras])
+        ptrs = (C.c_void_p * n)(*device_target_ptrs)
+        r = None if rates is None else rates.to_c()
+        _raise(self._lib.vsplat_enqueue_views(self._h, n, cams, ptrs, mode, None if r is None else C.byref(r)), "enqueue_views")
+
+    def synchronize_views(self) -> list:
+        stats = (CStats * 64)()
+        count = C.c_uint32()
+        _raise(self._lib.vsplat_synchronize_views(self._h, stats, 64, C.byref(count)), "synchronize_views")
+        return [StepStats.of(stats[k]) for k in range(min(count.value, 64))]
 
     # --- device-resident path
     def enqueue(self, camera: Camera, device_target_ptr: int, mode: int, rates: LearningRates | None = None):
