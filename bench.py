@@ -124,7 +124,8 @@ def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
     try:
         trainer.step_views(w.cameras, w.targets, rates)  # sizes every buffer and the entry reservation
         single = []
-        for cam, ptr in zip(w.cameras, ptrs):
+        for cam, target, ptr in zip(w.cameras, w.targets, ptrs):
+            trainer.step(cam, target, rates)  # sizes the single-view buffers for this view
             for _ in range(warmup):
                 trainer.enqueue(cam, ptr, vsplat.MODE_STEP, rates)
             trainer.synchronize()

@@ -99,6 +99,16 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def adam_corrections(step: int):
+    """1 - beta^t for Adam's step t (trainer.cpp:467-468), beta^t kept in float32
+    by repeated multiplication, as the trainer keeps it on the device."""
+    b1, b2 = np.float32(1.0), np.float32(1.0)
+    for _ in range(step):
+        b1 = np.float32(b1 * np.float32(0.9))
+        b2 = np.float32(b2 * np.float32(0.999))
+    return float(np.float32(1.0) - b1), float(np.float32(1.0) - b2)
+
+
 def _check(code, what):
     if code != 0:
         raise RuntimeError(f"{what} failed with {code}")
@@ -378,8 +388,7 @@ class Model:
         o.rates[:] = [float(v) for v in rates]
         o.rest_scale = rest_scale
         o.beta1, o.beta2, o.epsilon = 0.9, 0.999, 1e-15
-        o.correction1 = float(np.float32(1.0) - np.float32(0.9) ** np.float32(step))
-        o.correction2 = float(np.float32(1.0) - np.float32(0.999) ** np.float32(step))
+        o.correction1, o.correction2 = adam_corrections(step)
         o.seed = step
         o.sh_floats = 3 * self.cloud.coefficients
         return o
@@ -440,8 +449,7 @@ def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, 
     opt.rates[:] = [float(v) for v in rates]
     opt.rest_scale = rest_scale
     opt.beta1, opt.beta2, opt.epsilon = 0.9, 0.999, 1e-15
-    opt.correction1 = float(np.float32(1.0) - np.float32(0.9) ** np.float32(step))
-    opt.correction2 = float(np.float32(1.0) - np.float32(0.999) ** np.float32(step))
+    opt.correction1, opt.correction2 = adam_corrections(step)
     opt.seed = step
     opt.sh_floats = 3 * cloud.coefficients
     _check(lib().ob_adam(_p(pos), _p(sc), _p(rot), _p(op), _p(w), *[_p(s) for s in states], *[_p(x) for x in g], n, w.shape[1],

@@ -38,6 +38,15 @@ __device__ __forceinline__ void step_parameter(const ChainParams &o, float &valu
   state = bf16_store(bf16_store(state, 0, bf16_round(m)), 1, bf16_stochastic(v, index, o.seed));
 }
 
+// The step about to be taken, from the device state when there is one.
+__device__ __forceinline__ void adam_step_now(ChainParams &o) {
+  if (o.adamState == nullptr) return;
+  const uint32_t taken = o.adamState[0];
+  o.correction1 = 1.0f - as_float(o.adamState[1]) * o.beta1;
+  o.correction2 = 1.0f - as_float(o.adamState[2]) * o.beta2;
+  o.seed = taken + 1u;
+}
+
 // Half floats rounded at random in proportion to the distance (train_step.glsl:409-419).
 __device__ __forceinline__ uint32_t half_stochastic(float x, uint32_t index, uint32_t seed) {
   const uint32_t nearest = float_to_half(x);
@@ -46,55 +55,6 @@ __device__ __forceinline__ uint32_t half_stochastic(float x, uint32_t index, uin
   const uint32_t other = fabsf(x) > fabsf(rounded) ? nearest + 1u : nearest - 1u;
   const float gap = half_to_float(other) - rounded;
   return train_uniform(index, seed) < (x - rounded) / gap ? other : nearest;
-}
-
-// d basis / d direction (train_types.glsl:126-153).
-__device__ __forceinline__ void sh_basis_gradient(float x, float y, float z, uint32_t degree, float g[16][3]) {
-  const float C1 = 0.4886025119029199f, A = 1.0925484305920792f, K = 0.31539156525252005f, L = 0.5462742152960396f;
-  const float C = -0.5900435899266435f, E = 2.890611442640554f, F = -0.4570457994644658f, G = 0.3731763325901154f,
-              H = 1.445305721320277f;
-  const float xx = x * x, yy = y * y, zz = z * z;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0f;
-  if (degree > 0) {
-    g[1][1] = -C1;
-    g[2][2] = C1;
-    g[3][0] = -C1;
-  }
-  if (degree > 1) {
-    g[4][0] = A * y;
-    g[4][1] = A * x;
-    g[5][1] = -A * z;
-    g[5][2] = -A * y;
-    g[6][0] = K * (-2.0f * x);
-    g[6][1] = K * (-2.0f * y);
-    g[6][2] = K * (4.0f * z);
-    g[7][0] = -A * z;
-    g[7][2] = -A * x;
-    g[8][0] = L * (2.0f * x);
-    g[8][1] = L * (-2.0f * y);
-  }
-  if (degree > 2) {
-    g[9][0] = C * (6.0f * x * y);
-    g[9][1] = C * (3.0f * xx - 3.0f * yy);
-    g[10][0] = E * (y * z);
-    g[10][1] = E * (x * z);
-    g[10][2] = E * (x * y);
-    g[11][0] = F * (-2.0f * x * y);
-    g[11][1] = F * (4.0f * zz - xx - 3.0f * yy);
-    g[11][2] = F * (8.0f * y * z);
-    g[12][0] = G * (-6.0f * x * z);
-    g[12][1] = G * (-6.0f * y * z);
-    g[12][2] = G * (6.0f * zz - 3.0f * xx - 3.0f * yy);
-    g[13][0] = F * (4.0f * zz - 3.0f * xx - yy);
-    g[13][1] = F * (-2.0f * x * y);
-    g[13][2] = F * (8.0f * x * z);
-    g[14][0] = H * (2.0f * x * z);
-    g[14][1] = H * (-2.0f * y * z);
-    g[14][2] = H * (xx - yy);
-    g[15][0] = C * (3.0f * xx - 3.0f * yy);
-    g[15][1] = C * (-6.0f * x * y);
-  }
 }
 
 // What becomes of gradient g of parameter `at` (train_gradient.comp:57-72).
@@ -111,7 +71,7 @@ __device__ __forceinline__ void emit(const ChainParams &o, float *values, uint32
   }
 }
 
-// Row k of sh_basis_gradient: the same values, for one basis function (k >= 1).
+// d basis_k / d direction for one basis function k >= 1 (train_types.glsl:126-153).
 __device__ __forceinline__ void sh_basis_gradient_term(uint32_t k, float x, float y, float z, float g[3]) {
   const float C1 = 0.4886025119029199f, A = 1.0925484305920792f, K = 0.31539156525252005f, L = 0.5462742152960396f;
   const float C = -0.5900435899266435f, E = 2.890611442640554f, F = -0.4570457994644658f, G = 0.3731763325901154f,
@@ -142,6 +102,7 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.count) return;
   if (counters[kOverflow] != 0) return; // a view did not fit: this pass computed nothing
+  if (o.mode == kModeStep) adam_step_now(o);
   float g[kRawPerSplat];
   if (o.fixedPoint) {
     const long long *raw = reinterpret_cast<const long long *>(B.rawFixed) + static_cast<size_t>(i) * kRawPerSplat;
@@ -430,6 +391,7 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
   const uint32_t words = c.shWords;
   if (w >= c.count * words) return;
   if (counters[kOverflow] != 0) return;
+  adam_step_now(o);
   const uint32_t i = w / words, q = w - i * words;
   const float *g = shPass + static_cast<size_t>(i) * kShPassFloats;
   const uint32_t used = as_uint(g[9]);
@@ -468,9 +430,10 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
 __global__ __launch_bounds__(256) void adam_kernel(ChainBuffers B, ViewParams c, ChainParams o, uint32_t *counters) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (counters[kOverflow] != 0) {
-    if (t == 0) counters[kSkipped] += 1; // this step did not happen: the host rolls its count back
+    if (t == 0) counters[kSkipped] += 1; // this step did not happen (the device step count stays)
     return;
   }
+  adam_step_now(o);
   const uint32_t n = c.count;
   if (t < 3 * n) {
     emit(o, B.positions, B.statePos, nullptr, t, B.gradPos[t], o.rates[0], 0);
@@ -492,6 +455,7 @@ __global__ __launch_bounds__(256) void sh_adam_kernel(uint32_t *__restrict__ shW
   const uint32_t words = c.shWords;
   if (w >= c.count * words) return;
   if (counters[kOverflow] != 0) return;
+  adam_step_now(o);
   const uint32_t i = w / words, q = w - i * words;
   const bool zeros = (o.flags & kFlagSelective) == 0;
   const uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
@@ -517,6 +481,14 @@ __global__ __launch_bounds__(256) void sh_adam_kernel(uint32_t *__restrict__ shW
   if (changed) shWords[w] = word;
 }
 
+// After a step's last kernel: one more step taken, unless the step was skipped.
+__global__ void advance_adam_kernel(uint32_t *state, float beta1, float beta2, const uint32_t *counters) {
+  if (counters[kOverflow] != 0) return;
+  state[0] += 1u;
+  state[1] = as_uint(as_float(state[1]) * beta1);
+  state[2] = as_uint(as_float(state[2]) * beta2);
+}
+
 } // namespace
 
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
@@ -533,6 +505,9 @@ void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewPar
     const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
     hipLaunchKernelGGL(sh_step_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
                        buffers.stateSh, buffers.shPass, view, params, counters);
+    if (params.adamState != nullptr) {
+      hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
+    }
   }
 }
 
@@ -545,6 +520,9 @@ void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewPara
   const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
   hipLaunchKernelGGL(sh_adam_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
                      buffers.stateSh, buffers.gradSh, view, params, counters);
+  if (params.adamState != nullptr) {
+    hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
+  }
 }
 
 } // namespace vsplat

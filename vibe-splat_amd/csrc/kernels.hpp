@@ -29,6 +29,12 @@ struct ChainParams {
   uint32_t seed, shFloats;
   uint32_t fixedPoint;
   float fixedScale;
+  // The optimiser's step count on the device: [steps taken, beta1^taken,
+  // beta2^taken] (the powers as float bits, kept by repeated multiplication).
+  // When set, a step takes its bias corrections and rounding seed from it
+  // rather than from correction1, correction2 and seed, so that a captured
+  // graph can replay steps.
+  uint32_t *adamState;
 };
 
 struct ChainBuffers {

@@ -136,6 +136,9 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   filter3d_.ensure(n * 4);
   launch_fill_u32(stream_, filter3d_.as<uint32_t>(), n, 0u);
   readback_.ensure(kCounterWords * 4 + 2 * sizeof(double));
+  adamState_.ensure(4 * 4);
+  const uint32_t adamStart[4] = {0u, 0x3F800000u, 0x3F800000u, 0u}; // no steps; beta^0 = 1
+  check(hipMemcpy(adamState_.as<uint32_t>(), adamStart, sizeof(adamStart), hipMemcpyHostToDevice), "adam state");
   check(hipEventCreateWithFlags(&batchStart_, hipEventDisableTiming), "hipEventCreate");
   view_buffers(0);
   check(hipStreamSynchronize(stream_), "trainer set-up");
@@ -405,17 +408,16 @@ ChainParams Trainer::chain_params(const ViewParams &v, uint32_t mode, const vspl
   p.beta1 = 0.9f;
   p.beta2 = 0.999f;
   p.epsilon = 1e-15f; // trainer.cpp:23
+  p.adamState = adamState_.as<uint32_t>();
   if (mode == kModeStep) {
-    const uint32_t step = adamStep_ + 1;
     p.rates[0] = rates->position;
     p.rates[1] = rates->scale;
     p.rates[2] = rates->rotation;
     p.rates[3] = rates->opacity;
     p.rates[4] = rates->sh;
     p.restScale = rates->sh_rest_scale;
-    p.correction1 = 1.0f - std::pow(0.9f, static_cast<float>(step)); // trainer.cpp:467-468
-    p.correction2 = 1.0f - std::pow(0.999f, static_cast<float>(step));
-    p.seed = step;
+    // The bias corrections 1 - beta^t (trainer.cpp:467-468) come from the
+    // device's step count, beta^t kept by repeated multiplication.
   }
   p.shFloats = shFloats_;
   p.fixedPoint = fixedPoint_ ? 1u : 0u;
@@ -461,7 +463,7 @@ void Trainer::enqueue_chain(const ViewBuffers &b, const ViewParams &v, uint32_t 
   launch_chain(stream, chain_buffers(b), v, chain_params(v, mode, rates), counters);
 }
 
-vsplat_step_stats Trainer::read_stats(bool &overflow, uint32_t *skipped) {
+vsplat_step_stats Trainer::read_stats(bool &overflow) {
   ViewBuffers &b = *views_[0];
   uint8_t *host = readback_.as<uint8_t>();
   check(hipMemcpyAsync(host, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToHost, stream_), "read counters");
@@ -476,7 +478,6 @@ vsplat_step_stats Trainer::read_stats(bool &overflow, uint32_t *skipped) {
     throw Error(VSPLAT_DEVICE_ERROR, "a prefix-sum wait on the GPU timed out; the step's results are not valid");
   }
   overflow = counters[kOverflow] != 0;
-  if (skipped != nullptr) *skipped = counters[kSkipped];
   return make_stats(counters, sums, b.width, b.height);
 }
 
@@ -563,7 +564,6 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
   }
   drain_timing();
   if (mode == kModeGradients) haveGradients_ = true;
-  if (mode == kModeStep) ++adamStep_;
   lastView_ = v;
   return stats;
 }
@@ -598,7 +598,6 @@ void Trainer::apply_gradients(const vsplat_learning_rates &rates) {
   mark("adam step");
   check(hipStreamSynchronize(stream_), "apply gradients");
   drain_timing();
-  ++adamStep_;
 }
 
 void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates) {
@@ -612,18 +611,16 @@ void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, 
   const ViewParams v = view_params(camera);
   enqueue_passes(b, v, deviceTarget, mode, rates, timeQueued_);
   if (mode == kModeGradients) haveGradients_ = true;
-  if (mode == kModeStep) ++adamStep_;
   lastView_ = v;
 }
 
 vsplat_step_stats Trainer::synchronize() {
   bool overflow = false;
-  uint32_t skipped = 0;
-  const vsplat_step_stats stats = read_stats(overflow, &skipped);
+  const vsplat_step_stats stats = read_stats(overflow);
   drain_timing();
   if (overflow) {
-    // Adam's step count (and so its bias corrections) covers only the steps that ran.
-    adamStep_ -= skipped < adamStep_ ? skipped : adamStep_;
+    // Adam's step count on the device (and so its bias corrections) covers
+    // only the steps that ran: a skipped step does not advance it.
     clear_overflow();
     throw Error(VSPLAT_RUNTIME_ERROR, "a queued view needed more tile entries than were reserved; it and the views queued after it were skipped");
   }
@@ -691,13 +688,11 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
   lastView_ = params[n - 1];
   if (mode == kModeStep) {
     launch_adam(stream_, chain_buffers(*views_[0]), lastView_, chain_params(lastView_, kModeStep, rates), status);
-    ++adamStep_;
   }
 }
 
 uint32_t Trainer::synchronize_views(vsplat_step_stats *stats, uint32_t capacity, bool &overflow) {
-  uint32_t skipped = 0;
-  (void)read_stats(overflow, &skipped);
+  (void)read_stats(overflow);
   drain_timing();
   const uint32_t n = static_cast<uint32_t>(queuedViewSizes_.size());
   std::vector<uint8_t> host(static_cast<size_t>(n) * kViewStatsBytes);
@@ -709,10 +704,7 @@ uint32_t Trainer::synchronize_views(vsplat_step_stats *stats, uint32_t capacity,
     std::memcpy(sums, host.data() + static_cast<size_t>(k) * kViewStatsBytes + kCounterWords * 4, sizeof(sums));
     stats[k] = make_stats(counters, sums, queuedViewSizes_[k].first, queuedViewSizes_[k].second);
   }
-  if (overflow) {
-    adamStep_ -= skipped < adamStep_ ? skipped : adamStep_;
-    clear_overflow();
-  }
+  if (overflow) clear_overflow();
   return n;
 }
 

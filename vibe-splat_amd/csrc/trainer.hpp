@@ -153,14 +153,13 @@ private:
   void cancel_timing();
   void drain_timing();
   void wait_prefetch();
-  vsplat_step_stats read_stats(bool &overflow, uint32_t *skipped = nullptr);
+  vsplat_step_stats read_stats(bool &overflow);
 
   vsplat_trainer_options options_;
   uint32_t count_ = 0;
   int shDegree_ = 0, activeShDegree_ = 0;
   uint32_t shFloats_ = 3, shWords_ = 3;
   bool fixedPoint_ = true;
-  uint32_t adamStep_ = 0;
   float shDropout_ = 0.0f, dropoutNow_ = 0.0f;
   uint32_t shRetained_ = 0, shSeed_ = 0;
   bool haveGradients_ = false;
@@ -175,6 +174,7 @@ private:
   // Parameters and optimiser state, in GaussianCloud's layout; SH packed as halves.
   DeviceArray positions_, scales_, rotations_, opacities_, shWordsBuf_;
   DeviceArray state_[5], grad_[5];
+  DeviceArray adamState_; // steps taken, beta1^steps, beta2^steps (ChainParams::adamState)
   // Per Gaussian, shared by the views.
   DeviceArray shPass_, stats_, filter3d_;
   bool filterSet_ = false;
