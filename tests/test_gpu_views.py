@@ -5,6 +5,7 @@ the GPU against Oracle B:
   gradients from the FP32 model, added in view order in float32; the
   densification statistics likewise; each view's visible count, entries and
   loss;
+* the same over more views than one chain-rule launch takes;
 * a batch of one view steps exactly as vsplat_step does;
 * a batch of two views steps as Oracle B's Adam from the summed gradients;
 * the queued form with device targets gives the same result.
@@ -29,13 +30,13 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-def _ring(count=2000, degree=3, seed=77):
-    """A small version of the eight-view ring: three views of different sizes."""
+def _ring(count=2000, degree=3, seed=77, sizes=SIZES):
+    """A small version of the eight-view ring: views of different sizes."""
     cloud = F.synthetic_cloud(count, degree, seed, sh_sigma=0.3)
     up = (0.0, -1.0, 0.0)
     cams, targets = [], []
-    for k, (w, h) in enumerate(SIZES):
-        th = 2.0 * math.pi * k / 8.0
+    for k, (w, h) in enumerate(sizes):
+        th = 2.0 * math.pi * k / len(sizes)
         cams.append(F.look_at((3.0 * math.sin(th), 0.0, -3.0 * math.cos(th)), (0, 0, 0), up, 60.0, w, h))
         targets.append(F.smooth_target_explicit(w, h, seed * 16 + k))
     return cloud, cams, targets
@@ -72,6 +73,20 @@ def test_gradients_summed_over_views_bit_exact():
     for s, (visible, entries, l1, ssim) in zip(stats, per_view):
         assert s.visible_splats == visible and s.tile_entries == entries
         assert abs(s.l1 - l1) < 1e-6 and abs(s.ssim - ssim) < 1e-6
+
+
+def test_more_views_than_one_chain_launch_takes():
+    """Nine views: the chain rule runs in two chunks (eight views, then one
+    continuing the sums), which must not change the sums."""
+    cloud, cams, targets = _ring(1500, 2, 91, sizes=[(48, 32), (64, 40), (40, 40)] * 3)
+    want, want_stats, _, _ = _model_sum(cloud, cams, targets)
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+        t.step_views(cams, targets)
+        g = t.gradients()
+        dens = t.densify_stats()
+    for got, w in zip((g.positions, g.scales, g.rotations, g.opacities, g.sh), want):
+        np.testing.assert_array_equal(_bits(got.reshape(-1)), _bits(w))
+    np.testing.assert_array_equal(_bits(dens), _bits(want_stats))
 
 
 def test_batch_of_one_steps_as_step_does():

@@ -97,75 +97,69 @@ __device__ __forceinline__ void sh_basis_gradient_term(uint32_t k, float x, floa
   }
 }
 
-template <int Degree>
-__global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.count) return;
-  if (counters[kOverflow] != 0) return; // a view did not fit: this pass computed nothing
-  if (o.mode == kModeStep) adam_step_now(o);
-  float g[kRawPerSplat];
-  if (o.fixedPoint) {
-    const long long *raw = reinterpret_cast<const long long *>(B.rawFixed) + static_cast<size_t>(i) * kRawPerSplat;
-    const double inverse = 1.0 / static_cast<double>(o.fixedScale);
+// The fixed-point or float sums of one Gaussian in one view, as floats.
+__device__ __forceinline__ bool load_raw(const int64_t *rawFixed, const float *rawFloat, float fixedScale, uint32_t i,
+                                         float g[kRawPerSplat]) {
+  if (rawFixed != nullptr) {
+    const long long *raw = reinterpret_cast<const long long *>(rawFixed) + static_cast<size_t>(i) * kRawPerSplat;
+    const double inverse = 1.0 / static_cast<double>(fixedScale);
 #pragma unroll
     for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = static_cast<float>(static_cast<double>(raw[k]) * inverse);
   } else {
-    const float *raw = B.rawFloat + static_cast<size_t>(i) * kRawPerSplat;
+    const float *raw = rawFloat + static_cast<size_t>(i) * kRawPerSplat;
 #pragma unroll
     for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = raw[k];
   }
   bool touched = false;
 #pragma unroll
   for (uint32_t k = 0; k < kRawPerSplat; ++k) touched = touched || g[k] != 0.0f;
-  const bool zeros = o.mode == kModeGradients || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0);
-  const bool penalties = o.opacityReg > 0.0f || o.scaleReg > 0.0f;
-  const float W = static_cast<float>(c.width), H = static_cast<float>(c.height);
-  float *shPass = B.shPass + static_cast<size_t>(i) * kShPassFloats;
+  return touched;
+}
 
-  if ((touched && (o.flags & kFlagStatistics) != 0) && B.stats != nullptr) {
-    // DensifyStat (train_gradient.comp:94-101), per half image rather than per pixel.
-    float4 *st = reinterpret_cast<float4 *>(B.stats + static_cast<size_t>(i) * 8);
-    float4 a = st[0], b = st[1];
-    const float ax = (g[1] * 0.5f) * W, ay = (g[2] * 0.5f) * H;
-    a.x = a.x + sqrtf(fma32(ay, ay, ax * ax));
-    a.y = a.y + 1.0f;
-    a.z = maxf(a.z, B.splats[i].radius);
-    a.w = fma32((g[9] * 3.0f) * W, H, a.w);
-    const float bx = (g[10] * 0.5f) * W, by = (g[11] * 0.5f) * H;
-    b.x = b.x + sqrtf(fma32(by, by, bx * bx));
-    st[0] = a;
-    st[1] = b;
+// DensifyStat (train_gradient.comp:94-101), per half image rather than per pixel.
+__device__ __forceinline__ void add_statistics(float4 &a, float4 &b, const float g[kRawPerSplat], float W, float H, float radius) {
+  const float ax = (g[1] * 0.5f) * W, ay = (g[2] * 0.5f) * H;
+  a.x = a.x + sqrtf(fma32(ay, ay, ax * ax));
+  a.y = a.y + 1.0f;
+  a.z = maxf(a.z, radius);
+  a.w = fma32((g[9] * 3.0f) * W, H, a.w);
+  const float bx = (g[10] * 0.5f) * W, by = (g[11] * 0.5f) * H;
+  b.x = b.x + sqrtf(fma32(by, by, bx * bx));
+}
+
+// What one view gives the geometric parameters.
+struct ViewTerms {
+  float position[3], scale[3], rotation[4], opacity;
+};
+
+// The penalties' part alone, for a Gaussian no pixel of the view saw.
+__device__ __forceinline__ void untouched_terms(const ChainBuffers &B, const ChainParams &o, uint32_t i, bool penalties, ViewTerms &t) {
+  const float opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
+  const float op = penalties ? o.opacityReg * opacity * (1.0f - opacity) : 0.0f;
+#pragma unroll
+  for (uint32_t a = 0; a < 3; ++a) {
+    t.position[a] = 0.0f;
+    t.scale[a] = penalties ? o.scaleReg * vs_exp(B.scales[i * 3 + a]) : 0.0f;
   }
-  if (o.mode == kModeStatistics || (!touched && !zeros && !penalties)) {
-    if (o.mode == kModeStep) shPass[9] = as_float(0u); // nothing for the SH pass
-    return;
-  }
+#pragma unroll
+  for (uint32_t a = 0; a < 4; ++a) t.rotation[a] = 0.0f;
+  t.opacity = op;
+}
+
+// The chain rule for a Gaussian the view saw (train_gradient.comp:105-315):
+// from the screen-space sums g to the geometric terms, with the SH part handed
+// to sh(basis, colour gradient, direction, coefficients used) as soon as it is
+// known. Reads the parameters only.
+template <int Degree, typename ShSink>
+__device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+                                           const float g[kRawPerSplat], ViewTerms &t, ShSink &&sh) {
   const float logScale[3] = {B.scales[i * 3], B.scales[i * 3 + 1], B.scales[i * 3 + 2]};
   const float opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
   const float s2[3] = {vs_exp(2.0f * logScale[0]), vs_exp(2.0f * logScale[1]), vs_exp(2.0f * logScale[2])};
-  if (!touched) {
-    // Nothing from the pixels; the penalties still apply.
-    const float op = penalties ? o.opacityReg * opacity * (1.0f - opacity) : 0.0f;
-    for (uint32_t a = 0; a < 3; ++a) {
-      emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, 0.0f, o.rates[0], 0);
-      emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, penalties ? o.scaleReg * vs_exp(logScale[a]) : 0.0f, o.rates[1], 1);
-    }
-    for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, 0.0f, o.rates[2], 2);
-    emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, op, o.rates[3], 3);
-    if (o.mode == kModeGradients) {
-      float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
-      const bool add = (o.flags & kFlagAccumulate) != 0;
-      for (uint32_t j = 0; j < o.shFloats; ++j) out[j] = add ? out[j] + 0.0f : 0.0f;
-    } else {
-      shPass[9] = as_float(0u);
-    }
-    return;
-  }
-
   const float px = B.positions[i * 3], py = B.positions[i * 3 + 1], pz = B.positions[i * 3 + 2];
   // Colour, clamped at zero, and the view direction's part of the position
   // gradient, first: the SH work's temporaries are dead before the geometry's
-  // begin, which keeps the register count down. Only reads so far.
+  // begin, which keeps the register count down.
   const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
   const float dist = sqrtf(fma32(ddz, ddz, fma32(ddy, ddy, ddx * ddx)));
   const float idist = 1.0f / dist;
@@ -204,29 +198,7 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
 #pragma unroll
     for (int a = 0; a < 3; ++a) dirTerm[a] = fma32(-dir[a], dd, gDir[a]) * idist;
   }
-
-  if (o.mode == kModeGradients) {
-    // Coefficients above the active degree get zero (they are stored, not trained).
-    const uint32_t stored = o.shFloats / 3;
-    float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
-#pragma unroll
-    for (uint32_t k = 0; k < kCoefficients; ++k)
-#pragma unroll
-      for (uint32_t ch = 0; ch < 3; ++ch) {
-        const float v = k < used ? basis[k] * gcu[ch] : 0.0f;
-        out[k * 3 + ch] = (o.flags & kFlagAccumulate) != 0 ? out[k * 3 + ch] + v : v;
-      }
-    for (uint32_t j = kCoefficients * 3; j < stored * 3; ++j) out[j] = (o.flags & kFlagAccumulate) != 0 ? out[j] + 0.0f : 0.0f;
-  } else {
-    // For the SH pass: the colour gradient, the direction and the coefficients used.
-    shPass[0] = gcu[0];
-    shPass[1] = gcu[1];
-    shPass[2] = gcu[2];
-    shPass[3] = dir[0];
-    shPass[4] = dir[1];
-    shPass[5] = dir[2];
-    shPass[9] = as_float(used);
-  }
+  sh(basis, gcu, dir, used);
 
   // Keep the scheduler from hoisting the geometry's work into the SH section.
   __builtin_amdgcn_sched_barrier(0);
@@ -322,9 +294,8 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   gT0 = fma32(2.0f, trX, gT0);
   gT1 = fma32(2.0f, trY, gT1);
   gT2 = fma32(2.0f, trZ, gT2);
-  float gP[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) gP[k] = dot3(V[k * 4 + 0], V[k * 4 + 1], V[k * 4 + 2], gT0, gT1, gT2);
+  for (int k = 0; k < 3; ++k) t.position[k] = dot3(V[k * 4 + 0], V[k * 4 + 1], V[k * 4 + 2], gT0, gT1, gT2);
 
   // The 3D covariance: G3 = T^T G T, dL = tr(G3 dSigma).
   float GT[2][3];
@@ -338,7 +309,6 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int k = 0; k < 3; ++k) G3[r][k] = fma32(T[1][r], GT[1][k], T[0][r] * GT[0][k]);
-  float dScale[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     float v[3];
@@ -346,7 +316,7 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
     for (int r = 0; r < 3; ++r) v[r] = dot3(G3[r][0], G3[r][1], G3[r][2], R[0][a], R[1][a], R[2][a]);
     const float quad = dot3(R[0][a], R[1][a], R[2][a], v[0], v[1], v[2]);
     const float filterTerm = f != 0.0f ? ((((g[0] * k2) * opacity) * k3) * f) / (s2[a] + f) : 0.0f;
-    dScale[a] = fma32(o.scaleReg, vs_exp(logScale[a]), fma32(2.0f * s2[a], quad, filterTerm));
+    t.scale[a] = fma32(o.scaleReg, vs_exp(logScale[a]), fma32(2.0f * s2[a], quad, filterTerm));
   }
   // Rotation: P = G3 R D, dL/dq = 2 <2P, dR/dq> written out per component.
   float P[3][3];
@@ -364,24 +334,181 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
   const float qv[4] = {w, x, y, z};
   const float denom = qlen > 1e-12f ? qlen : 1e-12f;
   const float idenom = 1.0f / denom;
-  float tangent[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) tangent[k] = fma32(-qv[k], qd, gq[k]) * idenom;
+  for (int k = 0; k < 4; ++k) t.rotation[k] = fma32(-qv[k], qd, gq[k]) * idenom;
 
-  const float dOpacity = (fma32(g[0] * k2, k3, o.opacityReg) * opacity) * (1.0f - opacity);
+  t.opacity = (fma32(g[0] * k2, k3, o.opacityReg) * opacity) * (1.0f - opacity);
 
   if (Degree > 0) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) gP[a] = gP[a] + dirTerm[a];
+    for (int a = 0; a < 3; ++a) t.position[a] = t.position[a] + dirTerm[a];
   }
+}
 
-  // Everything above read the parameters before any of them changes.
+__device__ __forceinline__ void emit_terms(const ChainBuffers &B, const ChainParams &o, uint32_t i, const ViewTerms &t) {
   for (uint32_t a = 0; a < 3; ++a) {
-    emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, gP[a], o.rates[0], 0);
-    emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, dScale[a], o.rates[1], 1);
+    emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, t.position[a], o.rates[0], 0);
+    emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, t.scale[a], o.rates[1], 1);
   }
-  for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, tangent[a], o.rates[2], 2);
-  emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, dOpacity, o.rates[3], 3);
+  for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, t.rotation[a], o.rates[2], 2);
+  emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, t.opacity, o.rates[3], 3);
+}
+
+template <int Degree>
+__global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.count) return;
+  if (counters[kOverflow] != 0) return; // a view did not fit: this pass computed nothing
+  if (o.mode == kModeStep) adam_step_now(o);
+  float g[kRawPerSplat];
+  const bool touched = load_raw(o.fixedPoint ? B.rawFixed : nullptr, B.rawFloat, o.fixedScale, i, g);
+  const bool zeros = o.mode == kModeGradients || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0);
+  const bool penalties = o.opacityReg > 0.0f || o.scaleReg > 0.0f;
+  float *shPass = B.shPass + static_cast<size_t>(i) * kShPassFloats;
+
+  if ((touched && (o.flags & kFlagStatistics) != 0) && B.stats != nullptr) {
+    float4 *st = reinterpret_cast<float4 *>(B.stats + static_cast<size_t>(i) * 8);
+    float4 a = st[0], b = st[1];
+    add_statistics(a, b, g, static_cast<float>(c.width), static_cast<float>(c.height), B.splats[i].radius);
+    st[0] = a;
+    st[1] = b;
+  }
+  if (o.mode == kModeStatistics || (!touched && !zeros && !penalties)) {
+    if (o.mode == kModeStep) shPass[9] = as_float(0u); // nothing for the SH pass
+    return;
+  }
+  ViewTerms t;
+  if (!touched) {
+    // Nothing from the pixels; the penalties still apply.
+    untouched_terms(B, o, i, penalties, t);
+    emit_terms(B, o, i, t);
+    if (o.mode == kModeGradients) {
+      float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
+      const bool add = (o.flags & kFlagAccumulate) != 0;
+      for (uint32_t j = 0; j < o.shFloats; ++j) out[j] = add ? out[j] + 0.0f : 0.0f;
+    } else {
+      shPass[9] = as_float(0u);
+    }
+    return;
+  }
+  view_terms<Degree>(B, c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float dir[3], uint32_t used) {
+    constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
+    if (o.mode == kModeGradients) {
+      // Coefficients above the active degree get zero (they are stored, not trained).
+      const uint32_t stored = o.shFloats / 3;
+      float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
+#pragma unroll
+      for (uint32_t k = 0; k < kCoefficients; ++k)
+#pragma unroll
+        for (uint32_t ch = 0; ch < 3; ++ch) {
+          const float v = k < used ? basis[k] * gcu[ch] : 0.0f;
+          out[k * 3 + ch] = (o.flags & kFlagAccumulate) != 0 ? out[k * 3 + ch] + v : v;
+        }
+      for (uint32_t j = kCoefficients * 3; j < stored * 3; ++j) out[j] = (o.flags & kFlagAccumulate) != 0 ? out[j] + 0.0f : 0.0f;
+    } else {
+      // For the SH pass: the colour gradient, the direction and the coefficients used.
+      shPass[0] = gcu[0];
+      shPass[1] = gcu[1];
+      shPass[2] = gcu[2];
+      shPass[3] = dir[0];
+      shPass[4] = dir[1];
+      shPass[5] = dir[2];
+      shPass[9] = as_float(used);
+    }
+  });
+  // Everything above read the parameters before any of them changes.
+  emit_terms(B, o, i, t);
+}
+
+// The batched chain rule (no reference counterpart): for each Gaussian, every
+// view of the batch in turn, each view's terms added to the sums in view order
+// in float32 (the first view's are the sums' start), so the result is the
+// per-view chain rule's gradients added up in that order. The parameters are
+// read once per view from cache; the geometric sums stay in registers and the
+// SH sums in LDS, one row per coefficient so that a wave's lanes hit
+// consecutive banks. In gradients mode the sums are stored; in step mode the
+// geometric parameters take Adam's step from them here, and the SH sums are
+// stored for the SH step.
+template <int Degree>
+__global__ __launch_bounds__(256) void chain_views_kernel(ChainBuffers B, ChainViews views, ChainParams o, const uint32_t *counters) {
+  constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
+  __shared__ float shSum[kCoefficients * 3][256];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = blockIdx.x * blockDim.x + lane;
+  if (i >= views.view[0].c.count) return;
+  if (counters[kOverflow] != 0) return;
+  if (o.mode == kModeStep) adam_step_now(o);
+  const bool penalties = o.opacityReg > 0.0f || o.scaleReg > 0.0f;
+  const bool statistics = (o.flags & kFlagStatistics) != 0 && B.stats != nullptr;
+  // A later chunk of a batch continues from the sums the earlier ones stored.
+  ViewTerms sum;
+  bool started = (o.flags & kFlagAccumulate) != 0;
+  if (started) {
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+      sum.position[k] = B.gradPos[i * 3 + k];
+      sum.scale[k] = B.gradScale[i * 3 + k];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) sum.rotation[k] = B.gradRot[i * 4 + k];
+    sum.opacity = B.gradOpacity[i];
+    const float *prev = B.gradSh + static_cast<size_t>(i) * o.shFloats;
+#pragma unroll
+    for (uint32_t j = 0; j < kCoefficients * 3; ++j) shSum[j][lane] = prev[j];
+  }
+  o.flags &= ~kFlagAccumulate; // the sums below are complete: they are stored, not added
+  float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+  bool statsChanged = false;
+  for (uint32_t v = 0; v < views.count; ++v) {
+    const ChainView &w = views.view[v];
+    float g[kRawPerSplat];
+    const bool touched = load_raw(w.rawFixed, w.rawFloat, w.fixedScale, i, g);
+    if (touched && statistics) {
+      if (!statsChanged) {
+        const float4 *st = reinterpret_cast<const float4 *>(B.stats + static_cast<size_t>(i) * 8);
+        a = st[0];
+        b = st[1];
+        statsChanged = true;
+      }
+      add_statistics(a, b, g, static_cast<float>(w.c.width), static_cast<float>(w.c.height), w.splats[i].radius);
+    }
+    ViewTerms t;
+    if (touched) {
+      view_terms<Degree>(B, w.c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float[3], uint32_t used) {
+#pragma unroll
+        for (uint32_t k = 0; k < kCoefficients; ++k)
+#pragma unroll
+          for (uint32_t ch = 0; ch < 3; ++ch) {
+            const float x = k < used ? basis[k] * gcu[ch] : 0.0f;
+            shSum[k * 3 + ch][lane] = started ? shSum[k * 3 + ch][lane] + x : x;
+          }
+      });
+    } else {
+      untouched_terms(B, o, i, penalties, t);
+#pragma unroll
+      for (uint32_t j = 0; j < kCoefficients * 3; ++j) shSum[j][lane] = started ? shSum[j][lane] + 0.0f : 0.0f;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+      sum.position[k] = started ? sum.position[k] + t.position[k] : t.position[k];
+      sum.scale[k] = started ? sum.scale[k] + t.scale[k] : t.scale[k];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) sum.rotation[k] = started ? sum.rotation[k] + t.rotation[k] : t.rotation[k];
+    sum.opacity = started ? sum.opacity + t.opacity : t.opacity;
+    started = true;
+  }
+  if (statsChanged) {
+    float4 *st = reinterpret_cast<float4 *>(B.stats + static_cast<size_t>(i) * 8);
+    st[0] = a;
+    st[1] = b;
+  }
+  // Coefficients above the active degree get zero (they are stored, not trained).
+  float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
+#pragma unroll
+  for (uint32_t j = 0; j < kCoefficients * 3; ++j) out[j] = shSum[j][lane];
+  for (uint32_t j = kCoefficients * 3; j < o.shFloats; ++j) out[j] = 0.0f;
+  emit_terms(B, o, i, sum);
 }
 
 __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shWords, uint32_t *__restrict__ stateSh,
@@ -508,6 +635,30 @@ void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewPar
     if (params.adamState != nullptr) {
       hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
     }
+  }
+}
+
+void launch_chain_views(hipStream_t stream, const ChainBuffers &buffers, const ChainViews &views, const ChainParams &params,
+                        const uint32_t *counters) {
+  const uint32_t count = views.view[0].c.count;
+  if (count == 0 || views.count == 0) return;
+  const dim3 grid((count + 255) / 256), block(256);
+  switch (views.view[0].c.shDegree) {
+  case 0: hipLaunchKernelGGL(chain_views_kernel<0>, grid, block, 0, stream, buffers, views, params, counters); break;
+  case 1: hipLaunchKernelGGL(chain_views_kernel<1>, grid, block, 0, stream, buffers, views, params, counters); break;
+  case 2: hipLaunchKernelGGL(chain_views_kernel<2>, grid, block, 0, stream, buffers, views, params, counters); break;
+  default: hipLaunchKernelGGL(chain_views_kernel<3>, grid, block, 0, stream, buffers, views, params, counters); break;
+  }
+}
+
+void launch_sh_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
+                    const uint32_t *counters) {
+  if (view.count == 0) return;
+  const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
+  hipLaunchKernelGGL(sh_adam_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
+                     buffers.stateSh, buffers.gradSh, view, params, counters);
+  if (params.adamState != nullptr) {
+    hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
   }
 }
 

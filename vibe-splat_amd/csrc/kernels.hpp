@@ -50,6 +50,22 @@ struct ChainBuffers {
   float *gradPos, *gradScale, *gradRot, *gradOpacity, *gradSh;
 };
 
+// One view of a batched chain rule: its camera, its screen-space sums (fixed
+// point with its scale, or float when rawFixed is null) and its splats.
+struct ChainView {
+  ViewParams c;
+  const int64_t *rawFixed;
+  const float *rawFloat;
+  const Splat *splats;
+  float fixedScale;
+};
+constexpr uint32_t kChainViewsPerLaunch = 8;
+// Up to kChainViewsPerLaunch views, passed by value as the kernel's argument.
+struct ChainViews {
+  ChainView view[kChainViewsPerLaunch];
+  uint32_t count;
+};
+
 struct TileRecord;
 
 // Projection and tile coverage: splats, depth keys and values (the identity),
@@ -87,8 +103,17 @@ void launch_backward(hipStream_t stream, bool fixedPoint, const Splat *splats, c
 void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
                  float *fields, float4 *adjoint, float2 *tileTotals, double *sums);
 
-// One Adam step of every parameter from the summed gradients in buffers.grad*
-// (the batched step); params.mode is kModeStep.
+// The chain rule over the views of a batch chunk, summed per Gaussian in view
+// order (kFlagAccumulate: continuing from the sums an earlier chunk stored).
+// Gradients mode stores the sums; step mode takes the geometric parameters'
+// Adam step from them and stores the SH sums for launch_sh_adam.
+void launch_chain_views(hipStream_t stream, const ChainBuffers &buffers, const ChainViews &views, const ChainParams &params,
+                        const uint32_t *counters);
+// Adam for the SH coefficients from the stored SH gradients, then the step count advanced.
+void launch_sh_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
+                    const uint32_t *counters);
+// One Adam step of every parameter from the stored gradients in buffers.grad*
+// (Trainer::apply_gradients); params.mode is kModeStep.
 void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
                  uint32_t *counters);
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,

@@ -19,6 +19,9 @@
 
 namespace vsplat {
 
+// The fixed-point scale 3 W H 64 (trainer.cpp:586-591).
+static float fixed_scale(const ViewParams &v) { return 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f; }
+
 namespace {
 
 void check(hipError_t e, const char *what) {
@@ -384,7 +387,7 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &v, const uint8_t 
   mark("loss");
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
   // a resolution of 1/64 (trainer.cpp:586-591).
-  const float fixedScale = 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f;
+  const float fixedScale = fixed_scale(v);
   launch_backward(s, fixedPoint_, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height,
                   options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
                   b.rawFloat.as<float>());
@@ -421,7 +424,7 @@ ChainParams Trainer::chain_params(const ViewParams &v, uint32_t mode, const vspl
   }
   p.shFloats = shFloats_;
   p.fixedPoint = fixedPoint_ ? 1u : 0u;
-  p.fixedScale = 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f;
+  p.fixedScale = fixed_scale(v);
   return p;
 }
 
@@ -679,15 +682,32 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
     check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
     launch_merge_flags(stream_, status, views_[k]->counters.as<uint32_t>());
   }
-  for (uint32_t k = 0; k < n; ++k) {
-    accumulateNext_ = k > 0;
-    enqueue_chain(*views_[k], params[k], kModeGradients, nullptr, stream_, status);
+  // The chain rule over the views in chunks of kChainViewsPerLaunch, each
+  // chunk continuing the sums of the one before; the last takes the step.
+  const ChainBuffers buffers = chain_buffers(*views_[0]);
+  for (uint32_t first = 0; first < n; first += kChainViewsPerLaunch) {
+    const uint32_t count = std::min(n - first, kChainViewsPerLaunch);
+    const bool last = first + count == n;
+    ChainViews chunk{};
+    chunk.count = count;
+    for (uint32_t k = 0; k < count; ++k) {
+      ChainView &w = chunk.view[k];
+      const ViewBuffers &b = *views_[first + k];
+      w.c = params[first + k];
+      w.rawFixed = fixedPoint_ ? b.rawFixed.as<int64_t>() : nullptr;
+      w.rawFloat = b.rawFloat.as<float>();
+      w.splats = b.splats.as<Splat>();
+      w.fixedScale = fixed_scale(w.c);
+    }
+    accumulateNext_ = first > 0;
+    launch_chain_views(stream_, buffers, chunk, chain_params(params[first], last ? mode : kModeGradients, rates), status);
   }
   accumulateNext_ = false;
-  haveGradients_ = true;
   lastView_ = params[n - 1];
   if (mode == kModeStep) {
-    launch_adam(stream_, chain_buffers(*views_[0]), lastView_, chain_params(lastView_, kModeStep, rates), status);
+    launch_sh_adam(stream_, buffers, lastView_, chain_params(lastView_, kModeStep, rates), status);
+  } else {
+    haveGradients_ = true;
   }
 }
 
