@@ -142,6 +142,15 @@ def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
             trainer.enqueue_views(w.cameras, ptrs, vsplat.MODE_STEP, rates)
         trainer.synchronize_views()
         batch = (time.perf_counter() - t0) * 1000.0 / steps
+        # The same iteration captured once as a HIP graph and replayed.
+        trainer.capture_views(w.cameras, ptrs, vsplat.MODE_STEP, rates)
+        trainer.launch_captured(warmup)
+        trainer.synchronize_views()
+        t0 = time.perf_counter()
+        trainer.launch_captured(steps)
+        trainer.synchronize_views()
+        graph = (time.perf_counter() - t0) * 1000.0 / steps
+        trainer.release_captured()
     finally:
         for b in buffers:
             b.free()
@@ -150,7 +159,9 @@ def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
     return {"workload": w.description, "views": views, "sizes": [[c.width, c.height] for c in w.cameras],
             "single_view_ms": [round(x, 4) for x in single], "single_views_total_ms": round(sum(single), 4),
             "batched_ms": round(batch, 4), "batched_ms_per_view": round(batch / views, 4),
-            "speedup_per_view": round(sum(single) / batch, 4), "sh_storage": "DC fp32, rest fp16 (the reference's trainer layout)"}
+            "speedup_per_view": round(sum(single) / batch, 4),
+            "graph_ms": round(graph, 4), "graph_ms_per_view": round(graph / views, 4),
+            "graph_speedup_per_view": round(sum(single) / graph, 4), "sh_storage": "DC fp32, rest fp16 (the reference's trainer layout)"}
 
 
 def main():
@@ -252,7 +263,9 @@ def main():
     if not args.no_batched:
         log("batched form: the eight-view ring")
         result["batched"] = batched_ring(max(5, args.steps // 2), 2, args.count)
-        log(f"  per view {result['batched']['batched_ms_per_view']} ms batched, speedup {result['batched']['speedup_per_view']}")
+        bt = result["batched"]
+        log(f"  per view {bt['batched_ms_per_view']} ms batched (speedup {bt['speedup_per_view']}), "
+            f"{bt['graph_ms_per_view']} ms as a graph (speedup {bt['graph_speedup_per_view']})")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline on {threads} threads")

@@ -205,6 +205,24 @@ int vsplat_enqueue_views(vsplat_trainer *trainer, uint32_t n, const vsplat_camer
  * VSPLAT_RUNTIME_ERROR, as vsplat_synchronize, if a view did not fit: the batch
  * changed nothing. */
 int vsplat_synchronize_views(vsplat_trainer *trainer, vsplat_step_stats *stats, uint32_t capacity, uint32_t *count);
+/* Captures the batch vsplat_enqueue_views would queue as a HIP graph (replacing
+ * any captured before), without running it. The trainer's buffers are sized
+ * for it first; the tile-entry reservation is the one of the moment (a
+ * vsplat_step_views over the same views sizes it). Each replay is one training
+ * iteration with the captured cameras, device targets (which must stay valid)
+ * and rates; Adam's step count and bias corrections advance on the device, so
+ * replays continue the optimisation. */
+int vsplat_capture_views(vsplat_trainer *trainer, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *device_targets,
+                         int32_t mode, const vsplat_learning_rates *rates);
+/* Queues the captured batch `replays` times on the trainer's stream; wait with
+ * vsplat_synchronize_views (the last replay's stats; VSPLAT_RUNTIME_ERROR if a
+ * view did not fit, in which case that replay and every later one changed
+ * nothing). VSPLAT_LOGIC_ERROR if nothing is captured, or if a buffer the graph
+ * uses was reallocated or a setting baked into it (active SH degree, SH
+ * dropout, 3D filter) changed since the capture. */
+int vsplat_launch_captured(vsplat_trainer *trainer, uint32_t replays);
+/* Frees the captured graph. */
+int vsplat_release_captured(vsplat_trainer *trainer);
 
 /* ---- Introspection, for parity tests: no reference counterpart. ---- */
 

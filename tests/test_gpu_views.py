@@ -8,7 +8,9 @@ the GPU against Oracle B:
 * the same over more views than one chain-rule launch takes;
 * a batch of one view steps exactly as vsplat_step does;
 * a batch of two views steps as Oracle B's Adam from the summed gradients;
-* the queued form with device targets gives the same result.
+* the queued form with device targets gives the same result;
+* a batch captured as a HIP graph and replayed steps as the queued form does,
+  replay after replay, and refuses to replay once a buffer it uses is remade.
 """
 import math
 
@@ -133,3 +135,43 @@ def test_queued_batch_matches_the_synchronous_one():
             d.free()
     for x, y in zip((ca.positions, ca.opacities, ca.sh), (cb.positions, cb.opacities, cb.sh)):
         np.testing.assert_array_equal(_bits(x), _bits(y))
+
+
+def test_captured_batch_replays_as_queued_steps():
+    cloud, cams, targets = _ring()
+    options = vsplat.TrainerOptions(force_fixed_point=True)
+    buffers = [vsplat.DeviceBuffer(x) for x in targets]
+    ptrs = [d.ptr for d in buffers]
+    try:
+        with vsplat.Trainer(cloud, options) as a, vsplat.Trainer(cloud, options) as b:
+            for t in (a, b):
+                t.reserve_entries(1 << 16)
+            b.capture_views(cams, ptrs, vsplat.MODE_STEP, RATES)
+            for replay in range(3):
+                a.enqueue_views(cams, ptrs, vsplat.MODE_STEP, RATES)
+                want = a.synchronize_views()
+                b.launch_captured()
+                got = b.synchronize_views()
+                assert [s.tile_entries for s in got] == [s.tile_entries for s in want]
+                assert [s.loss for s in got] == [s.loss for s in want]
+                ca, cb = a.cloud(), b.cloud()
+                for x, y in zip((ca.positions, ca.scales, ca.rotations, ca.opacities, ca.sh),
+                                (cb.positions, cb.scales, cb.rotations, cb.opacities, cb.sh)):
+                    np.testing.assert_array_equal(_bits(x), _bits(y), err_msg=f"replay {replay}")
+            # Two replays queued back to back are two more steps.
+            for _ in range(2):
+                a.enqueue_views(cams, ptrs, vsplat.MODE_STEP, RATES)
+            a.synchronize_views()
+            b.launch_captured(2)
+            b.synchronize_views()
+            ca, cb = a.cloud(), b.cloud()
+            np.testing.assert_array_equal(_bits(ca.positions), _bits(cb.positions))
+            np.testing.assert_array_equal(_bits(ca.sh), _bits(cb.sh))
+            # A larger view remakes the first view's per-pixel buffers: the graph is stale.
+            big = F.look_at((0.0, 0.0, -3.0), (0, 0, 0), (0.0, -1.0, 0.0), 60.0, 160, 120)
+            b.render(big)
+            with pytest.raises(vsplat.LogicError, match="capture it again"):
+                b.launch_captured()
+    finally:
+        for d in buffers:
+            d.free()

@@ -209,6 +209,22 @@ int vsplat_synchronize_views(vsplat_trainer *t, vsplat_step_stats *stats, uint32
   });
 }
 
+int vsplat_capture_views(vsplat_trainer *t, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *device_targets, int32_t mode,
+                         const vsplat_learning_rates *rates) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.capture_views(n, cameras, device_targets, static_cast<uint32_t>(mode), rates); });
+}
+
+int vsplat_launch_captured(vsplat_trainer *t, uint32_t replays) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.launch_captured(replays); });
+}
+
+int vsplat_release_captured(vsplat_trainer *t) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.release_captured(); });
+}
+
 void *vsplat_stream(vsplat_trainer *t) { return t != nullptr ? static_cast<void *>(t->impl.stream()) : nullptr; }
 
 int vsplat_reserve_entries(vsplat_trainer *t, uint64_t entries) {

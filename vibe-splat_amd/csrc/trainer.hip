@@ -154,6 +154,7 @@ Trainer::~Trainer() {
   }
   prefetchChanged_.notify_all();
   if (prefetchWorker_.joinable()) prefetchWorker_.join();
+  release_captured();
   if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
   for (auto &v : views_) {
     if (v->ownsStream) {
@@ -645,7 +646,9 @@ void Trainer::clear_overflow() {
 // one bias correction. A view that does not fit skips the whole batch: its
 // overflow word is or-ed into the first view's, which the chain rule and the
 // Adam step check.
-void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+// Checks a batch's arguments and makes every buffer it needs (so that queueing
+// it allocates nothing).
+void Trainer::prepare_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
                             const vsplat_learning_rates *rates) {
   if (n == 0 || cameras == nullptr || targets == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a batch needs views and targets");
   if (mode != kModeGradients && mode != kModeStep) throw Error(VSPLAT_INVALID_ARGUMENT, "mode must be 1 (gradients) or 2 (step)");
@@ -657,6 +660,11 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
   }
   ensure_gradients();
   viewStats_.ensure(static_cast<size_t>(n) * kViewStatsBytes);
+}
+
+void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                            const vsplat_learning_rates *rates) {
+  prepare_views(n, cameras, targets, mode, rates);
   queuedViewSizes_.clear();
   dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
   statisticsNext_ = true;
@@ -709,6 +717,99 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
   } else {
     haveGradients_ = true;
   }
+}
+
+// What a captured batch depends on: every buffer it names (address and size)
+// and the settings baked into its kernels' arguments.
+uint64_t Trainer::capture_key(uint32_t n) const {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](uint64_t x) {
+    for (int k = 0; k < 8; ++k) {
+      h ^= (x >> (8 * k)) & 0xFFu;
+      h *= 1099511628211ull;
+    }
+  };
+  auto array = [&mix](const DeviceArray &a) {
+    mix(reinterpret_cast<uintptr_t>(a.as<void>()));
+    mix(a.bytes());
+  };
+  for (const DeviceArray *a : {&positions_, &scales_, &rotations_, &opacities_, &shWordsBuf_, &adamState_, &shPass_, &stats_, &filter3d_,
+                               &viewStats_})
+    array(*a);
+  for (int g = 0; g < 5; ++g) {
+    array(state_[g]);
+    array(grad_[g]);
+  }
+  mix(n);
+  for (uint32_t k = 0; k < n && k < views_.size(); ++k) {
+    const ViewBuffers &b = *views_[k];
+    for (const DeviceArray *a : {&b.splats, &b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount, &b.tileRecords,
+                                 &b.rawFixed, &b.rawFloat, &b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues, &b.image,
+                                 &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.sortScratch, &b.scanScratch, &b.counters,
+                                 &b.sums})
+      array(*a);
+    mix(b.entryCapacity);
+    mix(static_cast<uint64_t>(b.width) << 32 | static_cast<uint32_t>(b.height));
+  }
+  mix(count_);
+  mix(static_cast<uint64_t>(activeShDegree_));
+  mix(filterSet_ ? 1u : 0u);
+  uint32_t dropout;
+  std::memcpy(&dropout, &shDropout_, 4);
+  mix(dropout);
+  mix(static_cast<uint64_t>(shRetained_) << 32 | shSeed_);
+  return h;
+}
+
+void Trainer::release_captured() {
+  if (graph_ != nullptr) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipGraphExecDestroy(graph_);
+  }
+  graph_ = nullptr;
+  graphViewSizes_.clear();
+}
+
+// Captures one batch (enqueue_views) as a HIP graph on the trainer's stream.
+// Everything is allocated first: nothing may be allocated while a stream is
+// being captured. Replays run the batch exactly as captured: the same cameras,
+// targets and rates, the tile-entry reservation of the moment; Adam's step
+// count and bias corrections come from the device, so each replay is the next
+// step.
+void Trainer::capture_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                            const vsplat_learning_rates *rates) {
+  release_captured();
+  prepare_views(n, cameras, targets, mode, rates);
+  check(hipStreamSynchronize(stream_), "before capture");
+  check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+  hipGraph_t graph = nullptr;
+  try {
+    enqueue_views(n, cameras, targets, mode, rates);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &graph);
+    if (graph != nullptr) (void)hipGraphDestroy(graph);
+    queuedViewSizes_.clear();
+    throw;
+  }
+  check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  check(e, "hipGraphInstantiate");
+  graph_ = exec;
+  graphViews_ = n;
+  graphViewSizes_ = queuedViewSizes_;
+  graphKey_ = capture_key(n);
+  queuedViewSizes_.clear();
+}
+
+void Trainer::launch_captured(uint32_t replays) {
+  if (graph_ == nullptr) throw Error(VSPLAT_LOGIC_ERROR, "no batch is captured");
+  if (capture_key(graphViews_) != graphKey_) {
+    throw Error(VSPLAT_LOGIC_ERROR, "the trainer's buffers or settings changed since the batch was captured: capture it again");
+  }
+  for (uint32_t r = 0; r < replays; ++r) check(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
+  if (replays > 0) queuedViewSizes_ = graphViewSizes_;
 }
 
 uint32_t Trainer::synchronize_views(vsplat_step_stats *stats, uint32_t capacity, bool &overflow) {

@@ -109,6 +109,12 @@ public:
   uint32_t step_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, const size_t *bytes, uint32_t mode,
                       const vsplat_learning_rates *rates, vsplat_step_stats *stats);
   void reserve_entries(uint64_t entries);
+  // A batch captured as a HIP graph, replayed with launch_captured and waited
+  // for with synchronize_views.
+  void capture_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                     const vsplat_learning_rates *rates);
+  void launch_captured(uint32_t replays);
+  void release_captured();
 
   void gradients(float *positions, float *scales, float *rotations, float *opacities, float *sh) const;
   void cloud(float *positions, float *scales, float *rotations, float *opacities, float *sh) const;
@@ -146,6 +152,9 @@ private:
   ChainParams chain_params(const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates) const;
   ChainBuffers chain_buffers(const ViewBuffers &b) const;
   void ensure_gradients();
+  void prepare_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                     const vsplat_learning_rates *rates);
+  uint64_t capture_key(uint32_t n) const;
   void clear_overflow();
   vsplat_step_stats make_stats(const uint32_t *counters, const double *sums, int width, int height) const;
   void begin_timing(bool iteration);
@@ -181,6 +190,12 @@ private:
   // The views' buffers; the first is the single-view calls' and runs on stream_.
   std::vector<std::unique_ptr<ViewBuffers>> views_;
   hipEvent_t batchStart_ = nullptr;
+  // The captured batch: its executable graph, view count and sizes, and the
+  // capture_key it was made under.
+  hipGraphExec_t graph_ = nullptr;
+  uint32_t graphViews_ = 0;
+  std::vector<std::pair<int, int>> graphViewSizes_;
+  uint64_t graphKey_ = 0;
   uint64_t reservedEntries_ = 0; // what reserve_entries asked for, for views made later
   DeviceArray target_, viewStats_;
   std::vector<std::unique_ptr<DeviceArray>> viewTargets_; // step_views' targets, one per view

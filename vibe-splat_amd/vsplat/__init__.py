@@ -35,6 +35,7 @@ EXPORTS = [
     "vsplat_timings", "vsplat_last_error", "vsplat_enqueue", "vsplat_synchronize", "vsplat_stream",
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
+    "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
 ]
 
 
@@ -138,6 +139,9 @@ def lib():
         L.vsplat_step_views.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_int32, vp, vp]
         L.vsplat_enqueue_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_int32, vp]
         L.vsplat_synchronize_views.argtypes = [vp, vp, C.c_uint32, vp]
+        L.vsplat_capture_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_int32, vp]
+        L.vsplat_launch_captured.argtypes = [vp, C.c_uint32]
+        L.vsplat_release_captured.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -422,6 +426,21 @@ class Trainer:
         count = C.c_uint32()
         _raise(self._lib.vsplat_synchronize_views(self._h, stats, 64, C.byref(count)), "synchronize_views")
         return [StepStats.of(stats[k]) for k in range(min(count.value, 64))]
+
+    def capture_views(self, cameras: list, device_target_ptrs: list, mode: int, rates: LearningRates | None = None):
+        """Captures the batch enqueue_views would queue as a HIP graph (not run)."""
+        n = len(cameras)
+        cams = (CCamera * n)(*[camera_to_c(c) for c in cameras])
+        ptrs = (C.c_void_p * n)(*device_target_ptrs)
+        r = None if rates is None else rates.to_c()
+        _raise(self._lib.vsplat_capture_views(self._h, n, cams, ptrs, mode, None if r is None else C.byref(r)), "capture_views")
+
+    def launch_captured(self, replays: int = 1):
+        """Queues the captured batch `replays` times; wait with synchronize_views."""
+        _raise(self._lib.vsplat_launch_captured(self._h, replays), "launch_captured")
+
+    def release_captured(self):
+        _raise(self._lib.vsplat_release_captured(self._h), "release_captured")
 
     # --- device-resident path
     def enqueue(self, camera: Camera, device_target_ptr: int, mode: int, rates: LearningRates | None = None):
