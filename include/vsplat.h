@@ -68,7 +68,14 @@ typedef struct {
   int32_t mip_filter_2d;
   int32_t pose_gradients; /* must be 0: VSPLAT_UNSUPPORTED otherwise */
   uint64_t memory_budget; /* bytes; UINT64_MAX for none */
+  /* How the SH coefficients are stored on the GPU (no reference counterpart:
+   * the reference always uses VSPLAT_SH_HALF). VSPLAT_SH_BF16 stores every
+   * coefficient as bf16 (96 bytes per Gaussian at degree 3 instead of 104);
+   * arithmetic stays fp32, and a step rounds the stored values at random. */
+  int32_t sh_storage;
 } vsplat_trainer_options;
+
+enum { VSPLAT_SH_HALF = 0, VSPLAT_SH_BF16 = 1 };
 
 /* LearningRates (src/train/trainer.hpp:51-61). trust_region must be 0 (first-order Adam only). */
 typedef struct {

@@ -90,6 +90,7 @@ struct TrainerOptions {
   bool mipFilter2d = false;
   bool poseGradients = false;
   std::uint64_t memoryBudget = ~std::uint64_t{0};
+  bool bf16Sh = false; // this build only: SH stored as bf16 (vsplat_trainer_options::sh_storage)
 };
 
 // LearningRates (trainer.hpp:51-61).
@@ -153,6 +154,7 @@ public:
     o.mip_filter_2d = options.mipFilter2d ? 1 : 0;
     o.pose_gradients = options.poseGradients ? 1 : 0;
     o.memory_budget = options.memoryBudget;
+    o.sh_storage = options.bf16Sh ? VSPLAT_SH_BF16 : VSPLAT_SH_HALF;
     detail::check(vsplat_trainer_create(&c, &o, &trainer_));
   }
   ~Trainer() { vsplat_trainer_destroy(trainer_); }

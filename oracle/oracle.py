@@ -47,7 +47,8 @@ class OBCamera(C.Structure):
     _fields_ = [("view", C.c_float * 16), ("position", C.c_float * 3), ("near_plane", C.c_float), ("fx", C.c_float),
                 ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float), ("tan_half_x", C.c_float),
                 ("tan_half_y", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("count", C.c_uint32),
-                ("sh_degree", C.c_uint32), ("sh_words", C.c_uint32), ("sh_coefficients", C.c_uint32)]
+                ("sh_degree", C.c_uint32), ("sh_words", C.c_uint32), ("sh_coefficients", C.c_uint32),
+                ("sh_format", C.c_uint32)]
 
 
 class OBPreOptions(C.Structure):
@@ -87,7 +88,7 @@ def lib():
         L.ob_fixed_to_float.argtypes = [vp, C.c_uint64, C.c_float, vp]
         L.ob_chain.argtypes = [vp] * 21
         L.ob_sh_step.argtypes = [vp] * 5
-        L.ob_adam.argtypes = [vp] * 15 + [C.c_uint32, C.c_uint32, vp]
+        L.ob_adam.argtypes = [vp] * 15 + [C.c_uint32, C.c_uint32, C.c_uint32, vp]
         L.ob_exp_array.argtypes = [vp, vp, C.c_uint64]
         L.ob_log_array.argtypes = [vp, vp, C.c_uint64]
         L.ob_exp_array.restype = None
@@ -260,9 +261,9 @@ def log32(x):
     return y
 
 
-def ob_camera(cam, count, sh_degree, active_degree=None, near_plane=0.01):
+def ob_camera(cam, count, sh_degree, active_degree=None, near_plane=0.01, sh_storage=0):
     """The per-view constants block, as the product's host code fills it."""
-    from vsplat.fixtures import half_sh_words  # noqa: WPS433 (tests put the package on sys.path)
+    from vsplat.fixtures import sh_words  # noqa: WPS433 (tests put the package on sys.path)
     active = sh_degree if active_degree is None else active_degree
     c = OBCamera()
     c.view[:] = [float(v) for v in cam.view_column_major()]
@@ -274,8 +275,9 @@ def ob_camera(cam, count, sh_degree, active_degree=None, near_plane=0.01):
     c.width, c.height = cam.width, cam.height
     c.count = count
     c.sh_degree = active
-    c.sh_words = half_sh_words((sh_degree + 1) ** 2)
+    c.sh_words = sh_words((sh_degree + 1) ** 2, sh_storage)
     c.sh_coefficients = (active + 1) ** 2
+    c.sh_format = sh_storage
     return c
 
 
@@ -303,13 +305,14 @@ class Model:
     """Oracle B over one cloud and camera: each stage as the GPU runs it."""
 
     def __init__(self, cloud, cam, *, near_plane=0.01, mip_filter_2d=False, filter3d=None, background=(0, 0, 0),
-                 active_degree=None, sh_dropout=(0.0, 0, 0)):
-        from vsplat.fixtures import pack_half_sh
+                 active_degree=None, sh_dropout=(0.0, 0, 0), sh_storage=0):
+        from vsplat.fixtures import pack_sh
         self.cloud = cloud
         self.cam = cam
         self.n = cloud.count
-        self.words = pack_half_sh(cloud.sh)
-        self.camera = ob_camera(cam, self.n, cloud.sh_degree, active_degree, near_plane)
+        self.sh_storage = sh_storage
+        self.words = pack_sh(cloud.sh, sh_storage)
+        self.camera = ob_camera(cam, self.n, cloud.sh_degree, active_degree, near_plane, sh_storage)
         self.pre = OBPreOptions(int(mip_filter_2d), float(sh_dropout[0]), int(sh_dropout[1]), int(sh_dropout[2]))
         self.mip = mip_filter_2d
         self.filter3d = None if filter3d is None else np.ascontiguousarray(filter3d, np.float32)
@@ -432,7 +435,7 @@ class Model:
         return pos, sc, rot, op, words
 
 
-def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, selective=True):
+def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, selective=True, sh_storage=0):
     """The batched step's Adam (ob_adam) from zero optimiser state: the cloud's
     parameters (SH as packed words) after one step with the summed gradients
     grads = (positions, scales, rotations, opacities, sh) as flat float32 arrays."""
@@ -453,5 +456,5 @@ def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, 
     opt.seed = step
     opt.sh_floats = 3 * cloud.coefficients
     _check(lib().ob_adam(_p(pos), _p(sc), _p(rot), _p(op), _p(w), *[_p(s) for s in states], *[_p(x) for x in g], n, w.shape[1],
-                         C.byref(opt)), "ob_adam")
+                         sh_storage, C.byref(opt)), "ob_adam")
     return pos, sc, rot, op, w

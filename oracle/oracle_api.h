@@ -74,6 +74,7 @@ typedef struct {
   float tan_half_x, tan_half_y; /* 0.5 W / fx, 0.5 H / fy, computed in float */
   uint32_t width, height;
   uint32_t count, sh_degree, sh_words, sh_coefficients;
+  uint32_t sh_format; /* 0: DC float + rest halves (the reference's layout); 1: all bf16 */
 } ob_camera;
 
 typedef struct {
@@ -148,7 +149,7 @@ int ob_sh_step(uint32_t *sh_words, uint32_t *state_sh, const float *raw, const o
 int ob_adam(float *positions, float *scales, float *rotations, float *opacities, uint32_t *sh_words, uint32_t *state_pos,
             uint32_t *state_scale, uint32_t *state_rot, uint32_t *state_opacity, uint32_t *state_sh, const float *g_pos,
             const float *g_scale, const float *g_rot, const float *g_opacity, const float *g_sh, uint32_t count,
-            uint32_t sh_words_per, const ob_chain_options *opt);
+            uint32_t sh_words_per, uint32_t sh_format, const ob_chain_options *opt);
 
 /* The exp and log the model uses, for the math tests. */
 void ob_exp_array(const float *x, float *y, uint64_t n);

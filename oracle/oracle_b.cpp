@@ -77,8 +77,13 @@ std::uint32_t float_to_half(float f) {
   return sign | ((rounded - 0x38000000u) >> 13);
 }
 
-// The SH element j of Gaussian `base`'s words: DC as floats, the rest as halves (splat_types.glsl:305-310).
-float sh_value(const std::uint32_t *words, std::uint32_t base, std::uint32_t j) {
+// The SH element j of Gaussian `base`'s words: DC as floats, the rest as halves
+// (splat_types.glsl:305-310); or, format 1, every coefficient a bf16, two to a word.
+float sh_value(const std::uint32_t *words, std::uint32_t base, std::uint32_t j, std::uint32_t format) {
+  if (format == 1) {
+    const std::uint32_t w = words[base + j / 2];
+    return bits_to_float((j & 1u) ? (w & 0xFFFF0000u) : (w << 16));
+  }
   if (j < 3) return bits_to_float(words[base + j]);
   const std::uint32_t w = words[base + 3 + (j - 3) / 2];
   return half_to_float(((j - 3) & 1u) ? (w >> 16) : (w & 0xFFFFu));
@@ -450,7 +455,7 @@ extern "C" int ob_preprocess(const float *positions, const float *scales, const 
     const std::uint32_t base = i * c.sh_words;
     float col[3] = {0.5f, 0.5f, 0.5f};
     for (std::uint32_t k = 0; k < used; ++k)
-      for (std::uint32_t ch = 0; ch < 3; ++ch) col[ch] = fma(basis[k], sh_value(shWords, base, k * 3 + ch), col[ch]);
+      for (std::uint32_t ch = 0; ch < 3; ++ch) col[ch] = fma(basis[k], sh_value(shWords, base, k * 3 + ch, c.sh_format), col[ch]);
     if (!(std::fabs(col[0]) < 1e30f && std::fabs(col[1]) < 1e30f && std::fabs(col[2]) < 1e30f)) continue;
     const float reach = reach_of(opacityUsed);
     const float ex = std::sqrt(reach * a), ey = std::sqrt(reach * cc);
@@ -1048,7 +1053,7 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
     const std::uint32_t used = sh_coefficients_used(i, c.sh_coefficients, o.sh_dropout, o.sh_retained, o.sh_seed);
     float colour[3] = {0.5f, 0.5f, 0.5f};
     for (std::uint32_t k = 0; k < used; ++k)
-      for (std::uint32_t ch = 0; ch < 3; ++ch) colour[ch] = fma(basis[k], sh_value(shWords, base, k * 3 + ch), colour[ch]);
+      for (std::uint32_t ch = 0; ch < 3; ++ch) colour[ch] = fma(basis[k], sh_value(shWords, base, k * 3 + ch, c.sh_format), colour[ch]);
     const float unclamped[3] = {colour[0] > 0.0f ? 1.0f : 0.0f, colour[1] > 0.0f ? 1.0f : 0.0f, colour[2] > 0.0f ? 1.0f : 0.0f};
     const float gcu[3] = {gColour[0] * unclamped[0], gColour[1] * unclamped[1], gColour[2] * unclamped[2]};
     if (c.sh_degree > 0) {
@@ -1056,8 +1061,8 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
       sh_basis_gradient(dir[0], dir[1], dir[2], c.sh_degree, bg);
       float gDir[3] = {0.0f, 0.0f, 0.0f};
       for (std::uint32_t k = 1; k < used; ++k) {
-        const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value(shWords, base, k * 3), sh_value(shWords, base, k * 3 + 1),
-                             sh_value(shWords, base, k * 3 + 2));
+        const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value(shWords, base, k * 3, c.sh_format),
+                             sh_value(shWords, base, k * 3 + 1, c.sh_format), sh_value(shWords, base, k * 3 + 2, c.sh_format));
         for (int a = 0; a < 3; ++a) gDir[a] = fma(d, bg[k][a], gDir[a]);
       }
       const float dd = dot3(dir[0], dir[1], dir[2], gDir[0], gDir[1], gDir[2]);
@@ -1090,6 +1095,34 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
 }
 
 // train_sh_step.comp:19-70 (first-order Adam).
+// The SH floats packed word q holds, and one of them read from and stored
+// back into the word: halves (and the DC floats as they are) in the
+// reference's layout, bf16 in format 1; stored values are rounded at random.
+void sh_word_floats(std::uint32_t format, std::uint32_t q, std::uint32_t shFloats, std::uint32_t &first, std::uint32_t &last) {
+  if (format == 1) {
+    first = 2 * q;
+  } else {
+    first = q < 3 ? q : 3 + 2 * (q - 3);
+    if (q < 3) {
+      last = q;
+      return;
+    }
+  }
+  last = std::min(first + 1, shFloats - 1);
+}
+float sh_word_get(std::uint32_t format, std::uint32_t word, std::uint32_t j, std::uint32_t first) {
+  if (format == 1) return bf16_load(word, j - first);
+  if (j < 3) return bits_to_float(word);
+  return half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+}
+std::uint32_t sh_word_put(std::uint32_t format, std::uint32_t word, std::uint32_t j, std::uint32_t first, float value, std::uint32_t s,
+                          std::uint32_t seed) {
+  if (format == 1) return bf16_store(word, j - first, bf16_stochastic(value, s, seed ^ 0x5DEECE66u));
+  if (j < 3) return float_to_bits(value);
+  const std::uint32_t shift = 16u * ((j - 3) & 1u);
+  return (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, seed ^ 0x5DEECE66u) << shift);
+}
+
 extern "C" int ob_sh_step(uint32_t *shWords, uint32_t *stateSh, const float *raw, const ob_camera *cam, const ob_chain_options *opt) {
   const ob_camera &c = *cam;
   const ob_chain_options &o = *opt;
@@ -1103,8 +1136,8 @@ extern "C" int ob_sh_step(uint32_t *shWords, uint32_t *stateSh, const float *raw
     if (used == 0 && !zeros) continue;
     float basis[16];
     sh_basis(g[3], g[4], g[5], c.sh_degree, basis);
-    const std::uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
-    const std::uint32_t last = q < 3 ? q : std::min(first + 1, o.sh_floats - 1);
+    std::uint32_t first, last;
+    sh_word_floats(c.sh_format, q, o.sh_floats, first, last);
     std::uint32_t word = shWords[w];
     bool changed = false;
     for (std::uint32_t j = first; j <= last; ++j) {
@@ -1113,15 +1146,10 @@ extern "C" int ob_sh_step(uint32_t *shWords, uint32_t *stateSh, const float *raw
       if (gv == 0.0f && !zeros) continue;
       const std::uint32_t s = i * o.sh_floats + j;
       std::uint32_t state = stateSh[s];
-      float value = j < 3 ? bits_to_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+      float value = sh_word_get(c.sh_format, word, j, first);
       step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.rest_scale, s * 5u + 4u);
       stateSh[s] = state;
-      if (j < 3) {
-        word = float_to_bits(value);
-      } else {
-        const std::uint32_t shift = 16u * ((j - 3) & 1u);
-        word = (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, o.seed ^ 0x5DEECE66u) << shift);
-      }
+      word = sh_word_put(c.sh_format, word, j, first, value, s, o.seed);
       changed = true;
     }
     if (changed) shWords[w] = word;
@@ -1135,7 +1163,7 @@ extern "C" int ob_sh_step(uint32_t *shWords, uint32_t *stateSh, const float *raw
 extern "C" int ob_adam(float *positions, float *scales, float *rotations, float *opacities, uint32_t *shWords, uint32_t *statePos,
                        uint32_t *stateScale, uint32_t *stateRot, uint32_t *stateOpacity, uint32_t *stateSh, const float *gPos,
                        const float *gScale, const float *gRot, const float *gOpacity, const float *gSh, uint32_t count,
-                       uint32_t shWordsPer, const ob_chain_options *opt) {
+                       uint32_t shWordsPer, uint32_t shFormat, const ob_chain_options *opt) {
   const ob_chain_options &o = *opt;
   const bool zeros = (o.flags & kFlagSelective) == 0;
   const auto step = [&](float *values, std::uint32_t *state, const float *grad, std::uint32_t n, float rate, std::uint32_t group) {
@@ -1155,8 +1183,8 @@ extern "C" int ob_adam(float *positions, float *scales, float *rotations, float 
   step(opacities, stateOpacity, gOpacity, count, o.rates[3], 3);
   for (std::uint32_t w = 0; w < count * shWordsPer; ++w) {
     const std::uint32_t i = w / shWordsPer, q = w - i * shWordsPer;
-    const std::uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
-    const std::uint32_t last = q < 3 ? q : std::min(first + 1, o.sh_floats - 1);
+    std::uint32_t first, last;
+    sh_word_floats(shFormat, q, o.sh_floats, first, last);
     std::uint32_t word = shWords[w];
     bool changed = false;
     for (std::uint32_t j = first; j <= last; ++j) {
@@ -1164,15 +1192,10 @@ extern "C" int ob_adam(float *positions, float *scales, float *rotations, float 
       const float gv = gSh[s];
       if (gv == 0.0f && !zeros) continue;
       std::uint32_t state = stateSh[s];
-      float value = j < 3 ? bits_to_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+      float value = sh_word_get(shFormat, word, j, first);
       step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.rest_scale, s * 5u + 4u);
       stateSh[s] = state;
-      if (j < 3) {
-        word = float_to_bits(value);
-      } else {
-        const std::uint32_t shift = 16u * ((j - 3) & 1u);
-        word = (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, o.seed ^ 0x5DEECE66u) << shift);
-      }
+      word = sh_word_put(shFormat, word, j, first, value, s, o.seed);
       changed = true;
     }
     if (changed) shWords[w] = word;

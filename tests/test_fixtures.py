@@ -76,3 +76,31 @@ def test_configs_are_the_stated_workloads():
     assert w1.cloud.sh_degree == 3 and (w1.cameras[0].width, w1.cameras[0].height) == (1280, 720) and w1.fixed_point
     w3 = F.config(3, count=100)
     assert [(c.width, c.height) for c in w3.cameras] == F.RING_SIZES
+
+
+def test_bf16_packing_rounds_to_nearest_even():
+    # Known answers on the float32 bits: exact, ties to even both ways, carry into the exponent.
+    x = np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, 1 + 2 ** -8 + 2 ** -20, -2.0, float.fromhex("0x1.FFp0")], np.float32)
+    assert F.float_to_bf16_bits(x).tolist() == [0x3F80, 0x3F80, 0x3F82, 0x3F81, 0xC000, 0x4000]
+    # Against the nearest of the two bracketing bf16 values, in float64.
+    rng = np.random.default_rng(5)
+    v = (rng.standard_normal(100_000) * 10.0 ** rng.uniform(-6, 6, 100_000)).astype(np.float32)
+    bits = F.float_to_bf16_bits(v)
+    got = (bits << np.uint32(16)).view(np.float32).astype(np.float64)
+    down = (v.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32).astype(np.float64)
+    up = ((v.view(np.uint32) & np.uint32(0xFFFF0000)) + np.uint32(0x10000)).view(np.float32).astype(np.float64)
+    exact = v.astype(np.float64)
+    assert np.all(np.abs(got - exact) <= np.minimum(np.abs(down - exact), np.abs(up - exact)))
+    # Packing: two per word, low half first, and the round trip.
+    c = F.random_cloud(64, 3, 9)
+    words = F.pack_bf16_sh(c.sh)
+    assert words.shape == (64, F.bf16_sh_words(16)) and F.bf16_sh_words(16) == 24 and F.bf16_sh_words(1) == 2
+    flat = c.sh.reshape(64, -1)
+    assert (words[:, 0] & 0xFFFF).tolist() == F.float_to_bf16_bits(flat[:, 0]).tolist()
+    assert (words[:, 0] >> 16).tolist() == F.float_to_bf16_bits(flat[:, 1]).tolist()
+    back = F.unpack_bf16_sh(words, 16)
+    np.testing.assert_array_equal(F.pack_bf16_sh(back), words)
+    assert np.abs(back - c.sh).max() <= np.abs(c.sh).max() * 2 ** -8
+    # Degree 0: three coefficients in two words, the unused half zero.
+    w0 = F.pack_bf16_sh(F.random_cloud(5, 0, 3).sh)
+    assert w0.shape == (5, 2) and (w0[:, 1] >> 16 == 0).all()

@@ -68,9 +68,9 @@ def test_device_math_is_bit_exact():
     assert ((_bits(got) == _bits(want)) | (np.isnan(got) & np.isnan(want))).all()
 
 
-def _run_both(name, fixed=True, lam=0.2):
+def _run_both(name, fixed=True, lam=0.2, sh_storage=0):
     cloud, cam, target = _case(name)
-    model = O.Model(cloud, cam)
+    model = O.Model(cloud, cam, sh_storage=sh_storage)
     r = model.forward()
     model.loss(r, target, lam)
     if fixed:
@@ -78,7 +78,7 @@ def _run_both(name, fixed=True, lam=0.2):
     else:
         model.backward_float(r)
     g_model, st_model = model.gradients(r)
-    t = vsplat.Trainer(cloud, vsplat.TrainerOptions(lambda_ssim=lam, force_fixed_point=fixed))
+    t = vsplat.Trainer(cloud, vsplat.TrainerOptions(lambda_ssim=lam, force_fixed_point=fixed, sh_storage=sh_storage))
     stats = t.compute_gradients(cam, target)
     return cloud, cam, target, model, r, g_model, st_model, t, stats
 
@@ -106,6 +106,23 @@ def test_pipeline_bit_exact_fixed_point(name):
     for got, want in zip((g.positions, g.scales, g.rotations, g.opacities, g.sh), g_model):
         np.testing.assert_array_equal(_bits(got.reshape(-1)), _bits(want))
     np.testing.assert_array_equal(_bits(t.densify_stats()), _bits(st_model))
+
+
+@pytest.mark.parametrize("name", ["deg3", "config1_small"])
+def test_pipeline_bit_exact_with_bf16_sh(name):
+    """SH stored as bf16: splats, image, raw sums and gradients bit for bit
+    against Oracle B with the same storage."""
+    cloud, cam, target, model, r, g_model, st_model, t, stats = _run_both(name, fixed=True, sh_storage=vsplat.SH_BF16)
+    assert stats.visible_splats == r.visible and stats.tile_entries == r.entries
+    visible = r.values[r.keys != 0xFFFFFFFF]
+    np.testing.assert_array_equal(_bits(t.inspect("splats")[visible]), _bits(r.splats[visible]))
+    np.testing.assert_array_equal(_bits(t.inspect("image")), _bits(r.image))
+    np.testing.assert_array_equal(t.inspect("raw"), r.raw_fixed)
+    g = t.gradients()
+    for got, want in zip((g.positions, g.scales, g.rotations, g.opacities, g.sh), g_model):
+        np.testing.assert_array_equal(_bits(got.reshape(-1)), _bits(want))
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as half:
+        assert t.bytes_per_gaussian() == half.bytes_per_gaussian() - 8  # 24 SH words instead of 26
 
 
 def _check_close(a, b, what, min_cos, max_ratio):

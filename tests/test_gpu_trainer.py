@@ -40,47 +40,47 @@ def _case(name):
     raise KeyError(name)
 
 
-def _model_step(cloud, cam, target):
-    m = O.Model(cloud, cam)
+def _model_step(cloud, cam, target, sh_storage=0):
+    m = O.Model(cloud, cam, sh_storage=sh_storage)
     r = m.forward()
     m.loss(r, target, 0.2)
     m.backward_fixed(r)
     return m.step(r, RATE_TUPLE, rest_scale=RATES.sh_rest_scale, step=1)
 
 
-def _assert_cloud_equals(c, want, coefficients):
+def _assert_cloud_equals(c, want, coefficients, sh_storage=0):
     pos, sc, rot, op, words = want
     np.testing.assert_array_equal(_bits(c.positions.reshape(-1)), _bits(pos))
     np.testing.assert_array_equal(_bits(c.scales.reshape(-1)), _bits(sc))
     np.testing.assert_array_equal(_bits(c.rotations.reshape(-1)), _bits(rot))
     np.testing.assert_array_equal(_bits(c.opacities), _bits(op))
-    sh = F.unpack_half_sh(words, coefficients).reshape(c.sh.shape)
+    sh = F.unpack_sh(words, coefficients, sh_storage).reshape(c.sh.shape)
     np.testing.assert_array_equal(_bits(c.sh), _bits(sh))
 
 
-@pytest.mark.parametrize("name", ["ref60", "deg3", "config1_small"])
-def test_adam_step_bit_exact(name):
+@pytest.mark.parametrize("name,sh_storage", [("ref60", 0), ("deg3", 0), ("config1_small", 0), ("ref60", 1), ("deg3", 1)])
+def test_adam_step_bit_exact(name, sh_storage):
     cloud, cam, target = _case(name)
-    want = _model_step(cloud, cam, target)
+    want = _model_step(cloud, cam, target, sh_storage)
     coefficients = (cloud.sh_degree + 1) ** 2
-    options = vsplat.TrainerOptions(force_fixed_point=True)
+    options = vsplat.TrainerOptions(force_fixed_point=True, sh_storage=sh_storage)
     with vsplat.Trainer(cloud, options) as t:
         t.step(cam, target, RATES)
         stepped = t.cloud()
-    _assert_cloud_equals(stepped, want, coefficients)
+    _assert_cloud_equals(stepped, want, coefficients, sh_storage)
     moved = np.abs(stepped.opacities - cloud.opacities)
     assert moved.max() > 0.04  # the first step moves by about the rate
     # The same step as compute_gradients then apply_gradients (trainer.hpp:104, 121).
     with vsplat.Trainer(cloud, options) as t:
         t.compute_gradients(cam, target)
         t.apply_gradients(RATES)
-        _assert_cloud_equals(t.cloud(), want, coefficients)
+        _assert_cloud_equals(t.cloud(), want, coefficients, sh_storage)
     # And through the device-resident queue.
     with vsplat.Trainer(cloud, options) as t, vsplat.DeviceBuffer(target) as d:
         t.compute_gradients(cam, target)  # sizes the tile-entry reservation, changes no parameter
         t.enqueue(cam, d.ptr, vsplat.MODE_STEP, RATES)
         t.synchronize()
-        _assert_cloud_equals(t.cloud(), want, coefficients)
+        _assert_cloud_equals(t.cloud(), want, coefficients, sh_storage)
 
 
 def test_degenerate_gaussians_do_not_change_the_render():

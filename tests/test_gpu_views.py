@@ -44,11 +44,11 @@ def _ring(count=2000, degree=3, seed=77, sizes=SIZES):
     return cloud, cams, targets
 
 
-def _model_sum(cloud, cams, targets):
+def _model_sum(cloud, cams, targets, sh_storage=0):
     """Oracle B per view (fixed point), gradients and statistics summed in view order."""
     total, stats, per_view = None, None, []
     for cam, target in zip(cams, targets):
-        m = O.Model(cloud, cam)
+        m = O.Model(cloud, cam, sh_storage=sh_storage)
         r = m.forward()
         m.loss(r, target, 0.2)
         m.backward_fixed(r)
@@ -103,18 +103,20 @@ def test_batch_of_one_steps_as_step_does():
         np.testing.assert_array_equal(_bits(x), _bits(y))
 
 
-def test_batch_of_two_steps_as_the_model():
+@pytest.mark.parametrize("sh_storage", [0, 1])
+def test_batch_of_two_steps_as_the_model(sh_storage):
     cloud, cams, targets = _ring()
-    want_g, _, _, words = _model_sum(cloud, cams[:2], targets[:2])
-    pos, sc, rot, op, w = O.adam_from_gradients(cloud, words, want_g, RATE_TUPLE, rest_scale=RATES.sh_rest_scale)
-    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+    want_g, _, _, words = _model_sum(cloud, cams[:2], targets[:2], sh_storage)
+    pos, sc, rot, op, w = O.adam_from_gradients(cloud, words, want_g, RATE_TUPLE, rest_scale=RATES.sh_rest_scale,
+                                                sh_storage=sh_storage)
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, sh_storage=sh_storage)) as t:
         t.step_views(cams[:2], targets[:2], RATES)
         c = t.cloud()
     np.testing.assert_array_equal(_bits(c.positions.reshape(-1)), _bits(pos))
     np.testing.assert_array_equal(_bits(c.scales.reshape(-1)), _bits(sc))
     np.testing.assert_array_equal(_bits(c.rotations.reshape(-1)), _bits(rot))
     np.testing.assert_array_equal(_bits(c.opacities), _bits(op))
-    sh = F.unpack_half_sh(w, (cloud.sh_degree + 1) ** 2).reshape(c.sh.shape)
+    sh = F.unpack_sh(w, (cloud.sh_degree + 1) ** 2, sh_storage).reshape(c.sh.shape)
     np.testing.assert_array_equal(_bits(c.sh), _bits(sh))
 
 

@@ -279,3 +279,52 @@ def test_adam_from_gradients_equals_the_chain_rules_step():
     for a, b in zip(got, want):
         assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
     assert not np.array_equal(got[3], cloud.opacities)
+
+
+@pytest.mark.parametrize("fixed", [False, True])
+def test_bf16_sh_model_matches_fp64_reference(fixed):
+    """SH stored as bf16: Oracle B against Oracle A run on the same bf16-rounded
+    coefficients, with the reference's tolerances; the arithmetic stays fp32."""
+    cloud, cam, target = _case("deg3")
+    stored = cloud.copy()
+    stored.sh = F.unpack_bf16_sh(F.pack_bf16_sh(cloud.sh), cloud.coefficients).reshape(cloud.sh.shape)
+    assert not np.array_equal(stored.sh, cloud.sh)  # the rounding did something
+    ref = O.render(stored, cam, threads=2)
+    m = O.Model(cloud, cam, sh_storage=1)
+    r = m.forward()
+    assert np.abs(r.image[:, :3] - ref.rgb).max() < 1e-3
+    ref_loss = O.loss(ref, target, 0.2, threads=2)
+    m.loss(r, target, 0.2)
+    ref_g = O.gradients(stored, cam, ref_loss.adjoint, threads=2)
+    (m.backward_fixed if fixed else m.backward_float)(r)
+    g, _ = m.gradients(r)
+    cos, ratio = (0.97, 0.1) if fixed else (0.999, 0.02)
+    for got, want, what in zip(g, (ref_g.positions, ref_g.scales, ref_g.rotations, ref_g.opacities, ref_g.sh),
+                               ("positions", "scales", "rotations", "opacities", "sh")):
+        check_close(got, want, what, cos, ratio)
+
+
+def test_bf16_sh_step_rounds_to_a_neighbour():
+    """After a step with SH stored as bf16, every coefficient is one of the two
+    bf16 values around the fp32 result (stochastic rounding), and the DC terms
+    moved by about the rate like every other parameter."""
+    cloud, cam, target = _case("deg3")
+    rates = (1.6e-4, 0.005, 0.001, 0.05, 0.025)
+    stepped = []
+    for storage in (0, 1):
+        m = O.Model(cloud, cam, sh_storage=storage)
+        r = m.forward()
+        m.loss(r, target, 0.2)
+        m.backward_fixed(r)
+        stepped.append(m.step(r, rates, rest_scale=0.05, step=1))
+    words = stepped[1][4]
+    got = F.unpack_bf16_sh(words, cloud.coefficients).reshape(cloud.count, -1).astype(np.float64)
+    before = F.unpack_bf16_sh(F.pack_bf16_sh(cloud.sh), cloud.coefficients).reshape(cloud.count, -1).astype(np.float64)
+    moved = got != before
+    assert moved[:, :3].mean() > 0.5
+    # The two bf16 neighbours of any float x bracket x: truncation and one ulp away.
+    dc = np.abs(got[:, :3] - before[:, :3])
+    assert dc[moved[:, :3]].max() < 0.025 * 1.02 + np.abs(before[:, :3]).max() * 2 ** -7
+    # Geometry does not depend on the storage beyond the colour: positions move alike in sign.
+    assert np.mean(np.sign(stepped[0][0] - cloud.positions.reshape(-1)) ==
+                   np.sign(stepped[1][0] - cloud.positions.reshape(-1))) > 0.95

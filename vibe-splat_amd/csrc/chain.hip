@@ -150,7 +150,7 @@ __device__ __forceinline__ void untouched_terms(const ChainBuffers &B, const Cha
 // from the screen-space sums g to the geometric terms, with the SH part handed
 // to sh(basis, colour gradient, direction, coefficients used) as soon as it is
 // known. Reads the parameters only.
-template <int Degree, typename ShSink>
+template <int Degree, uint32_t Format, typename ShSink>
 __device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
                                            const float g[kRawPerSplat], ViewTerms &t, ShSink &&sh) {
   const float logScale[3] = {B.scales[i * 3], B.scales[i * 3 + 1], B.scales[i * 3 + 2]};
@@ -176,7 +176,7 @@ __device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewPara
   for (uint32_t k = 0; k < kCoefficients; ++k) {
 #pragma unroll
     for (uint32_t ch = 0; ch < 3; ++ch) {
-      const float v = sh_value(B.shWords, base, k * 3 + ch);
+      const float v = sh_value<Format>(B.shWords, base, k * 3 + ch);
       colour[ch] = k < used ? fma32(basis[k], v, colour[ch]) : colour[ch];
     }
   }
@@ -187,8 +187,8 @@ __device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewPara
     float gDir[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (uint32_t k = 1; k < kCoefficients; ++k) {
-      const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value(B.shWords, base, k * 3), sh_value(B.shWords, base, k * 3 + 1),
-                           sh_value(B.shWords, base, k * 3 + 2));
+      const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value<Format>(B.shWords, base, k * 3),
+                           sh_value<Format>(B.shWords, base, k * 3 + 1), sh_value<Format>(B.shWords, base, k * 3 + 2));
       float bg[3];
       sh_basis_gradient_term(k, dir[0], dir[1], dir[2], bg);
 #pragma unroll
@@ -354,7 +354,7 @@ __device__ __forceinline__ void emit_terms(const ChainBuffers &B, const ChainPar
   emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, t.opacity, o.rates[3], 3);
 }
 
-template <int Degree>
+template <int Degree, uint32_t Format>
 __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.count) return;
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
     }
     return;
   }
-  view_terms<Degree>(B, c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float dir[3], uint32_t used) {
+  view_terms<Degree, Format>(B, c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float dir[3], uint32_t used) {
     constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
     if (o.mode == kModeGradients) {
       // Coefficients above the active degree get zero (they are stored, not trained).
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
 // consecutive banks. In gradients mode the sums are stored; in step mode the
 // geometric parameters take Adam's step from them here, and the SH sums are
 // stored for the SH step.
-template <int Degree>
+template <int Degree, uint32_t Format>
 __global__ __launch_bounds__(256) void chain_views_kernel(ChainBuffers B, ChainViews views, ChainParams o, const uint32_t *counters) {
   constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
   __shared__ float shSum[kCoefficients * 3][256];
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(256) void chain_views_kernel(ChainBuffers B, ChainV
     }
     ViewTerms t;
     if (touched) {
-      view_terms<Degree>(B, w.c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float[3], uint32_t used) {
+      view_terms<Degree, Format>(B, w.c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float[3], uint32_t used) {
 #pragma unroll
         for (uint32_t k = 0; k < kCoefficients; ++k)
 #pragma unroll
@@ -511,6 +511,37 @@ __global__ __launch_bounds__(256) void chain_views_kernel(ChainBuffers B, ChainV
   emit_terms(B, o, i, sum);
 }
 
+// The SH floats packed word q holds: [first, last].
+__device__ __forceinline__ void sh_word_floats(uint32_t format, uint32_t q, uint32_t shFloats, uint32_t &first, uint32_t &last) {
+  if (format == kShBf16) {
+    first = 2 * q;
+  } else {
+    first = q < 3 ? q : 3 + 2 * (q - 3);
+    if (q < 3) {
+      last = q;
+      return;
+    }
+  }
+  last = first + 1 < shFloats - 1 ? first + 1 : shFloats - 1;
+}
+
+// SH float j (of the word whose first float is `first`) read from, and stored
+// back into, its packed word. Stored values are rounded at random to the
+// storage type (train_step.glsl:409-419 for the halves); the DC floats of
+// kShHalf are stored as they are.
+__device__ __forceinline__ float sh_word_get(uint32_t format, uint32_t word, uint32_t j, uint32_t first) {
+  if (format == kShBf16) return bf16_load(word, j - first);
+  if (j < 3) return as_float(word);
+  return half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+}
+__device__ __forceinline__ uint32_t sh_word_put(uint32_t format, uint32_t word, uint32_t j, uint32_t first, float value, uint32_t s,
+                                                uint32_t seed) {
+  if (format == kShBf16) return bf16_store(word, j - first, bf16_stochastic(value, s, seed ^ 0x5DEECE66u));
+  if (j < 3) return as_uint(value);
+  const uint32_t shift = 16u * ((j - 3) & 1u);
+  return (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, seed ^ 0x5DEECE66u) << shift);
+}
+
 __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shWords, uint32_t *__restrict__ stateSh,
                                                       const float *__restrict__ shPass, ViewParams c, ChainParams o,
                                                       const uint32_t *counters) {
@@ -526,8 +557,8 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
   if (used == 0 && !zeros) return;
   float basis[16];
   sh_basis(g[3], g[4], g[5], c.shDegree, basis);
-  const uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
-  const uint32_t last = q < 3 ? q : (first + 1 < o.shFloats - 1 ? first + 1 : o.shFloats - 1);
+  uint32_t first, last;
+  sh_word_floats(c.shFormat, q, o.shFloats, first, last);
   uint32_t word = shWords[w];
   bool changed = false;
   for (uint32_t j = first; j <= last; ++j) {
@@ -536,15 +567,10 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
     if (gv == 0.0f && !zeros) continue;
     const uint32_t s = i * o.shFloats + j;
     uint32_t state = stateSh[s];
-    float value = j < 3 ? as_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+    float value = sh_word_get(c.shFormat, word, j, first);
     step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.restScale, s * 5u + 4u);
     stateSh[s] = state;
-    if (j < 3) {
-      word = as_uint(value);
-    } else {
-      const uint32_t shift = 16u * ((j - 3) & 1u);
-      word = (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, o.seed ^ 0x5DEECE66u) << shift);
-    }
+    word = sh_word_put(c.shFormat, word, j, first, value, s, o.seed);
     changed = true;
   }
   if (changed) shWords[w] = word;
@@ -585,8 +611,8 @@ __global__ __launch_bounds__(256) void sh_adam_kernel(uint32_t *__restrict__ shW
   adam_step_now(o);
   const uint32_t i = w / words, q = w - i * words;
   const bool zeros = (o.flags & kFlagSelective) == 0;
-  const uint32_t first = q < 3 ? q : 3 + 2 * (q - 3);
-  const uint32_t last = q < 3 ? q : (first + 1 < o.shFloats - 1 ? first + 1 : o.shFloats - 1);
+  uint32_t first, last;
+  sh_word_floats(c.shFormat, q, o.shFloats, first, last);
   uint32_t word = shWords[w];
   bool changed = false;
   for (uint32_t j = first; j <= last; ++j) {
@@ -594,15 +620,10 @@ __global__ __launch_bounds__(256) void sh_adam_kernel(uint32_t *__restrict__ shW
     const float gv = gradSh[s];
     if (gv == 0.0f && !zeros) continue;
     uint32_t state = stateSh[s];
-    float value = j < 3 ? as_float(word) : half_to_float(((j - 3) & 1u) ? (word >> 16) : (word & 0xFFFFu));
+    float value = sh_word_get(c.shFormat, word, j, first);
     step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.restScale, s * 5u + 4u);
     stateSh[s] = state;
-    if (j < 3) {
-      word = as_uint(value);
-    } else {
-      const uint32_t shift = 16u * ((j - 3) & 1u);
-      word = (word & ~(0xFFFFu << shift)) | (half_stochastic(value, s, o.seed ^ 0x5DEECE66u) << shift);
-    }
+    word = sh_word_put(c.shFormat, word, j, first, value, s, o.seed);
     changed = true;
   }
   if (changed) shWords[w] = word;
@@ -618,16 +639,26 @@ __global__ void advance_adam_kernel(uint32_t *state, float beta1, float beta2, c
 
 } // namespace
 
+// Calls LAUNCH(degree, format) with both as compile-time constants.
+#define VSPLAT_SH_DISPATCH(degree, format, LAUNCH)                                                                                 \
+  switch ((degree) * 2 + ((format) == kShBf16 ? 1 : 0)) {                                                                          \
+  case 0: LAUNCH(0, kShHalf); break;                                                                                               \
+  case 1: LAUNCH(0, kShBf16); break;                                                                                               \
+  case 2: LAUNCH(1, kShHalf); break;                                                                                               \
+  case 3: LAUNCH(1, kShBf16); break;                                                                                               \
+  case 4: LAUNCH(2, kShHalf); break;                                                                                               \
+  case 5: LAUNCH(2, kShBf16); break;                                                                                               \
+  case 6: LAUNCH(3, kShHalf); break;                                                                                               \
+  default: LAUNCH(3, kShBf16); break;                                                                                              \
+  }
+
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
                   const uint32_t *counters) {
   if (view.count == 0) return;
   const dim3 grid((view.count + 255) / 256), block(256);
-  switch (view.shDegree) {
-  case 0: hipLaunchKernelGGL(chain_kernel<0>, grid, block, 0, stream, buffers, view, params, counters); break;
-  case 1: hipLaunchKernelGGL(chain_kernel<1>, grid, block, 0, stream, buffers, view, params, counters); break;
-  case 2: hipLaunchKernelGGL(chain_kernel<2>, grid, block, 0, stream, buffers, view, params, counters); break;
-  default: hipLaunchKernelGGL(chain_kernel<3>, grid, block, 0, stream, buffers, view, params, counters); break;
-  }
+#define VSPLAT_CHAIN(D, F) hipLaunchKernelGGL((chain_kernel<D, F>), grid, block, 0, stream, buffers, view, params, counters)
+  VSPLAT_SH_DISPATCH(view.shDegree, view.shFormat, VSPLAT_CHAIN)
+#undef VSPLAT_CHAIN
   if (params.mode == kModeStep) {
     const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
     hipLaunchKernelGGL(sh_step_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
@@ -643,12 +674,9 @@ void launch_chain_views(hipStream_t stream, const ChainBuffers &buffers, const C
   const uint32_t count = views.view[0].c.count;
   if (count == 0 || views.count == 0) return;
   const dim3 grid((count + 255) / 256), block(256);
-  switch (views.view[0].c.shDegree) {
-  case 0: hipLaunchKernelGGL(chain_views_kernel<0>, grid, block, 0, stream, buffers, views, params, counters); break;
-  case 1: hipLaunchKernelGGL(chain_views_kernel<1>, grid, block, 0, stream, buffers, views, params, counters); break;
-  case 2: hipLaunchKernelGGL(chain_views_kernel<2>, grid, block, 0, stream, buffers, views, params, counters); break;
-  default: hipLaunchKernelGGL(chain_views_kernel<3>, grid, block, 0, stream, buffers, views, params, counters); break;
-  }
+#define VSPLAT_CHAIN(D, F) hipLaunchKernelGGL((chain_views_kernel<D, F>), grid, block, 0, stream, buffers, views, params, counters)
+  VSPLAT_SH_DISPATCH(views.view[0].c.shDegree, views.view[0].c.shFormat, VSPLAT_CHAIN)
+#undef VSPLAT_CHAIN
 }
 
 void launch_sh_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,

@@ -33,7 +33,14 @@ struct ViewParams {
   float tanHalfX, tanHalfY; // 0.5 W / fx, 0.5 H / fy
   uint32_t width, height;
   uint32_t count, shDegree, shWords, shCoefficients; // active degree and coefficients
+  uint32_t shFormat;                                 // kShHalf or kShBf16
 };
+
+// How a Gaussian's SH coefficients are stored, in words of 32 bits:
+// kShHalf, the reference trainer's layout (the DC term as three floats, the rest
+// as halves two to a word, low half first; splat_types.glsl:299-310), or
+// kShBf16, every coefficient as bf16, two to a word, low half first.
+constexpr uint32_t kShHalf = 0, kShBf16 = 1;
 
 // The projected splat (train_types.glsl:26-31): conic xx xy yy, opacity, rgb,
 // radius, centre, extent. 48 bytes, read as three 16-byte vectors.
@@ -164,12 +171,19 @@ __device__ __forceinline__ uint32_t float_to_half(float f) {
   return sign | ((rounded - 0x38000000u) >> 13);
 }
 
-// Element j of a Gaussian's SH in the trainer's layout: the DC term as three
-// floats, the rest as halves two to a word (splat_types.glsl:299-310).
+// Element j of a Gaussian's SH (trainer_sh_value, splat_types.glsl:299-310, for
+// kShHalf). The format is a template argument: the kernels that read every
+// coefficient are compiled once per format.
+template <uint32_t Format>
 __device__ __forceinline__ float sh_value(const uint32_t *words, uint32_t base, uint32_t j) {
-  if (j < 3) return as_float(words[base + j]);
-  const uint32_t w = words[base + 3 + (j - 3) / 2];
-  return half_to_float(((j - 3) & 1u) ? (w >> 16) : (w & 0xFFFFu));
+  if constexpr (Format == kShBf16) {
+    const uint32_t w = words[base + j / 2];
+    return as_float((j & 1u) ? (w & 0xFFFF0000u) : (w << 16));
+  } else {
+    if (j < 3) return as_float(words[base + j]);
+    const uint32_t w = words[base + 3 + (j - 3) / 2];
+    return half_to_float(((j - 3) & 1u) ? (w >> 16) : (w & 0xFFFFu));
+  }
 }
 
 // World to camera, fused in the order x, y, z then the translation last.

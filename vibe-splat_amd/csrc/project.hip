@@ -36,7 +36,7 @@ struct GaussianIn {
 
 // One Gaussian: false if it is culled, else its splat and depth key. `sh` is
 // this Gaussian's packed SH words.
-template <int Degree>
+template <int Degree, uint32_t Format>
 __device__ __forceinline__ bool project_one(uint32_t i, const GaussianIn &g, const uint32_t *sh, const ViewParams &c,
                                             const ProjectOptions &opt, Splat &out, uint32_t &key) {
   const float px = g.px, py = g.py, pz = g.pz;
@@ -127,7 +127,7 @@ __device__ __forceinline__ bool project_one(uint32_t i, const GaussianIn &g, con
   for (uint32_t k = 0; k < (Degree + 1) * (Degree + 1); ++k) {
 #pragma unroll
     for (uint32_t ch = 0; ch < 3; ++ch) {
-      const float v = sh_value(sh, 0, k * 3 + ch);
+      const float v = sh_value<Format>(sh, 0, k * 3 + ch);
       col[ch] = k < used ? fma32(basis[k], v, col[ch]) : col[ch];
     }
   }
@@ -142,7 +142,7 @@ __device__ __forceinline__ bool project_one(uint32_t i, const GaussianIn &g, con
   return true;
 }
 
-template <int Degree>
+template <int Degree, uint32_t Format>
 __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ positions, const float *__restrict__ scales,
                                                       const float *__restrict__ rotations, const float *__restrict__ opacities,
                                                       const uint32_t *__restrict__ shWords, const float *__restrict__ filter3d,
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
 
   Splat s{};
   uint32_t key = kInvisible;
-  const bool visible = i < c.count && project_one<Degree>(i, g, stage + threadIdx.x * c.shWords, c, opt, s, key);
+  const bool visible = i < c.count && project_one<Degree, Format>(i, g, stage + threadIdx.x * c.shWords, c, opt, s, key);
   if (i < c.count) {
     keys[i] = key;
     values[i] = i;
@@ -269,15 +269,22 @@ void launch_project(hipStream_t stream, const float *positions, const float *sca
   if (view.count == 0) return;
   const ProjectOptions opt{mipFilter2d ? 1u : 0u, shDropout, shRetained, shSeed};
   const dim3 grid((view.count + 255) / 256), block(256);
-#define VSPLAT_PROJECT(D)                                                                                                          \
-  hipLaunchKernelGGL(project_kernel<D>, grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d, view,  \
-                     opt, gridW, gridH, splats, keys, values, tileCount, records, counters)
-  switch (view.shDegree) {
-  case 0: VSPLAT_PROJECT(0); break;
-  case 1: VSPLAT_PROJECT(1); break;
-  case 2: VSPLAT_PROJECT(2); break;
-  default: VSPLAT_PROJECT(3); break;
+#define VSPLAT_PROJECT(D, F)                                                                                                       \
+  hipLaunchKernelGGL((project_kernel<D, F>), grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d,    \
+                     view, opt, gridW, gridH, splats, keys, values, tileCount, records, counters)
+#define VSPLAT_PROJECT_DEGREE(F)                                                                                                   \
+  switch (view.shDegree) {                                                                                                         \
+  case 0: VSPLAT_PROJECT(0, F); break;                                                                                             \
+  case 1: VSPLAT_PROJECT(1, F); break;                                                                                             \
+  case 2: VSPLAT_PROJECT(2, F); break;                                                                                             \
+  default: VSPLAT_PROJECT(3, F); break;                                                                                            \
   }
+  if (view.shFormat == kShBf16) {
+    VSPLAT_PROJECT_DEGREE(kShBf16)
+  } else {
+    VSPLAT_PROJECT_DEGREE(kShHalf)
+  }
+#undef VSPLAT_PROJECT_DEGREE
 #undef VSPLAT_PROJECT
 }
 

@@ -29,6 +29,20 @@ void check(hipError_t e, const char *what) {
 }
 
 uint32_t half_words(uint32_t coefficients) { return 3u + (3u * (coefficients - 1u) + 1u) / 2u; }
+uint32_t bf16_words(uint32_t coefficients) { return (3u * coefficients + 1u) / 2u; }
+
+// Float to bf16, round to nearest even (as the device's bf16_round), and back.
+uint32_t float_to_bf16(float f) {
+  uint32_t bits;
+  std::memcpy(&bits, &f, 4);
+  return (bits + 0x7FFFu + ((bits >> 16) & 1u)) >> 16;
+}
+float bf16_to_float(uint32_t h) {
+  const uint32_t bits = h << 16;
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
 
 // Float to IEEE binary16, round to nearest even (as the device's float_to_half).
 uint16_t float_to_half(float f) {
@@ -99,7 +113,11 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   shDegree_ = activeShDegree_ = c.sh_degree;
   const uint32_t coefficients = static_cast<uint32_t>((c.sh_degree + 1) * (c.sh_degree + 1));
   shFloats_ = 3u * coefficients;
-  shWords_ = half_words(coefficients);
+  if (options.sh_storage != VSPLAT_SH_HALF && options.sh_storage != VSPLAT_SH_BF16) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, "sh_storage must be VSPLAT_SH_HALF or VSPLAT_SH_BF16");
+  }
+  shFormat_ = options.sh_storage == VSPLAT_SH_BF16 ? kShBf16 : kShHalf;
+  shWords_ = shFormat_ == kShBf16 ? bf16_words(coefficients) : half_words(coefficients);
   // Float atomics are the default on this GPU, as on any with them (trainer.cpp:181).
   fixedPoint_ = options.force_fixed_point != 0;
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -117,13 +135,18 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   check(hipMemcpy(scales_.as<float>(), c.scales, n * 12, hipMemcpyHostToDevice), "upload scales");
   check(hipMemcpy(rotations_.as<float>(), c.rotations, n * 16, hipMemcpyHostToDevice), "upload rotations");
   check(hipMemcpy(opacities_.as<float>(), c.opacities, n * 4, hipMemcpyHostToDevice), "upload opacities");
-  // pack_half_sh (src/splat/gaussian_cloud.cpp:537-550): DC as floats, the rest as halves.
+  // pack_half_sh (src/splat/gaussian_cloud.cpp:537-550): DC as floats, the rest
+  // as halves; or every coefficient as bf16.
   std::vector<uint32_t> words(n * shWords_, 0u);
   for (size_t i = 0; i < n; ++i) {
     const float *in = c.sh + i * shFloats_;
     uint32_t *to = words.data() + i * shWords_;
-    for (uint32_t j = 0; j < 3; ++j) std::memcpy(&to[j], &in[j], 4);
-    for (uint32_t j = 3; j < shFloats_; ++j) to[3 + (j - 3) / 2] |= static_cast<uint32_t>(float_to_half(in[j])) << (16 * ((j - 3) & 1));
+    if (shFormat_ == kShBf16) {
+      for (uint32_t j = 0; j < shFloats_; ++j) to[j / 2] |= float_to_bf16(in[j]) << (16 * (j & 1));
+    } else {
+      for (uint32_t j = 0; j < 3; ++j) std::memcpy(&to[j], &in[j], 4);
+      for (uint32_t j = 3; j < shFloats_; ++j) to[3 + (j - 3) / 2] |= static_cast<uint32_t>(float_to_half(in[j])) << (16 * ((j - 3) & 1));
+    }
   }
   check(hipMemcpy(shWordsBuf_.as<uint32_t>(), words.data(), words.size() * 4, hipMemcpyHostToDevice), "upload SH");
 
@@ -232,6 +255,7 @@ ViewParams Trainer::view_params(const vsplat_camera &cam) const {
   v.shDegree = static_cast<uint32_t>(activeShDegree_);
   v.shWords = shWords_;
   v.shCoefficients = static_cast<uint32_t>((activeShDegree_ + 1) * (activeShDegree_ + 1));
+  v.shFormat = shFormat_;
   return v;
 }
 
@@ -885,8 +909,12 @@ void Trainer::cloud(float *positions, float *scales, float *rotations, float *op
     for (size_t i = 0; i < n; ++i) {
       const uint32_t *in = words.data() + i * shWords_;
       float *to = sh + i * shFloats_;
-      for (uint32_t j = 0; j < 3; ++j) std::memcpy(&to[j], &in[j], 4);
-      for (uint32_t j = 3; j < shFloats_; ++j) to[j] = half_to_float((in[3 + (j - 3) / 2] >> (16 * ((j - 3) & 1))) & 0xFFFFu);
+      if (shFormat_ == kShBf16) {
+        for (uint32_t j = 0; j < shFloats_; ++j) to[j] = bf16_to_float((in[j / 2] >> (16 * (j & 1))) & 0xFFFFu);
+      } else {
+        for (uint32_t j = 0; j < 3; ++j) std::memcpy(&to[j], &in[j], 4);
+        for (uint32_t j = 3; j < shFloats_; ++j) to[j] = half_to_float((in[3 + (j - 3) / 2] >> (16 * ((j - 3) & 1))) & 0xFFFFu);
+      }
     }
   }
 }

@@ -168,6 +168,7 @@ private:
   uint32_t count_ = 0;
   int shDegree_ = 0, activeShDegree_ = 0;
   uint32_t shFloats_ = 3, shWords_ = 3;
+  uint32_t shFormat_ = 0; // kShHalf or kShBf16
   bool fixedPoint_ = true;
   float shDropout_ = 0.0f, dropoutNow_ = 0.0f;
   uint32_t shRetained_ = 0, shSeed_ = 0;

@@ -24,6 +24,7 @@ LIBRARY = Path(__file__).resolve().parent / "libvsplat.so"
 OK, RUNTIME_ERROR, BUDGET_EXCEEDED, INVALID_ARGUMENT, LOGIC_ERROR, DEVICE_ERROR, UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
 BYTES_PER_TILE_ENTRY = 16
 MODE_STATISTICS, MODE_GRADIENTS, MODE_STEP = 0, 1, 2
+SH_HALF, SH_BF16 = 0, 1  # TrainerOptions.sh_storage
 
 EXPORTS = [
     "vsplat_default_options", "vsplat_default_rates", "vsplat_trainer_create", "vsplat_trainer_destroy", "vsplat_render",
@@ -69,7 +70,7 @@ class COptions(C.Structure):
     _fields_ = [("lambda_ssim", C.c_float), ("background", C.c_float * 3), ("near_plane", C.c_float),
                 ("force_fixed_point", C.c_int32), ("selective_adam", C.c_int32), ("opacity_reg", C.c_float),
                 ("scale_reg", C.c_float), ("position_noise", C.c_float), ("mip_filter_2d", C.c_int32),
-                ("pose_gradients", C.c_int32), ("memory_budget", C.c_uint64)]
+                ("pose_gradients", C.c_int32), ("memory_budget", C.c_uint64), ("sh_storage", C.c_int32)]
 
 
 class CRates(C.Structure):
@@ -182,6 +183,7 @@ class TrainerOptions:
     mip_filter_2d: bool = False
     pose_gradients: bool = False
     memory_budget: int = 2 ** 64 - 1
+    sh_storage: int = 0  # SH_HALF (the reference's layout) or SH_BF16 (this build only)
 
     def to_c(self) -> COptions:
         o = COptions()
@@ -194,6 +196,7 @@ class TrainerOptions:
         o.mip_filter_2d = int(self.mip_filter_2d)
         o.pose_gradients = int(self.pose_gradients)
         o.memory_budget = int(self.memory_budget)
+        o.sh_storage = int(self.sh_storage)
         return o
 
 

@@ -364,3 +364,47 @@ def unpack_half_sh(words: np.ndarray, coefficients: int) -> np.ndarray:
 def half_sh_words(coefficients: int) -> int:
     """half_sh_words (src/splat/gaussian_cloud.hpp:44-46)."""
     return 3 + (3 * (coefficients - 1) + 1) // 2
+
+
+def bf16_sh_words(coefficients: int) -> int:
+    """Words per Gaussian when every SH coefficient is a bf16, two to a word."""
+    return (3 * coefficients + 1) // 2
+
+
+def float_to_bf16_bits(x: np.ndarray) -> np.ndarray:
+    """bf16 bits, round to nearest even on the float32 bits (the trainer's upload rule)."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint32) & np.uint32(0xFFFF)
+
+
+def pack_bf16_sh(sh: np.ndarray) -> np.ndarray:
+    """Every coefficient as bf16, two to a word, low half first."""
+    n = sh.shape[0]
+    flat = np.asarray(sh, np.float32).reshape(n, -1)
+    if flat.shape[1] % 2:
+        flat = np.concatenate([flat, np.zeros((n, 1), np.float32)], 1)
+    bits = float_to_bf16_bits(flat)
+    if flat.shape[1] > np.asarray(sh).reshape(n, -1).shape[1]:
+        bits[:, -1] = 0
+    return (bits[:, 0::2] | (bits[:, 1::2] << np.uint32(16))).astype(np.uint32)
+
+
+def unpack_bf16_sh(words: np.ndarray, coefficients: int) -> np.ndarray:
+    n = words.shape[0]
+    lo = (words << np.uint32(16)).view(np.float32)
+    hi = (words & np.uint32(0xFFFF0000)).view(np.float32)
+    inter = np.stack([lo, hi], 2).reshape(n, -1)
+    return np.ascontiguousarray(inter[:, : 3 * coefficients]).reshape(n, coefficients, 3)
+
+
+def sh_words(coefficients: int, storage: int = 0) -> int:
+    return bf16_sh_words(coefficients) if storage == 1 else half_sh_words(coefficients)
+
+
+def pack_sh(sh: np.ndarray, storage: int = 0) -> np.ndarray:
+    """The trainer's SH words: storage 0 the reference's half layout, 1 all bf16."""
+    return pack_bf16_sh(sh) if storage == 1 else pack_half_sh(sh)
+
+
+def unpack_sh(words: np.ndarray, coefficients: int, storage: int = 0) -> np.ndarray:
+    return unpack_bf16_sh(words, coefficients) if storage == 1 else unpack_half_sh(words, coefficients)
