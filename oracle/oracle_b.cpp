@@ -779,6 +779,11 @@ bool backward_pair(const Splat &s, const std::uint32_t box[4], std::uint32_t x, 
   return true;
 }
 
+// The pixel of lane `lane` of wave `wave`: the wave holds the 8x8 quadrant
+// (wave & 1, wave >> 1) of the tile, the lane its pixel (lane & 7, lane >> 3).
+std::uint32_t wave_x(const TileGeometry &g, std::uint32_t wave, std::uint32_t lane) { return g.x0 + (wave & 1u) * 8u + (lane & 7u); }
+std::uint32_t wave_y(const TileGeometry &g, std::uint32_t wave, std::uint32_t lane) { return g.y0 + (wave >> 1) * 8u + (lane >> 3); }
+
 // Visits every (tile, wave, entry) in order with the 64 lanes' values; `emit`
 // receives the wave's 14 totals.
 template <class Emit>
@@ -796,7 +801,7 @@ void backward_waves(const float *splats, const uint32_t *entryValues, const uint
     for (std::uint32_t wave = 0; wave < 4; ++wave) {
       PixelState px[64];
       for (std::uint32_t lane = 0; lane < 64; ++lane) {
-        const std::uint32_t x = geo.x0 + (lane & 15u), y = geo.y0 + wave * 4 + (lane >> 4);
+        const std::uint32_t x = wave_x(geo, wave, lane), y = wave_y(geo, wave, lane);
         PixelState &p = px[lane];
         p.inside = x < width && y < height;
         const std::size_t at = p.inside ? static_cast<std::size_t>(y) * width + x : 0;
@@ -816,7 +821,7 @@ void backward_waves(const float *splats, const uint32_t *entryValues, const uint
         float lanes[kRaw][64];
         bool any = false;
         for (std::uint32_t lane = 0; lane < 64; ++lane) {
-          const std::uint32_t x = geo.x0 + (lane & 15u), y = geo.y0 + wave * 4 + (lane >> 4);
+          const std::uint32_t x = wave_x(geo, wave, lane), y = wave_y(geo, wave, lane);
           float g[kRaw];
           any = backward_pair(s, box, x, y, e, perPixel, px[lane], g) || any;
           for (int k = 0; k < kRaw; ++k) lanes[k][lane] = g[k];

@@ -118,11 +118,13 @@ __device__ __forceinline__ float vs_log(float x) {
   return fma32(ef, 0x1.62e4p-1f, fma32(ef, 0x1.7f7d1cp-20f, lp));
 }
 
-// vs_exp on [-87, 87] without its special cases: the argument is clamped into
-// that range, where 2^n is a normal float and one multiplication applies it
-// exactly as vs_exp's two do. Equal to vs_exp(x) for every x in [-87, 87].
+// vs_exp on [-87, 87] without its special cases, for arguments that are never
+// NaN: the argument is clamped into that range (one med3), where the result
+// p 2^n is a normal float (n >= -126 with p > 1 at the low end), so ldexp
+// applies 2^n exactly, as vs_exp's two multiplications do. Equal to vs_exp(x)
+// for every x in [-87, 87].
 __device__ __forceinline__ float exp_finite(float x) {
-  x = x < -87.0f ? -87.0f : (x > 87.0f ? 87.0f : x);
+  x = __builtin_amdgcn_fmed3f(x, -87.0f, 87.0f);
   const float n = __builtin_rintf(x * 0x1.715476p+0f);
   float r = fma32(n, -0x1.62e4p-1f, x);
   r = fma32(n, -0x1.7f7d1cp-20f, r);
@@ -131,7 +133,7 @@ __device__ __forceinline__ float exp_finite(float x) {
   q = fma32(q, r, 0x1.5554dcp-3f);
   q = fma32(q, r, 0x1p-1f);
   const float p = fma32(r * r, q, r) + 1.0f;
-  return p * as_float(static_cast<uint32_t>(static_cast<int>(n) + 127) << 23);
+  return __builtin_ldexpf(p, static_cast<int>(n));
 }
 
 // vs_log for a positive normal finite x, without its special cases; equal to

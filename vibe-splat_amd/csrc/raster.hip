@@ -11,7 +11,11 @@
 // A wave skips a splat none of its lanes is a candidate for; otherwise every
 // lane evaluates the Gaussian without branches and the result is selected.
 //
-// Backward: wave w of a tile holds pixel rows 4w..4w+3. For each splat the 14
+// Wave w of a tile holds the 8x8 quadrant (w & 1, w >> 1), lane l its pixel
+// (l & 7, l >> 3): a splat's footprint fills more of a square block's lanes
+// than of a 16x4 strip's.
+//
+// Backward: for each splat the 14
 // per-pixel sums are reduced over the wave by a transposed butterfly in which
 // each step halves the values a lane carries: partner lanes l ^ 32 and l ^ 16
 // through the permlane swaps, l ^ 15 and l ^ 7 through DPP row and half-row
@@ -65,6 +69,10 @@ __device__ __forceinline__ Staged stage(const Splat &s, const TileGeometry &geo)
   return Staged{s.qx, s.qy, s.qz, s.o, s.r, s.g, s.b, 0.0f, s.cx, s.cy, mask, 0u};
 }
 
+// Thread t's pixel in the tile.
+__device__ __forceinline__ uint32_t local_x(uint32_t t) { return ((t >> 6) & 1u) * 8u + (t & 7u); }
+__device__ __forceinline__ uint32_t local_y(uint32_t t) { return (t >> 7) * 8u + ((t >> 3) & 7u); }
+
 __device__ __forceinline__ bool covers(uint32_t mask, uint32_t lx, uint32_t ly) { return ((mask >> lx) & (mask >> (16u + ly)) & 1u) != 0u; }
 
 // The Gaussian's exponent at offset (dx, dy) (splat_at, train_types.glsl:88).
@@ -79,7 +87,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ 
   __shared__ Staged batch[kTilePixels];
   const uint32_t tile = blockIdx.x;
   const TileGeometry geo = tile_geometry(tile, gridW, width, height);
-  const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+  const uint32_t lx = local_x(threadIdx.x), ly = local_y(threadIdx.x);
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
   __shared__ uint32_t tileStop;
   const uint32_t tile = blockIdx.x;
   const TileGeometry geo = tile_geometry(tile, gridW, width, height);
-  const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4, lane = threadIdx.x & 63u;
+  const uint32_t lx = local_x(threadIdx.x), ly = local_y(threadIdx.x), lane = threadIdx.x & 63u;
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
@@ -231,34 +239,37 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
       if (!__any(ok)) continue;
       const float logRest = log_normal(1.0f - alpha);
       const float T = exp_finite((logTotal - logBehind) - logRest);
-      const float Ta = T * alpha;
+      // A lane that takes nothing from the splat carries zero factors, so its
+      // values below are all zero (every other factor is finite); the sign of
+      // a zero cannot change a total that is not zero.
+      const float Ta = ok ? T * alpha : 0.0f;
       // At the 0.99 clamp alpha no longer depends on the parameters.
       const bool free = ok && a < 0.99f;
-      const float dAlpha = T * fma32(adj.z, s.b - Bb, fma32(adj.y, s.g - Bg, adj.x * (s.r - Br)));
+      const float dAlpha = free ? T * fma32(adj.z, s.b - Bb, fma32(adj.y, s.g - Bg, adj.x * (s.r - Br))) : 0.0f;
       const float dPower = dAlpha * alpha;
       const float vx = fma32(s.qy, dy, s.qx * dx), vy = fma32(s.qz, dy, s.qy * dx);
       const float g1 = dPower * vx, g2 = dPower * vy;
       const float hp = 0.5f * dPower;
       float v[16];
-      v[0] = free ? dAlpha * e : 0.0f;
-      v[1] = free ? g1 : 0.0f;
-      v[2] = free ? g2 : 0.0f;
-      v[3] = free ? (hp * vx) * vx : 0.0f;
-      v[4] = free ? (hp * vx) * vy : 0.0f;
-      v[5] = free ? (hp * vy) * vy : 0.0f;
-      v[6] = ok ? Ta * adj.x : 0.0f;
-      v[7] = ok ? Ta * adj.y : 0.0f;
-      v[8] = ok ? Ta * adj.z : 0.0f;
-      v[9] = ok ? Ta * perPixel : 0.0f;
-      v[10] = free ? fabsf(g1) : 0.0f;
-      v[11] = free ? fabsf(g2) : 0.0f;
+      v[0] = dAlpha * e;
+      v[1] = g1;
+      v[2] = g2;
+      v[3] = (hp * vx) * vx;
+      v[4] = (hp * vx) * vy;
+      v[5] = (hp * vy) * vy;
+      v[6] = Ta * adj.x;
+      v[7] = Ta * adj.y;
+      v[8] = Ta * adj.z;
+      v[9] = Ta * perPixel;
+      v[10] = fabsf(g1);
+      v[11] = fabsf(g2);
       v[12] = v[13] = v[14] = v[15] = 0.0f;
-      if (ok) {
-        Br = fma32(alpha, s.r - Br, Br);
-        Bg = fma32(alpha, s.g - Bg, Bg);
-        Bb = fma32(alpha, s.b - Bb, Bb);
-        logBehind = logBehind + logRest;
-      }
+      // The colour and log-transmittance behind: unchanged where alpha is taken as 0.
+      const float alphaBehind = ok ? alpha : 0.0f;
+      Br = fma32(alphaBehind, s.r - Br, Br);
+      Bg = fma32(alphaBehind, s.g - Bg, Bg);
+      Bb = fma32(alphaBehind, s.b - Bb, Bb);
+      logBehind = logBehind + (ok ? logRest : 0.0f);
       reduce_wave(v, lane);
       // Values 12 and 13 of the kRawPerSplat slots are always zero here.
       const uint32_t value = lane >> 2;
