@@ -73,6 +73,10 @@ typedef struct {
    * coefficient as bf16 (96 bytes per Gaussian at degree 3 instead of 104);
    * arithmetic stays fp32, and a step rounds the stored values at random. */
   int32_t sh_storage;
+  /* Nonzero: the backward pass recomputes each pixel's transmittance and stop
+   * with a front-to-back sweep of its tile instead of reading what the forward
+   * pass stored (no reference counterpart; the results are identical). */
+  int32_t recompute_transmittance;
 } vsplat_trainer_options;
 
 enum { VSPLAT_SH_HALF = 0, VSPLAT_SH_BF16 = 1 };
@@ -242,6 +246,15 @@ int vsplat_release_captured(vsplat_trainer *trainer);
  * Gaussian with fixed point, else 14 float), "counters" (4 words). *bytes
  * gets the size in use; at most `capacity` bytes are copied. */
 int vsplat_inspect(vsplat_trainer *trainer, const char *buffer, void *out, uint64_t capacity, uint64_t *bytes);
+
+/* Device memory for callers without a HIP runtime of their own (no reference
+ * counterpart): targets for vsplat_enqueue and the batched calls. Allocates
+ * bytes of GPU memory, copying them from host when it is not NULL. */
+int vsplat_device_alloc(uint64_t bytes, const void *host, void **out_device);
+/* Copies bytes from GPU memory to host memory. */
+int vsplat_device_read(const void *device, void *host, uint64_t bytes);
+/* Frees memory from vsplat_device_alloc. NULL is ignored. */
+void vsplat_device_free(void *device);
 
 /* Runs one of the device's elementwise functions over n floats (device
  * arithmetic check): 0 exp, 1 log, 2 sqrt, 3 reciprocal (1/x), 4 float to half

@@ -91,6 +91,7 @@ struct TrainerOptions {
   bool poseGradients = false;
   std::uint64_t memoryBudget = ~std::uint64_t{0};
   bool bf16Sh = false; // this build only: SH stored as bf16 (vsplat_trainer_options::sh_storage)
+  bool recomputeTransmittance = false; // this build only (vsplat_trainer_options::recompute_transmittance)
 };
 
 // LearningRates (trainer.hpp:51-61).
@@ -155,6 +156,7 @@ public:
     o.pose_gradients = options.poseGradients ? 1 : 0;
     o.memory_budget = options.memoryBudget;
     o.sh_storage = options.bf16Sh ? VSPLAT_SH_BF16 : VSPLAT_SH_HALF;
+    o.recompute_transmittance = options.recomputeTransmittance ? 1 : 0;
     detail::check(vsplat_trainer_create(&c, &o, &trainer_));
   }
   ~Trainer() { vsplat_trainer_destroy(trainer_); }

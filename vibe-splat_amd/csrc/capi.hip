@@ -248,6 +248,33 @@ extern "C" int vsplat_math_probe(int32_t function, const float *in, float *out, 
   return guarded([&] { vsplat::math_probe(function, in, out, n); });
 }
 
+extern "C" int vsplat_device_alloc(uint64_t bytes, const void *host, void **out) {
+  if (out == nullptr) return invalid("null argument");
+  *out = nullptr;
+  return guarded([&] {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes > 0 ? bytes : 1) != hipSuccess) throw vsplat::Error(VSPLAT_DEVICE_ERROR, "hipMalloc failed");
+    if (host != nullptr && bytes > 0 && hipMemcpy(p, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(p);
+      throw vsplat::Error(VSPLAT_DEVICE_ERROR, "hipMemcpy to the device failed");
+    }
+    *out = p;
+  });
+}
+
+extern "C" int vsplat_device_read(const void *device, void *host, uint64_t bytes) {
+  if ((device == nullptr || host == nullptr) && bytes > 0) return invalid("null argument");
+  return guarded([&] {
+    if (bytes > 0 && hipMemcpy(host, device, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      throw vsplat::Error(VSPLAT_DEVICE_ERROR, "hipMemcpy to the host failed");
+    }
+  });
+}
+
+extern "C" void vsplat_device_free(void *device) {
+  if (device != nullptr) (void)hipFree(device);
+}
+
 extern "C" int vsplat_set_stage_timing(vsplat_trainer *t, int32_t enabled) {
   if (t == nullptr) return invalid("null trainer");
   return guarded([&] { t->impl.set_queued_timing(enabled != 0); });

@@ -96,9 +96,11 @@ void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uin
 
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
                     uint32_t height, const float background[3], float4 *image, uint32_t *stops);
-void launch_backward(hipStream_t stream, bool fixedPoint, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges,
-                     uint32_t width, uint32_t height, const float background[3], const float4 *image, const float4 *adjoint,
-                     const uint32_t *stops, float fixedScale, int64_t *rawFixed, float *rawFloat);
+// recompute: the transmittance and stops are recomputed per tile rather than
+// read from the forward pass (image and stops may then be null).
+void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
+                     const uint2 *ranges, uint32_t width, uint32_t height, const float background[3], const float4 *image,
+                     const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed, float *rawFloat);
 
 void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
                  float *fields, float4 *adjoint, float2 *tileTotals, double *sums);

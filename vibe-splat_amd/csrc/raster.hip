@@ -80,20 +80,15 @@ __device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
   return fma32(-0.5f, fma32(s.qx * dx, dx, (s.qz * dy) * dy), -((s.qy * dx) * dy));
 }
 
-__global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
-                                                      const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
-                                                      float bgR, float bgG, float bgB, float4 *__restrict__ image,
-                                                      uint32_t *__restrict__ stops) {
-  __shared__ Staged batch[kTilePixels];
-  const uint32_t tile = blockIdx.x;
-  const TileGeometry geo = tile_geometry(tile, gridW, width, height);
-  const uint32_t lx = local_x(threadIdx.x), ly = local_y(threadIdx.x);
-  const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
-  const bool inside = x < width && y < height;
-  const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
-  const uint2 range = ranges[tile];
-  float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, logT = 0.0f;
-  uint32_t stop = range.x;
+// Front to back over a tile's list (train_forward.comp:38-100): the colour
+// (with Colour), the transmittance and its log, and the stop index of the
+// calling thread's pixel. The forward pass and the backward pass that
+// recomputes the transmittance both run this, so they agree bit for bit.
+template <bool Colour>
+__device__ __forceinline__ void composite_tile(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues, uint2 range,
+                                               const TileGeometry &geo, Staged *batch, bool inside, float px, float py, uint32_t lx,
+                                               uint32_t ly, float &r, float &g, float &b, float &T, float &logT, uint32_t &stop) {
+  stop = range.x;
   bool done = !inside;
   for (uint32_t end = range.y; end > range.x;) {
     const uint32_t count = end - range.x < kTilePixels ? end - range.x : kTilePixels;
@@ -122,16 +117,34 @@ __global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ 
           stop = end + k + 1;
           done = true;
         } else {
-          const float w = T * alpha;
-          r = fma32(w, s.r, r);
-          g = fma32(w, s.g, g);
-          b = fma32(w, s.b, b);
-          T = T * (1.0f - alpha);
+          if constexpr (Colour) {
+            const float w = T * alpha;
+            r = fma32(w, s.r, r);
+            g = fma32(w, s.g, g);
+            b = fma32(w, s.b, b);
+            T = T * (1.0f - alpha);
+          }
           logT = logT + logRest;
         }
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                                      const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
+                                                      float bgR, float bgG, float bgB, float4 *__restrict__ image,
+                                                      uint32_t *__restrict__ stops) {
+  __shared__ Staged batch[kTilePixels];
+  const uint32_t tile = blockIdx.x;
+  const TileGeometry geo = tile_geometry(tile, gridW, width, height);
+  const uint32_t lx = local_x(threadIdx.x), ly = local_y(threadIdx.x);
+  const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
+  const bool inside = x < width && y < height;
+  const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
+  float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, logT = 0.0f;
+  uint32_t stop;
+  composite_tile<true>(splats, entryValues, ranges[tile], geo, batch, inside, px, py, lx, ly, r, g, b, T, logT, stop);
   if (inside) {
     const uint32_t at = y * width + x;
     image[at] = make_float4(fma32(T, bgR, r), fma32(T, bgG, g), fma32(T, bgB, b), logT);
@@ -188,7 +201,10 @@ __device__ __forceinline__ void reduce_wave(float (&v)[16], uint32_t lane) {
   v[0] = v[0] + dpp_xor1(v[0]);
 }
 
-template <bool Fixed>
+// Back to front over the tile (train_backward.glsl:78-158). With Recompute the
+// pixel's log transmittance and stop come from a front-to-back sweep of its
+// own instead of the forward pass's image and stops, which are then not read.
+template <bool Fixed, bool Recompute>
 __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
                                                        const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                                        float bgR, float bgG, float bgB, const float4 *__restrict__ image,
@@ -205,11 +221,20 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   const uint32_t at = inside ? y * width + x : 0u;
-  const float logTotal = inside ? image[at].w : 0.0f;
   const float4 adj = inside ? adjoint[at] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float perPixel = 1.0f / (3.0f * static_cast<float>(width) * static_cast<float>(height));
   const uint2 range = ranges[tile];
-  const uint32_t stop = inside ? stops[at] : range.y;
+  float logTotal;
+  uint32_t stop;
+  if constexpr (Recompute) {
+    float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f;
+    logTotal = 0.0f;
+    composite_tile<false>(splats, entryValues, range, geo, batch, inside, px, py, lx, ly, r, g, b, T, logTotal, stop);
+    if (!inside) stop = range.y;
+  } else {
+    logTotal = inside ? image[at].w : 0.0f;
+    stop = inside ? stops[at] : range.y;
+  }
   if (threadIdx.x == 0) tileStop = range.y;
   __syncthreads();
   atomicMin(&tileStop, stop);
@@ -295,19 +320,28 @@ void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *ent
                      gridW, background[0], background[1], background[2], image, stops);
 }
 
-void launch_backward(hipStream_t stream, bool fixedPoint, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges,
-                     uint32_t width, uint32_t height, const float background[3], const float4 *image, const float4 *adjoint,
-                     const uint32_t *stops, float fixedScale, int64_t *rawFixed, float *rawFloat) {
+void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
+                     const uint2 *ranges, uint32_t width, uint32_t height, const float background[3], const float4 *image,
+                     const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed, float *rawFloat) {
   const uint32_t gridW = (width + kTileSize - 1) / kTileSize, gridH = (height + kTileSize - 1) / kTileSize;
+#define VSPLAT_BACKWARD(F, R)                                                                                                      \
+  hipLaunchKernelGGL((backward_kernel<F, R>), dim3(gridW * gridH), dim3(kTilePixels), 0, stream, splats, entryValues, ranges,     \
+                     width, height, gridW, background[0], background[1], background[2], image, adjoint, stops, fixedScale,         \
+                     reinterpret_cast<unsigned long long *>(rawFixed), rawFloat)
   if (fixedPoint) {
-    hipLaunchKernelGGL(backward_kernel<true>, dim3(gridW * gridH), dim3(kTilePixels), 0, stream, splats, entryValues, ranges, width,
-                       height, gridW, background[0], background[1], background[2], image, adjoint, stops, fixedScale,
-                       reinterpret_cast<unsigned long long *>(rawFixed), rawFloat);
+    if (recompute) {
+      VSPLAT_BACKWARD(true, true);
+    } else {
+      VSPLAT_BACKWARD(true, false);
+    }
   } else {
-    hipLaunchKernelGGL(backward_kernel<false>, dim3(gridW * gridH), dim3(kTilePixels), 0, stream, splats, entryValues, ranges, width,
-                       height, gridW, background[0], background[1], background[2], image, adjoint, stops, fixedScale,
-                       reinterpret_cast<unsigned long long *>(rawFixed), rawFloat);
+    if (recompute) {
+      VSPLAT_BACKWARD(false, true);
+    } else {
+      VSPLAT_BACKWARD(false, false);
+    }
   }
+#undef VSPLAT_BACKWARD
 }
 
 } // namespace vsplat

@@ -413,7 +413,7 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &v, const uint8_t 
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
   // a resolution of 1/64 (trainer.cpp:586-591).
   const float fixedScale = fixed_scale(v);
-  launch_backward(s, fixedPoint_, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height,
+  launch_backward(s, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height,
                   options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
                   b.rawFloat.as<float>());
   mark("backward");

@@ -37,6 +37,7 @@ EXPORTS = [
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
+    "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_free",
 ]
 
 
@@ -70,7 +71,8 @@ class COptions(C.Structure):
     _fields_ = [("lambda_ssim", C.c_float), ("background", C.c_float * 3), ("near_plane", C.c_float),
                 ("force_fixed_point", C.c_int32), ("selective_adam", C.c_int32), ("opacity_reg", C.c_float),
                 ("scale_reg", C.c_float), ("position_noise", C.c_float), ("mip_filter_2d", C.c_int32),
-                ("pose_gradients", C.c_int32), ("memory_budget", C.c_uint64), ("sh_storage", C.c_int32)]
+                ("pose_gradients", C.c_int32), ("memory_budget", C.c_uint64), ("sh_storage", C.c_int32),
+                ("recompute_transmittance", C.c_int32)]
 
 
 class CRates(C.Structure):
@@ -143,6 +145,10 @@ def lib():
         L.vsplat_capture_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_int32, vp]
         L.vsplat_launch_captured.argtypes = [vp, C.c_uint32]
         L.vsplat_release_captured.argtypes = [vp]
+        L.vsplat_device_alloc.argtypes = [C.c_uint64, vp, vp]
+        L.vsplat_device_read.argtypes = [vp, vp, C.c_uint64]
+        L.vsplat_device_free.argtypes = [vp]
+        L.vsplat_device_free.restype = None
         _LIB = L
     return _LIB
 
@@ -184,6 +190,7 @@ class TrainerOptions:
     pose_gradients: bool = False
     memory_budget: int = 2 ** 64 - 1
     sh_storage: int = 0  # SH_HALF (the reference's layout) or SH_BF16 (this build only)
+    recompute_transmittance: bool = False  # backward without the forward's stored log T and stops (this build only)
 
     def to_c(self) -> COptions:
         o = COptions()
@@ -197,6 +204,7 @@ class TrainerOptions:
         o.pose_gradients = int(self.pose_gradients)
         o.memory_budget = int(self.memory_budget)
         o.sh_storage = int(self.sh_storage)
+        o.recompute_transmittance = int(self.recompute_transmittance)
         return o
 
 
@@ -475,39 +483,25 @@ class Trainer:
         return out.reshape(-1, width) if width > 1 else out
 
 
-_HIP = None
-
-
-def _hip():
-    global _HIP
-    if _HIP is None:
-        lib()  # the HIP runtime comes in as the library's dependency
-        H = C.CDLL("/opt/rocm/lib/libamdhip64.so")
-        H.hipMalloc.argtypes = [C.c_void_p, C.c_size_t]
-        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        H.hipFree.argtypes = [C.c_void_p]
-        _HIP = H
-    return _HIP
-
-
 class DeviceBuffer:
     """GPU memory holding a copy of a host array, for the device-resident calls
-    (Trainer.enqueue takes its .ptr as the target)."""
+    (Trainer.enqueue takes its .ptr as the target). Allocated through the
+    library, so that it shares the library's HIP runtime."""
 
     def __init__(self, array: np.ndarray):
         a = np.ascontiguousarray(array)
-        hip = _hip()
         ptr = C.c_void_p()
-        if hip.hipMalloc(C.byref(ptr), max(a.nbytes, 1)) != 0:
-            raise DeviceError("hipMalloc failed")
+        _raise(lib().vsplat_device_alloc(a.nbytes, C.c_void_p(a.ctypes.data), C.byref(ptr)), "device_alloc")
         self.ptr, self.nbytes = ptr.value, a.nbytes
-        if hip.hipMemcpy(ptr, C.c_void_p(a.ctypes.data), a.nbytes, 1) != 0:  # host to device
-            self.free()
-            raise DeviceError("hipMemcpy failed")
+
+    def read(self, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        _raise(lib().vsplat_device_read(C.c_void_p(self.ptr), _p(out), out.nbytes), "device_read")
+        return out
 
     def free(self):
         if getattr(self, "ptr", None):
-            _hip().hipFree(C.c_void_p(self.ptr))
+            lib().vsplat_device_free(C.c_void_p(self.ptr))
             self.ptr = None
 
     def __enter__(self):
