@@ -165,6 +165,42 @@ def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
             "graph_speedup_per_view": round(sum(single) / graph, 4), "sh_storage": "bf16 (all 48 coefficients), fp32 arithmetic"}
 
 
+def other_form(index: int, steps: int, warmup: int, **options) -> dict:
+    """ms/iter of one more single-view workload, gradients mode, fixed point,
+    target in device memory: the precision form (3M Gaussians in 64 blobs, SH
+    stored as bf16, 1920x1080) or the differentiation stress (6M heavy-tailed
+    Gaussians, 1920x1080, the backward recomputing the transmittance)."""
+    w = F.config(index)
+    cam, target = w.cameras[0], w.targets[0]
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True, **options))
+    trainer.set_stage_timing(False)
+    buffer = vsplat.DeviceBuffer(target)
+    try:
+        first = trainer.compute_gradients(cam, target)
+        for _ in range(warmup):
+            trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
+        trainer.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
+        st = trainer.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        trainer.set_stage_timing(True)
+        trainer.reset_timings()
+        for _ in range(max(3, steps // 4)):
+            trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
+        trainer.synchronize()
+        stages = {k: round(v, 4) for k, v in trainer.timings()}
+        gpu_gb = trainer.gpu_bytes() / 1e9
+    finally:
+        buffer.free()
+        trainer.close()
+    return {"workload": w.description, "gaussians": w.cloud.count, "width": cam.width, "height": cam.height,
+            "options": {k: (int(v) if isinstance(v, bool) else v) for k, v in options.items()}, "ms_per_iter": round(ms, 4),
+            "visible": st.visible_splats, "entries": st.tile_entries, "loss": round(float(first.loss), 6),
+            "gpu_gb": round(gpu_gb, 3), "stages_ms": stages}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +212,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-batched", action="store_true", help="skip the eight-view batched measurement")
+    ap.add_argument("--no-other-forms", action="store_true", help="skip the 3M bf16 and 6M recompute workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
@@ -271,6 +308,14 @@ def main():
         bt["plain_vs_graph_per_view"] = round(result["value"] / bt["graph_ms_per_view"], 4)
         log(f"  per view {bt['batched_ms_per_view']} ms batched (speedup {bt['speedup_per_view']}), "
             f"{bt['graph_ms_per_view']} ms as a graph (speedup {bt['graph_speedup_per_view']})")
+    if not args.no_other_forms and args.count is None:
+        result["other_forms"] = {}
+        for key, index, options in (("precision_bf16_3M_1920x1080", 2, {"sh_storage": vsplat.SH_BF16}),
+                                    ("stress_recompute_6M_1920x1080", 4, {"recompute_transmittance": True})):
+            log(f"{key}: generating and timing")
+            form = other_form(index, max(5, args.steps // 2), 2, **options)
+            result["other_forms"][key] = form
+            log(f"  {form['ms_per_iter']} ms per iteration, {form['entries']} entries, {form['gpu_gb']} GB")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline on {threads} threads")

@@ -428,12 +428,41 @@ extern "C" int ob_preprocess(const float *positions, const float *scales, const 
     const float b = dot3(U[0][0], U[0][1], U[0][2], p.T[1][0], p.T[1][1], p.T[1][2]);
     const float c0 = dot3(U[1][0], U[1][1], U[1][2], p.T[1][0], p.T[1][1], p.T[1][2]);
     const float dilation = opt->mip_filter_2d ? 0.1f : 0.3f;
-    const float a = a0 + dilation, cc = c0 + dilation;
-    const float det = fma(a, cc, -(b * b));
+    float a = a0 + dilation, cc = c0 + dilation;
+    float det = fma(a, cc, -(b * b));
+    float rawDet = opt->mip_filter_2d ? fma(a0, c0, -(b * b)) : 0.0f;
+    // Near-degenerate covariance: the 2D covariance built again in double from
+    // the float M and T, and inverted there.
+    const bool degenerate = !(det > 1e-6f) || det < 0x1p-12f * (a * cc);
+    double da = 0.0, db = 0.0, dc = 0.0, ddet = 0.0;
+    if (degenerate) {
+      double SD[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) {
+          double acc = 0.0;
+          for (int j = 0; j < 3; ++j) acc = acc + static_cast<double>(M[r][j]) * static_cast<double>(M[k][j]);
+          SD[r][k] = r == k ? acc + static_cast<double>(f) : acc;
+        }
+      double UD[2][3];
+      for (int r = 0; r < 2; ++r)
+        for (int k = 0; k < 3; ++k) {
+          UD[r][k] = static_cast<double>(p.T[r][0]) * SD[0][k] + static_cast<double>(p.T[r][1]) * SD[1][k] +
+                     static_cast<double>(p.T[r][2]) * SD[2][k];
+        }
+      const double da0 = UD[0][0] * p.T[0][0] + UD[0][1] * p.T[0][1] + UD[0][2] * p.T[0][2];
+      db = UD[0][0] * p.T[1][0] + UD[0][1] * p.T[1][1] + UD[0][2] * p.T[1][2];
+      const double dc0 = UD[1][0] * p.T[1][0] + UD[1][1] * p.T[1][1] + UD[1][2] * p.T[1][2];
+      da = da0 + dilation;
+      dc = dc0 + dilation;
+      ddet = da * dc - db * db;
+      a = static_cast<float>(da);
+      cc = static_cast<float>(dc);
+      det = static_cast<float>(ddet);
+      if (opt->mip_filter_2d) rawDet = static_cast<float>(da0 * dc0 - db * db);
+    }
     if (!(det > 1e-6f)) continue;
     float opacityUsed = opacity3;
     if (opt->mip_filter_2d) {
-      const float rawDet = fma(a0, c0, -(b * b));
       opacityUsed = rawDet > 0.0f ? opacity3 * std::sqrt(rawDet / det) : 0.0f;
       if (!(opacityUsed >= kMinAlpha)) continue;
     }
@@ -461,9 +490,9 @@ extern "C" int ob_preprocess(const float *positions, const float *scales, const 
     const float ex = std::sqrt(reach * a), ey = std::sqrt(reach * cc);
     Splat out;
     const float id = 1.0f / det;
-    out.qx = cc * id;
-    out.qy = -b * id;
-    out.qz = a * id;
+    out.qx = degenerate ? static_cast<float>(dc / ddet) : cc * id;
+    out.qy = degenerate ? static_cast<float>(-db / ddet) : -b * id;
+    out.qz = degenerate ? static_cast<float>(da / ddet) : a * id;
     out.o = opacityUsed;
     out.r = maxf(col[0], 0.0f);
     out.g = maxf(col[1], 0.0f);

@@ -161,3 +161,22 @@ def test_against_fp64_reference(name, fixed):
                             (g.rotations, ref_g.rotations, "rotations"), (g.opacities, ref_g.opacities, "opacities"),
                             (g.sh, ref_g.sh, "sh")):
         _check_close(got, want, what, cos, ratio)
+
+
+def test_needle_splats_bit_exact():
+    """Long, thin splats whose 2D covariance is built and inverted in double:
+    splats, image and fixed-point sums bit for bit against Oracle B."""
+    from test_oracle_b import _needle_scene
+    cloud, cam = _needle_scene(60.0, 0.03), F.train_camera(72, 56)
+    target = F.smooth_target(72, 56, 5)
+    model = O.Model(cloud, cam)
+    r = model.forward()
+    model.loss(r, target, 0.2)
+    model.backward_fixed(r)
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+        t.compute_gradients(cam, target)
+        visible = r.values[r.keys != 0xFFFFFFFF]
+        assert {30, 31, 32} <= set(visible.tolist())
+        np.testing.assert_array_equal(_bits(t.inspect("splats")[visible]), _bits(r.splats[visible]))
+        np.testing.assert_array_equal(_bits(t.inspect("image")), _bits(r.image))
+        np.testing.assert_array_equal(t.inspect("raw"), r.raw_fixed)

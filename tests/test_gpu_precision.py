@@ -1,0 +1,65 @@
+"""The precision form against the FP64 restatement of the reference: 3M
+Gaussians in 64 blobs (sigma 0.2), SH degree 3 stored as bf16, fp32
+arithmetic, the near-degenerate 2D covariance in double; the errors of the
+image, the loss and each parameter group's gradient are stated (printed) and
+bounded, on a 128x128 crop at the centre of the 1920x1080 view (the FP64 path
+over the full view is too slow for a test).
+
+The FP64 side renders the cloud as the trainer stores it (the SH rounded to
+bf16): what is measured is the error of the arithmetic. What bf16 storage
+itself changes, against the cloud as given, is printed beside it.
+
+The sums are float: with fixed-point sums the gradients of small splats fall
+below the resolution, 1/64 of a pixel's 1/(3 W H) share per wave total
+(trainer.cpp:586-591), and the scale gradients here come out with a relative
+error near 0.6 (cosine 0.84). Fixed point is pinned bit for bit against
+Oracle B instead.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import gradcheck as G
+from oracle import oracle as O
+from vsplat import fixtures as F
+import vsplat
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b)), float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+
+
+def test_precision_form_errors_against_fp64():
+    fixed = False
+    w = F.config(2)
+    view = w.cameras[0]
+    crop = dataclasses.replace(view, width=128, height=128, cx=view.cx - (1920 - 128) / 2, cy=view.cy - (1080 - 128) / 2)
+    target = F.smooth_target_explicit(128, 128, 1002)
+    stored = G.as_stored(w.cloud, vsplat.SH_BF16)
+    ref = O.render(stored, crop)
+    ref_loss = O.loss(ref, target, 0.2)
+    ref_g = O.gradients(stored, crop, ref_loss.adjoint)
+    options = vsplat.TrainerOptions(lambda_ssim=0.2, force_fixed_point=fixed, sh_storage=vsplat.SH_BF16)
+    with vsplat.Trainer(w.cloud, options) as t:
+        image = t.render(crop)
+        stats = t.compute_gradients(crop, target)
+        g = t.gradients()
+    image_err = float(np.abs(image[:, :3] - ref.rgb).max())
+    log_t_err = float(np.abs(image[:, 3] - ref.log_t).max())
+    loss_err = abs(stats.loss - ref_loss.loss)
+    print(f"\nfixed={fixed}: visible {stats.visible_splats}, entries {stats.tile_entries}")
+    print(f"  image max abs err {image_err:.3e}, log T {log_t_err:.3e}, loss {loss_err:.3e} "
+          f"(L1 {abs(stats.l1 - ref_loss.l1):.3e}, SSIM {abs(stats.ssim - ref_loss.ssim):.3e})")
+    bars = (1e-3, 0.999999)
+    for got, want, name in zip((g.positions, g.scales, g.rotations, g.opacities, g.sh),
+                               (ref_g.positions, ref_g.scales, ref_g.rotations, ref_g.opacities, ref_g.sh), G.GROUPS):
+        rel, cos = _rel(got, want)
+        print(f"  {name:9s} gradient rel L2 err {rel:.3e}, cosine {cos:.9f}", flush=True)
+        assert rel <= bars[0] and cos > bars[1], name
+    assert image_err < 1e-4 and log_t_err < 1e-3 and loss_err < 1e-5
+    given = O.render(w.cloud, crop)
+    print(f"  bf16 storage itself: image max abs change {np.abs(given.rgb - ref.rgb).max():.3e}")

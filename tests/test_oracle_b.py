@@ -328,3 +328,38 @@ def test_bf16_sh_step_rounds_to_a_neighbour():
     # Geometry does not depend on the storage beyond the colour: positions move alike in sign.
     assert np.mean(np.sign(stepped[0][0] - cloud.positions.reshape(-1)) ==
                    np.sign(stepped[1][0] - cloud.positions.reshape(-1))) > 0.95
+
+
+def _needle_scene(length=20.0, thin=0.03):
+    """30 ordinary Gaussians and three long, thin ones lying across the image,
+    whose 2D covariance is nearly singular: a c - b b cancels in float."""
+    base = F.random_cloud(30, 1, 29)
+    c = F.Cloud.zeros(33, 1)
+    for k in ("positions", "scales", "rotations", "opacities", "sh"):
+        getattr(c, k)[:30] = getattr(base, k)
+    for j, angle in enumerate([45.0, -30.0, 60.0]):
+        i = 30 + j
+        c.positions[i] = (0.1 * j, -0.1 * j, 4.0 + j)
+        c.scales[i] = (math.log(length), math.log(thin), math.log(thin))
+        h = math.radians(angle) / 2
+        c.rotations[i] = (math.cos(h), 0.0, 0.0, math.sin(h))
+        c.opacities[i] = 2.0
+        c.sh[i, 0] = (0.8, -0.5, 0.3)
+    return c
+
+
+@pytest.mark.parametrize("length,thin", [(20.0, 0.03), (60.0, 0.03), (200.0, 0.05), (600.0, 0.3)])
+def test_near_degenerate_covariance_in_double_matches_fp64(length, thin):
+    """Long, thin splats lying across the image: in float their small axis is
+    lost in the covariance sums (image errors of 2e-3 to 0.2 against FP64 when
+    only float is used); built and inverted in double they render within a
+    tenth of the reference's forward tolerance."""
+    cloud, cam = _needle_scene(length, thin), F.train_camera(72, 56)
+    r = O.Model(cloud, cam).forward()
+    ref = O.render(cloud, cam, threads=2)
+    assert np.abs(r.image[:, :3] - ref.rgb).max() < 1e-4
+    assert np.abs(r.image[:, 3] - ref.log_t).max() < 1e-3
+    for i in (30, 31, 32):
+        qx, qy, qz = r.splats[i, :3].astype(np.float64)
+        eig = np.linalg.eigvalsh(np.array([[qx, qy], [qy, qz]]))
+        assert eig[0] > 0 and eig[1] / eig[0] > 1e3  # the splat is a needle

@@ -95,12 +95,49 @@ __device__ __forceinline__ bool project_one(uint32_t i, const GaussianIn &g, con
   const float b = dot3(U[0][0], U[0][1], U[0][2], T[1][0], T[1][1], T[1][2]);
   const float c0 = dot3(U[1][0], U[1][1], U[1][2], T[1][0], T[1][1], T[1][2]);
   const float dilation = opt.mipFilter2d ? 0.1f : 0.3f;
-  const float a = a0 + dilation, cc = c0 + dilation;
-  const float det = fma32(a, cc, -(b * b));
+  float a = a0 + dilation, cc = c0 + dilation;
+  float det = fma32(a, cc, -(b * b));
+  float rawDet = opt.mipFilter2d ? fma32(a0, c0, -(b * b)) : 0.0f;
+  // A near-degenerate covariance (a long, thin splat) loses its small axis in
+  // float: in the sums of Sigma and T Sigma T^T, and then in a c - b b. When
+  // the float determinant is under the cull threshold or under 2^-12 of a c,
+  // the 2D covariance is built again in double from the float M and T, and
+  // inverted there. (With the dilation the true determinant is at least 0.09.)
+  const bool degenerate = !(det > 1e-6f) || det < 0x1p-12f * (a * cc);
+  double da = 0.0, db = 0.0, dc = 0.0, ddet = 0.0;
+  if (degenerate) {
+    double SD[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc = acc + static_cast<double>(M[r][j]) * static_cast<double>(M[k][j]);
+        SD[r][k] = r == k ? acc + static_cast<double>(f) : acc;
+      }
+    double UD[2][3];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        UD[r][k] = static_cast<double>(T[r][0]) * SD[0][k] + static_cast<double>(T[r][1]) * SD[1][k] +
+                   static_cast<double>(T[r][2]) * SD[2][k];
+      }
+    const double da0 = UD[0][0] * T[0][0] + UD[0][1] * T[0][1] + UD[0][2] * T[0][2];
+    db = UD[0][0] * T[1][0] + UD[0][1] * T[1][1] + UD[0][2] * T[1][2];
+    const double dc0 = UD[1][0] * T[1][0] + UD[1][1] * T[1][1] + UD[1][2] * T[1][2];
+    da = da0 + dilation;
+    dc = dc0 + dilation;
+    ddet = da * dc - db * db;
+    a = static_cast<float>(da);
+    cc = static_cast<float>(dc);
+    det = static_cast<float>(ddet);
+    if (opt.mipFilter2d) rawDet = static_cast<float>(da0 * dc0 - db * db);
+  }
   if (!(det > 1e-6f)) return false;
   float opacityUsed = opacity3;
   if (opt.mipFilter2d) {
-    const float rawDet = fma32(a0, c0, -(b * b));
     opacityUsed = rawDet > 0.0f ? opacity3 * sqrtf(rawDet / det) : 0.0f;
     if (!(opacityUsed >= kMinAlpha)) return false;
   }
@@ -136,7 +173,10 @@ __device__ __forceinline__ bool project_one(uint32_t i, const GaussianIn &g, con
   const float reach = reach_of(opacityUsed);
   const float ex = sqrtf(reach * a), ey = sqrtf(reach * cc);
   const float id = 1.0f / det;
-  out = Splat{cc * id, -b * id, a * id, opacityUsed, maxf(col[0], 0.0f), maxf(col[1], 0.0f), maxf(col[2], 0.0f), radius,
+  const float conicX = degenerate ? static_cast<float>(dc / ddet) : cc * id;
+  const float conicY = degenerate ? static_cast<float>(-db / ddet) : -b * id;
+  const float conicZ = degenerate ? static_cast<float>(da / ddet) : a * id;
+  out = Splat{conicX, conicY, conicZ, opacityUsed, maxf(col[0], 0.0f), maxf(col[1], 0.0f), maxf(col[2], 0.0f), radius,
               centerX, centerY, minf(ex, radius), minf(ey, radius)};
   key = 0xFFFFFFFFu - as_uint(tz);
   return true;
