@@ -268,7 +268,7 @@ def main():
     # on this workload (profiles/), when this run is that workload.
     traffic = None
     traffic_file = ROOT / "profiles" / "traffic_1M_1280x720_fixed_gradients.json"
-    kernel_name = {"forward": "forward_kernel", "backward": "backward_kernel<true>"}[dominant[1]]
+    kernel_name = {"forward": "forward_kernel", "backward": "backward_kernel<true, false>"}[dominant[1]]
     if traffic_file.exists() and fixed and N == 1_000_000 and args.mode == "gradients":
         entry = json.loads(traffic_file.read_text())["kernels"].get(kernel_name)
         traffic = None if entry is None else entry["hbm_bytes_per_launch"]
