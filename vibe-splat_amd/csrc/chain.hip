@@ -97,23 +97,31 @@ __device__ __forceinline__ void sh_basis_gradient_term(uint32_t k, float x, floa
   }
 }
 
-// The fixed-point or float sums of one Gaussian in one view, as floats.
-__device__ __forceinline__ bool load_raw(const int64_t *rawFixed, const float *rawFloat, float fixedScale, uint32_t i,
-                                         float g[kRawPerSplat]) {
-  if (rawFixed != nullptr) {
-    const long long *raw = reinterpret_cast<const long long *>(rawFixed) + static_cast<size_t>(i) * kRawPerSplat;
+// The fixed-point or float sums of one Gaussian in one view, as floats, from
+// `raw` (the Gaussian's kRawPerSplat int64 or float values, in global memory or LDS).
+__device__ __forceinline__ bool convert_raw(const uint32_t *raw, bool fixed, float fixedScale, float g[kRawPerSplat]) {
+  if (fixed) {
     const double inverse = 1.0 / static_cast<double>(fixedScale);
 #pragma unroll
-    for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = static_cast<float>(static_cast<double>(raw[k]) * inverse);
+    for (uint32_t k = 0; k < kRawPerSplat; ++k) {
+      const long long v = static_cast<long long>(static_cast<uint64_t>(raw[2 * k]) | (static_cast<uint64_t>(raw[2 * k + 1]) << 32));
+      g[k] = static_cast<float>(static_cast<double>(v) * inverse);
+    }
   } else {
-    const float *raw = rawFloat + static_cast<size_t>(i) * kRawPerSplat;
 #pragma unroll
-    for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = raw[k];
+    for (uint32_t k = 0; k < kRawPerSplat; ++k) g[k] = as_float(raw[k]);
   }
   bool touched = false;
 #pragma unroll
   for (uint32_t k = 0; k < kRawPerSplat; ++k) touched = touched || g[k] != 0.0f;
   return touched;
+}
+__device__ __forceinline__ bool load_raw(const int64_t *rawFixed, const float *rawFloat, float fixedScale, uint32_t i,
+                                         float g[kRawPerSplat]) {
+  if (rawFixed != nullptr) {
+    return convert_raw(reinterpret_cast<const uint32_t *>(rawFixed + static_cast<size_t>(i) * kRawPerSplat), true, fixedScale, g);
+  }
+  return convert_raw(reinterpret_cast<const uint32_t *>(rawFloat + static_cast<size_t>(i) * kRawPerSplat), false, fixedScale, g);
 }
 
 // DensifyStat (train_gradient.comp:94-101), per half image rather than per pixel.
@@ -150,16 +158,12 @@ __device__ __forceinline__ void untouched_terms(const ChainBuffers &B, const Cha
 // from the screen-space sums g to the geometric terms, with the SH part handed
 // to sh(basis, colour gradient, direction, coefficients used) as soon as it is
 // known. Reads the parameters only.
+// Its SH part: the colour, clamped at zero, the SH terms handed to sh(...),
+// and dirTerm, the view direction's part of the position gradient.
 template <int Degree, uint32_t Format, typename ShSink>
-__device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
-                                           const float g[kRawPerSplat], ViewTerms &t, ShSink &&sh) {
-  const float logScale[3] = {B.scales[i * 3], B.scales[i * 3 + 1], B.scales[i * 3 + 2]};
-  const float opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
-  const float s2[3] = {vs_exp(2.0f * logScale[0]), vs_exp(2.0f * logScale[1]), vs_exp(2.0f * logScale[2])};
+__device__ __forceinline__ void view_sh_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+                                              const uint32_t *words, const float g[kRawPerSplat], float dirTerm[3], ShSink &&sh) {
   const float px = B.positions[i * 3], py = B.positions[i * 3 + 1], pz = B.positions[i * 3 + 2];
-  // Colour, clamped at zero, and the view direction's part of the position
-  // gradient, first: the SH work's temporaries are dead before the geometry's
-  // begin, which keeps the register count down.
   const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
   const float dist = sqrtf(fma32(ddz, ddz, fma32(ddy, ddy, ddx * ddx)));
   const float idist = 1.0f / dist;
@@ -168,27 +172,28 @@ __device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewPara
   float basis[16];
   sh_basis(dir[0], dir[1], dir[2], Degree, basis);
   const uint32_t used = sh_coefficients_used(i, c.shCoefficients, o.shDropout, o.shRetained, o.shSeed);
-  // The coefficients are read where they are used (colour, then the direction
-  // term) rather than held, which keeps the kernel's register count down.
-  const uint32_t base = i * c.shWords;
+  // The coefficients (`words`: this Gaussian's, staged in LDS) are read where
+  // they are used (colour, then the direction term) rather than held, which
+  // keeps the kernel's register count down.
+  const uint32_t base = 0;
   float colour[3] = {0.5f, 0.5f, 0.5f};
 #pragma unroll
   for (uint32_t k = 0; k < kCoefficients; ++k) {
 #pragma unroll
     for (uint32_t ch = 0; ch < 3; ++ch) {
-      const float v = sh_value<Format>(B.shWords, base, k * 3 + ch);
+      const float v = sh_value<Format>(words, base, k * 3 + ch);
       colour[ch] = k < used ? fma32(basis[k], v, colour[ch]) : colour[ch];
     }
   }
   const float gcu[3] = {g[6] * (colour[0] > 0.0f ? 1.0f : 0.0f), g[7] * (colour[1] > 0.0f ? 1.0f : 0.0f),
                         g[8] * (colour[2] > 0.0f ? 1.0f : 0.0f)};
-  float dirTerm[3] = {0.0f, 0.0f, 0.0f};
+  dirTerm[0] = dirTerm[1] = dirTerm[2] = 0.0f;
   if (Degree > 0) {
     float gDir[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (uint32_t k = 1; k < kCoefficients; ++k) {
-      const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value<Format>(B.shWords, base, k * 3),
-                           sh_value<Format>(B.shWords, base, k * 3 + 1), sh_value<Format>(B.shWords, base, k * 3 + 2));
+      const float d = dot3(gcu[0], gcu[1], gcu[2], sh_value<Format>(words, base, k * 3),
+                           sh_value<Format>(words, base, k * 3 + 1), sh_value<Format>(words, base, k * 3 + 2));
       float bg[3];
       sh_basis_gradient_term(k, dir[0], dir[1], dir[2], bg);
 #pragma unroll
@@ -199,9 +204,16 @@ __device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewPara
     for (int a = 0; a < 3; ++a) dirTerm[a] = fma32(-dir[a], dd, gDir[a]) * idist;
   }
   sh(basis, gcu, dir, used);
+}
 
-  // Keep the scheduler from hoisting the geometry's work into the SH section.
-  __builtin_amdgcn_sched_barrier(0);
+// Its geometric part: position (with dirTerm added when the colour depends on
+// the view), log-scale, rotation and opacity terms.
+__device__ __forceinline__ void view_geometry_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+                                                    const float g[kRawPerSplat], const float dirTerm[3], bool withDir, ViewTerms &t) {
+  const float logScale[3] = {B.scales[i * 3], B.scales[i * 3 + 1], B.scales[i * 3 + 2]};
+  const float opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
+  const float s2[3] = {vs_exp(2.0f * logScale[0]), vs_exp(2.0f * logScale[1]), vs_exp(2.0f * logScale[2])};
+  const float px = B.positions[i * 3], py = B.positions[i * 3 + 1], pz = B.positions[i * 3 + 2];
   float qw = B.rotations[i * 4], qx = B.rotations[i * 4 + 1], qy = B.rotations[i * 4 + 2], qz = B.rotations[i * 4 + 3];
   const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
   if (qlen > 0.0f) {
@@ -339,10 +351,21 @@ __device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewPara
 
   t.opacity = (fma32(g[0] * k2, k3, o.opacityReg) * opacity) * (1.0f - opacity);
 
-  if (Degree > 0) {
+  if (withDir) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) t.position[a] = t.position[a] + dirTerm[a];
   }
+}
+
+// Both parts in one thread (the batched chain rule).
+template <int Degree, uint32_t Format, typename ShSink>
+__device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+                                           const uint32_t *words, const float g[kRawPerSplat], ViewTerms &t, ShSink &&sh) {
+  float dirTerm[3];
+  view_sh_terms<Degree, Format>(B, c, o, i, words, g, dirTerm, sh);
+  // Keep the scheduler from hoisting the geometry's work into the SH section.
+  __builtin_amdgcn_sched_barrier(0);
+  view_geometry_terms(B, c, o, i, g, dirTerm, Degree > 0, t);
 }
 
 __device__ __forceinline__ void emit_terms(const ChainBuffers &B, const ChainParams &o, uint32_t i, const ViewTerms &t) {
@@ -354,57 +377,66 @@ __device__ __forceinline__ void emit_terms(const ChainBuffers &B, const ChainPar
   emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, t.opacity, o.rates[3], 3);
 }
 
+constexpr uint32_t kChainThreads = 256;
+
+// The SH gradients of one Gaussian row: zero (or unchanged, accumulating) where
+// value(j) is zero; rows of a multiple of four floats go out four at a time.
+template <typename Value>
+__device__ __forceinline__ void store_sh_row(float *out, uint32_t floats, bool add, Value &&value) {
+  if ((floats & 3u) == 0u) {
+    float4 *out4 = reinterpret_cast<float4 *>(out);
+#pragma unroll
+    for (uint32_t q = 0; q < 12; ++q) {
+      if (4 * q >= floats) break;
+      float4 v = make_float4(value(4 * q), value(4 * q + 1), value(4 * q + 2), value(4 * q + 3));
+      if (add) {
+        const float4 p = out4[q];
+        v = make_float4(p.x + v.x, p.y + v.y, p.z + v.z, p.w + v.w);
+      }
+      out4[q] = v;
+    }
+  } else {
+    for (uint32_t j = 0; j < floats; ++j) out[j] = add ? out[j] + value(j) : value(j);
+  }
+}
+
+// One view, in two kernels over the Gaussians: the SH part (colour, SH
+// gradients or the SH step's inputs, and the view direction's term of the
+// position gradient, kept in dirTerms), then the geometric part with the
+// statistics and the emits. Split, each needs far fewer registers than the
+// whole: the chain rule's long dependent chains (divisions, square roots, the
+// fixed-point conversion) are hidden by more waves per SIMD. The SH part runs
+// first because a step changes the positions it reads.
 template <int Degree, uint32_t Format>
-__global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
+__global__ __launch_bounds__(kChainThreads) void chain_sh_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.count) return;
   if (counters[kOverflow] != 0) return; // a view did not fit: this pass computed nothing
-  if (o.mode == kModeStep) adam_step_now(o);
   float g[kRawPerSplat];
   const bool touched = load_raw(o.fixedPoint ? B.rawFixed : nullptr, B.rawFloat, o.fixedScale, i, g);
   const bool zeros = o.mode == kModeGradients || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0);
   const bool penalties = o.opacityReg > 0.0f || o.scaleReg > 0.0f;
   float *shPass = B.shPass + static_cast<size_t>(i) * kShPassFloats;
-
-  if ((touched && (o.flags & kFlagStatistics) != 0) && B.stats != nullptr) {
-    float4 *st = reinterpret_cast<float4 *>(B.stats + static_cast<size_t>(i) * 8);
-    float4 a = st[0], b = st[1];
-    add_statistics(a, b, g, static_cast<float>(c.width), static_cast<float>(c.height), B.splats[i].radius);
-    st[0] = a;
-    st[1] = b;
-  }
-  if (o.mode == kModeStatistics || (!touched && !zeros && !penalties)) {
-    if (o.mode == kModeStep) shPass[9] = as_float(0u); // nothing for the SH pass
-    return;
-  }
-  ViewTerms t;
-  if (!touched) {
-    // Nothing from the pixels; the penalties still apply.
-    untouched_terms(B, o, i, penalties, t);
-    emit_terms(B, o, i, t);
-    if (o.mode == kModeGradients) {
-      float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
-      const bool add = (o.flags & kFlagAccumulate) != 0;
-      for (uint32_t j = 0; j < o.shFloats; ++j) out[j] = add ? out[j] + 0.0f : 0.0f;
-    } else {
-      shPass[9] = as_float(0u);
+  if (o.mode == kModeStatistics || !touched) {
+    // Nothing from the pixels: no SH step, and zero SH gradients where the
+    // single kernel wrote them (gradients mode, or penalties in step mode).
+    if (o.mode == kModeStep) shPass[9] = as_float(0u);
+    if (o.mode == kModeGradients && (zeros || penalties)) {
+      store_sh_row(B.gradSh + static_cast<size_t>(i) * o.shFloats, o.shFloats, (o.flags & kFlagAccumulate) != 0,
+                   [](uint32_t) { return 0.0f; });
     }
     return;
   }
-  view_terms<Degree, Format>(B, c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float dir[3], uint32_t used) {
-    constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
+  constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
+  float dirTerm[3];
+  view_sh_terms<Degree, Format>(B, c, o, i, B.shWords + static_cast<size_t>(i) * c.shWords, g, dirTerm,
+                                [&](const float basis[16], const float gcu[3], const float dir[3], uint32_t used) {
     if (o.mode == kModeGradients) {
       // Coefficients above the active degree get zero (they are stored, not trained).
-      const uint32_t stored = o.shFloats / 3;
-      float *out = B.gradSh + static_cast<size_t>(i) * o.shFloats;
-#pragma unroll
-      for (uint32_t k = 0; k < kCoefficients; ++k)
-#pragma unroll
-        for (uint32_t ch = 0; ch < 3; ++ch) {
-          const float v = k < used ? basis[k] * gcu[ch] : 0.0f;
-          out[k * 3 + ch] = (o.flags & kFlagAccumulate) != 0 ? out[k * 3 + ch] + v : v;
-        }
-      for (uint32_t j = kCoefficients * 3; j < stored * 3; ++j) out[j] = (o.flags & kFlagAccumulate) != 0 ? out[j] + 0.0f : 0.0f;
+      store_sh_row(B.gradSh + static_cast<size_t>(i) * o.shFloats, o.shFloats, (o.flags & kFlagAccumulate) != 0, [&](uint32_t j) {
+        const uint32_t k = j / 3, ch = j - 3 * k;
+        return j < kCoefficients * 3 && k < used ? basis[k] * gcu[ch] : 0.0f;
+      });
     } else {
       // For the SH pass: the colour gradient, the direction and the coefficients used.
       shPass[0] = gcu[0];
@@ -416,6 +448,36 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
       shPass[9] = as_float(used);
     }
   });
+  B.dirTerms[i] = make_float4(dirTerm[0], dirTerm[1], dirTerm[2], 0.0f);
+}
+
+__global__ __launch_bounds__(kChainThreads) void chain_geometry_kernel(ChainBuffers B, ViewParams c, ChainParams o,
+                                                                      const uint32_t *counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.count) return;
+  if (counters[kOverflow] != 0) return;
+  if (o.mode == kModeStep) adam_step_now(o);
+  float g[kRawPerSplat];
+  const bool touched = load_raw(o.fixedPoint ? B.rawFixed : nullptr, B.rawFloat, o.fixedScale, i, g);
+  const bool zeros = o.mode == kModeGradients || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0);
+  const bool penalties = o.opacityReg > 0.0f || o.scaleReg > 0.0f;
+  if ((touched && (o.flags & kFlagStatistics) != 0) && B.stats != nullptr) {
+    float4 *st = reinterpret_cast<float4 *>(B.stats + static_cast<size_t>(i) * 8);
+    float4 a = st[0], b = st[1];
+    add_statistics(a, b, g, static_cast<float>(c.width), static_cast<float>(c.height), B.splats[i].radius);
+    st[0] = a;
+    st[1] = b;
+  }
+  if (o.mode == kModeStatistics || (!touched && !zeros && !penalties)) return;
+  ViewTerms t;
+  if (!touched) {
+    // Nothing from the pixels; the penalties still apply.
+    untouched_terms(B, o, i, penalties, t);
+  } else {
+    const float4 d = B.dirTerms[i];
+    const float dirTerm[3] = {d.x, d.y, d.z};
+    view_geometry_terms(B, c, o, i, g, dirTerm, c.shDegree > 0, t);
+  }
   // Everything above read the parameters before any of them changes.
   emit_terms(B, o, i, t);
 }
@@ -430,13 +492,25 @@ __global__ __launch_bounds__(256) void chain_kernel(ChainBuffers B, ViewParams c
 // geometric parameters take Adam's step from them here, and the SH sums are
 // stored for the SH step.
 template <int Degree, uint32_t Format>
-__global__ __launch_bounds__(256) void chain_views_kernel(ChainBuffers B, ChainViews views, ChainParams o, const uint32_t *counters) {
+__global__ __launch_bounds__(kChainThreads) void chain_views_kernel(ChainBuffers B, ChainViews views, ChainParams o,
+                                                                   const uint32_t *counters) {
   constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
-  __shared__ float shSum[kCoefficients * 3][256];
+  __shared__ float shSum[kCoefficients * 3][kChainThreads];
+  __shared__ uint32_t stage[kChainThreads * 26];
   const uint32_t lane = threadIdx.x;
   const uint32_t i = blockIdx.x * blockDim.x + lane;
-  if (i >= views.view[0].c.count) return;
+  const uint32_t count = views.view[0].c.count, shWords = views.view[0].c.shWords;
   if (counters[kOverflow] != 0) return;
+  {
+    // The block's SH words, read by every view: staged once through LDS with
+    // 16-byte loads (measured faster here than each view reading them again).
+    const uint32_t first = blockIdx.x * kChainThreads;
+    const uint32_t inBlock = count - first < kChainThreads ? count - first : kChainThreads;
+    stage_words<kChainThreads>(stage, B.shWords + static_cast<size_t>(first) * shWords, inBlock * shWords);
+    __syncthreads();
+  }
+  if (i >= count) return;
+  const uint32_t *words = stage + lane * shWords;
   if (o.mode == kModeStep) adam_step_now(o);
   const bool penalties = o.opacityReg > 0.0f || o.scaleReg > 0.0f;
   const bool statistics = (o.flags & kFlagStatistics) != 0 && B.stats != nullptr;
@@ -474,7 +548,7 @@ __global__ __launch_bounds__(256) void chain_views_kernel(ChainBuffers B, ChainV
     }
     ViewTerms t;
     if (touched) {
-      view_terms<Degree, Format>(B, w.c, o, i, g, t, [&](const float basis[16], const float gcu[3], const float[3], uint32_t used) {
+      view_terms<Degree, Format>(B, w.c, o, i, words, g, t, [&](const float basis[16], const float gcu[3], const float[3], uint32_t used) {
 #pragma unroll
         for (uint32_t k = 0; k < kCoefficients; ++k)
 #pragma unroll
@@ -656,9 +730,10 @@ void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewPar
                   const uint32_t *counters) {
   if (view.count == 0) return;
   const dim3 grid((view.count + 255) / 256), block(256);
-#define VSPLAT_CHAIN(D, F) hipLaunchKernelGGL((chain_kernel<D, F>), grid, block, 0, stream, buffers, view, params, counters)
+#define VSPLAT_CHAIN(D, F) hipLaunchKernelGGL((chain_sh_kernel<D, F>), grid, block, 0, stream, buffers, view, params, counters)
   VSPLAT_SH_DISPATCH(view.shDegree, view.shFormat, VSPLAT_CHAIN)
 #undef VSPLAT_CHAIN
+  hipLaunchKernelGGL(chain_geometry_kernel, grid, block, 0, stream, buffers, view, params, counters);
   if (params.mode == kModeStep) {
     const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
     hipLaunchKernelGGL(sh_step_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,

@@ -188,6 +188,22 @@ __device__ __forceinline__ float sh_value(const uint32_t *words, uint32_t base, 
   }
 }
 
+// Copies `words` consecutive 32-bit words from global memory to LDS with
+// 16-byte loads, the block's threads striding over them (both ends 16-byte
+// aligned); a tail of fewer than four words goes word by word. Neighbouring
+// threads read neighbouring vectors, which is what keeps the address and data
+// units from touching a cache line once per lane when each thread's data is a
+// record of its own.
+template <uint32_t Threads>
+__device__ __forceinline__ void stage_words(uint32_t *lds, const uint32_t *src, uint32_t words) {
+  const uint32_t vectors = words / 4;
+  const uint4 *from = reinterpret_cast<const uint4 *>(src);
+  uint4 *to = reinterpret_cast<uint4 *>(lds);
+  for (uint32_t v = threadIdx.x; v < vectors; v += Threads) to[v] = from[v];
+  const uint32_t tail = vectors * 4 + threadIdx.x;
+  if (tail < words) lds[tail] = src[tail];
+}
+
 // World to camera, fused in the order x, y, z then the translation last.
 __device__ __forceinline__ void to_camera(const ViewParams &c, float px, float py, float pz, float &tx, float &ty, float &tz) {
   const float *V = c.view;

@@ -48,6 +48,7 @@ struct ChainBuffers {
   float *stats; // 8 per Gaussian, or null
   uint32_t *statePos, *stateScale, *stateRot, *stateOpacity, *stateSh;
   float *gradPos, *gradScale, *gradRot, *gradOpacity, *gradSh;
+  float4 *dirTerms; // per Gaussian: the view direction's term of the position gradient (between the chain rule's two kernels)
 };
 
 // One view of a batched chain rule: its camera, its screen-space sums (fixed

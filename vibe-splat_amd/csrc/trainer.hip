@@ -156,6 +156,7 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
     launch_fill_u32(stream_, state_[g].as<uint32_t>(), stateFloats[g], 0u);
   }
   shPass_.ensure(n * kShPassFloats * 4);
+  dirTerms_.ensure(n * 16);
   launch_fill_u32(stream_, shPass_.as<uint32_t>(), n * kShPassFloats, 0u);
   stats_.ensure(n * 32);
   launch_fill_u32(stream_, stats_.as<uint32_t>(), n * 8, 0u);
@@ -482,6 +483,7 @@ ChainBuffers Trainer::chain_buffers(const ViewBuffers &v) const {
   b.gradRot = grad_[2].as<float>();
   b.gradOpacity = grad_[3].as<float>();
   b.gradSh = grad_[4].as<float>();
+  b.dirTerms = dirTerms_.as<float4>();
   return b;
 }
 
@@ -957,9 +959,9 @@ uint64_t Trainer::bytes_per_gaussian() const {
   // Values (SH as packed words) and one optimiser word per parameter.
   const uint64_t parameters = (floats - shFloats_ + shWords_) * 4 + floats * 4;
   // Splat, sort keys and values with copies, coverage, tile count, filter, tile
-  // record, sums, SH-pass data, statistics.
+  // record, sums, SH-pass data, the direction term between the chain kernels, statistics.
   const uint64_t raw = fixedPoint_ ? kRawPerSplat * 8 : kRawPerSplat * 4;
-  return parameters + sizeof(Splat) + 4 * 7 + sizeof(TileRecord) + raw + kShPassFloats * 4 + 32;
+  return parameters + sizeof(Splat) + 4 * 7 + sizeof(TileRecord) + raw + kShPassFloats * 4 + 16 + 32;
 }
 
 uint64_t Trainer::gpu_bytes() const {

@@ -186,7 +186,7 @@ private:
   DeviceArray state_[5], grad_[5];
   DeviceArray adamState_; // steps taken, beta1^steps, beta2^steps (ChainParams::adamState)
   // Per Gaussian, shared by the views.
-  DeviceArray shPass_, stats_, filter3d_;
+  DeviceArray shPass_, stats_, filter3d_, dirTerms_;
   bool filterSet_ = false;
   // The views' buffers; the first is the single-view calls' and runs on stream_.
   std::vector<std::unique_ptr<ViewBuffers>> views_;
