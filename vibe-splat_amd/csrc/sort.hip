@@ -85,9 +85,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
                                                                      const uint32_t *__restrict__ totals) {
   constexpr uint32_t kRadix = 1u << Bits;
   constexpr uint32_t kWaves = kSortThreads / 64;
-  __shared__ uint32_t digitBase[kRadix];
+  __shared__ uint32_t digitBase[kRadix]; // the digit's global start for this tile, less its start within the tile
   __shared__ uint32_t waveCount[kWaves][kRadix];
   __shared__ uint32_t waveTotals[kWaves];
+  __shared__ uint32_t sortedKeys[kSortTile], sortedValues[kSortTile];
   const uint32_t count = count_of(countDev, countHost);
   const uint32_t tileStart = blockIdx.x * kSortTile;
   if (tileStart >= count) return; // the same for the whole workgroup
@@ -129,23 +130,50 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
     if (valid && lower == 0) waveCount[wave][digit] = prior + static_cast<uint32_t>(__popcll(same));
   }
   __syncthreads();
-  // Per digit: each wave's offset within the tile.
-  for (uint32_t d = threadIdx.x; d < kRadix; d += kSortThreads) {
-    uint32_t running = digitBase[d];
+  // Per digit: where its keys start within the tile (in digit order), and each
+  // wave's offset from there.
+  {
+    const uint32_t d = threadIdx.x;
+    uint32_t tileCount = 0;
+    if (d < kRadix) {
 #pragma unroll
-    for (uint32_t w = 0; w < kWaves; ++w) {
-      const uint32_t c = waveCount[w][d];
-      waveCount[w][d] = running;
-      running += c;
+      for (uint32_t w = 0; w < kWaves; ++w) tileCount += waveCount[w][d];
+    }
+    uint32_t localStart;
+    block_exclusive_sum(tileCount, localStart, waveTotals);
+    if (d < kRadix) {
+      digitBase[d] -= localStart;
+      uint32_t running = localStart;
+#pragma unroll
+      for (uint32_t w = 0; w < kWaves; ++w) {
+        const uint32_t c = waveCount[w][d];
+        waveCount[w][d] = running;
+        running += c;
+      }
     }
   }
   __syncthreads();
+  // The tile's keys in digit order in LDS, then written out run by run: a
+  // digit's keys are consecutive in both, so neighbouring threads store to
+  // neighbouring addresses.
 #pragma unroll
   for (uint32_t r = 0; r < kSortPerThread; ++r) {
     if (first + r * 64 < count) {
-      const uint32_t dst = waveCount[wave][(key[r] >> shift) & (kRadix - 1u)] + rank[r];
-      keysOut[dst] = key[r];
-      valsOut[dst] = val[r];
+      const uint32_t at = waveCount[wave][(key[r] >> shift) & (kRadix - 1u)] + rank[r];
+      sortedKeys[at] = key[r];
+      sortedValues[at] = val[r];
+    }
+  }
+  __syncthreads();
+  const uint32_t inTile = count - tileStart < kSortTile ? count - tileStart : kSortTile;
+#pragma unroll
+  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+    const uint32_t j = r * kSortThreads + threadIdx.x;
+    if (j < inTile) {
+      const uint32_t k = sortedKeys[j];
+      const uint32_t dst = digitBase[(k >> shift) & (kRadix - 1u)] + j;
+      keysOut[dst] = k;
+      valsOut[dst] = sortedValues[j];
     }
   }
 }
