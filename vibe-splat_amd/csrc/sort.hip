@@ -2,7 +2,7 @@
 // reference's radix_histogram.comp, radix_scatter.comp and scan_blocks.comp,
 // driven by src/core/radix_sort.cpp:58-96). Per digit pass, three launches:
 //
-//   histogram  per tile of 2048 keys, how many hold each digit, stored
+//   histogram  per tile of 2048 or 4096 keys, how many hold each digit, stored
 //              digit-major (hist[digit * tiles + tile]);
 //   scan       one workgroup per digit: the exclusive sum over tiles in place,
 //              and the digit's total;
@@ -13,10 +13,11 @@
 // Ranking uses wave64 ballots instead of the reference's barrier-separated
 // 1-bit splits: for each digit bit a lane's ballot, and-ed together, gives the
 // lanes of its wave holding the same digit, and its rank among them is the
-// popcount below it. Each wave ranks 512 consecutive keys of the tile in
-// order, keeping its running digit counts in LDS, so a tile needs two
-// barriers and the sort is stable, as the tile lists need
-// (test_radix_sort.cpp:15-60 checks the reference's against std::stable_sort).
+// popcount below it. Each wave ranks a quarter of the tile's keys in
+// order, keeping its running digit counts in LDS, so the sort is stable, as
+// the tile lists need (test_radix_sort.cpp:15-60 checks the reference's
+// against std::stable_sort). The tile's keys then go through LDS in digit
+// order, so that each digit's run is written to memory contiguously.
 //
 // Counts may live in device memory (the tile entries are only known on the
 // device): grids are sized for the capacity and tiles past the count exit.
@@ -29,20 +30,23 @@ namespace vsplat {
 namespace {
 
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortPerThread = 8;
-constexpr uint32_t kSortTile = kSortThreads * kSortPerThread; // 2048 keys
-constexpr uint32_t kWaveKeys = kSortTile / (kSortThreads / 64);  // 512 consecutive keys per wave
+// Keys per thread: 8 (tiles of 2048) below kLargeSort keys, 16 (tiles of 4096)
+// from there; the larger tiles halve the histogram and its scan, but at 1M keys
+// they would leave the 256 compute units short of work.
+constexpr uint32_t kSortPerThreadSmall = 8, kSortPerThreadLarge = 16;
+constexpr uint32_t kLargeSort = 4u << 20;
 constexpr uint32_t kScanThreads = 1024;
 
 __device__ __forceinline__ uint32_t count_of(const uint32_t *countDev, uint32_t countHost) {
   return countDev != nullptr ? *countDev : countHost;
 }
 
-template <uint32_t Bits>
+template <uint32_t Bits, uint32_t PerThread>
 __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(const uint32_t *__restrict__ keys, const uint32_t *countDev,
                                                                        uint32_t countHost, uint32_t shift, uint32_t tiles,
                                                                        uint32_t *__restrict__ hist) {
   constexpr uint32_t kRadix = 1u << Bits;
+  constexpr uint32_t kSortTile = kSortThreads * PerThread;
   __shared__ uint32_t counts[kRadix];
   for (uint32_t d = threadIdx.x; d < kRadix; d += kSortThreads) counts[d] = 0;
   __syncthreads();
@@ -50,7 +54,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(const uin
   const uint32_t base = blockIdx.x * kSortTile;
   if (base < count) {
 #pragma unroll
-    for (uint32_t r = 0; r < kSortPerThread; ++r) {
+    for (uint32_t r = 0; r < PerThread; ++r) {
       const uint32_t i = base + r * kSortThreads + threadIdx.x;
       if (i < count) atomicAdd(&counts[(keys[i] >> shift) & (kRadix - 1u)], 1u);
     }
@@ -77,7 +81,7 @@ __global__ __launch_bounds__(kScanThreads) void radix_scan_kernel(uint32_t *__re
   if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-template <uint32_t Bits>
+template <uint32_t Bits, uint32_t PerThread>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
                                                                      uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                                      const uint32_t *countDev, uint32_t countHost, uint32_t shift,
@@ -85,6 +89,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
                                                                      const uint32_t *__restrict__ totals) {
   constexpr uint32_t kRadix = 1u << Bits;
   constexpr uint32_t kWaves = kSortThreads / 64;
+  constexpr uint32_t kSortTile = kSortThreads * PerThread;
+  constexpr uint32_t kWaveKeys = kSortTile / kWaves; // consecutive keys per wave
   __shared__ uint32_t digitBase[kRadix]; // the digit's global start for this tile, less its start within the tile
   __shared__ uint32_t waveCount[kWaves][kRadix];
   __shared__ uint32_t waveTotals[kWaves];
@@ -104,16 +110,16 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
 
   // Rank within the wave's 512 keys, in order.
   const uint64_t below = (uint64_t{1} << lane) - 1u;
-  uint32_t key[kSortPerThread], val[kSortPerThread], rank[kSortPerThread];
+  uint32_t key[PerThread], val[PerThread], rank[PerThread];
   const uint32_t first = tileStart + wave * kWaveKeys + lane;
 #pragma unroll
-  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+  for (uint32_t r = 0; r < PerThread; ++r) {
     const uint32_t i = first + r * 64;
     key[r] = i < count ? keysIn[i] : 0u;
     val[r] = i < count ? valsIn[i] : 0u;
   }
 #pragma unroll
-  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+  for (uint32_t r = 0; r < PerThread; ++r) {
     const bool valid = first + r * 64 < count;
     const uint32_t digit = (key[r] >> shift) & (kRadix - 1u);
     uint64_t same = __ballot(valid);
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
   // digit's keys are consecutive in both, so neighbouring threads store to
   // neighbouring addresses.
 #pragma unroll
-  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+  for (uint32_t r = 0; r < PerThread; ++r) {
     if (first + r * 64 < count) {
       const uint32_t at = waveCount[wave][(key[r] >> shift) & (kRadix - 1u)] + rank[r];
       sortedKeys[at] = key[r];
@@ -167,7 +173,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
   __syncthreads();
   const uint32_t inTile = count - tileStart < kSortTile ? count - tileStart : kSortTile;
 #pragma unroll
-  for (uint32_t r = 0; r < kSortPerThread; ++r) {
+  for (uint32_t r = 0; r < PerThread; ++r) {
     const uint32_t j = r * kSortThreads + threadIdx.x;
     if (j < inTile) {
       const uint32_t k = sortedKeys[j];
@@ -180,23 +186,35 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
 
 uint32_t div_up(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
-template <uint32_t Bits>
+template <uint32_t Bits, uint32_t PerThread>
 void sort_pass(hipStream_t stream, const uint32_t *kIn, const uint32_t *vIn, uint32_t *kOut, uint32_t *vOut, const uint32_t *countDev,
                uint32_t capacity, uint32_t shift, uint32_t *scratch) {
   constexpr uint32_t kRadix = 1u << Bits;
-  const uint32_t tiles = div_up(capacity, kSortTile);
+  const uint32_t tiles = div_up(capacity, kSortThreads * PerThread);
   uint32_t *totals = scratch;
   uint32_t *hist = scratch + 256;
-  hipLaunchKernelGGL((radix_histogram_kernel<Bits>), dim3(tiles), dim3(kSortThreads), 0, stream, kIn, countDev, capacity, shift, tiles,
-                     hist);
+  hipLaunchKernelGGL((radix_histogram_kernel<Bits, PerThread>), dim3(tiles), dim3(kSortThreads), 0, stream, kIn, countDev, capacity,
+                     shift, tiles, hist);
   hipLaunchKernelGGL(radix_scan_kernel, dim3(kRadix), dim3(kScanThreads), 0, stream, hist, tiles, totals);
-  hipLaunchKernelGGL((radix_scatter_kernel<Bits>), dim3(tiles), dim3(kSortThreads), 0, stream, kIn, vIn, kOut, vOut, countDev, capacity,
-                     shift, tiles, hist, totals);
+  hipLaunchKernelGGL((radix_scatter_kernel<Bits, PerThread>), dim3(tiles), dim3(kSortThreads), 0, stream, kIn, vIn, kOut, vOut,
+                     countDev, capacity, shift, tiles, hist, totals);
+}
+
+template <uint32_t Bits>
+void sort_pass(hipStream_t stream, const uint32_t *kIn, const uint32_t *vIn, uint32_t *kOut, uint32_t *vOut, const uint32_t *countDev,
+               uint32_t capacity, uint32_t shift, uint32_t *scratch) {
+  if (capacity >= kLargeSort) {
+    sort_pass<Bits, kSortPerThreadLarge>(stream, kIn, vIn, kOut, vOut, countDev, capacity, shift, scratch);
+  } else {
+    sort_pass<Bits, kSortPerThreadSmall>(stream, kIn, vIn, kOut, vOut, countDev, capacity, shift, scratch);
+  }
 }
 
 } // namespace
 
-size_t radix_sort_scratch_words(uint64_t capacity) { return 256 + static_cast<size_t>(div_up(capacity, kSortTile)) * 256u; }
+size_t radix_sort_scratch_words(uint64_t capacity) {
+  return 256 + static_cast<size_t>(div_up(capacity, kSortThreads * kSortPerThreadSmall)) * 256u;
+}
 
 // Sorts (keys, values) stably by the low `bits` bits of the keys (bits <= 32),
 // in ceil(bits / 8) passes of equal width. The result ends in keys/values;
