@@ -77,7 +77,15 @@ typedef struct {
    * with a front-to-back sweep of its tile instead of reading what the forward
    * pass stored (no reference counterpart; the results are identical). */
   int32_t recompute_transmittance;
+  /* The raster tile shape (no reference counterpart; the reference's is 16x16):
+   * VSPLAT_TILES_16X16, _8X8, _32X8, or _AUTO, which times the three on the
+   * first synchronous call for each view size (vsplat_step, compute_gradients,
+   * step_views, capture_views, or vsplat_autotune_tiles) and keeps the fastest.
+   * Every shape gives the same images and sums; only the tile lists differ. */
+  int32_t tile_shape;
 } vsplat_trainer_options;
+
+enum { VSPLAT_TILES_16X16 = 0, VSPLAT_TILES_8X8 = 1, VSPLAT_TILES_32X8 = 2, VSPLAT_TILES_AUTO = 3 };
 
 enum { VSPLAT_SH_HALF = 0, VSPLAT_SH_BF16 = 1 };
 
@@ -234,6 +242,12 @@ int vsplat_capture_views(vsplat_trainer *trainer, uint32_t n, const vsplat_camer
 int vsplat_launch_captured(vsplat_trainer *trainer, uint32_t replays);
 /* Frees the captured graph. */
 int vsplat_release_captured(vsplat_trainer *trainer);
+/* Times the three tile shapes on one view (forward, loss and backward, no
+ * parameter change; device_target as for vsplat_enqueue), keeps the fastest
+ * for the view's size (used when tile_shape is VSPLAT_TILES_AUTO) and returns
+ * it in *tile_width, *tile_height (either may be NULL). */
+int vsplat_autotune_tiles(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *device_target, uint32_t repeats,
+                          uint32_t *tile_width, uint32_t *tile_height);
 
 /* ---- Introspection, for parity tests: no reference counterpart. ---- */
 

@@ -92,6 +92,7 @@ struct TrainerOptions {
   std::uint64_t memoryBudget = ~std::uint64_t{0};
   bool bf16Sh = false; // this build only: SH stored as bf16 (vsplat_trainer_options::sh_storage)
   bool recomputeTransmittance = false; // this build only (vsplat_trainer_options::recompute_transmittance)
+  int tileShape = VSPLAT_TILES_16X16;  // this build only (vsplat_trainer_options::tile_shape)
 };
 
 // LearningRates (trainer.hpp:51-61).
@@ -157,6 +158,7 @@ public:
     o.memory_budget = options.memoryBudget;
     o.sh_storage = options.bf16Sh ? VSPLAT_SH_BF16 : VSPLAT_SH_HALF;
     o.recompute_transmittance = options.recomputeTransmittance ? 1 : 0;
+    o.tile_shape = options.tileShape;
     detail::check(vsplat_trainer_create(&c, &o, &trainer_));
   }
   ~Trainer() { vsplat_trainer_destroy(trainer_); }

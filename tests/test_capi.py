@@ -45,7 +45,7 @@ def test_defaults_are_the_reference_defaults():
 def test_struct_layouts_match_the_header():
     assert C.sizeof(vsplat.CCamera) == 4 * (16 + 3 + 4 + 2)
     assert C.sizeof(vsplat.CCloud) == 5 * 8 + 8
-    assert C.sizeof(vsplat.COptions) == 4 * 12 + 8 + 8  # sh_storage, recompute_transmittance
+    assert C.sizeof(vsplat.COptions) == 4 * 12 + 8 + 16  # sh_storage, recompute_transmittance, tile_shape, padding
     assert C.sizeof(vsplat.CStats) == 4 * (3 + 2 + 12 + 6)
 
 

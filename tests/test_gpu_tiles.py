@@ -1,0 +1,120 @@
+"""Raster tile shapes (no reference counterpart; the reference's is 16x16):
+8x8, 32x8 and the autotuner's choice, on the GPU.
+
+Each wave holds the same 8x8 block of pixels whatever the tile, and a tile's
+list holds its splats in depth order, so the image, the fixed-point sums, the
+gradients and the statistics are bit for bit those of 16x16 tiles (which are
+pinned against Oracle B in test_gpu_parity.py). The lists themselves differ;
+they are checked for what fixes them: sorted by tile, each tile's entries in
+depth order, the ranges partitioning them, and every pixel a splat reaches
+(brute force in double) lying in a tile listed for it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from vsplat import fixtures as F
+import vsplat
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {vsplat.TILES_8X8: (8, 8), vsplat.TILES_32X8: (32, 8)}
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def _case(name):
+    if name == "ref300":
+        return F.random_cloud(300, 1, 7), F.train_camera(100, 75), F.smooth_target(100, 75, 23)
+    if name == "deg3":
+        return F.random_cloud(300, 3, 7), F.train_camera(72, 56), F.smooth_target(72, 56, 17)
+    if name == "config1_small":
+        w = F.config(1, count=20_000)
+        return w.cloud, w.cameras[0], w.targets[0]
+    raise KeyError(name)
+
+
+def _run(cloud, cam, target, tile_shape):
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=tile_shape)) as t:
+        stats = t.compute_gradients(cam, target)
+        out = {name: t.inspect(name) for name in ("image", "raw", "keys", "values", "entry_keys", "entry_values", "ranges", "splats")}
+        g = t.gradients()
+        out["gradients"] = [g.positions, g.scales, g.rotations, g.opacities, g.sh]
+        out["stats"] = t.densify_stats()
+    return stats, out
+
+
+@pytest.mark.parametrize("name", ["ref300", "deg3", "config1_small"])
+@pytest.mark.parametrize("shape", [vsplat.TILES_8X8, vsplat.TILES_32X8])
+def test_tile_shape_gives_the_same_results(name, shape):
+    cloud, cam, target = _case(name)
+    base_stats, base = _run(cloud, cam, target, vsplat.TILES_16X16)
+    stats, got = _run(cloud, cam, target, shape)
+    print(f"{name} {SHAPES[shape]}: entries {stats.tile_entries} (16x16: {base_stats.tile_entries})", flush=True)
+    assert stats.visible_splats == base_stats.visible_splats
+    assert stats.loss == base_stats.loss and stats.l1 == base_stats.l1 and stats.ssim == base_stats.ssim
+    np.testing.assert_array_equal(_bits(got["image"]), _bits(base["image"]))
+    np.testing.assert_array_equal(got["raw"], base["raw"])
+    for a, b in zip(got["gradients"], base["gradients"]):
+        np.testing.assert_array_equal(_bits(a), _bits(b))
+    np.testing.assert_array_equal(_bits(got["stats"]), _bits(base["stats"]))
+
+    # The lists: sorted by tile, depth order within a tile, ranges partitioning them.
+    tw, th = SHAPES[shape]
+    grid_w, grid_h = (cam.width + tw - 1) // tw, (cam.height + th - 1) // th
+    keys, values = got["entry_keys"][: stats.tile_entries], got["entry_values"][: stats.tile_entries]
+    assert (np.diff(keys.astype(np.int64)) >= 0).all() and keys.max() < grid_w * grid_h
+    rank = np.empty(cloud.count, np.int64)
+    rank[got["values"]] = np.arange(cloud.count)
+    same_tile = np.diff(keys.astype(np.int64)) == 0
+    assert (np.diff(rank[values])[same_tile] > 0).all()
+    ranges = got["ranges"][: grid_w * grid_h]
+    total = 0
+    for t in range(grid_w * grid_h):
+        a, b = ranges[t]
+        if b > a:
+            assert (keys[a:b] == t).all()
+            total += b - a
+    assert total == stats.tile_entries
+
+    # Brute force: every pixel where a splat's alpha reaches 1/255 lies in a tile listed for it.
+    listed = {}
+    for key, value in zip(keys, values):
+        listed.setdefault(int(value), set()).add(int(key))
+    splats = got["splats"]
+    visible = got["values"][got["keys"] != 0xFFFFFFFF]
+    rng = np.random.default_rng(0)
+    for i in rng.choice(visible, min(len(visible), 60), replace=False):
+        s = splats[i].astype(np.float64)
+        radius = int(s[7]) + 1
+        cx, cy = int(s[8]), int(s[9])
+        for y in range(max(0, cy - radius), min(cam.height, cy + radius + 1)):
+            for x in range(max(0, cx - radius), min(cam.width, cx + radius + 1)):
+                dx, dy = x + 0.5 - s[8], y + 0.5 - s[9]
+                if abs(dx) > s[7] or abs(dy) > s[7]:
+                    continue
+                power = -0.5 * (s[0] * dx * dx + s[2] * dy * dy) - s[1] * dx * dy
+                alpha = min(0.99, s[3] * math.exp(power)) if power <= 0 else 0.0
+                if alpha >= (1 / 255) * (1 + 1e-3):
+                    assert (y // th) * grid_w + x // tw in listed.get(int(i), set()), (i, x, y)
+
+
+def test_autotuned_shape_steps_as_16x16():
+    cloud, cam, target = _case("config1_small")
+    rates = vsplat.LearningRates()
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as a, \
+         vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=vsplat.TILES_AUTO)) as b, \
+         vsplat.DeviceBuffer(target) as d:
+        shape = b.autotune_tiles(cam, d.ptr, 2)
+        print(f"autotuned {shape} for {cam.width}x{cam.height}", flush=True)
+        assert shape in ((16, 16), (8, 8), (32, 8))
+        for _ in range(2):
+            a.step(cam, target, rates)
+            b.step(cam, target, rates)
+        ca, cb = a.cloud(), b.cloud()
+    for x, y in zip((ca.positions, ca.scales, ca.rotations, ca.opacities, ca.sh),
+                    (cb.positions, cb.scales, cb.rotations, cb.opacities, cb.sh)):
+        np.testing.assert_array_equal(_bits(x), _bits(y))

@@ -112,7 +112,7 @@ __global__ void merge_flags_kernel(uint32_t *status, const uint32_t *counters) {
 // repeat the test on the stored splat.
 __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
                                                        const Splat *__restrict__ splats, const TileRecord *__restrict__ records,
-                                                       uint32_t count, uint32_t gridW, const uint32_t *__restrict__ offsets,
+                                                       uint32_t count, uint32_t gridW, TileShape shape, const uint32_t *__restrict__ offsets,
                                                        const uint32_t *counters, uint32_t *__restrict__ entryKeys,
                                                        uint32_t *__restrict__ entryValues) {
   __shared__ WalkOwner owners[256]; // per wave: its splats with candidates, packed
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
       box_tile(o, local, tx, ty);
       const uint32_t size = (k + 1 < static_cast<uint32_t>(__popcll(owning)) ? owners[wave + k + 1].start : total) - o.start;
       const uint64_t mask = static_cast<uint64_t>(o.maskLo) | (static_cast<uint64_t>(o.maskHi) << 32);
-      hit = size <= kMaskTiles ? ((mask >> local) & 1ull) != 0 : reaches_tile(o.test, tx, ty);
+      hit = size <= kMaskTiles ? ((mask >> local) & 1ull) != 0 : reaches_tile(o.test, tx, ty, shape);
       gaussian = o.gaussian;
     }
     const uint64_t round = __ballot(hit);
@@ -200,11 +200,11 @@ void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *co
 }
 
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
-                     uint32_t count, uint32_t gridW, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
-                     uint32_t *entryValues) {
+                     uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, const uint32_t *offsets, const uint32_t *counters,
+                     uint32_t *entryKeys, uint32_t *entryValues) {
   if (count == 0) return;
   hipLaunchKernelGGL(bin_emit_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, splats, records, count, gridW,
-                     offsets, counters, entryKeys, entryValues);
+                     TileShape{tileW, tileH}, offsets, counters, entryKeys, entryValues);
 }
 
 void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges) {

@@ -220,6 +220,16 @@ int vsplat_launch_captured(vsplat_trainer *t, uint32_t replays) {
   return guarded([&] { t->impl.launch_captured(replays); });
 }
 
+int vsplat_autotune_tiles(vsplat_trainer *t, const vsplat_camera *camera, const uint8_t *device_target, uint32_t repeats,
+                          uint32_t *tile_width, uint32_t *tile_height) {
+  if (t == nullptr || camera == nullptr) return invalid("null argument");
+  return guarded([&] {
+    const auto shape = t->impl.autotune_tiles(*camera, device_target, repeats);
+    if (tile_width) *tile_width = shape.first;
+    if (tile_height) *tile_height = shape.second;
+  });
+}
+
 int vsplat_release_captured(vsplat_trainer *t) {
   if (t == nullptr) return invalid("null trainer");
   return guarded([&] { t->impl.release_captured(); });

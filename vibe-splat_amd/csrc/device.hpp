@@ -34,6 +34,7 @@ struct ViewParams {
   uint32_t width, height;
   uint32_t count, shDegree, shWords, shCoefficients; // active degree and coefficients
   uint32_t shFormat;                                 // kShHalf or kShBf16
+  uint32_t tileW, tileH;                             // the raster tile shape (powers of two: 8, 16 or 32 by 8 or 16)
 };
 
 // How a Gaussian's SH coefficients are stored, in words of 32 bits:

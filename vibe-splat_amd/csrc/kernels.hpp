@@ -91,17 +91,19 @@ void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32
 // status[overflow] and status[fault] |= counters' (a batch's views).
 void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters);
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
-                     uint32_t count, uint32_t gridW, const uint32_t *offsets, const uint32_t *counters, uint32_t *entryKeys,
-                     uint32_t *entryValues);
+                     uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, const uint32_t *offsets, const uint32_t *counters,
+                     uint32_t *entryKeys, uint32_t *entryValues);
 void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges);
 
+// The raster kernels for one tile shape: 8x8, 16x16 or 32x8 (tileW x tileH).
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
-                    uint32_t height, const float background[3], float4 *image, uint32_t *stops);
+                    uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops);
 // recompute: the transmittance and stops are recomputed per tile rather than
 // read from the forward pass (image and stops may then be null).
 void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
-                     const uint2 *ranges, uint32_t width, uint32_t height, const float background[3], const float4 *image,
-                     const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed, float *rawFloat);
+                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
+                     const float4 *image, const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed,
+                     float *rawFloat);
 
 void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
                  float *fields, float4 *adjoint, float2 *tileTotals, double *sums);

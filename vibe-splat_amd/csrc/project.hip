@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   uint32_t t[4] = {0, 0, 0, 0};
   uint32_t candidates = 0;
   if (visible) {
-    splat_tiles(s.cx, s.cy, s.ex, s.ey, gridW, gridH, t);
+    splat_tiles(s.cx, s.cy, s.ex, s.ey, gridW, gridH, TileShape{c.tileW, c.tileH}, t);
     candidates = (t[2] - t[0] + 1) * (t[3] - t[1] + 1);
   }
   const uint32_t width = t[2] - t[0] + 1;
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
       const WalkOwner &o = owners[wave + slot];
       uint32_t tx, ty;
       box_tile(o, j - o.start, tx, ty);
-      hit = reaches_tile(o.test, tx, ty);
+      hit = reaches_tile(o.test, tx, ty, TileShape{c.tileW, c.tileH});
     }
     const uint64_t round = __ballot(hit);
     // This lane's own candidates within the round: lanes start - base .. start + candidates - base.

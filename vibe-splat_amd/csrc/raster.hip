@@ -11,9 +11,9 @@
 // A wave skips a splat none of its lanes is a candidate for; otherwise every
 // lane evaluates the Gaussian without branches and the result is selected.
 //
-// Wave w of a tile holds the 8x8 quadrant (w & 1, w >> 1), lane l its pixel
-// (l & 7, l >> 3): a splat's footprint fills more of a square block's lanes
-// than of a 16x4 strip's.
+// A tile is 8x8, 16x16 (the reference's) or 32x8 pixels, one thread per pixel.
+// Each wave holds an 8x8 block of the tile, lane l its pixel (l & 7, l >> 3):
+// a splat's footprint fills more of a square block's lanes than of a strip's.
 //
 // Backward: for each splat the 14
 // per-pixel sums are reduced over the wave by a transposed butterfly in which
@@ -34,46 +34,53 @@ struct Staged {
   float qx, qy, qz, o;
   float r, g, b, pad0;
   float cx, cy;
-  uint32_t mask; // columns in bits 0-15, rows in bits 16-31, tile-local
-  uint32_t pad1;
+  uint32_t cols, rows; // the tile's columns and rows the splat can touch, bit per tile-local index
 };
 static_assert(sizeof(Staged) == 48, "48-byte staged splat");
 
 struct TileGeometry {
   uint32_t x0, y0, x1, y1;
 };
+template <uint32_t TW, uint32_t TH>
 __device__ __forceinline__ TileGeometry tile_geometry(uint32_t tile, uint32_t gridW, uint32_t width, uint32_t height) {
   const uint32_t tx = tile % gridW, ty = tile / gridW;
-  const uint32_t x0 = tx * kTileSize, y0 = ty * kTileSize;
-  return {x0, y0, x0 + kTileSize < width ? x0 + kTileSize : width, y0 + kTileSize < height ? y0 + kTileSize : height};
+  const uint32_t x0 = tx * TW, y0 = ty * TH;
+  return {x0, y0, x0 + TW < width ? x0 + TW : width, y0 + TH < height ? y0 + TH : height};
 }
 
 // The pixels of the tile a splat can touch: inside the extent box and within
 // the radius (|d| <= r on both axes, d from the pixel centre).
+template <uint32_t TW, uint32_t TH>
 __device__ __forceinline__ Staged stage(const Splat &s, const TileGeometry &geo) {
   const uint32_t box = splat_box(s, geo.x0, geo.y0, geo.x1, geo.y1);
-  uint32_t mask = 0;
+  uint32_t cols = 0, rows = 0;
   if (box != 0u) {
     const uint32_t bx0 = box & 0xFFu, by0 = (box >> 8) & 0xFFu, bx1 = (box >> 16) & 0xFFu, by1 = box >> 24;
+    constexpr uint32_t kSide = TW > TH ? TW : TH;
     // Kept as a loop: unrolled, its temporaries would set the kernel's register count.
 #pragma unroll 1
-    for (uint32_t k = 0; k < 16; ++k) {
+    for (uint32_t k = 0; k < kSide; ++k) {
       const float dx = (static_cast<float>(geo.x0 + k) + 0.5f) - s.cx;
       const float dy = (static_cast<float>(geo.y0 + k) + 0.5f) - s.cy;
       const bool col = k >= bx0 && k < bx1 && !(fabsf(dx) > s.radius);
       const bool row = k >= by0 && k < by1 && !(fabsf(dy) > s.radius);
-      mask |= (col ? 1u : 0u) << k;
-      mask |= (row ? 1u : 0u) << (16 + k);
+      cols |= (col ? 1u : 0u) << k;
+      rows |= (row ? 1u : 0u) << k;
     }
   }
-  return Staged{s.qx, s.qy, s.qz, s.o, s.r, s.g, s.b, 0.0f, s.cx, s.cy, mask, 0u};
+  return Staged{s.qx, s.qy, s.qz, s.o, s.r, s.g, s.b, 0.0f, s.cx, s.cy, cols, rows};
 }
 
-// Thread t's pixel in the tile.
-__device__ __forceinline__ uint32_t local_x(uint32_t t) { return ((t >> 6) & 1u) * 8u + (t & 7u); }
-__device__ __forceinline__ uint32_t local_y(uint32_t t) { return (t >> 7) * 8u + ((t >> 3) & 7u); }
+// Thread t's pixel in a tile TW pixels wide: wave w holds the 8x8 block
+// (w mod TW/8, w div TW/8), lane l its pixel (l & 7, l >> 3). Every tile shape
+// has the same 8x8 blocks for its waves, so the per-wave sums of the backward
+// (and so the fixed-point results) do not depend on the shape.
+template <uint32_t TW>
+__device__ __forceinline__ uint32_t local_x(uint32_t t) { return ((t >> 6) % (TW / 8u)) * 8u + (t & 7u); }
+template <uint32_t TW>
+__device__ __forceinline__ uint32_t local_y(uint32_t t) { return ((t >> 6) / (TW / 8u)) * 8u + ((t >> 3) & 7u); }
 
-__device__ __forceinline__ bool covers(uint32_t mask, uint32_t lx, uint32_t ly) { return ((mask >> lx) & (mask >> (16u + ly)) & 1u) != 0u; }
+__device__ __forceinline__ bool covers(const Staged &s, uint32_t lx, uint32_t ly) { return ((s.cols >> lx) & (s.rows >> ly) & 1u) != 0u; }
 
 // The Gaussian's exponent at offset (dx, dy) (splat_at, train_types.glsl:88).
 __device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
@@ -84,22 +91,23 @@ __device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
 // (with Colour), the transmittance and its log, and the stop index of the
 // calling thread's pixel. The forward pass and the backward pass that
 // recomputes the transmittance both run this, so they agree bit for bit.
-template <bool Colour>
+template <bool Colour, uint32_t TW, uint32_t TH>
 __device__ __forceinline__ void composite_tile(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues, uint2 range,
                                                const TileGeometry &geo, Staged *batch, bool inside, float px, float py, uint32_t lx,
                                                uint32_t ly, float &r, float &g, float &b, float &T, float &logT, uint32_t &stop) {
+  constexpr uint32_t kBatch = TW * TH; // one splat staged per thread
   stop = range.x;
   bool done = !inside;
   for (uint32_t end = range.y; end > range.x;) {
-    const uint32_t count = end - range.x < kTilePixels ? end - range.x : kTilePixels;
+    const uint32_t count = end - range.x < kBatch ? end - range.x : kBatch;
     end -= count;
     // Also the barrier that keeps the batch from being overwritten while read.
     if (__syncthreads_count(!done) == 0) break;
-    if (threadIdx.x < count) batch[threadIdx.x] = stage(load_splat(splats, entryValues[end + threadIdx.x]), geo);
+    if (threadIdx.x < count) batch[threadIdx.x] = stage<TW, TH>(load_splat(splats, entryValues[end + threadIdx.x]), geo);
     __syncthreads();
     for (uint32_t k = count; k-- > 0;) {
       const Staged s = batch[k];
-      const bool candidate = !done && covers(s.mask, lx, ly);
+      const bool candidate = !done && covers(s, lx, ly);
       if (!__any(candidate)) {
         if (__all(done)) break;
         continue;
@@ -131,20 +139,21 @@ __device__ __forceinline__ void composite_tile(const Splat *__restrict__ splats,
   }
 }
 
-__global__ __launch_bounds__(256) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
-                                                      const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
-                                                      float bgR, float bgG, float bgB, float4 *__restrict__ image,
-                                                      uint32_t *__restrict__ stops) {
-  __shared__ Staged batch[kTilePixels];
+template <uint32_t TW, uint32_t TH>
+__global__ __launch_bounds__(TW * TH) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
+                                                          uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
+                                                          uint32_t *__restrict__ stops) {
+  __shared__ Staged batch[TW * TH];
   const uint32_t tile = blockIdx.x;
-  const TileGeometry geo = tile_geometry(tile, gridW, width, height);
-  const uint32_t lx = local_x(threadIdx.x), ly = local_y(threadIdx.x);
+  const TileGeometry geo = tile_geometry<TW, TH>(tile, gridW, width, height);
+  const uint32_t lx = local_x<TW>(threadIdx.x), ly = local_y<TW>(threadIdx.x);
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, logT = 0.0f;
   uint32_t stop;
-  composite_tile<true>(splats, entryValues, ranges[tile], geo, batch, inside, px, py, lx, ly, r, g, b, T, logT, stop);
+  composite_tile<true, TW, TH>(splats, entryValues, ranges[tile], geo, batch, inside, px, py, lx, ly, r, g, b, T, logT, stop);
   if (inside) {
     const uint32_t at = y * width + x;
     image[at] = make_float4(fma32(T, bgR, r), fma32(T, bgG, g), fma32(T, bgB, b), logT);
@@ -204,19 +213,20 @@ __device__ __forceinline__ void reduce_wave(float (&v)[16], uint32_t lane) {
 // Back to front over the tile (train_backward.glsl:78-158). With Recompute the
 // pixel's log transmittance and stop come from a front-to-back sweep of its
 // own instead of the forward pass's image and stops, which are then not read.
-template <bool Fixed, bool Recompute>
-__global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+template <bool Fixed, bool Recompute, uint32_t TW, uint32_t TH>
+__global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
                                                        const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                                        float bgR, float bgG, float bgB, const float4 *__restrict__ image,
                                                        const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
                                                        float fixedScale, unsigned long long *__restrict__ rawFixed,
                                                        float *__restrict__ rawFloat) {
-  __shared__ Staged batch[kTilePixels];
-  __shared__ uint32_t batchIndex[kTilePixels];
+  constexpr uint32_t kBatch = TW * TH; // one splat staged per thread
+  __shared__ Staged batch[kBatch];
+  __shared__ uint32_t batchIndex[kBatch];
   __shared__ uint32_t tileStop;
   const uint32_t tile = blockIdx.x;
-  const TileGeometry geo = tile_geometry(tile, gridW, width, height);
-  const uint32_t lx = local_x(threadIdx.x), ly = local_y(threadIdx.x), lane = threadIdx.x & 63u;
+  const TileGeometry geo = tile_geometry<TW, TH>(tile, gridW, width, height);
+  const uint32_t lx = local_x<TW>(threadIdx.x), ly = local_y<TW>(threadIdx.x), lane = threadIdx.x & 63u;
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
@@ -229,7 +239,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
   if constexpr (Recompute) {
     float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f;
     logTotal = 0.0f;
-    composite_tile<false>(splats, entryValues, range, geo, batch, inside, px, py, lx, ly, r, g, b, T, logTotal, stop);
+    composite_tile<false, TW, TH>(splats, entryValues, range, geo, batch, inside, px, py, lx, ly, r, g, b, T, logTotal, stop);
     if (!inside) stop = range.y;
   } else {
     logTotal = inside ? image[at].w : 0.0f;
@@ -242,18 +252,18 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
   const uint32_t first = tileStop;
   float Br = bgR, Bg = bgG, Bb = bgB, logBehind = 0.0f;
 
-  for (uint32_t start = first; start < range.y; start += kTilePixels) {
-    const uint32_t count = range.y - start < kTilePixels ? range.y - start : kTilePixels;
+  for (uint32_t start = first; start < range.y; start += kBatch) {
+    const uint32_t count = range.y - start < kBatch ? range.y - start : kBatch;
     __syncthreads();
     if (threadIdx.x < count) {
       const uint32_t index = entryValues[start + threadIdx.x];
       batchIndex[threadIdx.x] = index;
-      batch[threadIdx.x] = stage(load_splat(splats, index), geo);
+      batch[threadIdx.x] = stage<TW, TH>(load_splat(splats, index), geo);
     }
     __syncthreads();
     for (uint32_t k = 0; k < count; ++k) {
       const Staged s = batch[k];
-      const bool candidate = inside && start + k >= stop && covers(s.mask, lx, ly);
+      const bool candidate = inside && start + k >= stop && covers(s, lx, ly);
       if (!__any(candidate)) continue;
       const float dx = px - s.cx, dy = py - s.cy;
       const float power = power_at(s, dx, dy);
@@ -313,35 +323,57 @@ __global__ __launch_bounds__(256) void backward_kernel(const Splat *__restrict__
 
 } // namespace
 
+// Calls LAUNCH(TW, TH) with the tile shape as compile-time constants.
+#define VSPLAT_TILE_DISPATCH(tileW, tileH, LAUNCH)                                                                                 \
+  if ((tileW) == 8 && (tileH) == 8) {                                                                                              \
+    LAUNCH(8, 8);                                                                                                                  \
+  } else if ((tileW) == 32 && (tileH) == 8) {                                                                                      \
+    LAUNCH(32, 8);                                                                                                                 \
+  } else {                                                                                                                         \
+    LAUNCH(16, 16);                                                                                                                \
+  }
+
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
-                    uint32_t height, const float background[3], float4 *image, uint32_t *stops) {
-  const uint32_t gridW = (width + kTileSize - 1) / kTileSize, gridH = (height + kTileSize - 1) / kTileSize;
-  hipLaunchKernelGGL(forward_kernel, dim3(gridW * gridH), dim3(kTilePixels), 0, stream, splats, entryValues, ranges, width, height,
-                     gridW, background[0], background[1], background[2], image, stops);
+                    uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops) {
+  const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
+#define VSPLAT_FORWARD(W, H)                                                                                                       \
+  hipLaunchKernelGGL((forward_kernel<W, H>), dim3(gridW * gridH), dim3(W * H), 0, stream, splats, entryValues, ranges, width,     \
+                     height, gridW, background[0], background[1], background[2], image, stops)
+  VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD)
+#undef VSPLAT_FORWARD
 }
 
 void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
-                     const uint2 *ranges, uint32_t width, uint32_t height, const float background[3], const float4 *image,
-                     const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed, float *rawFloat) {
-  const uint32_t gridW = (width + kTileSize - 1) / kTileSize, gridH = (height + kTileSize - 1) / kTileSize;
-#define VSPLAT_BACKWARD(F, R)                                                                                                      \
-  hipLaunchKernelGGL((backward_kernel<F, R>), dim3(gridW * gridH), dim3(kTilePixels), 0, stream, splats, entryValues, ranges,     \
+                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
+                     const float4 *image, const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed,
+                     float *rawFloat) {
+  const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
+#define VSPLAT_BACKWARD_SHAPE(F, R, W, H)                                                                                          \
+  hipLaunchKernelGGL((backward_kernel<F, R, W, H>), dim3(gridW * gridH), dim3(W * H), 0, stream, splats, entryValues, ranges,     \
                      width, height, gridW, background[0], background[1], background[2], image, adjoint, stops, fixedScale,         \
                      reinterpret_cast<unsigned long long *>(rawFixed), rawFloat)
+#define VSPLAT_BACKWARD_TT(W, H) VSPLAT_BACKWARD_SHAPE(true, true, W, H)
+#define VSPLAT_BACKWARD_TF(W, H) VSPLAT_BACKWARD_SHAPE(true, false, W, H)
+#define VSPLAT_BACKWARD_FT(W, H) VSPLAT_BACKWARD_SHAPE(false, true, W, H)
+#define VSPLAT_BACKWARD_FF(W, H) VSPLAT_BACKWARD_SHAPE(false, false, W, H)
   if (fixedPoint) {
     if (recompute) {
-      VSPLAT_BACKWARD(true, true);
+      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TT)
     } else {
-      VSPLAT_BACKWARD(true, false);
+      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TF)
     }
   } else {
     if (recompute) {
-      VSPLAT_BACKWARD(false, true);
+      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FT)
     } else {
-      VSPLAT_BACKWARD(false, false);
+      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FF)
     }
   }
-#undef VSPLAT_BACKWARD
+#undef VSPLAT_BACKWARD_FF
+#undef VSPLAT_BACKWARD_FT
+#undef VSPLAT_BACKWARD_TF
+#undef VSPLAT_BACKWARD_TT
+#undef VSPLAT_BACKWARD_SHAPE
 }
 
 } // namespace vsplat

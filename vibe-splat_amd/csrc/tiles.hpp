@@ -1,4 +1,5 @@
-// Which 16x16 tiles a splat reaches (train_types.glsl:34-77), and the
+// Which tiles a splat reaches (train_types.glsl:34-77, for the reference's
+// 16x16; here any power-of-two width and height, 8 to 32), and the
 // wave-cooperative walk over candidate tiles that projection and list
 // emission share.
 //
@@ -15,13 +16,21 @@
 
 namespace vsplat {
 
+// A tile shape: width and height in pixels, powers of two (the reference's is
+// 16 x 16). Dividing by them is multiplying by an exact power of two.
+struct TileShape {
+  uint32_t w, h;
+};
+
 // Inclusive tile box of a splat on a gridW x gridH grid (train_types.glsl:34-39).
-__device__ __forceinline__ void splat_tiles(float cx, float cy, float ex, float ey, uint32_t gridW, uint32_t gridH, uint32_t t[4]) {
+__device__ __forceinline__ void splat_tiles(float cx, float cy, float ex, float ey, uint32_t gridW, uint32_t gridH, TileShape shape,
+                                            uint32_t t[4]) {
   const float lastX = static_cast<float>(gridW) - 1.0f, lastY = static_cast<float>(gridH) - 1.0f;
-  t[0] = static_cast<uint32_t>(clampf(floorf((cx - ex - 1.0f) * 0.0625f), 0.0f, lastX));
-  t[1] = static_cast<uint32_t>(clampf(floorf((cy - ey - 1.0f) * 0.0625f), 0.0f, lastY));
-  t[2] = static_cast<uint32_t>(clampf(floorf((cx + ex + 1.0f) * 0.0625f), 0.0f, lastX));
-  t[3] = static_cast<uint32_t>(clampf(floorf((cy + ey + 1.0f) * 0.0625f), 0.0f, lastY));
+  const float ix = 1.0f / static_cast<float>(shape.w), iy = 1.0f / static_cast<float>(shape.h);
+  t[0] = static_cast<uint32_t>(clampf(floorf((cx - ex - 1.0f) * ix), 0.0f, lastX));
+  t[1] = static_cast<uint32_t>(clampf(floorf((cy - ey - 1.0f) * iy), 0.0f, lastY));
+  t[2] = static_cast<uint32_t>(clampf(floorf((cx + ex + 1.0f) * ix), 0.0f, lastX));
+  t[3] = static_cast<uint32_t>(clampf(floorf((cy + ey + 1.0f) * iy), 0.0f, lastY));
 }
 
 // What the reaches-tile test needs of a splat: the conic, the centre, the
@@ -48,9 +57,10 @@ __device__ __forceinline__ float edge_min_horizontal(const TileTest &s, float v,
 // Whether the splat's 1/255 ellipse reaches a pixel centre of tile (tx, ty)
 // (train_types.glsl:56-77), with the reference's slack so that rounding never
 // drops a pixel the pixel loops would composite.
-__device__ __forceinline__ bool reaches_tile(const TileTest &s, uint32_t tx, uint32_t ty) {
-  const float lowx = (static_cast<float>(tx) * 16.0f + 0.5f) - s.cx, lowy = (static_cast<float>(ty) * 16.0f + 0.5f) - s.cy;
-  const float highx = lowx + 15.0f, highy = lowy + 15.0f;
+__device__ __forceinline__ bool reaches_tile(const TileTest &s, uint32_t tx, uint32_t ty, TileShape shape) {
+  const float w = static_cast<float>(shape.w), h = static_cast<float>(shape.h);
+  const float lowx = (static_cast<float>(tx) * w + 0.5f) - s.cx, lowy = (static_cast<float>(ty) * h + 0.5f) - s.cy;
+  const float highx = lowx + (w - 1.0f), highy = lowy + (h - 1.0f);
   if (lowx <= 0.0f && lowy <= 0.0f && highx >= 0.0f && highy >= 0.0f) return true;
   float least = 1e30f;
   least = minf(least, edge_min_vertical(s, lowx, lowy, highy));

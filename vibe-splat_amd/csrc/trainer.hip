@@ -117,6 +117,9 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
     throw Error(VSPLAT_INVALID_ARGUMENT, "sh_storage must be VSPLAT_SH_HALF or VSPLAT_SH_BF16");
   }
   shFormat_ = options.sh_storage == VSPLAT_SH_BF16 ? kShBf16 : kShHalf;
+  if (options.tile_shape < VSPLAT_TILES_16X16 || options.tile_shape > VSPLAT_TILES_AUTO) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, "tile_shape must be one of VSPLAT_TILES_16X16, _8X8, _32X8 or _AUTO");
+  }
   shWords_ = shFormat_ == kShBf16 ? bf16_words(coefficients) : half_words(coefficients);
   // Float atomics are the default on this GPU, as on any with them (trainer.cpp:181).
   fixedPoint_ = options.force_fixed_point != 0;
@@ -260,27 +263,112 @@ ViewParams Trainer::view_params(const vsplat_camera &cam) const {
   return v;
 }
 
+// The raster tile shape for a view size: the option's, or in auto mode the
+// one the autotuner chose for that size (16x16 until it has run).
+std::pair<uint32_t, uint32_t> Trainer::tile_shape_for(int width, int height) const {
+  switch (options_.tile_shape) {
+  case VSPLAT_TILES_8X8: return {8u, 8u};
+  case VSPLAT_TILES_32X8: return {32u, 8u};
+  case VSPLAT_TILES_AUTO: {
+    const auto found = tuned_.find({width, height});
+    return found != tuned_.end() ? found->second : std::make_pair(16u, 16u);
+  }
+  default: return {16u, 16u};
+  }
+}
+
+bool Trainer::needs_tuning(int width, int height) const {
+  return options_.tile_shape == VSPLAT_TILES_AUTO && tuned_.find({width, height}) == tuned_.end();
+}
+
 void Trainer::ensure_size(ViewBuffers &b, int width, int height) {
+  const auto shape = tile_shape_for(width, height);
+  ensure_layout(b, width, height, shape.first, shape.second);
+}
+
+void Trainer::ensure_layout(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH) {
   if (width <= 0 || height <= 0) throw Error(VSPLAT_RUNTIME_ERROR, "bad image size");
-  if (width == b.width && height == b.height) return;
-  const uint64_t gridW = div_up(static_cast<uint64_t>(width), kTileSize), gridH = div_up(static_cast<uint64_t>(height), kTileSize);
+  if (width == b.width && height == b.height && tileW == b.tileW && tileH == b.tileH) return;
+  const uint64_t gridW = div_up(static_cast<uint64_t>(width), tileW), gridH = div_up(static_cast<uint64_t>(height), tileH);
   if (gridW * gridH > (1u << 24)) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too large for the tile index");
   // Tile coordinates are kept as 16-bit pairs per splat.
   if (gridW > 0xFFFFu || gridH > 0xFFFFu) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too wide or too tall");
   b.width = width;
   b.height = height;
+  b.tileW = tileW;
+  b.tileH = tileH;
   b.gridW = static_cast<uint32_t>(gridW);
   b.gridH = static_cast<uint32_t>(gridH);
   b.tiles = b.gridW * b.gridH;
   b.tileBits = 1;
   while ((uint64_t{1} << b.tileBits) < b.tiles) ++b.tileBits;
+  // The loss works on 16x16 tiles of its own, whatever the raster's shape.
+  b.lossTiles = static_cast<uint32_t>(div_up(static_cast<uint64_t>(width), 16) * div_up(static_cast<uint64_t>(height), 16));
   const size_t pixels = static_cast<size_t>(width) * static_cast<size_t>(height);
   b.image.ensure(pixels * 16);
   b.stops.ensure(pixels * 4);
   b.adjoint.ensure(pixels * 16);
   b.fields.ensure(pixels * 36);
   b.ranges.ensure(static_cast<size_t>(b.tiles) * 8);
-  b.tileTotals.ensure(static_cast<size_t>(b.tiles) * 8);
+  b.tileTotals.ensure(static_cast<size_t>(b.lossTiles) * 8);
+}
+
+// Times the view's passes up to the backward (forward, loss, backward: no
+// parameter changes) with each tile shape and keeps the fastest for the view's
+// size. Every shape gives the same images and sums (each wave holds the same
+// 8x8 block of pixels), so the choice changes only the time.
+std::pair<uint32_t, uint32_t> Trainer::autotune_tiles(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t repeats) {
+  if (deviceTarget == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no target");
+  ViewBuffers &b = *views_[0];
+  const ViewParams v = view_params(camera);
+  const float savedDropout = dropoutNow_;
+  dropoutNow_ = 0.0f;
+  hipEvent_t begin = nullptr, end = nullptr;
+  check(hipEventCreate(&begin), "hipEventCreate");
+  check(hipEventCreate(&end), "hipEventCreate");
+  const std::pair<uint32_t, uint32_t> shapes[3] = {{16u, 16u}, {8u, 8u}, {32u, 8u}};
+  std::pair<uint32_t, uint32_t> best = shapes[0];
+  float bestMs = 0.0f;
+  try {
+    for (const auto &shape : shapes) {
+      ensure_layout(b, camera.width, camera.height, shape.first, shape.second);
+      // One run sizes the tile-entry buffers for this shape.
+      for (int attempt = 0;; ++attempt) {
+        enqueue_passes(b, v, deviceTarget, kModeGradients, nullptr, false, false);
+        bool overflow = false;
+        const vsplat_step_stats stats = read_stats(overflow);
+        if (!overflow) break;
+        launch_fill_u32(stream_, b.counters.as<uint32_t>() + kOverflow, 2, 0u);
+        if (attempt > 0 && stats.tile_entries <= b.entryCapacity) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+        grow_entries(b, stats.tile_entries);
+      }
+      float least = 0.0f;
+      for (uint32_t r = 0; r < (repeats > 0 ? repeats : 1); ++r) {
+        check(hipEventRecord(begin, stream_), "hipEventRecord");
+        enqueue_passes(b, v, deviceTarget, kModeGradients, nullptr, false, false);
+        check(hipEventRecord(end, stream_), "hipEventRecord");
+        check(hipEventSynchronize(end), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, begin, end), "hipEventElapsedTime");
+        least = r == 0 ? ms : std::min(least, ms);
+      }
+      if (bestMs == 0.0f || least < bestMs) {
+        bestMs = least;
+        best = shape;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(begin);
+    (void)hipEventDestroy(end);
+    dropoutNow_ = savedDropout;
+    throw;
+  }
+  (void)hipEventDestroy(begin);
+  (void)hipEventDestroy(end);
+  dropoutNow_ = savedDropout;
+  tuned_[{camera.width, camera.height}] = best;
+  ensure_size(b, camera.width, camera.height);
+  return best;
 }
 
 void Trainer::grow_entries(ViewBuffers &b, uint64_t entries) {
@@ -372,12 +460,15 @@ std::vector<std::pair<std::string, double>> Trainer::timings() const {
   return out;
 }
 
-void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &v, const uint8_t *deviceTarget, uint32_t mode,
+void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode,
                              const vsplat_learning_rates *rates, bool timed, bool chain) {
   hipStream_t s = b.stream;
   current_ = nullptr;
   if (timed) begin_timing(true);
   uint32_t *counters = b.counters.as<uint32_t>();
+  ViewParams v = view;
+  v.tileW = b.tileW;
+  v.tileH = b.tileH;
   launch_fill_u32(s, counters, 3, 0u); // not the sticky overflow word
   launch_project(s, positions_.as<float>(), scales_.as<float>(), rotations_.as<float>(), opacities_.as<float>(),
                  shWordsBuf_.as<uint32_t>(), filterSet_ ? filter3d_.as<float>() : nullptr, v, options_.mip_filter_2d != 0, dropoutNow_,
@@ -393,14 +484,14 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &v, const uint8_t 
   launch_fill_u32(s, b.ranges.as<uint32_t>(), static_cast<uint64_t>(b.tiles) * 2, 0u);
   if (b.entryCapacity > 0) {
     launch_bin_emit(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.splats.as<Splat>(), b.tileRecords.as<TileRecord>(), count_, b.gridW,
-                    b.coverage.as<uint32_t>(), counters, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>());
+                    b.tileW, b.tileH, b.coverage.as<uint32_t>(), counters, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>());
     launch_radix_sort(s, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tmpEntryKeys.as<uint32_t>(), b.tmpEntryValues.as<uint32_t>(),
                       counters + kEntriesUsed, static_cast<uint32_t>(b.entryCapacity), b.tileBits, b.sortScratch.as<uint32_t>());
     launch_tile_ranges(s, b.entryKeys.as<uint32_t>(), counters, static_cast<uint32_t>(b.entryCapacity), b.ranges.as<uint2>());
   }
   mark("tile lists");
-  launch_forward(s, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height, options_.background,
-                 b.image.as<float4>(), b.stops.as<uint32_t>());
+  launch_forward(s, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH,
+                 options_.background, b.image.as<float4>(), b.stops.as<uint32_t>());
   mark("composite");
   if (mode == kModeRender) return;
   launch_loss(s, b.image.as<float4>(), deviceTarget, v.width, v.height, options_.lambda_ssim, b.fields.as<float>(), b.adjoint.as<float4>(),
@@ -414,8 +505,8 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &v, const uint8_t 
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
   // a resolution of 1/64 (trainer.cpp:586-591).
   const float fixedScale = fixed_scale(v);
-  launch_backward(s, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height,
-                  options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
+  launch_backward(s, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
+                  b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
                   b.rawFloat.as<float>());
   mark("backward");
   if (!chain) return;
@@ -579,6 +670,7 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
   dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
   stage_target(target, bytes);
   if (next != nullptr && nextBytes > 0) prefetch_target(next, nextBytes);
+  if (needs_tuning(camera.width, camera.height)) autotune_tiles(camera, target_.as<uint8_t>(), 2);
   const ViewParams v = view_params(camera);
   statisticsNext_ = true; // run() always adds to the densification statistics (trainer.cpp:600)
   vsplat_step_stats stats{};
@@ -776,6 +868,7 @@ uint64_t Trainer::capture_key(uint32_t n) const {
       array(*a);
     mix(b.entryCapacity);
     mix(static_cast<uint64_t>(b.width) << 32 | static_cast<uint32_t>(b.height));
+    mix(static_cast<uint64_t>(b.tileW) << 32 | b.tileH);
   }
   mix(count_);
   mix(static_cast<uint64_t>(activeShDegree_));
@@ -805,6 +898,9 @@ void Trainer::release_captured() {
 void Trainer::capture_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
                             const vsplat_learning_rates *rates) {
   release_captured();
+  for (uint32_t k = 0; k < n && cameras != nullptr && targets != nullptr; ++k) {
+    if (needs_tuning(cameras[k].width, cameras[k].height) && targets[k] != nullptr) autotune_tiles(cameras[k], targets[k], 2);
+  }
   prepare_views(n, cameras, targets, mode, rates);
   check(hipStreamSynchronize(stream_), "before capture");
   check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -870,6 +966,9 @@ uint32_t Trainer::step_views(uint32_t n, const vsplat_camera *cameras, const uin
     viewTargets_[k]->ensure(bytes[k]);
     check(hipMemcpyAsync(viewTargets_[k]->as<uint8_t>(), targets[k], bytes[k], hipMemcpyHostToDevice, stream_), "upload target");
     device[k] = viewTargets_[k]->as<uint8_t>();
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    if (needs_tuning(cameras[k].width, cameras[k].height)) autotune_tiles(cameras[k], device[k], 2);
   }
   std::vector<vsplat_step_stats> all(n);
   for (int attempt = 0;; ++attempt) {
@@ -1014,7 +1113,7 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
       {"stops", &b.stops, pixels * 4},
       {"adjoint", &b.adjoint, pixels * 16},
       {"fields", &b.fields, pixels * 36},
-      {"tile_totals", &b.tileTotals, static_cast<uint64_t>(b.tiles) * 8},
+      {"tile_totals", &b.tileTotals, static_cast<uint64_t>(b.lossTiles) * 8},
       {"raw", fixedPoint_ ? &b.rawFixed : &b.rawFloat, n * kRawPerSplat * (fixedPoint_ ? 8u : 4u)},
       {"counters", &b.counters, kCounterWords * 4},
   };

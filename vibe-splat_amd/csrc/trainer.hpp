@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <condition_variable>
+#include <map>
 #include <memory>
 #include <cstdint>
 #include <mutex>
@@ -74,7 +75,9 @@ struct ViewBuffers {
   bool ownsStream = false;
   hipEvent_t done = nullptr; // recorded on `stream` after the view's backward pass
   int width = 0, height = 0;
+  uint32_t tileW = 16, tileH = 16; // the raster tile shape
   uint32_t gridW = 0, gridH = 0, tiles = 0, tileBits = 0;
+  uint32_t lossTiles = 0;          // the loss's 16x16 tiles
   uint64_t entryCapacity = 0;
   // Per Gaussian.
   DeviceArray splats, keys, values, tmpKeys, tmpValues, coverage, tileCount, tileRecords, rawFixed, rawFloat;
@@ -109,6 +112,8 @@ public:
   uint32_t step_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, const size_t *bytes, uint32_t mode,
                       const vsplat_learning_rates *rates, vsplat_step_stats *stats);
   void reserve_entries(uint64_t entries);
+  // Times each tile shape on this view and keeps the fastest for its size.
+  std::pair<uint32_t, uint32_t> autotune_tiles(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t repeats);
   // A batch captured as a HIP graph, replayed with launch_captured and waited
   // for with synchronize_views.
   void capture_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
@@ -142,6 +147,9 @@ private:
   ViewParams view_params(const vsplat_camera &camera) const;
   ViewBuffers &view_buffers(size_t k); // the k-th view's buffers, made on first use
   void ensure_size(ViewBuffers &b, int width, int height);
+  void ensure_layout(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH);
+  std::pair<uint32_t, uint32_t> tile_shape_for(int width, int height) const;
+  bool needs_tuning(int width, int height) const;
   void grow_entries(ViewBuffers &b, uint64_t entries);
   void stage_target(const uint8_t *target, size_t bytes);
   // The passes up to the backward pass on b's stream, then (chain) the chain rule.
@@ -198,6 +206,7 @@ private:
   std::vector<std::pair<int, int>> graphViewSizes_;
   uint64_t graphKey_ = 0;
   uint64_t reservedEntries_ = 0; // what reserve_entries asked for, for views made later
+  std::map<std::pair<int, int>, std::pair<uint32_t, uint32_t>> tuned_; // view size -> tile shape the autotuner chose
   DeviceArray target_, viewStats_;
   std::vector<std::unique_ptr<DeviceArray>> viewTargets_; // step_views' targets, one per view
   PinnedArray staging_, next_, readback_;

@@ -25,6 +25,7 @@ OK, RUNTIME_ERROR, BUDGET_EXCEEDED, INVALID_ARGUMENT, LOGIC_ERROR, DEVICE_ERROR,
 BYTES_PER_TILE_ENTRY = 16
 MODE_STATISTICS, MODE_GRADIENTS, MODE_STEP = 0, 1, 2
 SH_HALF, SH_BF16 = 0, 1  # TrainerOptions.sh_storage
+TILES_16X16, TILES_8X8, TILES_32X8, TILES_AUTO = 0, 1, 2, 3  # TrainerOptions.tile_shape
 
 EXPORTS = [
     "vsplat_default_options", "vsplat_default_rates", "vsplat_trainer_create", "vsplat_trainer_destroy", "vsplat_render",
@@ -37,7 +38,7 @@ EXPORTS = [
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
-    "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_free",
+    "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_free", "vsplat_autotune_tiles",
 ]
 
 
@@ -72,7 +73,7 @@ class COptions(C.Structure):
                 ("force_fixed_point", C.c_int32), ("selective_adam", C.c_int32), ("opacity_reg", C.c_float),
                 ("scale_reg", C.c_float), ("position_noise", C.c_float), ("mip_filter_2d", C.c_int32),
                 ("pose_gradients", C.c_int32), ("memory_budget", C.c_uint64), ("sh_storage", C.c_int32),
-                ("recompute_transmittance", C.c_int32)]
+                ("recompute_transmittance", C.c_int32), ("tile_shape", C.c_int32)]
 
 
 class CRates(C.Structure):
@@ -149,6 +150,7 @@ def lib():
         L.vsplat_device_read.argtypes = [vp, vp, C.c_uint64]
         L.vsplat_device_free.argtypes = [vp]
         L.vsplat_device_free.restype = None
+        L.vsplat_autotune_tiles.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
         _LIB = L
     return _LIB
 
@@ -191,6 +193,7 @@ class TrainerOptions:
     memory_budget: int = 2 ** 64 - 1
     sh_storage: int = 0  # SH_HALF (the reference's layout) or SH_BF16 (this build only)
     recompute_transmittance: bool = False  # backward without the forward's stored log T and stops (this build only)
+    tile_shape: int = 0  # TILES_16X16 (the reference's), TILES_8X8, TILES_32X8 or TILES_AUTO (this build only)
 
     def to_c(self) -> COptions:
         o = COptions()
@@ -205,6 +208,7 @@ class TrainerOptions:
         o.memory_budget = int(self.memory_budget)
         o.sh_storage = int(self.sh_storage)
         o.recompute_transmittance = int(self.recompute_transmittance)
+        o.tile_shape = int(self.tile_shape)
         return o
 
 
@@ -449,6 +453,14 @@ class Trainer:
     def launch_captured(self, replays: int = 1):
         """Queues the captured batch `replays` times; wait with synchronize_views."""
         _raise(self._lib.vsplat_launch_captured(self._h, replays), "launch_captured")
+
+    def autotune_tiles(self, camera: Camera, device_target_ptr: int, repeats: int = 2) -> tuple:
+        """Times the tile shapes on this view, keeps the fastest for its size; returns (width, height)."""
+        cam = camera_to_c(camera)
+        w, h = C.c_uint32(), C.c_uint32()
+        _raise(self._lib.vsplat_autotune_tiles(self._h, C.byref(cam), C.c_void_p(device_target_ptr), repeats, C.byref(w), C.byref(h)),
+               "autotune_tiles")
+        return w.value, h.value
 
     def release_captured(self):
         _raise(self._lib.vsplat_release_captured(self._h), "release_captured")
