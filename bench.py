@@ -46,14 +46,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def touched_entries(ranges: np.ndarray, stops: np.ndarray, width: int, height: int) -> int:
+def touched_entries(ranges: np.ndarray, stops: np.ndarray, width: int, height: int, tiles=(16, 16)) -> int:
     """Ep = sum over tiles of (range end - the lowest pixel stop): the entries the
     forward and backward passes must visit."""
-    gw, gh = (width + 15) // 16, (height + 15) // 16
+    tw, th = tiles
+    gw, gh = (width + tw - 1) // tw, (height + th - 1) // th
     s = stops.reshape(height, width)
-    pad = np.full((gh * 16, gw * 16), np.iinfo(np.uint32).max, np.uint64)
+    pad = np.full((gh * th, gw * tw), np.iinfo(np.uint32).max, np.uint64)
     pad[:height, :width] = s
-    low = pad.reshape(gh, 16, gw, 16).min(axis=(1, 3)).reshape(-1)
+    low = pad.reshape(gh, th, gw, tw).min(axis=(1, 3)).reshape(-1)
     end = ranges[:, 1].astype(np.int64)
     low = np.minimum(low.astype(np.int64), end)
     return int((end - low).sum())
@@ -109,7 +110,10 @@ def cpu_baseline(workload, threads: int) -> dict:
                       f"extrapolated linearly in rows to 720"}
 
 
-def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
+TILE_SHAPES = {"auto": vsplat.TILES_AUTO, "16x16": vsplat.TILES_16X16, "8x8": vsplat.TILES_8X8, "32x8": vsplat.TILES_32X8}
+
+
+def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) -> dict:
     """The batched form on the eight-view ring (1M Gaussians, SH 3 stored as
     bf16, views of 640x480 to 1920x1080 on a ring of radius 3), fixed point, with Adam: each
     view timed as its own step (vsplat_enqueue), then the eight as one batched
@@ -118,11 +122,16 @@ def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
     w = F.config(3, count=count)
     rates = vsplat.LearningRates()
     trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True,
-                                                            sh_storage=vsplat.SH_BF16))
+                                                            sh_storage=vsplat.SH_BF16, tile_shape=TILE_SHAPES[tile_shape]))
     trainer.set_stage_timing(False)
     buffers = [vsplat.DeviceBuffer(t) for t in w.targets]
     ptrs = [b.ptr for b in buffers]
+    shapes = {}
     try:
+        if tile_shape == "auto":
+            for cam, ptr in zip(w.cameras, ptrs):
+                if (cam.width, cam.height) not in shapes:
+                    shapes[(cam.width, cam.height)] = trainer.autotune_tiles(cam, ptr, 2)
         trainer.step_views(w.cameras, w.targets, rates)  # sizes every buffer and the entry reservation
         single = []
         for cam, target, ptr in zip(w.cameras, w.targets, ptrs):
@@ -162,20 +171,25 @@ def batched_ring(steps: int, warmup: int, count: int | None) -> dict:
             "batched_ms": round(batch, 4), "batched_ms_per_view": round(batch / views, 4),
             "speedup_per_view": round(sum(single) / batch, 4),
             "graph_ms": round(graph, 4), "graph_ms_per_view": round(graph / views, 4),
-            "graph_speedup_per_view": round(sum(single) / graph, 4), "sh_storage": "bf16 (all 48 coefficients), fp32 arithmetic"}
+            "graph_speedup_per_view": round(sum(single) / graph, 4), "sh_storage": "bf16 (all 48 coefficients), fp32 arithmetic",
+            "tile_shape": tile_shape, "tiles_chosen": {f"{k[0]}x{k[1]}": f"{v[0]}x{v[1]}" for k, v in shapes.items()}}
 
 
-def other_form(index: int, steps: int, warmup: int, **options) -> dict:
+def other_form(index: int, steps: int, warmup: int, tile_shape: str, **options) -> dict:
     """ms/iter of one more single-view workload, gradients mode, fixed point,
     target in device memory: the precision form (3M Gaussians in 64 blobs, SH
     stored as bf16, 1920x1080) or the differentiation stress (6M heavy-tailed
     Gaussians, 1920x1080, the backward recomputing the transmittance)."""
     w = F.config(index)
     cam, target = w.cameras[0], w.targets[0]
-    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True, **options))
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True,
+                                                            tile_shape=TILE_SHAPES[tile_shape], **options))
     trainer.set_stage_timing(False)
     buffer = vsplat.DeviceBuffer(target)
+    chosen = None
     try:
+        if tile_shape == "auto":
+            chosen = trainer.autotune_tiles(cam, buffer.ptr, 2)
         first = trainer.compute_gradients(cam, target)
         for _ in range(warmup):
             trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
@@ -198,7 +212,8 @@ def other_form(index: int, steps: int, warmup: int, **options) -> dict:
     return {"workload": w.description, "gaussians": w.cloud.count, "width": cam.width, "height": cam.height,
             "options": {k: (int(v) if isinstance(v, bool) else v) for k, v in options.items()}, "ms_per_iter": round(ms, 4),
             "visible": st.visible_splats, "entries": st.tile_entries, "loss": round(float(first.loss), 6),
-            "gpu_gb": round(gpu_gb, 3), "stages_ms": stages}
+            "gpu_gb": round(gpu_gb, 3), "tile_shape": tile_shape,
+            "tiles_chosen": None if chosen is None else f"{chosen[0]}x{chosen[1]}", "stages_ms": stages}
 
 
 def main():
@@ -213,6 +228,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-batched", action="store_true", help="skip the eight-view batched measurement")
     ap.add_argument("--no-other-forms", action="store_true", help="skip the 3M bf16 and 6M recompute workloads")
+    ap.add_argument("--tile-shape", choices=sorted(TILE_SHAPES), default="auto",
+                    help="raster tiles (auto: timed per view size; every shape gives the same results)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
@@ -223,11 +240,17 @@ def main():
     w = F.config(1, count=args.count)
     cam, target = w.cameras[0], w.targets[0]
     N, P = w.cloud.count, cam.width * cam.height
-    T = ((cam.width + 15) // 16) * ((cam.height + 15) // 16)
     fixed = not args.float_atomics
-    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed))
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed,
+                                                            tile_shape=TILE_SHAPES[args.tile_shape]))
     device_target = vsplat.DeviceBuffer(target)
     d_target = device_target.ptr
+    if args.tile_shape == "auto":
+        log("tile shape: timing 16x16, 8x8 and 32x8")
+        tiles = trainer.autotune_tiles(cam, d_target, 3)
+        log(f"  chose {tiles[0]}x{tiles[1]}")
+    else:
+        tiles = tuple(int(v) for v in args.tile_shape.split("x"))
     rates = vsplat.LearningRates()
     mode = vsplat.MODE_GRADIENTS if args.mode == "gradients" else vsplat.MODE_STEP
 
@@ -257,7 +280,8 @@ def main():
     log(f"  {ms_staged:.4f} ms per step: " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
 
     V, E = stats.visible_splats, stats.tile_entries
-    Ep = touched_entries(trainer.inspect("ranges"), trainer.inspect("stops"), cam.width, cam.height)
+    T = ((cam.width + tiles[0] - 1) // tiles[0]) * ((cam.height + tiles[1] - 1) // tiles[1])
+    Ep = touched_entries(trainer.inspect("ranges")[:T], trainer.inspect("stops"), cam.width, cam.height, tiles)
     coefficients = (w.cloud.sh_degree + 1) ** 2
     Bp = (3 + 3 + 4 + 1) * 4 + F.half_sh_words(coefficients) * 4  # as stored: SH DC fp32, rest fp16
     total, per_stage = algorithmic_bytes(N, V, E, Ep, P, T, Bp)
@@ -268,7 +292,8 @@ def main():
     # on this workload (profiles/), when this run is that workload.
     traffic = None
     traffic_file = ROOT / "profiles" / "traffic_1M_1280x720_fixed_gradients.json"
-    kernel_name = {"forward": "forward_kernel", "backward": "backward_kernel<true, false>"}[dominant[1]]
+    kernel_name = {"forward": f"forward_kernel<{tiles[0]}u, {tiles[1]}u>",
+                   "backward": f"backward_kernel<true, false, {tiles[0]}u, {tiles[1]}u>"}[dominant[1]]
     if traffic_file.exists() and fixed and N == 1_000_000 and args.mode == "gradients":
         entry = json.loads(traffic_file.read_text())["kernels"].get(kernel_name)
         traffic = None if entry is None else entry["hbm_bytes_per_launch"]
@@ -289,7 +314,7 @@ def main():
         "config": {"workload": "1M Gaussians SH3, one 1280x720 view, 0.8 L1 + 0.2 D-SSIM, "
                                + ("fixed-point" if fixed else "float-atomic") + " backward, " + args.mode,
                    "gaussians": N, "width": cam.width, "height": cam.height, "sh_degree": w.cloud.sh_degree,
-                   "form": "plain fp32, single view"},
+                   "form": "plain fp32, single view", "tile_shape": args.tile_shape, "tiles_chosen": f"{tiles[0]}x{tiles[1]}"},
         "roofline": {"bound": "hbm", "kernel": dominant[1], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": int(per_stage[dominant[1]]), "kernel_ms": round(kernel_ms, 4)},
@@ -300,7 +325,7 @@ def main():
     }
     if not args.no_batched:
         log("batched form: the eight-view ring")
-        result["batched"] = batched_ring(max(5, args.steps // 2), 2, args.count)
+        result["batched"] = batched_ring(max(5, args.steps // 2), 2, args.count, args.tile_shape)
         bt = result["batched"]
         # The headline ratio: the plain fp32 single 1280x720 view against the
         # batched bf16 form's graph, per view (the ring's views average 1.25x its pixels).
@@ -313,7 +338,7 @@ def main():
         for key, index, options in (("precision_bf16_3M_1920x1080", 2, {"sh_storage": vsplat.SH_BF16}),
                                     ("stress_recompute_6M_1920x1080", 4, {"recompute_transmittance": True})):
             log(f"{key}: generating and timing")
-            form = other_form(index, max(5, args.steps // 2), 2, **options)
+            form = other_form(index, max(5, args.steps // 2), 2, args.tile_shape, **options)
             result["other_forms"][key] = form
             log(f"  {form['ms_per_iter']} ms per iteration, {form['entries']} entries, {form['gpu_gb']} GB")
     if not args.no_cpu_baseline:
