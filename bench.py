@@ -90,24 +90,60 @@ def cpu_baseline(workload, threads: int) -> dict:
     cam, target = workload.cameras[0], workload.targets[0]
     H = cam.height
 
-    def run(rows):
+    def band(rows):
         top = (H - rows) // 2
-        c = dataclasses.replace(cam, height=rows, cy=float(np.float32(cam.cy) - np.float32(top)))
+        return top, dataclasses.replace(cam, height=rows, cy=float(np.float32(cam.cy) - np.float32(top)))
+
+    def oracle_a(rows, n):
+        top, c = band(rows)
         t0 = time.perf_counter()
-        img = O.render(workload.cloud, c, threads=threads)
-        ls = O.loss(img, np.ascontiguousarray(target[top:top + rows]), workload.lambda_ssim, threads=threads)
-        O.gradients(workload.cloud, c, ls.adjoint, threads=threads)
+        img = O.render(workload.cloud, c, threads=n)
+        ls = O.loss(img, np.ascontiguousarray(target[top:top + rows]), workload.lambda_ssim, threads=n)
+        O.gradients(workload.cloud, c, ls.adjoint, threads=n)
         return time.perf_counter() - t0
 
-    band = 90
-    t_fixed = run(2)
-    t_band = run(band)
-    per_row = (t_band - t_fixed) / (band - 2)
-    full = t_fixed + per_row * (H - 2)
+    def oracle_b(rows, n):
+        top, c = band(rows)
+        t0 = time.perf_counter()
+        m = O.Model(workload.cloud, c)
+        r = m.forward()
+        m.loss(r, np.ascontiguousarray(target[top:top + rows]), workload.lambda_ssim)
+        m.backward_fixed(r)
+        m.gradients(r)
+        return time.perf_counter() - t0
+
+    def extrapolate(run, n, rows):
+        t_fixed, t_band = run(2, n), run(rows, n)
+        per_row = (t_band - t_fixed) / (rows - 2)
+        return t_fixed + per_row * (H - 2), t_band, t_fixed
+
+    full, t_band, t_fixed = extrapolate(oracle_a, threads, 90)
+    one, one_band, one_fixed = extrapolate(oracle_a, 1, 12)
+    b_full, b_band, b_fixed = extrapolate(oracle_b, 1, 90)
     return {"value": round(full * 1000.0, 1), "unit": "ms/iter", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(),
             "sample": f"FP64 restatement of reference.cpp, fwd+loss+bwd+chain rule on {threads} threads: "
-                      f"a {band}-row band ({t_band:.2f} s) and a 2-row band ({t_fixed:.2f} s) of the 1280x720 view, "
-                      f"extrapolated linearly in rows to 720"}
+                      f"a 90-row band ({t_band:.2f} s) and a 2-row band ({t_fixed:.2f} s) of the 1280x720 view, "
+                      f"extrapolated linearly in rows to 720",
+            "single_core": {"value": round(one * 1000.0, 1), "unit": "ms/iter", "cores": 1,
+                            "sample": f"the same on 1 thread: a 12-row band ({one_band:.2f} s) and a 2-row band "
+                                      f"({one_fixed:.2f} s), extrapolated to 720 rows"},
+            "fp32_tile_model": {"value": round(b_full * 1000.0, 1), "unit": "ms/iter", "cores": 1,
+                                "sample": f"the FP32 model of this build's GPU algorithm (same lists, sums and fixed "
+                                          f"point; single-threaded): a 90-row band ({b_band:.2f} s) and a 2-row band "
+                                          f"({b_fixed:.2f} s), extrapolated to 720 rows"}}
+
+
+def copy_rate(nbytes: int = 1 << 30, repeats: int = 20) -> dict:
+    """The device's measured copy rate: one GiB copied between two GPU buffers,
+    back to back, timed with HIP events (read + write bytes per second)."""
+    src, dst = vsplat.DeviceBuffer.empty(nbytes), vsplat.DeviceBuffer.empty(nbytes)
+    try:
+        ms = dst.copy_from(src, repeats)
+    finally:
+        src.free()
+        dst.free()
+    return {"bytes": nbytes, "ms_per_copy": round(ms, 4), "gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
 TILE_SHAPES = {"auto": vsplat.TILES_AUTO, "16x16": vsplat.TILES_16X16, "8x8": vsplat.TILES_8X8, "32x8": vsplat.TILES_32X8}
@@ -219,8 +255,8 @@ def other_form(index: int, steps: int, warmup: int, tile_shape: str, **options) 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--count", type=int, default=None, help="Gaussians (default: the workload's 1,000,000)")
     ap.add_argument("--mode", choices=["gradients", "step"], default="gradients")
     ap.add_argument("--float-atomics", action="store_true", help="float-atomic backward instead of fixed point")
@@ -288,6 +324,9 @@ def main():
     dominant = max(("composite", "forward"), ("backward", "backward"), key=lambda s: stages.get(s[0], 0.0))
     kernel_ms = stages[dominant[0]]
     achieved = per_stage[dominant[1]] / (kernel_ms * 1e-3) / 1e9
+    log("device copy rate")
+    copy = copy_rate()
+    log(f"  {copy['gbs']} GB/s")
     # HBM bytes per launch of that kernel, from the FETCH_SIZE / WRITE_SIZE counter runs
     # on this workload (profiles/), when this run is that workload.
     traffic = None
@@ -317,8 +356,10 @@ def main():
                    "form": "plain fp32, single view", "tile_shape": args.tile_shape, "tiles_chosen": f"{tiles[0]}x{tiles[1]}"},
         "roofline": {"bound": "hbm", "kernel": dominant[1], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "algorithmic_bytes": int(per_stage[dominant[1]]), "kernel_ms": round(kernel_ms, 4)},
+                     "algorithmic_bytes": int(per_stage[dominant[1]]), "kernel_ms": round(kernel_ms, 4),
+                     "measured_copy_gbs": copy["gbs"], "frac_of_measured_copy": round(achieved / copy["gbs"], 5)},
         "iteration": {"algorithmic_bytes": int(total), "hbm_frac": round(total / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                      "frac_of_measured_copy": round(total / (ms * 1e-3) / 1e9 / copy["gbs"], 5),
                       "V": V, "E": E, "Ep": Ep, "P": P, "T": T, "Bp": Bp, "loss": round(float(stats.loss), 6)},
         "ms_per_step_with_stage_events": round(ms_staged, 4),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},

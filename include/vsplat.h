@@ -267,6 +267,10 @@ int vsplat_inspect(vsplat_trainer *trainer, const char *buffer, void *out, uint6
 int vsplat_device_alloc(uint64_t bytes, const void *host, void **out_device);
 /* Copies bytes from GPU memory to host memory. */
 int vsplat_device_read(const void *device, void *host, uint64_t bytes);
+/* Copies bytes from one GPU buffer to another `repeats` times, back to back on
+ * a stream of its own, and sets *ms_per_copy (may be NULL) to the mean time of
+ * one copy by HIP events: the device's copy rate, for measurements. */
+int vsplat_device_copy(void *dst, const void *src, uint64_t bytes, uint32_t repeats, float *ms_per_copy);
 /* Frees memory from vsplat_device_alloc. NULL is ignored. */
 void vsplat_device_free(void *device);
 

@@ -281,6 +281,40 @@ extern "C" int vsplat_device_read(const void *device, void *host, uint64_t bytes
   });
 }
 
+extern "C" int vsplat_device_copy(void *dst, const void *src, uint64_t bytes, uint32_t repeats, float *ms_per_copy) {
+  if ((dst == nullptr || src == nullptr) && bytes > 0) return invalid("null argument");
+  if (repeats == 0) return invalid("no copies asked for");
+  return guarded([&] {
+    const auto check = [](hipError_t e, const char *what) {
+      if (e != hipSuccess) throw vsplat::Error(VSPLAT_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipStream_t stream = nullptr;
+    hipEvent_t begin = nullptr, end = nullptr;
+    auto release = [&] {
+      if (begin != nullptr) (void)hipEventDestroy(begin);
+      if (end != nullptr) (void)hipEventDestroy(end);
+      if (stream != nullptr) (void)hipStreamDestroy(stream);
+    };
+    try {
+      check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+      check(hipEventCreate(&begin), "hipEventCreate");
+      check(hipEventCreate(&end), "hipEventCreate");
+      check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream), "hipMemcpyAsync"); // warm-up
+      check(hipEventRecord(begin, stream), "hipEventRecord");
+      for (uint32_t r = 0; r < repeats; ++r) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream), "hipMemcpyAsync");
+      check(hipEventRecord(end, stream), "hipEventRecord");
+      check(hipEventSynchronize(end), "hipEventSynchronize");
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, begin, end), "hipEventElapsedTime");
+      if (ms_per_copy != nullptr) *ms_per_copy = ms / static_cast<float>(repeats);
+    } catch (...) {
+      release();
+      throw;
+    }
+    release();
+  });
+}
+
 extern "C" void vsplat_device_free(void *device) {
   if (device != nullptr) (void)hipFree(device);
 }

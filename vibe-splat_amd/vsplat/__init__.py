@@ -38,7 +38,7 @@ EXPORTS = [
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
-    "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_free", "vsplat_autotune_tiles",
+    "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free", "vsplat_autotune_tiles",
 ]
 
 
@@ -148,6 +148,7 @@ def lib():
         L.vsplat_release_captured.argtypes = [vp]
         L.vsplat_device_alloc.argtypes = [C.c_uint64, vp, vp]
         L.vsplat_device_read.argtypes = [vp, vp, C.c_uint64]
+        L.vsplat_device_copy.argtypes = [vp, vp, C.c_uint64, C.c_uint32, vp]
         L.vsplat_device_free.argtypes = [vp]
         L.vsplat_device_free.restype = None
         L.vsplat_autotune_tiles.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
@@ -506,10 +507,28 @@ class DeviceBuffer:
         _raise(lib().vsplat_device_alloc(a.nbytes, C.c_void_p(a.ctypes.data), C.byref(ptr)), "device_alloc")
         self.ptr, self.nbytes = ptr.value, a.nbytes
 
+    @classmethod
+    def empty(cls, nbytes: int) -> "DeviceBuffer":
+        """Uninitialised GPU memory of nbytes."""
+        self = cls.__new__(cls)
+        ptr = C.c_void_p()
+        _raise(lib().vsplat_device_alloc(nbytes, None, C.byref(ptr)), "device_alloc")
+        self.ptr, self.nbytes = ptr.value, nbytes
+        return self
+
     def read(self, dtype=np.uint8) -> np.ndarray:
         out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
         _raise(lib().vsplat_device_read(C.c_void_p(self.ptr), _p(out), out.nbytes), "device_read")
         return out
+
+    def copy_from(self, other: "DeviceBuffer", repeats: int = 1) -> float:
+        """Copies `other` into this buffer `repeats` times; returns ms per copy."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("copy between buffers of different sizes")
+        ms = C.c_float()
+        _raise(lib().vsplat_device_copy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), self.nbytes, repeats, C.byref(ms)),
+               "device_copy")
+        return ms.value
 
     def free(self):
         if getattr(self, "ptr", None):
