@@ -134,6 +134,29 @@ def cpu_baseline(workload, threads: int) -> dict:
                                           f"({b_fixed:.2f} s), extrapolated to 720 rows"}}
 
 
+def apply_changes_leg(trainer, n: int) -> dict:
+    """Trainer::apply_changes after a densification round on the benchmark
+    trainer (run last: it changes the cloud): every tenth Gaussian pruned and
+    every tenth copied, first with the values gathered on the device (no
+    transfer), then the same table with the values handed over from the host
+    as densify() does (the cloud downloaded, the new one uploaded). Wall time of
+    the synchronous calls, allocation of the new arrays included."""
+    index = np.arange(n)
+    source = np.concatenate([index[index % 10 != 3], index[::10]]).astype(np.int64)
+    t0 = time.perf_counter()
+    trainer.apply_changes(None, source)
+    device_ms = (time.perf_counter() - t0) * 1000.0
+    m = len(source)
+    t0 = time.perf_counter()
+    cloud = trainer.cloud()
+    trainer.apply_changes(cloud, np.arange(m))
+    host_ms = (time.perf_counter() - t0) * 1000.0
+    # Per new Gaussian: parameters as stored (148 bytes at SH 3) and Adam's state
+    # (4 bytes per parameter, 236), each read once and written once.
+    return {"gaussians_before": n, "gaussians_after": m, "device_ms": round(device_ms, 3),
+            "host_cloud_ms": round(host_ms, 3), "algorithmic_bytes_device": 2 * m * (148 + 236)}
+
+
 def copy_rate(nbytes: int = 1 << 30, repeats: int = 20) -> dict:
     """The device's measured copy rate: one GiB copied between two GPU buffers,
     back to back, timed with HIP events (read + write bytes per second)."""
@@ -264,6 +287,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-batched", action="store_true", help="skip the eight-view batched measurement")
     ap.add_argument("--no-other-forms", action="store_true", help="skip the 3M bf16 and 6M recompute workloads")
+    ap.add_argument("--no-densify", action="store_true", help="skip the apply_changes measurement")
     ap.add_argument("--tile-shape", choices=sorted(TILE_SHAPES), default="auto",
                     help="raster tiles (auto: timed per view size; every shape gives the same results)")
     args = ap.parse_args()
@@ -382,6 +406,11 @@ def main():
             form = other_form(index, max(5, args.steps // 2), 2, args.tile_shape, **options)
             result["other_forms"][key] = form
             log(f"  {form['ms_per_iter']} ms per iteration, {form['entries']} entries, {form['gpu_gb']} GB")
+    if not args.no_densify:
+        log("apply_changes: 10 % pruned, 10 % copied")
+        result["apply_changes"] = apply_changes_leg(trainer, N)
+        log(f"  {result['apply_changes']['device_ms']} ms on the device, "
+            f"{result['apply_changes']['host_cloud_ms']} ms with the values from the host")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline on {threads} threads")

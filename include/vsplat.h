@@ -150,6 +150,19 @@ int vsplat_get_cloud(vsplat_trainer *trainer, float *positions, float *scales, f
 int vsplat_get_densify_stats(vsplat_trainer *trainer, float *out);
 /* Trainer::reset_densify_stats (trainer.hpp:178). */
 int vsplat_reset_densify_stats(vsplat_trainer *trainer);
+/* Trainer::apply_changes (trainer.hpp:204, trainer.cpp:749-787), the device
+ * side of densify, relocate and the remove_* calls (trainer.hpp:156-183): the
+ * Gaussians become `count` new ones, DensifyResult's cloud and source
+ * (densify.hpp:52-63). New Gaussian g keeps the optimiser state of old Gaussian
+ * source[g], gathered on the GPU, or starts with zero moments when source[g]
+ * is negative. Its values come from `cloud` (count Gaussians of the trainer's
+ * SH degree), or, when cloud is NULL, from old Gaussian source[g] on the GPU,
+ * which needs every source >= 0 (pruning and copies, with no upload). The
+ * densification statistics, the 3D filter, stored gradients and a captured
+ * batch are dropped; Adam's step count is kept. VSPLAT_INVALID_ARGUMENT for a
+ * source past the old count or a mismatched cloud, VSPLAT_RUNTIME_ERROR for
+ * count 0 (the reference: "densification removed every Gaussian"). */
+int vsplat_apply_changes(vsplat_trainer *trainer, const vsplat_cloud *cloud, const int64_t *source, uint32_t count);
 
 /* Trainer::set_active_sh_degree (trainer.hpp:151), clamped to [0, degree]. */
 int vsplat_set_active_sh_degree(vsplat_trainer *trainer, int32_t degree);
@@ -257,7 +270,10 @@ int vsplat_autotune_tiles(vsplat_trainer *trainer, const vsplat_camera *camera, 
  * "entry_values" (4 bytes per tile entry), "ranges" (8 bytes per tile),
  * "image" (16 bytes per pixel: rgb, log T), "stops", "adjoint" (16 per pixel),
  * "fields" (36 per pixel), "tile_totals" (8 per tile), "raw" (14 int64 per
- * Gaussian with fixed point, else 14 float), "counters" (4 words). *bytes
+ * Gaussian with fixed point, else 14 float), "counters" (4 words),
+ * "state_positions", "state_scales", "state_rotations", "state_opacities",
+ * "state_sh" (Adam's state, one word per parameter: bf16 m low, bf16 v high),
+ * "stats" (8 floats per Gaussian). *bytes
  * gets the size in use; at most `capacity` bytes are copied. */
 int vsplat_inspect(vsplat_trainer *trainer, const char *buffer, void *out, uint64_t capacity, uint64_t *bytes);
 

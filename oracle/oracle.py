@@ -6,6 +6,10 @@ this module; the product never does. Build with ``make -C oracle``.
 Oracle A (``render``, ``loss``, ``gradients``, ``contributions``) is the
 double-precision restatement of the reference's src/train/reference.cpp.
 Oracle B (``Model``) is the single-precision model of the GPU pipeline.
+``gather`` restates the state gather of a changed cloud (train_gather.comp);
+it is pinned by tests/test_densify.py: a brute-force loop on small tables, the
+identity table, the composition of two tables, and fresh (zero) state for
+negative sources.
 """
 from __future__ import annotations
 
@@ -98,6 +102,21 @@ def lib():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def gather(old: np.ndarray, source, stride: int) -> np.ndarray:
+    """A per-Gaussian array of a changed cloud (train_gather.comp:16-24, used by
+    trainer.cpp:749-787): element k of new Gaussian g is element k of old
+    Gaussian source[g], or 0 where source[g] is negative (fresh optimiser
+    state). `old` holds `stride` elements per Gaussian; returns a flat array."""
+    table = np.asarray(old).reshape(-1, stride)
+    src = np.asarray(source, np.int64).reshape(-1)
+    if src.size and src.max() >= table.shape[0]:
+        raise ValueError("a source index is past the old cloud")
+    out = np.zeros((src.size, stride), table.dtype)
+    keep = src >= 0
+    out[keep] = table[src[keep]]
+    return out.reshape(-1)
 
 
 def adam_corrections(step: int):
@@ -435,17 +454,23 @@ class Model:
         return pos, sc, rot, op, words
 
 
-def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, selective=True, sh_storage=0):
-    """The batched step's Adam (ob_adam) from zero optimiser state: the cloud's
-    parameters (SH as packed words) after one step with the summed gradients
-    grads = (positions, scales, rotations, opacities, sh) as flat float32 arrays."""
+def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, selective=True, sh_storage=0, states=None):
+    """The batched step's Adam (ob_adam): the cloud's parameters (SH as packed
+    words) after Adam's step number `step` with the summed gradients grads =
+    (positions, scales, rotations, opacities, sh) as flat float32 arrays, from
+    zero optimiser state or from `states` (five uint32 arrays, one word per
+    parameter, updated in place)."""
     n = cloud.count
     pos = np.ascontiguousarray(cloud.positions, np.float32).reshape(-1).copy()
     sc = np.ascontiguousarray(cloud.scales, np.float32).reshape(-1).copy()
     rot = np.ascontiguousarray(cloud.rotations, np.float32).reshape(-1).copy()
     op = np.ascontiguousarray(cloud.opacities, np.float32).copy()
     w = np.ascontiguousarray(words, np.uint32).copy()
-    states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * cloud.coefficients)]
+    if states is None:
+        states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * cloud.coefficients)]
+    for s, k in zip(states, (n * 3, n * 3, n * 4, n, n * 3 * cloud.coefficients)):
+        if s.dtype != np.uint32 or s.size != k or not s.flags.c_contiguous:
+            raise ValueError("states: five contiguous uint32 arrays, one word per parameter")
     g = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in grads]
     opt = OBChainOptions()
     opt.mode, opt.flags = 2, 4 if selective else 0

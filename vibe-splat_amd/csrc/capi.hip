@@ -127,6 +127,11 @@ int vsplat_reset_densify_stats(vsplat_trainer *t) {
   return guarded([&] { t->impl.reset_densify_stats(); });
 }
 
+int vsplat_apply_changes(vsplat_trainer *t, const vsplat_cloud *cloud, const int64_t *source, uint32_t count) {
+  if (t == nullptr || (source == nullptr && count > 0)) return invalid("null argument");
+  return guarded([&] { t->impl.apply_changes(cloud, source, count); });
+}
+
 int vsplat_set_active_sh_degree(vsplat_trainer *t, int32_t degree) {
   if (t == nullptr) return invalid("null trainer");
   return guarded([&] { t->impl.set_active_sh_degree(degree); });

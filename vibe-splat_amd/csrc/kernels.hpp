@@ -83,6 +83,17 @@ void launch_radix_sort(hipStream_t stream, uint32_t *keys, uint32_t *values, uin
 void launch_copy_u32(hipStream_t stream, const uint32_t *src, uint32_t *dst, const uint32_t *countDev, uint32_t capacity);
 void launch_fill_u32(hipStream_t stream, uint32_t *dst, uint64_t count, uint32_t value);
 
+// One per-Gaussian array for launch_gather: `stride` 32-bit words per Gaussian.
+struct GatherArray {
+  const uint32_t *from;
+  uint32_t *to;
+  uint32_t stride;
+};
+constexpr uint32_t kMaxGatherArrays = 10;
+// to[g * stride + k] = source[g] < 0 ? 0 : from[source[g] * stride + k] for the
+// new Gaussians g < gaussians, for each of `count` (at most kMaxGatherArrays) arrays.
+void launch_gather(hipStream_t stream, const int32_t *source, uint32_t gaussians, const GatherArray *arrays, uint32_t count);
+
 // Scratch (zeroed once when allocated) for launch_coverage_scan over count splats.
 size_t coverage_scan_words(uint32_t count);
 // Offsets of each depth-sorted splat's entries, the entry count and the capacity check.

@@ -129,17 +129,35 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
     for (auto &e : slot.events) check(hipEventCreate(&e), "hipEventCreate");
 
   const size_t n = count_;
-  positions_.ensure(n * 12);
-  scales_.ensure(n * 12);
-  rotations_.ensure(n * 16);
-  opacities_.ensure(n * 4);
-  shWordsBuf_.ensure(n * shWords_ * 4);
-  check(hipMemcpy(positions_.as<float>(), c.positions, n * 12, hipMemcpyHostToDevice), "upload positions");
-  check(hipMemcpy(scales_.as<float>(), c.scales, n * 12, hipMemcpyHostToDevice), "upload scales");
-  check(hipMemcpy(rotations_.as<float>(), c.rotations, n * 16, hipMemcpyHostToDevice), "upload rotations");
-  check(hipMemcpy(opacities_.as<float>(), c.opacities, n * 4, hipMemcpyHostToDevice), "upload opacities");
-  // pack_half_sh (src/splat/gaussian_cloud.cpp:537-550): DC as floats, the rest
-  // as halves; or every coefficient as bf16.
+  DeviceArray *values[5] = {&positions_, &scales_, &rotations_, &opacities_, &shWordsBuf_};
+  upload_cloud(c, values);
+  const size_t stateFloats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
+  for (int g = 0; g < 5; ++g) {
+    state_[g].ensure(stateFloats[g] * 4);
+    launch_fill_u32(stream_, state_[g].as<uint32_t>(), stateFloats[g], 0u);
+  }
+  allocate_per_gaussian();
+  readback_.ensure(kCounterWords * 4 + 2 * sizeof(double));
+  adamState_.ensure(4 * 4);
+  const uint32_t adamStart[4] = {0u, 0x3F800000u, 0x3F800000u, 0u}; // no steps; beta^0 = 1
+  check(hipMemcpy(adamState_.as<uint32_t>(), adamStart, sizeof(adamStart), hipMemcpyHostToDevice), "adam state");
+  check(hipEventCreateWithFlags(&batchStart_, hipEventDisableTiming), "hipEventCreate");
+  view_buffers(0);
+  check(hipStreamSynchronize(stream_), "trainer set-up");
+}
+
+// The cloud's values in the trainer's layout: positions, scales, rotations and
+// opacities as floats; the SH packed as pack_half_sh does
+// (src/splat/gaussian_cloud.cpp:537-550: DC as floats, the rest as halves), or
+// every coefficient as bf16.
+void Trainer::upload_cloud(const vsplat_cloud &c, DeviceArray *const values[5]) const {
+  const size_t n = c.count;
+  const float *from[4] = {c.positions, c.scales, c.rotations, c.opacities};
+  const size_t floats[4] = {3, 3, 4, 1};
+  for (int g = 0; g < 4; ++g) {
+    values[g]->ensure(n * floats[g] * 4);
+    check(hipMemcpy(values[g]->as<float>(), from[g], n * floats[g] * 4, hipMemcpyHostToDevice), "upload parameters");
+  }
   std::vector<uint32_t> words(n * shWords_, 0u);
   for (size_t i = 0; i < n; ++i) {
     const float *in = c.sh + i * shFloats_;
@@ -151,27 +169,36 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
       for (uint32_t j = 3; j < shFloats_; ++j) to[3 + (j - 3) / 2] |= static_cast<uint32_t>(float_to_half(in[j])) << (16 * ((j - 3) & 1));
     }
   }
-  check(hipMemcpy(shWordsBuf_.as<uint32_t>(), words.data(), words.size() * 4, hipMemcpyHostToDevice), "upload SH");
+  values[4]->ensure(n * shWords_ * 4);
+  check(hipMemcpy(values[4]->as<uint32_t>(), words.data(), words.size() * 4, hipMemcpyHostToDevice), "upload SH");
+}
 
-  const size_t stateFloats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
-  for (int g = 0; g < 5; ++g) {
-    state_[g].ensure(stateFloats[g] * 4);
-    launch_fill_u32(stream_, state_[g].as<uint32_t>(), stateFloats[g], 0u);
-  }
+// The per-Gaussian buffers the views share (trainer.cpp:242-262), cleared:
+// densification statistics, 3D filter values, the SH pass's scratch.
+void Trainer::allocate_per_gaussian() {
+  const size_t n = count_;
   shPass_.ensure(n * kShPassFloats * 4);
-  dirTerms_.ensure(n * 16);
   launch_fill_u32(stream_, shPass_.as<uint32_t>(), n * kShPassFloats, 0u);
+  dirTerms_.ensure(n * 16);
   stats_.ensure(n * 32);
   launch_fill_u32(stream_, stats_.as<uint32_t>(), n * 8, 0u);
   filter3d_.ensure(n * 4);
   launch_fill_u32(stream_, filter3d_.as<uint32_t>(), n, 0u);
-  readback_.ensure(kCounterWords * 4 + 2 * sizeof(double));
-  adamState_.ensure(4 * 4);
-  const uint32_t adamStart[4] = {0u, 0x3F800000u, 0x3F800000u, 0u}; // no steps; beta^0 = 1
-  check(hipMemcpy(adamState_.as<uint32_t>(), adamStart, sizeof(adamStart), hipMemcpyHostToDevice), "adam state");
-  check(hipEventCreateWithFlags(&batchStart_, hipEventDisableTiming), "hipEventCreate");
-  view_buffers(0);
-  check(hipStreamSynchronize(stream_), "trainer set-up");
+  filterSet_ = false;
+}
+
+// Waits for the views' streams and frees their buffers and streams (the first
+// view's stream is the trainer's and stays).
+void Trainer::release_views() {
+  if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
+  for (auto &v : views_) {
+    if (v->ownsStream) {
+      (void)hipStreamSynchronize(v->stream);
+      (void)hipStreamDestroy(v->stream);
+    }
+    if (v->done != nullptr) (void)hipEventDestroy(v->done);
+  }
+  views_.clear();
 }
 
 Trainer::~Trainer() {
@@ -182,15 +209,7 @@ Trainer::~Trainer() {
   prefetchChanged_.notify_all();
   if (prefetchWorker_.joinable()) prefetchWorker_.join();
   release_captured();
-  if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
-  for (auto &v : views_) {
-    if (v->ownsStream) {
-      (void)hipStreamSynchronize(v->stream);
-      (void)hipStreamDestroy(v->stream);
-    }
-    if (v->done != nullptr) (void)hipEventDestroy(v->done);
-  }
-  views_.clear();
+  release_views();
   if (batchStart_ != nullptr) (void)hipEventDestroy(batchStart_);
   for (auto &slot : slots_)
     for (auto &e : slot.events)
@@ -1025,6 +1044,73 @@ void Trainer::densify_stats(float *out) const {
   check(hipMemcpy(out, stats_.as<float>(), static_cast<size_t>(count_) * 32, hipMemcpyDeviceToHost), "download stats");
 }
 
+// Trainer::apply_changes (trainer.cpp:749-787): the Gaussians replaced by a
+// changed cloud of `count`. New Gaussian g keeps the optimiser state of old
+// Gaussian source[g] (gathered on the GPU), or starts fresh when source[g] < 0.
+// Its values come from `cloud`, or, when cloud is null, from old Gaussian
+// source[g] as well (every source >= 0 then), so that pruning and copying need
+// no upload. Everything is allocated before anything changes, so a failure
+// leaves the trainer as it was. The densification statistics, the 3D filter,
+// the stored gradients and a captured batch are dropped, as the reference's
+// allocate_per_gaussian clears them; Adam's step count is kept.
+void Trainer::apply_changes(const vsplat_cloud *cloud, const int64_t *source, uint32_t count) {
+  if (count == 0) throw Error(VSPLAT_RUNTIME_ERROR, "the changes remove every Gaussian");
+  if (source == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no source table");
+  if (cloud != nullptr) {
+    if (cloud->count != count) throw Error(VSPLAT_INVALID_ARGUMENT, "the cloud's count differs from the source table's");
+    if (cloud->sh_degree != shDegree_) throw Error(VSPLAT_INVALID_ARGUMENT, "the cloud's SH degree differs from the trainer's");
+    if (cloud->positions == nullptr || cloud->scales == nullptr || cloud->rotations == nullptr || cloud->opacities == nullptr ||
+        cloud->sh == nullptr) {
+      throw Error(VSPLAT_INVALID_ARGUMENT, "the cloud has a null array");
+    }
+  }
+  std::vector<int32_t> table(count);
+  for (uint32_t g = 0; g < count; ++g) {
+    const int64_t from = source[g];
+    if (from >= static_cast<int64_t>(count_)) throw Error(VSPLAT_INVALID_ARGUMENT, "a source index is past the old cloud");
+    if (from < 0 && cloud == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a new Gaussian (source < 0) needs values: pass the cloud");
+    table[g] = from < 0 ? -1 : static_cast<int32_t>(from);
+  }
+  release_captured();
+  for (auto &v : views_) check(hipStreamSynchronize(v->stream), "queued work");
+
+  const size_t n = count;
+  const uint32_t valueWords[5] = {3, 3, 4, 1, shWords_}, stateWords[5] = {3, 3, 4, 1, shFloats_};
+  DeviceArray sourceTable, values[5], state[5];
+  sourceTable.ensure(n * 4);
+  for (int g = 0; g < 5; ++g) {
+    state[g].ensure(n * stateWords[g] * 4);
+    if (cloud == nullptr) values[g].ensure(n * valueWords[g] * 4);
+  }
+  if (cloud != nullptr) {
+    DeviceArray *to[5] = {&values[0], &values[1], &values[2], &values[3], &values[4]};
+    upload_cloud(*cloud, to);
+  }
+  check(hipMemcpy(sourceTable.as<int32_t>(), table.data(), n * 4, hipMemcpyHostToDevice), "upload source table");
+  DeviceArray *old[5] = {&positions_, &scales_, &rotations_, &opacities_, &shWordsBuf_};
+  GatherArray arrays[kMaxGatherArrays];
+  uint32_t k = 0;
+  for (int g = 0; g < 5; ++g) arrays[k++] = {state_[g].as<uint32_t>(), state[g].as<uint32_t>(), stateWords[g]};
+  if (cloud == nullptr)
+    for (int g = 0; g < 5; ++g) arrays[k++] = {old[g]->as<uint32_t>(), values[g].as<uint32_t>(), valueWords[g]};
+  launch_gather(stream_, sourceTable.as<int32_t>(), count, arrays, k);
+  check(hipGetLastError(), "gather launch");
+  check(hipStreamSynchronize(stream_), "gather");
+
+  for (int g = 0; g < 5; ++g) {
+    old[g]->swap(values[g]);
+    state_[g].swap(state[g]);
+    grad_[g].release();
+  }
+  count_ = count;
+  haveGradients_ = false;
+  accumulateNext_ = false;
+  release_views();
+  allocate_per_gaussian();
+  view_buffers(0);
+  check(hipStreamSynchronize(stream_), "per-Gaussian buffers");
+}
+
 void Trainer::reset_densify_stats() {
   launch_fill_u32(stream_, stats_.as<uint32_t>(), static_cast<uint64_t>(count_) * 8, 0u);
   check(hipStreamSynchronize(stream_), "reset stats");
@@ -1116,6 +1202,12 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
       {"tile_totals", &b.tileTotals, static_cast<uint64_t>(b.lossTiles) * 8},
       {"raw", fixedPoint_ ? &b.rawFixed : &b.rawFloat, n * kRawPerSplat * (fixedPoint_ ? 8u : 4u)},
       {"counters", &b.counters, kCounterWords * 4},
+      {"state_positions", &state_[0], n * 12},
+      {"state_scales", &state_[1], n * 12},
+      {"state_rotations", &state_[2], n * 16},
+      {"state_opacities", &state_[3], n * 4},
+      {"state_sh", &state_[4], n * shFloats_ * 4},
+      {"stats", &stats_, n * 32},
   };
   for (const Item &item : items) {
     if (name != item.name) continue;

@@ -37,6 +37,10 @@ public:
   void release();
   template <class T> T *as() const { return static_cast<T *>(ptr_); }
   size_t bytes() const { return bytes_; }
+  void swap(DeviceArray &other) {
+    std::swap(ptr_, other.ptr_);
+    std::swap(bytes_, other.bytes_);
+  }
 
 private:
   void *ptr_ = nullptr;
@@ -125,6 +129,7 @@ public:
   void cloud(float *positions, float *scales, float *rotations, float *opacities, float *sh) const;
   void densify_stats(float *out) const;
   void reset_densify_stats();
+  void apply_changes(const vsplat_cloud *cloud, const int64_t *source, uint32_t count);
   void set_active_sh_degree(int degree);
   void set_filter3d(const float *filter, size_t count);
   void set_sh_dropout(float probability, int retained, uint32_t seed);
@@ -146,6 +151,9 @@ public:
 private:
   ViewParams view_params(const vsplat_camera &camera) const;
   ViewBuffers &view_buffers(size_t k); // the k-th view's buffers, made on first use
+  void release_views();
+  void upload_cloud(const vsplat_cloud &cloud, DeviceArray *const values[5]) const;
+  void allocate_per_gaussian();
   void ensure_size(ViewBuffers &b, int width, int height);
   void ensure_layout(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH);
   std::pair<uint32_t, uint32_t> tile_shape_for(int width, int height) const;

@@ -38,7 +38,7 @@ EXPORTS = [
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
-    "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free", "vsplat_autotune_tiles",
+    "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free", "vsplat_autotune_tiles",
 ]
 
 
@@ -148,6 +148,7 @@ def lib():
         L.vsplat_release_captured.argtypes = [vp]
         L.vsplat_device_alloc.argtypes = [C.c_uint64, vp, vp]
         L.vsplat_device_read.argtypes = [vp, vp, C.c_uint64]
+        L.vsplat_apply_changes.argtypes = [vp, vp, vp, C.c_uint32]
         L.vsplat_device_copy.argtypes = [vp, vp, C.c_uint64, C.c_uint32, vp]
         L.vsplat_device_free.argtypes = [vp]
         L.vsplat_device_free.restype = None
@@ -265,7 +266,9 @@ def camera_to_c(cam: Camera) -> CCamera:
 INSPECT_TYPES = {"splats": (np.float32, 12), "keys": (np.uint32, 1), "values": (np.uint32, 1), "coverage": (np.uint32, 1),
                  "entry_keys": (np.uint32, 1), "entry_values": (np.uint32, 1), "ranges": (np.uint32, 2),
                  "image": (np.float32, 4), "stops": (np.uint32, 1), "adjoint": (np.float32, 4), "fields": (np.float32, 9),
-                 "tile_totals": (np.float32, 2), "counters": (np.uint32, 1)}
+                 "tile_totals": (np.float32, 2), "counters": (np.uint32, 1),
+                 "state_positions": (np.uint32, 3), "state_scales": (np.uint32, 3), "state_rotations": (np.uint32, 4),
+                 "state_opacities": (np.uint32, 1), "state_sh": (np.uint32, 1), "stats": (np.float32, 8)}
 
 
 class Trainer:
@@ -364,6 +367,21 @@ class Trainer:
 
     def reset_densify_stats(self):
         _raise(self._lib.vsplat_reset_densify_stats(self._h), "reset_densify_stats")
+
+    def apply_changes(self, cloud: Cloud | None, source) -> None:
+        """Trainer::apply_changes (trainer.cpp:749-787): the Gaussians become
+        len(source) new ones; new Gaussian g keeps old Gaussian source[g]'s
+        optimiser state (fresh when negative) and takes its values from
+        `cloud`, or from old Gaussian source[g] when cloud is None."""
+        src = np.ascontiguousarray(source, np.int64)
+        count = src.size
+        cc = None
+        if cloud is not None:
+            arrays = [np.ascontiguousarray(a, np.float32) for a in
+                      (cloud.positions, cloud.scales, cloud.rotations, cloud.opacities, cloud.sh)]
+            cc = CCloud(*[_p(a) for a in arrays], cloud.count, cloud.sh_degree)
+        _raise(self._lib.vsplat_apply_changes(self._h, None if cc is None else C.byref(cc), _p(src), count), "apply_changes")
+        self.count = count
 
     # --- settings
     def set_active_sh_degree(self, degree: int):
