@@ -255,6 +255,25 @@ public:
   }
   void reset_densify_stats() { detail::check(vsplat_reset_densify_stats(trainer_)); }
 
+  // Trainer::apply_changes (trainer.cpp:749-787) with DensifyResult's cloud and
+  // source (densify.hpp:52-63): new Gaussian g keeps old Gaussian source[g]'s
+  // optimiser state, or starts fresh when source[g] < 0.
+  void apply_changes(const GaussianCloud &cloud, const std::vector<std::int64_t> &source) {
+    if (cloud.empty()) throw std::runtime_error("densification removed every Gaussian");
+    const std::size_t n = cloud.size();
+    if (source.size() != n || cloud.positions.size() != 3 * n || cloud.scales.size() != 3 * n || cloud.rotations.size() != 4 * n ||
+        cloud.sh.size() != 3 * n * static_cast<std::size_t>(cloud.shCoeffs()))
+      throw std::invalid_argument("the cloud's arrays or the source table do not match its size");
+    vsplat_cloud c{cloud.positions.data(), cloud.scales.data(), cloud.rotations.data(), cloud.opacities.data(),
+                   cloud.sh.data(),        static_cast<std::uint32_t>(n), cloud.shDegree};
+    detail::check(vsplat_apply_changes(trainer_, &c, source.data(), static_cast<std::uint32_t>(n)));
+  }
+  // The same with the values gathered on the device as well (every source >= 0):
+  // prunes and plain copies, with nothing transferred but the table.
+  void apply_changes(const std::vector<std::int64_t> &source) {
+    detail::check(vsplat_apply_changes(trainer_, nullptr, source.data(), static_cast<std::uint32_t>(source.size())));
+  }
+
   std::vector<std::pair<std::string, double>> timings() const {
     std::uint32_t count = 0;
     detail::check(vsplat_timings(trainer_, nullptr, 0, &count));
