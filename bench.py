@@ -187,10 +187,9 @@ def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) ->
     ptrs = [b.ptr for b in buffers]
     shapes = {}
     try:
-        if tile_shape == "auto":
-            for cam, ptr in zip(w.cameras, ptrs):
-                if (cam.width, cam.height) not in shapes:
-                    shapes[(cam.width, cam.height)] = trainer.autotune_tiles(cam, ptr, 2)
+        if tile_shape == "auto":  # the sizes' shapes tuned together, on the batch
+            for cam, shape in zip(w.cameras, trainer.autotune_views(w.cameras, ptrs, 2)):
+                shapes[(cam.width, cam.height)] = shape
         trainer.step_views(w.cameras, w.targets, rates)  # sizes every buffer and the entry reservation
         single = []
         for cam, target, ptr in zip(w.cameras, w.targets, ptrs):
@@ -232,6 +231,45 @@ def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) ->
             "graph_ms": round(graph, 4), "graph_ms_per_view": round(graph / views, 4),
             "graph_speedup_per_view": round(sum(single) / graph, 4), "sh_storage": "bf16 (all 48 coefficients), fp32 arithmetic",
             "tile_shape": tile_shape, "tiles_chosen": {f"{k[0]}x{k[1]}": f"{v[0]}x{v[1]}" for k, v in shapes.items()}}
+
+
+def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str, views: int = 8) -> dict:
+    """The target comparison at one size: the 1M-Gaussian cloud of `value`, seen
+    by eight 1280x720 views on a ring of radius 3 (the ring workload's cameras,
+    all at 1280x720), gradients mode (fwd+bwd, gradients summed over the views),
+    SH stored as bf16, fixed point, captured as one HIP graph and replayed; per
+    view against the plain fp32 single 1280x720 view."""
+    import math
+    w = F.config(1, count=count)
+    cams = [F.look_at((3.0 * math.sin(2.0 * math.pi * k / views), 0.0, -3.0 * math.cos(2.0 * math.pi * k / views)), (0, 0, 0),
+                      (0.0, -1.0, 0.0), 60.0, 1280, 720) for k in range(views)]
+    targets = [F.smooth_target_explicit(1280, 720, 1016 + k) for k in range(views)]
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True,
+                                                            sh_storage=vsplat.SH_BF16, tile_shape=TILE_SHAPES[tile_shape]))
+    trainer.set_stage_timing(False)
+    buffers = [vsplat.DeviceBuffer(t) for t in targets]
+    ptrs = [b.ptr for b in buffers]
+    chosen = None
+    try:
+        if tile_shape == "auto":
+            chosen = trainer.autotune_views(cams, ptrs, 2)[0]
+        trainer.step_views(cams, targets)  # gradients mode; sizes every view's buffers
+        trainer.capture_views(cams, ptrs, vsplat.MODE_GRADIENTS)
+        trainer.launch_captured(warmup)
+        trainer.synchronize_views()
+        t0 = time.perf_counter()
+        trainer.launch_captured(steps)
+        trainer.synchronize_views()
+        graph = (time.perf_counter() - t0) * 1000.0 / steps
+        trainer.release_captured()
+    finally:
+        for b in buffers:
+            b.free()
+        trainer.close()
+    return {"workload": f"1M Gaussians, SH degree 3 stored as bf16, {views} views of 1280x720 on a ring of radius 3, "
+                        "gradients mode, one captured graph", "views": views, "graph_ms": round(graph, 4),
+            "graph_ms_per_view": round(graph / views, 4), "tile_shape": tile_shape,
+            "tiles_chosen": None if chosen is None else f"{chosen[0]}x{chosen[1]}"}
 
 
 def other_form(index: int, steps: int, warmup: int, tile_shape: str, **options) -> dict:
@@ -398,6 +436,12 @@ def main():
         bt["plain_vs_graph_per_view"] = round(result["value"] / bt["graph_ms_per_view"], 4)
         log(f"  per view {bt['batched_ms_per_view']} ms batched (speedup {bt['speedup_per_view']}), "
             f"{bt['graph_ms_per_view']} ms as a graph (speedup {bt['graph_speedup_per_view']})")
+        log("batched bf16 fwd+bwd: eight 1280x720 views")
+        same = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape)
+        same["plain_single_view_ms"] = result["value"]
+        same["plain_vs_batched_per_view"] = round(result["value"] / same["graph_ms_per_view"], 4)
+        result["batched_1280x720"] = same
+        log(f"  {same['graph_ms_per_view']} ms per view, {same['plain_vs_batched_per_view']}x the plain single view")
     if not args.no_other_forms and args.count is None:
         result["other_forms"] = {}
         for key, index, options in (("precision_bf16_3M_1920x1080", 2, {"sh_storage": vsplat.SH_BF16}),

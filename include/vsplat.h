@@ -261,6 +261,17 @@ int vsplat_release_captured(vsplat_trainer *trainer);
  * it in *tile_width, *tile_height (either may be NULL). */
 int vsplat_autotune_tiles(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *device_target, uint32_t repeats,
                           uint32_t *tile_width, uint32_t *tile_height);
+/* Tunes the tile shapes of a batch's view sizes together (no reference
+ * counterpart): the batch's passes up to the backward are timed as a whole
+ * (the views overlap on their streams), first with each size's current shape,
+ * then each size in turn with the other shapes, keeping those that make the
+ * batch faster. Used by vsplat_step_views and vsplat_capture_views with
+ * VSPLAT_TILES_AUTO the first time they meet a batch's list of sizes; the
+ * shapes kept apply to those sizes from then on. Each view's shape goes to
+ * tile_widths[k], tile_heights[k] (either may be NULL). Device targets as for
+ * vsplat_enqueue_views. No parameter, gradient or statistic changes. */
+int vsplat_autotune_views(vsplat_trainer *trainer, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *device_targets,
+                          uint32_t repeats, uint32_t *tile_widths, uint32_t *tile_heights);
 
 /* ---- Introspection, for parity tests: no reference counterpart. ---- */
 

@@ -118,3 +118,31 @@ def test_autotuned_shape_steps_as_16x16():
     for x, y in zip((ca.positions, ca.scales, ca.rotations, ca.opacities, ca.sh),
                     (cb.positions, cb.scales, cb.rotations, cb.opacities, cb.sh)):
         np.testing.assert_array_equal(_bits(x), _bits(y))
+
+
+def test_batch_autotuner_keeps_the_batch_results():
+    """The batch tuner (vsplat_autotune_views) picks a shape per view size by
+    timing the whole batch, changes no parameter, gradient or statistic, and
+    the batch's gradient sums then equal those of 16x16 tiles bit for bit."""
+    w = F.config(3, count=20_000)
+    cams, targets = w.cameras[:4], w.targets[:4]
+    buffers = [vsplat.DeviceBuffer(t) for t in targets]
+    try:
+        with vsplat.Trainer(w.cloud, vsplat.TrainerOptions(force_fixed_point=True)) as a, \
+             vsplat.Trainer(w.cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=vsplat.TILES_AUTO)) as b:
+            shapes = b.autotune_views(cams, [x.ptr for x in buffers], 2)
+            print(f"batch shapes {shapes}", flush=True)
+            assert len(shapes) == len(cams) and all(s in ((16, 16), (8, 8), (32, 8)) for s in shapes)
+            for cam, shape in zip(cams, shapes):  # one shape per size
+                assert all(t == shape for c, t in zip(cams, shapes) if (c.width, c.height) == (cam.width, cam.height))
+            assert not np.any(b.densify_stats())
+            a.step_views(cams, targets)
+            b.step_views(cams, targets)
+            ga, gb = a.gradients(), b.gradients()
+            sa, sb = a.densify_stats(), b.densify_stats()
+    finally:
+        for x in buffers:
+            x.free()
+    for name in ("positions", "scales", "rotations", "opacities", "sh"):
+        np.testing.assert_array_equal(_bits(getattr(ga, name)), _bits(getattr(gb, name)))
+    np.testing.assert_array_equal(_bits(sa), _bits(sb))

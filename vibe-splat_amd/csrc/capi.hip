@@ -235,6 +235,12 @@ int vsplat_autotune_tiles(vsplat_trainer *t, const vsplat_camera *camera, const 
   });
 }
 
+int vsplat_autotune_views(vsplat_trainer *t, uint32_t n, const vsplat_camera *cameras, const uint8_t *const *device_targets,
+                          uint32_t repeats, uint32_t *tile_widths, uint32_t *tile_heights) {
+  if (t == nullptr || cameras == nullptr || device_targets == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.autotune_views(n, cameras, device_targets, repeats, tile_widths, tile_heights); });
+}
+
 int vsplat_release_captured(vsplat_trainer *t) {
   if (t == nullptr) return invalid("null trainer");
   return guarded([&] { t->impl.release_captured(); });

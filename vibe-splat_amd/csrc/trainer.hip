@@ -390,6 +390,102 @@ std::pair<uint32_t, uint32_t> Trainer::autotune_tiles(const vsplat_camera &camer
   return best;
 }
 
+// The tile shapes of a batch's view sizes, tuned together: the batch's passes
+// up to the backward (all a shape changes; the chain rule does not depend on
+// it) are timed with events around the whole batch, first with each size's
+// current shape (16x16 for a size not tuned yet), then for each size in turn
+// with the other two shapes, keeping a shape that makes the batch faster. A
+// batch's views overlap on their streams, so the fastest shape for a view on
+// its own is not always the fastest inside the batch. Nothing of the trainer's
+// state changes but the chosen shapes and the buffers sized for them.
+void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t repeats,
+                             uint32_t *tileW, uint32_t *tileH) {
+  prepare_views(n, cameras, targets, kModeGradients, nullptr);
+  std::vector<std::pair<int, int>> sizes;
+  for (uint32_t k = 0; k < n; ++k) {
+    const std::pair<int, int> size{cameras[k].width, cameras[k].height};
+    if (std::find(sizes.begin(), sizes.end(), size) == sizes.end()) sizes.push_back(size);
+    if (tuned_.find(size) == tuned_.end()) tuned_[size] = tile_shape_for(size.first, size.second);
+  }
+  const float savedDropout = dropoutNow_;
+  dropoutNow_ = 0.0f;
+  hipEvent_t begin = nullptr, end = nullptr;
+  std::vector<ViewParams> params(n);
+  std::vector<vsplat_step_stats> stats(n);
+  const auto time_batch = [&]() -> float {
+    for (uint32_t k = 0; k < n; ++k) ensure_size(*views_[k], cameras[k].width, cameras[k].height);
+    // One run sizes the tile-entry buffers for these shapes.
+    for (int attempt = 0;; ++attempt) {
+      enqueue_view_passes(n, cameras, targets, params.data());
+      bool overflow = false;
+      synchronize_views(stats.data(), n, overflow);
+      if (!overflow) break;
+      bool grew = false;
+      for (uint32_t k = 0; k < n; ++k) {
+        grew = grew || stats[k].tile_entries > views_[k]->entryCapacity;
+        grow_entries(*views_[k], stats[k].tile_entries);
+      }
+      if (attempt > 0 && !grew) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+    }
+    float least = 0.0f;
+    for (uint32_t r = 0; r < (repeats > 0 ? repeats : 1); ++r) {
+      check(hipEventRecord(begin, stream_), "hipEventRecord");
+      enqueue_view_passes(n, cameras, targets, params.data());
+      check(hipEventRecord(end, stream_), "hipEventRecord");
+      check(hipEventSynchronize(end), "hipEventSynchronize");
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, begin, end), "hipEventElapsedTime");
+      least = r == 0 ? ms : std::min(least, ms);
+    }
+    queuedViewSizes_.clear();
+    return least;
+  };
+  const std::pair<uint32_t, uint32_t> shapes[3] = {{16u, 16u}, {8u, 8u}, {32u, 8u}};
+  try {
+    check(hipEventCreate(&begin), "hipEventCreate");
+    check(hipEventCreate(&end), "hipEventCreate");
+    float best = time_batch();
+    for (const auto &size : sizes) {
+      auto keep = tuned_[size];
+      for (const auto &shape : shapes) {
+        if (shape == keep) continue;
+        tuned_[size] = shape;
+        const float ms = time_batch();
+        if (ms < best) {
+          best = ms;
+          keep = shape;
+        }
+      }
+      tuned_[size] = keep;
+    }
+  } catch (...) {
+    if (begin != nullptr) (void)hipEventDestroy(begin);
+    if (end != nullptr) (void)hipEventDestroy(end);
+    dropoutNow_ = savedDropout;
+    throw;
+  }
+  (void)hipEventDestroy(begin);
+  (void)hipEventDestroy(end);
+  dropoutNow_ = savedDropout;
+  for (uint32_t k = 0; k < n; ++k) {
+    ensure_size(*views_[k], cameras[k].width, cameras[k].height);
+    const auto shape = tuned_[{cameras[k].width, cameras[k].height}];
+    if (tileW != nullptr) tileW[k] = shape.first;
+    if (tileH != nullptr) tileH[k] = shape.second;
+  }
+  batchTunedKey_ = sizes_key(n, cameras);
+}
+
+// The view sizes of a batch, in order, as one number.
+uint64_t Trainer::sizes_key(uint32_t n, const vsplat_camera *cameras) {
+  uint64_t h = 1469598103934665603ull ^ n;
+  for (uint32_t k = 0; k < n; ++k) {
+    h = (h ^ static_cast<uint32_t>(cameras[k].width)) * 1099511628211ull;
+    h = (h ^ static_cast<uint32_t>(cameras[k].height)) * 1099511628211ull;
+  }
+  return h;
+}
+
 void Trainer::grow_entries(ViewBuffers &b, uint64_t entries) {
   if (entries <= b.entryCapacity) return;
   if (entries > 0xFFFFFFFFull) throw Error(VSPLAT_BUDGET_EXCEEDED, std::to_string(entries) + " tile entries exceed the 32-bit index");
@@ -799,17 +895,15 @@ void Trainer::prepare_views(uint32_t n, const vsplat_camera *cameras, const uint
   viewStats_.ensure(static_cast<size_t>(n) * kViewStatsBytes);
 }
 
-void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
-                            const vsplat_learning_rates *rates) {
-  prepare_views(n, cameras, targets, mode, rates);
+// Each view's passes up to its backward pass, on the view's stream, after
+// everything already queued on the trainer's stream (the previous step's
+// update of the parameters); each view's counters and loss sums copied to its
+// stats slot; the trainer's stream then waits for every view and gathers their
+// sticky flags.
+void Trainer::enqueue_view_passes(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params) {
   queuedViewSizes_.clear();
-  dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
-  statisticsNext_ = true;
   current_ = nullptr; // no stage events in a batch
-  // The views start after everything already queued on the trainer's stream
-  // (the previous step's update of the parameters).
   check(hipEventRecord(batchStart_, stream_), "hipEventRecord");
-  std::vector<ViewParams> params(n);
   for (uint32_t k = 0; k < n; ++k) {
     ViewBuffers &b = *views_[k];
     if (b.stream != stream_) check(hipStreamWaitEvent(b.stream, batchStart_, 0), "hipStreamWaitEvent");
@@ -827,6 +921,16 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
     check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
     launch_merge_flags(stream_, status, views_[k]->counters.as<uint32_t>());
   }
+}
+
+void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                            const vsplat_learning_rates *rates) {
+  prepare_views(n, cameras, targets, mode, rates);
+  dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
+  statisticsNext_ = true;
+  std::vector<ViewParams> params(n);
+  enqueue_view_passes(n, cameras, targets, params.data());
+  uint32_t *status = views_[0]->counters.as<uint32_t>();
   // The chain rule over the views in chunks of kChainViewsPerLaunch, each
   // chunk continuing the sums of the one before; the last takes the step.
   const ChainBuffers buffers = chain_buffers(*views_[0]);
@@ -917,8 +1021,9 @@ void Trainer::release_captured() {
 void Trainer::capture_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
                             const vsplat_learning_rates *rates) {
   release_captured();
-  for (uint32_t k = 0; k < n && cameras != nullptr && targets != nullptr; ++k) {
-    if (needs_tuning(cameras[k].width, cameras[k].height) && targets[k] != nullptr) autotune_tiles(cameras[k], targets[k], 2);
+  if (options_.tile_shape == VSPLAT_TILES_AUTO && n > 0 && cameras != nullptr && targets != nullptr &&
+      sizes_key(n, cameras) != batchTunedKey_) {
+    autotune_views(n, cameras, targets, 2, nullptr, nullptr);
   }
   prepare_views(n, cameras, targets, mode, rates);
   check(hipStreamSynchronize(stream_), "before capture");
@@ -986,8 +1091,8 @@ uint32_t Trainer::step_views(uint32_t n, const vsplat_camera *cameras, const uin
     check(hipMemcpyAsync(viewTargets_[k]->as<uint8_t>(), targets[k], bytes[k], hipMemcpyHostToDevice, stream_), "upload target");
     device[k] = viewTargets_[k]->as<uint8_t>();
   }
-  for (uint32_t k = 0; k < n; ++k) {
-    if (needs_tuning(cameras[k].width, cameras[k].height)) autotune_tiles(cameras[k], device[k], 2);
+  if (options_.tile_shape == VSPLAT_TILES_AUTO && sizes_key(n, cameras) != batchTunedKey_) {
+    autotune_views(n, cameras, device.data(), 2, nullptr, nullptr);
   }
   std::vector<vsplat_step_stats> all(n);
   for (int attempt = 0;; ++attempt) {

@@ -118,6 +118,10 @@ public:
   void reserve_entries(uint64_t entries);
   // Times each tile shape on this view and keeps the fastest for its size.
   std::pair<uint32_t, uint32_t> autotune_tiles(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t repeats);
+  // Tunes the shapes of a batch's view sizes together, timing the whole batch;
+  // each view's chosen shape into tileW[k], tileH[k] (either may be null).
+  void autotune_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t repeats, uint32_t *tileW,
+                      uint32_t *tileH);
   // A batch captured as a HIP graph, replayed with launch_captured and waited
   // for with synchronize_views.
   void capture_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
@@ -152,6 +156,8 @@ private:
   ViewParams view_params(const vsplat_camera &camera) const;
   ViewBuffers &view_buffers(size_t k); // the k-th view's buffers, made on first use
   void release_views();
+  void enqueue_view_passes(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params);
+  static uint64_t sizes_key(uint32_t n, const vsplat_camera *cameras);
   void upload_cloud(const vsplat_cloud &cloud, DeviceArray *const values[5]) const;
   void allocate_per_gaussian();
   void ensure_size(ViewBuffers &b, int width, int height);
@@ -215,6 +221,7 @@ private:
   uint64_t graphKey_ = 0;
   uint64_t reservedEntries_ = 0; // what reserve_entries asked for, for views made later
   std::map<std::pair<int, int>, std::pair<uint32_t, uint32_t>> tuned_; // view size -> tile shape the autotuner chose
+  uint64_t batchTunedKey_ = 0; // sizes_key of the batch autotune_views last tuned
   DeviceArray target_, viewStats_;
   std::vector<std::unique_ptr<DeviceArray>> viewTargets_; // step_views' targets, one per view
   PinnedArray staging_, next_, readback_;

@@ -38,7 +38,8 @@ EXPORTS = [
     "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
-    "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free", "vsplat_autotune_tiles",
+    "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free",
+    "vsplat_autotune_tiles", "vsplat_autotune_views",
 ]
 
 
@@ -153,6 +154,7 @@ def lib():
         L.vsplat_device_free.argtypes = [vp]
         L.vsplat_device_free.restype = None
         L.vsplat_autotune_tiles.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
+        L.vsplat_autotune_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
         _LIB = L
     return _LIB
 
@@ -480,6 +482,16 @@ class Trainer:
         _raise(self._lib.vsplat_autotune_tiles(self._h, C.byref(cam), C.c_void_p(device_target_ptr), repeats, C.byref(w), C.byref(h)),
                "autotune_tiles")
         return w.value, h.value
+
+    def autotune_views(self, cameras: list, device_target_ptrs: list, repeats: int = 2) -> list:
+        """Tunes the tile shapes of the batch's view sizes together, timing the
+        whole batch; returns each view's (width, height)."""
+        n = len(cameras)
+        cams = (CCamera * n)(*[camera_to_c(c) for c in cameras])
+        ptrs = (C.c_void_p * n)(*device_target_ptrs)
+        w, h = (C.c_uint32 * n)(), (C.c_uint32 * n)()
+        _raise(self._lib.vsplat_autotune_views(self._h, n, cams, ptrs, repeats, w, h), "autotune_views")
+        return [(w[k], h[k]) for k in range(n)]
 
     def release_captured(self):
         _raise(self._lib.vsplat_release_captured(self._h), "release_captured")
