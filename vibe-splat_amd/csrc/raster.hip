@@ -15,14 +15,14 @@
 // Each wave holds an 8x8 block of the tile, lane l its pixel (l & 7, l >> 3):
 // a splat's footprint fills more of a square block's lanes than of a strip's.
 //
-// Backward: for each splat the 14
-// per-pixel sums are reduced over the wave by a transposed butterfly in which
-// each step halves the values a lane carries: partner lanes l ^ 32 and l ^ 16
+// Backward: for each splat the 12 non-zero of the 14 per-pixel sums are
+// reduced over the wave by a transposed butterfly in which the steps halve the
+// values a lane carries where they can: partner lanes l ^ 32 and l ^ 16
 // through the permlane swaps, l ^ 15 and l ^ 7 through DPP row and half-row
-// mirrors, then l ^ 2 and l ^ 1 through quad permutations. Lane 4j ends with
-// total j, which goes to global memory with one atomic: 64-bit fixed point
-// (each wave total quantised, so the integer sums are the same in any order)
-// or float.
+// mirrors, then l ^ 2 and l ^ 1 through quad permutations. Twelve lanes end
+// with one total each, which goes to global memory with one atomic: 64-bit
+// fixed point (each wave total quantised, so the integer sums are the same in
+// any order) or float.
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -119,22 +119,23 @@ __device__ __forceinline__ void composite_tile(const Splat *__restrict__ splats,
       const float a = s.o * e;
       const float alpha = a < 0.99f ? a : 0.99f;
       const bool ok = candidate && !(power > 0.0f) && alpha >= kMinAlpha;
+      if (!__any(ok)) continue;
+      // Without branches: a lane that takes nothing adds a zero weight
+      // (fma(0, c, r) is r) and keeps its transmittance by selection.
       const float logRest = log_normal(1.0f - alpha);
-      if (ok) {
-        if (logT + logRest < kLogMinTransmittance) {
-          stop = end + k + 1;
-          done = true;
-        } else {
-          if constexpr (Colour) {
-            const float w = T * alpha;
-            r = fma32(w, s.r, r);
-            g = fma32(w, s.g, g);
-            b = fma32(w, s.b, b);
-            T = T * (1.0f - alpha);
-          }
-          logT = logT + logRest;
-        }
+      const float logNext = logT + logRest;
+      const bool stopHere = ok && logNext < kLogMinTransmittance;
+      const bool take = ok && !stopHere;
+      if constexpr (Colour) {
+        const float w = take ? T * alpha : 0.0f;
+        r = fma32(w, s.r, r);
+        g = fma32(w, s.g, g);
+        b = fma32(w, s.b, b);
+        T = take ? T * (1.0f - alpha) : T;
       }
+      logT = take ? logNext : logT;
+      stop = stopHere ? end + k + 1 : stop;
+      done = done || stopHere;
     }
   }
 }
@@ -174,41 +175,52 @@ __device__ __forceinline__ float dpp_half_mirror(float v) { return as_float(__bu
 __device__ __forceinline__ float dpp_xor2(float v) { return as_float(__builtin_amdgcn_update_dpp(0, static_cast<int>(as_uint(v)), 0x4E, 0xF, 0xF, false)); }
 __device__ __forceinline__ float dpp_xor1(float v) { return as_float(__builtin_amdgcn_update_dpp(0, static_cast<int>(as_uint(v)), 0xB1, 0xF, 0xF, false)); }
 
-// Sixteen values per lane to one total per lane group: afterwards lane l holds
-// the wave's total of value (l >> 2) & 15 in v[0].
-__device__ __forceinline__ void reduce_wave(float (&v)[16], uint32_t lane) {
-  // Partner l ^ 32: lanes 0-31 keep values 0-7, lanes 32-63 values 8-15.
+// Twelve values per lane to the wave's twelve totals, every value through the
+// same tree (partners l ^ 32, 16, 15, 7, 2, 1 in turn; a + b and b + a are the
+// same float). Afterwards lane l holds in v[0] the total of value
+// reduced_value(l); wave_emits(l) picks one lane per value.
+__device__ __forceinline__ void reduce_wave(float (&v)[12], uint32_t lane) {
+  // Partner l ^ 32: lanes 0-31 keep values 0-5, lanes 32-63 values 6-11.
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const auto p = __builtin_amdgcn_permlane32_swap(as_uint(v[i]), as_uint(v[i + 8]), false, false);
+  for (int i = 0; i < 6; ++i) {
+    const auto p = __builtin_amdgcn_permlane32_swap(as_uint(v[i]), as_uint(v[i + 6]), false, false);
     v[i] = as_float(p[0]) + as_float(p[1]);
   }
-  // Partner l ^ 16: rows 0 and 2 keep the lower four, rows 1 and 3 the upper.
+  // Partner l ^ 16: rows 0 and 2 keep their first three, rows 1 and 3 the others.
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const auto p = __builtin_amdgcn_permlane16_swap(as_uint(v[i]), as_uint(v[i + 4]), false, false);
+  for (int i = 0; i < 3; ++i) {
+    const auto p = __builtin_amdgcn_permlane16_swap(as_uint(v[i]), as_uint(v[i + 3]), false, false);
     v[i] = as_float(p[0]) + as_float(p[1]);
   }
-  // Partner l ^ 15 (row mirror): lanes with bit 3 clear keep the lower two.
-  {
-    const bool upper = (lane & 8u) != 0u;
+  // Partner l ^ 15 (row mirror), all three kept.
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const float send = upper ? v[i] : v[i + 2];
-      const float keep = upper ? v[i + 2] : v[i];
-      v[i] = keep + dpp_row_mirror(send);
-    }
-  }
-  // Partner l ^ 7 (half-row mirror): lanes with bit 2 clear keep the lower one.
+  for (int i = 0; i < 3; ++i) v[i] = v[i] + dpp_row_mirror(v[i]);
+  // Partner l ^ 7 (half-row mirror): lanes with bit 2 clear keep the first of
+  // the first two, the others the second; all keep the third.
   {
     const bool upper = (lane & 4u) != 0u;
     const float send = upper ? v[0] : v[1];
     const float keep = upper ? v[1] : v[0];
     v[0] = keep + dpp_half_mirror(send);
+    v[2] = v[2] + dpp_half_mirror(v[2]);
   }
-  v[0] = v[0] + dpp_xor2(v[0]);
+  // Partner l ^ 2: lanes with bit 1 clear keep the one they kept, the others the third.
+  {
+    const bool upper = (lane & 2u) != 0u;
+    const float send = upper ? v[0] : v[2];
+    const float keep = upper ? v[2] : v[0];
+    v[0] = keep + dpp_xor2(send);
+  }
   v[0] = v[0] + dpp_xor1(v[0]);
 }
+// The value whose total lane l holds after reduce_wave: 6 b5 + 3 b4 + (b1 ? 2 : b2),
+// with bk bit k of l.
+__device__ __forceinline__ uint32_t reduced_value(uint32_t lane) {
+  return 6u * ((lane >> 5) & 1u) + 3u * ((lane >> 4) & 1u) + (((lane >> 1) & 1u) != 0u ? 2u : (lane >> 2) & 1u);
+}
+// One lane per value: bits 0 and 3 clear, and not both bits 1 and 2 (those
+// lanes hold the third value twice).
+__device__ __forceinline__ bool wave_emits(uint32_t lane) { return (lane & 9u) == 0u && (lane & 6u) != 6u; }
 
 // Back to front over the tile (train_backward.glsl:78-158). With Recompute the
 // pixel's log transmittance and stop come from a front-to-back sweep of its
@@ -285,7 +297,7 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
       const float vx = fma32(s.qy, dy, s.qx * dx), vy = fma32(s.qz, dy, s.qy * dx);
       const float g1 = dPower * vx, g2 = dPower * vy;
       const float hp = 0.5f * dPower;
-      float v[16];
+      float v[12];
       v[0] = dAlpha * e;
       v[1] = g1;
       v[2] = g2;
@@ -298,7 +310,6 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
       v[9] = Ta * perPixel;
       v[10] = fabsf(g1);
       v[11] = fabsf(g2);
-      v[12] = v[13] = v[14] = v[15] = 0.0f;
       // The colour and log-transmittance behind: unchanged where alpha is taken as 0.
       const float alphaBehind = ok ? alpha : 0.0f;
       Br = fma32(alphaBehind, s.r - Br, Br);
@@ -307,8 +318,8 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
       logBehind = logBehind + (ok ? logRest : 0.0f);
       reduce_wave(v, lane);
       // Values 12 and 13 of the kRawPerSplat slots are always zero here.
-      const uint32_t value = lane >> 2;
-      if ((lane & 3u) == 0 && value < 12u) {
+      const uint32_t value = reduced_value(lane);
+      if (wave_emits(lane)) {
         const size_t slot = static_cast<size_t>(batchIndex[k]) * kRawPerSplat + value;
         if constexpr (Fixed) {
           const int32_t q = quantise(v[0], fixedScale);
