@@ -94,7 +94,8 @@ typedef struct {
   float position, scale, rotation, opacity, sh, sh_rest_scale, trust_region;
 } vsplat_learning_rates;
 
-/* StepStats (src/train/trainer.hpp:63-73). The exposure and pose gradients stay zero. */
+/* StepStats (src/train/trainer.hpp:63-73). The pose gradient stays zero; the
+ * exposure gradient is filled when an exposure is set (vsplat_set_exposure). */
 typedef struct {
   float loss, l1, ssim;
   uint32_t visible_splats;
@@ -128,6 +129,20 @@ int vsplat_compute_gradients(vsplat_trainer *trainer, const vsplat_camera *camer
 /* Trainer::collect_statistics (trainer.hpp:106): densification statistics only. */
 int vsplat_collect_statistics(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *target, size_t target_bytes,
                               vsplat_step_stats *stats);
+/* Trainer::score_view (trainer.hpp:111, trainer.cpp:519-532): statistics of the
+ * view with its high-error pixels marked (error at least error_share of the
+ * view's worst), then each Gaussian's marked pixels, those times the view's
+ * loss, and its edge score (the target's edge weight times its blending
+ * weight) added to its densification statistics, in pixels times
+ * pixel_weight. VSPLAT_INVALID_ARGUMENT for error_share <= 0. */
+int vsplat_score_view(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *target, size_t target_bytes,
+                      float error_share, float pixel_weight, vsplat_step_stats *stats);
+/* Trainer::set_exposure (trainer.hpp:154): the view's affine colour transform,
+ * 12 floats, rows of (r, g, b, 1): the loss compares T (rgb, 1) with the
+ * target, and the synchronous calls report dL/dT in stats->exposure_gradient.
+ * The neutral transform (1 0 0 0, 0 1 0 0, 0 0 1 0) skips the passes. The
+ * queued and batched calls refuse a non-neutral one (VSPLAT_UNSUPPORTED). */
+int vsplat_set_exposure(vsplat_trainer *trainer, const float *transform);
 /* Trainer::apply_gradients (trainer.hpp:121): one Adam step from the last compute_gradients. */
 int vsplat_apply_gradients(vsplat_trainer *trainer, const vsplat_learning_rates *rates);
 /* Trainer::step (trainer.hpp:125-126): compute_gradients and the Adam step in one

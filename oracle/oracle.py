@@ -7,6 +7,14 @@ Oracle A (``render``, ``loss``, ``gradients``, ``contributions``) is the
 double-precision restatement of the reference's src/train/reference.cpp.
 Oracle B (``Model``) is the single-precision model of the GPU pipeline.
 ``gather`` restates the state gather of a changed cloud (train_gather.comp);
+``exposed_image``, ``exposure_backward``, ``score_fields``, ``error_mask`` and
+``score_statistics`` restate the exposure and scoring passes elementwise in
+float32, in this build's operation order; ``marked_pixels`` (Oracle A) is
+reference.cpp's FP64 count of high-error pixels per Gaussian. The float32
+restatements are pinned by tests/test_scoring.py (closed forms: a neutral
+exposure is the identity, the exposure gradient against FP64 central
+differences of Oracle A's loss, the edge weight of flat and stepped targets,
+the error mask's threshold) and, on the GPU, by Oracle A's marked pixels;
 it is pinned by tests/test_densify.py: a brute-force loop on small tables, the
 identity table, the composition of two tables, and fresh (zero) state for
 negative sources.
@@ -79,6 +87,7 @@ def lib():
         vp = C.c_void_p
         L.oa_render.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp]
         L.oa_contributions.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.oa_marked_pixels.argtypes = [vp, vp, vp, C.c_double, vp, C.c_int, vp]
         L.oa_loss.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp]
         L.oa_gradients.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
         L.ob_preprocess.argtypes = [vp] * 11
@@ -87,8 +96,8 @@ def lib():
         L.ob_tile_ranges.argtypes = [vp, C.c_uint64, C.c_uint32, vp]
         L.ob_forward.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]
         L.ob_loss.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_float, vp, vp, vp]
-        L.ob_backward_fixed.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_float, C.c_uint32, vp]
-        L.ob_backward_float.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_uint32, vp]
+        L.ob_backward_fixed.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_float, C.c_uint32, vp, vp]
+        L.ob_backward_float.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_uint32, vp, vp]
         L.ob_fixed_to_float.argtypes = [vp, C.c_uint64, C.c_float, vp]
         L.ob_chain.argtypes = [vp] * 21
         L.ob_sh_step.argtypes = [vp] * 5
@@ -102,6 +111,100 @@ def lib():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+NEUTRAL_EXPOSURE = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
+
+
+def exposed_image(image: np.ndarray, transform) -> np.ndarray:
+    """The image as the view's exposure makes it (train_exposure.comp:17-24):
+    rgb' = T (rgb, 1), each row's dot product added left to right in float32;
+    the fourth value copied."""
+    t = np.asarray(transform, np.float32).reshape(3, 4)
+    img = np.asarray(image, np.float32).reshape(-1, 4)
+    out = img.copy()
+    for row in range(3):
+        a = t[row]
+        out[:, row] = ((a[0] * img[:, 0] + a[1] * img[:, 1]) + a[2] * img[:, 2]) + a[3]
+    return out
+
+
+def _tile_sums(values: np.ndarray) -> np.ndarray:
+    """Per 256-value group, the sum in the GPU's workgroup tree: within each
+    64-lane wave lane l plus lane l ^ m for m = 32 ... 1, then the four waves
+    added in order."""
+    n = values.shape[0]
+    groups = (n + 255) // 256
+    v = np.zeros((groups * 256,) + values.shape[1:], np.float32)
+    v[:n] = values
+    v = v.reshape((groups, 4, 64) + values.shape[1:])
+    lanes = np.arange(64)
+    for m in (32, 16, 8, 4, 2, 1):
+        v = (v + v[:, :, lanes ^ m]).astype(np.float32)
+    w = v[:, :, 0]
+    return (((w[:, 0] + w[:, 1]).astype(np.float32) + w[:, 2]).astype(np.float32) + w[:, 3]).astype(np.float32)
+
+
+def exposure_backward(image: np.ndarray, adjoint: np.ndarray, transform):
+    """The adjoint taken back through the exposure and the twelve sums of
+    dL/dT per group of 256 pixels (train_exposure_backward.comp:22-40):
+    back = A^T g added row by row; totals g[row] (rgb, 1)[column]."""
+    t = np.asarray(transform, np.float32).reshape(3, 4)
+    img = np.asarray(image, np.float32).reshape(-1, 4)
+    adj = np.asarray(adjoint, np.float32).reshape(-1, 4).copy()
+    g = adj[:, :3].copy()
+    for k in range(3):
+        adj[:, k] = (g[:, 0] * t[0, k] + g[:, 1] * t[1, k]) + g[:, 2] * t[2, k]
+    rgb1 = np.concatenate([img[:, :3], np.ones((img.shape[0], 1), np.float32)], 1)
+    products = (g[:, :, None] * rgb1[:, None, :]).reshape(-1, 12).astype(np.float32)
+    return adj, _tile_sums(products)
+
+
+def _unorm8(b):
+    return np.asarray(b, np.float32) * np.float32(1.0 / 255.0)
+
+
+def score_fields(image: np.ndarray, target: np.ndarray, width: int, height: int):
+    """What the loss pass leaves when scoring (train_loss.comp:34-51, 100-110):
+    each pixel's mean absolute error over the channels, the view's worst in
+    1/65535 (truncated), and the target's edge weight |Laplacian of the
+    luminance| / 4 clamped to [0, 0.999], the luminance 0.299 r + 0.587 g +
+    0.114 b added left to right, the Laplacian as differences from the centre."""
+    img = np.asarray(image, np.float32).reshape(-1, 4)
+    tgt = np.asarray(target, np.uint8).reshape(height, width, 4)
+    y = _unorm8(tgt[..., :3]).reshape(-1, 3)
+    d = np.abs(img[:, :3] - y)
+    own = ((d[:, 0] + d[:, 1]).astype(np.float32) + d[:, 2]).astype(np.float32)
+    error = (own / np.float32(3.0)).astype(np.float32)
+    worst = int((np.clip(error, 0, 1) * np.float32(65535.0)).astype(np.float32).astype(np.uint32).max())
+    lum = ((_unorm8(tgt[..., 0]) * np.float32(0.299) + _unorm8(tgt[..., 1]) * np.float32(0.587)).astype(np.float32) +
+           _unorm8(tgt[..., 2]) * np.float32(0.114)).astype(np.float32)
+    pad = np.pad(lum, 1, mode="edge")
+    c = pad[1:-1, 1:-1]
+    lap = ((((pad[1:-1, :-2] - c) + (pad[1:-1, 2:] - c)).astype(np.float32) + (pad[:-2, 1:-1] - c)).astype(np.float32) +
+           (pad[2:, 1:-1] - c)).astype(np.float32)
+    edge = np.clip(np.abs(lap) / np.float32(4.0), np.float32(0.0), np.float32(0.999)).astype(np.float32).reshape(-1)
+    return error, worst, edge
+
+
+def error_mask(error: np.ndarray, worst: int, share: float) -> np.ndarray:
+    """train_error_mask.comp:14-19: 1 where the error is at least share of the
+    worst (share x worst / 65535 in float32), else 0."""
+    limit = (np.float32(share) * np.float32(worst)) / np.float32(65535.0)
+    return np.where((worst > 0) & (np.asarray(error, np.float32) >= limit), np.float32(1.0), np.float32(0.0)).astype(np.float32)
+
+
+def score_statistics(stats: np.ndarray, raw: np.ndarray, to_pixels: float, loss: float) -> np.ndarray:
+    """train_score.comp:17-26: to each Gaussian's statistics (8 floats) its
+    marked pixels raw[12] x to_pixels, those times the loss, and its edge
+    score raw[13] x to_pixels, in float32."""
+    out = np.asarray(stats, np.float32).reshape(-1, 8).copy()
+    r = np.asarray(raw, np.float32).reshape(-1, 14)
+    pixels = (r[:, 12] * np.float32(to_pixels)).astype(np.float32)
+    out[:, 5] = out[:, 5] + pixels
+    out[:, 6] = out[:, 6] + (pixels * np.float32(loss)).astype(np.float32)
+    out[:, 7] = out[:, 7] + (r[:, 13] * np.float32(to_pixels)).astype(np.float32)
+    return out
 
 
 def gather(old: np.ndarray, source, stride: int) -> np.ndarray:
@@ -220,6 +323,18 @@ def contributions(cloud, cam, options=None, threads=None) -> np.ndarray:
     cc, ca, co = _ccloud(cloud, held), _ccamera(cam), _coptions(options, held)
     _check(lib().oa_contributions(C.byref(cc), C.byref(ca), C.byref(co), threads or default_threads(), _p(out)),
            "oa_contributions")
+    return out
+
+
+def marked_pixels(cloud, cam, target, share, options=None, threads=None) -> np.ndarray:
+    """reference.cpp:248-272 (FP64): per Gaussian, its composited pixels whose
+    mean error is at least `share` of the view's worst."""
+    held = _Held()
+    out = np.zeros(cloud.count)
+    tgt = np.ascontiguousarray(target, np.uint8)
+    cc, ca, co = _ccloud(cloud, held), _ccamera(cam), _coptions(options, held)
+    _check(lib().oa_marked_pixels(C.byref(cc), C.byref(ca), _p(tgt), float(share), C.byref(co), threads or default_threads(),
+                                  _p(out)), "oa_marked_pixels")
     return out
 
 
@@ -371,22 +486,38 @@ class Model:
                             _p(image), _p(stops)), "ob_forward")
         return ModelResult(splats, keys, values, coverage, int(counts[0]), E, ek, ev, ranges, image, stops)
 
-    def loss(self, r: ModelResult, target, lam):
+    def loss(self, r: ModelResult, target, lam, *, exposure=None, score_share=None):
+        """The loss and its adjoint (ob_loss). With an exposure (12 floats) the
+        loss sees the exposed image and the adjoint is taken back through it,
+        r.exposure_gradient holding dL/dT; with score_share the adjoint's fourth
+        value marks the high-error pixels and r.edge holds the edge weights."""
         P = self.cam.width * self.cam.height
         tgt = np.ascontiguousarray(target, np.uint8).reshape(-1)
         r.fields = np.zeros((P, 9), np.float32)
         r.adjoint = np.zeros((P, 4), np.float32)
         r.tile_totals = np.zeros((self.tiles, 2), np.float32)
-        _check(lib().ob_loss(_p(r.image), _p(tgt), self.cam.width, self.cam.height, float(lam), _p(r.fields),
-                             _p(r.adjoint), _p(r.tile_totals)), "ob_loss")
+        seen = r.image if exposure is None else exposed_image(r.image, exposure)
+        _check(lib().ob_loss(_p(np.ascontiguousarray(seen, np.float32)), _p(tgt), self.cam.width, self.cam.height, float(lam),
+                             _p(r.fields), _p(r.adjoint), _p(r.tile_totals)), "ob_loss")
+        if exposure is not None:
+            r.adjoint, totals = exposure_backward(r.image, r.adjoint, exposure)
+            # The groups' totals added in double in order (trainer.cpp:620-628).
+            r.exposure_gradient = np.cumsum(totals.astype(np.float64), axis=0)[-1].astype(np.float32)
+        r.edge = None
+        if score_share is not None:
+            error, worst, r.edge = score_fields(seen, target, self.cam.width, self.cam.height)
+            r.error_max = worst
+            r.adjoint = r.adjoint.copy()
+            r.adjoint[:, 3] = error_mask(error, worst, score_share)
         return r
 
     def backward_fixed(self, r: ModelResult, adjoint=None):
         adj = r.adjoint if adjoint is None else np.ascontiguousarray(adjoint, np.float32)
+        edge = getattr(r, "edge", None)
         r.raw_fixed = np.zeros((self.n, 14), np.int64)
         _check(lib().ob_backward_fixed(_p(r.splats), _p(r.entry_values), _p(r.ranges), self.cam.width, self.cam.height,
                                        _p(self.background), _p(r.image), _p(adj), _p(r.stops), self.fixed_scale(),
-                                       self.n, _p(r.raw_fixed)), "ob_backward_fixed")
+                                       self.n, _p(r.raw_fixed), None if edge is None else _p(edge)), "ob_backward_fixed")
         r.raw = np.zeros((self.n, 14), np.float32)
         _check(lib().ob_fixed_to_float(_p(r.raw_fixed), self.n * 14, self.fixed_scale(), _p(r.raw)), "ob_fixed_to_float")
         return r
@@ -394,9 +525,10 @@ class Model:
     def backward_float(self, r: ModelResult, adjoint=None):
         adj = r.adjoint if adjoint is None else np.ascontiguousarray(adjoint, np.float32)
         r.raw = np.zeros((self.n, 14), np.float32)
+        edge = getattr(r, "edge", None)
         _check(lib().ob_backward_float(_p(r.splats), _p(r.entry_values), _p(r.ranges), self.cam.width, self.cam.height,
-                                       _p(self.background), _p(r.image), _p(adj), _p(r.stops), self.n, _p(r.raw)),
-               "ob_backward_float")
+                                       _p(self.background), _p(r.image), _p(adj), _p(r.stops), self.n, _p(r.raw),
+                                       None if edge is None else _p(edge)), "ob_backward_float")
         return r
 
     def chain_options(self, mode=1, flags=1, opacity_reg=0.0, scale_reg=0.0, rates=(0, 0, 0, 0, 0), rest_scale=0.05,
@@ -431,6 +563,23 @@ class Model:
                               _p(r.splats), _p(st), *[_p(s) for s in states], *[_p(x) for x in g], C.byref(opt)),
                "ob_chain")
         return g, st
+
+    def statistics(self, r: ModelResult, stats=None) -> np.ndarray:
+        """Chain rule in statistics mode (collect_statistics, score_view): only
+        the densification statistics, added to `stats` (8 floats per Gaussian)."""
+        n = self.n
+        c = self.cloud
+        pos = np.ascontiguousarray(c.positions, np.float32).copy(); sc = np.ascontiguousarray(c.scales, np.float32).copy()
+        rot = np.ascontiguousarray(c.rotations, np.float32).copy(); op = np.ascontiguousarray(c.opacities, np.float32).copy()
+        words = self.words.copy()
+        raw = r.raw.copy()
+        st = np.zeros((n, 8), np.float32) if stats is None else stats
+        states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
+        opt = self.chain_options(0, 1)
+        _check(lib().ob_chain(_p(pos), _p(sc), _p(rot), _p(op), _p(words), _p(self.filter3d), C.byref(self.camera), _p(raw),
+                              _p(r.splats), _p(st), *[_p(x) for x in states], None, None, None, None, None, C.byref(opt)),
+               "ob_chain")
+        return st
 
     def step(self, r: ModelResult, rates, *, rest_scale=0.05, step=1, selective=True, opacity_reg=0.0, scale_reg=0.0):
         """Chain rule in step mode, then SH's part of Adam (train_gradient.comp,

@@ -400,6 +400,42 @@ extern "C" int oa_contributions(const oc_cloud *c, const oc_camera *camera, cons
   return 0;
 }
 
+// reference.cpp:248-272: per Gaussian, the pixels it is composited into whose
+// mean error (over the channels, against the target / 255) is at least
+// `share` of the view's worst.
+extern "C" int oa_marked_pixels(const oc_cloud *c, const oc_camera *camera, const uint8_t *target, double share,
+                                const oa_options *opt, int threads, double *out) {
+  if (c == nullptr || camera == nullptr || target == nullptr || out == nullptr) return -1;
+  const Cloud cloud{c};
+  const View cam = view_of(camera);
+  const Options options = options_of(opt);
+  const Image image = composite(cloud, cam, options, threads, nullptr);
+  std::vector<double> errors(image.rgb.size(), 0.0);
+  double worst = 0.0;
+  for (std::size_t at = 0; at < errors.size(); ++at) {
+    const double rgb[3] = {image.rgb[at].x, image.rgb[at].y, image.rgb[at].z};
+    for (std::size_t k = 0; k < 3; ++k) errors[at] += std::abs(rgb[k] - target[at * 4 + k] / 255.0) / 3.0;
+    worst = std::max(worst, errors[at]);
+  }
+  std::fill(out, out + cloud.size(), 0.0);
+  const std::vector<Splat> splats = prepare(cloud, cam, options, threads);
+  for (std::size_t j = 0; j < splats.size(); ++j) {
+    const Splat &s = splats[j];
+    int x0, x1, y0, y1;
+    bounds(s, cam, options.noCutoffs, x0, x1, y0, y1);
+    for (int y = y0; y <= y1; ++y) {
+      for (int x = x0; x <= x1; ++x) {
+        D2 d;
+        double power = 0.0, alpha = 0.0;
+        if (!splat_at(s, x, y, d, power, alpha, options.noCutoffs)) continue;
+        const std::size_t at = static_cast<std::size_t>(y) * cam.width + x;
+        if (j >= image.stop[at] && errors[at] >= share * worst) out[s.index] += 1.0;
+      }
+    }
+  }
+  return 0;
+}
+
 // reference.cpp:276-341
 extern "C" int oa_loss(const double *rgb, const uint8_t *target, int w, int h, double lambda, int threads, double *adjoint,
                        double *stats) {

@@ -53,6 +53,10 @@ typedef struct {
 int oa_render(const oc_cloud *cloud, const oc_camera *cam, const oa_options *opt, int threads, double *rgb, double *log_t,
               uint64_t *stop);
 int oa_contributions(const oc_cloud *cloud, const oc_camera *cam, const oa_options *opt, int threads, double *out);
+/* reference.cpp:248-272: per Gaussian, the composited pixels whose mean error is
+ * at least share of the view's worst (target RGBA8, width*height*4 bytes). */
+int oa_marked_pixels(const oc_cloud *cloud, const oc_camera *cam, const uint8_t *target, double share, const oa_options *opt,
+                     int threads, double *out);
 int oa_loss(const double *rgb, const uint8_t *target, int width, int height, double lambda, int threads, double *adjoint,
             double *loss_l1_ssim);
 int oa_gradients(const oc_cloud *cloud, const oc_camera *cam, const double *adjoint, const oa_options *opt, int threads,
@@ -111,12 +115,13 @@ int ob_loss(const float *image, const uint8_t *target, uint32_t width, uint32_t 
  * quantised wave totals. fixed_scale as the product computes it. */
 int ob_backward_fixed(const float *splats, const uint32_t *entry_values, const uint32_t *ranges, uint32_t width,
                       uint32_t height, const float *background, const float *image, const float *adjoint,
-                      const uint32_t *stops, float fixed_scale, uint32_t count, int64_t *raw);
+                      const uint32_t *stops, float fixed_scale, uint32_t count, int64_t *raw,
+                      const float *edge /* a scored view's edge weights, else NULL */);
 /* The same sums in float per wave total, added in a deterministic order; the
  * GPU's float atomics add them in some order, so this is for tolerance checks. */
 int ob_backward_float(const float *splats, const uint32_t *entry_values, const uint32_t *ranges, uint32_t width,
                       uint32_t height, const float *background, const float *image, const float *adjoint,
-                      const uint32_t *stops, uint32_t count, float *raw);
+                      const uint32_t *stops, uint32_t count, float *raw, const float *edge);
 /* Fixed-point sums to floats, as the chain-rule kernel reads them. */
 int ob_fixed_to_float(const int64_t *raw, uint64_t n, float fixed_scale, float *out);
 

@@ -769,6 +769,7 @@ namespace {
 // false when the pixel takes nothing from the splat. B, logBehind advance.
 struct PixelState {
   float logTotal, A[3], B[3], logBehind;
+  float marked, edge; // a scored view: the adjoint's fourth value and the target's edge weight
   std::uint32_t stop;
   bool inside;
 };
@@ -787,6 +788,9 @@ bool backward_pair(const Splat &s, const std::uint32_t box[4], std::uint32_t x, 
   g[7] = Ta * p.A[1];
   g[8] = Ta * p.A[2];
   g[9] = Ta * perPixel;
+  // A scored view's two more sums (train_backward.glsl:130-131): zero otherwise.
+  g[12] = p.marked * perPixel;
+  g[13] = (p.edge * Ta) * perPixel;
   if (s.o * ep < 0.99f) {
     const float dAlpha = T * fma(p.A[2], s.b - p.B[2], fma(p.A[1], s.g - p.B[1], p.A[0] * (s.r - p.B[0])));
     g[0] = dAlpha * ep;
@@ -817,7 +821,8 @@ std::uint32_t wave_y(const TileGeometry &g, std::uint32_t wave, std::uint32_t la
 // receives the wave's 14 totals.
 template <class Emit>
 void backward_waves(const float *splats, const uint32_t *entryValues, const uint32_t *ranges, uint32_t width, uint32_t height,
-                    const float *background, const float *image, const float *adjoint, const uint32_t *stops, Emit &&emit) {
+                    const float *background, const float *image, const float *adjoint, const uint32_t *stops, const float *edge,
+                    Emit &&emit) {
   const std::uint32_t gridW = (width + kTile - 1) / kTile, gridH = (height + kTile - 1) / kTile;
   const float perPixel = 1.0f / (3.0f * static_cast<float>(width) * static_cast<float>(height));
   for (std::uint32_t tile = 0; tile < gridW * gridH; ++tile) {
@@ -840,6 +845,8 @@ void backward_waves(const float *splats, const uint32_t *entryValues, const uint
           p.B[ch] = background[ch];
         }
         p.logBehind = 0.0f;
+        p.marked = p.inside && edge != nullptr ? adjoint[at * 4 + 3] : 0.0f;
+        p.edge = p.inside && edge != nullptr ? edge[at] : 0.0f;
         p.stop = p.inside ? stops[at] : end;
       }
       for (std::uint32_t e = tileStop; e < end; ++e) {
@@ -868,9 +875,9 @@ void backward_waves(const float *splats, const uint32_t *entryValues, const uint
 
 extern "C" int ob_backward_fixed(const float *splats, const uint32_t *entryValues, const uint32_t *ranges, uint32_t width,
                                  uint32_t height, const float *background, const float *image, const float *adjoint,
-                                 const uint32_t *stops, float fixedScale, uint32_t count, int64_t *raw) {
+                                 const uint32_t *stops, float fixedScale, uint32_t count, int64_t *raw, const float *edge) {
   std::fill(raw, raw + static_cast<std::size_t>(count) * kRaw, 0);
-  backward_waves(splats, entryValues, ranges, width, height, background, image, adjoint, stops,
+  backward_waves(splats, entryValues, ranges, width, height, background, image, adjoint, stops, edge,
                  [&](std::uint32_t index, const float totals[kRaw]) {
                    for (int k = 0; k < kRaw; ++k) raw[static_cast<std::size_t>(index) * kRaw + k] += quantise(totals[k], fixedScale);
                  });
@@ -879,9 +886,9 @@ extern "C" int ob_backward_fixed(const float *splats, const uint32_t *entryValue
 
 extern "C" int ob_backward_float(const float *splats, const uint32_t *entryValues, const uint32_t *ranges, uint32_t width,
                                  uint32_t height, const float *background, const float *image, const float *adjoint,
-                                 const uint32_t *stops, uint32_t count, float *raw) {
+                                 const uint32_t *stops, uint32_t count, float *raw, const float *edge) {
   std::fill(raw, raw + static_cast<std::size_t>(count) * kRaw, 0.0f);
-  backward_waves(splats, entryValues, ranges, width, height, background, image, adjoint, stops,
+  backward_waves(splats, entryValues, ranges, width, height, background, image, adjoint, stops, edge,
                  [&](std::uint32_t index, const float totals[kRaw]) {
                    for (int k = 0; k < kRaw; ++k) raw[static_cast<std::size_t>(index) * kRaw + k] += totals[k];
                  });

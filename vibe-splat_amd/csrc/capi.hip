@@ -88,6 +88,20 @@ int vsplat_collect_statistics(vsplat_trainer *t, const vsplat_camera *camera, co
   });
 }
 
+int vsplat_score_view(vsplat_trainer *t, const vsplat_camera *camera, const uint8_t *target, size_t bytes, float error_share,
+                      float pixel_weight, vsplat_step_stats *stats) {
+  if (t == nullptr || camera == nullptr) return invalid("null argument");
+  return guarded([&] {
+    const vsplat_step_stats s = t->impl.score_view(*camera, target, bytes, error_share, pixel_weight);
+    if (stats) *stats = s;
+  });
+}
+
+int vsplat_set_exposure(vsplat_trainer *t, const float *transform) {
+  if (t == nullptr || transform == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.set_exposure(transform); });
+}
+
 int vsplat_apply_gradients(vsplat_trainer *t, const vsplat_learning_rates *rates) {
   if (t == nullptr || rates == nullptr) return invalid("null argument");
   return guarded([&] { t->impl.apply_gradients(*rates); });

@@ -714,6 +714,25 @@ __global__ void advance_adam_kernel(uint32_t *state, float beta1, float beta2, c
   state[2] = as_uint(as_float(state[2]) * beta2);
 }
 
+// After a scored view (train_score.comp:17-26): each Gaussian's marked pixels
+// (raw sum 12, over 3 pixels of the image like the derivatives, times
+// toPixels), that count times the view's loss, and its edge score (raw sum 13
+// likewise) added to the second half of its statistics.
+__global__ __launch_bounds__(256) void score_kernel(const int64_t *__restrict__ rawFixed, const float *__restrict__ rawFloat,
+                                                    float fixedScale, float4 *__restrict__ stats, uint32_t count, float toPixels,
+                                                    float loss) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= count) return;
+  float g[kRawPerSplat];
+  (void)load_raw(rawFixed, rawFloat, fixedScale, i, g);
+  float4 more = stats[static_cast<size_t>(i) * 2 + 1];
+  const float pixels = g[12] * toPixels;
+  more.y = more.y + pixels;
+  more.z = more.z + pixels * loss;
+  more.w = more.w + g[13] * toPixels;
+  stats[static_cast<size_t>(i) * 2 + 1] = more;
+}
+
 } // namespace
 
 // Calls LAUNCH(degree, format) with both as compile-time constants.
@@ -766,6 +785,13 @@ void launch_sh_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewP
   if (params.adamState != nullptr) {
     hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
   }
+}
+
+void launch_score(hipStream_t stream, const int64_t *rawFixed, const float *rawFloat, float fixedScale, float4 *stats, uint32_t count,
+                  float toPixels, float loss) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(score_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, rawFixed, rawFloat, fixedScale, stats, count,
+                     toPixels, loss);
 }
 
 void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,

@@ -110,14 +110,30 @@ void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uin
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
                     uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops);
 // recompute: the transmittance and stops are recomputed per tile rather than
-// read from the forward pass (image and stops may then be null).
+// read from the forward pass (image and stops may then be null). edge: a
+// scored view's edge weights (the adjoint's fourth value its marks), which
+// fill the last two raw sums; null otherwise.
 void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
                      const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
                      const float4 *image, const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed,
-                     float *rawFloat);
+                     float *rawFloat, const float *edge = nullptr);
 
+// errorMax and edge: when scoring, the view's worst error (cleared before)
+// and the target's edge weights, and the adjoint's fourth value the pixel's
+// error; both null otherwise.
 void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
-                 float *fields, float4 *adjoint, float2 *tileTotals, double *sums);
+                 float *fields, float4 *adjoint, float2 *tileTotals, double *sums, uint32_t *errorMax, float *edge);
+
+// A view's affine colour transform (Trainer::set_exposure): rows of (r, g, b, 1).
+struct Exposure {
+  float transform[12];
+};
+// Workgroups of launch_exposure_backward: twelve totals each.
+uint32_t exposure_groups(uint32_t pixels);
+void launch_exposure(hipStream_t stream, const float4 *image, float4 *exposed, uint32_t pixels, const Exposure &t);
+void launch_exposure_backward(hipStream_t stream, const float4 *image, float4 *adjoint, uint32_t pixels, const Exposure &t,
+                              float *groupTotals);
+void launch_error_mask(hipStream_t stream, float4 *adjoint, const uint32_t *errorMax, uint32_t pixels, float share);
 
 // The chain rule over the views of a batch chunk, summed per Gaussian in view
 // order (kFlagAccumulate: continuing from the sums an earlier chunk stored).
@@ -134,5 +150,9 @@ void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewPara
                  uint32_t *counters);
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
                   const uint32_t *counters);
+// A scored view's marked pixels, marked loss and edge score added to each
+// Gaussian's statistics (stats: two float4 per Gaussian) from raw sums 12 and 13.
+void launch_score(hipStream_t stream, const int64_t *rawFixed, const float *rawFloat, float fixedScale, float4 *stats, uint32_t count,
+                  float toPixels, float loss);
 
 } // namespace vsplat

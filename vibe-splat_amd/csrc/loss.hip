@@ -130,9 +130,35 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
   if (threadIdx.x == 0) tileTotals[tile].x = total;
 }
 
+// The target's luminance at a pixel, clamped to the image (train_loss.comp:34-40):
+// 0.299 r + 0.587 g + 0.114 b, added left to right.
+__device__ __forceinline__ float target_luminance(const uchar4 *__restrict__ target, int x, int y, int W, int H) {
+  x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+  y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+  const uchar4 t = target[static_cast<size_t>(y) * W + x];
+  return (unorm8(t.x) * 0.299f + unorm8(t.y) * 0.587f) + unorm8(t.z) * 0.114f;
+}
+
+// The edge weight of the edge-aware score (train_loss.comp:42-51): |Laplacian
+// of the target's luminance| / 4 in [0, 0.999], as differences from the centre
+// so that a flat patch gives exactly zero.
+__device__ __forceinline__ float target_edge(const uchar4 *__restrict__ target, int x, int y, int W, int H) {
+  const float c = target_luminance(target, x, y, W, H);
+  const float laplacian = (((target_luminance(target, x - 1, y, W, H) - c) + (target_luminance(target, x + 1, y, W, H) - c)) +
+                           (target_luminance(target, x, y - 1, W, H) - c)) +
+                          (target_luminance(target, x, y + 1, W, H) - c);
+  const float e = fabsf(laplacian) / 4.0f;
+  return e < 0.0f ? 0.0f : (e > 0.999f ? 0.999f : e);
+}
+
+// When scoring (errorMax not null): the adjoint's fourth value is the pixel's
+// mean error, errorMax the worst over the view in 1/65535 (one atomic per
+// wave; a maximum does not depend on the order), and edge the target's edge
+// weight (train_loss.comp:106-110). Otherwise the fourth value is 0.
 __global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ image, const uchar4 *__restrict__ target, uint32_t width,
                                                    uint32_t height, uint32_t gridW, float lambda, const float *__restrict__ fields,
-                                                   float4 *__restrict__ adjoint, float2 *__restrict__ tileTotals) {
+                                                   float4 *__restrict__ adjoint, float2 *__restrict__ tileTotals,
+                                                   uint32_t *__restrict__ errorMax, float *__restrict__ edge) {
   __shared__ float h[9][kHalfPixels];
   __shared__ float waveTotals[4];
   const int W = static_cast<int>(width), H = static_cast<int>(height);
@@ -160,6 +186,7 @@ __global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ im
 
   const int lx = static_cast<int>(threadIdx.x & 15u), px = x0 + lx, py = y0 + static_cast<int>(threadIdx.x >> 4);
   float own = 0.0f;
+  uint32_t worst = 0;
   if (px < W && py < H) {
     float acc[9];
 #pragma unroll
@@ -187,10 +214,85 @@ __global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ im
       const float inner = fma32(ys[ch], acc[6 + ch], fma32(2.0f * xs[ch], acc[3 + ch], acc[ch]));
       out[ch] = fma32(-lambda, inner, keep * sgn) * perValue;
     }
-    adjoint[at] = make_float4(out[0], out[1], out[2], 0.0f);
+    float error = 0.0f;
+    if (errorMax != nullptr) {
+      error = own / 3.0f;
+      worst = static_cast<uint32_t>((error < 0.0f ? 0.0f : (error > 1.0f ? 1.0f : error)) * 65535.0f);
+      edge[at] = target_edge(target, px, py, W, H);
+    }
+    adjoint[at] = make_float4(out[0], out[1], out[2], error);
+  }
+  if (errorMax != nullptr) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(worst), m, 64));
+      worst = other > worst ? other : worst;
+    }
+    if ((threadIdx.x & 63u) == 0 && worst != 0) atomicMax(errorMax, worst);
   }
   const float total = tile_sum(own, waveTotals);
   if (threadIdx.x == 0) tileTotals[tile].y = total;
+}
+
+// The exposed image (train_exposure.comp:17-24): rgb' = T (rgb, 1), each row's
+// dot product added left to right; the fourth value (log T) copied.
+__global__ __launch_bounds__(256) void exposure_kernel(const float4 *__restrict__ image, float4 *__restrict__ exposed, uint32_t pixels,
+                                                       Exposure t) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= pixels) return;
+  const float4 c = image[i];
+  float out[3];
+#pragma unroll
+  for (int row = 0; row < 3; ++row) {
+    const float *a = t.transform + 4 * row;
+    out[row] = ((a[0] * c.x + a[1] * c.y) + a[2] * c.z) + a[3];
+  }
+  exposed[i] = make_float4(out[0], out[1], out[2], c.w);
+}
+
+// The adjoint taken back through the exposure, in place, and per workgroup of
+// 256 pixels the twelve sums dL/dT[row][column] = g[row] (rgb, 1)[column]
+// (train_exposure_backward.comp:22-40), each summed in the tile_sum tree.
+__global__ __launch_bounds__(256) void exposure_backward_kernel(const float4 *__restrict__ image, float4 *__restrict__ adjoint,
+                                                                uint32_t pixels, Exposure t, float *__restrict__ groupTotals) {
+  __shared__ float waveTotals[4];
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  float g[3] = {0.0f, 0.0f, 0.0f}, rgb1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (i < pixels) {
+    const float4 a = adjoint[i];
+    const float4 c = image[i];
+    g[0] = a.x;
+    g[1] = a.y;
+    g[2] = a.z;
+    rgb1[0] = c.x;
+    rgb1[1] = c.y;
+    rgb1[2] = c.z;
+    rgb1[3] = 1.0f;
+    float back[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) back[k] = (g[0] * t.transform[k] + g[1] * t.transform[4 + k]) + g[2] * t.transform[8 + k];
+    adjoint[i] = make_float4(back[0], back[1], back[2], a.w);
+  }
+#pragma unroll 1
+  for (int row = 0; row < 3; ++row) {
+#pragma unroll 1
+    for (int column = 0; column < 4; ++column) {
+      const float total = tile_sum(g[row] * rgb1[column], waveTotals);
+      if (threadIdx.x == 0) groupTotals[blockIdx.x * 12u + row * 4u + column] = total;
+    }
+  }
+}
+
+// When scoring, the adjoint's fourth value becomes the mark of a high-error
+// pixel (train_error_mask.comp:14-19): 1 where its error is at least `share`
+// of the view's worst, else 0.
+__global__ __launch_bounds__(256) void error_mask_kernel(float4 *__restrict__ adjoint, const uint32_t *__restrict__ errorMax,
+                                                         uint32_t pixels, float share) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= pixels) return;
+  const uint32_t worst = *errorMax;
+  const float limit = (share * static_cast<float>(worst)) / 65535.0f;
+  adjoint[i].w = worst > 0u && adjoint[i].w >= limit ? 1.0f : 0.0f;
 }
 
 // Sums the tiles' totals in double, in a fixed order: thread t adds tiles t,
@@ -221,13 +323,32 @@ __global__ __launch_bounds__(256) void total_kernel(const float2 *__restrict__ t
 } // namespace
 
 void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
-                 float *fields, float4 *adjoint, float2 *tileTotals, double *sums) {
+                 float *fields, float4 *adjoint, float2 *tileTotals, double *sums, uint32_t *errorMax, float *edge) {
   const uint32_t gridW = (width + 15) / 16, tiles = gridW * ((height + 15) / 16);
   const uchar4 *t = reinterpret_cast<const uchar4 *>(target);
   hipLaunchKernelGGL(ssim_kernel, dim3(tiles), dim3(256), 0, stream, image, t, width, height, gridW, fields, tileTotals);
   hipLaunchKernelGGL(loss_kernel, dim3(tiles), dim3(256), 0, stream, image, t, width, height, gridW, lambda, fields, adjoint,
-                     tileTotals);
+                     tileTotals, errorMax, edge);
   hipLaunchKernelGGL(total_kernel, dim3(1), dim3(256), 0, stream, tileTotals, tiles, sums);
+}
+
+uint32_t exposure_groups(uint32_t pixels) { return (pixels + 255) / 256; }
+
+void launch_exposure(hipStream_t stream, const float4 *image, float4 *exposed, uint32_t pixels, const Exposure &t) {
+  if (pixels == 0) return;
+  hipLaunchKernelGGL(exposure_kernel, dim3(exposure_groups(pixels)), dim3(256), 0, stream, image, exposed, pixels, t);
+}
+
+void launch_exposure_backward(hipStream_t stream, const float4 *image, float4 *adjoint, uint32_t pixels, const Exposure &t,
+                              float *groupTotals) {
+  if (pixels == 0) return;
+  hipLaunchKernelGGL(exposure_backward_kernel, dim3(exposure_groups(pixels)), dim3(256), 0, stream, image, adjoint, pixels, t,
+                     groupTotals);
+}
+
+void launch_error_mask(hipStream_t stream, float4 *adjoint, const uint32_t *errorMax, uint32_t pixels, float share) {
+  if (pixels == 0) return;
+  hipLaunchKernelGGL(error_mask_kernel, dim3(exposure_groups(pixels)), dim3(256), 0, stream, adjoint, errorMax, pixels, share);
 }
 
 } // namespace vsplat

@@ -213,6 +213,39 @@ __device__ __forceinline__ void reduce_wave(float (&v)[12], uint32_t lane) {
   }
   v[0] = v[0] + dpp_xor1(v[0]);
 }
+// Sixteen values per lane to one total per lane group, for the scored pass's
+// fourteen sums: afterwards lane l holds the wave's total of value (l >> 2) & 15
+// in v[0]. The same partner tree as reduce_wave.
+__device__ __forceinline__ void reduce_wave16(float (&v)[16], uint32_t lane) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const auto p = __builtin_amdgcn_permlane32_swap(as_uint(v[i]), as_uint(v[i + 8]), false, false);
+    v[i] = as_float(p[0]) + as_float(p[1]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const auto p = __builtin_amdgcn_permlane16_swap(as_uint(v[i]), as_uint(v[i + 4]), false, false);
+    v[i] = as_float(p[0]) + as_float(p[1]);
+  }
+  {
+    const bool upper = (lane & 8u) != 0u;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float send = upper ? v[i] : v[i + 2];
+      const float keep = upper ? v[i + 2] : v[i];
+      v[i] = keep + dpp_row_mirror(send);
+    }
+  }
+  {
+    const bool upper = (lane & 4u) != 0u;
+    const float send = upper ? v[0] : v[1];
+    const float keep = upper ? v[1] : v[0];
+    v[0] = keep + dpp_half_mirror(send);
+  }
+  v[0] = v[0] + dpp_xor2(v[0]);
+  v[0] = v[0] + dpp_xor1(v[0]);
+}
+
 // The value whose total lane l holds after reduce_wave: 6 b5 + 3 b4 + (b1 ? 2 : b2),
 // with bk bit k of l.
 __device__ __forceinline__ uint32_t reduced_value(uint32_t lane) {
@@ -225,13 +258,16 @@ __device__ __forceinline__ bool wave_emits(uint32_t lane) { return (lane & 9u) =
 // Back to front over the tile (train_backward.glsl:78-158). With Recompute the
 // pixel's log transmittance and stop come from a front-to-back sweep of its
 // own instead of the forward pass's image and stops, which are then not read.
-template <bool Fixed, bool Recompute, uint32_t TW, uint32_t TH>
+// With Score (a scored view) the last two sums are filled too: the pixel's
+// high-error mark (the adjoint's fourth value) and its edge weight times the
+// blending weight, each over 3 pixels of the image.
+template <bool Fixed, bool Recompute, bool Score, uint32_t TW, uint32_t TH>
 __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
                                                        const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                                        float bgR, float bgG, float bgB, const float4 *__restrict__ image,
                                                        const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
                                                        float fixedScale, unsigned long long *__restrict__ rawFixed,
-                                                       float *__restrict__ rawFloat) {
+                                                       float *__restrict__ rawFloat, const float *__restrict__ edge) {
   constexpr uint32_t kBatch = TW * TH; // one splat staged per thread
   __shared__ Staged batch[kBatch];
   __shared__ uint32_t batchIndex[kBatch];
@@ -244,6 +280,7 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   const uint32_t at = inside ? y * width + x : 0u;
   const float4 adj = inside ? adjoint[at] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float edgeWeight = Score && inside ? edge[at] : 0.0f;
   const float perPixel = 1.0f / (3.0f * static_cast<float>(width) * static_cast<float>(height));
   const uint2 range = ranges[tile];
   float logTotal;
@@ -297,7 +334,8 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
       const float vx = fma32(s.qy, dy, s.qx * dx), vy = fma32(s.qz, dy, s.qy * dx);
       const float g1 = dPower * vx, g2 = dPower * vy;
       const float hp = 0.5f * dPower;
-      float v[12];
+      constexpr int kValues = Score ? 16 : 12;
+      float v[kValues];
       v[0] = dAlpha * e;
       v[1] = g1;
       v[2] = g2;
@@ -310,16 +348,30 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
       v[9] = Ta * perPixel;
       v[10] = fabsf(g1);
       v[11] = fabsf(g2);
+      if constexpr (Score) {
+        v[12] = ok ? adj.w * perPixel : 0.0f;
+        v[13] = (edgeWeight * Ta) * perPixel;
+        v[14] = v[15] = 0.0f;
+      }
       // The colour and log-transmittance behind: unchanged where alpha is taken as 0.
       const float alphaBehind = ok ? alpha : 0.0f;
       Br = fma32(alphaBehind, s.r - Br, Br);
       Bg = fma32(alphaBehind, s.g - Bg, Bg);
       Bb = fma32(alphaBehind, s.b - Bb, Bb);
       logBehind = logBehind + (ok ? logRest : 0.0f);
-      reduce_wave(v, lane);
-      // Values 12 and 13 of the kRawPerSplat slots are always zero here.
-      const uint32_t value = reduced_value(lane);
-      if (wave_emits(lane)) {
+      uint32_t value;
+      bool emits;
+      if constexpr (Score) {
+        reduce_wave16(v, lane);
+        value = lane >> 2;
+        emits = (lane & 3u) == 0 && value < kRawPerSplat;
+      } else {
+        // Values 12 and 13 of the kRawPerSplat slots are zero without scoring.
+        reduce_wave(v, lane);
+        value = reduced_value(lane);
+        emits = wave_emits(lane);
+      }
+      if (emits) {
         const size_t slot = static_cast<size_t>(batchIndex[k]) * kRawPerSplat + value;
         if constexpr (Fixed) {
           const int32_t q = quantise(v[0], fixedScale);
@@ -357,33 +409,40 @@ void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *ent
 void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
                      const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
                      const float4 *image, const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed,
-                     float *rawFloat) {
+                     float *rawFloat, const float *edge) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
-#define VSPLAT_BACKWARD_SHAPE(F, R, W, H)                                                                                          \
-  hipLaunchKernelGGL((backward_kernel<F, R, W, H>), dim3(gridW * gridH), dim3(W * H), 0, stream, splats, entryValues, ranges,     \
+#define VSPLAT_BACKWARD_SHAPE(F, R, S, W, H)                                                                                       \
+  hipLaunchKernelGGL((backward_kernel<F, R, S, W, H>), dim3(gridW * gridH), dim3(W * H), 0, stream, splats, entryValues, ranges,  \
                      width, height, gridW, background[0], background[1], background[2], image, adjoint, stops, fixedScale,         \
-                     reinterpret_cast<unsigned long long *>(rawFixed), rawFloat)
-#define VSPLAT_BACKWARD_TT(W, H) VSPLAT_BACKWARD_SHAPE(true, true, W, H)
-#define VSPLAT_BACKWARD_TF(W, H) VSPLAT_BACKWARD_SHAPE(true, false, W, H)
-#define VSPLAT_BACKWARD_FT(W, H) VSPLAT_BACKWARD_SHAPE(false, true, W, H)
-#define VSPLAT_BACKWARD_FF(W, H) VSPLAT_BACKWARD_SHAPE(false, false, W, H)
-  if (fixedPoint) {
-    if (recompute) {
-      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TT)
-    } else {
-      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TF)
-    }
-  } else {
-    if (recompute) {
-      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FT)
-    } else {
-      VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FF)
-    }
+                     reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge)
+#define VSPLAT_BACKWARD_TTT(W, H) VSPLAT_BACKWARD_SHAPE(true, true, true, W, H)
+#define VSPLAT_BACKWARD_TTF(W, H) VSPLAT_BACKWARD_SHAPE(true, true, false, W, H)
+#define VSPLAT_BACKWARD_TFT(W, H) VSPLAT_BACKWARD_SHAPE(true, false, true, W, H)
+#define VSPLAT_BACKWARD_TFF(W, H) VSPLAT_BACKWARD_SHAPE(true, false, false, W, H)
+#define VSPLAT_BACKWARD_FTT(W, H) VSPLAT_BACKWARD_SHAPE(false, true, true, W, H)
+#define VSPLAT_BACKWARD_FTF(W, H) VSPLAT_BACKWARD_SHAPE(false, true, false, W, H)
+#define VSPLAT_BACKWARD_FFT(W, H) VSPLAT_BACKWARD_SHAPE(false, false, true, W, H)
+#define VSPLAT_BACKWARD_FFF(W, H) VSPLAT_BACKWARD_SHAPE(false, false, false, W, H)
+  const bool score = edge != nullptr;
+  const int variant = (fixedPoint ? 4 : 0) | (recompute ? 2 : 0) | (score ? 1 : 0);
+  switch (variant) {
+  case 7: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TTT) break;
+  case 6: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TTF) break;
+  case 5: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TFT) break;
+  case 4: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_TFF) break;
+  case 3: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FTT) break;
+  case 2: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FTF) break;
+  case 1: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FFT) break;
+  default: VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_FFF) break;
   }
-#undef VSPLAT_BACKWARD_FF
-#undef VSPLAT_BACKWARD_FT
-#undef VSPLAT_BACKWARD_TF
-#undef VSPLAT_BACKWARD_TT
+#undef VSPLAT_BACKWARD_FFF
+#undef VSPLAT_BACKWARD_FFT
+#undef VSPLAT_BACKWARD_FTF
+#undef VSPLAT_BACKWARD_FTT
+#undef VSPLAT_BACKWARD_TFF
+#undef VSPLAT_BACKWARD_TFT
+#undef VSPLAT_BACKWARD_TTF
+#undef VSPLAT_BACKWARD_TTT
 #undef VSPLAT_BACKWARD_SHAPE
 }
 

@@ -609,8 +609,32 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
                  options_.background, b.image.as<float4>(), b.stops.as<uint32_t>());
   mark("composite");
   if (mode == kModeRender) return;
-  launch_loss(s, b.image.as<float4>(), deviceTarget, v.width, v.height, options_.lambda_ssim, b.fields.as<float>(), b.adjoint.as<float4>(),
-              b.tileTotals.as<float2>(), b.sums.as<double>());
+  // The loss sees the image as this view's exposure makes it (trainer.cpp:397-425).
+  const uint32_t pixels = v.width * v.height;
+  const float4 *seen = b.image.as<float4>();
+  // Their buffers are made on first use; the queued, batched and captured
+  // paths, which must not allocate, refuse exposure and never score.
+  if (exposing_) {
+    b.exposed.ensure(static_cast<size_t>(pixels) * 16);
+    b.exposureTotals.ensure(static_cast<size_t>(exposure_groups(pixels)) * 12 * 4);
+  }
+  if (scoreShare_ > 0.0f) {
+    b.edge.ensure(static_cast<size_t>(pixels) * 4);
+    b.errorMax.ensure(4);
+  }
+  if (exposing_) {
+    launch_exposure(s, b.image.as<float4>(), b.exposed.as<float4>(), pixels, exposure_);
+    seen = b.exposed.as<float4>();
+  }
+  const bool scoring = scoreShare_ > 0.0f;
+  if (scoring) launch_fill_u32(s, b.errorMax.as<uint32_t>(), 1, 0u);
+  launch_loss(s, seen, deviceTarget, v.width, v.height, options_.lambda_ssim, b.fields.as<float>(), b.adjoint.as<float4>(),
+              b.tileTotals.as<float2>(), b.sums.as<double>(), scoring ? b.errorMax.as<uint32_t>() : nullptr,
+              scoring ? b.edge.as<float>() : nullptr);
+  if (exposing_) {
+    launch_exposure_backward(s, b.image.as<float4>(), b.adjoint.as<float4>(), pixels, exposure_, b.exposureTotals.as<float>());
+  }
+  if (scoring) launch_error_mask(s, b.adjoint.as<float4>(), b.errorMax.as<uint32_t>(), pixels, scoreShare_);
   if (fixedPoint_) {
     launch_fill_u32(s, b.rawFixed.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat * 2, 0u);
   } else {
@@ -622,7 +646,7 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   const float fixedScale = fixed_scale(v);
   launch_backward(s, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
                   b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
-                  b.rawFloat.as<float>());
+                  b.rawFloat.as<float>(), scoring ? b.edge.as<float>() : nullptr);
   mark("backward");
   if (!chain) return;
   enqueue_chain(b, v, mode, rates, s, counters);
@@ -800,9 +824,50 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
     grow_entries(b, stats.tile_entries);
   }
   drain_timing();
+  if (exposing_) {
+    // dL/d(transform): the workgroups' totals summed in double in order (trainer.cpp:620-628).
+    const uint32_t groups = exposure_groups(static_cast<uint32_t>(camera.width) * static_cast<uint32_t>(camera.height));
+    std::vector<float> totals(static_cast<size_t>(groups) * 12);
+    check(hipMemcpy(totals.data(), b.exposureTotals.as<float>(), totals.size() * 4, hipMemcpyDeviceToHost), "read exposure totals");
+    double sums[12] = {};
+    for (uint32_t g = 0; g < groups; ++g)
+      for (int k = 0; k < 12; ++k) sums[k] += static_cast<double>(totals[static_cast<size_t>(g) * 12 + k]);
+    for (int k = 0; k < 12; ++k) stats.exposure_gradient[k] = static_cast<float>(sums[k]);
+  }
   if (mode == kModeGradients) haveGradients_ = true;
   lastView_ = v;
   return stats;
+}
+
+// Trainer::score_view (trainer.cpp:519-532): statistics of the view with the
+// high-error pixels marked (errorShare of the view's worst error), then each
+// Gaussian's marked pixels, that count times the view's loss, and its edge
+// score added to its statistics, scaled to pixels times pixelWeight.
+vsplat_step_stats Trainer::score_view(const vsplat_camera &camera, const uint8_t *target, size_t bytes, float errorShare,
+                                      float pixelWeight) {
+  if (!(errorShare > 0.0f)) throw Error(VSPLAT_INVALID_ARGUMENT, "the error share must be above 0");
+  scoreShare_ = errorShare;
+  vsplat_step_stats stats{};
+  try {
+    stats = run(camera, target, bytes, kModeStatistics, nullptr, nullptr, 0);
+  } catch (...) {
+    scoreShare_ = 0.0f;
+    throw;
+  }
+  scoreShare_ = 0.0f;
+  const ViewBuffers &b = *views_[0];
+  const float toPixels = 3.0f * static_cast<float>(camera.width) * static_cast<float>(camera.height) * pixelWeight;
+  launch_score(stream_, fixedPoint_ ? b.rawFixed.as<int64_t>() : nullptr, b.rawFloat.as<float>(), fixed_scale(lastView_),
+               stats_.as<float4>(), count_, toPixels, stats.loss);
+  check(hipStreamSynchronize(stream_), "score");
+  return stats;
+}
+
+void Trainer::set_exposure(const float transform[12]) {
+  static const float neutral[12] = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f};
+  std::memcpy(exposure_.transform, transform, sizeof(exposure_.transform));
+  // A neutral exposure changes nothing, so its passes are left out (trainer.cpp:399-400).
+  exposing_ = std::memcmp(transform, neutral, sizeof(neutral)) != 0;
 }
 
 void Trainer::render(const vsplat_camera &camera, float *out) {
@@ -841,6 +906,7 @@ void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, 
   if (mode != kModeGradients && mode != kModeStep) throw Error(VSPLAT_INVALID_ARGUMENT, "mode must be 1 (gradients) or 2 (step)");
   if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
   if (deviceTarget == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no target");
+  if (exposing_) throw Error(VSPLAT_UNSUPPORTED, "queued steps take the neutral exposure: its gradient is read by the synchronous calls");
   ViewBuffers &b = *views_[0];
   ensure_size(b, camera.width, camera.height);
   dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
@@ -887,6 +953,7 @@ void Trainer::prepare_views(uint32_t n, const vsplat_camera *cameras, const uint
   if (mode != kModeGradients && mode != kModeStep) throw Error(VSPLAT_INVALID_ARGUMENT, "mode must be 1 (gradients) or 2 (step)");
   if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
   if (rates != nullptr && rates->trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
+  if (exposing_) throw Error(VSPLAT_UNSUPPORTED, "a batch of views takes the neutral exposure (one exposure is one photo's)");
   for (uint32_t k = 0; k < n; ++k) {
     if (targets[k] == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no target for a view");
     ensure_size(view_buffers(k), cameras[k].width, cameras[k].height);
@@ -1260,7 +1327,7 @@ uint64_t Trainer::gpu_bytes() const {
   for (size_t k = 0; k < views_.size(); ++k) {
     const ViewBuffers &b = *views_[k];
     for (const DeviceArray *a : {&b.image, &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.entryKeys, &b.entryValues,
-                                 &b.tmpEntryKeys, &b.tmpEntryValues}) {
+                                 &b.tmpEntryKeys, &b.tmpEntryValues, &b.exposed, &b.edge, &b.exposureTotals, &b.errorMax}) {
       total += a->bytes();
     }
     if (k == 0) continue;

@@ -89,6 +89,9 @@ struct ViewBuffers {
   DeviceArray entryKeys, entryValues, tmpEntryKeys, tmpEntryValues;
   // Per pixel and per tile.
   DeviceArray image, stops, adjoint, fields, ranges, tileTotals;
+  // Exposure and scoring (used only when set): the exposed image, the target's
+  // edge weights, the exposure backward's workgroup totals, the worst error.
+  DeviceArray exposed, edge, exposureTotals, errorMax;
   // Scratch and counters.
   DeviceArray sortScratch, scanScratch, counters, sums;
 };
@@ -104,6 +107,8 @@ public:
   vsplat_step_stats run(const vsplat_camera &camera, const uint8_t *target, size_t bytes, uint32_t mode,
                         const vsplat_learning_rates *rates, const uint8_t *next, size_t nextBytes);
   void apply_gradients(const vsplat_learning_rates &rates);
+  vsplat_step_stats score_view(const vsplat_camera &camera, const uint8_t *target, size_t bytes, float errorShare, float pixelWeight);
+  void set_exposure(const float transform[12]);
   void prefetch_target(const uint8_t *target, size_t bytes);
 
   void enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, uint32_t mode, const vsplat_learning_rates *rates);
@@ -193,6 +198,9 @@ private:
   uint32_t shFormat_ = 0; // kShHalf or kShBf16
   bool fixedPoint_ = true;
   float shDropout_ = 0.0f, dropoutNow_ = 0.0f;
+  Exposure exposure_{{1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f}};
+  bool exposing_ = false;     // exposure_ is not the neutral one
+  float scoreShare_ = 0.0f;   // while score_view runs: the error share that marks a pixel
   uint32_t shRetained_ = 0, shSeed_ = 0;
   bool haveGradients_ = false;
   bool statisticsNext_ = true; // the next chain pass adds to the densification statistics

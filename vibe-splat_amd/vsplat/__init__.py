@@ -39,7 +39,7 @@ EXPORTS = [
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
     "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free",
-    "vsplat_autotune_tiles", "vsplat_autotune_views",
+    "vsplat_autotune_tiles", "vsplat_autotune_views", "vsplat_score_view", "vsplat_set_exposure",
 ]
 
 
@@ -155,6 +155,8 @@ def lib():
         L.vsplat_device_free.restype = None
         L.vsplat_autotune_tiles.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
         L.vsplat_autotune_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
+        L.vsplat_score_view.argtypes = [vp, vp, vp, C.c_size_t, C.c_float, C.c_float, vp]
+        L.vsplat_set_exposure.argtypes = [vp, vp]
         _LIB = L
     return _LIB
 
@@ -241,10 +243,11 @@ class StepStats:
     ssim: float
     visible_splats: int
     tile_entries: int
+    exposure_gradient: tuple = (0.0,) * 12
 
     @staticmethod
     def of(c: CStats) -> "StepStats":
-        return StepStats(c.loss, c.l1, c.ssim, c.visible_splats, c.tile_entries)
+        return StepStats(c.loss, c.l1, c.ssim, c.visible_splats, c.tile_entries, tuple(c.exposure_gradient))
 
 
 @dataclass
@@ -326,6 +329,22 @@ class Trainer:
         cam, st = camera_to_c(camera), CStats()
         _raise(self._lib.vsplat_collect_statistics(self._h, C.byref(cam), _p(t), t.nbytes, C.byref(st)), "collect_statistics")
         return StepStats.of(st)
+
+    def score_view(self, camera: Camera, target: np.ndarray, error_share: float, pixel_weight: float = 1.0) -> StepStats:
+        """Trainer::score_view (trainer.cpp:519-532): marked pixels, marked loss
+        and edge score added to the densification statistics."""
+        t = np.ascontiguousarray(target, np.uint8)
+        cam, st = camera_to_c(camera), CStats()
+        _raise(self._lib.vsplat_score_view(self._h, C.byref(cam), _p(t), t.nbytes, error_share, pixel_weight, C.byref(st)),
+               "score_view")
+        return StepStats.of(st)
+
+    def set_exposure(self, transform):
+        """Trainer::set_exposure (trainer.hpp:154): 12 floats, rows of (r, g, b, 1)."""
+        t = np.ascontiguousarray(transform, np.float32).reshape(-1)
+        if t.size != 12:
+            raise ValueError("an exposure is 12 floats")
+        _raise(self._lib.vsplat_set_exposure(self._h, _p(t)), "set_exposure")
 
     def step(self, camera: Camera, target: np.ndarray, rates: LearningRates, next_target: np.ndarray | None = None) -> StepStats:
         t = np.ascontiguousarray(target, np.uint8)
