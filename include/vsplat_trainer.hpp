@@ -106,6 +106,10 @@ struct LearningRates {
   float trustRegion = 0.f;
 };
 
+// Exposure and kNeutralExposure (trainer.hpp:76-77).
+using Exposure = std::array<float, 12>;
+inline constexpr Exposure kNeutralExposure{1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+
 // StepStats (trainer.hpp:63-73).
 struct StepStats {
   float loss = 0.f;
@@ -186,6 +190,17 @@ public:
     detail::check(vsplat_collect_statistics(trainer_, &c, target.data, target.size, &s));
     return from_c(s);
   }
+
+  // Trainer::score_view (trainer.hpp:111).
+  StepStats score_view(const Camera &camera, Bytes target, float errorShare, float pixelWeight = 1.f) {
+    const vsplat_camera c = to_c(camera);
+    vsplat_step_stats s{};
+    detail::check(vsplat_score_view(trainer_, &c, target.data, target.size, errorShare, pixelWeight, &s));
+    return from_c(s);
+  }
+
+  // Trainer::set_exposure (trainer.hpp:154): rows of (r, g, b, 1).
+  void set_exposure(const Exposure &exposure) { detail::check(vsplat_set_exposure(trainer_, exposure.data())); }
 
   void apply_gradients(const LearningRates &rates) {
     const vsplat_learning_rates r = to_c(rates);
