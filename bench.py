@@ -157,6 +157,25 @@ def apply_changes_leg(trainer, n: int) -> dict:
             "host_cloud_ms": round(host_ms, 3), "algorithmic_bytes_device": 2 * m * (148 + 236)}
 
 
+def ply_leg(cloud) -> dict:
+    """3DGS .ply writing and reading of the benchmark cloud on the host
+    (vsplat_write_ply, vsplat_read_ply), through a temporary file."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cloud.ply")
+        t0 = time.perf_counter()
+        vsplat.save_ply(cloud, path)
+        write_ms = (time.perf_counter() - t0) * 1000.0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        loaded = vsplat.load_ply(path)
+        read_ms = (time.perf_counter() - t0) * 1000.0
+    ok = all(np.array_equal(getattr(loaded, k), getattr(cloud, k)) for k in ("positions", "scales", "rotations", "opacities", "sh"))
+    return {"gaussians": cloud.count, "bytes": size, "write_ms": round(write_ms, 1), "read_ms": round(read_ms, 1),
+            "write_gbs": round(size / (write_ms * 1e-3) / 1e9, 3), "read_gbs": round(size / (read_ms * 1e-3) / 1e9, 3),
+            "round_trip_exact": bool(ok)}
+
+
 def copy_rate(nbytes: int = 1 << 30, repeats: int = 20) -> dict:
     """The device's measured copy rate: one GiB copied between two GPU buffers,
     back to back, timed with HIP events (read + write bytes per second)."""
@@ -451,6 +470,9 @@ def main():
             result["other_forms"][key] = form
             log(f"  {form['ms_per_iter']} ms per iteration, {form['entries']} entries, {form['gpu_gb']} GB")
     if not args.no_densify:
+        log("ply: write and read the cloud")
+        result["ply"] = ply_leg(w.cloud)
+        log(f"  write {result['ply']['write_ms']} ms, read {result['ply']['read_ms']} ms for {result['ply']['bytes']} bytes")
         log("apply_changes: 10 % pruned, 10 % copied")
         result["apply_changes"] = apply_changes_leg(trainer, N)
         log(f"  {result['apply_changes']['device_ms']} ms on the device, "

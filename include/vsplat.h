@@ -143,6 +143,20 @@ int vsplat_score_view(vsplat_trainer *trainer, const vsplat_camera *camera, cons
  * The neutral transform (1 0 0 0, 0 1 0 0, 0 0 1 0) skips the passes. The
  * queued and batched calls refuse a non-neutral one (VSPLAT_UNSUPPORTED). */
 int vsplat_set_exposure(vsplat_trainer *trainer, const float *transform);
+/* load_ply (src/splat/gaussian_cloud.hpp:60-63): a 3DGS .ply, binary
+ * little-endian or ASCII; unknown properties such as normals are ignored and
+ * the SH degree follows from the number of f_rest_* properties (0, 9, 24, 45).
+ * vsplat_ply_info reads the header: the vertex count and SH degree, for sizing
+ * the arrays vsplat_read_ply fills in GaussianCloud's layout (count must be the
+ * file's). A file that is not a Gaussian splat file, truncated or malformed
+ * gives VSPLAT_RUNTIME_ERROR with the reference's message. No GPU is used. */
+int vsplat_ply_info(const char *path, uint64_t *count, int32_t *sh_degree);
+int vsplat_read_ply(const char *path, float *positions, float *scales, float *rotations, float *opacities, float *sh, uint64_t count);
+/* save_ply (src/splat/gaussian_cloud.hpp:65-67): binary little-endian in the
+ * standard 3DGS layout (zero normals, f_rest channel-major). No GPU is used. */
+int vsplat_write_ply(const char *path, const vsplat_cloud *cloud);
+/* The trainer's cloud (Trainer::cloud) written as by vsplat_write_ply. */
+int vsplat_save_ply(vsplat_trainer *trainer, const char *path);
 /* Trainer::apply_gradients (trainer.hpp:121): one Adam step from the last compute_gradients. */
 int vsplat_apply_gradients(vsplat_trainer *trainer, const vsplat_learning_rates *rates);
 /* Trainer::step (trainer.hpp:125-126): compute_gradients and the Adam step in one

@@ -204,3 +204,44 @@ def test_prefetched_target_gives_the_same_step():
         ca, cb = a.cloud(), b.cloud()
     for x, y in zip((ca.positions, ca.opacities, ca.sh), (cb.positions, cb.opacities, cb.sh)):
         np.testing.assert_array_equal(_bits(x), _bits(y))
+
+
+def test_save_ply_writes_the_trainers_cloud(tmp_path):
+    """vsplat_save_ply: the trainer's cloud after a step (SH as stored) written
+    as a 3DGS .ply reads back bit for bit."""
+    cloud, cam, target = _case("deg3")
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+        t.step(cam, target, RATES)
+        want = t.cloud()
+        t.save_ply(tmp_path / "trained.ply")
+    got = vsplat.load_ply(tmp_path / "trained.ply")
+    for name in ("positions", "scales", "rotations", "opacities", "sh"):
+        np.testing.assert_array_equal(_bits(getattr(got, name)), _bits(getattr(want, name)))
+
+
+@pytest.mark.parametrize("sh_storage", [0, 1])
+def test_parameter_buffers_hold_the_cloud_in_the_render_layout(sh_storage):
+    """Trainer::parameters (trainer.hpp:137; the viewer reads these buffers in
+    place, viewer.cpp:326-337): device pointers and sizes of the five groups,
+    positions to opacities as floats and the SH packed as half_sh_words
+    (gaussian_cloud.hpp:40-49) or as bf16 words; after a step they hold what
+    cloud() reads."""
+    import ctypes as C
+    cloud, cam, target = _case("deg3")
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, sh_storage=sh_storage)) as t:
+        t.step(cam, target, RATES)
+        c = t.cloud()
+        groups = []
+        for g in range(5):
+            ptr, size = t.parameters(g)
+            out = np.zeros(size // 4, np.uint32)
+            assert vsplat.lib().vsplat_device_read(C.c_void_p(ptr), out.ctypes.data_as(C.c_void_p), out.nbytes) == 0
+            groups.append(out)
+        with pytest.raises(ValueError):
+            t.parameters(5)
+    n = cloud.count
+    for got, want in zip(groups[:4], (c.positions, c.scales, c.rotations, c.opacities)):
+        np.testing.assert_array_equal(got[: want.size], _bits(want).reshape(-1))
+    words = F.pack_sh(c.sh, sh_storage).reshape(-1)
+    assert groups[4].size >= words.size and words.size == n * F.sh_words((cloud.sh_degree + 1) ** 2, sh_storage)
+    np.testing.assert_array_equal(groups[4][: words.size], words)

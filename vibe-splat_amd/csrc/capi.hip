@@ -1,10 +1,12 @@
 // The C entry points of include/vsplat.h: argument checks, then the Trainer,
 // with every exception turned into a status and a per-thread message.
 #include "trainer.hpp"
+#include "ply.hpp"
 
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 struct vsplat_trainer {
   vsplat::Trainer impl;
@@ -94,6 +96,44 @@ int vsplat_score_view(vsplat_trainer *t, const vsplat_camera *camera, const uint
   return guarded([&] {
     const vsplat_step_stats s = t->impl.score_view(*camera, target, bytes, error_share, pixel_weight);
     if (stats) *stats = s;
+  });
+}
+
+int vsplat_ply_info(const char *path, uint64_t *count, int32_t *sh_degree) {
+  if (path == nullptr) return invalid("null argument");
+  return guarded([&] {
+    const vsplat::PlyInfo info = vsplat::ply_info(path);
+    if (count) *count = info.count;
+    if (sh_degree) *sh_degree = info.shDegree;
+  });
+}
+
+int vsplat_read_ply(const char *path, float *positions, float *scales, float *rotations, float *opacities, float *sh, uint64_t count) {
+  if (path == nullptr || positions == nullptr || scales == nullptr || rotations == nullptr || opacities == nullptr || sh == nullptr) {
+    return invalid("null argument");
+  }
+  return guarded([&] {
+    if (vsplat::ply_info(path).count != count) throw vsplat::Error(VSPLAT_INVALID_ARGUMENT, "the arrays are not sized for the file's vertices");
+    vsplat::read_ply(path, vsplat::PlyCloud{positions, scales, rotations, opacities, sh});
+  });
+}
+
+int vsplat_write_ply(const char *path, const vsplat_cloud *cloud) {
+  if (path == nullptr || cloud == nullptr) return invalid("null argument");
+  return guarded([&] {
+    vsplat::write_ply(path, vsplat::ConstPlyCloud{cloud->positions, cloud->scales, cloud->rotations, cloud->opacities, cloud->sh,
+                                                  cloud->count, cloud->sh_degree});
+  });
+}
+
+int vsplat_save_ply(vsplat_trainer *t, const char *path) {
+  if (t == nullptr || path == nullptr) return invalid("null argument");
+  return guarded([&] {
+    const size_t n = t->impl.count(), coeffs = static_cast<size_t>((t->impl.sh_degree() + 1) * (t->impl.sh_degree() + 1));
+    std::vector<float> positions(n * 3), scales(n * 3), rotations(n * 4), opacities(n), sh(n * coeffs * 3);
+    t->impl.cloud(positions.data(), scales.data(), rotations.data(), opacities.data(), sh.data());
+    vsplat::write_ply(path, vsplat::ConstPlyCloud{positions.data(), scales.data(), rotations.data(), opacities.data(), sh.data(), n,
+                                                  t->impl.sh_degree()});
   });
 }
 

@@ -40,6 +40,7 @@ EXPORTS = [
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
     "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free",
     "vsplat_autotune_tiles", "vsplat_autotune_views", "vsplat_score_view", "vsplat_set_exposure",
+    "vsplat_ply_info", "vsplat_read_ply", "vsplat_write_ply", "vsplat_save_ply",
 ]
 
 
@@ -157,6 +158,10 @@ def lib():
         L.vsplat_autotune_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
         L.vsplat_score_view.argtypes = [vp, vp, vp, C.c_size_t, C.c_float, C.c_float, vp]
         L.vsplat_set_exposure.argtypes = [vp, vp]
+        L.vsplat_ply_info.argtypes = [C.c_char_p, vp, vp]
+        L.vsplat_read_ply.argtypes = [C.c_char_p, vp, vp, vp, vp, vp, C.c_uint64]
+        L.vsplat_write_ply.argtypes = [C.c_char_p, vp]
+        L.vsplat_save_ply.argtypes = [vp, C.c_char_p]
         _LIB = L
     return _LIB
 
@@ -338,6 +343,10 @@ class Trainer:
         _raise(self._lib.vsplat_score_view(self._h, C.byref(cam), _p(t), t.nbytes, error_share, pixel_weight, C.byref(st)),
                "score_view")
         return StepStats.of(st)
+
+    def save_ply(self, path):
+        """The trainer's cloud written as a 3DGS .ply."""
+        _raise(self._lib.vsplat_save_ply(self._h, str(path).encode()), "save_ply")
 
     def set_exposure(self, transform):
         """Trainer::set_exposure (trainer.hpp:154): 12 floats, rows of (r, g, b, 1)."""
@@ -543,6 +552,25 @@ class Trainer:
         out = np.zeros(size.value // np.dtype(dtype).itemsize, dtype)
         _raise(self._lib.vsplat_inspect(self._h, name.encode(), _p(out), out.nbytes, C.byref(size)), "inspect")
         return out.reshape(-1, width) if width > 1 else out
+
+
+def load_ply(path) -> Cloud:
+    """load_ply (src/splat/gaussian_cloud.hpp:60-63) through the library."""
+    p = str(path).encode()
+    count, degree = C.c_uint64(), C.c_int32()
+    _raise(lib().vsplat_ply_info(p, C.byref(count), C.byref(degree)), "load_ply")
+    c = Cloud.zeros(count.value, degree.value)
+    _raise(lib().vsplat_read_ply(p, _p(c.positions), _p(c.scales), _p(c.rotations), _p(c.opacities), _p(c.sh), count.value),
+           "load_ply")
+    return c
+
+
+def save_ply(cloud: Cloud, path) -> None:
+    """save_ply (src/splat/gaussian_cloud.hpp:65-67) through the library."""
+    arrays = [np.ascontiguousarray(a, np.float32) for a in
+              (cloud.positions, cloud.scales, cloud.rotations, cloud.opacities, cloud.sh)]
+    cc = CCloud(*[_p(a) for a in arrays], cloud.count, cloud.sh_degree)
+    _raise(lib().vsplat_write_ply(str(path).encode(), C.byref(cc)), "save_ply")
 
 
 class DeviceBuffer:
