@@ -413,7 +413,7 @@ def main():
     traffic = None
     traffic_file = ROOT / "profiles" / "traffic_1M_1280x720_fixed_gradients.json"
     kernel_name = {"forward": f"forward_kernel<{tiles[0]}u, {tiles[1]}u>",
-                   "backward": f"backward_kernel<true, false, {tiles[0]}u, {tiles[1]}u>"}[dominant[1]]
+                   "backward": f"backward_kernel<true, false, false, {tiles[0]}u, {tiles[1]}u>"}[dominant[1]]
     if traffic_file.exists() and fixed and N == 1_000_000 and args.mode == "gradients":
         entry = json.loads(traffic_file.read_text())["kernels"].get(kernel_name)
         traffic = None if entry is None else entry["hbm_bytes_per_launch"]
