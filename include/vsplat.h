@@ -89,7 +89,9 @@ enum { VSPLAT_TILES_16X16 = 0, VSPLAT_TILES_8X8 = 1, VSPLAT_TILES_32X8 = 2, VSPL
 
 enum { VSPLAT_SH_HALF = 0, VSPLAT_SH_BF16 = 1 };
 
-/* LearningRates (src/train/trainer.hpp:51-61). trust_region must be 0 (first-order Adam only). */
+/* LearningRates (src/train/trainer.hpp:51-61). trust_region: once
+ * vsplat_enable_second_order was called, the epsilon of the trust-region steps
+ * (position, scale, opacity and SH; rotation keeps Adam). */
 typedef struct {
   float position, scale, rotation, opacity, sh, sh_rest_scale, trust_region;
 } vsplat_learning_rates;
@@ -137,6 +139,16 @@ int vsplat_collect_statistics(vsplat_trainer *trainer, const vsplat_camera *came
  * pixel_weight. VSPLAT_INVALID_ARGUMENT for error_share <= 0. */
 int vsplat_score_view(vsplat_trainer *trainer, const vsplat_camera *camera, const uint8_t *target, size_t target_bytes,
                       float error_share, float pixel_weight, vsplat_step_stats *stats);
+/* Trainer::enable_second_order and Trainer::update_curvature (trainer.hpp:
+ * 113-118, trainer.cpp:642-667): from enable_second_order on, the states'
+ * second halves hold a running average of diag(J^T J) (rotation keeps Adam),
+ * and vsplat_step / vsplat_enqueue step within the trust region that
+ * rates->trust_region sets (3DGS^2-TR). update_curvature adds one Hutchinson
+ * estimate from the last view's lists and image (VSPLAT_LOGIC_ERROR before any
+ * view). The batched steps and vsplat_apply_gradients then refuse to step
+ * (VSPLAT_UNSUPPORTED). */
+int vsplat_enable_second_order(vsplat_trainer *trainer);
+int vsplat_update_curvature(vsplat_trainer *trainer);
 /* Trainer::set_exposure (trainer.hpp:154): the view's affine colour transform,
  * 12 floats, rows of (r, g, b, 1): the loss compares T (rgb, 1) with the
  * target, and the synchronous calls report dL/dT in stats->exposure_gradient.

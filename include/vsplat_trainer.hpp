@@ -199,6 +199,10 @@ public:
     return from_c(s);
   }
 
+  // Trainer::enable_second_order and Trainer::update_curvature (trainer.hpp:117-118).
+  void enable_second_order() { detail::check(vsplat_enable_second_order(trainer_)); }
+  void update_curvature() { detail::check(vsplat_update_curvature(trainer_)); }
+
   // Trainer::set_exposure (trainer.hpp:154): rows of (r, g, b, 1).
   void set_exposure(const Exposure &exposure) { detail::check(vsplat_set_exposure(trainer_, exposure.data())); }
 

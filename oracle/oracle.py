@@ -73,7 +73,20 @@ class OBChainOptions(C.Structure):
                 ("sh_dropout", C.c_float), ("sh_retained", C.c_uint32), ("sh_seed", C.c_uint32),
                 ("mode", C.c_uint32), ("flags", C.c_uint32), ("rates", C.c_float * 5), ("rest_scale", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float), ("correction1", C.c_float),
-                ("correction2", C.c_float), ("seed", C.c_uint32), ("sh_floats", C.c_uint32)]
+                ("correction2", C.c_float), ("seed", C.c_uint32), ("sh_floats", C.c_uint32), ("trust_region", C.c_float),
+                ("curvature_scale", C.c_float), ("curvature_beta", C.c_float), ("per_value", C.c_float)]
+
+
+CURVATURE_BETA = np.float32(0.999)
+
+
+def curvature_scale(updates: int) -> float:
+    """1 / (1 - beta^k) for the curvature's running average after k updates
+    (trainer.cpp:460-461), beta^k kept in float32 by repeated multiplication."""
+    b = np.float32(1.0)
+    for _ in range(max(updates, 1)):
+        b = np.float32(b * CURVATURE_BETA)
+    return float(np.float32(1.0) / np.float32(np.float32(1.0) - b))
 
 
 def lib():
@@ -581,10 +594,13 @@ class Model:
                "ob_chain")
         return st
 
-    def step(self, r: ModelResult, rates, *, rest_scale=0.05, step=1, selective=True, opacity_reg=0.0, scale_reg=0.0):
-        """Chain rule in step mode, then SH's part of Adam (train_gradient.comp,
-        train_sh_step.comp), from zero optimiser state: the parameters after one
-        Adam step, as (positions, scales, rotations, opacities, sh_words)."""
+    def step(self, r: ModelResult, rates, *, rest_scale=0.05, step=1, selective=True, opacity_reg=0.0, scale_reg=0.0, states=None,
+             second_order=False, trust_region=0.0, curvature_updates=1):
+        """Chain rule in step mode, then SH's part of the step (train_gradient.comp,
+        train_sh_step.comp), from zero optimiser state or from `states` (five
+        uint32 arrays, updated in place): the parameters after Adam's step
+        number `step`, or with second_order the trust-region step of
+        train_step.glsl:44-61, as (positions, scales, rotations, opacities, sh_words)."""
         n = self.n
         c = self.cloud
         pos = np.ascontiguousarray(c.positions, np.float32).reshape(-1).copy()
@@ -594,13 +610,64 @@ class Model:
         words = self.words.copy()
         raw = r.raw.copy()
         st = np.zeros((n, 8), np.float32)
-        states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
-        opt = self.chain_options(2, 4 if selective else 0, opacity_reg, scale_reg, rates=rates, rest_scale=rest_scale, step=step)
+        if states is None:
+            states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
+        flags = (4 if selective else 0) | (16 if second_order else 0)
+        opt = self.chain_options(2, flags, opacity_reg, scale_reg, rates=rates, rest_scale=rest_scale, step=step)
+        opt.trust_region = float(trust_region)
+        opt.curvature_scale = curvature_scale(curvature_updates)
+        opt.curvature_beta = float(CURVATURE_BETA)
         _check(lib().ob_chain(_p(pos), _p(sc), _p(rot), _p(op), _p(words), _p(self.filter3d), C.byref(self.camera), _p(raw),
                               _p(r.splats), _p(st), *[_p(s) for s in states], None, None, None, None, None, C.byref(opt)),
                "ob_chain")
         _check(lib().ob_sh_step(_p(words), _p(states[4]), _p(raw), C.byref(self.camera), C.byref(opt)), "ob_sh_step")
         return pos, sc, rot, op, words
+
+    def curvature(self, r: ModelResult, states, updates: int):
+        """Trainer::update_curvature's chain (train_gradient.comp and
+        train_sh_step.comp in curvature mode): from the raw sums of the
+        backward pass of a Hutchinson adjoint (hutchinson_adjoint), each
+        parameter's squared gradient per image value into the running average
+        in its state's second half; rotations keep Adam. `updates` is the
+        count including this one (the rounding seed)."""
+        c = self.cloud
+        pos = np.ascontiguousarray(c.positions, np.float32).reshape(-1).copy()
+        sc = np.ascontiguousarray(c.scales, np.float32).reshape(-1).copy()
+        rot = np.ascontiguousarray(c.rotations, np.float32).reshape(-1).copy()
+        op = np.ascontiguousarray(c.opacities, np.float32).copy()
+        words = self.words.copy()
+        raw = r.raw.copy()
+        st = np.zeros((self.n, 8), np.float32)
+        opt = self.chain_options(3, 0)
+        opt.seed = int(updates)
+        opt.curvature_beta = float(CURVATURE_BETA)
+        opt.per_value = float(np.float32(1.0) / np.float32(3.0 * self.cam.width * self.cam.height))
+        _check(lib().ob_chain(_p(pos), _p(sc), _p(rot), _p(op), _p(words), _p(self.filter3d), C.byref(self.camera), _p(raw),
+                              _p(r.splats), _p(st), *[_p(s) for s in states], None, None, None, None, None, C.byref(opt)),
+               "ob_chain")
+        _check(lib().ob_sh_step(_p(words), _p(states[4]), _p(raw), C.byref(self.camera), C.byref(opt)), "ob_sh_step")
+        return states
+
+
+def _hash(x):
+    """train_hutchinson.comp:14-21, in uint64 reduced to 32 bits after each product."""
+    x = np.atleast_1d(np.asarray(x, np.uint64)) & np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(16)
+    return x.astype(np.uint32)
+
+
+def hutchinson_adjoint(pixels: int, seed: int) -> np.ndarray:
+    """train_hutchinson.comp:15-29: per pixel and channel a random sign from
+    hash(i ^ hash(seed)), bits 0, 1, 2; the fourth value 0."""
+    bits = _hash(np.arange(pixels, dtype=np.uint32) ^ _hash(seed)[0])
+    out = np.zeros((pixels, 4), np.float32)
+    for ch in range(3):
+        out[:, ch] = np.where((bits >> np.uint32(ch)) & np.uint32(1), np.float32(1.0), np.float32(-1.0))
+    return out
 
 
 def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, selective=True, sh_storage=0, states=None):

@@ -130,11 +130,15 @@ typedef struct {
   int32_t mip_filter_2d;
   float sh_dropout;
   uint32_t sh_retained, sh_seed;
-  uint32_t mode;  /* 0 statistics, 1 gradients, 2 step */
-  uint32_t flags; /* 1 statistics, 4 selective */
+  uint32_t mode;  /* 0 statistics, 1 gradients, 2 step, 3 curvature */
+  uint32_t flags; /* 1 statistics, 4 selective, 16 second order */
   float rates[5];
   float rest_scale, beta1, beta2, epsilon, correction1, correction2;
   uint32_t seed, sh_floats;
+  /* Second order (train_step.glsl:44-67): the trust region's epsilon, the
+   * curvature's bias correction 1 / (1 - beta^k), beta, and (curvature mode)
+   * 1 / the values in the image. */
+  float trust_region, curvature_scale, curvature_beta, per_value;
 } ob_chain_options;
 
 /* Chain rule from the 14 raw float sums per Gaussian. In gradients mode

@@ -353,12 +353,28 @@ std::uint32_t bf16_store(std::uint32_t word, std::uint32_t slot, std::uint32_t b
   return slot == 0 ? (word & 0xFFFF0000u) | bits : (word & 0xFFFFu) | (bits << 16);
 }
 
-// Adam with bf16 moments (train_step.glsl:388-399), first order only.
-void step_parameter(const ob_chain_options &o, float &value, std::uint32_t &state, float g, float rate, std::uint32_t index) {
+// Adam with bf16 moments (train_step.glsl:44-61), or with radius >= 0 the
+// second-order step: the averaged gradient over the curvature (the state's
+// second half), clipped to the radius; only the first moment is stored.
+void step_parameter(const ob_chain_options &o, float &value, std::uint32_t &state, float g, float rate, float radius,
+                    std::uint32_t index) {
   const float m = fma(o.beta1, bf16_load(state, 0), (1.0f - o.beta1) * g);
+  if (radius >= 0.0f) {
+    const float change = (m / o.correction1) / std::max(bf16_load(state, 1) * o.curvature_scale, 1e-30f);
+    value = value - std::min(std::max(change, -radius), radius);
+    state = bf16_store(state, 0, bf16_round(m));
+    return;
+  }
   const float v = fma(o.beta2, bf16_load(state, 1), ((1.0f - o.beta2) * g) * g);
   value = value - (rate * (m / o.correction1)) / (std::sqrt(v / o.correction2) + o.epsilon);
   state = bf16_store(bf16_store(state, 0, bf16_round(m)), 1, bf16_stochastic(v, index, o.seed));
+}
+
+// One estimate of diag(J^T J) into its running average, the state's second
+// half rounded at random (train_step.glsl:63-67).
+std::uint32_t with_curvature(const ob_chain_options &o, std::uint32_t state, float g, std::uint32_t index) {
+  const float c = o.curvature_beta * bf16_load(state, 1) + (((1.0f - o.curvature_beta) * g) * g) * o.per_value;
+  return bf16_store(state, 1, bf16_stochastic(c, index, o.seed));
 }
 
 // Half floats rounded at random (train_step.glsl:409-419).
@@ -371,8 +387,30 @@ std::uint32_t half_stochastic(float x, std::uint32_t index, std::uint32_t seed) 
   return train_uniform(index, seed) < (x - rounded) / gap ? other : nearest;
 }
 
-constexpr std::uint32_t kModeStatistics = 0, kModeGradients = 1, kModeStep = 2;
-constexpr std::uint32_t kFlagStatistics = 1, kFlagSelective = 4;
+constexpr std::uint32_t kModeStatistics = 0, kModeGradients = 1, kModeStep = 2, kModeCurvature = 3;
+constexpr std::uint32_t kFlagStatistics = 1, kFlagSelective = 4, kFlagSecondOrder = 16;
+
+// The trust-region radii of 3DGS^2-TR (train_gradient.comp:131-143): how far
+// a parameter may move before the Gaussian's contribution changes by more than
+// e in squared Hellinger distance; -1 (Adam) without second order. Rotation
+// always takes Adam's step.
+constexpr float kShC0 = 0.28209479177387814f;
+struct Radii {
+  float position, scale, opacity, sh[3];
+};
+Radii radii_of(const ob_chain_options &o, float opacity, const float s2[3], const float dc[3]) {
+  Radii r{-1.0f, -1.0f, -1.0f, {-1.0f, -1.0f, -1.0f}};
+  if ((o.flags & kFlagSecondOrder) == 0) return r;
+  const float alpha = std::max(opacity, 1e-3f), e = o.trust_region;
+  r.position = std::sqrt((((8.0f * ((s2[0] + s2[1]) + s2[2])) / 3.0f) * e) / alpha);
+  r.scale = std::sqrt((2.0f * e) / alpha);
+  r.opacity = std::sqrt((4.0f * alpha) * e) / std::max(alpha * (1.0f - alpha), 1e-3f);
+  for (int c = 0; c < 3; ++c) {
+    const float colour = std::min(std::max(0.5f + kShC0 * dc[c], 0.02f), 1.0f);
+    r.sh[c] = std::sqrt(((4.0f * colour) * e) / alpha) / kShC0;
+  }
+  return r;
+}
 
 } // namespace
 } // namespace oracle
@@ -937,14 +975,23 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
     const float opacity = 1.0f / (1.0f + fexp(-opacities[i]));
     const float s2[3] = {fexp(2.0f * logScale[0]), fexp(2.0f * logScale[1]), fexp(2.0f * logScale[2])};
     const std::uint32_t base = i * c.sh_words, stored = o.sh_floats / 3;
-    // Emitting: gradients stored, or Adam's step (train_gradient.comp:57-72).
-    const auto emit = [&](float *values, std::uint32_t *state, float *grad, std::uint32_t at, float gv, float rate, std::uint32_t group) {
+    const float dc[3] = {sh_value(shWords, base, 0, c.sh_format), sh_value(shWords, base, 1, c.sh_format),
+                         sh_value(shWords, base, 2, c.sh_format)};
+    const Radii radii = radii_of(o, opacity, s2, dc);
+    // Emitting: gradients stored, Adam's or the second-order step, or the
+    // curvature (train_gradient.comp:57-72).
+    const auto emit = [&](float *values, std::uint32_t *state, float *grad, std::uint32_t at, float gv, float rate, float radius,
+                          std::uint32_t group) {
       if (o.mode == kModeGradients) {
         grad[at] = gv;
-      } else if (gv != 0.0f || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0)) {
+      } else if (gv != 0.0f || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0 && radius < 0.0f)) {
+        if (o.mode == kModeCurvature) {
+          state[at] = with_curvature(o, state[at], gv, at * 5u + group);
+          return;
+        }
         float v = values[at];
         std::uint32_t st = state[at];
-        step_parameter(o, v, st, gv, rate, at * 5u + group);
+        step_parameter(o, v, st, gv, rate, radius, at * 5u + group);
         values[at] = v;
         state[at] = st;
       }
@@ -952,11 +999,12 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
     if (!touched) {
       const float op = penalties ? o.opacity_reg * opacity * (1.0f - opacity) : 0.0f;
       for (std::uint32_t a = 0; a < 3; ++a) {
-        emit(positions, statePos, gPos, i * 3 + a, 0.0f, o.rates[0], 0);
-        emit(scales, stateScale, gScale, i * 3 + a, penalties ? o.scale_reg * fexp(logScale[a]) : 0.0f, o.rates[1], 1);
+        emit(positions, statePos, gPos, i * 3 + a, 0.0f, o.rates[0], radii.position, 0);
+        emit(scales, stateScale, gScale, i * 3 + a, penalties ? o.scale_reg * fexp(logScale[a]) : 0.0f, o.rates[1], radii.scale, 1);
       }
-      for (std::uint32_t a = 0; a < 4; ++a) emit(rotations, stateRot, gRot, i * 4 + a, 0.0f, o.rates[2], 2);
-      emit(opacities, stateOpacity, gOpacity, i, op, o.rates[3], 3);
+      if (o.mode != kModeCurvature)
+        for (std::uint32_t a = 0; a < 4; ++a) emit(rotations, stateRot, gRot, i * 4 + a, 0.0f, o.rates[2], -1.0f, 2);
+      emit(opacities, stateOpacity, gOpacity, i, op, o.rates[3], radii.opacity, 3);
       if (o.mode == kModeGradients)
         for (std::uint32_t j = 0; j < o.sh_floats; ++j) gSh[static_cast<std::size_t>(i) * o.sh_floats + j] = 0.0f;
       continue;
@@ -1111,11 +1159,12 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
     }
 
     for (std::uint32_t a = 0; a < 3; ++a) {
-      emit(positions, statePos, gPos, i * 3 + a, gP[a], o.rates[0], 0);
-      emit(scales, stateScale, gScale, i * 3 + a, dScale[a], o.rates[1], 1);
+      emit(positions, statePos, gPos, i * 3 + a, gP[a], o.rates[0], radii.position, 0);
+      emit(scales, stateScale, gScale, i * 3 + a, dScale[a], o.rates[1], radii.scale, 1);
     }
-    for (std::uint32_t a = 0; a < 4; ++a) emit(rotations, stateRot, gRot, i * 4 + a, tangent[a], o.rates[2], 2);
-    emit(opacities, stateOpacity, gOpacity, i, dOpacity, o.rates[3], 3);
+    if (o.mode != kModeCurvature)
+      for (std::uint32_t a = 0; a < 4; ++a) emit(rotations, stateRot, gRot, i * 4 + a, tangent[a], o.rates[2], -1.0f, 2);
+    emit(opacities, stateOpacity, gOpacity, i, dOpacity, o.rates[3], radii.opacity, 3);
     if (o.mode == kModeGradients) {
       for (std::uint32_t k = 0; k < stored; ++k)
         for (std::uint32_t ch = 0; ch < 3; ++ch)
@@ -1128,7 +1177,9 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
       g[3] = dir[0];
       g[4] = dir[1];
       g[5] = dir[2];
-      g[6] = g[7] = g[8] = -1.0f;
+      g[6] = radii.sh[0];
+      g[7] = radii.sh[1];
+      g[8] = radii.sh[2];
       g[9] = bits_to_float(used);
     }
   }
@@ -1173,7 +1224,8 @@ extern "C" int ob_sh_step(uint32_t *shWords, uint32_t *stateSh, const float *raw
     const float *g = raw + static_cast<std::size_t>(i) * kRaw;
     const float gColour[3] = {g[0], g[1], g[2]};
     const std::uint32_t used = float_to_bits(g[9]);
-    const bool zeros = o.mode == kModeStep && (o.flags & kFlagSelective) == 0;
+    const bool secondOrder = (o.flags & kFlagSecondOrder) != 0;
+    const bool zeros = o.mode == kModeStep && (o.flags & kFlagSelective) == 0 && !secondOrder;
     if (used == 0 && !zeros) continue;
     float basis[16];
     sh_basis(g[3], g[4], g[5], c.sh_degree, basis);
@@ -1187,8 +1239,13 @@ extern "C" int ob_sh_step(uint32_t *shWords, uint32_t *stateSh, const float *raw
       if (gv == 0.0f && !zeros) continue;
       const std::uint32_t s = i * o.sh_floats + j;
       std::uint32_t state = stateSh[s];
+      if (o.mode == kModeCurvature) {
+        stateSh[s] = with_curvature(o, state, gv, s * 5u + 4u);
+        continue;
+      }
       float value = sh_word_get(c.sh_format, word, j, first);
-      step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.rest_scale, s * 5u + 4u);
+      step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.rest_scale, secondOrder ? g[6 + ch] : -1.0f,
+                     s * 5u + 4u);
       stateSh[s] = state;
       word = sh_word_put(c.sh_format, word, j, first, value, s, o.seed);
       changed = true;
@@ -1213,7 +1270,7 @@ extern "C" int ob_adam(float *positions, float *scales, float *rotations, float 
       if (gv == 0.0f && !zeros) continue;
       float v = values[at];
       std::uint32_t st = state[at];
-      step_parameter(o, v, st, gv, rate, at * 5u + group);
+      step_parameter(o, v, st, gv, rate, -1.0f, at * 5u + group);
       values[at] = v;
       state[at] = st;
     }
@@ -1234,7 +1291,7 @@ extern "C" int ob_adam(float *positions, float *scales, float *rotations, float 
       if (gv == 0.0f && !zeros) continue;
       std::uint32_t state = stateSh[s];
       float value = sh_word_get(shFormat, word, j, first);
-      step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.rest_scale, s * 5u + 4u);
+      step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.rest_scale, -1.0f, s * 5u + 4u);
       stateSh[s] = state;
       word = sh_word_put(shFormat, word, j, first, value, s, o.seed);
       changed = true;

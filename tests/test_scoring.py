@@ -122,3 +122,21 @@ def test_edge_score_on_and_off_the_edge():
     assert stats[:8, 7].sum() > 0.0
     assert stats[8:, 7].sum() == 0.0
     assert np.all(stats[:, 5] > 0.0)
+
+
+def test_hutchinson_signs_known_answer():
+    """train_hutchinson.comp's hash (a 32-bit integer mixer): known values
+    worked by hand for seed 1, and signs roughly balanced."""
+    def mix(x):
+        x &= 0xFFFFFFFF
+        x ^= x >> 16
+        x = (x * 0x7FEB352D) & 0xFFFFFFFF
+        x ^= x >> 15
+        x = (x * 0x846CA68B) & 0xFFFFFFFF
+        x ^= x >> 16
+        return x
+    adj = O.hutchinson_adjoint(4096, 1)
+    for i in (0, 1, 777, 4095):
+        bits = mix(i ^ mix(1))
+        assert list(adj[i, :3]) == [1.0 if bits & (1 << c) else -1.0 for c in range(3)]
+    assert abs(adj[:, :3].mean()) < 0.05 and not np.any(adj[:, 3])

@@ -137,6 +137,16 @@ int vsplat_save_ply(vsplat_trainer *t, const char *path) {
   });
 }
 
+int vsplat_enable_second_order(vsplat_trainer *t) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.enable_second_order(); });
+}
+
+int vsplat_update_curvature(vsplat_trainer *t) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.update_curvature(); });
+}
+
 int vsplat_set_exposure(vsplat_trainer *t, const float *transform) {
   if (t == nullptr || transform == nullptr) return invalid("null argument");
   return guarded([&] { t->impl.set_exposure(transform); });

@@ -29,13 +29,55 @@ __device__ __forceinline__ uint32_t bf16_store(uint32_t word, uint32_t slot, uin
   return slot == 0 ? (word & 0xFFFF0000u) | bits : (word & 0xFFFFu) | (bits << 16);
 }
 
-// Adam (train_step.glsl:388-399, first order): the step uses both moments
-// before they are rounded; m is rounded to nearest, v at random.
-__device__ __forceinline__ void step_parameter(const ChainParams &o, float &value, uint32_t &state, float g, float rate, uint32_t index) {
+// Adam (train_step.glsl:44-61): the step uses both moments before they are
+// rounded; m is rounded to nearest, v at random. With radius >= 0 the
+// second-order step of 3DGS^2-TR instead: the averaged gradient over the
+// curvature (the state's second half), clipped to the radius; only m is stored.
+__device__ __forceinline__ void step_parameter(const ChainParams &o, float &value, uint32_t &state, float g, float rate, float radius,
+                                               uint32_t index) {
   const float m = fma32(o.beta1, bf16_load(state, 0), (1.0f - o.beta1) * g);
+  if (radius >= 0.0f) {
+    const float curvature = bf16_load(state, 1) * o.curvatureScale;
+    const float change = (m / o.correction1) / (curvature > 1e-30f ? curvature : 1e-30f);
+    value = value - (change < -radius ? -radius : (change > radius ? radius : change));
+    state = bf16_store(state, 0, bf16_round(m));
+    return;
+  }
   const float v = fma32(o.beta2, bf16_load(state, 1), ((1.0f - o.beta2) * g) * g);
   value = value - (rate * (m / o.correction1)) / (sqrtf(v / o.correction2) + o.epsilon);
   state = bf16_store(bf16_store(state, 0, bf16_round(m)), 1, bf16_stochastic(v, index, o.seed));
+}
+
+// One estimate of diag(J^T J) into its running average in the state's second
+// half, rounded at random (train_step.glsl:63-67).
+__device__ __forceinline__ uint32_t with_curvature(const ChainParams &o, uint32_t state, float g, uint32_t index) {
+  const float c = o.curvatureBeta * bf16_load(state, 1) + (((1.0f - o.curvatureBeta) * g) * g) * o.perValue;
+  return bf16_store(state, 1, bf16_stochastic(c, index, o.seed));
+}
+
+// The trust-region radii of one Gaussian (train_gradient.comp:131-143), from
+// its parameters before the step: how far each may move before its
+// contribution changes by more than epsilon in squared Hellinger distance;
+// -1 (Adam's step) without second order. Rotation always takes Adam's step.
+struct Radii {
+  float position, scale, opacity;
+};
+__device__ __forceinline__ Radii radii_of(const ChainParams &o, const float *opacities, const float *scales, uint32_t i) {
+  if ((o.flags & kFlagSecondOrder) == 0) return Radii{-1.0f, -1.0f, -1.0f};
+  const float opacity = 1.0f / (1.0f + vs_exp(-opacities[i]));
+  const float alpha = opacity > 1e-3f ? opacity : 1e-3f, e = o.trustRegion;
+  const float s2[3] = {vs_exp(2.0f * scales[i * 3]), vs_exp(2.0f * scales[i * 3 + 1]), vs_exp(2.0f * scales[i * 3 + 2])};
+  const float spread = alpha * (1.0f - alpha);
+  return Radii{sqrtf((((8.0f * ((s2[0] + s2[1]) + s2[2])) / 3.0f) * e) / alpha), sqrtf((2.0f * e) / alpha),
+               sqrtf((4.0f * alpha) * e) / (spread > 1e-3f ? spread : 1e-3f)};
+}
+// The SH coefficients' radius for colour channel ch, from its DC coefficient.
+__device__ __forceinline__ float sh_radius(const ChainParams &o, float opacity, float dc) {
+  const float alpha = opacity > 1e-3f ? opacity : 1e-3f;
+  const float shC0 = 0.28209479177387814f;
+  const float raw = 0.5f + shC0 * dc;
+  const float colour = raw < 0.02f ? 0.02f : (raw > 1.0f ? 1.0f : raw);
+  return sqrtf(((4.0f * colour) * o.trustRegion) / alpha) / shC0;
 }
 
 // The step about to be taken, from the device state when there is one.
@@ -57,15 +99,21 @@ __device__ __forceinline__ uint32_t half_stochastic(float x, uint32_t index, uin
   return train_uniform(index, seed) < (x - rounded) / gap ? other : nearest;
 }
 
-// What becomes of gradient g of parameter `at` (train_gradient.comp:57-72).
+// What becomes of gradient g of parameter `at` (train_gradient.comp:57-72):
+// stored, a step taken (Adam's, or within `radius` when >= 0), or (curvature
+// mode) its square added to the curvature.
 __device__ __forceinline__ void emit(const ChainParams &o, float *values, uint32_t *state, float *grad, uint32_t at, float g, float rate,
-                                     uint32_t group) {
+                                     float radius, uint32_t group) {
   if (o.mode == kModeGradients) {
     grad[at] = (o.flags & kFlagAccumulate) != 0 ? grad[at] + g : g;
-  } else if (g != 0.0f || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0)) {
+  } else if (g != 0.0f || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0 && radius < 0.0f)) {
+    if (o.mode == kModeCurvature) {
+      state[at] = with_curvature(o, state[at], g, at * 5u + group);
+      return;
+    }
     float v = values[at];
     uint32_t s = state[at];
-    step_parameter(o, v, s, g, rate, at * 5u + group);
+    step_parameter(o, v, s, g, rate, radius, at * 5u + group);
     values[at] = v;
     state[at] = s;
   }
@@ -368,13 +416,15 @@ __device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewPara
   view_geometry_terms(B, c, o, i, g, dirTerm, Degree > 0, t);
 }
 
-__device__ __forceinline__ void emit_terms(const ChainBuffers &B, const ChainParams &o, uint32_t i, const ViewTerms &t) {
+__device__ __forceinline__ void emit_terms(const ChainBuffers &B, const ChainParams &o, uint32_t i, const ViewTerms &t, const Radii &r) {
   for (uint32_t a = 0; a < 3; ++a) {
-    emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, t.position[a], o.rates[0], 0);
-    emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, t.scale[a], o.rates[1], 1);
+    emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, t.position[a], o.rates[0], r.position, 0);
+    emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, t.scale[a], o.rates[1], r.scale, 1);
   }
-  for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, t.rotation[a], o.rates[2], 2);
-  emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, t.opacity, o.rates[3], 3);
+  if (o.mode != kModeCurvature) {
+    for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, t.rotation[a], o.rates[2], -1.0f, 2);
+  }
+  emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, t.opacity, o.rates[3], r.opacity, 3);
 }
 
 constexpr uint32_t kChainThreads = 256;
@@ -420,7 +470,7 @@ __global__ __launch_bounds__(kChainThreads) void chain_sh_kernel(ChainBuffers B,
   if (o.mode == kModeStatistics || !touched) {
     // Nothing from the pixels: no SH step, and zero SH gradients where the
     // single kernel wrote them (gradients mode, or penalties in step mode).
-    if (o.mode == kModeStep) shPass[9] = as_float(0u);
+    if (o.mode == kModeStep || o.mode == kModeCurvature) shPass[9] = as_float(0u);
     if (o.mode == kModeGradients && (zeros || penalties)) {
       store_sh_row(B.gradSh + static_cast<size_t>(i) * o.shFloats, o.shFloats, (o.flags & kFlagAccumulate) != 0,
                    [](uint32_t) { return 0.0f; });
@@ -445,6 +495,12 @@ __global__ __launch_bounds__(kChainThreads) void chain_sh_kernel(ChainBuffers B,
       shPass[3] = dir[0];
       shPass[4] = dir[1];
       shPass[5] = dir[2];
+      if ((o.flags & kFlagSecondOrder) != 0) {
+        const float opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
+        const uint32_t *words = B.shWords + static_cast<size_t>(i) * c.shWords;
+#pragma unroll
+        for (uint32_t ch = 0; ch < 3; ++ch) shPass[6 + ch] = sh_radius(o, opacity, sh_value<Format>(words, 0, ch));
+      }
       shPass[9] = as_float(used);
     }
   });
@@ -479,7 +535,7 @@ __global__ __launch_bounds__(kChainThreads) void chain_geometry_kernel(ChainBuff
     view_geometry_terms(B, c, o, i, g, dirTerm, c.shDegree > 0, t);
   }
   // Everything above read the parameters before any of them changes.
-  emit_terms(B, o, i, t);
+  emit_terms(B, o, i, t, radii_of(o, B.opacities, B.scales, i));
 }
 
 // The batched chain rule (no reference counterpart): for each Gaussian, every
@@ -585,7 +641,7 @@ __global__ __launch_bounds__(kChainThreads) void chain_views_kernel(ChainBuffers
 #pragma unroll
   for (uint32_t j = 0; j < kCoefficients * 3; ++j) out[j] = shSum[j][lane];
   for (uint32_t j = kCoefficients * 3; j < o.shFloats; ++j) out[j] = 0.0f;
-  emit_terms(B, o, i, sum);
+  emit_terms(B, o, i, sum, Radii{-1.0f, -1.0f, -1.0f}); // a batch takes Adam's step
 }
 
 // The SH floats packed word q holds: [first, last].
@@ -626,11 +682,12 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
   const uint32_t words = c.shWords;
   if (w >= c.count * words) return;
   if (counters[kOverflow] != 0) return;
-  adam_step_now(o);
+  if (o.mode == kModeStep) adam_step_now(o);
   const uint32_t i = w / words, q = w - i * words;
   const float *g = shPass + static_cast<size_t>(i) * kShPassFloats;
   const uint32_t used = as_uint(g[9]);
-  const bool zeros = o.mode == kModeStep && (o.flags & kFlagSelective) == 0;
+  const bool secondOrder = (o.flags & kFlagSecondOrder) != 0;
+  const bool zeros = o.mode == kModeStep && (o.flags & kFlagSelective) == 0 && !secondOrder;
   if (used == 0 && !zeros) return;
   float basis[16];
   sh_basis(g[3], g[4], g[5], c.shDegree, basis);
@@ -644,8 +701,12 @@ __global__ __launch_bounds__(256) void sh_step_kernel(uint32_t *__restrict__ shW
     if (gv == 0.0f && !zeros) continue;
     const uint32_t s = i * o.shFloats + j;
     uint32_t state = stateSh[s];
+    if (o.mode == kModeCurvature) {
+      stateSh[s] = with_curvature(o, state, gv, s * 5u + 4u);
+      continue;
+    }
     float value = sh_word_get(c.shFormat, word, j, first);
-    step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.restScale, s * 5u + 4u);
+    step_parameter(o, value, state, gv, k == 0 ? o.rates[4] : o.rates[4] * o.restScale, secondOrder ? g[6 + ch] : -1.0f, s * 5u + 4u);
     stateSh[s] = state;
     word = sh_word_put(c.shFormat, word, j, first, value, s, o.seed);
     changed = true;
@@ -666,13 +727,13 @@ __global__ __launch_bounds__(256) void adam_kernel(ChainBuffers B, ViewParams c,
   adam_step_now(o);
   const uint32_t n = c.count;
   if (t < 3 * n) {
-    emit(o, B.positions, B.statePos, nullptr, t, B.gradPos[t], o.rates[0], 0);
+    emit(o, B.positions, B.statePos, nullptr, t, B.gradPos[t], o.rates[0], -1.0f, 0);
   } else if (t < 6 * n) {
-    emit(o, B.scales, B.stateScale, nullptr, t - 3 * n, B.gradScale[t - 3 * n], o.rates[1], 1);
+    emit(o, B.scales, B.stateScale, nullptr, t - 3 * n, B.gradScale[t - 3 * n], o.rates[1], -1.0f, 1);
   } else if (t < 10 * n) {
-    emit(o, B.rotations, B.stateRot, nullptr, t - 6 * n, B.gradRot[t - 6 * n], o.rates[2], 2);
+    emit(o, B.rotations, B.stateRot, nullptr, t - 6 * n, B.gradRot[t - 6 * n], o.rates[2], -1.0f, 2);
   } else if (t < 11 * n) {
-    emit(o, B.opacities, B.stateOpacity, nullptr, t - 10 * n, B.gradOpacity[t - 10 * n], o.rates[3], 3);
+    emit(o, B.opacities, B.stateOpacity, nullptr, t - 10 * n, B.gradOpacity[t - 10 * n], o.rates[3], -1.0f, 3);
   }
 }
 
@@ -698,7 +759,7 @@ __global__ __launch_bounds__(256) void sh_adam_kernel(uint32_t *__restrict__ shW
     if (gv == 0.0f && !zeros) continue;
     uint32_t state = stateSh[s];
     float value = sh_word_get(c.shFormat, word, j, first);
-    step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.restScale, s * 5u + 4u);
+    step_parameter(o, value, state, gv, j < 3 ? o.rates[4] : o.rates[4] * o.restScale, -1.0f, s * 5u + 4u);
     stateSh[s] = state;
     word = sh_word_put(c.shFormat, word, j, first, value, s, o.seed);
     changed = true;
@@ -733,6 +794,31 @@ __global__ __launch_bounds__(256) void score_kernel(const int64_t *__restrict__ 
   stats[static_cast<size_t>(i) * 2 + 1] = more;
 }
 
+// Second-order training: an adjoint of random signs, one per pixel and
+// channel (train_hutchinson.comp:15-29). Taken through the backward pass it
+// gives J^T u, whose square has the diagonal of J^T J as its expectation
+// (Hutchinson's estimator in image space).
+__device__ __forceinline__ uint32_t mix_bits(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+__global__ __launch_bounds__(256) void hutchinson_kernel(float4 *__restrict__ adjoint, uint32_t pixels, uint32_t seed) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= pixels) return;
+  const uint32_t bits = mix_bits(i ^ mix_bits(seed));
+  adjoint[i] = make_float4((bits & 1u) ? 1.0f : -1.0f, (bits & 2u) ? 1.0f : -1.0f, (bits & 4u) ? 1.0f : -1.0f, 0.0f);
+}
+
+// Trainer::enable_second_order (trainer.cpp:642-654): the curvature takes the
+// place of the second moment, from zero; the first moment is kept.
+__global__ __launch_bounds__(256) void keep_first_moment_kernel(uint32_t *__restrict__ state, uint64_t count) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += static_cast<uint64_t>(gridDim.x) * 256u) state[i] &= 0xFFFFu;
+}
+
 } // namespace
 
 // Calls LAUNCH(degree, format) with both as compile-time constants.
@@ -756,11 +842,11 @@ void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewPar
   VSPLAT_SH_DISPATCH(view.shDegree, view.shFormat, VSPLAT_CHAIN)
 #undef VSPLAT_CHAIN
   hipLaunchKernelGGL(chain_geometry_kernel, grid, block, 0, stream, buffers, view, params, counters);
-  if (params.mode == kModeStep) {
+  if (params.mode == kModeStep || params.mode == kModeCurvature) {
     const uint64_t words = static_cast<uint64_t>(view.count) * view.shWords;
     hipLaunchKernelGGL(sh_step_kernel, dim3(static_cast<uint32_t>((words + 255) / 256)), dim3(256), 0, stream, buffers.shWords,
                        buffers.stateSh, buffers.shPass, view, params, counters);
-    if (params.adamState != nullptr) {
+    if (params.mode == kModeStep && params.adamState != nullptr) {
       hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
     }
   }
@@ -785,6 +871,18 @@ void launch_sh_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewP
   if (params.adamState != nullptr) {
     hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
   }
+}
+
+void launch_hutchinson(hipStream_t stream, float4 *adjoint, uint32_t pixels, uint32_t seed) {
+  if (pixels == 0) return;
+  hipLaunchKernelGGL(hutchinson_kernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, adjoint, pixels, seed);
+}
+
+void launch_keep_first_moment(hipStream_t stream, uint32_t *state, uint64_t count) {
+  if (count == 0) return;
+  const uint64_t blocks64 = (count + 255) / 256;
+  hipLaunchKernelGGL(keep_first_moment_kernel, dim3(blocks64 < 8192 ? static_cast<uint32_t>(blocks64) : 8192u), dim3(256), 0, stream, state,
+                     count);
 }
 
 void launch_score(hipStream_t stream, const int64_t *rawFixed, const float *rawFloat, float fixedScale, float4 *stats, uint32_t count,

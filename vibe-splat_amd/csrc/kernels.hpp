@@ -13,9 +13,11 @@ namespace vsplat {
 
 // What the chain-rule pass does with the gradients (train_step.glsl:346-353).
 constexpr uint32_t kModeStatistics = 0, kModeGradients = 1, kModeStep = 2;
+constexpr uint32_t kModeCurvature = 3; // the curvature's running average (Trainer::update_curvature)
 constexpr uint32_t kModeRender = 0xFFFFFFFFu; // forward only (Trainer::render)
 constexpr uint32_t kFlagStatistics = 1, kFlagSelective = 4;
 constexpr uint32_t kFlagAccumulate = 8; // gradients mode: add to the gradient buffers instead of writing them
+constexpr uint32_t kFlagSecondOrder = 16; // step mode: the trust-region step, curvature in the states' second halves
 constexpr uint32_t kShPassFloats = 10; // per Gaussian, what the SH step needs from the chain rule
 
 struct ChainParams {
@@ -35,6 +37,10 @@ struct ChainParams {
   // rather than from correction1, correction2 and seed, so that a captured
   // graph can replay steps.
   uint32_t *adamState;
+  // Second order (kFlagSecondOrder, kModeCurvature): the trust region's
+  // epsilon, the curvature's bias correction 1 / (1 - beta^k), beta, and in
+  // curvature mode 1 / the values in the image (seed: the update count).
+  float trustRegion, curvatureScale, curvatureBeta, perValue;
 };
 
 struct ChainBuffers {
@@ -150,6 +156,10 @@ void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewPara
                  uint32_t *counters);
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
                   const uint32_t *counters);
+// Second order: the Hutchinson adjoint of random signs (update number seed),
+// and the states' second halves cleared (enable_second_order).
+void launch_hutchinson(hipStream_t stream, float4 *adjoint, uint32_t pixels, uint32_t seed);
+void launch_keep_first_moment(hipStream_t stream, uint32_t *state, uint64_t count);
 // A scored view's marked pixels, marked loss and edge score added to each
 // Gaussian's statistics (stats: two float4 per Gaussian) from raw sums 12 and 13.
 void launch_score(hipStream_t stream, const int64_t *rawFixed, const float *rawFloat, float fixedScale, float4 *stats, uint32_t count,

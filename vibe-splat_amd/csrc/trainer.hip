@@ -663,7 +663,19 @@ ChainParams Trainer::chain_params(const ViewParams &v, uint32_t mode, const vspl
   p.shSeed = shSeed_;
   p.mode = mode;
   p.flags = (options_.selective_adam ? kFlagSelective : 0u) | (statisticsNext_ ? kFlagStatistics : 0u) |
-            (accumulateNext_ ? kFlagAccumulate : 0u);
+            (accumulateNext_ ? kFlagAccumulate : 0u) | (secondOrder_ ? kFlagSecondOrder : 0u);
+  // Second order (trainer.cpp:455-474): the trust region, and the curvature's
+  // bias correction 1 / (1 - beta^k), beta^k kept in float by repeated
+  // multiplication over k = max(updates, 1).
+  p.curvatureBeta = 0.999f;
+  float betaPower = 1.0f;
+  for (uint32_t k = 0; k < std::max(curvatureUpdates_, 1u); ++k) betaPower = betaPower * p.curvatureBeta;
+  p.curvatureScale = 1.0f / (1.0f - betaPower);
+  p.trustRegion = rates != nullptr ? rates->trust_region : 0.0f;
+  if (mode == kModeCurvature) {
+    p.seed = curvatureUpdates_;
+    p.perValue = 1.0f / (3.0f * static_cast<float>(v.width) * static_cast<float>(v.height));
+  }
   p.beta1 = 0.9f;
   p.beta2 = 0.999f;
   p.epsilon = 1e-15f; // trainer.cpp:23
@@ -804,7 +816,6 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
   const size_t expected = static_cast<size_t>(b.width) * static_cast<size_t>(b.height) * 4;
   if (target == nullptr || bytes != expected) throw Error(VSPLAT_RUNTIME_ERROR, "the target image does not match the camera");
   if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
-  if (rates != nullptr && rates->trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
   // Dropout trains; scoring and statistics see the whole model (trainer.cpp:569).
   dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
   stage_target(target, bytes);
@@ -890,7 +901,7 @@ void Trainer::render(const vsplat_camera &camera, float *out) {
 
 void Trainer::apply_gradients(const vsplat_learning_rates &rates) {
   if (!haveGradients_) throw Error(VSPLAT_LOGIC_ERROR, "no gradients: compute_gradients() comes first");
-  if (rates.trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
+  if (secondOrder_) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are taken by step(), not from stored gradients");
   dropoutNow_ = 0.0f;
   statisticsNext_ = false;
   begin_timing(false);
@@ -952,7 +963,7 @@ void Trainer::prepare_views(uint32_t n, const vsplat_camera *cameras, const uint
   if (n == 0 || cameras == nullptr || targets == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a batch needs views and targets");
   if (mode != kModeGradients && mode != kModeStep) throw Error(VSPLAT_INVALID_ARGUMENT, "mode must be 1 (gradients) or 2 (step)");
   if (mode == kModeStep && rates == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "a step needs learning rates");
-  if (rates != nullptr && rates->trust_region != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "second-order steps are not supported by this build");
+  if (secondOrder_ && mode == kModeStep) throw Error(VSPLAT_UNSUPPORTED, "a batch of views takes Adam's step, not the second-order one");
   if (exposing_) throw Error(VSPLAT_UNSUPPORTED, "a batch of views takes the neutral exposure (one exposure is one photo's)");
   for (uint32_t k = 0; k < n; ++k) {
     if (targets[k] == nullptr) throw Error(VSPLAT_INVALID_ARGUMENT, "no target for a view");
@@ -1281,6 +1292,48 @@ void Trainer::apply_changes(const vsplat_cloud *cloud, const int64_t *source, ui
   allocate_per_gaussian();
   view_buffers(0);
   check(hipStreamSynchronize(stream_), "per-Gaussian buffers");
+}
+
+// Trainer::enable_second_order (trainer.cpp:642-654): from now on the states'
+// second halves hold the curvature, from zero (rotations keep Adam's moments).
+void Trainer::enable_second_order() {
+  if (secondOrder_) return;
+  release_captured();
+  const size_t n = count_;
+  const size_t floats[5] = {n * 3, n * 3, n * 4, n, n * shFloats_};
+  for (int g = 0; g < 5; ++g) {
+    if (g == 2) continue; // rotation stays with Adam
+    launch_keep_first_moment(stream_, state_[g].as<uint32_t>(), floats[g]);
+  }
+  check(hipStreamSynchronize(stream_), "enable second order");
+  secondOrder_ = true;
+  curvatureUpdates_ = 0;
+}
+
+// Trainer::update_curvature (trainer.cpp:656-667): a Hutchinson adjoint through
+// the backward pass of the last view's lists and image, then the chain rule
+// adding each parameter's squared gradient to its curvature.
+void Trainer::update_curvature() {
+  ViewBuffers &b = *views_[0];
+  if (b.width == 0 || lastView_.width == 0) throw Error(VSPLAT_LOGIC_ERROR, "update_curvature() needs a view: step() or compute_gradients() first");
+  ++curvatureUpdates_;
+  const ViewParams &v = lastView_;
+  const uint32_t pixels = v.width * v.height;
+  uint32_t *counters = b.counters.as<uint32_t>();
+  launch_hutchinson(stream_, b.adjoint.as<float4>(), pixels, curvatureUpdates_);
+  if (fixedPoint_) {
+    launch_fill_u32(stream_, b.rawFixed.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat * 2, 0u);
+  } else {
+    launch_fill_u32(stream_, b.rawFloat.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat, 0u);
+  }
+  launch_backward(stream_, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
+                  b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
+                  b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixed_scale(v), b.rawFixed.as<int64_t>(), b.rawFloat.as<float>());
+  const bool statistics = statisticsNext_;
+  statisticsNext_ = false; // the curvature adds nothing to the densification statistics
+  enqueue_chain(b, v, kModeCurvature, nullptr, stream_, counters);
+  statisticsNext_ = statistics;
+  check(hipStreamSynchronize(stream_), "update curvature");
 }
 
 void Trainer::reset_densify_stats() {

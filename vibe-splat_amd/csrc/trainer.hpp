@@ -139,6 +139,9 @@ public:
   void densify_stats(float *out) const;
   void reset_densify_stats();
   void apply_changes(const vsplat_cloud *cloud, const int64_t *source, uint32_t count);
+  void enable_second_order();
+  void update_curvature();
+  bool second_order() const { return secondOrder_; }
   void set_active_sh_degree(int degree);
   void set_filter3d(const float *filter, size_t count);
   void set_sh_dropout(float probability, int retained, uint32_t seed);
@@ -201,6 +204,8 @@ private:
   Exposure exposure_{{1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f}};
   bool exposing_ = false;     // exposure_ is not the neutral one
   float scoreShare_ = 0.0f;   // while score_view runs: the error share that marks a pixel
+  bool secondOrder_ = false;  // enable_second_order() was called
+  uint32_t curvatureUpdates_ = 0;
   uint32_t shRetained_ = 0, shSeed_ = 0;
   bool haveGradients_ = false;
   bool statisticsNext_ = true; // the next chain pass adds to the densification statistics

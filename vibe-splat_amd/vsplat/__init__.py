@@ -41,6 +41,7 @@ EXPORTS = [
     "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free",
     "vsplat_autotune_tiles", "vsplat_autotune_views", "vsplat_score_view", "vsplat_set_exposure",
     "vsplat_ply_info", "vsplat_read_ply", "vsplat_write_ply", "vsplat_save_ply",
+    "vsplat_enable_second_order", "vsplat_update_curvature",
 ]
 
 
@@ -158,6 +159,8 @@ def lib():
         L.vsplat_autotune_views.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
         L.vsplat_score_view.argtypes = [vp, vp, vp, C.c_size_t, C.c_float, C.c_float, vp]
         L.vsplat_set_exposure.argtypes = [vp, vp]
+        L.vsplat_enable_second_order.argtypes = [vp]
+        L.vsplat_update_curvature.argtypes = [vp]
         L.vsplat_ply_info.argtypes = [C.c_char_p, vp, vp]
         L.vsplat_read_ply.argtypes = [C.c_char_p, vp, vp, vp, vp, vp, C.c_uint64]
         L.vsplat_write_ply.argtypes = [C.c_char_p, vp]
@@ -347,6 +350,14 @@ class Trainer:
     def save_ply(self, path):
         """The trainer's cloud written as a 3DGS .ply."""
         _raise(self._lib.vsplat_save_ply(self._h, str(path).encode()), "save_ply")
+
+    def enable_second_order(self):
+        """Trainer::enable_second_order (trainer.hpp:117)."""
+        _raise(self._lib.vsplat_enable_second_order(self._h), "enable_second_order")
+
+    def update_curvature(self):
+        """Trainer::update_curvature (trainer.hpp:118)."""
+        _raise(self._lib.vsplat_update_curvature(self._h), "update_curvature")
 
     def set_exposure(self, transform):
         """Trainer::set_exposure (trainer.hpp:154): 12 floats, rows of (r, g, b, 1)."""
