@@ -29,7 +29,7 @@ extern "C" {
 #define VSPLAT_INVALID_ARGUMENT (-3) /* std::invalid_argument (trainer.cpp:802), and null or malformed arguments */
 #define VSPLAT_LOGIC_ERROR (-4)      /* std::logic_error: gradients before compute_gradients (trainer.cpp:853) */
 #define VSPLAT_DEVICE_ERROR (-5)     /* a HIP call failed, or no GPU */
-#define VSPLAT_UNSUPPORTED (-6)      /* an option this build does not implement (TrainerOptions::positionNoise, poseGradients) */
+#define VSPLAT_UNSUPPORTED (-6)      /* an option this build does not implement (TrainerOptions::poseGradients) */
 
 /* Trainer::kBytesPerTileEntry (trainer.hpp:194): entry keys, values and their sort copies. */
 #define VSPLAT_BYTES_PER_TILE_ENTRY 16u
@@ -64,7 +64,7 @@ typedef struct {
   int32_t force_fixed_point; /* 64-bit fixed-point sums, deterministic; else float atomics */
   int32_t selective_adam;
   float opacity_reg, scale_reg;
-  float position_noise;   /* must be 0: VSPLAT_UNSUPPORTED otherwise */
+  float position_noise;   /* 3DGS-MCMC's noise_lr: after each step, position noise scaled by it times rates->position; 0 for none */
   int32_t mip_filter_2d;
   int32_t pose_gradients; /* must be 0: VSPLAT_UNSUPPORTED otherwise */
   uint64_t memory_budget; /* bytes; UINT64_MAX for none */
@@ -149,6 +149,18 @@ int vsplat_score_view(vsplat_trainer *trainer, const vsplat_camera *camera, cons
  * (VSPLAT_UNSUPPORTED). */
 int vsplat_enable_second_order(vsplat_trainer *trainer);
 int vsplat_update_curvature(vsplat_trainer *trainer);
+/* Trainer::reset_opacity (trainer.hpp:162-163, trainer.cpp:810-819): every
+ * opacity capped at `maximum` (its logit caps the stored value) and the
+ * opacity's optimiser state cleared. VSPLAT_INVALID_ARGUMENT unless
+ * 0 < maximum < 1. */
+int vsplat_reset_opacity(vsplat_trainer *trainer, float maximum);
+/* Trainer::add_position_noise (trainer.hpp:164-165, trainer.cpp:806-808,
+ * train_noise.comp): one round of 3DGS-MCMC's position noise, each position
+ * moved by scaler * gate(1 - opacity) * Sigma n with n three normals drawn
+ * from a hash of the Gaussian's index and `iteration`. A step with
+ * options.position_noise > 0 adds the same round itself, scaler
+ * position_noise * rates->position, iteration the steps taken so far. */
+int vsplat_add_position_noise(vsplat_trainer *trainer, float scaler, uint32_t iteration);
 /* Trainer::set_exposure (trainer.hpp:154): the view's affine colour transform,
  * 12 floats, rows of (r, g, b, 1): the loss compares T (rgb, 1) with the
  * target, and the synchronous calls report dL/dT in stats->exposure_gradient.

@@ -203,6 +203,12 @@ public:
   void enable_second_order() { detail::check(vsplat_enable_second_order(trainer_)); }
   void update_curvature() { detail::check(vsplat_update_curvature(trainer_)); }
 
+  // Trainer::reset_opacity and Trainer::add_position_noise (trainer.hpp:162-165).
+  void reset_opacity(float maximum) { detail::check(vsplat_reset_opacity(trainer_, maximum)); }
+  void add_position_noise(float scaler, std::uint32_t iteration) {
+    detail::check(vsplat_add_position_noise(trainer_, scaler, iteration));
+  }
+
   // Trainer::set_exposure (trainer.hpp:154): rows of (r, g, b, 1).
   void set_exposure(const Exposure &exposure) { detail::check(vsplat_set_exposure(trainer_, exposure.data())); }
 

@@ -17,7 +17,11 @@ differences of Oracle A's loss, the edge weight of flat and stepped targets,
 the error mask's threshold) and, on the GPU, by Oracle A's marked pixels;
 it is pinned by tests/test_densify.py: a brute-force loop on small tables, the
 identity table, the composition of two tables, and fresh (zero) state for
-negative sources.
+negative sources. ``position_noise`` restates train_noise.comp in float64
+from the shader's float32 uniforms; tests/test_noise.py pins it by the
+statistics the reference's own test checks (tests/test_train.cpp:322-350),
+the normals' moments and the covariance of the moves (Sigma^2 for a rotated,
+anisotropic Gaussian), and the gate's closed form.
 """
 from __future__ import annotations
 
@@ -668,6 +672,71 @@ def hutchinson_adjoint(pixels: int, seed: int) -> np.ndarray:
     for ch in range(3):
         out[:, ch] = np.where((bits >> np.uint32(ch)) & np.uint32(1), np.float32(1.0), np.float32(-1.0))
     return out
+
+
+_M32 = np.uint64(0xFFFFFFFF)
+
+
+def pcg_hash(v):
+    """train_noise.comp:23-28: PCG's output permutation of a 32-bit state, in
+    uint64 reduced to 32 bits after each product."""
+    v = np.atleast_1d(np.asarray(v, np.uint64)) & _M32
+    state = (v * np.uint64(747796405) + np.uint64(2891336453)) & _M32
+    word = (((state >> ((state >> np.uint64(28)) + np.uint64(4))) ^ state) * np.uint64(277803737)) & _M32
+    return ((word >> np.uint64(22)) ^ word).astype(np.uint32)
+
+
+def noise_uniforms(count: int, iteration: int) -> np.ndarray:
+    """train_noise.comp:31-33: four uniforms in (0, 1] per Gaussian, float32 as
+    the shader rounds them: float(hash) + 1, then / 4294967297 (which float32
+    holds as 2^32)."""
+    i = np.arange(count, dtype=np.uint64)
+    salt = np.uint64((int(iteration) * 0x9E3779B9) & 0xFFFFFFFF)
+    out = np.empty((count, 4), np.float32)
+    for k in range(4):
+        h = pcg_hash(pcg_hash((i * np.uint64(4) + np.uint64(k)) & _M32).astype(np.uint64) ^ salt)
+        out[:, k] = (h.astype(np.float32) + np.float32(1.0)) / np.float32(4294967297.0)
+    return out
+
+
+def position_noise(positions, scales, rotations, opacities, scaler: float, iteration: int) -> np.ndarray:
+    """train_noise.comp:35-70 in float64 from the shader's float32 uniforms:
+    positions + scaler * gate * Sigma n, gate = logistic(100 ((1 - opacity) -
+    0.995)), n three Box-Muller normals, Sigma = R diag(exp(2 s)) R^T of the
+    normalised rotation; Gaussians whose gate is below 1e-6 or whose move is
+    not finite stay where they are."""
+    p = np.asarray(positions, np.float64).reshape(-1, 3)
+    n_ = p.shape[0]
+    o = 1.0 / (1.0 + np.exp(-np.asarray(opacities, np.float64).reshape(-1)))
+    with np.errstate(over="ignore"):
+        gate = 1.0 / (1.0 + np.exp(-100.0 * ((1.0 - o) - 0.995)))
+    u = noise_uniforms(n_, iteration).astype(np.float64)
+    r0, r1 = np.sqrt(-2.0 * np.log(u[:, 0])), np.sqrt(-2.0 * np.log(u[:, 2]))
+    two_pi = 2.0 * np.pi
+    normal = np.stack([r0 * np.cos(two_pi * u[:, 1]), r0 * np.sin(two_pi * u[:, 1]), r1 * np.cos(two_pi * u[:, 3])], 1)
+    q = np.asarray(rotations, np.float64).reshape(-1, 4).copy()
+    length = np.linalg.norm(q, axis=1)
+    q[length > 0] /= length[length > 0, None]
+    q[length <= 0] = [1.0, 0.0, 0.0, 0.0]
+    w, x, y, z = q.T
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], 1),
+                  np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], 1),
+                  np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)], 1)
+    s2 = np.exp(2.0 * np.asarray(scales, np.float64).reshape(-1, 3))
+    sigma = np.einsum("nij,nj,nkj->nik", R, s2, R)
+    with np.errstate(over="ignore", invalid="ignore"):
+        move = float(scaler) * gate[:, None] * np.einsum("nij,nj->ni", sigma, normal)
+    keep = (gate >= 1e-6) & np.all(np.isfinite(move), axis=1)
+    out = p.copy()
+    out[keep] += move[keep]
+    return out
+
+
+def opacity_limit(maximum: float) -> float:
+    """trainer.cpp:811: the logit log(m / (1 - m)) that reset_opacity caps
+    stored opacities at, in float64."""
+    m = float(np.float32(maximum))
+    return float(np.log(m / (1.0 - m)))
 
 
 def adam_from_gradients(cloud, words, grads, rates, *, rest_scale=0.05, step=1, selective=True, sh_storage=0, states=None):

@@ -65,7 +65,10 @@ def test_argument_errors_come_before_any_device_call():
     assert code == vsplat.RUNTIME_ERROR and b"empty" in lib.vsplat_last_error()
     code, _ = _create(F.Cloud.zeros(4, 4))
     assert code == vsplat.INVALID_ARGUMENT
-    o = vsplat.TrainerOptions(position_noise=0.1).to_c()
+    o = vsplat.TrainerOptions(position_noise=-0.1).to_c()
+    code, _ = _create(F.random_cloud(4, 0, 1), o)
+    assert code == vsplat.INVALID_ARGUMENT
+    o = vsplat.TrainerOptions(pose_gradients=True).to_c()
     code, _ = _create(F.random_cloud(4, 0, 1), o)
     assert code == vsplat.UNSUPPORTED
     assert lib.vsplat_render(None, None, None) == vsplat.INVALID_ARGUMENT

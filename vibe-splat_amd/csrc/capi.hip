@@ -147,6 +147,16 @@ int vsplat_update_curvature(vsplat_trainer *t) {
   return guarded([&] { t->impl.update_curvature(); });
 }
 
+int vsplat_reset_opacity(vsplat_trainer *t, float maximum) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.reset_opacity(maximum); });
+}
+
+int vsplat_add_position_noise(vsplat_trainer *t, float scaler, uint32_t iteration) {
+  if (t == nullptr) return invalid("null trainer");
+  return guarded([&] { t->impl.add_position_noise(scaler, iteration); });
+}
+
 int vsplat_set_exposure(vsplat_trainer *t, const float *transform) {
   if (t == nullptr || transform == nullptr) return invalid("null argument");
   return guarded([&] { t->impl.set_exposure(transform); });

@@ -794,6 +794,71 @@ __global__ __launch_bounds__(256) void score_kernel(const int64_t *__restrict__ 
   stats[static_cast<size_t>(i) * 2 + 1] = more;
 }
 
+// 3DGS-MCMC's exploration (train_noise.comp:30-70): each Gaussian's position
+// moves by scaler * gate(1 - opacity) * Sigma * n, with n three standard
+// normals (Box-Muller from a hash of the index and the iteration), Sigma its
+// covariance and the gate a steep logistic, so that only the nearly transparent
+// wander. iterationDev, when set, holds the step count on the device (after a
+// step: the step just taken), and a step that was skipped moves nothing.
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+  const uint32_t state = v * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+__global__ __launch_bounds__(256) void noise_kernel(float *__restrict__ positions, const float *__restrict__ scales,
+                                                   const float *__restrict__ rotations, const float *__restrict__ opacities, uint32_t count,
+                                                   float scaler, const uint32_t *iterationDev, uint32_t iterationHost,
+                                                   const uint32_t *counters) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= count) return;
+  if (counters != nullptr && counters[kOverflow] != 0) return;
+  const uint32_t iteration = iterationDev != nullptr ? iterationDev[0] : iterationHost;
+  const float opacity = 1.0f / (1.0f + vs_exp(-opacities[i]));
+  const float gate = 1.0f / (1.0f + vs_exp(-100.0f * ((1.0f - opacity) - 0.995f)));
+  if (gate < 1e-6f) return;
+  float u[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    // Uniform in (0, 1], never 0, so that the logarithm is finite.
+    u[k] = (static_cast<float>(pcg_hash(pcg_hash(i * 4u + k) ^ (iteration * 0x9E3779B9u))) + 1.0f) / 4294967297.0f;
+  }
+  const float r0 = sqrtf(-2.0f * vs_log(u[0])), r1 = sqrtf(-2.0f * vs_log(u[2]));
+  const float twoPi = 6.283185307f;
+  const float n[3] = {r0 * cosf(twoPi * u[1]), r0 * sinf(twoPi * u[1]), r1 * cosf(twoPi * u[3])};
+  float q[4] = {rotations[i * 4], rotations[i * 4 + 1], rotations[i * 4 + 2], rotations[i * 4 + 3]};
+  const float length = sqrtf(fma32(q[3], q[3], fma32(q[2], q[2], fma32(q[1], q[1], q[0] * q[0]))));
+  if (length > 0.0f) {
+    for (int k = 0; k < 4; ++k) q[k] = q[k] / length;
+  } else {
+    q[0] = 1.0f;
+    q[1] = q[2] = q[3] = 0.0f;
+  }
+  float R[3][3];
+  quat_to_mat(q[0], q[1], q[2], q[3], R);
+  const float s2[3] = {vs_exp(2.0f * scales[i * 3]), vs_exp(2.0f * scales[i * 3 + 1]), vs_exp(2.0f * scales[i * 3 + 2])};
+  // Sigma n = R (s2 * (R^T n)).
+  float m[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) m[k] = s2[k] * dot3(R[0][k], R[1][k], R[2][k], n[0], n[1], n[2]);
+  float move[3];
+  bool finite = true;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    move[r] = (scaler * gate) * dot3(R[r][0], R[r][1], R[r][2], m[0], m[1], m[2]);
+    finite = finite && fabsf(move[r]) <= 3.4e38f; // false for NaN and infinity
+  }
+  if (!finite) return;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) positions[i * 3 + r] = positions[i * 3 + r] + move[r];
+}
+
+// Trainer::reset_opacity (trainer.cpp:810-819): opacities above the limit (a
+// logit) brought down to it.
+__global__ __launch_bounds__(256) void clamp_opacity_kernel(float *__restrict__ opacities, uint32_t count, float limit) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < count) opacities[i] = opacities[i] > limit ? limit : opacities[i];
+}
+
 // Second-order training: an adjoint of random signs, one per pixel and
 // channel (train_hutchinson.comp:15-29). Taken through the backward pass it
 // gives J^T u, whose square has the diagonal of J^T J as its expectation
@@ -871,6 +936,18 @@ void launch_sh_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewP
   if (params.adamState != nullptr) {
     hipLaunchKernelGGL(advance_adam_kernel, dim3(1), dim3(1), 0, stream, params.adamState, params.beta1, params.beta2, counters);
   }
+}
+
+void launch_noise(hipStream_t stream, const ChainBuffers &buffers, uint32_t count, float scaler, const uint32_t *iterationDev,
+                  uint32_t iterationHost, const uint32_t *counters) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(noise_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, buffers.positions, buffers.scales, buffers.rotations,
+                     buffers.opacities, count, scaler, iterationDev, iterationHost, counters);
+}
+
+void launch_clamp_opacity(hipStream_t stream, float *opacities, uint32_t count, float limit) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(clamp_opacity_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, opacities, count, limit);
 }
 
 void launch_hutchinson(hipStream_t stream, float4 *adjoint, uint32_t pixels, uint32_t seed) {

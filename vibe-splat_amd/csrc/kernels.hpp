@@ -156,6 +156,12 @@ void launch_adam(hipStream_t stream, const ChainBuffers &buffers, const ViewPara
                  uint32_t *counters);
 void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewParams &view, const ChainParams &params,
                   const uint32_t *counters);
+// Position noise after a step (the iteration from *iterationDev when set, the
+// device's step count; nothing when counters say the step was skipped), and
+// opacities clamped to a logit (reset_opacity).
+void launch_noise(hipStream_t stream, const ChainBuffers &buffers, uint32_t count, float scaler, const uint32_t *iterationDev,
+                  uint32_t iterationHost, const uint32_t *counters);
+void launch_clamp_opacity(hipStream_t stream, float *opacities, uint32_t count, float limit);
 // Second order: the Hutchinson adjoint of random signs (update number seed),
 // and the states' second halves cleared (enable_second_order).
 void launch_hutchinson(hipStream_t stream, float4 *adjoint, uint32_t pixels, uint32_t seed);

@@ -105,7 +105,7 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   if (c.positions == nullptr || c.scales == nullptr || c.rotations == nullptr || c.opacities == nullptr || c.sh == nullptr) {
     throw Error(VSPLAT_INVALID_ARGUMENT, "the cloud has a null array");
   }
-  if (options.position_noise != 0.0f) throw Error(VSPLAT_UNSUPPORTED, "position noise is not supported by this build");
+  if (!(options.position_noise >= 0.0f)) throw Error(VSPLAT_INVALID_ARGUMENT, "position noise must be 0 or above");
   if (options.pose_gradients != 0) throw Error(VSPLAT_UNSUPPORTED, "pose gradients are not supported by this build");
   int devices = 0;
   if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) throw Error(VSPLAT_DEVICE_ERROR, "no GPU is visible");
@@ -733,6 +733,32 @@ void Trainer::enqueue_chain(const ViewBuffers &b, const ViewParams &v, uint32_t 
                             hipStream_t stream, uint32_t *counters) {
   if (mode == kModeGradients) ensure_gradients();
   launch_chain(stream, chain_buffers(b), v, chain_params(v, mode, rates), counters);
+  if (mode == kModeStep) enqueue_noise(b, rates, stream, counters);
+}
+
+// After a step with TrainerOptions::position_noise set (trainer.cpp:601): the
+// noise scaled by the position rate, its iteration the device's step count
+// (the step just taken), nothing when the step was skipped.
+void Trainer::enqueue_noise(const ViewBuffers &b, const vsplat_learning_rates *rates, hipStream_t stream, const uint32_t *counters) {
+  if (!(options_.position_noise > 0.0f) || rates == nullptr) return;
+  launch_noise(stream, chain_buffers(b), count_, options_.position_noise * rates->position, adamState_.as<uint32_t>(), 0u, counters);
+}
+
+// Trainer::add_position_noise (trainer.cpp:806-808): one round of the noise
+// with the caller's scaler and iteration.
+void Trainer::add_position_noise(float scaler, uint32_t iteration) {
+  launch_noise(stream_, chain_buffers(*views_[0]), count_, scaler, nullptr, iteration, nullptr);
+  check(hipStreamSynchronize(stream_), "position noise");
+}
+
+// Trainer::reset_opacity (trainer.cpp:810-819): opacities capped at the logit
+// of `maximum`, and the opacity's optimiser state cleared.
+void Trainer::reset_opacity(float maximum) {
+  if (!(maximum > 0.0f && maximum < 1.0f)) throw Error(VSPLAT_INVALID_ARGUMENT, "the maximum opacity must lie in (0, 1)");
+  const float limit = std::log(maximum / (1.0f - maximum));
+  launch_clamp_opacity(stream_, opacities_.as<float>(), count_, limit);
+  launch_fill_u32(stream_, state_[3].as<uint32_t>(), count_, 0u); // one word per opacity: both moments
+  check(hipStreamSynchronize(stream_), "reset opacity");
 }
 
 vsplat_step_stats Trainer::read_stats(bool &overflow) {
@@ -1033,6 +1059,7 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
   lastView_ = params[n - 1];
   if (mode == kModeStep) {
     launch_sh_adam(stream_, buffers, lastView_, chain_params(lastView_, kModeStep, rates), status);
+    enqueue_noise(*views_[0], rates, stream_, status);
   } else {
     haveGradients_ = true;
   }

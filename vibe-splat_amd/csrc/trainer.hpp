@@ -141,6 +141,8 @@ public:
   void apply_changes(const vsplat_cloud *cloud, const int64_t *source, uint32_t count);
   void enable_second_order();
   void update_curvature();
+  void reset_opacity(float maximum);
+  void add_position_noise(float scaler, uint32_t iteration);
   bool second_order() const { return secondOrder_; }
   void set_active_sh_degree(int degree);
   void set_filter3d(const float *filter, size_t count);
@@ -179,6 +181,7 @@ private:
                       const vsplat_learning_rates *rates, bool timed, bool chain = true);
   void enqueue_chain(const ViewBuffers &b, const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates,
                      hipStream_t stream, uint32_t *counters);
+  void enqueue_noise(const ViewBuffers &b, const vsplat_learning_rates *rates, hipStream_t stream, const uint32_t *counters);
   ChainParams chain_params(const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates) const;
   ChainBuffers chain_buffers(const ViewBuffers &b) const;
   void ensure_gradients();

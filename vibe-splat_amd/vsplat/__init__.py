@@ -41,7 +41,7 @@ EXPORTS = [
     "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free",
     "vsplat_autotune_tiles", "vsplat_autotune_views", "vsplat_score_view", "vsplat_set_exposure",
     "vsplat_ply_info", "vsplat_read_ply", "vsplat_write_ply", "vsplat_save_ply",
-    "vsplat_enable_second_order", "vsplat_update_curvature",
+    "vsplat_enable_second_order", "vsplat_update_curvature", "vsplat_reset_opacity", "vsplat_add_position_noise",
 ]
 
 
@@ -161,6 +161,8 @@ def lib():
         L.vsplat_set_exposure.argtypes = [vp, vp]
         L.vsplat_enable_second_order.argtypes = [vp]
         L.vsplat_update_curvature.argtypes = [vp]
+        L.vsplat_reset_opacity.argtypes = [vp, C.c_float]
+        L.vsplat_add_position_noise.argtypes = [vp, C.c_float, C.c_uint32]
         L.vsplat_ply_info.argtypes = [C.c_char_p, vp, vp]
         L.vsplat_read_ply.argtypes = [C.c_char_p, vp, vp, vp, vp, vp, C.c_uint64]
         L.vsplat_write_ply.argtypes = [C.c_char_p, vp]
@@ -358,6 +360,14 @@ class Trainer:
     def update_curvature(self):
         """Trainer::update_curvature (trainer.hpp:118)."""
         _raise(self._lib.vsplat_update_curvature(self._h), "update_curvature")
+
+    def reset_opacity(self, maximum):
+        """Trainer::reset_opacity (trainer.hpp:163): opacities capped at `maximum`, their optimiser state cleared."""
+        _raise(self._lib.vsplat_reset_opacity(self._h, float(maximum)), "reset_opacity")
+
+    def add_position_noise(self, scaler, iteration):
+        """Trainer::add_position_noise (trainer.hpp:165): one round of 3DGS-MCMC's position noise."""
+        _raise(self._lib.vsplat_add_position_noise(self._h, float(scaler), int(iteration)), "add_position_noise")
 
     def set_exposure(self, transform):
         """Trainer::set_exposure (trainer.hpp:154): 12 floats, rows of (r, g, b, 1)."""
