@@ -29,7 +29,7 @@ extern "C" {
 #define VSPLAT_INVALID_ARGUMENT (-3) /* std::invalid_argument (trainer.cpp:802), and null or malformed arguments */
 #define VSPLAT_LOGIC_ERROR (-4)      /* std::logic_error: gradients before compute_gradients (trainer.cpp:853) */
 #define VSPLAT_DEVICE_ERROR (-5)     /* a HIP call failed, or no GPU */
-#define VSPLAT_UNSUPPORTED (-6)      /* an option this build does not implement (TrainerOptions::poseGradients) */
+#define VSPLAT_UNSUPPORTED (-6)      /* a call this build does not take in that state (a batch with second order or an exposure) */
 
 /* Trainer::kBytesPerTileEntry (trainer.hpp:194): entry keys, values and their sort copies. */
 #define VSPLAT_BYTES_PER_TILE_ENTRY 16u
@@ -66,7 +66,7 @@ typedef struct {
   float opacity_reg, scale_reg;
   float position_noise;   /* 3DGS-MCMC's noise_lr: after each step, position noise scaled by it times rates->position; 0 for none */
   int32_t mip_filter_2d;
-  int32_t pose_gradients; /* must be 0: VSPLAT_UNSUPPORTED otherwise */
+  int32_t pose_gradients; /* fill vsplat_step_stats::pose_gradient in vsplat_compute_gradients and vsplat_step */
   uint64_t memory_budget; /* bytes; UINT64_MAX for none */
   /* How the SH coefficients are stored on the GPU (no reference counterpart:
    * the reference always uses VSPLAT_SH_HALF). VSPLAT_SH_BF16 stores every
@@ -96,8 +96,11 @@ typedef struct {
   float position, scale, rotation, opacity, sh, sh_rest_scale, trust_region;
 } vsplat_learning_rates;
 
-/* StepStats (src/train/trainer.hpp:63-73). The pose gradient stays zero; the
- * exposure gradient is filled when an exposure is set (vsplat_set_exposure). */
+/* StepStats (src/train/trainer.hpp:63-73). The exposure gradient is filled
+ * when an exposure is set (vsplat_set_exposure); the pose gradient, dL/d(w, v)
+ * of the camera in camera space (see vsplat_refined_pose), with
+ * options.pose_gradients, by the synchronous calls (the queued and batched
+ * calls leave it zero). */
 typedef struct {
   float loss, l1, ssim;
   uint32_t visible_splats;
@@ -167,6 +170,11 @@ int vsplat_add_position_noise(vsplat_trainer *trainer, float scaler, uint32_t it
  * The neutral transform (1 0 0 0, 0 1 0 0, 0 0 1 0) skips the passes. The
  * queued and batched calls refuse a non-neutral one (VSPLAT_UNSUPPORTED). */
 int vsplat_set_exposure(vsplat_trainer *trainer, const float *transform);
+/* refined_pose (src/train/trainer.hpp:78-80, trainer.cpp:120-137): the camera
+ * moved by move = (w, v), a rotation w (axis times angle) and then a
+ * translation v in camera space, the move a pose gradient is taken for. A CPU
+ * function: no trainer, no device. */
+int vsplat_refined_pose(const vsplat_camera *camera, const float *move, vsplat_camera *out);
 /* load_ply (src/splat/gaussian_cloud.hpp:60-63): a 3DGS .ply, binary
  * little-endian or ASCII; unknown properties such as normals are ignored and
  * the SH degree follows from the number of f_rest_* properties (0, 9, 24, 45).

@@ -349,4 +349,18 @@ private:
   vsplat_trainer *trainer_ = nullptr;
 };
 
+// refined_pose (trainer.hpp:78-80): the camera moved by (rotation w, translation v).
+[[nodiscard]] inline Camera refined_pose(const Camera &camera, const std::array<float, 6> &move) {
+  vsplat_camera c{};
+  for (int i = 0; i < 16; ++i) c.view[i] = camera.view[i];
+  for (int i = 0; i < 3; ++i) c.position[i] = camera.position[i];
+  c.fx = camera.fx, c.fy = camera.fy, c.cx = camera.cx, c.cy = camera.cy, c.width = camera.width, c.height = camera.height;
+  vsplat_camera moved{};
+  detail::check(vsplat_refined_pose(&c, move.data(), &moved));
+  Camera out = camera;
+  for (int i = 0; i < 16; ++i) out.view[i] = moved.view[i];
+  for (int i = 0; i < 3; ++i) out.position[i] = moved.position[i];
+  return out;
+}
+
 } // namespace vsplat

@@ -26,6 +26,7 @@ anisotropic Gaussian), and the gate's closed form.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass
 from pathlib import Path
@@ -78,7 +79,8 @@ class OBChainOptions(C.Structure):
                 ("mode", C.c_uint32), ("flags", C.c_uint32), ("rates", C.c_float * 5), ("rest_scale", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float), ("correction1", C.c_float),
                 ("correction2", C.c_float), ("seed", C.c_uint32), ("sh_floats", C.c_uint32), ("trust_region", C.c_float),
-                ("curvature_scale", C.c_float), ("curvature_beta", C.c_float), ("per_value", C.c_float)]
+                ("curvature_scale", C.c_float), ("curvature_beta", C.c_float), ("per_value", C.c_float),
+                ("pose_terms", C.c_void_p)]
 
 
 CURVATURE_BETA = np.float32(0.999)
@@ -581,6 +583,27 @@ class Model:
                "ob_chain")
         return g, st
 
+    def pose(self, r: ModelResult) -> tuple[np.ndarray, np.ndarray]:
+        """The pose gradient of a gradients-mode chain rule: the per-Gaussian
+        terms (w = t x v, v = Rv dL/dp; zero where no pixel touched) and their
+        total as the product sums it (``pose_totals``)."""
+        n = self.n
+        c = self.cloud
+        pos = np.ascontiguousarray(c.positions, np.float32).copy(); sc = np.ascontiguousarray(c.scales, np.float32).copy()
+        rot = np.ascontiguousarray(c.rotations, np.float32).copy(); op = np.ascontiguousarray(c.opacities, np.float32).copy()
+        words = self.words.copy()
+        raw = r.raw.copy()
+        st = np.zeros((n, 8), np.float32)
+        states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
+        g = [np.zeros(k, np.float32) for k in (n * 3, n * 3, n * 4, n, n * 3 * c.coefficients)]
+        terms = np.zeros((n, 6), np.float32)
+        opt = self.chain_options(1, 1, 0.0, 0.0)
+        opt.pose_terms = terms.ctypes.data
+        _check(lib().ob_chain(_p(pos), _p(sc), _p(rot), _p(op), _p(words), _p(self.filter3d), C.byref(self.camera), _p(raw),
+                              _p(r.splats), _p(st), *[_p(s) for s in states], *[_p(x) for x in g], C.byref(opt)),
+               "ob_chain")
+        return terms, pose_totals(terms)
+
     def statistics(self, r: ModelResult, stats=None) -> np.ndarray:
         """Chain rule in statistics mode (collect_statistics, score_view): only
         the densification statistics, added to `stats` (8 floats per Gaussian)."""
@@ -651,6 +674,50 @@ class Model:
                "ob_chain")
         _check(lib().ob_sh_step(_p(words), _p(states[4]), _p(raw), C.byref(self.camera), C.byref(opt)), "ob_sh_step")
         return states
+
+
+def pose_totals(terms: np.ndarray) -> np.ndarray:
+    """The pose gradient from per-Gaussian terms (trainer.cpp:630-637 with
+    train_gradient.comp:323-337): per group of 256 Gaussians a float32 tree
+    (element l += element l + s for s = 128, 64, ..., 1), then the groups'
+    totals summed in double in order and rounded to float."""
+    t = np.ascontiguousarray(terms, np.float32).reshape(-1, 6)
+    groups = (t.shape[0] + 255) // 256
+    x = np.zeros((groups * 256, 6), np.float32)
+    x[: t.shape[0]] = t
+    x = x.reshape(groups, 256, 6)
+    s = 128
+    while s >= 1:
+        x[:, :s] = x[:, :s] + x[:, s:2 * s]
+        s //= 2
+    total = np.zeros(6, np.float64)
+    for row in x[:, 0].astype(np.float64):
+        total = total + row
+    return total.astype(np.float32)
+
+
+def refined_pose(cam, move):
+    """trainer.cpp:120-137: the camera moved by (w, v), a rotation w (axis times
+    angle, Rodrigues' formula with its small-angle limits) applied after the
+    view and then a translation v, in camera space, in float64; the view and
+    position rounded to float32."""
+    w = np.asarray([float(np.float32(m)) for m in move[:3]], np.float64)
+    v = np.asarray([float(np.float32(m)) for m in move[3:6]], np.float64)
+    angle = float(np.linalg.norm(w))
+    K = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    a = math.sin(angle) / angle if angle > 1e-8 else 1.0
+    b = (1.0 - math.cos(angle)) / (angle * angle) if angle > 1e-8 else 0.5
+    R = np.eye(3) + a * K + b * (K @ K)
+    before = np.asarray(cam.view, np.float32).astype(np.float64)
+    rotation = R @ before[:3, :3]
+    translation = R @ before[:3, 3] + v
+    after = np.eye(4)
+    after[:3, :3] = rotation
+    after[:3, 3] = translation
+    out = type(cam)(**{k: getattr(cam, k) for k in ("fx", "fy", "cx", "cy", "width", "height")})
+    out.view = after.astype(np.float32)
+    out.position = (-(rotation.T @ translation)).astype(np.float32)
+    return out
 
 
 def _hash(x):

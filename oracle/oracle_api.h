@@ -139,6 +139,10 @@ typedef struct {
    * curvature's bias correction 1 / (1 - beta^k), beta, and (curvature mode)
    * 1 / the values in the image. */
   float trust_region, curvature_scale, curvature_beta, per_value;
+  /* When set (gradients and step modes): 6 floats per Gaussian, t x v then
+   * v = Rv dL/dp with t the camera-space position (train_gradient.comp:276-279);
+   * left as they are for Gaussians no pixel touched. */
+  float *pose_terms;
 } ob_chain_options;
 
 /* Chain rule from the 14 raw float sums per Gaussian. In gradients mode

@@ -1158,6 +1158,19 @@ extern "C" int ob_chain(float *positions, float *scales, float *rotations, float
       for (int a = 0; a < 3; ++a) gP[a] = gP[a] + fma(-dir[a], dd, gDir[a]) * idist;
     }
 
+    if (o.pose_terms != nullptr && o.mode != kModeCurvature && (gP[0] != 0.0f || gP[1] != 0.0f || gP[2] != 0.0f)) {
+      // train_gradient.comp:276-279: v = Rv gP, w = t x v.
+      const float *V = c.view;
+      float v[3];
+      for (int r = 0; r < 3; ++r) v[r] = dot3(V[r], V[4 + r], V[8 + r], gP[0], gP[1], gP[2]);
+      float *out = o.pose_terms + static_cast<std::size_t>(i) * 6;
+      out[0] = fma(ty, v[2], -(tz * v[1]));
+      out[1] = fma(tz, v[0], -(tx * v[2]));
+      out[2] = fma(tx, v[1], -(ty * v[0]));
+      out[3] = v[0];
+      out[4] = v[1];
+      out[5] = v[2];
+    }
     for (std::uint32_t a = 0; a < 3; ++a) {
       emit(positions, statePos, gPos, i * 3 + a, gP[a], o.rates[0], radii.position, 0);
       emit(scales, stateScale, gScale, i * 3 + a, dScale[a], o.rates[1], radii.scale, 1);

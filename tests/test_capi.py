@@ -68,9 +68,6 @@ def test_argument_errors_come_before_any_device_call():
     o = vsplat.TrainerOptions(position_noise=-0.1).to_c()
     code, _ = _create(F.random_cloud(4, 0, 1), o)
     assert code == vsplat.INVALID_ARGUMENT
-    o = vsplat.TrainerOptions(pose_gradients=True).to_c()
-    code, _ = _create(F.random_cloud(4, 0, 1), o)
-    assert code == vsplat.UNSUPPORTED
     assert lib.vsplat_render(None, None, None) == vsplat.INVALID_ARGUMENT
     assert lib.vsplat_gaussian_count(None) == 0 and lib.vsplat_uses_fixed_point(None) == 0
     lib.vsplat_trainer_destroy(None)
@@ -79,8 +76,8 @@ def test_argument_errors_come_before_any_device_call():
 def test_python_errors_map_to_reference_exceptions():
     with pytest.raises(RuntimeError, match="empty"):
         vsplat.Trainer(F.Cloud.zeros(0, 1))
-    with pytest.raises(vsplat.Unsupported):
-        vsplat.Trainer(F.random_cloud(4, 0, 1), vsplat.TrainerOptions(pose_gradients=True))
+    with pytest.raises(ValueError):
+        vsplat.Trainer(F.random_cloud(4, 0, 1), vsplat.TrainerOptions(position_noise=-1.0))
     with pytest.raises(ValueError):
         vsplat.math_probe(9, np.zeros(4, np.float32))
 
@@ -105,6 +102,7 @@ WRAPPER_PROGRAM = r"""
 #include "vsplat_trainer.hpp"
 #include <cstdio>
 #include <stdexcept>
+#include <type_traits>
 int main() {
   vsplat::GaussianCloud empty;
   try { vsplat::Trainer t(empty); return 1; } catch (const std::runtime_error &e) { std::printf("empty:%s\n", e.what()); }
@@ -118,8 +116,9 @@ int main() {
   try { vsplat::Trainer t(c); return 2; } catch (const std::runtime_error &e) { std::printf("sizes:%s\n", e.what()); }
   c.sh.push_back(0.f);
   vsplat::TrainerOptions o;
-  o.poseGradients = true;
-  try { vsplat::Trainer t(c, o); return 3; } catch (const vsplat::Unsupported &e) { std::printf("unsupported\n"); }
+  o.positionNoise = -1.f;
+  try { vsplat::Trainer t(c, o); return 3; } catch (const std::invalid_argument &e) { std::printf("invalid\n"); }
+  static_assert(std::is_base_of<std::logic_error, vsplat::Unsupported>::value, "");
   static_assert(vsplat::Trainer::kBytesPerTileEntry == 16, "");
   vsplat::LearningRates r;
   std::printf("rates:%g %g\n", r.position, r.shRestScale);
@@ -141,4 +140,4 @@ def test_cpp_wrapper_maps_statuses_to_reference_exceptions(tmp_path, std):
     subprocess.run(["g++", f"-std={std}", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
                     f"-L{libdir}", "-lvsplat", f"-Wl,-rpath,{libdir}"], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
-    assert "empty:" in out and "sizes:" in out and "unsupported" in out and "rates:0.00016 0.05" in out
+    assert "empty:" in out and "sizes:" in out and "invalid" in out and "rates:0.00016 0.05" in out

@@ -3,6 +3,7 @@
 #include "trainer.hpp"
 #include "ply.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -97,6 +98,51 @@ int vsplat_score_view(vsplat_trainer *t, const vsplat_camera *camera, const uint
     const vsplat_step_stats s = t->impl.score_view(*camera, target, bytes, error_share, pixel_weight);
     if (stats) *stats = s;
   });
+}
+
+// refined_pose (trainer.cpp:120-137): the rotation w by Rodrigues' formula
+// (with the limits of its factors at small angles) turns the view, then v
+// shifts it, in double; the view and position are rounded to float.
+int vsplat_refined_pose(const vsplat_camera *camera, const float *move, vsplat_camera *out) {
+  if (camera == nullptr || move == nullptr || out == nullptr) return invalid("null argument");
+  const double w[3] = {move[0], move[1], move[2]}, v[3] = {move[3], move[4], move[5]};
+  const double angle = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double a = angle > 1e-8 ? std::sin(angle) / angle : 1.0;
+  const double b = angle > 1e-8 ? (1.0 - std::cos(angle)) / (angle * angle) : 0.5;
+  // K is the cross-product matrix of w: K x = w x x.
+  const double K[3][3] = {{0.0, -w[2], w[1]}, {w[2], 0.0, -w[0]}, {-w[1], w[0], 0.0}};
+  double R[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double kk = 0.0;
+      for (int j = 0; j < 3; ++j) kk += K[r][j] * K[j][c];
+      R[r][c] = (r == c ? 1.0 : 0.0) + a * K[r][c] + b * kk;
+    }
+  const float *V = camera->view; // column-major: V[c * 4 + r]
+  double rotation[3][3], translation[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int j = 0; j < 3; ++j) s += R[r][j] * static_cast<double>(V[c * 4 + j]);
+      rotation[r][c] = s;
+    }
+    double s = 0.0;
+    for (int j = 0; j < 3; ++j) s += R[r][j] * static_cast<double>(V[12 + j]);
+    translation[r] = s + v[r];
+  }
+  vsplat_camera result = *camera;
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      double x = r == 3 ? (c == 3 ? 1.0 : 0.0) : (c == 3 ? translation[r] : rotation[r][c]);
+      result.view[c * 4 + r] = static_cast<float>(x);
+    }
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int r = 0; r < 3; ++r) s += rotation[r][c] * translation[r];
+    result.position[c] = static_cast<float>(-s);
+  }
+  *out = result;
+  return VSPLAT_OK;
 }
 
 int vsplat_ply_info(const char *path, uint64_t *count, int32_t *sh_degree) {

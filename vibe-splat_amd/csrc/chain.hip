@@ -533,6 +533,24 @@ __global__ __launch_bounds__(kChainThreads) void chain_geometry_kernel(ChainBuff
     const float4 d = B.dirTerms[i];
     const float dirTerm[3] = {d.x, d.y, d.z};
     view_geometry_terms(B, c, o, i, g, dirTerm, c.shDegree > 0, t);
+    const float *gP = t.position;
+    if ((o.flags & kFlagPose) != 0 && o.mode != kModeCurvature && (gP[0] != 0.0f || gP[1] != 0.0f || gP[2] != 0.0f)) {
+      // How the loss moves with the camera through this Gaussian's position
+      // (train_gradient.comp:276-279): v = Rv gP, w = t x v.
+      float tx, ty, tz;
+      to_camera(c, B.positions[i * 3], B.positions[i * 3 + 1], B.positions[i * 3 + 2], tx, ty, tz);
+      const float *V = c.view;
+      float v[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) v[r] = dot3(V[r], V[4 + r], V[8 + r], gP[0], gP[1], gP[2]);
+      float *out = B.poseTerms + static_cast<size_t>(i) * 6;
+      out[0] = fma32(ty, v[2], -(tz * v[1]));
+      out[1] = fma32(tz, v[0], -(tx * v[2]));
+      out[2] = fma32(tx, v[1], -(ty * v[0]));
+      out[3] = v[0];
+      out[4] = v[1];
+      out[5] = v[2];
+    }
   }
   // Everything above read the parameters before any of them changes.
   emit_terms(B, o, i, t, radii_of(o, B.opacities, B.scales, i));
@@ -852,6 +870,26 @@ __global__ __launch_bounds__(256) void noise_kernel(float *__restrict__ position
   for (int r = 0; r < 3; ++r) positions[i * 3 + r] = positions[i * 3 + r] + move[r];
 }
 
+// The pose gradient's per-group totals (train_gradient.comp:323-337): the
+// terms of 256 Gaussians summed by a tree in LDS (element l plus element
+// l + s, s = 128, 64, ..., 1), one float per component, so the result does not
+// depend on scheduling. The host sums the groups in double, in order.
+__global__ __launch_bounds__(256) void pose_totals_kernel(const float *__restrict__ terms, uint32_t count, float *__restrict__ totals) {
+  __shared__ float lds[6][256];
+  const uint32_t l = threadIdx.x, i = blockIdx.x * 256u + l;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) lds[k][l] = i < count ? terms[static_cast<size_t>(i) * 6 + k] : 0.0f;
+  __syncthreads();
+  for (uint32_t s = 128; s >= 1; s >>= 1) {
+    if (l < s) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) lds[k][l] = lds[k][l] + lds[k][l + s];
+    }
+    __syncthreads();
+  }
+  if (l < 6) totals[blockIdx.x * 6u + l] = lds[l][0];
+}
+
 // Trainer::reset_opacity (trainer.cpp:810-819): opacities above the limit (a
 // logit) brought down to it.
 __global__ __launch_bounds__(256) void clamp_opacity_kernel(float *__restrict__ opacities, uint32_t count, float limit) {
@@ -943,6 +981,11 @@ void launch_noise(hipStream_t stream, const ChainBuffers &buffers, uint32_t coun
   if (count == 0) return;
   hipLaunchKernelGGL(noise_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, buffers.positions, buffers.scales, buffers.rotations,
                      buffers.opacities, count, scaler, iterationDev, iterationHost, counters);
+}
+
+void launch_pose_totals(hipStream_t stream, const float *terms, uint32_t count, float *totals) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(pose_totals_kernel, dim3(pose_groups(count)), dim3(256), 0, stream, terms, count, totals);
 }
 
 void launch_clamp_opacity(hipStream_t stream, float *opacities, uint32_t count, float limit) {

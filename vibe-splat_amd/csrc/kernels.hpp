@@ -18,6 +18,7 @@ constexpr uint32_t kModeRender = 0xFFFFFFFFu; // forward only (Trainer::render)
 constexpr uint32_t kFlagStatistics = 1, kFlagSelective = 4;
 constexpr uint32_t kFlagAccumulate = 8; // gradients mode: add to the gradient buffers instead of writing them
 constexpr uint32_t kFlagSecondOrder = 16; // step mode: the trust-region step, curvature in the states' second halves
+constexpr uint32_t kFlagPose = 32;        // gradients and step modes: each Gaussian's pose terms into poseTerms
 constexpr uint32_t kShPassFloats = 10; // per Gaussian, what the SH step needs from the chain rule
 
 struct ChainParams {
@@ -55,6 +56,7 @@ struct ChainBuffers {
   uint32_t *statePos, *stateScale, *stateRot, *stateOpacity, *stateSh;
   float *gradPos, *gradScale, *gradRot, *gradOpacity, *gradSh;
   float4 *dirTerms; // per Gaussian: the view direction's term of the position gradient (between the chain rule's two kernels)
+  float *poseTerms; // 6 per Gaussian with kFlagPose: t x v, then v = Rv dL/dp; untouched Gaussians' left as they are
 };
 
 // One view of a batched chain rule: its camera, its screen-space sums (fixed
@@ -162,6 +164,9 @@ void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewPar
 void launch_noise(hipStream_t stream, const ChainBuffers &buffers, uint32_t count, float scaler, const uint32_t *iterationDev,
                   uint32_t iterationHost, const uint32_t *counters);
 void launch_clamp_opacity(hipStream_t stream, float *opacities, uint32_t count, float limit);
+// The pose terms' totals per group of 256 Gaussians (6 floats per group).
+void launch_pose_totals(hipStream_t stream, const float *terms, uint32_t count, float *totals);
+inline uint32_t pose_groups(uint32_t count) { return (count + 255) / 256; }
 // Second order: the Hutchinson adjoint of random signs (update number seed),
 // and the states' second halves cleared (enable_second_order).
 void launch_hutchinson(hipStream_t stream, float4 *adjoint, uint32_t pixels, uint32_t seed);

@@ -106,7 +106,6 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
     throw Error(VSPLAT_INVALID_ARGUMENT, "the cloud has a null array");
   }
   if (!(options.position_noise >= 0.0f)) throw Error(VSPLAT_INVALID_ARGUMENT, "position noise must be 0 or above");
-  if (options.pose_gradients != 0) throw Error(VSPLAT_UNSUPPORTED, "pose gradients are not supported by this build");
   int devices = 0;
   if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) throw Error(VSPLAT_DEVICE_ERROR, "no GPU is visible");
   count_ = c.count;
@@ -663,7 +662,8 @@ ChainParams Trainer::chain_params(const ViewParams &v, uint32_t mode, const vspl
   p.shSeed = shSeed_;
   p.mode = mode;
   p.flags = (options_.selective_adam ? kFlagSelective : 0u) | (statisticsNext_ ? kFlagStatistics : 0u) |
-            (accumulateNext_ ? kFlagAccumulate : 0u) | (secondOrder_ ? kFlagSecondOrder : 0u);
+            (accumulateNext_ ? kFlagAccumulate : 0u) | (secondOrder_ ? kFlagSecondOrder : 0u) |
+            (poseNext_ && (mode == kModeGradients || mode == kModeStep) ? kFlagPose : 0u);
   // Second order (trainer.cpp:455-474): the trust region, and the curvature's
   // bias correction 1 / (1 - beta^k), beta^k kept in float by repeated
   // multiplication over k = max(updates, 1).
@@ -709,6 +709,7 @@ ChainBuffers Trainer::chain_buffers(const ViewBuffers &v) const {
   b.rotations = rotations_.as<float>();
   b.opacities = opacities_.as<float>();
   b.shWords = shWordsBuf_.as<uint32_t>();
+  b.poseTerms = poseNext_ ? poseTerms_.as<float>() : nullptr;
   b.filter3d = filterSet_ ? filter3d_.as<float>() : nullptr;
   b.splats = v.splats.as<Splat>();
   b.rawFixed = v.rawFixed.as<int64_t>();
@@ -732,7 +733,10 @@ ChainBuffers Trainer::chain_buffers(const ViewBuffers &v) const {
 void Trainer::enqueue_chain(const ViewBuffers &b, const ViewParams &v, uint32_t mode, const vsplat_learning_rates *rates,
                             hipStream_t stream, uint32_t *counters) {
   if (mode == kModeGradients) ensure_gradients();
+  const bool pose = poseNext_ && (mode == kModeGradients || mode == kModeStep);
+  if (pose) launch_fill_u32(stream, poseTerms_.as<uint32_t>(), static_cast<uint64_t>(count_) * 6, 0u);
   launch_chain(stream, chain_buffers(b), v, chain_params(v, mode, rates), counters);
+  if (pose) launch_pose_totals(stream, poseTerms_.as<float>(), count_, poseTotals_.as<float>());
   if (mode == kModeStep) enqueue_noise(b, rates, stream, counters);
 }
 
@@ -849,6 +853,16 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
   if (needs_tuning(camera.width, camera.height)) autotune_tiles(camera, target_.as<uint8_t>(), 2);
   const ViewParams v = view_params(camera);
   statisticsNext_ = true; // run() always adds to the densification statistics (trainer.cpp:600)
+  // The pose gradient (trainer.cpp:455, 630-637), from the chain rule's pass.
+  poseNext_ = options_.pose_gradients != 0 && (mode == kModeGradients || mode == kModeStep);
+  struct ClearOnExit {
+    bool &flag;
+    ~ClearOnExit() { flag = false; } // also when a pass throws: later calls write no pose terms
+  } clearPose{poseNext_};
+  if (poseNext_) {
+    poseTerms_.ensure(static_cast<size_t>(count_) * 6 * 4);
+    poseTotals_.ensure(static_cast<size_t>(pose_groups(count_)) * 6 * 4);
+  }
   vsplat_step_stats stats{};
   for (int attempt = 0;; ++attempt) {
     enqueue_passes(b, v, target_.as<uint8_t>(), mode, rates, true);
@@ -870,6 +884,16 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
     for (uint32_t g = 0; g < groups; ++g)
       for (int k = 0; k < 12; ++k) sums[k] += static_cast<double>(totals[static_cast<size_t>(g) * 12 + k]);
     for (int k = 0; k < 12; ++k) stats.exposure_gradient[k] = static_cast<float>(sums[k]);
+  }
+  if (poseNext_) {
+    // dL/d(w, v): the groups' totals summed in double in order (trainer.cpp:630-637).
+    const uint32_t groups = pose_groups(count_);
+    std::vector<float> totals(static_cast<size_t>(groups) * 6);
+    check(hipMemcpy(totals.data(), poseTotals_.as<float>(), totals.size() * 4, hipMemcpyDeviceToHost), "read pose totals");
+    double sums[6] = {};
+    for (uint32_t g = 0; g < groups; ++g)
+      for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(totals[static_cast<size_t>(g) * 6 + k]);
+    for (int k = 0; k < 6; ++k) stats.pose_gradient[k] = static_cast<float>(sums[k]);
   }
   if (mode == kModeGradients) haveGradients_ = true;
   lastView_ = v;
@@ -1417,6 +1441,7 @@ uint64_t Trainer::gpu_bytes() const {
     }
   }
   for (const DeviceArray &a : grad_) total += a.bytes();
+  total += poseTerms_.bytes() + poseTotals_.bytes();
   return total;
 }
 

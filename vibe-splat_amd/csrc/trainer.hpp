@@ -212,6 +212,8 @@ private:
   uint32_t shRetained_ = 0, shSeed_ = 0;
   bool haveGradients_ = false;
   bool statisticsNext_ = true; // the next chain pass adds to the densification statistics
+  bool poseNext_ = false;      // the next chain pass writes the pose terms (the synchronous calls, with pose_gradients)
+  DeviceArray poseTerms_, poseTotals_; // 6 floats per Gaussian, and per group of 256
   bool accumulateNext_ = false; // the next chain pass (gradients mode) adds to the gradient sums
   // Per queued view of a batch: its counters and loss sums, copied on the device.
   static constexpr size_t kViewStatsBytes = 48;

@@ -42,6 +42,7 @@ EXPORTS = [
     "vsplat_autotune_tiles", "vsplat_autotune_views", "vsplat_score_view", "vsplat_set_exposure",
     "vsplat_ply_info", "vsplat_read_ply", "vsplat_write_ply", "vsplat_save_ply",
     "vsplat_enable_second_order", "vsplat_update_curvature", "vsplat_reset_opacity", "vsplat_add_position_noise",
+    "vsplat_refined_pose",
 ]
 
 
@@ -164,6 +165,7 @@ def lib():
         L.vsplat_reset_opacity.argtypes = [vp, C.c_float]
         L.vsplat_add_position_noise.argtypes = [vp, C.c_float, C.c_uint32]
         L.vsplat_ply_info.argtypes = [C.c_char_p, vp, vp]
+        L.vsplat_refined_pose.argtypes = [vp, vp, vp]
         L.vsplat_read_ply.argtypes = [C.c_char_p, vp, vp, vp, vp, vp, C.c_uint64]
         L.vsplat_write_ply.argtypes = [C.c_char_p, vp]
         L.vsplat_save_ply.argtypes = [vp, C.c_char_p]
@@ -254,10 +256,11 @@ class StepStats:
     visible_splats: int
     tile_entries: int
     exposure_gradient: tuple = (0.0,) * 12
+    pose_gradient: tuple = (0.0,) * 6
 
     @staticmethod
     def of(c: CStats) -> "StepStats":
-        return StepStats(c.loss, c.l1, c.ssim, c.visible_splats, c.tile_entries, tuple(c.exposure_gradient))
+        return StepStats(c.loss, c.l1, c.ssim, c.visible_splats, c.tile_entries, tuple(c.exposure_gradient), tuple(c.pose_gradient))
 
 
 @dataclass
@@ -573,6 +576,18 @@ class Trainer:
         out = np.zeros(size.value // np.dtype(dtype).itemsize, dtype)
         _raise(self._lib.vsplat_inspect(self._h, name.encode(), _p(out), out.nbytes, C.byref(size)), "inspect")
         return out.reshape(-1, width) if width > 1 else out
+
+
+def refined_pose(camera: Camera, move) -> Camera:
+    """refined_pose (src/train/trainer.hpp:78-80) through the library: the camera
+    moved by (rotation w, translation v) in camera space."""
+    m = np.ascontiguousarray(move, np.float32).reshape(-1)
+    if m.size != 6:
+        raise ValueError("a move is 6 floats: w, then v")
+    c, out = camera_to_c(camera), CCamera()
+    _raise(lib().vsplat_refined_pose(C.byref(c), _p(m), C.byref(out)), "refined_pose")
+    view = np.array(out.view[:], np.float32).reshape(4, 4).T.copy()
+    return Camera(view, np.array(out.position[:], np.float32), out.fx, out.fy, out.cx, out.cy, out.width, out.height)
 
 
 def load_ply(path) -> Cloud:
