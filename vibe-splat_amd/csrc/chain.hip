@@ -458,10 +458,27 @@ __device__ __forceinline__ void store_sh_row(float *out, uint32_t floats, bool a
 // fixed-point conversion) are hidden by more waves per SIMD. The SH part runs
 // first because a step changes the positions it reads.
 template <int Degree, uint32_t Format>
+__device__ __forceinline__ void chain_sh_gradients(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+                                                   float *shRows);
+
+// In gradients mode the SH gradient rows go through LDS (dynamic shared
+// memory, kShRowStride floats per row, 64 rows per wave): each lane writes its
+// row there, then the block stores its rows, which lie consecutively in
+// gradSh, with consecutive lanes on consecutive floats. Stored one row per
+// lane, every store instruction would touch 64 lines 192 bytes apart.
+constexpr uint32_t kShRowStride = 49; // odd, so that a row's lanes hit distinct banks
+constexpr uint32_t kShRowsLds = kChainThreads * kShRowStride * 4;
+
+template <int Degree, uint32_t Format>
 __global__ __launch_bounds__(kChainThreads) void chain_sh_kernel(ChainBuffers B, ViewParams c, ChainParams o, const uint32_t *counters) {
+  extern __shared__ float shRows[];
+  if (counters[kOverflow] != 0) return; // a view did not fit: this pass computed nothing (uniform)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o.mode == kModeGradients) {
+    chain_sh_gradients<Degree, Format>(B, c, o, i, shRows);
+    return;
+  }
   if (i >= c.count) return;
-  if (counters[kOverflow] != 0) return; // a view did not fit: this pass computed nothing
   float g[kRawPerSplat];
   const bool touched = load_raw(o.fixedPoint ? B.rawFixed : nullptr, B.rawFloat, o.fixedScale, i, g);
   const bool zeros = o.mode == kModeGradients || (o.mode == kModeStep && (o.flags & kFlagSelective) == 0);
@@ -505,6 +522,53 @@ __global__ __launch_bounds__(kChainThreads) void chain_sh_kernel(ChainBuffers B,
     }
   });
   B.dirTerms[i] = make_float4(dirTerm[0], dirTerm[1], dirTerm[2], 0.0f);
+}
+
+// chain_sh_kernel's gradients mode: the same terms, each lane's row (zero for
+// a Gaussian no pixel touched, or above the active degree) written to LDS, then
+// the block's rows stored (or added, kFlagAccumulate) in order.
+template <int Degree, uint32_t Format>
+__device__ __forceinline__ void chain_sh_gradients(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+                                                   float *shRows) {
+  const uint32_t floats = o.shFloats;
+  float *row = shRows + threadIdx.x * kShRowStride;
+  if (i < c.count) {
+    float g[kRawPerSplat];
+    const bool touched = load_raw(o.fixedPoint ? B.rawFixed : nullptr, B.rawFloat, o.fixedScale, i, g);
+    for (uint32_t j = 0; j < floats; ++j) row[j] = 0.0f;
+    if (touched) {
+      constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
+      float dirTerm[3];
+      view_sh_terms<Degree, Format>(B, c, o, i, B.shWords + static_cast<size_t>(i) * c.shWords, g, dirTerm,
+                                    [&](const float basis[16], const float gcu[3], const float *, uint32_t used) {
+        for (uint32_t k = 0; k < kCoefficients && k < used; ++k) {
+#pragma unroll
+          for (uint32_t ch = 0; ch < 3; ++ch) row[3 * k + ch] = basis[k] * gcu[ch];
+        }
+      });
+      B.dirTerms[i] = make_float4(dirTerm[0], dirTerm[1], dirTerm[2], 0.0f);
+    }
+  }
+  __syncthreads();
+  const uint32_t first = blockIdx.x * blockDim.x;
+  const uint32_t rows = c.count - first < blockDim.x ? c.count - first : blockDim.x;
+  const uint32_t total = rows * floats;
+  float *out = B.gradSh + static_cast<size_t>(first) * floats;
+  const bool add = (o.flags & kFlagAccumulate) != 0;
+  // Element e of the block's rows is row e / floats, column e % floats; kept
+  // incrementally as e advances by the block size.
+  uint32_t r = threadIdx.x / floats, col = threadIdx.x - r * floats;
+  const uint32_t stepRows = blockDim.x / floats, stepCols = blockDim.x - stepRows * floats;
+  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const float v = shRows[r * kShRowStride + col];
+    out[e] = add ? out[e] + v : v;
+    r += stepRows;
+    col += stepCols;
+    if (col >= floats) {
+      col -= floats;
+      ++r;
+    }
+  }
 }
 
 __global__ __launch_bounds__(kChainThreads) void chain_geometry_kernel(ChainBuffers B, ViewParams c, ChainParams o,
@@ -941,7 +1005,8 @@ void launch_chain(hipStream_t stream, const ChainBuffers &buffers, const ViewPar
                   const uint32_t *counters) {
   if (view.count == 0) return;
   const dim3 grid((view.count + 255) / 256), block(256);
-#define VSPLAT_CHAIN(D, F) hipLaunchKernelGGL((chain_sh_kernel<D, F>), grid, block, 0, stream, buffers, view, params, counters)
+  const uint32_t rowsLds = params.mode == kModeGradients ? kShRowsLds : 0u;
+#define VSPLAT_CHAIN(D, F) hipLaunchKernelGGL((chain_sh_kernel<D, F>), grid, block, rowsLds, stream, buffers, view, params, counters)
   VSPLAT_SH_DISPATCH(view.shDegree, view.shFormat, VSPLAT_CHAIN)
 #undef VSPLAT_CHAIN
   hipLaunchKernelGGL(chain_geometry_kernel, grid, block, 0, stream, buffers, view, params, counters);
