@@ -9,6 +9,8 @@ Trainer::reset_opacity, TrainerOptions::positionNoise; trainer.cpp:601,
   every other parameter as a step without noise does, bit for bit, and moves
   the positions by the noise of the step just taken (iteration 1, scaler
   position_noise * rates.position) on the stepped cloud;
+* replays of a captured batch take the noise of each step in turn (the
+  iteration is the step count on the device), against the oracle;
 * reset_opacity caps the opacities at the logit of the maximum, leaves lower
   ones bit for bit, clears the opacity's optimiser state and nothing else
   (tests/test_densify.cpp:619-622).
@@ -132,3 +134,36 @@ def test_reset_opacity_caps_and_clears_the_opacity_state():
         np.testing.assert_array_equal(v, states[g])
     for name in ("positions", "scales", "rotations", "sh"):
         np.testing.assert_array_equal(_bits(getattr(after, name)), _bits(getattr(before, name)))
+
+
+def test_captured_steps_take_the_noise_of_each_step():
+    """Replays of a captured batch read the step count on the device: with every
+    rate zero but a vanishing position rate (its Adam update rounds away), the
+    positions after three replays are the noise of iterations 1, 2 and 3 added
+    in turn, and nothing else changes."""
+    cloud = _spread_cloud(400, 11)
+    cam, target = F.train_camera(72, 56), F.smooth_target(72, 56, 3)
+    rates = vsplat.LearningRates(position=1e-30, scale=0.0, rotation=0.0, opacity=0.0, sh=0.0, sh_rest_scale=0.05)
+    options = vsplat.TrainerOptions(force_fixed_point=True, position_noise=1e30)
+    with vsplat.DeviceBuffer(target) as d, vsplat.Trainer(cloud, options) as t:
+        stored = t.cloud()
+        t.reserve_entries(1 << 16)
+        t.capture_views([cam], [d.ptr], vsplat.MODE_STEP, rates)
+        t.launch_captured()
+        t.synchronize_views()
+        t.launch_captured(2)
+        t.synchronize_views()
+        got = t.cloud()
+        t.release_captured()
+    for name in OTHERS:
+        np.testing.assert_array_equal(_bits(getattr(got, name)), _bits(getattr(stored, name)))
+    scaler = float(np.float32(1e30) * np.float32(1e-30))
+    want = stored.positions.astype(np.float32)
+    for iteration in (1, 2, 3):
+        want = O.position_noise(want, stored.scales, stored.rotations, stored.opacities, scaler, iteration).astype(np.float32)
+    move = np.abs(want.astype(np.float64) - stored.positions.astype(np.float64))
+    tol = 1e-4 * move.max(1, keepdims=True) + 1e-6 * np.maximum(np.abs(want), 1.0)
+    err = np.abs(got.positions.astype(np.float64) - want.astype(np.float64))
+    print(f"moved {np.mean(move.max(1) > 0):.3f}, worst error / tolerance {np.max(err / tol):.3f}", flush=True)
+    assert np.mean(move.max(1) > 0) > 0.3
+    assert np.all(err <= tol)
