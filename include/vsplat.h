@@ -348,6 +348,11 @@ int vsplat_autotune_views(vsplat_trainer *trainer, uint32_t n, const vsplat_came
  * "stats" (8 floats per Gaussian). *bytes
  * gets the size in use; at most `capacity` bytes are copied. */
 int vsplat_inspect(vsplat_trainer *trainer, const char *buffer, void *out, uint64_t capacity, uint64_t *bytes);
+/* The same for view `view` of the last batch (its k-th camera; view 0 is also
+ * the single-view calls'); the shared buffers (state_*, stats) are the same
+ * for every view. VSPLAT_INVALID_ARGUMENT for a view no batch has made. */
+int vsplat_inspect_view(vsplat_trainer *trainer, uint32_t view, const char *buffer, void *out, uint64_t capacity,
+                        uint64_t *bytes);
 
 /* Device memory for callers without a HIP runtime of their own (no reference
  * counterpart): targets for vsplat_enqueue and the batched calls. Allocates

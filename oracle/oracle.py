@@ -84,6 +84,10 @@ class OBChainOptions(C.Structure):
 
 
 CURVATURE_BETA = np.float32(0.999)
+# Fixed-point steps per unit of a wave total (2^20), and the curvature pass's
+# scale: its adjoint is +-1 per value, without the loss's 1 / (3 W H).
+FIXED_STEPS = np.float32(1048576.0)
+CURVATURE_FIXED_SCALE = float(FIXED_STEPS)
 
 
 def curvature_scale(updates: int) -> float:
@@ -125,6 +129,12 @@ def lib():
         L.ob_log_array.argtypes = [vp, vp, C.c_uint64]
         L.ob_exp_array.restype = None
         L.ob_log_array.restype = None
+        L.ob_hash_array.argtypes = [vp, C.c_uint32, C.c_uint64, vp]
+        L.ob_uniform_array.argtypes = [vp, C.c_uint32, C.c_uint64, vp]
+        L.ob_bf16_stochastic_array.argtypes = [vp, vp, C.c_uint32, C.c_uint64, vp]
+        L.ob_half_stochastic_array.argtypes = [vp, vp, C.c_uint32, C.c_uint64, vp]
+        for f in (L.ob_hash_array, L.ob_uniform_array, L.ob_bf16_stochastic_array, L.ob_half_stochastic_array):
+            f.restype = None
     return _LIB
 
 
@@ -414,6 +424,41 @@ def log32(x):
     return y
 
 
+def train_hash(index, seed: int) -> np.ndarray:
+    """Oracle B's 32-bit hash behind train_uniform and the bf16 rounding."""
+    idx = np.ascontiguousarray(index, np.uint32)
+    out = np.empty(idx.shape, np.uint32)
+    lib().ob_hash_array(_p(idx), int(seed) & 0xFFFFFFFF, idx.size, _p(out))
+    return out
+
+
+def train_uniform(index, seed: int) -> np.ndarray:
+    """Oracle B's dropout and rounding hash (train_types.glsl:155-160): PCG's
+    output permutation of index ^ (seed * 0x9E3779B9), as a float in [0, 1)."""
+    idx = np.ascontiguousarray(index, np.uint32)
+    out = np.empty(idx.shape, np.float32)
+    lib().ob_uniform_array(_p(idx), int(seed) & 0xFFFFFFFF, idx.size, _p(out))
+    return out
+
+
+def bf16_stochastic(x, index, seed: int) -> np.ndarray:
+    """Oracle B's bf16 stochastic rounding (bf16.glsl:16-23): the bf16 bits."""
+    x = np.ascontiguousarray(x, np.float32)
+    idx = np.ascontiguousarray(np.broadcast_to(np.asarray(index, np.uint32), x.shape))
+    out = np.empty(x.shape, np.uint32)
+    lib().ob_bf16_stochastic_array(_p(x), _p(idx), int(seed) & 0xFFFFFFFF, x.size, _p(out))
+    return out
+
+
+def half_stochastic(x, index, seed: int) -> np.ndarray:
+    """Oracle B's fp16 stochastic rounding (train_step.glsl:70-80): the half bits."""
+    x = np.ascontiguousarray(x, np.float32)
+    idx = np.ascontiguousarray(np.broadcast_to(np.asarray(index, np.uint32), x.shape))
+    out = np.empty(x.shape, np.uint32)
+    lib().ob_half_stochastic_array(_p(x), _p(idx), int(seed) & 0xFFFFFFFF, x.size, _p(out))
+    return out
+
+
 def ob_camera(cam, count, sh_degree, active_degree=None, near_plane=0.01, sh_storage=0):
     """The per-view constants block, as the product's host code fills it."""
     from vsplat.fixtures import sh_words  # noqa: WPS433 (tests put the package on sys.path)
@@ -467,6 +512,7 @@ class Model:
         self.words = pack_sh(cloud.sh, sh_storage)
         self.camera = ob_camera(cam, self.n, cloud.sh_degree, active_degree, near_plane, sh_storage)
         self.pre = OBPreOptions(int(mip_filter_2d), float(sh_dropout[0]), int(sh_dropout[1]), int(sh_dropout[2]))
+        self.sh_dropout = tuple(sh_dropout)
         self.mip = mip_filter_2d
         self.filter3d = None if filter3d is None else np.ascontiguousarray(filter3d, np.float32)
         self.background = np.asarray(background, np.float32)
@@ -475,7 +521,9 @@ class Model:
         self.tiles = self.grid_w * self.grid_h
 
     def fixed_scale(self):
-        return float(np.float32(3.0) * np.float32(self.cam.width) * np.float32(self.cam.height) * np.float32(64.0))
+        """3 W H 2^20: the per-pixel 1 / (3 W H) undone, then 2^20 steps per
+        unit (trainer.cpp:586-591 has 64; these sums are 64-bit)."""
+        return float(np.float32(3.0) * np.float32(self.cam.width) * np.float32(self.cam.height) * FIXED_STEPS)
 
     def forward(self) -> ModelResult:
         L = lib()
@@ -530,15 +578,18 @@ class Model:
             r.adjoint[:, 3] = error_mask(error, worst, score_share)
         return r
 
-    def backward_fixed(self, r: ModelResult, adjoint=None):
+    def backward_fixed(self, r: ModelResult, adjoint=None, scale=None):
+        """Fixed-point sums; scale defaults to fixed_scale(), and is
+        CURVATURE_FIXED_SCALE for the curvature's +-1 adjoint."""
         adj = r.adjoint if adjoint is None else np.ascontiguousarray(adjoint, np.float32)
+        scale = self.fixed_scale() if scale is None else float(scale)
         edge = getattr(r, "edge", None)
         r.raw_fixed = np.zeros((self.n, 14), np.int64)
         _check(lib().ob_backward_fixed(_p(r.splats), _p(r.entry_values), _p(r.ranges), self.cam.width, self.cam.height,
-                                       _p(self.background), _p(r.image), _p(adj), _p(r.stops), self.fixed_scale(),
+                                       _p(self.background), _p(r.image), _p(adj), _p(r.stops), scale,
                                        self.n, _p(r.raw_fixed), None if edge is None else _p(edge)), "ob_backward_fixed")
         r.raw = np.zeros((self.n, 14), np.float32)
-        _check(lib().ob_fixed_to_float(_p(r.raw_fixed), self.n * 14, self.fixed_scale(), _p(r.raw)), "ob_fixed_to_float")
+        _check(lib().ob_fixed_to_float(_p(r.raw_fixed), self.n * 14, scale, _p(r.raw)), "ob_fixed_to_float")
         return r
 
     def backward_float(self, r: ModelResult, adjoint=None):
@@ -551,7 +602,11 @@ class Model:
         return r
 
     def chain_options(self, mode=1, flags=1, opacity_reg=0.0, scale_reg=0.0, rates=(0, 0, 0, 0, 0), rest_scale=0.05,
-                      step=1, sh_dropout=(0.0, 0, 0)):
+                      step=1, sh_dropout=None):
+        """The chain rule's options; the SH dropout defaults to the model's own
+        (the product applies it in steps only: model a step with it, and
+        compute_gradients without)."""
+        sh_dropout = self.sh_dropout if sh_dropout is None else sh_dropout
         o = OBChainOptions()
         o.opacity_reg = float(np.float32(opacity_reg) / np.float32(self.n))
         o.scale_reg = float(np.float32(scale_reg) / np.float32(self.n))

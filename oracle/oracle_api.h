@@ -167,6 +167,12 @@ int ob_adam(float *positions, float *scales, float *rotations, float *opacities,
 /* The exp and log the model uses, for the math tests. */
 void ob_exp_array(const float *x, float *y, uint64_t n);
 void ob_log_array(const float *x, float *y, uint64_t n);
+/* The dropout and rounding hash (32 bits, and as a float in [0, 1)) and the
+ * two stochastic roundings, elementwise. */
+void ob_hash_array(const uint32_t *index, uint32_t seed, uint64_t n, uint32_t *out);
+void ob_uniform_array(const uint32_t *index, uint32_t seed, uint64_t n, float *out);
+void ob_bf16_stochastic_array(const float *x, const uint32_t *index, uint32_t seed, uint64_t n, uint32_t *out);
+void ob_half_stochastic_array(const float *x, const uint32_t *index, uint32_t seed, uint64_t n, uint32_t *out);
 
 #ifdef __cplusplus
 }

@@ -166,12 +166,17 @@ void sh_basis_gradient(float x, float y, float z, std::uint32_t degree, float g[
   }
 }
 
-// PCG's output permutation as a hash to [0, 1) (train_types.glsl:156-160).
-float train_uniform(std::uint32_t index, std::uint32_t seed) {
+// PCG's output permutation (RXS M XS, O'Neill 2014; the GPU hash of Jarzynski
+// and Olano 2020) of index ^ (seed * 0x9E3779B9) after one LCG step
+// (train_types.glsl:156-160, bf16.glsl:19-22).
+std::uint32_t train_hash(std::uint32_t index, std::uint32_t seed) {
   const std::uint32_t state = (index ^ (seed * 0x9E3779B9u)) * 747796405u + 2891336453u;
   const std::uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
-  return static_cast<float>((word >> 22u) ^ word) * 0x1p-32f;
+  return (word >> 22u) ^ word;
 }
+
+// The hash as a float in [0, 1) (train_types.glsl:156-160).
+float train_uniform(std::uint32_t index, std::uint32_t seed) { return static_cast<float>(train_hash(index, seed)) * 0x1p-32f; }
 
 // train_types.glsl:165-170
 std::uint32_t sh_coefficients_used(std::uint32_t index, std::uint32_t coefficients, float dropout, std::uint32_t retained,
@@ -316,12 +321,15 @@ TileGeometry tile_geometry(std::uint32_t tile, std::uint32_t gridW, std::uint32_
   return {tx * kTile, ty * kTile, std::min(tx * kTile + kTile, width), std::min(ty * kTile + kTile, height)};
 }
 
-// The fixed-point quantisation of one wave total (train_backward_fixed.comp:19-21), NaN as 0.
-std::int32_t quantise(float v, float scale) {
+// The fixed-point quantisation of one wave total (train_backward_fixed.comp:19-21), NaN as 0,
+// in 64 bits: the clamp is the reference's 1e9 times 2^14, for a scale of 2^20
+// steps per unit where the reference's has 64 (the caller's fixedScale).
+constexpr float kFixedClamp = 1.6384e13f;
+std::int64_t quantise(float v, float scale) {
   float s = v * scale;
   if (s != s) s = 0.0f;
-  s = s > 1e9f ? 1e9f : (s < -1e9f ? -1e9f : s);
-  return static_cast<std::int32_t>(std::rint(s));
+  s = s > kFixedClamp ? kFixedClamp : (s < -kFixedClamp ? -kFixedClamp : s);
+  return static_cast<std::int64_t>(std::rint(s));
 }
 
 // The backward pass's reduction tree over a wave: lane l and lane l ^ m are
@@ -344,10 +352,10 @@ std::uint32_t bf16_round(float x) {
   const std::uint32_t bits = float_to_bits(x);
   return (bits + 0x7FFFu + ((bits >> 16) & 1u)) >> 16;
 }
+// Up or down at random: the low 16 bits of the hash added to the float's bits
+// carry into the kept half with probability (low 16 bits of x) / 2^16.
 std::uint32_t bf16_stochastic(float x, std::uint32_t index, std::uint32_t seed) {
-  const std::uint32_t state = (index ^ (seed * 0x9E3779B9u)) * 747796405u + 2891336453u;
-  const std::uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
-  return (float_to_bits(x) + (((word >> 22u) ^ word) & 0xFFFFu)) >> 16;
+  return (float_to_bits(x) + (train_hash(index, seed) & 0xFFFFu)) >> 16;
 }
 std::uint32_t bf16_store(std::uint32_t word, std::uint32_t slot, std::uint32_t bits) {
   return slot == 0 ? (word & 0xFFFF0000u) | bits : (word & 0xFFFFu) | (bits << 16);
@@ -1319,4 +1327,20 @@ extern "C" void ob_exp_array(const float *x, float *y, uint64_t n) {
 }
 extern "C" void ob_log_array(const float *x, float *y, uint64_t n) {
   for (std::uint64_t i = 0; i < n; ++i) y[i] = flog(x[i]);
+}
+
+// The hash and the two stochastic roundings elementwise, for the tests that pin
+// them: train_hash(index[i], seed), train_uniform(index[i], seed), bf16_stochastic(x[i], index[i], seed)
+// and half_stochastic(x[i], index[i], seed) (16 bits per output word).
+extern "C" void ob_hash_array(const uint32_t *index, uint32_t seed, uint64_t n, uint32_t *out) {
+  for (std::uint64_t i = 0; i < n; ++i) out[i] = oracle::train_hash(index[i], seed);
+}
+extern "C" void ob_uniform_array(const uint32_t *index, uint32_t seed, uint64_t n, float *out) {
+  for (std::uint64_t i = 0; i < n; ++i) out[i] = oracle::train_uniform(index[i], seed);
+}
+extern "C" void ob_bf16_stochastic_array(const float *x, const uint32_t *index, uint32_t seed, uint64_t n, uint32_t *out) {
+  for (std::uint64_t i = 0; i < n; ++i) out[i] = oracle::bf16_stochastic(x[i], index[i], seed);
+}
+extern "C" void ob_half_stochastic_array(const float *x, const uint32_t *index, uint32_t seed, uint64_t n, uint32_t *out) {
+  for (std::uint64_t i = 0; i < n; ++i) out[i] = oracle::half_stochastic(x[i], index[i], seed);
 }

@@ -67,7 +67,8 @@ def test_curvature_and_trust_region_step_match_the_model(sh_storage):
     r = m.forward()
     m.loss(r, target, 0.2)
     r.adjoint = O.hutchinson_adjoint(cam.width * cam.height, 1)
-    m.backward_fixed(r)
+    # The curvature's sums carry no 1 / (3 W H): their own fixed-point scale.
+    m.backward_fixed(r, scale=O.CURVATURE_FIXED_SCALE)
     n = cloud.count
     states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * cloud_s.coefficients)]
     m.curvature(r, states, 1)
@@ -109,3 +110,44 @@ def test_second_order_steps_fit_a_view_within_their_trust_region():
             t.apply_gradients(RATES)
     print(f"loss {first.loss:.6f} -> {loss:.6f}", flush=True)
     assert np.isfinite(loss) and loss < first.loss
+
+
+def test_curvature_after_a_batch_takes_the_last_views_buffers():
+    """After a batch of two sizes, update_curvature works on the last view
+    (its camera, lists and image), bit for bit against Oracle B there; the
+    larger last view must not be read through the first view's buffers."""
+    cloud = F.random_cloud(120, 1, 67)
+    cams = [F.train_camera(72, 56), F.train_camera(100, 75)]
+    targets = [F.random_target(72, 56, 61), F.random_target(100, 75, 62)]
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+        t.enable_second_order()
+        t.step_views(cams, targets)  # gradients mode: no parameter changes
+        t.update_curvature()
+        curved = _states(t)
+    m = O.Model(cloud, cams[1])
+    r = m.forward()
+    m.loss(r, targets[1], 0.2)
+    r.adjoint = O.hutchinson_adjoint(cams[1].width * cams[1].height, 1)
+    m.backward_fixed(r, scale=O.CURVATURE_FIXED_SCALE)
+    n = cloud.count
+    states = [np.zeros(k, np.uint32) for k in (n * 3, n * 3, n * 4, n, n * 3 * cloud.coefficients)]
+    m.curvature(r, states, 1)
+    assert sum(int(np.count_nonzero(s >> 16)) for s in curved) > 0
+    for got, want in zip(curved, states):
+        np.testing.assert_array_equal(got, want)
+
+
+def test_curvature_needs_a_view_that_ran():
+    """Nothing ran after an autotune: update_curvature refuses (a logic error)
+    before any launch."""
+    cloud, cam, target = _case()
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t, vsplat.DeviceBuffer(target) as d:
+        t.enable_second_order()
+        with pytest.raises(vsplat.LogicError):
+            t.update_curvature()
+        t.compute_gradients(cam, target)
+        t.autotune_tiles(cam, d.ptr)
+        with pytest.raises(vsplat.LogicError):
+            t.update_curvature()
+        t.compute_gradients(cam, target)
+        t.update_curvature()

@@ -177,3 +177,23 @@ def test_captured_batch_replays_as_queued_steps():
     finally:
         for d in buffers:
             d.free()
+
+
+def test_captured_batch_refuses_a_later_exposure():
+    """An exposure set after the capture would be ignored by a replay (the
+    batch takes the neutral one): the replay refuses instead."""
+    cloud, cams, targets = _ring()
+    buffers = [vsplat.DeviceBuffer(x) for x in targets]
+    try:
+        with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+            t.reserve_entries(1 << 16)
+            t.capture_views(cams, [d.ptr for d in buffers], vsplat.MODE_STEP, RATES)
+            t.set_exposure([1.1, 0, 0, 0.01, 0, 1, 0, 0, 0, 0, 1, 0])
+            with pytest.raises(vsplat.LogicError):
+                t.launch_captured()
+            t.set_exposure(O.NEUTRAL_EXPOSURE)
+            t.launch_captured()
+            t.synchronize_views()
+    finally:
+        for d in buffers:
+            d.free()

@@ -384,6 +384,12 @@ extern "C" int vsplat_inspect(vsplat_trainer *t, const char *buffer, void *out, 
   return guarded([&] { t->impl.inspect(buffer, out, capacity, bytes); });
 }
 
+extern "C" int vsplat_inspect_view(vsplat_trainer *t, uint32_t view, const char *buffer, void *out, uint64_t capacity,
+                                   uint64_t *bytes) {
+  if (t == nullptr || buffer == nullptr) return invalid("null argument");
+  return guarded([&] { t->impl.inspect(buffer, out, capacity, bytes, view); });
+}
+
 extern "C" int vsplat_math_probe(int32_t function, const float *in, float *out, uint64_t n) {
   if ((in == nullptr || out == nullptr) && n > 0) return invalid("null argument");
   return guarded([&] { vsplat::math_probe(function, in, out, n); });

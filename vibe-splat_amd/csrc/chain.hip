@@ -927,7 +927,7 @@ __global__ __launch_bounds__(256) void noise_kernel(float *__restrict__ position
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     move[r] = (scaler * gate) * dot3(R[r][0], R[r][1], R[r][2], m[0], m[1], m[2]);
-    finite = finite && fabsf(move[r]) <= 3.4e38f; // false for NaN and infinity
+    finite = finite && __builtin_isfinite(move[r]); // NaN or infinity skips the Gaussian (train_noise.comp)
   }
   if (!finite) return;
 #pragma unroll

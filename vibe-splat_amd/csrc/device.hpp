@@ -21,6 +21,14 @@ constexpr uint32_t kTileSize = 16;       // train_types.glsl:6
 constexpr uint32_t kTilePixels = 256;    // one workgroup of 256 threads per tile
 constexpr uint32_t kInvisible = 0xFFFFFFFFu;
 constexpr uint32_t kRawPerSplat = 14;    // train_types.glsl:15
+// Fixed-point sums (train_backward_fixed.comp:19-21, trainer.cpp:586-591, widened
+// to 64 bits): each wave total is quantised at kFixedSteps steps per unit and
+// clamped to kFixedClamp steps, the reference's 64 steps and 1e9 times 2^14.
+// A unit is 1 / (3 W H) for the loss's adjoint and 1 for the curvature's.
+// At most kFixedClamp per wave keeps a Gaussian's sum inside int64 for up to
+// 2^19 waves (over 33 million pixels).
+constexpr float kFixedSteps = 1048576.0f; // 2^20
+constexpr float kFixedClamp = 1.6384e13f; // 1e9 * 2^14, exact in float
 
 // The per-view constants, passed by value as a kernel argument (camera block of
 // splat_types.glsl:313-319, with the near plane and the frustum half-tangents

@@ -162,12 +162,15 @@ __global__ __launch_bounds__(TW * TH) void forward_kernel(const Splat *__restric
   }
 }
 
-// The fixed-point quantisation of a wave total (train_backward_fixed.comp:19-21), NaN as 0.
-__device__ __forceinline__ int32_t quantise(float v, float scale) {
+// The fixed-point quantisation of a wave total (train_backward_fixed.comp:19-21),
+// NaN as 0, widened to 64 bits: the scale carries 2^20 steps per unit where the
+// reference's carries 64, and the clamp is the reference's 1e9 times the same
+// 2^14, so a total spans the same range in units at 16384 times the resolution.
+__device__ __forceinline__ int64_t quantise(float v, float scale) {
   float s = v * scale;
   if (s != s) s = 0.0f;
-  s = s > 1e9f ? 1e9f : (s < -1e9f ? -1e9f : s);
-  return static_cast<int32_t>(__builtin_rintf(s));
+  s = s > kFixedClamp ? kFixedClamp : (s < -kFixedClamp ? -kFixedClamp : s);
+  return static_cast<int64_t>(__builtin_rintf(s));
 }
 
 __device__ __forceinline__ float dpp_row_mirror(float v) { return as_float(__builtin_amdgcn_update_dpp(0, static_cast<int>(as_uint(v)), 0x140, 0xF, 0xF, false)); }
@@ -374,8 +377,8 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
       if (emits) {
         const size_t slot = static_cast<size_t>(batchIndex[k]) * kRawPerSplat + value;
         if constexpr (Fixed) {
-          const int32_t q = quantise(v[0], fixedScale);
-          if (q != 0) atomicAdd(&rawFixed[slot], static_cast<unsigned long long>(static_cast<long long>(q)));
+          const int64_t q = quantise(v[0], fixedScale);
+          if (q != 0) atomicAdd(&rawFixed[slot], static_cast<unsigned long long>(q));
         } else {
           if (v[0] != 0.0f) atomicAdd(&rawFloat[slot], v[0]);
         }

@@ -19,8 +19,12 @@
 
 namespace vsplat {
 
-// The fixed-point scale 3 W H 64 (trainer.cpp:586-591).
-static float fixed_scale(const ViewParams &v) { return 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * 64.0f; }
+// The fixed-point scale of the loss's sums: per-pixel derivatives carry
+// 1 / (3 W H), which 3 W H undoes, then kFixedSteps steps per unit
+// (trainer.cpp:586-591 has 64 steps; the sums here are 64-bit).
+static float fixed_scale(const ViewParams &v) { return 3.0f * static_cast<float>(v.width) * static_cast<float>(v.height) * kFixedSteps; }
+// The curvature's Hutchinson adjoint is +-1 per value, without the 1 / (3 W H).
+static float curvature_fixed_scale() { return kFixedSteps; }
 
 namespace {
 
@@ -386,6 +390,7 @@ std::pair<uint32_t, uint32_t> Trainer::autotune_tiles(const vsplat_camera &camer
   dropoutNow_ = savedDropout;
   tuned_[{camera.width, camera.height}] = best;
   ensure_size(b, camera.width, camera.height);
+  lastView_.width = 0; // the buffers hold the timed passes of some shape, not a step
   return best;
 }
 
@@ -473,6 +478,7 @@ void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uin
     if (tileH != nullptr) tileH[k] = shape.second;
   }
   batchTunedKey_ = sizes_key(n, cameras);
+  lastView_.width = 0; // the buffers hold the timed passes, not a step
 }
 
 // The view sizes of a batch, in order, as one number.
@@ -641,7 +647,7 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   }
   mark("loss");
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
-  // a resolution of 1/64 (trainer.cpp:586-591).
+  // a resolution of 2^-20 (trainer.cpp:586-591 keeps 1/64 in 32 bits).
   const float fixedScale = fixed_scale(v);
   launch_backward(s, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
                   b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
@@ -692,7 +698,7 @@ ChainParams Trainer::chain_params(const ViewParams &v, uint32_t mode, const vspl
   }
   p.shFloats = shFloats_;
   p.fixedPoint = fixedPoint_ ? 1u : 0u;
-  p.fixedScale = fixed_scale(v);
+  p.fixedScale = mode == kModeCurvature ? curvature_fixed_scale() : fixed_scale(v);
   return p;
 }
 
@@ -897,6 +903,7 @@ vsplat_step_stats Trainer::run(const vsplat_camera &camera, const uint8_t *targe
   }
   if (mode == kModeGradients) haveGradients_ = true;
   lastView_ = v;
+  lastViewIndex_ = 0;
   return stats;
 }
 
@@ -945,6 +952,9 @@ void Trainer::render(const vsplat_camera &camera, float *out) {
     if (attempt > 0 && stats.tile_entries <= b.entryCapacity) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
     grow_entries(b, stats.tile_entries);
   }
+  // As the reference's render uploads its camera, a curvature update now takes this view.
+  lastView_ = v;
+  lastViewIndex_ = 0;
   const size_t pixels = static_cast<size_t>(b.width) * static_cast<size_t>(b.height);
   check(hipMemcpy(out, b.image.as<float4>(), pixels * 16, hipMemcpyDeviceToHost), "download image");
 }
@@ -976,6 +986,7 @@ void Trainer::enqueue(const vsplat_camera &camera, const uint8_t *deviceTarget, 
   enqueue_passes(b, v, deviceTarget, mode, rates, timeQueued_);
   if (mode == kModeGradients) haveGradients_ = true;
   lastView_ = v;
+  lastViewIndex_ = 0;
 }
 
 vsplat_step_stats Trainer::synchronize() {
@@ -1081,6 +1092,7 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
   }
   accumulateNext_ = false;
   lastView_ = params[n - 1];
+  lastViewIndex_ = n - 1;
   if (mode == kModeStep) {
     launch_sh_adam(stream_, buffers, lastView_, chain_params(lastView_, kModeStep, rates), status);
     enqueue_noise(*views_[0], rates, stream_, status);
@@ -1129,6 +1141,9 @@ uint64_t Trainer::capture_key(uint32_t n) const {
   std::memcpy(&dropout, &shDropout_, 4);
   mix(dropout);
   mix(static_cast<uint64_t>(shRetained_) << 32 | shSeed_);
+  // Settings the batch refuses or bakes in: an exposure set since the capture
+  // would be ignored by a replay, and second order changes the step.
+  mix((exposing_ ? 1u : 0u) | (secondOrder_ ? 2u : 0u));
   return h;
 }
 
@@ -1175,7 +1190,9 @@ void Trainer::capture_views(uint32_t n, const vsplat_camera *cameras, const uint
   graphViews_ = n;
   graphViewSizes_ = queuedViewSizes_;
   graphKey_ = capture_key(n);
+  graphLastView_ = lastView_;
   queuedViewSizes_.clear();
+  lastView_.width = 0; // captured, not run
 }
 
 void Trainer::launch_captured(uint32_t replays) {
@@ -1184,7 +1201,11 @@ void Trainer::launch_captured(uint32_t replays) {
     throw Error(VSPLAT_LOGIC_ERROR, "the trainer's buffers or settings changed since the batch was captured: capture it again");
   }
   for (uint32_t r = 0; r < replays; ++r) check(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
-  if (replays > 0) queuedViewSizes_ = graphViewSizes_;
+  if (replays > 0) {
+    queuedViewSizes_ = graphViewSizes_;
+    lastView_ = graphLastView_;
+    lastViewIndex_ = graphViews_ - 1;
+  }
 }
 
 uint32_t Trainer::synchronize_views(vsplat_step_stats *stats, uint32_t capacity, bool &overflow) {
@@ -1339,6 +1360,7 @@ void Trainer::apply_changes(const vsplat_cloud *cloud, const int64_t *source, ui
   count_ = count;
   haveGradients_ = false;
   accumulateNext_ = false;
+  lastView_.width = 0;
   release_views();
   allocate_per_gaussian();
   view_buffers(0);
@@ -1365,8 +1387,16 @@ void Trainer::enable_second_order() {
 // the backward pass of the last view's lists and image, then the chain rule
 // adding each parameter's squared gradient to its curvature.
 void Trainer::update_curvature() {
-  ViewBuffers &b = *views_[0];
-  if (b.width == 0 || lastView_.width == 0) throw Error(VSPLAT_LOGIC_ERROR, "update_curvature() needs a view: step() or compute_gradients() first");
+  // The last view that ran, on the buffers it ran on (a batch's last view has
+  // its own); nothing after an autotune or a capture, which leave the buffers
+  // with passes of other shapes or none.
+  if (lastView_.width == 0 || lastViewIndex_ >= views_.size()) {
+    throw Error(VSPLAT_LOGIC_ERROR, "update_curvature() needs a view: step() or compute_gradients() first");
+  }
+  ViewBuffers &b = *views_[lastViewIndex_];
+  if (b.width != static_cast<int>(lastView_.width) || b.height != static_cast<int>(lastView_.height)) {
+    throw Error(VSPLAT_LOGIC_ERROR, "update_curvature(): the last view's buffers were remade; run a step first");
+  }
   ++curvatureUpdates_;
   const ViewParams &v = lastView_;
   const uint32_t pixels = v.width * v.height;
@@ -1379,7 +1409,8 @@ void Trainer::update_curvature() {
   }
   launch_backward(stream_, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
                   b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
-                  b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixed_scale(v), b.rawFixed.as<int64_t>(), b.rawFloat.as<float>());
+                  b.adjoint.as<float4>(), b.stops.as<uint32_t>(), curvature_fixed_scale(), b.rawFixed.as<int64_t>(),
+                  b.rawFloat.as<float>());
   const bool statistics = statisticsNext_;
   statisticsNext_ = false; // the curvature adds nothing to the densification statistics
   enqueue_chain(b, v, kModeCurvature, nullptr, stream_, counters);
@@ -1449,8 +1480,9 @@ uint64_t Trainer::gpu_bytes() const {
 
 namespace vsplat {
 
-void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uint64_t *bytes) const {
-  const ViewBuffers &b = *views_[0];
+void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uint64_t *bytes, uint32_t view) const {
+  if (view >= views_.size()) throw Error(VSPLAT_INVALID_ARGUMENT, "no such view: a batch makes one per view");
+  const ViewBuffers &b = *views_[view];
   const uint64_t n = count_, pixels = static_cast<uint64_t>(b.width) * static_cast<uint64_t>(b.height);
   uint64_t entries = 0;
   if (name == "entry_keys" || name == "entry_values") {

@@ -151,7 +151,8 @@ public:
   std::vector<std::pair<std::string, double>> timings() const;
   void reset_timings();
   void set_queued_timing(bool on) { timeQueued_ = on; }
-  void inspect(const std::string &buffer, void *out, uint64_t capacity, uint64_t *bytes) const;
+  // One of view `view`'s buffers (a batch's k-th view; 0 for the single-view calls), or a shared one.
+  void inspect(const std::string &buffer, void *out, uint64_t capacity, uint64_t *bytes, uint32_t view = 0) const;
 
   uint32_t count() const { return count_; }
   int sh_degree() const { return shDegree_; }
@@ -218,7 +219,8 @@ private:
   // Per queued view of a batch: its counters and loss sums, copied on the device.
   static constexpr size_t kViewStatsBytes = 48;
   std::vector<std::pair<int, int>> queuedViewSizes_;
-  ViewParams lastView_{};
+  ViewParams lastView_{};   // the camera of the last view that ran (width 0: none)
+  size_t lastViewIndex_ = 0; // whose buffers hold its lists, image and sums
 
   hipStream_t stream_ = nullptr;
   // Parameters and optimiser state, in GaussianCloud's layout; SH packed as halves.
@@ -237,6 +239,7 @@ private:
   uint32_t graphViews_ = 0;
   std::vector<std::pair<int, int>> graphViewSizes_;
   uint64_t graphKey_ = 0;
+  ViewParams graphLastView_{}; // the captured batch's last view
   uint64_t reservedEntries_ = 0; // what reserve_entries asked for, for views made later
   std::map<std::pair<int, int>, std::pair<uint32_t, uint32_t>> tuned_; // view size -> tile shape the autotuner chose
   uint64_t batchTunedKey_ = 0; // sizes_key of the batch autotune_views last tuned

@@ -35,7 +35,7 @@ EXPORTS = [
     "vsplat_parameters", "vsplat_gaussian_count", "vsplat_sh_degree", "vsplat_active_sh_degree",
     "vsplat_uses_fixed_point", "vsplat_gpu_bytes", "vsplat_bytes_per_gaussian", "vsplat_staging_bytes",
     "vsplat_timings", "vsplat_last_error", "vsplat_enqueue", "vsplat_synchronize", "vsplat_stream",
-    "vsplat_reserve_entries", "vsplat_inspect", "vsplat_math_probe", "vsplat_reset_timings",
+    "vsplat_reserve_entries", "vsplat_inspect", "vsplat_inspect_view", "vsplat_math_probe", "vsplat_reset_timings",
     "vsplat_set_stage_timing", "vsplat_step_views", "vsplat_enqueue_views", "vsplat_synchronize_views",
     "vsplat_capture_views", "vsplat_launch_captured", "vsplat_release_captured",
     "vsplat_apply_changes", "vsplat_device_alloc", "vsplat_device_read", "vsplat_device_copy", "vsplat_device_free",
@@ -141,6 +141,7 @@ def lib():
         L.vsplat_stream.restype = vp
         L.vsplat_reserve_entries.argtypes = [vp, C.c_uint64]
         L.vsplat_inspect.argtypes = [vp, C.c_char_p, vp, C.c_uint64, vp]
+        L.vsplat_inspect_view.argtypes = [vp, C.c_uint32, C.c_char_p, vp, C.c_uint64, vp]
         L.vsplat_math_probe.argtypes = [C.c_int32, vp, vp, C.c_uint64]
         L.vsplat_reset_timings.argtypes = [vp]
         L.vsplat_set_stage_timing.argtypes = [vp, C.c_int32]
@@ -566,15 +567,17 @@ class Trainer:
     def reserve_entries(self, entries: int):
         _raise(self._lib.vsplat_reserve_entries(self._h, entries), "reserve_entries")
 
-    def inspect(self, name: str) -> np.ndarray:
+    def inspect(self, name: str, view: int = 0) -> np.ndarray:
+        """An intermediate buffer as the last call left it; `view` picks a batch's view."""
         size = C.c_uint64()
-        _raise(self._lib.vsplat_inspect(self._h, name.encode(), None, 0, C.byref(size)), "inspect")
+        L = self._lib
+        _raise(L.vsplat_inspect_view(self._h, view, name.encode(), None, 0, C.byref(size)), "inspect")
         if name == "raw":
             dtype, width = (np.int64, 14) if self.uses_fixed_point() else (np.float32, 14)
         else:
             dtype, width = INSPECT_TYPES[name]
         out = np.zeros(size.value // np.dtype(dtype).itemsize, dtype)
-        _raise(self._lib.vsplat_inspect(self._h, name.encode(), _p(out), out.nbytes, C.byref(size)), "inspect")
+        _raise(L.vsplat_inspect_view(self._h, view, name.encode(), _p(out), out.nbytes, C.byref(size)), "inspect")
         return out.reshape(-1, width) if width > 1 else out
 
 
