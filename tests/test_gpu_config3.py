@@ -82,7 +82,12 @@ def test_config3_captured_step_bit_exact_at_20k():
     finally:
         for d in buffers:
             d.free()
-    assert [(s.visible_splats, s.tile_entries) for s in stats] == counts
+    # Oracle B lists 16x16 tiles; the entry count of a view the tuner gave
+    # another shape differs by construction (its sums and steps do not).
+    assert [s.visible_splats for s in stats] == [v for v, _ in counts]
+    for shape, s, (_, entries) in zip(shapes, stats, counts):
+        if tuple(shape) == (16, 16):
+            assert s.tile_entries == entries
     np.testing.assert_array_equal(_bits(c.positions), _bits(pos))
     np.testing.assert_array_equal(_bits(c.scales), _bits(sc))
     np.testing.assert_array_equal(_bits(c.rotations), _bits(rot))
