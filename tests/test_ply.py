@@ -97,3 +97,58 @@ def test_files_that_are_not_splats_are_refused(tmp_path):
         vsplat.load_ply(cut)
     with pytest.raises(RuntimeError, match="Cannot open"):
         vsplat.load_ply(tmp_path / "absent.ply")
+
+
+_SPLAT_PROPS = "".join(f"property float {p}\n" for p in ("x", "y", "z", "f_dc_0", "f_dc_1", "f_dc_2", "opacity", "scale_0", "scale_1",
+                                                            "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"))
+
+
+@pytest.mark.parametrize("header, message", [
+    # The header rules of gaussian_cloud.cpp:96-170, each with its message.
+    ("ply\nformat binary_big_endian 1.0\nelement vertex 1\n" + _SPLAT_PROPS, "unsupported PLY format 'binary_big_endian'"),
+    ("ply\nformat ascii 1.0\nelement face 2\nproperty list uchar int vertex_indices\nelement vertex 1\n" + _SPLAT_PROPS,
+     "element 'face' before the vertex element is not supported"),
+    ("ply\nformat ascii 1.0\nelement vertex 1\n" + _SPLAT_PROPS + "element vertex 1\n", "more than one vertex element"),
+    ("ply\nformat ascii 1.0\nelement vertex 1\nproperty list uchar int idx\n" + _SPLAT_PROPS,
+     "list properties on vertices are not supported"),
+    ("ply\nformat ascii 1.0\nelement vertex 1\nproperty half x\n" + _SPLAT_PROPS, "unknown property type 'half'"),
+    ("ply\nelement vertex 1\n" + _SPLAT_PROPS, "PLY header has no format or vertex element"),
+    ("ply\nformat ascii 1.0\nelement vertex 1\n" + _SPLAT_PROPS + "property float f_rest_0\n",
+     "unexpected number of f_rest properties \\(1\\)"),
+])
+def test_header_errors_carry_the_reference_messages(tmp_path, header, message):
+    path = tmp_path / "bad.ply"
+    path.write_text(header + "end_header\n" + " ".join(["0"] * 14) + "\n")
+    with pytest.raises(RuntimeError, match=message):
+        vsplat.load_ply(path)
+
+
+def test_empty_elements_before_the_vertices_and_ascii_layout_are_accepted(tmp_path):
+    # An empty element before the vertices is allowed; ASCII values may wrap
+    # lines, carry a '+' sign, and properties after the vertex element are skipped.
+    path = tmp_path / "loose.ply"
+    head = "ply\r\nformat ascii 1.0\r\nelement camera 0\r\nelement vertex 2\r\n" + _SPLAT_PROPS.replace("\n", "\r\n")
+    head += "element face 0\r\nproperty list uchar int vertex_indices\r\nend_header\r\n"
+    body = "1 2 3 +0.5 0.25\n-0.125 4 -1 -2 -3 1 0 0 0\n\n  5 6 7 1e-1 2E1 3 0.5 -4 -5 -6 0 1 0 0"
+    path.write_bytes((head + body).encode())
+    c = vsplat.load_ply(path)
+    assert c.count == 2 and c.sh_degree == 0
+    np.testing.assert_array_equal(c.positions, [[1, 2, 3], [5, 6, 7]])
+    np.testing.assert_array_equal(c.sh[:, 0], np.array([[0.5, 0.25, -0.125], [0.1, 20, 3]], np.float32))
+    np.testing.assert_array_equal(c.opacities, [4, 0.5])
+    np.testing.assert_array_equal(c.rotations, [[1, 0, 0, 0], [0, 1, 0, 0]])
+    cut = tmp_path / "cut.ply"
+    cut.write_bytes((head + body.rsplit(" ", 3)[0]).encode())
+    with pytest.raises(RuntimeError, match="truncated"):
+        vsplat.load_ply(cut)
+
+
+def test_large_binary_file_decoded_over_threads(tmp_path):
+    # More than one decoding thread's share of records (65536 each).
+    c = _cloud(3, 200_003)
+    path = tmp_path / "large.ply"
+    vsplat.save_ply(c, path)
+    assert path.read_bytes() == P.write_bytes(c.positions, c.scales, c.rotations, c.opacities, c.sh, 3)
+    loaded = vsplat.load_ply(path)
+    for name in ("positions", "scales", "rotations", "opacities", "sh"):
+        np.testing.assert_array_equal(_bits(getattr(loaded, name)), _bits(getattr(c, name)))

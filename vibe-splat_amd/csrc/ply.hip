@@ -1,280 +1,372 @@
-// 3DGS .ply files (src/splat/gaussian_cloud.cpp:95-365): the header parsed
-// as the reference parses it, binary little-endian or ASCII vertex records
-// read into a cloud's arrays, and a cloud written in the standard 3DGS binary
-// layout. Host code: a file is read once, in chunks, and each record is
-// scattered into the arrays through a table of (offset, type) per value,
-// built once from the header, instead of a lookup per property per vertex.
+// 3DGS .ply files. The rules (header keywords, accepted scalar types, the
+// property names, channel-major f_rest, the written layout) and the error
+// messages are the reference's load_ply / save_ply
+// (src/splat/gaussian_cloud.cpp:95-365); the code is organised its own way:
+//
+// * Reading maps the file and works on its bytes in place: the header is cut
+//   into lines and words without copies, and interpreting it yields a schema
+//   (format, vertex count, the vertex record's properties with their byte
+//   offsets and scalar types). A plan then names, for every float of a
+//   Gaussian in GaussianCloud's layout, the property it comes from, and the
+//   records are decoded through that plan, split over host threads for large
+//   binary files. Truncation is known before any record is decoded.
+// * Writing describes the output as a table of columns (name, source array,
+//   stride, component); the same table makes the header lines and fills the
+//   records, a chunk of vertices at a time.
 #include "ply.hpp"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
-#include <cmath>
+#include <cctype>
+#include <charconv>
+#include <cstdio>
 #include <cstring>
-#include <fstream>
-#include <sstream>
+#include <string_view>
+#include <thread>
 #include <unordered_map>
 
 namespace vsplat {
 
 namespace {
 
-// PLY scalar types (gaussian_cloud.cpp:34-58).
-enum class PlyType : uint8_t { I8, U8, I16, U16, I32, U32, F32, F64 };
+// A scalar type's names, size and conversion to float. The library is built
+// for x86-64 only, whose byte order is the file's (little-endian).
+using Converter = float (*)(const unsigned char *);
 
-bool parse_type(const std::string &t, PlyType &out) {
-  if (t == "char" || t == "int8") out = PlyType::I8;
-  else if (t == "uchar" || t == "uint8") out = PlyType::U8;
-  else if (t == "short" || t == "int16") out = PlyType::I16;
-  else if (t == "ushort" || t == "uint16") out = PlyType::U16;
-  else if (t == "int" || t == "int32") out = PlyType::I32;
-  else if (t == "uint" || t == "uint32") out = PlyType::U32;
-  else if (t == "float" || t == "float32") out = PlyType::F32;
-  else if (t == "double" || t == "float64") out = PlyType::F64;
-  else return false;
-  return true;
+template <class T> float convert(const unsigned char *p) {
+  T v;
+  std::memcpy(&v, p, sizeof(T));
+  return static_cast<float>(v);
 }
 
-size_t type_size(PlyType t) {
-  switch (t) {
-  case PlyType::I8:
-  case PlyType::U8: return 1;
-  case PlyType::I16:
-  case PlyType::U16: return 2;
-  case PlyType::F64: return 8;
-  default: return 4;
-  }
-}
-
-// A little-endian value of the given type at p, as a float (gaussian_cloud.cpp:60-81).
-float read_value(PlyType t, const unsigned char *p) {
-  switch (t) {
-  case PlyType::I8: return static_cast<float>(static_cast<int8_t>(p[0]));
-  case PlyType::U8: return static_cast<float>(p[0]);
-  case PlyType::I16: return static_cast<float>(static_cast<int16_t>(static_cast<uint16_t>(p[0] | (p[1] << 8))));
-  case PlyType::U16: return static_cast<float>(static_cast<uint16_t>(p[0] | (p[1] << 8)));
-  case PlyType::I32:
-  case PlyType::U32:
-  case PlyType::F32: {
-    const uint32_t u = static_cast<uint32_t>(p[0]) | (static_cast<uint32_t>(p[1]) << 8) | (static_cast<uint32_t>(p[2]) << 16) |
-                       (static_cast<uint32_t>(p[3]) << 24);
-    if (t == PlyType::I32) return static_cast<float>(static_cast<int32_t>(u));
-    if (t == PlyType::U32) return static_cast<float>(u);
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-  }
-  case PlyType::F64: {
-    uint64_t u = 0;
-    for (int k = 7; k >= 0; --k) u = (u << 8) | p[k];
-    double d;
-    std::memcpy(&d, &u, 8);
-    return static_cast<float>(d);
-  }
-  }
-  return 0.0f;
-}
-
-struct Property {
-  std::string name;
-  PlyType type;
-  size_t offset;
+struct ScalarType {
+  std::string_view name, alias;
+  uint32_t bytes;
+  Converter decode;
 };
 
-struct Header {
-  bool binary = true;
-  size_t vertexCount = 0;
-  std::vector<Property> props;
-  size_t stride = 0;
+constexpr ScalarType kScalarTypes[] = {
+    {"char", "int8", 1, convert<int8_t>},       {"uchar", "uint8", 1, convert<uint8_t>},
+    {"short", "int16", 2, convert<int16_t>},    {"ushort", "uint16", 2, convert<uint16_t>},
+    {"int", "int32", 4, convert<int32_t>},      {"uint", "uint32", 4, convert<uint32_t>},
+    {"float", "float32", 4, convert<float>},    {"double", "float64", 8, convert<double>},
 };
+
+const ScalarType *scalar_type(std::string_view name) {
+  for (const ScalarType &t : kScalarTypes)
+    if (name == t.name || name == t.alias) return &t;
+  return nullptr;
+}
 
 [[noreturn]] void fail(const std::string &path, const std::string &what) { throw Error(VSPLAT_RUNTIME_ERROR, path + ": " + what); }
 
-// gaussian_cloud.cpp:96-170.
-Header parse_header(std::istream &in, const std::string &path) {
-  std::string line;
-  if (!std::getline(in, line) || line.substr(0, 3) != "ply") fail(path, "not a PLY file");
-  Header h;
-  bool inVertex = false, sawVertex = false, sawFormat = false;
-  while (std::getline(in, line)) {
-    if (!line.empty() && line.back() == '\r') line.pop_back();
-    std::istringstream ls(line);
-    std::string word;
-    ls >> word;
-    if (word == "end_header") break;
-    if (word == "comment" || word == "obj_info" || word.empty()) continue;
-    if (word == "format") {
-      std::string format;
-      ls >> format;
-      if (format == "binary_little_endian") h.binary = true;
-      else if (format == "ascii") h.binary = false;
-      else fail(path, "unsupported PLY format '" + format + "'");
-      sawFormat = true;
-    } else if (word == "element") {
-      std::string name;
-      size_t count = 0;
-      ls >> name >> count;
-      if (name == "vertex") {
-        if (sawVertex) fail(path, "more than one vertex element");
-        inVertex = sawVertex = true;
-        h.vertexCount = count;
-      } else {
-        if (!sawVertex && count > 0) fail(path, "element '" + name + "' before the vertex element is not supported");
-        inVertex = false;
-      }
-    } else if (word == "property") {
-      if (!inVertex) continue; // properties of elements after "vertex" are never read
-      std::string typeName, name;
-      ls >> typeName;
-      if (typeName == "list") fail(path, "list properties on vertices are not supported");
-      ls >> name;
-      PlyType type;
-      if (!parse_type(typeName, type)) fail(path, "unknown property type '" + typeName + "'");
-      h.props.push_back({name, type, h.stride});
-      h.stride += type_size(type);
+// A read-only mapping of a whole file (empty for an empty or irregular file).
+class MappedFile {
+public:
+  explicit MappedFile(const std::string &path) {
+    fd_ = ::open(path.c_str(), O_RDONLY | O_CLOEXEC);
+    if (fd_ < 0) throw Error(VSPLAT_RUNTIME_ERROR, "Cannot open '" + path + "'");
+    struct stat st {};
+    if (::fstat(fd_, &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0) {
+      void *p = ::mmap(nullptr, static_cast<size_t>(st.st_size), PROT_READ, MAP_PRIVATE, fd_, 0);
+      if (p == MAP_FAILED) throw Error(VSPLAT_RUNTIME_ERROR, "Cannot open '" + path + "'");
+      data_ = p;
+      size_ = static_cast<size_t>(st.st_size);
+      (void)::madvise(data_, size_, MADV_SEQUENTIAL);
     }
   }
-  if (!sawFormat || !sawVertex) fail(path, "PLY header has no format or vertex element");
-  return h;
-}
+  MappedFile(const MappedFile &) = delete;
+  MappedFile &operator=(const MappedFile &) = delete;
+  ~MappedFile() {
+    if (data_ != nullptr) ::munmap(data_, size_);
+    if (fd_ >= 0) ::close(fd_);
+  }
+  std::string_view text() const { return {static_cast<const char *>(data_), size_}; }
+  const unsigned char *bytes() const { return static_cast<const unsigned char *>(data_); }
 
-// Where each value of a Gaussian comes from in a record: the cloud's
-// positions (3), scales (3), rotations (4), opacity, then the SH in the
-// cloud's layout (coefficient-major, channels inner), from f_dc_* and the
-// channel-major f_rest_* (gaussian_cloud.cpp:236-275, 289-305).
-struct Layout {
-  int degree = 0;
-  uint32_t coefficients = 1;
-  std::vector<uint32_t> source; // property index per value
+private:
+  int fd_ = -1;
+  void *data_ = nullptr;
+  size_t size_ = 0;
 };
 
-Layout layout_of(const Header &h, const std::string &path) {
-  std::unordered_map<std::string, uint32_t> index;
-  for (uint32_t i = 0; i < h.props.size(); ++i) index[h.props[i].name] = i;
-  const auto require = [&](const std::string &name) {
-    const auto it = index.find(name);
-    if (it == index.end()) fail(path, "missing property '" + name + "' (is this a Gaussian splat file?)");
-    return it->second;
-  };
-  size_t rest = 0;
-  while (index.count("f_rest_" + std::to_string(rest)) != 0) ++rest;
-  Layout l;
-  switch (rest) {
-  case 0: l.degree = 0; break;
-  case 9: l.degree = 1; break;
-  case 24: l.degree = 2; break;
-  case 45: l.degree = 3; break;
-  default: fail(path, "unexpected number of f_rest properties (" + std::to_string(rest) + ")");
+constexpr std::string_view kBlank = " \t\r\v\f";
+
+// Up to `most` whitespace-separated words of a line.
+size_t split_words(std::string_view line, std::string_view *words, size_t most) {
+  size_t n = 0, at = 0;
+  while (n < most) {
+    const size_t begin = line.find_first_not_of(kBlank, at);
+    if (begin == std::string_view::npos) break;
+    const size_t end = std::min(line.find_first_of(kBlank, begin), line.size());
+    words[n++] = line.substr(begin, end - begin);
+    at = end;
   }
-  l.coefficients = static_cast<uint32_t>((l.degree + 1) * (l.degree + 1));
-  for (const char *name : {"x", "y", "z", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3", "opacity"}) {
-    l.source.push_back(require(name));
-  }
-  for (uint32_t k = 0; k < l.coefficients; ++k) {
-    for (uint32_t c = 0; c < 3; ++c) {
-      l.source.push_back(k == 0 ? require("f_dc_" + std::to_string(c)) : index.at("f_rest_" + std::to_string(c * (l.coefficients - 1) + (k - 1))));
-    }
-  }
-  return l;
+  return n;
 }
 
-// One record's values into the cloud's arrays at Gaussian v.
-void store(const std::vector<float> &record, const Layout &l, size_t v, const PlyCloud &out) {
-  const uint32_t *s = l.source.data();
-  for (int a = 0; a < 3; ++a) out.positions[v * 3 + a] = record[s[a]];
-  for (int a = 0; a < 3; ++a) out.scales[v * 3 + a] = record[s[3 + a]];
-  for (int a = 0; a < 4; ++a) out.rotations[v * 4 + a] = record[s[6 + a]];
-  out.opacities[v] = record[s[10]];
-  const size_t shFloats = static_cast<size_t>(l.coefficients) * 3;
-  for (size_t j = 0; j < shFloats; ++j) out.sh[v * shFloats + j] = record[s[11 + j]];
+struct VertexProperty {
+  std::string name;
+  const ScalarType *type;
+  size_t offset; // in the binary record
+};
+
+struct Schema {
+  bool ascii = false;
+  uint64_t vertices = 0;
+  std::vector<VertexProperty> properties;
+  size_t stride = 0; // bytes per binary record
+  size_t body = 0;   // where the records start
+};
+
+// The header's lines, interpreted in order with the reference's rules: only
+// the vertex element's properties are kept, an element with entries before it
+// or a list property on it is refused, and the first offending line decides
+// the message.
+Schema read_schema(std::string_view file, const std::string &path) {
+  size_t at = 0;
+  const auto next_line = [&](std::string_view &line) {
+    if (at >= file.size()) return false;
+    const size_t end = std::min(file.find('\n', at), file.size());
+    line = file.substr(at, end - at);
+    at = end < file.size() ? end + 1 : end;
+    return true;
+  };
+  std::string_view line;
+  if (!next_line(line) || line.substr(0, 3) != "ply") fail(path, "not a PLY file");
+  Schema s;
+  enum class Section { None, Vertex, Other } section = Section::None;
+  bool haveFormat = false, haveVertex = false;
+  while (next_line(line)) {
+    std::string_view w[3];
+    const size_t words = split_words(line, w, 3);
+    const std::string_view key = words > 0 ? w[0] : std::string_view{};
+    if (key == "end_header") break;
+    if (key.empty() || key == "comment" || key == "obj_info") continue;
+    if (key == "format") {
+      if (w[1] == "ascii") s.ascii = true;
+      else if (w[1] == "binary_little_endian") s.ascii = false;
+      else fail(path, "unsupported PLY format '" + std::string(w[1]) + "'");
+      haveFormat = true;
+    } else if (key == "element") {
+      uint64_t entries = 0;
+      if (words > 2) (void)std::from_chars(w[2].data(), w[2].data() + w[2].size(), entries);
+      if (w[1] == "vertex") {
+        if (haveVertex) fail(path, "more than one vertex element");
+        haveVertex = true;
+        section = Section::Vertex;
+        s.vertices = entries;
+      } else {
+        if (!haveVertex && entries > 0) fail(path, "element '" + std::string(w[1]) + "' before the vertex element is not supported");
+        section = Section::Other;
+      }
+    } else if (key == "property" && section == Section::Vertex) {
+      if (w[1] == "list") fail(path, "list properties on vertices are not supported");
+      const ScalarType *type = scalar_type(w[1]);
+      if (type == nullptr) fail(path, "unknown property type '" + std::string(w[1]) + "'");
+      s.properties.push_back({std::string(w[2]), type, s.stride});
+      s.stride += type->bytes;
+    }
+  }
+  if (!haveFormat || !haveVertex) fail(path, "PLY header has no format or vertex element");
+  s.body = at;
+  return s;
+}
+
+// Which property each float of a Gaussian comes from, in GaussianCloud's
+// order: position (3), log-scale (3), rotation (4), logit opacity, then the SH
+// coefficient-major with channels inner, from f_dc_* and the channel-major
+// f_rest_*. The SH degree follows from how many f_rest_i there are.
+struct Plan {
+  int degree = 0;
+  uint32_t coefficients = 1;
+  std::vector<uint32_t> from;
+  uint32_t floats() const { return static_cast<uint32_t>(from.size()); }
+};
+
+Plan plan_for(const Schema &s, const std::string &path) {
+  std::unordered_map<std::string_view, uint32_t> where; // a repeated name: its last property
+  for (uint32_t i = 0; i < s.properties.size(); ++i) where[s.properties[i].name] = i;
+  const auto property = [&](const std::string &name) {
+    const auto found = where.find(name);
+    if (found == where.end()) fail(path, "missing property '" + name + "' (is this a Gaussian splat file?)");
+    return found->second;
+  };
+  uint32_t rest = 0;
+  while (where.count("f_rest_" + std::to_string(rest)) != 0) ++rest;
+  Plan p;
+  constexpr std::pair<uint32_t, int> kDegreeOfRest[] = {{0u, 0}, {9u, 1}, {24u, 2}, {45u, 3}};
+  const auto degree = std::find_if(std::begin(kDegreeOfRest), std::end(kDegreeOfRest), [&](const auto &d) { return d.first == rest; });
+  if (degree == std::end(kDegreeOfRest)) fail(path, "unexpected number of f_rest properties (" + std::to_string(rest) + ")");
+  p.degree = degree->second;
+  p.coefficients = static_cast<uint32_t>((p.degree + 1) * (p.degree + 1));
+  for (const char *name : {"x", "y", "z", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3", "opacity"})
+    p.from.push_back(property(name));
+  std::vector<uint32_t> dc;
+  for (int c = 0; c < 3; ++c) dc.push_back(property("f_dc_" + std::to_string(c)));
+  const uint32_t perChannel = p.coefficients - 1;
+  for (uint32_t k = 0; k < p.coefficients; ++k)
+    for (uint32_t c = 0; c < 3; ++c) p.from.push_back(k == 0 ? dc[c] : where.at("f_rest_" + std::to_string(c * perChannel + k - 1)));
+  return p;
+}
+
+// The destination of float j of Gaussian v: one of the cloud's five arrays.
+struct Destination {
+  float *base;
+  uint32_t width; // floats per Gaussian in that array
+  uint32_t index; // which of them
+};
+
+std::vector<Destination> destinations(const PlyCloud &out, const Plan &p) {
+  std::vector<Destination> d;
+  for (uint32_t a = 0; a < 3; ++a) d.push_back({out.positions, 3, a});
+  for (uint32_t a = 0; a < 3; ++a) d.push_back({out.scales, 3, a});
+  for (uint32_t a = 0; a < 4; ++a) d.push_back({out.rotations, 4, a});
+  d.push_back({out.opacities, 1, 0});
+  const uint32_t shFloats = 3 * p.coefficients;
+  for (uint32_t j = 0; j < shFloats; ++j) d.push_back({out.sh, shFloats, j});
+  return d;
+}
+
+// Binary records [first, last) through the plan.
+void decode_binary(const unsigned char *records, const Schema &s, const Plan &p, const std::vector<Destination> &to, uint64_t first,
+                   uint64_t last) {
+  const uint32_t floats = p.floats();
+  std::vector<Converter> decode(floats);
+  std::vector<size_t> offset(floats);
+  for (uint32_t j = 0; j < floats; ++j) {
+    decode[j] = s.properties[p.from[j]].type->decode;
+    offset[j] = s.properties[p.from[j]].offset;
+  }
+  for (uint64_t v = first; v < last; ++v) {
+    const unsigned char *record = records + v * s.stride;
+    for (uint32_t j = 0; j < floats; ++j) to[j].base[v * to[j].width + to[j].index] = decode[j](record + offset[j]);
+  }
+}
+
+// The next number of an ASCII body, as iostream extraction of a double reads
+// it (optional sign, digits, point, exponent), then narrowed to float.
+bool next_number(const char *&at, const char *end, float &value) {
+  while (at < end && std::isspace(static_cast<unsigned char>(*at))) ++at;
+  if (at == end) return false;
+  const char *start = at;
+  if (*start == '+') ++start;
+  if (start == end || !(std::isdigit(static_cast<unsigned char>(*start)) || *start == '.' || *start == '-')) return false;
+  double d = 0.0;
+  const auto parsed = std::from_chars(start, end, d);
+  if (parsed.ec != std::errc()) return false;
+  at = parsed.ptr;
+  value = static_cast<float>(d);
+  return true;
 }
 
 struct Opened {
-  std::ifstream in;
-  Header header;
-  Layout layout;
+  MappedFile file;
+  Schema schema;
+  Plan plan;
+  Opened(const std::string &path) : file(path), schema(read_schema(file.text(), path)), plan(plan_for(schema, path)) {}
 };
-
-void open_ply(const std::string &path, Opened &f) {
-  f.in.open(path, std::ios::binary);
-  if (!f.in) throw Error(VSPLAT_RUNTIME_ERROR, "Cannot open '" + path + "'");
-  f.header = parse_header(f.in, path);
-  f.layout = layout_of(f.header, path);
-}
 
 } // namespace
 
 PlyInfo ply_info(const std::string &path) {
-  Opened f;
-  open_ply(path, f);
-  return PlyInfo{f.header.vertexCount, f.layout.degree};
+  const Opened f(path);
+  return PlyInfo{static_cast<size_t>(f.schema.vertices), f.plan.degree};
 }
 
-// load_ply (gaussian_cloud.cpp:236-327), into caller arrays sized from ply_info.
 void read_ply(const std::string &path, const PlyCloud &out) {
-  Opened f;
-  open_ply(path, f);
-  const Header &h = f.header;
-  std::vector<float> record(h.props.size());
-  if (h.binary) {
-    constexpr size_t kChunk = 65536;
-    std::vector<unsigned char> buffer(h.stride * kChunk);
-    for (size_t first = 0; first < h.vertexCount; first += kChunk) {
-      const size_t n = std::min(kChunk, h.vertexCount - first);
-      f.in.read(reinterpret_cast<char *>(buffer.data()), static_cast<std::streamsize>(n * h.stride));
-      if (static_cast<size_t>(f.in.gcount()) != n * h.stride) fail(path, "file is truncated");
-      for (size_t i = 0; i < n; ++i) {
-        const unsigned char *rec = buffer.data() + i * h.stride;
-        for (size_t p = 0; p < h.props.size(); ++p) record[p] = read_value(h.props[p].type, rec + h.props[p].offset);
-        store(record, f.layout, first + i, out);
-      }
+  const Opened f(path);
+  const Schema &s = f.schema;
+  const std::vector<Destination> to = destinations(out, f.plan);
+  const std::string_view text = f.file.text();
+  if (!s.ascii) {
+    const uint64_t available = text.size() - s.body;
+    if (s.stride != 0 && s.vertices > available / s.stride) fail(path, "file is truncated");
+    const unsigned char *records = f.file.bytes() + s.body;
+    // Large files are decoded by several host threads, each a contiguous range of records.
+    const uint64_t perThread = uint64_t{1} << 16;
+    const uint32_t threads = static_cast<uint32_t>(std::min<uint64_t>({8, std::max(1u, std::thread::hardware_concurrency()),
+                                                                       (s.vertices + perThread - 1) / perThread}));
+    if (threads <= 1) {
+      decode_binary(records, s, f.plan, to, 0, s.vertices);
+      return;
     }
-  } else {
-    for (size_t v = 0; v < h.vertexCount; ++v) {
-      for (size_t p = 0; p < h.props.size(); ++p) {
-        double d = 0.0;
-        if (!(f.in >> d)) fail(path, "file is truncated");
-        record[p] = static_cast<float>(d);
-      }
-      store(record, f.layout, v, out);
+    std::vector<std::thread> pool;
+    const uint64_t share = (s.vertices + threads - 1) / threads;
+    for (uint32_t t = 0; t < threads; ++t) {
+      const uint64_t first = std::min(s.vertices, t * share), last = std::min(s.vertices, first + share);
+      pool.emplace_back(decode_binary, records, std::cref(s), std::cref(f.plan), std::cref(to), first, last);
     }
+    for (std::thread &t : pool) t.join();
+    return;
+  }
+  // ASCII: every property of a record is read (kept or not), in order.
+  const char *at = text.data() + s.body, *end = text.data() + text.size();
+  std::vector<float> record(s.properties.size());
+  for (uint64_t v = 0; v < s.vertices; ++v) {
+    for (float &value : record)
+      if (!next_number(at, end, value)) fail(path, "file is truncated");
+    for (uint32_t j = 0; j < f.plan.floats(); ++j) to[j].base[v * to[j].width + to[j].index] = record[f.plan.from[j]];
   }
 }
 
-// save_ply (gaussian_cloud.cpp:330-365): binary little-endian, the standard
-// 3DGS property order, zero normals, f_rest channel-major.
 void write_ply(const std::string &path, const ConstPlyCloud &c) {
   if (c.shDegree < 0 || c.shDegree > 3) throw Error(VSPLAT_RUNTIME_ERROR, "GaussianCloud: SH degree must be 0..3");
-  std::ofstream out(path, std::ios::binary);
-  if (!out) throw Error(VSPLAT_RUNTIME_ERROR, "Cannot write '" + path + "'");
-  const size_t n = c.count, coeffs = static_cast<size_t>((c.shDegree + 1) * (c.shDegree + 1)), rest = 3 * (coeffs - 1);
-  std::string header = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(n) + "\n";
-  for (const char *name : {"x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"}) header += std::string("property float ") + name + "\n";
-  for (size_t i = 0; i < rest; ++i) header += "property float f_rest_" + std::to_string(i) + "\n";
-  for (const char *name : {"opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"})
-    header += std::string("property float ") + name + "\n";
+  const uint32_t coefficients = static_cast<uint32_t>((c.shDegree + 1) * (c.shDegree + 1)), shFloats = 3 * coefficients;
+  // The standard 3DGS record: position, zero normals, f_dc, f_rest channel-major
+  // (all of red's higher coefficients, then green's, then blue's), opacity,
+  // log-scales, rotation; every property a float.
+  struct Column {
+    std::string name;
+    const float *base; // value of vertex v: base[v * width + index]
+    uint32_t width, index;
+  };
+  static const float kZero = 0.0f; // the normals: width 0 reads it for every vertex
+  std::vector<Column> columns;
+  for (uint32_t a = 0; a < 3; ++a) columns.push_back({std::string(1, "xyz"[a]), c.positions, 3, a});
+  for (uint32_t a = 0; a < 3; ++a) columns.push_back({std::string("n") + "xyz"[a], &kZero, 0, 0});
+  for (uint32_t ch = 0; ch < 3; ++ch) columns.push_back({"f_dc_" + std::to_string(ch), c.sh, shFloats, ch});
+  for (uint32_t ch = 0; ch < 3; ++ch)
+    for (uint32_t k = 1; k < coefficients; ++k)
+      columns.push_back({"f_rest_" + std::to_string(ch * (coefficients - 1) + k - 1), c.sh, shFloats, k * 3 + ch});
+  columns.push_back({"opacity", c.opacities, 1, 0});
+  for (uint32_t a = 0; a < 3; ++a) columns.push_back({"scale_" + std::to_string(a), c.scales, 3, a});
+  for (uint32_t a = 0; a < 4; ++a) columns.push_back({"rot_" + std::to_string(a), c.rotations, 4, a});
+
+  std::string header = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(c.count) + "\n";
+  for (const Column &col : columns) header += "property float " + col.name + "\n";
   header += "end_header\n";
-  out.write(header.data(), static_cast<std::streamsize>(header.size()));
-  const size_t per = 9 + rest + 8;
-  constexpr size_t kChunk = 65536;
-  std::vector<float> records(per * std::min(n, kChunk));
-  for (size_t first = 0; first < n; first += kChunk) {
-    const size_t m = std::min(kChunk, n - first);
-    for (size_t i = 0; i < m; ++i) {
-      const size_t v = first + i;
-      float *r = records.data() + i * per;
-      size_t k = 0;
-      for (int a = 0; a < 3; ++a) r[k++] = c.positions[v * 3 + a];
-      for (int a = 0; a < 3; ++a) r[k++] = 0.0f; // normals
-      for (int a = 0; a < 3; ++a) r[k++] = c.sh[v * coeffs * 3 + a];
-      for (size_t ch = 0; ch < 3 && coeffs > 1; ++ch)
-        for (size_t j = 1; j < coeffs; ++j) r[k++] = c.sh[(v * coeffs + j) * 3 + ch];
-      r[k++] = c.opacities[v];
-      for (int a = 0; a < 3; ++a) r[k++] = c.scales[v * 3 + a];
-      for (int a = 0; a < 4; ++a) r[k++] = c.rotations[v * 4 + a];
-    }
-    // This library is built for little-endian hosts only (x86-64 with gfx950).
-    out.write(reinterpret_cast<const char *>(records.data()), static_cast<std::streamsize>(m * per * sizeof(float)));
+
+  std::FILE *file = std::fopen(path.c_str(), "wb");
+  if (file == nullptr) throw Error(VSPLAT_RUNTIME_ERROR, "Cannot write '" + path + "'");
+  bool ok = std::fwrite(header.data(), 1, header.size(), file) == header.size();
+  const size_t width = columns.size();
+  // The columns' sources as (array, component), the arrays' rows found once per vertex.
+  const float *arrays[] = {c.positions, &kZero, c.sh, c.opacities, c.scales, c.rotations};
+  const uint32_t widths[] = {3, 0, shFloats, 1, 3, 4};
+  std::vector<std::pair<uint32_t, uint32_t>> from;
+  for (const Column &col : columns) {
+    const uint32_t a = static_cast<uint32_t>(std::find(std::begin(arrays), std::end(arrays), col.base) - std::begin(arrays));
+    from.push_back({a, col.index});
   }
-  if (!out) throw Error(VSPLAT_RUNTIME_ERROR, "Failed writing '" + path + "'");
+  constexpr size_t kVerticesPerChunk = size_t{1} << 14;
+  std::vector<float> chunk(width * std::min(c.count, kVerticesPerChunk));
+  for (size_t first = 0; ok && first < c.count; first += kVerticesPerChunk) {
+    const size_t n = std::min(kVerticesPerChunk, c.count - first);
+    float *record = chunk.data();
+    for (size_t v = first; v < first + n; ++v) {
+      const float *row[6];
+      for (int a = 0; a < 6; ++a) row[a] = arrays[a] + v * widths[a];
+      for (const auto &f : from) *record++ = row[f.first][f.second];
+    }
+    ok = std::fwrite(chunk.data(), sizeof(float), n * width, file) == n * width;
+  }
+  ok = (std::fclose(file) == 0) && ok;
+  if (!ok) throw Error(VSPLAT_RUNTIME_ERROR, "Failed writing '" + path + "'");
 }
 
 } // namespace vsplat

@@ -205,12 +205,7 @@ void Trainer::release_views() {
 }
 
 Trainer::~Trainer() {
-  {
-    std::lock_guard<std::mutex> lock(prefetchMutex_);
-    stopWorker_ = true;
-  }
-  prefetchChanged_.notify_all();
-  if (prefetchWorker_.joinable()) prefetchWorker_.join();
+  if (prefetchCopy_.valid()) prefetchCopy_.wait();
   release_captured();
   release_views();
   if (batchStart_ != nullptr) (void)hipEventDestroy(batchStart_);
@@ -802,33 +797,20 @@ vsplat_step_stats Trainer::make_stats(const uint32_t *counters, const double *su
 }
 
 void Trainer::wait_prefetch() {
-  std::unique_lock<std::mutex> lock(prefetchMutex_);
-  prefetchChanged_.wait(lock, [&] { return !prefetchPending_; });
+  if (prefetchCopy_.valid()) prefetchCopy_.get();
 }
 
+// Starts copying the next step's target into pinned memory on another host
+// thread; the step that takes this image waits for the copy and swaps the
+// pinned buffers instead of copying (stage_target).
 void Trainer::prefetch_target(const uint8_t *target, size_t bytes) {
   if (target == nullptr || bytes == 0) return;
-  wait_prefetch();
+  wait_prefetch(); // the pinned buffer may be resized below
   next_.ensure(bytes);
-  if (!prefetchWorker_.joinable()) {
-    prefetchWorker_ = std::thread([this] {
-      std::unique_lock<std::mutex> lock(prefetchMutex_);
-      for (;;) {
-        prefetchChanged_.wait(lock, [&] { return prefetchPending_ || stopWorker_; });
-        if (stopWorker_) return;
-        std::memcpy(next_.as<uint8_t>(), prefetched_, prefetchedBytes_);
-        prefetchPending_ = false;
-        prefetchChanged_.notify_all();
-      }
-    });
-  }
-  {
-    std::lock_guard<std::mutex> lock(prefetchMutex_);
-    prefetched_ = target;
-    prefetchedBytes_ = bytes;
-    prefetchPending_ = true;
-  }
-  prefetchChanged_.notify_all();
+  uint8_t *into = next_.as<uint8_t>();
+  prefetched_ = target;
+  prefetchedBytes_ = bytes;
+  prefetchCopy_ = std::async(std::launch::async, [into, target, bytes] { std::memcpy(into, target, bytes); });
 }
 
 void Trainer::stage_target(const uint8_t *target, size_t bytes) {

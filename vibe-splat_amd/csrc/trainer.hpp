@@ -5,7 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
-#include <condition_variable>
+#include <future>
 #include <map>
 #include <memory>
 #include <cstdint>
@@ -247,11 +247,10 @@ private:
   std::vector<std::unique_ptr<DeviceArray>> viewTargets_; // step_views' targets, one per view
   PinnedArray staging_, next_, readback_;
 
-  // Prefetch worker (trainer.cpp:539-564): copies the next step's target into pinned memory.
-  std::mutex prefetchMutex_;
-  std::condition_variable prefetchChanged_;
-  std::thread prefetchWorker_;
-  bool prefetchPending_ = false, stopWorker_ = false;
+  // The next step's target (the reference's prefetch, trainer.cpp:539-564):
+  // one asynchronous host copy into pinned memory (next_) per prefetch, of the
+  // caller's image at prefetched_, which stays borrowed until the next step.
+  std::future<void> prefetchCopy_;
   const uint8_t *prefetched_ = nullptr;
   size_t prefetchedBytes_ = 0;
 
