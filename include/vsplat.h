@@ -126,6 +126,22 @@ void vsplat_trainer_destroy(vsplat_trainer *trainer);
 
 /* Trainer::render (trainer.hpp:98): width*height*4 floats, rgb and log transmittance. */
 int vsplat_render(vsplat_trainer *trainer, const vsplat_camera *camera, float *out_rgba);
+/* No counterpart in the reference. vsplat_render without allocating, for
+ * callers that cannot allocate device memory (tight serving or viewer loops,
+ * harnesses that forbid hipMalloc). The image (width*height*4 floats, rgb and
+ * log transmittance, as vsplat_render) is written to device_rgba, device memory
+ * of capacity_bytes. The lists and per-pixel state live in one static device
+ * array per GPU of VSPLAT_NOALLOC_SCRATCH_BYTES (held until the process ends),
+ * so calls that overlap on one GPU race: make them one after another. Clouds,
+ * views and tile grids past the limits below are refused
+ * (VSPLAT_INVALID_ARGUMENT) before anything runs; a view that needs more tile
+ * entries returns VSPLAT_BUDGET_EXCEEDED. Synchronous, on the trainer's stream. */
+#define VSPLAT_NOALLOC_MAX_GAUSSIANS (1u << 17)
+#define VSPLAT_NOALLOC_MAX_ENTRIES (1u << 21)
+#define VSPLAT_NOALLOC_MAX_PIXELS (1u << 21)
+#define VSPLAT_NOALLOC_MAX_TILES (1u << 16)
+#define VSPLAT_NOALLOC_SCRATCH_BYTES (53ull << 20)
+int vsplat_render_noalloc(vsplat_trainer *trainer, const vsplat_camera *camera, float *device_rgba, uint64_t capacity_bytes);
 /* Trainer::compute_gradients (trainer.hpp:104): forward, loss, backward; the
  * gradients stay on the GPU for vsplat_get_gradients and vsplat_apply_gradients.
  * target is RGBA8, width*height*4 bytes. stats may be NULL. */

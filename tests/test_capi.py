@@ -69,6 +69,7 @@ def test_argument_errors_come_before_any_device_call():
     code, _ = _create(F.random_cloud(4, 0, 1), o)
     assert code == vsplat.INVALID_ARGUMENT
     assert lib.vsplat_render(None, None, None) == vsplat.INVALID_ARGUMENT
+    assert lib.vsplat_render_noalloc(None, None, None, 0) == vsplat.INVALID_ARGUMENT
     assert lib.vsplat_gaussian_count(None) == 0 and lib.vsplat_uses_fixed_point(None) == 0
     lib.vsplat_trainer_destroy(None)
 

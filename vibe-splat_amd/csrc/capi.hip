@@ -75,6 +75,15 @@ int vsplat_render(vsplat_trainer *t, const vsplat_camera *camera, float *out) {
   return guarded([&] { t->impl.render(*camera, out); });
 }
 
+int vsplat_render_noalloc(vsplat_trainer *t, const vsplat_camera *camera, float *device_rgba, uint64_t capacity_bytes) {
+  if (t == nullptr || camera == nullptr || device_rgba == nullptr) return invalid("null argument");
+  return guarded([&] {
+    uint64_t bytes = 0;
+    void *scratch = vsplat::noalloc_scratch(&bytes);
+    t->impl.render_into(*camera, device_rgba, capacity_bytes, scratch, bytes);
+  });
+}
+
 int vsplat_compute_gradients(vsplat_trainer *t, const vsplat_camera *camera, const uint8_t *target, size_t bytes, vsplat_step_stats *stats) {
   if (t == nullptr || camera == nullptr) return invalid("null argument");
   return guarded([&] {

@@ -82,14 +82,16 @@ uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 void DeviceArray::ensure(size_t bytes) {
   if (bytes <= bytes_ && ptr_ != nullptr) return;
+  if (!owned_) throw Error(VSPLAT_RUNTIME_ERROR, "a borrowed device array is too small");
   release();
   check(hipMalloc(&ptr_, std::max<size_t>(bytes, 256)), "hipMalloc");
   bytes_ = std::max<size_t>(bytes, 256);
 }
 void DeviceArray::release() {
-  if (ptr_ != nullptr) (void)hipFree(ptr_);
+  if (ptr_ != nullptr && owned_) (void)hipFree(ptr_);
   ptr_ = nullptr;
   bytes_ = 0;
+  owned_ = true;
 }
 void PinnedArray::ensure(size_t bytes) {
   if (bytes <= bytes_ && ptr_ != nullptr) return;
@@ -306,6 +308,19 @@ void Trainer::ensure_size(ViewBuffers &b, int width, int height) {
 void Trainer::ensure_layout(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH) {
   if (width <= 0 || height <= 0) throw Error(VSPLAT_RUNTIME_ERROR, "bad image size");
   if (width == b.width && height == b.height && tileW == b.tileW && tileH == b.tileH) return;
+  set_geometry(b, width, height, tileW, tileH);
+  const size_t pixels = static_cast<size_t>(width) * static_cast<size_t>(height);
+  b.image.ensure(pixels * 16);
+  b.stops.ensure(pixels * 4);
+  b.adjoint.ensure(pixels * 16);
+  b.fields.ensure(pixels * 36);
+  b.ranges.ensure(static_cast<size_t>(b.tiles) * 8);
+  b.tileTotals.ensure(static_cast<size_t>(b.lossTiles) * 8);
+}
+
+// The tile grid of a view size and tile shape (no buffers).
+void Trainer::set_geometry(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH) {
+  if (width <= 0 || height <= 0) throw Error(VSPLAT_RUNTIME_ERROR, "bad image size");
   const uint64_t gridW = div_up(static_cast<uint64_t>(width), tileW), gridH = div_up(static_cast<uint64_t>(height), tileH);
   if (gridW * gridH > (1u << 24)) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too large for the tile index");
   // Tile coordinates are kept as 16-bit pairs per splat.
@@ -321,13 +336,6 @@ void Trainer::ensure_layout(ViewBuffers &b, int width, int height, uint32_t tile
   while ((uint64_t{1} << b.tileBits) < b.tiles) ++b.tileBits;
   // The loss works on 16x16 tiles of its own, whatever the raster's shape.
   b.lossTiles = static_cast<uint32_t>(div_up(static_cast<uint64_t>(width), 16) * div_up(static_cast<uint64_t>(height), 16));
-  const size_t pixels = static_cast<size_t>(width) * static_cast<size_t>(height);
-  b.image.ensure(pixels * 16);
-  b.stops.ensure(pixels * 4);
-  b.adjoint.ensure(pixels * 16);
-  b.fields.ensure(pixels * 36);
-  b.ranges.ensure(static_cast<size_t>(b.tiles) * 8);
-  b.tileTotals.ensure(static_cast<size_t>(b.lossTiles) * 8);
 }
 
 // Times the view's passes up to the backward (forward, loss, backward: no
@@ -939,6 +947,68 @@ void Trainer::render(const vsplat_camera &camera, float *out) {
   lastViewIndex_ = 0;
   const size_t pixels = static_cast<size_t>(b.width) * static_cast<size_t>(b.height);
   check(hipMemcpy(out, b.image.as<float4>(), pixels * 16, hipMemcpyDeviceToHost), "download image");
+}
+
+// render() without allocating (vsplat_render_noalloc). The per-Gaussian,
+// per-entry and per-pixel buffers are carved out of `scratch` at the sizes of
+// the VSPLAT_NOALLOC_MAX_* limits, the same layout on every call; the image is
+// written straight into the caller's buffer. The counters and the tile-count
+// scan's control and status words are cleared first (the scratch keeps the last
+// call's contents); every other carved buffer is written before it is read. The
+// trainer's own view buffers, its last view and its statistics are not touched.
+void Trainer::render_into(const vsplat_camera &camera, float *deviceOut, uint64_t capacity, void *scratch, uint64_t scratchBytes) {
+  if (count_ > VSPLAT_NOALLOC_MAX_GAUSSIANS) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, std::to_string(count_) + " Gaussians: the allocation-free render takes at most " +
+                                             std::to_string(VSPLAT_NOALLOC_MAX_GAUSSIANS));
+  }
+  if (camera.width <= 0 || camera.height <= 0) throw Error(VSPLAT_RUNTIME_ERROR, "bad image size");
+  const uint64_t pixels = static_cast<uint64_t>(camera.width) * static_cast<uint64_t>(camera.height);
+  if (pixels > VSPLAT_NOALLOC_MAX_PIXELS) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, std::to_string(pixels) + " pixels: the allocation-free render takes at most " +
+                                             std::to_string(VSPLAT_NOALLOC_MAX_PIXELS));
+  }
+  if (capacity < pixels * 16) throw Error(VSPLAT_INVALID_ARGUMENT, "the output holds fewer than width*height*16 bytes");
+  ViewBuffers b;
+  b.stream = stream_;
+  const auto shape = tile_shape_for(camera.width, camera.height);
+  set_geometry(b, camera.width, camera.height, shape.first, shape.second);
+  if (b.tiles > VSPLAT_NOALLOC_MAX_TILES) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, std::to_string(b.tiles) + " tiles: the allocation-free render takes at most " +
+                                             std::to_string(VSPLAT_NOALLOC_MAX_TILES));
+  }
+  uint8_t *base = static_cast<uint8_t *>(scratch);
+  uint64_t used = 0;
+  const auto carve = [&](DeviceArray &a, uint64_t bytes) {
+    bytes = div_up(bytes, 256) * 256;
+    if (used + bytes > scratchBytes) throw Error(VSPLAT_RUNTIME_ERROR, "the allocation-free render's scratch is smaller than its layout");
+    a.borrow(base + used, bytes);
+    used += bytes;
+  };
+  const uint64_t n = VSPLAT_NOALLOC_MAX_GAUSSIANS, entries = VSPLAT_NOALLOC_MAX_ENTRIES;
+  carve(b.splats, n * sizeof(Splat));
+  for (DeviceArray *a : {&b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount}) carve(*a, n * 4);
+  carve(b.tileRecords, n * sizeof(TileRecord));
+  for (DeviceArray *a : {&b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues}) carve(*a, entries * 4);
+  carve(b.stops, uint64_t{VSPLAT_NOALLOC_MAX_PIXELS} * 4);
+  carve(b.ranges, uint64_t{VSPLAT_NOALLOC_MAX_TILES} * 8);
+  carve(b.sortScratch, radix_sort_scratch_words(std::max(entries, n)) * 4);
+  carve(b.scanScratch, coverage_scan_words(static_cast<uint32_t>(n)) * 4);
+  carve(b.counters, kCounterWords * 4);
+  b.image.borrow(deviceOut, capacity);
+  b.entryCapacity = entries;
+  // Fill kernels rather than memsets, as everywhere in a step.
+  launch_fill_u32(stream_, b.counters.as<uint32_t>(), kCounterWords, 0u);
+  launch_fill_u32(stream_, b.scanScratch.as<uint32_t>(), b.scanScratch.bytes() / 4, 0u);
+  dropoutNow_ = 0.0f; // as render(): the whole model
+  enqueue_passes(b, view_params(camera), nullptr, kModeRender, nullptr, false);
+  uint32_t *host = readback_.as<uint32_t>(); // pinned at construction
+  check(hipMemcpyAsync(host, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToHost, stream_), "read counters");
+  check(hipStreamSynchronize(stream_), "render");
+  if (host[kFault] != 0) throw Error(VSPLAT_DEVICE_ERROR, "a prefix-sum wait on the GPU timed out; the image is not valid");
+  if (host[kOverflow] != 0) {
+    throw Error(VSPLAT_BUDGET_EXCEEDED, std::to_string(host[kEntries]) + " tile entries: the allocation-free render lists at most " +
+                                            std::to_string(VSPLAT_NOALLOC_MAX_ENTRIES));
+  }
 }
 
 void Trainer::apply_gradients(const vsplat_learning_rates &rates) {

@@ -26,7 +26,13 @@ struct Error : std::runtime_error {
   Error(int s, const std::string &what) : std::runtime_error(what), status(s) {}
 };
 
-// One hipMalloc'd array; grows by reallocation, contents are not kept.
+// The allocation-free render's scratch on the current GPU (noalloc.hip): its
+// device address, looked up once per device, and its size in bytes.
+void *noalloc_scratch(uint64_t *bytes);
+
+// One hipMalloc'd array; grows by reallocation, contents are not kept. A
+// borrowed array points into memory it does not own (never freed, never grown:
+// ensure() beyond its size throws).
 class DeviceArray {
 public:
   DeviceArray() = default;
@@ -35,16 +41,24 @@ public:
   ~DeviceArray() { release(); }
   void ensure(size_t bytes);
   void release();
+  void borrow(void *ptr, size_t bytes) {
+    release();
+    ptr_ = ptr;
+    bytes_ = bytes;
+    owned_ = false;
+  }
   template <class T> T *as() const { return static_cast<T *>(ptr_); }
   size_t bytes() const { return bytes_; }
   void swap(DeviceArray &other) {
     std::swap(ptr_, other.ptr_);
     std::swap(bytes_, other.bytes_);
+    std::swap(owned_, other.owned_);
   }
 
 private:
   void *ptr_ = nullptr;
   size_t bytes_ = 0;
+  bool owned_ = true;
 };
 
 class PinnedArray {
@@ -104,6 +118,11 @@ public:
   Trainer &operator=(const Trainer &) = delete;
 
   void render(const vsplat_camera &camera, float *out);
+  // render() without allocating: the view's lists and per-pixel state in
+  // `scratch` (scratchBytes, 256-byte aligned device memory), the image written
+  // to deviceOut (capacity bytes). Sizes past the limits in include/vsplat.h
+  // are refused before anything runs.
+  void render_into(const vsplat_camera &camera, float *deviceOut, uint64_t capacity, void *scratch, uint64_t scratchBytes);
   vsplat_step_stats run(const vsplat_camera &camera, const uint8_t *target, size_t bytes, uint32_t mode,
                         const vsplat_learning_rates *rates, const uint8_t *next, size_t nextBytes);
   void apply_gradients(const vsplat_learning_rates &rates);
@@ -173,6 +192,7 @@ private:
   void allocate_per_gaussian();
   void ensure_size(ViewBuffers &b, int width, int height);
   void ensure_layout(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH);
+  static void set_geometry(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH);
   std::pair<uint32_t, uint32_t> tile_shape_for(int width, int height) const;
   bool needs_tuning(int width, int height) const;
   void grow_entries(ViewBuffers &b, uint64_t entries);

@@ -26,9 +26,12 @@ BYTES_PER_TILE_ENTRY = 16
 MODE_STATISTICS, MODE_GRADIENTS, MODE_STEP = 0, 1, 2
 SH_HALF, SH_BF16 = 0, 1  # TrainerOptions.sh_storage
 TILES_16X16, TILES_8X8, TILES_32X8, TILES_AUTO = 0, 1, 2, 3  # TrainerOptions.tile_shape
+# vsplat_render_noalloc's limits (VSPLAT_NOALLOC_MAX_*, include/vsplat.h).
+NOALLOC_MAX_GAUSSIANS, NOALLOC_MAX_ENTRIES, NOALLOC_MAX_PIXELS, NOALLOC_MAX_TILES = 1 << 17, 1 << 21, 1 << 21, 1 << 16
 
 EXPORTS = [
     "vsplat_default_options", "vsplat_default_rates", "vsplat_trainer_create", "vsplat_trainer_destroy", "vsplat_render",
+    "vsplat_render_noalloc",
     "vsplat_compute_gradients", "vsplat_collect_statistics", "vsplat_apply_gradients", "vsplat_step",
     "vsplat_prefetch_target", "vsplat_get_gradients", "vsplat_get_cloud", "vsplat_get_densify_stats",
     "vsplat_reset_densify_stats", "vsplat_set_active_sh_degree", "vsplat_set_filter3d", "vsplat_set_sh_dropout",
@@ -113,6 +116,7 @@ def lib():
         L.vsplat_trainer_destroy.argtypes = [vp]
         L.vsplat_trainer_destroy.restype = None
         L.vsplat_render.argtypes = [vp, vp, vp]
+        L.vsplat_render_noalloc.argtypes = [vp, vp, vp, C.c_uint64]
         L.vsplat_compute_gradients.argtypes = [vp, vp, vp, C.c_size_t, vp]
         L.vsplat_collect_statistics.argtypes = [vp, vp, vp, C.c_size_t, vp]
         L.vsplat_apply_gradients.argtypes = [vp, vp]
@@ -331,6 +335,12 @@ class Trainer:
         cam = camera_to_c(camera)
         _raise(self._lib.vsplat_render(self._h, C.byref(cam), _p(out)), "render")
         return out
+
+    def render_noalloc(self, camera: Camera, out: "DeviceBuffer") -> None:
+        """vsplat_render_noalloc: the render into GPU memory (out), with no
+        allocation; the image is out's first width*height*16 bytes."""
+        cam = camera_to_c(camera)
+        _raise(self._lib.vsplat_render_noalloc(self._h, C.byref(cam), C.c_void_p(out.ptr), out.nbytes), "render_noalloc")
 
     def compute_gradients(self, camera: Camera, target: np.ndarray) -> StepStats:
         t = np.ascontiguousarray(target, np.uint8)
