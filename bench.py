@@ -80,11 +80,11 @@ def algorithmic_bytes(N, V, E, Ep, P, T, Bp):
 
 def cpu_baseline(workload, threads: int) -> dict:
     """The reference's CPU path (its src/train/reference.cpp, as the FP64 oracle
-    restates it): forward, loss and backward with the chain rule, on `threads`
-    host threads. Timed on a band of 90 of the 720 rows (the same per-pixel
-    work as the full view) and on a 2-row band (mostly the per-Gaussian costs:
-    projection and sort, and the chain rule); the full view is the 2-row time
-    plus the per-row difference times the remaining 718 rows."""
+    restates it): forward, loss and backward with the chain rule over the whole
+    1280x720 view, timed once on `threads` host threads. Beside it: the same on
+    one thread, from a 12-row and a 2-row band of the view (the per-row work
+    scaled to 720 rows: a whole single-thread iteration takes about a minute), and
+    the FP32 model of this build's GPU algorithm on one thread, a whole iteration."""
     from oracle import oracle as O
 
     cam, target = workload.cameras[0], workload.targets[0]
@@ -102,36 +102,64 @@ def cpu_baseline(workload, threads: int) -> dict:
         O.gradients(workload.cloud, c, ls.adjoint, threads=n)
         return time.perf_counter() - t0
 
-    def oracle_b(rows, n):
-        top, c = band(rows)
+    def oracle_b():
         t0 = time.perf_counter()
-        m = O.Model(workload.cloud, c)
+        m = O.Model(workload.cloud, cam)
         r = m.forward()
-        m.loss(r, np.ascontiguousarray(target[top:top + rows]), workload.lambda_ssim)
+        m.loss(r, target, workload.lambda_ssim)
         m.backward_fixed(r)
         m.gradients(r)
         return time.perf_counter() - t0
 
-    def extrapolate(run, n, rows):
-        t_fixed, t_band = run(2, n), run(rows, n)
-        per_row = (t_band - t_fixed) / (rows - 2)
-        return t_fixed + per_row * (H - 2), t_band, t_fixed
-
-    full, t_band, t_fixed = extrapolate(oracle_a, threads, 90)
-    one, one_band, one_fixed = extrapolate(oracle_a, 1, 12)
-    b_full, b_band, b_fixed = extrapolate(oracle_b, 1, 90)
+    full = oracle_a(H, threads)
+    one_fixed, one_band = oracle_a(2, 1), oracle_a(12, 1)
+    one = one_fixed + (one_band - one_fixed) / 10.0 * (H - 2)
+    b_full = oracle_b()
     return {"value": round(full * 1000.0, 1), "unit": "ms/iter", "cores": threads, "kind": "port",
             "host_cpus": os.cpu_count(),
-            "sample": f"FP64 restatement of reference.cpp, fwd+loss+bwd+chain rule on {threads} threads: "
-                      f"a 90-row band ({t_band:.2f} s) and a 2-row band ({t_fixed:.2f} s) of the 1280x720 view, "
-                      f"extrapolated linearly in rows to 720",
+            "sample": f"FP64 restatement of reference.cpp, one whole iteration (fwd+loss+bwd+chain rule) of the "
+                      f"1280x720 view on {threads} threads",
             "single_core": {"value": round(one * 1000.0, 1), "unit": "ms/iter", "cores": 1,
-                            "sample": f"the same on 1 thread: a 12-row band ({one_band:.2f} s) and a 2-row band "
-                                      f"({one_fixed:.2f} s), extrapolated to 720 rows"},
+                            "sample": f"the same on 1 thread, from a 12-row band ({one_band:.2f} s) and a 2-row band "
+                                      f"({one_fixed:.2f} s) of the view, the per-row time scaled to 720 rows"},
             "fp32_tile_model": {"value": round(b_full * 1000.0, 1), "unit": "ms/iter", "cores": 1,
-                                "sample": f"the FP32 model of this build's GPU algorithm (same lists, sums and fixed "
-                                          f"point; single-threaded): a 90-row band ({b_band:.2f} s) and a 2-row band "
-                                          f"({b_fixed:.2f} s), extrapolated to 720 rows"}}
+                                "sample": "the FP32 model of this build's GPU algorithm (same lists, sums and fixed "
+                                          "point), one whole iteration on 1 thread"}}
+
+
+def fp64_errors(forms, crop_size: int = 128) -> dict:
+    """Each timed form's errors against the FP64 restatement of the reference
+    (the oracle as the checker), on a crop_size x crop_size crop at the centre
+    of its view: the image, the loss, and each parameter group's gradient as a
+    relative L2 error. The GPU runs the form's own options (its SH storage and
+    its fixed-point sums); the FP64 side takes the cloud as the trainer stores it."""
+    from oracle import oracle as O
+
+    out = {}
+    for name, cloud, cam, options in forms:
+        crop = dataclasses.replace(cam, width=crop_size, height=crop_size, cx=cam.cx - (cam.width - crop_size) / 2,
+                                   cy=cam.cy - (cam.height - crop_size) / 2)
+        target = F.smooth_target_explicit(crop_size, crop_size, 1234)
+        stored = cloud.copy()
+        stored.sh = F.unpack_sh(F.pack_sh(cloud.sh, options.sh_storage), cloud.coefficients,
+                                options.sh_storage).reshape(cloud.sh.shape).copy()
+        ref = O.render(stored, crop)
+        ref_loss = O.loss(ref, target, options.lambda_ssim)
+        ref_g = O.gradients(stored, crop, ref_loss.adjoint)
+        with vsplat.Trainer(cloud, options) as t:
+            image = t.render(crop)
+            st = t.compute_gradients(crop, target)
+            g = t.gradients()
+        errors = {}
+        for group in ("positions", "scales", "rotations", "opacities", "sh"):
+            a = np.asarray(getattr(g, group), np.float64).ravel()
+            b = np.asarray(getattr(ref_g, group), np.float64).ravel()
+            errors[group] = float(f"{np.linalg.norm(a - b) / np.linalg.norm(b):.3e}")
+        out[name] = {"crop": f"{crop_size}x{crop_size} at the centre of {cam.width}x{cam.height}",
+                     "image_max_abs": float(f"{np.abs(image[:, :3] - ref.rgb).max():.3e}"),
+                     "loss_abs": float(f"{abs(st.loss - ref_loss.loss):.3e}"),
+                     "gradient_rel_l2": errors, "gradient_rel_l2_max": max(errors.values())}
+    return out
 
 
 def apply_changes_leg(trainer, n: int) -> dict:
@@ -252,18 +280,19 @@ def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) ->
             "tile_shape": tile_shape, "tiles_chosen": {f"{k[0]}x{k[1]}": f"{v[0]}x{v[1]}" for k, v in shapes.items()}}
 
 
-def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str, views: int = 8) -> dict:
+def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str, views: int = 8, fixed: bool = True) -> dict:
     """The target comparison at one size: the 1M-Gaussian cloud of `value`, seen
     by eight 1280x720 views on a ring of radius 3 (the ring workload's cameras,
     all at 1280x720), gradients mode (fwd+bwd, gradients summed over the views),
-    SH stored as bf16, fixed point, captured as one HIP graph and replayed; per
-    view against the plain fp32 single 1280x720 view."""
+    SH stored as bf16, the sums in 64-bit fixed point (fixed) or float32
+    atomics, captured as one HIP graph and replayed; per view against the plain
+    fp32 single 1280x720 view."""
     import math
     w = F.config(1, count=count)
     cams = [F.look_at((3.0 * math.sin(2.0 * math.pi * k / views), 0.0, -3.0 * math.cos(2.0 * math.pi * k / views)), (0, 0, 0),
                       (0.0, -1.0, 0.0), 60.0, 1280, 720) for k in range(views)]
     targets = [F.smooth_target_explicit(1280, 720, 1016 + k) for k in range(views)]
-    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True,
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed,
                                                             sh_storage=vsplat.SH_BF16, tile_shape=TILE_SHAPES[tile_shape]))
     trainer.set_stage_timing(False)
     buffers = [vsplat.DeviceBuffer(t) for t in targets]
@@ -286,19 +315,20 @@ def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str
             b.free()
         trainer.close()
     return {"workload": f"1M Gaussians, SH degree 3 stored as bf16, {views} views of 1280x720 on a ring of radius 3, "
-                        "gradients mode, one captured graph", "views": views, "graph_ms": round(graph, 4),
+                        "gradients mode, one captured graph", "sums": "64-bit fixed point" if fixed else "float32 atomics",
+            "views": views, "graph_ms": round(graph, 4),
             "graph_ms_per_view": round(graph / views, 4), "tile_shape": tile_shape,
             "tiles_chosen": None if chosen is None else f"{chosen[0]}x{chosen[1]}"}
 
 
-def other_form(index: int, steps: int, warmup: int, tile_shape: str, **options) -> dict:
+def other_form(index: int, steps: int, warmup: int, tile_shape: str, fixed: bool = True, **options) -> dict:
     """ms/iter of one more single-view workload, gradients mode, fixed point,
     target in device memory: the precision form (3M Gaussians in 64 blobs, SH
     stored as bf16, 1920x1080) or the differentiation stress (6M heavy-tailed
     Gaussians, 1920x1080, the backward recomputing the transmittance)."""
     w = F.config(index)
     cam, target = w.cameras[0], w.targets[0]
-    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=True,
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed,
                                                             tile_shape=TILE_SHAPES[tile_shape], **options))
     trainer.set_stage_timing(False)
     buffer = vsplat.DeviceBuffer(target)
@@ -326,7 +356,8 @@ def other_form(index: int, steps: int, warmup: int, tile_shape: str, **options) 
         buffer.free()
         trainer.close()
     return {"workload": w.description, "gaussians": w.cloud.count, "width": cam.width, "height": cam.height,
-            "options": {k: (int(v) if isinstance(v, bool) else v) for k, v in options.items()}, "ms_per_iter": round(ms, 4),
+            "options": {k: (int(v) if isinstance(v, bool) else v) for k, v in options.items()},
+            "sums": "64-bit fixed point" if fixed else "float32 atomics", "ms_per_iter": round(ms, 4),
             "visible": st.visible_splats, "entries": st.tile_entries, "loss": round(float(first.loss), 6),
             "gpu_gb": round(gpu_gb, 3), "tile_shape": tile_shape,
             "tiles_chosen": None if chosen is None else f"{chosen[0]}x{chosen[1]}", "stages_ms": stages}
@@ -358,8 +389,8 @@ def main():
     cam, target = w.cameras[0], w.targets[0]
     N, P = w.cloud.count, cam.width * cam.height
     fixed = not args.float_atomics
-    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed,
-                                                            tile_shape=TILE_SHAPES[args.tile_shape]))
+    options = vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed, tile_shape=TILE_SHAPES[args.tile_shape])
+    trainer = vsplat.Trainer(w.cloud, options)
     device_target = vsplat.DeviceBuffer(target)
     d_target = device_target.ptr
     if args.tile_shape == "auto":
@@ -461,12 +492,20 @@ def main():
         same["plain_vs_batched_per_view"] = round(result["value"] / same["graph_ms_per_view"], 4)
         result["batched_1280x720"] = same
         log(f"  {same['graph_ms_per_view']} ms per view, {same['plain_vs_batched_per_view']}x the plain single view")
+        log("batched bf16 fwd+bwd, sums accumulated in float32: eight 1280x720 views")
+        same_float = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed=False)
+        same_float["plain_single_view_ms"] = result["value"]
+        same_float["plain_vs_batched_per_view"] = round(result["value"] / same_float["graph_ms_per_view"], 4)
+        result["batched_1280x720_float_sums"] = same_float
+        log(f"  {same_float['graph_ms_per_view']} ms per view, {same_float['plain_vs_batched_per_view']}x the plain single view")
     if not args.no_other_forms and args.count is None:
         result["other_forms"] = {}
-        for key, index, options in (("precision_bf16_3M_1920x1080", 2, {"sh_storage": vsplat.SH_BF16}),
-                                    ("stress_recompute_6M_1920x1080", 4, {"recompute_transmittance": True})):
+        for key, index, fixed_sums, form_options in (
+                ("precision_bf16_3M_1920x1080", 2, True, {"sh_storage": vsplat.SH_BF16}),
+                ("precision_bf16_3M_1920x1080_float_sums", 2, False, {"sh_storage": vsplat.SH_BF16}),
+                ("stress_recompute_6M_1920x1080", 4, True, {"recompute_transmittance": True})):
             log(f"{key}: generating and timing")
-            form = other_form(index, max(5, args.steps // 2), 2, args.tile_shape, **options)
+            form = other_form(index, max(5, args.steps // 2), 2, args.tile_shape, fixed=fixed_sums, **form_options)
             result["other_forms"][key] = form
             log(f"  {form['ms_per_iter']} ms per iteration, {form['entries']} entries, {form['gpu_gb']} GB")
     if not args.no_densify:
@@ -481,6 +520,20 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline on {threads} threads")
         result["cpu_baseline"] = cpu_baseline(w, threads)
+        # The same CPU path checks the timed forms: their errors against FP64.
+        log("errors of the timed forms against FP64 (128x128 crops)")
+        bf16 = dataclasses.replace(options, sh_storage=vsplat.SH_BF16)
+        bf16_float = dataclasses.replace(bf16, force_fixed_point=False)
+        forms = [("plain_fp32_fixed_1M_1280x720", w.cloud, cam, options),
+                 ("batched_bf16_fixed_1M_1280x720", w.cloud, cam, bf16),
+                 ("batched_bf16_float_sums_1M_1280x720", w.cloud, cam, bf16_float)]
+        if args.count is None and not args.no_other_forms:
+            w2 = F.config(2)
+            for name, o in (("precision_bf16_fixed_3M_1920x1080", bf16), ("precision_bf16_float_sums_3M_1920x1080", bf16_float)):
+                forms.append((name, w2.cloud, w2.cameras[0], dataclasses.replace(o, lambda_ssim=w2.lambda_ssim)))
+        result["fp64_errors"] = fp64_errors(forms)
+        for k, v in result["fp64_errors"].items():
+            log(f"  {k}: gradient rel L2 up to {v['gradient_rel_l2_max']:.2e}, image {v['image_max_abs']:.1e}")
     device_target.free()
     trainer.close()
     print(json.dumps(result), flush=True)

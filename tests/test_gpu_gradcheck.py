@@ -57,11 +57,18 @@ def _cases():
 @pytest.mark.parametrize("case", list(range(3)))
 def test_recomputed_transmittance_gives_the_same_sums(case):
     name, cloud, cam, target = list(_cases())[case]
-    raws = []
+    raws, images, sizes = [], [], []
     for recompute in (False, True):
         with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, recompute_transmittance=recompute)) as t:
             t.compute_gradients(cam, target)
             raws.append(t.inspect("raw"))
+            images.append(t.inspect("image"))
+            sizes.append(t.gpu_bytes())
+            if recompute:  # no per-pixel stops are kept for a backward that recomputes them
+                with pytest.raises(vsplat.LogicError, match="stops"):
+                    t.inspect("stops")
+    np.testing.assert_array_equal(images[1], images[0])
+    assert sizes[0] - sizes[1] >= cam.width * cam.height * 4
     print(f"{name}: {int((raws[0] != 0).any(axis=-1).sum()) if raws[0].ndim > 1 else int((raws[0] != 0).sum())} non-zero", flush=True)
     np.testing.assert_array_equal(raws[1], raws[0])
     if name == "crop":

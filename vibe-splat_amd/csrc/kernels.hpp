@@ -117,6 +117,7 @@ void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uin
 // The raster kernels for one tile shape: 8x8, 16x16 or 32x8 (tileW x tileH).
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
                     uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops);
+// (stops may be null: not kept when the backward recomputes them.)
 // recompute: the transmittance and stops are recomputed per tile rather than
 // read from the forward pass (image and stops may then be null). edge: a
 // scored view's edge weights (the adjoint's fourth value its marks), which

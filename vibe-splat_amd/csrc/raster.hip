@@ -1,19 +1,21 @@
 // Per-tile compositing and its derivative (train_forward.comp:38-100,
-// train_backward.glsl:78-158). One workgroup of 256 threads per 16x16 tile,
-// one thread per pixel; the tile's splats are staged through LDS in batches
-// of 256, each with the tile's pixels it can touch already worked out.
+// train_backward.glsl:78-158). The lists are per tile (16x16 by default, or
+// 8x8 or 32x8); the pixels are worked in 8x8 blocks, one workgroup of one
+// wave64 per block, lane l holding pixel (l & 7, l >> 3). A block reads its
+// tile's list in batches of 64 entries: each lane stages one splat with the
+// block's pixels it can touch already worked out, and only the splats that
+// can touch a pixel of the block are kept, packed in list order. A block
+// stops when its own pixels are done, with no barrier across the tile.
 //
-// Staging: for each splat, a 16-bit column mask and a 16-bit row mask of the
+// Staging: for each splat, an 8-bit column mask and an 8-bit row mask of the
 // pixels inside both its extent box (splat_pixels, train_types.glsl:43-48) and
 // its radius square (the first test of splat_at): a pixel is a candidate iff
 // its column and row bits are set. The masks are computed with the same float
 // operations as the per-pixel tests they replace, so the set is the same.
 // A wave skips a splat none of its lanes is a candidate for; otherwise every
 // lane evaluates the Gaussian without branches and the result is selected.
-//
-// A tile is 8x8, 16x16 (the reference's) or 32x8 pixels, one thread per pixel.
-// Each wave holds an 8x8 block of the tile, lane l its pixel (l & 7, l >> 3):
-// a splat's footprint fills more of a square block's lanes than of a strip's.
+// Every list shape has the same 8x8 blocks, so images and sums do not depend
+// on the shape.
 //
 // Backward: for each splat the 12 non-zero of the 14 per-pixel sums are
 // reduced over the wave by a transposed butterfly in which the steps halve the
@@ -32,55 +34,86 @@ namespace {
 
 struct Staged {
   float qx, qy, qz, o;
-  float r, g, b, pad0;
+  float r, g, b;
+  uint32_t entry; // the splat's index in the tile's list
   float cx, cy;
-  uint32_t cols, rows; // the tile's columns and rows the splat can touch, bit per tile-local index
+  uint32_t cols, rows; // the block's columns (bits 0-7) and rows (bits 16-23) the splat can touch; rows unused
 };
 static_assert(sizeof(Staged) == 48, "48-byte staged splat");
 
-struct TileGeometry {
+constexpr uint32_t kBlock = 64; // pixels of an 8x8 block, threads of its workgroup, entries per batch
+
+struct BlockGeometry {
   uint32_t x0, y0, x1, y1;
 };
+
+// Workgroup `item` works 8x8 block q of list tile t, item = t * blocks + q,
+// the tile's blocks row-major (TW / 8 across): a tile's blocks are neighbours
+// in the grid, so its list is read from cache after the first.
 template <uint32_t TW, uint32_t TH>
-__device__ __forceinline__ TileGeometry tile_geometry(uint32_t tile, uint32_t gridW, uint32_t width, uint32_t height) {
-  const uint32_t tx = tile % gridW, ty = tile / gridW;
-  const uint32_t x0 = tx * TW, y0 = ty * TH;
-  return {x0, y0, x0 + TW < width ? x0 + TW : width, y0 + TH < height ? y0 + TH : height};
+__device__ __forceinline__ BlockGeometry block_geometry(uint32_t item, uint32_t gridW, uint32_t width, uint32_t height,
+                                                        uint32_t &tile) {
+  constexpr uint32_t kAcross = TW / 8u, kPerTile = (TW / 8u) * (TH / 8u);
+  tile = item / kPerTile;
+  const uint32_t q = item - tile * kPerTile;
+  const uint32_t x0 = (tile % gridW) * TW + (q % kAcross) * 8u, y0 = (tile / gridW) * TH + (q / kAcross) * 8u;
+  return {x0, y0, x0 + 8u < width ? x0 + 8u : width, y0 + 8u < height ? y0 + 8u : height};
 }
 
-// The pixels of the tile a splat can touch: inside the extent box and within
-// the radius (|d| <= r on both axes, d from the pixel centre).
-template <uint32_t TW, uint32_t TH>
-__device__ __forceinline__ Staged stage(const Splat &s, const TileGeometry &geo) {
+// The block's pixels a splat can touch: inside the extent box and within the
+// radius (|d| <= r on both axes, d from the pixel centre).
+__device__ __forceinline__ Staged stage(const Splat &s, const BlockGeometry &geo, uint32_t entry) {
   const uint32_t box = splat_box(s, geo.x0, geo.y0, geo.x1, geo.y1);
-  uint32_t cols = 0, rows = 0;
+  uint32_t mask = 0;
   if (box != 0u) {
     const uint32_t bx0 = box & 0xFFu, by0 = (box >> 8) & 0xFFu, bx1 = (box >> 16) & 0xFFu, by1 = box >> 24;
-    constexpr uint32_t kSide = TW > TH ? TW : TH;
     // Kept as a loop: unrolled, its temporaries would set the kernel's register count.
 #pragma unroll 1
-    for (uint32_t k = 0; k < kSide; ++k) {
+    for (uint32_t k = 0; k < 8u; ++k) {
       const float dx = (static_cast<float>(geo.x0 + k) + 0.5f) - s.cx;
       const float dy = (static_cast<float>(geo.y0 + k) + 0.5f) - s.cy;
       const bool col = k >= bx0 && k < bx1 && !(fabsf(dx) > s.radius);
       const bool row = k >= by0 && k < by1 && !(fabsf(dy) > s.radius);
-      cols |= (col ? 1u : 0u) << k;
-      rows |= (row ? 1u : 0u) << k;
+      mask |= ((col ? 1u : 0u) << k) | ((row ? 1u : 0u) << (16u + k));
     }
   }
-  return Staged{s.qx, s.qy, s.qz, s.o, s.r, s.g, s.b, 0.0f, s.cx, s.cy, cols, rows};
+  return Staged{s.qx, s.qy, s.qz, s.o, s.r, s.g, s.b, entry, s.cx, s.cy, mask, 0u};
 }
 
-// Thread t's pixel in a tile TW pixels wide: wave w holds the 8x8 block
-// (w mod TW/8, w div TW/8), lane l its pixel (l & 7, l >> 3). Every tile shape
-// has the same 8x8 blocks for its waves, so the per-wave sums of the backward
-// (and so the fixed-point results) do not depend on the shape.
-template <uint32_t TW>
-__device__ __forceinline__ uint32_t local_x(uint32_t t) { return ((t >> 6) % (TW / 8u)) * 8u + (t & 7u); }
-template <uint32_t TW>
-__device__ __forceinline__ uint32_t local_y(uint32_t t) { return ((t >> 6) / (TW / 8u)) * 8u + ((t >> 3) & 7u); }
+// Lane l's pixel in its block and its two mask bits.
+__device__ __forceinline__ uint32_t lane_bits(uint32_t lx, uint32_t ly) { return (1u << lx) | (1u << (16u + ly)); }
+__device__ __forceinline__ bool covers(const Staged &s, uint32_t bits) { return (s.cols & bits) == bits; }
 
-__device__ __forceinline__ bool covers(const Staged &s, uint32_t lx, uint32_t ly) { return ((s.cols >> lx) & (s.rows >> ly) & 1u) != 0u; }
+// The wave votes as scalar masks: a ballot of a lane predicate is a scalar
+// comparison away from a uniform branch.
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0u; }
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0u; }
+
+// Entries first .. first + count - 1 of the list: each lane stages one, and the
+// splats that can touch a pixel of the block are packed at the front of
+// `batch` in list order (with their Gaussians in `index`, when given). Returns
+// how many were kept. The caller has made sure the previous batch is read.
+__device__ __forceinline__ uint32_t stage_batch(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                                uint32_t first, uint32_t count, const BlockGeometry &geo, Staged *batch,
+                                                uint32_t *index) {
+  const uint32_t lane = threadIdx.x;
+  Staged st{};
+  uint32_t gaussian = 0;
+  bool keep = false;
+  if (lane < count) {
+    gaussian = entryValues[first + lane];
+    st = stage(load_splat(splats, gaussian), geo, first + lane);
+    keep = (st.cols & 0xFFu) != 0u && (st.cols >> 16) != 0u;
+  }
+  const uint64_t kept = __ballot(keep);
+  if (keep) {
+    const uint32_t slot = static_cast<uint32_t>(__popcll(kept & ((1ull << lane) - 1ull)));
+    batch[slot] = st;
+    if (index != nullptr) index[slot] = gaussian;
+  }
+  __syncthreads();
+  return static_cast<uint32_t>(__popcll(kept));
+}
 
 // The Gaussian's exponent at offset (dx, dy) (splat_at, train_types.glsl:88).
 __device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
@@ -89,29 +122,24 @@ __device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
 
 // Front to back over a tile's list (train_forward.comp:38-100): the colour
 // (with Colour), the transmittance and its log, and the stop index of the
-// calling thread's pixel. The forward pass and the backward pass that
+// calling lane's pixel. The forward pass and the backward pass that
 // recomputes the transmittance both run this, so they agree bit for bit.
-template <bool Colour, uint32_t TW, uint32_t TH>
-__device__ __forceinline__ void composite_tile(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues, uint2 range,
-                                               const TileGeometry &geo, Staged *batch, bool inside, float px, float py, uint32_t lx,
-                                               uint32_t ly, float &r, float &g, float &b, float &T, float &logT, uint32_t &stop) {
-  constexpr uint32_t kBatch = TW * TH; // one splat staged per thread
+template <bool Colour>
+__device__ __forceinline__ void composite_block(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues, uint2 range,
+                                                const BlockGeometry &geo, Staged *batch, bool inside, float px, float py, uint32_t bits,
+                                                float &r, float &g, float &b, float &T, float &logT, uint32_t &stop) {
   stop = range.x;
   bool done = !inside;
   for (uint32_t end = range.y; end > range.x;) {
-    const uint32_t count = end - range.x < kBatch ? end - range.x : kBatch;
+    if (wave_all(done)) break;
+    const uint32_t count = end - range.x < kBlock ? end - range.x : kBlock;
     end -= count;
-    // Also the barrier that keeps the batch from being overwritten while read.
-    if (__syncthreads_count(!done) == 0) break;
-    if (threadIdx.x < count) batch[threadIdx.x] = stage<TW, TH>(load_splat(splats, entryValues[end + threadIdx.x]), geo);
-    __syncthreads();
-    for (uint32_t k = count; k-- > 0;) {
+    __syncthreads(); // the previous batch is read
+    const uint32_t kept = stage_batch(splats, entryValues, end, count, geo, batch, nullptr);
+    for (uint32_t k = kept; k-- > 0;) {
       const Staged s = batch[k];
-      const bool candidate = !done && covers(s, lx, ly);
-      if (!__any(candidate)) {
-        if (__all(done)) break;
-        continue;
-      }
+      const bool candidate = !done && covers(s, bits);
+      if (!wave_any(candidate)) continue;
       const float dx = px - s.cx, dy = py - s.cy;
       const float power = power_at(s, dx, dy);
       // Below -87 alpha is far under 1/255 either way, so the clamp in exp_finite decides nothing.
@@ -119,7 +147,7 @@ __device__ __forceinline__ void composite_tile(const Splat *__restrict__ splats,
       const float a = s.o * e;
       const float alpha = a < 0.99f ? a : 0.99f;
       const bool ok = candidate && !(power > 0.0f) && alpha >= kMinAlpha;
-      if (!__any(ok)) continue;
+      if (!wave_any(ok)) continue;
       // Without branches: a lane that takes nothing adds a zero weight
       // (fma(0, c, r) is r) and keeps its transmittance by selection.
       const float logRest = log_normal(1.0f - alpha);
@@ -134,31 +162,33 @@ __device__ __forceinline__ void composite_tile(const Splat *__restrict__ splats,
         T = take ? T * (1.0f - alpha) : T;
       }
       logT = take ? logNext : logT;
-      stop = stopHere ? end + k + 1 : stop;
+      stop = stopHere ? s.entry + 1u : stop;
       done = done || stopHere;
+      if (wave_all(done)) break;
     }
   }
 }
 
 template <uint32_t TW, uint32_t TH>
-__global__ __launch_bounds__(TW * TH) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
-                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
-                                                          uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
-                                                          uint32_t *__restrict__ stops) {
-  __shared__ Staged batch[TW * TH];
-  const uint32_t tile = blockIdx.x;
-  const TileGeometry geo = tile_geometry<TW, TH>(tile, gridW, width, height);
-  const uint32_t lx = local_x<TW>(threadIdx.x), ly = local_y<TW>(threadIdx.x);
+__global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                                         const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
+                                                         uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
+                                                         uint32_t *__restrict__ stops) {
+  __shared__ Staged batch[kBlock];
+  uint32_t tile;
+  const BlockGeometry geo = block_geometry<TW, TH>(blockIdx.x, gridW, width, height, tile);
+  if (geo.x0 >= width || geo.y0 >= height) return; // a block past the image's edge (uniform)
+  const uint32_t lx = threadIdx.x & 7u, ly = threadIdx.x >> 3;
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, logT = 0.0f;
   uint32_t stop;
-  composite_tile<true, TW, TH>(splats, entryValues, ranges[tile], geo, batch, inside, px, py, lx, ly, r, g, b, T, logT, stop);
+  composite_block<true>(splats, entryValues, ranges[tile], geo, batch, inside, px, py, lane_bits(lx, ly), r, g, b, T, logT, stop);
   if (inside) {
     const uint32_t at = y * width + x;
     image[at] = make_float4(fma32(T, bgR, r), fma32(T, bgG, g), fma32(T, bgB, b), logT);
-    stops[at] = stop;
+    if (stops != nullptr) stops[at] = stop; // none when the backward recomputes them
   }
 }
 
@@ -258,26 +288,27 @@ __device__ __forceinline__ uint32_t reduced_value(uint32_t lane) {
 // lanes hold the third value twice).
 __device__ __forceinline__ bool wave_emits(uint32_t lane) { return (lane & 9u) == 0u && (lane & 6u) != 6u; }
 
-// Back to front over the tile (train_backward.glsl:78-158). With Recompute the
-// pixel's log transmittance and stop come from a front-to-back sweep of its
-// own instead of the forward pass's image and stops, which are then not read.
-// With Score (a scored view) the last two sums are filled too: the pixel's
-// high-error mark (the adjoint's fourth value) and its edge weight times the
-// blending weight, each over 3 pixels of the image.
+// Back to front over the tile's list (train_backward.glsl:78-158), for the
+// block's pixels, from the block's lowest stop. With Recompute the pixel's log
+// transmittance and stop come from a front-to-back sweep of its own instead
+// of the forward pass's image and stops, which are then not read. With Score
+// (a scored view) the last two sums are filled too: the pixel's high-error
+// mark (the adjoint's fourth value) and its edge weight times the blending
+// weight, each over 3 pixels of the image.
 template <bool Fixed, bool Recompute, bool Score, uint32_t TW, uint32_t TH>
-__global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
-                                                       const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
-                                                       float bgR, float bgG, float bgB, const float4 *__restrict__ image,
-                                                       const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
-                                                       float fixedScale, unsigned long long *__restrict__ rawFixed,
-                                                       float *__restrict__ rawFloat, const float *__restrict__ edge) {
-  constexpr uint32_t kBatch = TW * TH; // one splat staged per thread
-  __shared__ Staged batch[kBatch];
-  __shared__ uint32_t batchIndex[kBatch];
-  __shared__ uint32_t tileStop;
-  const uint32_t tile = blockIdx.x;
-  const TileGeometry geo = tile_geometry<TW, TH>(tile, gridW, width, height);
-  const uint32_t lx = local_x<TW>(threadIdx.x), ly = local_y<TW>(threadIdx.x), lane = threadIdx.x & 63u;
+__global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
+                                                          float bgR, float bgG, float bgB, const float4 *__restrict__ image,
+                                                          const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
+                                                          float fixedScale, unsigned long long *__restrict__ rawFixed,
+                                                          float *__restrict__ rawFloat, const float *__restrict__ edge) {
+  __shared__ Staged batch[kBlock];
+  __shared__ uint32_t batchIndex[kBlock];
+  uint32_t tile;
+  const BlockGeometry geo = block_geometry<TW, TH>(blockIdx.x, gridW, width, height, tile);
+  if (geo.x0 >= width || geo.y0 >= height) return; // a block past the image's edge (uniform)
+  const uint32_t lane = threadIdx.x, lx = lane & 7u, ly = lane >> 3;
+  const uint32_t bits = lane_bits(lx, ly);
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
@@ -291,39 +322,36 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
   if constexpr (Recompute) {
     float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f;
     logTotal = 0.0f;
-    composite_tile<false, TW, TH>(splats, entryValues, range, geo, batch, inside, px, py, lx, ly, r, g, b, T, logTotal, stop);
+    composite_block<false>(splats, entryValues, range, geo, batch, inside, px, py, bits, r, g, b, T, logTotal, stop);
     if (!inside) stop = range.y;
   } else {
     logTotal = inside ? image[at].w : 0.0f;
     stop = inside ? stops[at] : range.y;
   }
-  if (threadIdx.x == 0) tileStop = range.y;
-  __syncthreads();
-  atomicMin(&tileStop, stop);
-  __syncthreads();
-  const uint32_t first = tileStop;
+  // The block's lowest stop: entries before it have no candidate lane here.
+  uint32_t first = stop;
+#pragma unroll
+  for (uint32_t d = 32; d >= 1; d >>= 1) {
+    const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(first), static_cast<int>(d), 64));
+    first = other < first ? other : first;
+  }
   float Br = bgR, Bg = bgG, Bb = bgB, logBehind = 0.0f;
 
-  for (uint32_t start = first; start < range.y; start += kBatch) {
-    const uint32_t count = range.y - start < kBatch ? range.y - start : kBatch;
-    __syncthreads();
-    if (threadIdx.x < count) {
-      const uint32_t index = entryValues[start + threadIdx.x];
-      batchIndex[threadIdx.x] = index;
-      batch[threadIdx.x] = stage<TW, TH>(load_splat(splats, index), geo);
-    }
-    __syncthreads();
-    for (uint32_t k = 0; k < count; ++k) {
+  for (uint32_t start = first; start < range.y; start += kBlock) {
+    const uint32_t count = range.y - start < kBlock ? range.y - start : kBlock;
+    __syncthreads(); // the previous batch is read
+    const uint32_t kept = stage_batch(splats, entryValues, start, count, geo, batch, batchIndex);
+    for (uint32_t k = 0; k < kept; ++k) {
       const Staged s = batch[k];
-      const bool candidate = inside && start + k >= stop && covers(s, lx, ly);
-      if (!__any(candidate)) continue;
+      const bool candidate = inside && s.entry >= stop && covers(s, bits);
+      if (!wave_any(candidate)) continue;
       const float dx = px - s.cx, dy = py - s.cy;
       const float power = power_at(s, dx, dy);
       const float e = exp_finite(power);
       const float a = s.o * e;
       const float alpha = a < 0.99f ? a : 0.99f;
       const bool ok = candidate && !(power > 0.0f) && alpha >= kMinAlpha;
-      if (!__any(ok)) continue;
+      if (!wave_any(ok)) continue;
       const float logRest = log_normal(1.0f - alpha);
       const float T = exp_finite((logTotal - logBehind) - logRest);
       // A lane that takes nothing from the splat carries zero factors, so its
@@ -377,7 +405,15 @@ __global__ __launch_bounds__(TW * TH) void backward_kernel(const Splat *__restri
       if (emits) {
         const size_t slot = static_cast<size_t>(batchIndex[k]) * kRawPerSplat + value;
         if constexpr (Fixed) {
-          const int64_t q = quantise(v[0], fixedScale);
+          // Totals under 2^31 steps (nearly all) round and convert in 32 bits,
+          // exactly as quantise() would; the wave takes quantise() if any is larger.
+          const float scaled = v[0] * fixedScale;
+          int64_t q;
+          if (__builtin_amdgcn_ballot_w64(!(fabsf(scaled) < 2147483648.0f)) == 0u) {
+            q = static_cast<int32_t>(__builtin_rintf(scaled));
+          } else {
+            q = quantise(v[0], fixedScale);
+          }
           if (q != 0) atomicAdd(&rawFixed[slot], static_cast<unsigned long long>(q));
         } else {
           if (v[0] != 0.0f) atomicAdd(&rawFloat[slot], v[0]);
@@ -403,8 +439,8 @@ void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *ent
                     uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_FORWARD(W, H)                                                                                                       \
-  hipLaunchKernelGGL((forward_kernel<W, H>), dim3(gridW * gridH), dim3(W * H), 0, stream, splats, entryValues, ranges, width,     \
-                     height, gridW, background[0], background[1], background[2], image, stops)
+  hipLaunchKernelGGL((forward_kernel<W, H>), dim3(gridW * gridH * (W / 8) * (H / 8)), dim3(kBlock), 0, stream, splats, entryValues, \
+                     ranges, width, height, gridW, background[0], background[1], background[2], image, stops)
   VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD)
 #undef VSPLAT_FORWARD
 }
@@ -415,9 +451,9 @@ void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const 
                      float *rawFloat, const float *edge) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_BACKWARD_SHAPE(F, R, S, W, H)                                                                                       \
-  hipLaunchKernelGGL((backward_kernel<F, R, S, W, H>), dim3(gridW * gridH), dim3(W * H), 0, stream, splats, entryValues, ranges,  \
-                     width, height, gridW, background[0], background[1], background[2], image, adjoint, stops, fixedScale,         \
-                     reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge)
+  hipLaunchKernelGGL((backward_kernel<F, R, S, W, H>), dim3(gridW * gridH * (W / 8) * (H / 8)), dim3(kBlock), 0, stream, splats,   \
+                     entryValues, ranges, width, height, gridW, background[0], background[1], background[2], image, adjoint, stops, \
+                     fixedScale, reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge)
 #define VSPLAT_BACKWARD_TTT(W, H) VSPLAT_BACKWARD_SHAPE(true, true, true, W, H)
 #define VSPLAT_BACKWARD_TTF(W, H) VSPLAT_BACKWARD_SHAPE(true, true, false, W, H)
 #define VSPLAT_BACKWARD_TFT(W, H) VSPLAT_BACKWARD_SHAPE(true, false, true, W, H)

@@ -311,7 +311,8 @@ void Trainer::ensure_layout(ViewBuffers &b, int width, int height, uint32_t tile
   set_geometry(b, width, height, tileW, tileH);
   const size_t pixels = static_cast<size_t>(width) * static_cast<size_t>(height);
   b.image.ensure(pixels * 16);
-  b.stops.ensure(pixels * 4);
+  // A backward that recomputes the transmittance reads no stops: none are kept.
+  if (options_.recompute_transmittance == 0) b.stops.ensure(pixels * 4);
   b.adjoint.ensure(pixels * 16);
   b.fields.ensure(pixels * 36);
   b.ranges.ensure(static_cast<size_t>(b.tiles) * 8);
@@ -614,7 +615,7 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   }
   mark("tile lists");
   launch_forward(s, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH,
-                 options_.background, b.image.as<float4>(), b.stops.as<uint32_t>());
+                 options_.background, b.image.as<float4>(), options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>());
   mark("composite");
   if (mode == kModeRender) return;
   // The loss sees the image as this view's exposure makes it (trainer.cpp:397-425).
@@ -1090,7 +1091,8 @@ void Trainer::prepare_views(uint32_t n, const vsplat_camera *cameras, const uint
 // everything already queued on the trainer's stream (the previous step's
 // update of the parameters); each view's counters and loss sums copied to its
 // stats slot; the trainer's stream then waits for every view and gathers their
-// sticky flags.
+// sticky flags into the first view's counters, the batch's status: a view that
+// did not fit skips the batch's whole chain rule and step.
 void Trainer::enqueue_view_passes(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params) {
   queuedViewSizes_.clear();
   current_ = nullptr; // no stage events in a batch
@@ -1114,16 +1116,15 @@ void Trainer::enqueue_view_passes(uint32_t n, const vsplat_camera *cameras, cons
   }
 }
 
-void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
-                            const vsplat_learning_rates *rates) {
-  prepare_views(n, cameras, targets, mode, rates);
-  dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
-  statisticsNext_ = true;
-  std::vector<ViewParams> params(n);
-  enqueue_view_passes(n, cameras, targets, params.data());
+// The batch's chain rule on the trainer's stream, after every view's passes
+// (enqueue_view_passes): chain_views_kernel over the views in chunks of
+// kChainViewsPerLaunch, each chunk continuing the sums of the one before; the
+// last takes the step. (Each view's chain rule on its own stream as soon as
+// its backward had run, in view order, measured slower: the views' streams
+// share the device's four hardware queues, so a view's chain pass waited
+// behind a later view's passes queued before it.)
+void Trainer::enqueue_view_chains(uint32_t n, const ViewParams *params, uint32_t mode, const vsplat_learning_rates *rates) {
   uint32_t *status = views_[0]->counters.as<uint32_t>();
-  // The chain rule over the views in chunks of kChainViewsPerLaunch, each
-  // chunk continuing the sums of the one before; the last takes the step.
   const ChainBuffers buffers = chain_buffers(*views_[0]);
   for (uint32_t first = 0; first < n; first += kChainViewsPerLaunch) {
     const uint32_t count = std::min(n - first, kChainViewsPerLaunch);
@@ -1143,6 +1144,18 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
     launch_chain_views(stream_, buffers, chunk, chain_params(params[first], last ? mode : kModeGradients, rates), status);
   }
   accumulateNext_ = false;
+}
+
+void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, uint32_t mode,
+                            const vsplat_learning_rates *rates) {
+  prepare_views(n, cameras, targets, mode, rates);
+  dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
+  statisticsNext_ = true;
+  std::vector<ViewParams> params(n);
+  enqueue_view_passes(n, cameras, targets, params.data());
+  uint32_t *status = views_[0]->counters.as<uint32_t>();
+  enqueue_view_chains(n, params.data(), mode, rates);
+  const ChainBuffers buffers = chain_buffers(*views_[0]);
   lastView_ = params[n - 1];
   lastViewIndex_ = n - 1;
   if (mode == kModeStep) {
@@ -1570,6 +1583,9 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
       {"state_sh", &state_[4], n * shFloats_ * 4},
       {"stats", &stats_, n * 32},
   };
+  if (name == "stops" && options_.recompute_transmittance != 0) {
+    throw Error(VSPLAT_LOGIC_ERROR, "no stops are kept: the backward recomputes them");
+  }
   for (const Item &item : items) {
     if (name != item.name) continue;
     if (bytes != nullptr) *bytes = item.size;

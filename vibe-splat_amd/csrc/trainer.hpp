@@ -187,6 +187,7 @@ private:
   ViewBuffers &view_buffers(size_t k); // the k-th view's buffers, made on first use
   void release_views();
   void enqueue_view_passes(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params);
+  void enqueue_view_chains(uint32_t n, const ViewParams *params, uint32_t mode, const vsplat_learning_rates *rates);
   static uint64_t sizes_key(uint32_t n, const vsplat_camera *cameras);
   void upload_cloud(const vsplat_cloud &cloud, DeviceArray *const values[5]) const;
   void allocate_per_gaussian();
