@@ -140,7 +140,7 @@ int vsplat_render(vsplat_trainer *trainer, const vsplat_camera *camera, float *o
 #define VSPLAT_NOALLOC_MAX_ENTRIES (1u << 21)
 #define VSPLAT_NOALLOC_MAX_PIXELS (1u << 21)
 #define VSPLAT_NOALLOC_MAX_TILES (1u << 16)
-#define VSPLAT_NOALLOC_SCRATCH_BYTES (53ull << 20)
+#define VSPLAT_NOALLOC_SCRATCH_BYTES (56ull << 20)
 int vsplat_render_noalloc(vsplat_trainer *trainer, const vsplat_camera *camera, float *device_rgba, uint64_t capacity_bytes);
 /* Trainer::compute_gradients (trainer.hpp:104): forward, loss, backward; the
  * gradients stay on the GPU for vsplat_get_gradients and vsplat_apply_gradients.

@@ -11,10 +11,13 @@
 // choice, and is tied to the FP64 restatement of the reference (oracle_a.cpp)
 // by the tolerance tests in tests/test_oracle_b.py.
 //
-// Parity unpinned: which neighbour the hash-driven stochastic rounding picks
-// (the bf16 second moment, bf16.glsl:16-23, and the fp16 SH after a step,
-// train_step.glsl:70-80). The tests check only that it is one of the two
-// neighbours; the reference holds no vectors that would fix the choice.
+// The hash-driven stochastic roundings (the bf16 second moment, bf16.glsl:16-23,
+// and the fp16 SH after a step, train_step.glsl:70-80) are pinned by
+// tests/test_rounding.py: the hash exactly, by inverting it through the
+// published PCG constants, and each rounding rule by its unbiasedness over
+// 2^18 draws per value (a truncating or a round-to-nearest copy fails). The
+// reference holds no vectors of its own for them: the reference GPU's bit
+// patterns themselves are not available to compare with.
 #include "fp32_math.hpp"
 #include "oracle_api.h"
 

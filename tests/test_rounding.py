@@ -177,3 +177,53 @@ def test_dropout_share_is_the_probability(p):
     for seed in (1, 9):
         dropped = int(np.count_nonzero(O.train_uniform(np.arange(n, dtype=np.uint32), seed) < np.float32(p)))
         assert _binomial_ok(dropped, p, n), (seed, dropped / n)
+
+
+# The published constants of the hash (Jarzynski and Olano, "Hash Functions for
+# GPU Rendering", JCGT 9(3), 2020, the "pcg" entry): one LCG step of PCG's
+# 32-bit generator, multiplier 747796405 and increment 2891336453, then the
+# RXS M XS output permutation with multiplier 277803737; the reference mixes
+# the seed into the index first with 0x9E3779B9 (train_types.glsl:156-160).
+_LCG_MUL, _LCG_ADD, _RXS_MUL, _SEED_MUL = 747796405, 2891336453, 277803737, 0x9E3779B9
+
+
+def _unhash(h, seed):
+    """The hash undone step by step with the published constants: xorshift 22
+    undone, the multiplier's inverse modulo 2^32, the data-dependent xorshift
+    undone (its shift is in the top four bits, which it leaves alone), the LCG
+    step undone, the seed's mix undone. Nothing here restates the forward hash."""
+    m = (1 << 32) - 1
+    out = np.empty(h.size, np.uint32)
+    inv_rxs, inv_lcg = pow(_RXS_MUL, -1, 1 << 32), pow(_LCG_MUL, -1, 1 << 32)
+    for k, y in enumerate(h.tolist()):
+        w = y ^ (y >> 22)
+        s = (w * inv_rxs) & m
+        r = (s >> 28) + 4
+        state, shifted = s, s >> r
+        while shifted:
+            state ^= shifted
+            shifted >>= r
+        index = ((state - _LCG_ADD) * inv_lcg) & m
+        out[k] = index ^ ((seed * _SEED_MUL) & m)
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 7, 0xFFFFFFFF])
+def test_hash_is_the_published_pcg_permutation(seed):
+    """Known answers by inversion: every hash of 20000 indices (spread over the
+    32-bit range, with 0 and the largest) leads back to its index through the
+    published constants, so the oracle's hash is exactly that function."""
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([np.array([0, 1, 2, 0xFFFFFFFF], np.uint32), rng.integers(0, 1 << 32, 20000, dtype=np.uint64).astype(np.uint32)])
+    np.testing.assert_array_equal(_unhash(O.train_hash(idx, seed), seed), idx)
+
+
+def test_the_inversion_fails_a_hash_with_another_constant():
+    """The inversion has the power to fail: a hash whose output multiplier is
+    off by two does not lead back to its indices."""
+    v = np.arange(1000, dtype=np.uint64)
+    m = np.uint64(0xFFFFFFFF)
+    state = (v * np.uint64(_LCG_MUL) + np.uint64(_LCG_ADD)) & m
+    word = (((state >> ((state >> np.uint64(28)) + np.uint64(4))) ^ state) * np.uint64(_RXS_MUL + 2)) & m
+    h = ((word >> np.uint64(22)) ^ word).astype(np.uint32)
+    assert np.count_nonzero(_unhash(h, 0) != v.astype(np.uint32)) > 990

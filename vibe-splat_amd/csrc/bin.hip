@@ -104,19 +104,19 @@ __global__ void merge_flags_kernel(uint32_t *status, const uint32_t *counters) {
   status[kFault] |= counters[kFault];
 }
 
-// A wave takes 64 consecutive depth-sorted splats, whose entries fill one
-// contiguous run of the list starting at the first one's offset. It walks their
-// candidate tiles 64 at a time (tiles.hpp); the reached ones of a round are
-// numbered by a ballot prefix count, so each round's stores are contiguous.
-// Small splats read the outcome from the bits the projection kept; large ones
-// repeat the test on the stored splat.
+// A wave takes 64 consecutive depth-sorted splats and walks their candidate
+// tiles 64 at a time (tiles.hpp), reading each small splat's outcome from the
+// bits the projection kept. An entry goes to its splat's first entry (the
+// scan's offset) plus the number of its splat's reached tiles before it, so a
+// round's stores are contiguous except where a big splat's run lies between
+// two small ones. A big splat's first entry is left in its record for
+// big_emit_kernel.
 __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
-                                                       const Splat *__restrict__ splats, const TileRecord *__restrict__ records,
-                                                       uint32_t count, uint32_t gridW, TileShape shape, const uint32_t *__restrict__ offsets,
-                                                       const uint32_t *counters, uint32_t *__restrict__ entryKeys,
-                                                       uint32_t *__restrict__ entryValues) {
-  __shared__ WalkOwner owners[256]; // per wave: its splats with candidates, packed
-  __shared__ uint32_t flags[256];    // per wave: the walk's round marks
+                                                       TileRecord *__restrict__ records, uint32_t count, uint32_t gridW,
+                                                       const uint32_t *__restrict__ offsets, const uint32_t *counters,
+                                                       uint32_t *__restrict__ entryKeys, uint32_t *__restrict__ entryValues) {
+  __shared__ EmitOwner owners[256]; // per wave: its small splats with candidates, packed
+  __shared__ uint32_t flags[256];   // per wave: the walk's round marks
   if (counters[kOverflow] != 0) return; // the same for every thread of the launch
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x & ~63u;
@@ -127,6 +127,10 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
     rec = records[i];
     candidates = (rec.size & 0xFFFFu) * (rec.size >> 16);
   }
+  if (candidates > kMaskTiles) {
+    records[i].maskLo = offsets[r]; // a big splat: listed by big_emit_kernel
+    candidates = 0;
+  }
   const bool owner = candidates != 0;
   const uint64_t owning = __ballot(owner);
   const uint32_t end = wave_inclusive_sum(candidates, lane);
@@ -135,52 +139,118 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
   const uint32_t start = end - candidates;
   if (owner) {
     const uint32_t width = rec.size & 0xFFFFu;
-    TileTest test{};
-    if (candidates > kMaskTiles) {
-      const Splat s = load_splat(splats, i);
-      test = tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy);
-    }
     const uint32_t k = static_cast<uint32_t>(__popcll(owning & ((1ull << lane) - 1ull)));
-    owners[wave + k] = WalkOwner{test, rec.origin & 0xFFFFu, rec.origin >> 16, width, start, 1.0f / static_cast<float>(width), i,
-                                 rec.maskLo, rec.maskHi};
+    owners[wave + k] = EmitOwner{rec.origin & 0xFFFFu, rec.origin >> 16, width, start, 1.0f / static_cast<float>(width), i,
+                                 rec.maskLo, rec.maskHi, offsets[r], {0u, 0u, 0u}};
   }
   __builtin_amdgcn_wave_barrier();
-  uint32_t slot = offsets[r - lane]; // the wave's first splat is in range when it has candidates
   const uint32_t limit = counters[kEntriesUsed];
   uint32_t before = 0;
   for (uint32_t base = 0; base < total; base += 64) {
     const uint32_t k = walk_round(flags + wave, lane, base, owner, start, before);
     const uint32_t j = base + lane;
-    bool hit = false;
-    uint32_t tx = 0, ty = 0, gaussian = 0;
     if (j < total) {
-      const WalkOwner &o = owners[wave + k];
+      const EmitOwner &o = owners[wave + k];
       const uint32_t local = j - o.start;
-      box_tile(o, local, tx, ty);
-      const uint32_t size = (k + 1 < static_cast<uint32_t>(__popcll(owning)) ? owners[wave + k + 1].start : total) - o.start;
       const uint64_t mask = static_cast<uint64_t>(o.maskLo) | (static_cast<uint64_t>(o.maskHi) << 32);
-      hit = size <= kMaskTiles ? ((mask >> local) & 1ull) != 0 : reaches_tile(o.test, tx, ty, shape);
-      gaussian = o.gaussian;
+      if (((mask >> local) & 1ull) != 0) {
+        uint32_t tx, ty;
+        box_tile(o, local, tx, ty);
+        const uint32_t at = o.offset + static_cast<uint32_t>(__popcll(mask & ((1ull << local) - 1ull)));
+        // The hits are the ones the projection counted, so at < limit; the test only guards the buffer.
+        if (at < limit) {
+          entryKeys[at] = ty * gridW + tx;
+          entryValues[at] = o.gaussian;
+        }
+      }
     }
-    const uint64_t round = __ballot(hit);
-    const uint32_t at = slot + static_cast<uint32_t>(__popcll(round & ((1ull << lane) - 1ull)));
-    // The hits are the ones the projection counted, so at < limit; the test only guards the buffer.
-    if (hit && at < limit) {
-      entryKeys[at] = ty * gridW + tx;
-      entryValues[at] = gaussian;
-    }
-    slot += static_cast<uint32_t>(__popcll(round));
   }
 }
 
-// Start and end of each tile's run; every write belongs to one thread (train_tile_ranges.comp:11-22).
+// Big splats' entries, listed by the waves: a wave loads 64 listed splats at
+// once (one per lane), then walks each one's box in turn, 64 candidates a
+// round, row-major, each round's outcome read from the bits the counting pass
+// kept (or tested again where it kept none), the reached ones numbered by a
+// prefix count from the splat's first entry. Waves take the listed splats 64
+// at a time, in turn.
+__global__ __launch_bounds__(256) void big_emit_kernel(const Splat *__restrict__ splats, const TileRecord *__restrict__ records,
+                                                       const uint32_t *__restrict__ bigList, const uint64_t *__restrict__ bigBits,
+                                                       uint32_t gridW, TileShape shape, const uint32_t *counters,
+                                                       uint32_t *__restrict__ entryKeys, uint32_t *__restrict__ entryValues) {
+  if (counters[kOverflow] != 0) return; // the same for every thread of the launch
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u), wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+  const uint32_t listed = counters[kBigSplats], limit = counters[kEntriesUsed];
+  for (uint32_t base = wave * 64u; base < listed; base += waves * 64u) {
+    BigSplat mine{};
+    uint32_t bits = kNoBits;
+    if (base + lane < listed) {
+      const uint32_t i = bigList[base + lane];
+      const TileRecord rec = records[i];
+      bits = rec.maskHi;
+      TileTest test{};
+      if (bits == kNoBits) {
+        const Splat s = load_splat(splats, i);
+        test = tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy);
+      }
+      mine = BigSplat{test, rec.origin & 0xFFFFu, rec.origin >> 16, rec.size & 0xFFFFu, (rec.size & 0xFFFFu) * (rec.size >> 16), i,
+                      rec.maskLo};
+    }
+    const uint32_t inChunk = listed - base < 64u ? listed - base : 64u;
+    for (uint32_t j = 0; j < inChunk; ++j) {
+      const BigSplat b = big_splat_of_lane(mine, j);
+      const uint32_t bitStart = lane_u32(bits, j);
+      BoxCursor cursor(lane, b.width);
+      uint32_t next = b.first; // the splat's first entry
+      for (uint32_t k = lane, round = 0; k - lane < b.candidates; k += 64u, ++round) {
+        const uint32_t tx = b.tx0 + cursor.col, ty = b.ty0 + cursor.row;
+        // The counting pass's outcome of this round, or the test again where it kept none.
+        const uint64_t mask = bitStart != kNoBits ? bigBits[bitStart + round]
+                                                  : __ballot(k < b.candidates && reaches_tile(b.test, tx, ty, shape));
+        const bool hit = ((mask >> lane) & 1ull) != 0;
+        const uint32_t at = next + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+        if (hit && at < limit) {
+          entryKeys[at] = ty * gridW + tx;
+          entryValues[at] = b.gaussian;
+        }
+        next += static_cast<uint32_t>(__popcll(mask));
+        cursor.advance();
+      }
+    }
+  }
+}
+
+// Start and end of each tile's run; every write belongs to one thread
+// (train_tile_ranges.comp:11-22). A thread takes four consecutive keys with one
+// 16-byte load, and the keys either side of them.
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *__restrict__ entryKeys, const uint32_t *counters,
                                                           uint2 *__restrict__ ranges) {
   const uint32_t count = counters[kEntriesUsed];
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
-    const uint32_t tile = entryKeys[k];
-    if (k == 0 || entryKeys[k - 1] != tile) ranges[tile].x = k;
-    if (k + 1 == count || entryKeys[k + 1] != tile) ranges[tile].y = k + 1;
+  const uint32_t quads = (count + 3u) / 4u;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+    const uint32_t k0 = q * 4u;
+    uint32_t key[4];
+    if (k0 + 4u <= count) {
+      const uint4 v = reinterpret_cast<const uint4 *>(entryKeys)[q];
+      key[0] = v.x;
+      key[1] = v.y;
+      key[2] = v.z;
+      key[3] = v.w;
+    } else {
+#pragma unroll
+      for (uint32_t j = 0; j < 4u; ++j) key[j] = k0 + j < count ? entryKeys[k0 + j] : ~0u;
+    }
+    const uint32_t before = k0 == 0 ? ~0u : entryKeys[k0 - 1u];
+    const uint32_t after = k0 + 4u < count ? entryKeys[k0 + 4u] : ~0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+      const uint32_t k = k0 + j;
+      if (k >= count) break;
+      const uint32_t prev = j == 0 ? before : key[j - 1u];
+      const uint32_t next = j == 3u ? after : key[j + 1u];
+      if (k == 0 || prev != key[j]) ranges[key[j]].x = k;
+      if (k + 1u == count || next != key[j]) ranges[key[j]].y = k + 1u;
+    }
   }
 }
 
@@ -199,17 +269,20 @@ void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *co
   hipLaunchKernelGGL(merge_flags_kernel, dim3(1), dim3(1), 0, stream, status, counters);
 }
 
-void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
+void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, TileRecord *records,
                      uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, const uint32_t *offsets, const uint32_t *counters,
-                     uint32_t *entryKeys, uint32_t *entryValues) {
+                     const uint32_t *bigList, const uint64_t *bigBits, uint32_t *entryKeys, uint32_t *entryValues) {
   if (count == 0) return;
-  hipLaunchKernelGGL(bin_emit_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, splats, records, count, gridW,
-                     TileShape{tileW, tileH}, offsets, counters, entryKeys, entryValues);
+  hipLaunchKernelGGL(bin_emit_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, records, count, gridW, offsets,
+                     counters, entryKeys, entryValues);
+  hipLaunchKernelGGL(big_emit_kernel, dim3(kBigSplatBlocks), dim3(256), 0, stream, splats, records, bigList, bigBits, gridW,
+                     TileShape{tileW, tileH}, counters, entryKeys, entryValues);
 }
 
 void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges) {
   if (capacity == 0) return;
-  const uint32_t blocks = (capacity + 255) / 256 < 16384 ? (capacity + 255) / 256 : 16384;
+  const uint32_t quads = (capacity + 3) / 4;
+  const uint32_t blocks = (quads + 255) / 256 < 16384 ? (quads + 255) / 256 : 16384;
   hipLaunchKernelGGL(tile_ranges_kernel, dim3(blocks), dim3(256), 0, stream, entryKeys, counters, ranges);
 }
 

@@ -66,10 +66,13 @@ enum Counter : uint32_t {
   kVisible = 0,      // splats in view
   kEntries = 1,      // tile entries this view needs
   kEntriesUsed = 2,  // entries actually listed: kEntries, or 0 if they did not fit
-  kOverflow = 3,     // sticky: a view needed more entries than the buffers hold; every later pass is skipped until the host clears it
-  kSkipped = 4,      // step-mode passes skipped that way since the host last cleared it
-  kFault = 5,        // a kernel gave up a wait that should have ended (see lookback.hpp)
-  kCounterWords = 6,
+  kBigSplats = 3,    // splats whose tile box is larger than a TileRecord's bits (listed for their own kernels)
+  kBigBitWords = 4,  // 64-bit words of the big splats' reached-tile bits handed out so far
+  kOverflow = 5,     // sticky: a view needed more entries than the buffers hold; every later pass is skipped until the host clears it
+  kSkipped = 6,      // step-mode passes skipped that way since the host last cleared it
+  kFault = 7,        // a kernel gave up a wait that should have ended (see lookback.hpp)
+  kCounterWords = 8,
+  kClearedPerPass = 5, // the first five are cleared at the start of every pass
 };
 
 __device__ __forceinline__ float fma32(float a, float b, float c) { return __builtin_fmaf(a, b, c); }

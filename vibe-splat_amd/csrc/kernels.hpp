@@ -77,12 +77,21 @@ struct ChainViews {
 
 struct TileRecord;
 
+// Workgroups (of four waves) of the kernels that count and list the big splats
+// (boxes of more than kMaskTiles tiles), whose waves take the listed splats in turn.
+constexpr uint32_t kBigSplatBlocks = 2048;
+
 // Projection and tile coverage: splats, depth keys and values (the identity),
-// and per Gaussian the tile count and TileRecord; adds the visible count.
+// and per Gaussian the tile count and TileRecord; adds the visible count. The
+// big splats are listed in bigList (count[kBigSplats], cleared before) and
+// counted by their own kernel. bigList holds one word per Gaussian; bigBits
+// (bitCapacity words) receives their reached-tile bits, one word per 64
+// candidates, for the list pass (a splat that finds no room keeps none).
 void launch_project(hipStream_t stream, const float *positions, const float *scales, const float *rotations, const float *opacities,
                     const uint32_t *shWords, const float *filter3d, const ViewParams &view, bool mipFilter2d, float shDropout,
                     uint32_t shRetained, uint32_t shSeed, uint32_t gridW, uint32_t gridH, Splat *splats, uint32_t *keys,
-                    uint32_t *values, uint32_t *tileCount, TileRecord *records, uint32_t *counters);
+                    uint32_t *values, uint32_t *tileCount, TileRecord *records, uint32_t *counters, uint32_t *bigList,
+                    uint64_t *bigBits, uint32_t bitCapacity);
 
 // Scratch for launch_radix_sort over up to capacity keys.
 size_t radix_sort_scratch_words(uint64_t capacity);
@@ -109,9 +118,12 @@ void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32
                           uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *control);
 // status[overflow] and status[fault] |= counters' (a batch's views).
 void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters);
-void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, const TileRecord *records,
+// The tile lists in depth order: the small splats' entries from their bits,
+// then the big splats' (bigList, with the bits launch_project kept). records is written: a
+// big splat's first entry is kept in its maskLo for the second kernel.
+void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, TileRecord *records,
                      uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, const uint32_t *offsets, const uint32_t *counters,
-                     uint32_t *entryKeys, uint32_t *entryValues);
+                     const uint32_t *bigList, const uint64_t *bigBits, uint32_t *entryKeys, uint32_t *entryValues);
 void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges);
 
 // The raster kernels for one tile shape: 8x8, 16x16 or 32x8 (tileW x tileH).

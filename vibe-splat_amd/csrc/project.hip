@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
                                                       ViewParams c, ProjectOptions opt, uint32_t gridW, uint32_t gridH,
                                                       Splat *__restrict__ splats, uint32_t *__restrict__ keys,
                                                       uint32_t *__restrict__ values, uint32_t *__restrict__ tileCount,
-                                                      TileRecord *__restrict__ records, uint32_t *counters) {
+                                                      TileRecord *__restrict__ records, uint32_t *counters, uint32_t *__restrict__ bigList) {
   // The block's SH words, staged with coalesced loads; afterwards the walk's owners.
   __shared__ uint32_t stage[kProjectThreads * kMaxShWords];
   __shared__ uint32_t flags[kProjectThreads]; // per wave: the walk's round marks
@@ -252,21 +252,35 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   }
   __syncthreads(); // the SH words are read; the stage now holds the walk's owners
   WalkOwner *owners = reinterpret_cast<WalkOwner *>(stage);
-  const uint64_t seen = __ballot(visible);
 
-  // Candidate tiles: the bounding box of each visible splat.
+  // Candidate tiles: the bounding box of each visible splat. A box of more
+  // than kMaskTiles tiles is a big splat's: it is listed for big_count_kernel
+  // (one slot per wave from one atomic) and left out of the walk.
   uint32_t t[4] = {0, 0, 0, 0};
-  uint32_t candidates = 0;
+  uint32_t box = 0;
   if (visible) {
     splat_tiles(s.cx, s.cy, s.ex, s.ey, gridW, gridH, TileShape{c.tileW, c.tileH}, t);
-    candidates = (t[2] - t[0] + 1) * (t[3] - t[1] + 1);
+    box = (t[2] - t[0] + 1) * (t[3] - t[1] + 1);
   }
+  const bool big = box > kMaskTiles;
+  const uint64_t bigLanes = __ballot(big);
+  if (bigLanes != 0) {
+    uint32_t base = 0;
+    if (lane == static_cast<uint32_t>(__builtin_ctzll(bigLanes))) {
+      base = atomicAdd(&counters[kBigSplats], static_cast<uint32_t>(__popcll(bigLanes)));
+    }
+    base = __shfl(base, static_cast<int>(__builtin_ctzll(bigLanes)), 64);
+    if (big) bigList[base + static_cast<uint32_t>(__popcll(bigLanes & ((1ull << lane) - 1ull)))] = i;
+  }
+  const bool walks = visible && !big;
+  const uint32_t candidates = walks ? box : 0u;
+  const uint64_t walkers = __ballot(walks);
   const uint32_t width = t[2] - t[0] + 1;
   const uint32_t end = wave_inclusive_sum(candidates, lane);
   const uint32_t start = end - candidates;
   const uint32_t total = __shfl(end, 63, 64);
-  if (visible) {
-    const uint32_t k = static_cast<uint32_t>(__popcll(seen & ((1ull << lane) - 1ull)));
+  if (walks) {
+    const uint32_t k = static_cast<uint32_t>(__popcll(walkers & ((1ull << lane) - 1ull)));
     owners[wave + k] = WalkOwner{tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy), t[0], t[1], width, start,
                                  1.0f / static_cast<float>(width), i, 0u, 0u};
   }
@@ -275,7 +289,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
   uint32_t reached = 0, before = 0;
   uint64_t bits = 0;
   for (uint32_t base = 0; base < total; base += 64) {
-    const uint32_t slot = walk_round(flags + wave, lane, base, visible, start, before);
+    const uint32_t slot = walk_round(flags + wave, lane, base, walks, start, before);
     const uint32_t j = base + lane;
     bool hit = false;
     if (j < total) {
@@ -295,23 +309,75 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
     }
   }
   if (i < c.count) {
-    tileCount[i] = reached;
+    if (!big) tileCount[i] = reached; // a big splat's count comes from big_count_kernel
     const uint32_t height = t[3] - t[1] + 1;
     records[i] = TileRecord{static_cast<uint32_t>(bits), static_cast<uint32_t>(bits >> 32), t[0] | (t[1] << 16),
                             visible ? (width | (height << 16)) : 0u};
   }
 }
 
+// Big splats' reached tiles, counted by the waves: a wave loads 64 listed
+// splats at once (one per lane), then walks each one's box in turn, 64
+// candidates a round. Waves take the listed splats 64 at a time, in turn.
+__global__ __launch_bounds__(256) void big_count_kernel(const Splat *__restrict__ splats, TileRecord *__restrict__ records,
+                                                        const uint32_t *__restrict__ bigList, uint32_t *counters, TileShape shape,
+                                                        uint64_t *__restrict__ bigBits, uint32_t bitCapacity,
+                                                        uint32_t *__restrict__ tileCount) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u), wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+  const uint32_t listed = counters[kBigSplats];
+  for (uint32_t base = wave * 64u; base < listed; base += waves * 64u) {
+    BigSplat mine{};
+    uint32_t rounds = 0;
+    if (base + lane < listed) {
+      const uint32_t i = bigList[base + lane];
+      const Splat s = load_splat(splats, i);
+      const TileRecord rec = records[i];
+      mine = BigSplat{tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy), rec.origin & 0xFFFFu, rec.origin >> 16, rec.size & 0xFFFFu,
+                      (rec.size & 0xFFFFu) * (rec.size >> 16), i, 0u};
+      rounds = (mine.candidates + 63u) / 64u;
+    }
+    // Room for each splat's bits, one word a round, from one atomic per wave.
+    const uint32_t end = wave_inclusive_sum(rounds, lane);
+    uint32_t wordBase = 0;
+    if (lane == 63u && end != 0) wordBase = atomicAdd(&counters[kBigBitWords], end);
+    wordBase = __shfl(wordBase, 63, 64);
+    const uint32_t bitStart = wordBase + (end - rounds);
+    // Past the capacity a splat keeps no bits, and its list pass tests its tiles again.
+    mine.first = bitStart + rounds <= bitCapacity ? bitStart : kNoBits;
+    const uint32_t inChunk = listed - base < 64u ? listed - base : 64u;
+    uint32_t myHits = 0;
+    for (uint32_t j = 0; j < inChunk; ++j) {
+      const BigSplat b = big_splat_of_lane(mine, j);
+      BoxCursor at(lane, b.width);
+      uint32_t hits = 0, round = 0;
+      for (uint32_t k = lane; k - lane < b.candidates; k += 64u, ++round) {
+        const bool hit = k < b.candidates && reaches_tile(b.test, b.tx0 + at.col, b.ty0 + at.row, shape);
+        const uint64_t mask = __ballot(hit);
+        if (lane == 0 && b.first != kNoBits) bigBits[b.first + round] = mask;
+        hits += static_cast<uint32_t>(__popcll(mask));
+        at.advance();
+      }
+      if (lane == j) myHits = hits;
+    }
+    if (base + lane < listed) {
+      tileCount[mine.gaussian] = myHits;
+      records[mine.gaussian].maskHi = mine.first; // where its bits are, for the list pass
+    }
+  }
+}
+
 void launch_project(hipStream_t stream, const float *positions, const float *scales, const float *rotations, const float *opacities,
                     const uint32_t *shWords, const float *filter3d, const ViewParams &view, bool mipFilter2d, float shDropout,
                     uint32_t shRetained, uint32_t shSeed, uint32_t gridW, uint32_t gridH, Splat *splats, uint32_t *keys,
-                    uint32_t *values, uint32_t *tileCount, TileRecord *records, uint32_t *counters) {
+                    uint32_t *values, uint32_t *tileCount, TileRecord *records, uint32_t *counters, uint32_t *bigList,
+                    uint64_t *bigBits, uint32_t bitCapacity) {
   if (view.count == 0) return;
   const ProjectOptions opt{mipFilter2d ? 1u : 0u, shDropout, shRetained, shSeed};
   const dim3 grid((view.count + 255) / 256), block(256);
 #define VSPLAT_PROJECT(D, F)                                                                                                       \
   hipLaunchKernelGGL((project_kernel<D, F>), grid, block, 0, stream, positions, scales, rotations, opacities, shWords, filter3d,    \
-                     view, opt, gridW, gridH, splats, keys, values, tileCount, records, counters)
+                     view, opt, gridW, gridH, splats, keys, values, tileCount, records, counters, bigList)
 #define VSPLAT_PROJECT_DEGREE(F)                                                                                                   \
   switch (view.shDegree) {                                                                                                         \
   case 0: VSPLAT_PROJECT(0, F); break;                                                                                             \
@@ -326,6 +392,9 @@ void launch_project(hipStream_t stream, const float *positions, const float *sca
   }
 #undef VSPLAT_PROJECT_DEGREE
 #undef VSPLAT_PROJECT
+  // The big splats' counts; the grid takes them in turn (how many there are is only known on the device).
+  hipLaunchKernelGGL(big_count_kernel, dim3(kBigSplatBlocks), dim3(256), 0, stream, splats, records, bigList, counters,
+                     TileShape{view.tileW, view.tileH}, bigBits, bitCapacity, tileCount);
 }
 
 } // namespace vsplat

@@ -92,9 +92,22 @@ struct WalkOwner {
 };
 static_assert(sizeof(WalkOwner) == 64, "64-byte walk owner");
 
+// A small splat as the list pass reads it from LDS: its box, its reached-tile
+// bits and where its entries start in the list.
+struct EmitOwner {
+  uint32_t tx0, ty0, width, start; // the box, row-major, and its first candidate in the wave's list
+  float invWidth;                  // 1 / width, rounded
+  uint32_t gaussian;
+  uint32_t maskLo, maskHi;
+  uint32_t offset; // its first entry
+  uint32_t pad[3];
+};
+static_assert(sizeof(EmitOwner) == 48, "48-byte emit owner");
+
 // Candidate `local` of a box `width` tiles wide: its tile. The quotient from
 // the rounded reciprocal is at most a few units off and is corrected exactly.
-__device__ __forceinline__ void box_tile(const WalkOwner &o, uint32_t local, uint32_t &tx, uint32_t &ty) {
+template <class Owner>
+__device__ __forceinline__ void box_tile(const Owner &o, uint32_t local, uint32_t &tx, uint32_t &ty) {
   int q = static_cast<int>(static_cast<float>(local) * o.invWidth);
   int r = static_cast<int>(local) - static_cast<int>(__umul24(static_cast<uint32_t>(q), o.width));
   while (r < 0) {
@@ -136,9 +149,58 @@ __device__ __forceinline__ uint64_t bit_range(uint32_t lo, uint32_t hi) {
   return below_hi & ~below_lo;
 }
 
+// A wave's walk over one big splat's box, 64 candidates a round, row-major:
+// lane l starts at candidate l and every round moves 64 on (a carry of at most
+// one row past the wrap, since 64 mod width < width).
+struct BoxCursor {
+  uint32_t col, row, stepCols, stepRows, width;
+  __device__ __forceinline__ BoxCursor(uint32_t lane, uint32_t w) : width(w) {
+    // lane / w and 64 / w from a rounded reciprocal, corrected exactly (w >= 1).
+    const float inv = 1.0f / static_cast<float>(w);
+    row = static_cast<uint32_t>(static_cast<float>(lane) * inv);
+    while (row * w > lane) --row;
+    while ((row + 1u) * w <= lane) ++row;
+    col = lane - row * w;
+    stepRows = static_cast<uint32_t>(64.0f * inv);
+    while (stepRows * w > 64u) --stepRows;
+    while ((stepRows + 1u) * w <= 64u) ++stepRows;
+    stepCols = 64u - stepRows * w;
+  }
+  __device__ __forceinline__ void advance() {
+    col += stepCols;
+    row += stepRows;
+    if (col >= width) {
+      col -= width;
+      row += 1u;
+    }
+  }
+};
+
+// One of 64 listed big splats, as every lane of a wave reads it from the lane
+// that loaded it (a wave loads 64 at once so that their loads overlap).
+struct BigSplat {
+  TileTest test;
+  uint32_t tx0, ty0, width, candidates, gaussian, first; // first: its first list entry, or (counting) its first bit word
+};
+// A big splat whose reached-tile bits did not fit the bit buffer (its list pass tests its tiles again).
+constexpr uint32_t kNoBits = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t j) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(j)));
+}
+__device__ __forceinline__ float lane_f32(float v, uint32_t j) { return __uint_as_float(lane_u32(__float_as_uint(v), j)); }
+__device__ __forceinline__ BigSplat big_splat_of_lane(const BigSplat &b, uint32_t j) {
+  return BigSplat{TileTest{lane_f32(b.test.qx, j), lane_f32(b.test.qy, j), lane_f32(b.test.qz, j), lane_f32(b.test.cx, j),
+                           lane_f32(b.test.cy, j), lane_f32(b.test.limit, j), lane_f32(b.test.iqx, j), lane_f32(b.test.iqz, j)},
+                  lane_u32(b.tx0, j), lane_u32(b.ty0, j), lane_u32(b.width, j), lane_u32(b.candidates, j), lane_u32(b.gaussian, j),
+                  lane_u32(b.first, j)};
+}
+
 // Splats whose bounding box has at most this many tiles keep the outcome of
-// every candidate as one bit (row-major over the box); larger ones are tested
-// again when the lists are written.
+// every candidate as one bit (row-major over the box) and are counted and
+// listed by the waves' shared walks; larger ones ("big" splats, a few per
+// scene with ordinary sizes, very many with heavy-tailed scales) are counted
+// and listed by their own kernels, a wave per splat, so that one large box
+// does not hold up a wave's other 63 splats.
 constexpr uint32_t kMaskTiles = 64;
 
 // Per Gaussian, in Gaussian order: the reached-tile bits and the box they refer to.

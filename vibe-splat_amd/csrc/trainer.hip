@@ -78,7 +78,14 @@ float half_to_float(uint32_t h) {
 
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// The big splats' bit words a view keeps room for: one per Gaussian and one per
+// 16 tile entries (boxes with up to four candidates per listed entry keep
+// their bits; the rest are tested again when listed).
+uint64_t big_bit_words(const ViewBuffers &b) { return std::min<uint64_t>(b.bigBits.bytes() / 8, 0xFFFFFFFFull); }
+
 } // namespace
+
+uint64_t Trainer::big_bit_words_for(uint64_t entries) const { return count_ + entries / 16; }
 
 void DeviceArray::ensure(size_t bytes) {
   if (bytes <= bytes_ && ptr_ != nullptr) return;
@@ -238,6 +245,8 @@ ViewBuffers &Trainer::view_buffers(size_t k) {
     v->coverage.ensure(n * 4);
     v->tileCount.ensure(n * 4);
     v->tileRecords.ensure(n * sizeof(TileRecord));
+    v->bigList.ensure(n * 4);
+    v->bigBits.ensure(n * 8); // grows with the entries (grow_entries)
     if (fixedPoint_) {
       v->rawFixed.ensure(n * kRawPerSplat * 8);
       launch_fill_u32(v->stream, v->rawFixed.as<uint32_t>(), n * kRawPerSplat * 2, 0u);
@@ -514,6 +523,7 @@ void Trainer::grow_entries(ViewBuffers &b, uint64_t entries) {
     a->ensure(capacity * 4);
   }
   b.sortScratch.ensure(std::max(radix_sort_scratch_words(capacity), radix_sort_scratch_words(count_)) * 4);
+  b.bigBits.ensure(big_bit_words_for(capacity) * 8);
   b.entryCapacity = capacity;
 }
 
@@ -593,11 +603,12 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   ViewParams v = view;
   v.tileW = b.tileW;
   v.tileH = b.tileH;
-  launch_fill_u32(s, counters, 3, 0u); // not the sticky overflow word
+  launch_fill_u32(s, counters, kClearedPerPass, 0u); // not the sticky overflow word
   launch_project(s, positions_.as<float>(), scales_.as<float>(), rotations_.as<float>(), opacities_.as<float>(),
                  shWordsBuf_.as<uint32_t>(), filterSet_ ? filter3d_.as<float>() : nullptr, v, options_.mip_filter_2d != 0, dropoutNow_,
                  shRetained_, shSeed_, b.gridW, b.gridH, b.splats.as<Splat>(), b.keys.as<uint32_t>(), b.values.as<uint32_t>(),
-                 b.tileCount.as<uint32_t>(), b.tileRecords.as<TileRecord>(), counters);
+                 b.tileCount.as<uint32_t>(), b.tileRecords.as<TileRecord>(), counters, b.bigList.as<uint32_t>(),
+                 b.bigBits.as<uint64_t>(), big_bit_words(b));
   mark("project");
   launch_radix_sort(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.tmpKeys.as<uint32_t>(), b.tmpValues.as<uint32_t>(), nullptr, count_,
                     32, b.sortScratch.as<uint32_t>());
@@ -608,7 +619,8 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   launch_fill_u32(s, b.ranges.as<uint32_t>(), static_cast<uint64_t>(b.tiles) * 2, 0u);
   if (b.entryCapacity > 0) {
     launch_bin_emit(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.splats.as<Splat>(), b.tileRecords.as<TileRecord>(), count_, b.gridW,
-                    b.tileW, b.tileH, b.coverage.as<uint32_t>(), counters, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>());
+                    b.tileW, b.tileH, b.coverage.as<uint32_t>(), counters, b.bigList.as<uint32_t>(), b.bigBits.as<uint64_t>(),
+                    b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>());
     launch_radix_sort(s, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tmpEntryKeys.as<uint32_t>(), b.tmpEntryValues.as<uint32_t>(),
                       counters + kEntriesUsed, static_cast<uint32_t>(b.entryCapacity), b.tileBits, b.sortScratch.as<uint32_t>());
     launch_tile_ranges(s, b.entryKeys.as<uint32_t>(), counters, static_cast<uint32_t>(b.entryCapacity), b.ranges.as<uint2>());
@@ -989,6 +1001,8 @@ void Trainer::render_into(const vsplat_camera &camera, float *deviceOut, uint64_
   carve(b.splats, n * sizeof(Splat));
   for (DeviceArray *a : {&b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount}) carve(*a, n * 4);
   carve(b.tileRecords, n * sizeof(TileRecord));
+  carve(b.bigList, n * 4);
+  carve(b.bigBits, (n + entries / 16) * 8);
   for (DeviceArray *a : {&b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues}) carve(*a, entries * 4);
   carve(b.stops, uint64_t{VSPLAT_NOALLOC_MAX_PIXELS} * 4);
   carve(b.ranges, uint64_t{VSPLAT_NOALLOC_MAX_TILES} * 8);
@@ -1191,7 +1205,7 @@ uint64_t Trainer::capture_key(uint32_t n) const {
   for (uint32_t k = 0; k < n && k < views_.size(); ++k) {
     const ViewBuffers &b = *views_[k];
     for (const DeviceArray *a : {&b.splats, &b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount, &b.tileRecords,
-                                 &b.rawFixed, &b.rawFloat, &b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues, &b.image,
+                                 &b.bigList, &b.bigBits, &b.rawFixed, &b.rawFloat, &b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues, &b.image,
                                  &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.sortScratch, &b.scanScratch, &b.counters,
                                  &b.sums})
       array(*a);
@@ -1515,10 +1529,11 @@ uint64_t Trainer::bytes_per_gaussian() const {
   const uint64_t floats = 3 + 3 + 4 + 1 + shFloats_;
   // Values (SH as packed words) and one optimiser word per parameter.
   const uint64_t parameters = (floats - shFloats_ + shWords_) * 4 + floats * 4;
-  // Splat, sort keys and values with copies, coverage, tile count, filter, tile
-  // record, sums, SH-pass data, the direction term between the chain kernels, statistics.
+  // Splat, sort keys and values with copies, coverage, tile count, filter, big
+  // splat list, tile record, sums, SH-pass data, the direction term between the
+  // chain kernels, statistics.
   const uint64_t raw = fixedPoint_ ? kRawPerSplat * 8 : kRawPerSplat * 4;
-  return parameters + sizeof(Splat) + 4 * 7 + sizeof(TileRecord) + raw + kShPassFloats * 4 + 16 + 32;
+  return parameters + sizeof(Splat) + 4 * 8 + sizeof(TileRecord) + raw + kShPassFloats * 4 + 16 + 32;
 }
 
 uint64_t Trainer::gpu_bytes() const {
@@ -1527,12 +1542,12 @@ uint64_t Trainer::gpu_bytes() const {
   for (size_t k = 0; k < views_.size(); ++k) {
     const ViewBuffers &b = *views_[k];
     for (const DeviceArray *a : {&b.image, &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.entryKeys, &b.entryValues,
-                                 &b.tmpEntryKeys, &b.tmpEntryValues, &b.exposed, &b.edge, &b.exposureTotals, &b.errorMax}) {
+                                 &b.tmpEntryKeys, &b.tmpEntryValues, &b.bigBits, &b.exposed, &b.edge, &b.exposureTotals, &b.errorMax}) {
       total += a->bytes();
     }
     if (k == 0) continue;
     for (const DeviceArray *a : {&b.splats, &b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount, &b.tileRecords,
-                                 &b.rawFixed, &b.rawFloat, &b.sortScratch, &b.scanScratch}) {
+                                 &b.bigList, &b.rawFixed, &b.rawFloat, &b.sortScratch, &b.scanScratch}) {
       total += a->bytes();
     }
   }

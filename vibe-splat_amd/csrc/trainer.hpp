@@ -98,7 +98,7 @@ struct ViewBuffers {
   uint32_t lossTiles = 0;          // the loss's 16x16 tiles
   uint64_t entryCapacity = 0;
   // Per Gaussian.
-  DeviceArray splats, keys, values, tmpKeys, tmpValues, coverage, tileCount, tileRecords, rawFixed, rawFloat;
+  DeviceArray splats, keys, values, tmpKeys, tmpValues, coverage, tileCount, tileRecords, bigList, bigBits, rawFixed, rawFloat;
   // Per tile entry.
   DeviceArray entryKeys, entryValues, tmpEntryKeys, tmpEntryValues;
   // Per pixel and per tile.
@@ -197,6 +197,7 @@ private:
   std::pair<uint32_t, uint32_t> tile_shape_for(int width, int height) const;
   bool needs_tuning(int width, int height) const;
   void grow_entries(ViewBuffers &b, uint64_t entries);
+  uint64_t big_bit_words_for(uint64_t entries) const;
   void stage_target(const uint8_t *target, size_t bytes);
   // The passes up to the backward pass on b's stream, then (chain) the chain rule.
   void enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode,
