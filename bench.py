@@ -280,7 +280,8 @@ def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) ->
             "tile_shape": tile_shape, "tiles_chosen": {f"{k[0]}x{k[1]}": f"{v[0]}x{v[1]}" for k, v in shapes.items()}}
 
 
-def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str, views: int = 8, fixed: bool = True) -> dict:
+def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str, views: int = 8, fixed: bool = True,
+                     schedule: int = vsplat.BATCH_VIEW_STREAMS) -> dict:
     """The target comparison at one size: the 1M-Gaussian cloud of `value`, seen
     by eight 1280x720 views on a ring of radius 3 (the ring workload's cameras,
     all at 1280x720), gradients mode (fwd+bwd, gradients summed over the views),
@@ -293,7 +294,8 @@ def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str
                       (0.0, -1.0, 0.0), 60.0, 1280, 720) for k in range(views)]
     targets = [F.smooth_target_explicit(1280, 720, 1016 + k) for k in range(views)]
     trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed,
-                                                            sh_storage=vsplat.SH_BF16, tile_shape=TILE_SHAPES[tile_shape]))
+                                                            sh_storage=vsplat.SH_BF16, tile_shape=TILE_SHAPES[tile_shape],
+                                                            batch_raster=schedule))
     trainer.set_stage_timing(False)
     buffers = [vsplat.DeviceBuffer(t) for t in targets]
     ptrs = [b.ptr for b in buffers]
@@ -316,6 +318,8 @@ def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str
         trainer.close()
     return {"workload": f"1M Gaussians, SH degree 3 stored as bf16, {views} views of 1280x720 on a ring of radius 3, "
                         "gradients mode, one captured graph", "sums": "64-bit fixed point" if fixed else "float32 atomics",
+            "schedule": "one forward and one backward launch over all views" if schedule == vsplat.BATCH_ONE_LAUNCH
+            else "each view's passes on its own stream",
             "views": views, "graph_ms": round(graph, 4),
             "graph_ms_per_view": round(graph / views, 4), "tile_shape": tile_shape,
             "tiles_chosen": None if chosen is None else f"{chosen[0]}x{chosen[1]}"}
@@ -498,6 +502,14 @@ def main():
         same_float["plain_vs_batched_per_view"] = round(result["value"] / same_float["graph_ms_per_view"], 4)
         result["batched_1280x720_float_sums"] = same_float
         log(f"  {same_float['graph_ms_per_view']} ms per view, {same_float['plain_vs_batched_per_view']}x the plain single view")
+        log("batched bf16 fwd+bwd, one forward and one backward launch over all views: eight 1280x720 views")
+        for key, fixed_sums in (("batched_1280x720_one_launch", True), ("batched_1280x720_one_launch_float_sums", False)):
+            one = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed=fixed_sums,
+                                   schedule=vsplat.BATCH_ONE_LAUNCH)
+            one["plain_single_view_ms"] = result["value"]
+            one["plain_vs_batched_per_view"] = round(result["value"] / one["graph_ms_per_view"], 4)
+            result[key] = one
+            log(f"  {key}: {one['graph_ms_per_view']} ms per view, {one['plain_vs_batched_per_view']}x the plain single view")
     if not args.no_other_forms and args.count is None:
         result["other_forms"] = {}
         for key, index, fixed_sums, form_options in (

@@ -83,7 +83,15 @@ typedef struct {
    * step_views, capture_views, or vsplat_autotune_tiles) and keeps the fastest.
    * Every shape gives the same images and sums; only the tile lists differ. */
   int32_t tile_shape;
+  /* How a batch of views runs (no reference counterpart; the results are the
+   * same bits either way): VSPLAT_BATCH_VIEW_STREAMS, each view's passes on a
+   * stream of its own, the views overlapping; or VSPLAT_BATCH_ONE_LAUNCH, each
+   * view's lists on its own stream, then one forward launch and one backward
+   * launch over every view's 8x8 pixel blocks (per tile shape). */
+  int32_t batch_raster;
 } vsplat_trainer_options;
+
+enum { VSPLAT_BATCH_VIEW_STREAMS = 0, VSPLAT_BATCH_ONE_LAUNCH = 1 };
 
 enum { VSPLAT_TILES_16X16 = 0, VSPLAT_TILES_8X8 = 1, VSPLAT_TILES_32X8 = 2, VSPLAT_TILES_AUTO = 3 };
 

@@ -4,7 +4,9 @@ autotuner, one fwd+bwd+Adam step captured as a HIP graph.
 
 * At 20k Gaussians, bit for bit against Oracle B: each view's fixed-point sums
   and chain rule, the gradients added in view order in float32, then Oracle
-  B's Adam (ob_adam) from the sum; each view's visible count and entries.
+  B's Adam (ob_adam) from the sum; each view's visible count and entries;
+  under both batch schedules (the views' passes on their own streams, or one
+  forward and one backward launch over all views' 8x8 blocks per tile shape).
 * At the full 1M Gaussians, through properties: each view's tile lists
   sorted by tile and, within a tile, in depth order (the stable sort), the
   ranges partitioning the entries; two trainers replaying the captured step
@@ -62,7 +64,8 @@ def _size_entries(t, cams, targets):
     t.reset_densify_stats()
 
 
-def test_config3_captured_step_bit_exact_at_20k():
+@pytest.mark.parametrize("schedule", [vsplat.BATCH_VIEW_STREAMS, vsplat.BATCH_ONE_LAUNCH])
+def test_config3_captured_step_bit_exact_at_20k(schedule):
     w = F.config(3, count=20_000)
     assert [(c.width, c.height) for c in w.cameras] == SIZES
     cloud, cams, targets = w.cloud, w.cameras, w.targets
@@ -71,7 +74,7 @@ def test_config3_captured_step_bit_exact_at_20k():
                                                  sh_storage=vsplat.SH_BF16)
     buffers = [vsplat.DeviceBuffer(x) for x in targets]
     try:
-        with vsplat.Trainer(cloud, OPTIONS) as t:
+        with vsplat.Trainer(cloud, dataclasses.replace(OPTIONS, batch_raster=schedule)) as t:
             shapes = t.autotune_views(cams, [d.ptr for d in buffers])
             print(f"tile shapes: {shapes}", flush=True)
             _size_entries(t, cams, targets)

@@ -11,6 +11,10 @@ the GPU against Oracle B:
 * the queued form with device targets gives the same result;
 * a batch captured as a HIP graph and replayed steps as the queued form does,
   replay after replay, and refuses to replay once a buffer it uses is remade.
+
+The sums, steps and captured replays are checked under both batch schedules:
+each view's passes on a stream of its own, and one forward launch and one
+backward launch over every view's 8x8 blocks (batch_raster).
 """
 import math
 
@@ -61,10 +65,14 @@ def _model_sum(cloud, cams, targets, sh_storage=0):
     return total, stats, per_view, m.words
 
 
-def test_gradients_summed_over_views_bit_exact():
+SCHEDULES = [vsplat.BATCH_VIEW_STREAMS, vsplat.BATCH_ONE_LAUNCH]
+
+
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_gradients_summed_over_views_bit_exact(schedule):
     cloud, cams, targets = _ring()
     want, want_stats, per_view, _ = _model_sum(cloud, cams, targets)
-    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, batch_raster=schedule)) as t:
         stats = t.step_views(cams, targets)
         g = t.gradients()
         dens = t.densify_stats()
@@ -77,12 +85,14 @@ def test_gradients_summed_over_views_bit_exact():
         assert abs(s.l1 - l1) < 1e-6 and abs(s.ssim - ssim) < 1e-6
 
 
-def test_more_views_than_one_chain_launch_takes():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_more_views_than_one_chain_launch_takes(schedule):
     """Nine views: the chain rule runs in two chunks (eight views, then one
-    continuing the sums), which must not change the sums."""
+    continuing the sums), and the one-launch schedule's raster launches take
+    eight views each, which must not change the sums."""
     cloud, cams, targets = _ring(1500, 2, 91, sizes=[(48, 32), (64, 40), (40, 40)] * 3)
     want, want_stats, _, _ = _model_sum(cloud, cams, targets)
-    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True)) as t:
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, batch_raster=schedule)) as t:
         t.step_views(cams, targets)
         g = t.gradients()
         dens = t.densify_stats()
@@ -103,13 +113,15 @@ def test_batch_of_one_steps_as_step_does():
         np.testing.assert_array_equal(_bits(x), _bits(y))
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("sh_storage", [0, 1])
-def test_batch_of_two_steps_as_the_model(sh_storage):
+def test_batch_of_two_steps_as_the_model(sh_storage, schedule):
     cloud, cams, targets = _ring()
     want_g, _, _, words = _model_sum(cloud, cams[:2], targets[:2], sh_storage)
     pos, sc, rot, op, w = O.adam_from_gradients(cloud, words, want_g, RATE_TUPLE, rest_scale=RATES.sh_rest_scale,
                                                 sh_storage=sh_storage)
-    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, sh_storage=sh_storage)) as t:
+    options = vsplat.TrainerOptions(force_fixed_point=True, sh_storage=sh_storage, batch_raster=schedule)
+    with vsplat.Trainer(cloud, options) as t:
         t.step_views(cams[:2], targets[:2], RATES)
         c = t.cloud()
     np.testing.assert_array_equal(_bits(c.positions.reshape(-1)), _bits(pos))
@@ -139,9 +151,10 @@ def test_queued_batch_matches_the_synchronous_one():
         np.testing.assert_array_equal(_bits(x), _bits(y))
 
 
-def test_captured_batch_replays_as_queued_steps():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_captured_batch_replays_as_queued_steps(schedule):
     cloud, cams, targets = _ring()
-    options = vsplat.TrainerOptions(force_fixed_point=True)
+    options = vsplat.TrainerOptions(force_fixed_point=True, batch_raster=schedule)
     buffers = [vsplat.DeviceBuffer(x) for x in targets]
     ptrs = [d.ptr for d in buffers]
     try:

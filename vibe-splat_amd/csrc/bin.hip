@@ -171,8 +171,8 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
 // once (one per lane), then walks each one's box in turn, 64 candidates a
 // round, row-major, each round's outcome read from the bits the counting pass
 // kept (or tested again where it kept none), the reached ones numbered by a
-// prefix count from the splat's first entry. Waves take the listed splats 64
-// at a time, in turn.
+// prefix count from the splat's first entry. The listed splats are dealt out
+// over all the waves.
 __global__ __launch_bounds__(256) void big_emit_kernel(const Splat *__restrict__ splats, const TileRecord *__restrict__ records,
                                                        const uint32_t *__restrict__ bigList, const uint64_t *__restrict__ bigBits,
                                                        uint32_t gridW, TileShape shape, const uint32_t *counters,
@@ -181,11 +181,14 @@ __global__ __launch_bounds__(256) void big_emit_kernel(const Splat *__restrict__
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t waves = gridDim.x * (blockDim.x / 64u), wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
   const uint32_t listed = counters[kBigSplats], limit = counters[kEntriesUsed];
-  for (uint32_t base = wave * 64u; base < listed; base += waves * 64u) {
+  // Wave w takes listed splats w, w + waves, w + 2 waves, ..., 64 at a time (one
+  // per lane), so that a short list still spreads over every wave.
+  for (uint32_t base = 0; wave + base * waves < listed; base += 64u) {
+    const uint32_t mineAt = wave + (base + lane) * waves;
     BigSplat mine{};
     uint32_t bits = kNoBits;
-    if (base + lane < listed) {
-      const uint32_t i = bigList[base + lane];
+    if (mineAt < listed) {
+      const uint32_t i = bigList[mineAt];
       const TileRecord rec = records[i];
       bits = rec.maskHi;
       TileTest test{};
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(256) void big_emit_kernel(const Splat *__restrict__
       mine = BigSplat{test, rec.origin & 0xFFFFu, rec.origin >> 16, rec.size & 0xFFFFu, (rec.size & 0xFFFFu) * (rec.size >> 16), i,
                       rec.maskLo};
     }
-    const uint32_t inChunk = listed - base < 64u ? listed - base : 64u;
+    const uint32_t inChunk = static_cast<uint32_t>(__popcll(__ballot(mineAt < listed))); // a prefix of the lanes
     for (uint32_t j = 0; j < inChunk; ++j) {
       const BigSplat b = big_splat_of_lane(mine, j);
       const uint32_t bitStart = lane_u32(bits, j);

@@ -126,6 +126,32 @@ void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *v
                      const uint32_t *bigList, const uint64_t *bigBits, uint32_t *entryKeys, uint32_t *entryValues);
 void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges);
 
+// One view of a batched raster launch (its buffers and sizes), and up to
+// kRasterViewsPerLaunch of them, passed by value as the kernel's argument.
+// The views' 8x8 blocks follow each other: view v's are firstBlock ..
+// firstBlock + blocks - 1 of the grid.
+struct RasterView {
+  const Splat *splats;
+  const uint32_t *entryValues;
+  const uint2 *ranges;
+  float4 *image;
+  uint32_t *stops;
+  const float4 *adjoint;
+  int64_t *rawFixed;
+  float *rawFloat;
+  float fixedScale;
+  uint32_t width, height, gridW, firstBlock, blocks;
+};
+constexpr uint32_t kRasterViewsPerLaunch = 8;
+struct RasterViews {
+  RasterView view[kRasterViewsPerLaunch];
+  uint32_t count;
+};
+// The batched forward and backward over views whose lists share one tile shape.
+void launch_forward_views(hipStream_t stream, const RasterViews &views, uint32_t tileW, uint32_t tileH, const float background[3]);
+void launch_backward_views(hipStream_t stream, bool fixedPoint, bool recompute, const RasterViews &views, uint32_t tileW, uint32_t tileH,
+                           const float background[3]);
+
 // The raster kernels for one tile shape: 8x8, 16x16 or 32x8 (tileW x tileH).
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
                     uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops);

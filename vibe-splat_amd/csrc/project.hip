@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
 
 // Big splats' reached tiles, counted by the waves: a wave loads 64 listed
 // splats at once (one per lane), then walks each one's box in turn, 64
-// candidates a round. Waves take the listed splats 64 at a time, in turn.
+// candidates a round. The listed splats are dealt out over all the waves.
 __global__ __launch_bounds__(256) void big_count_kernel(const Splat *__restrict__ splats, TileRecord *__restrict__ records,
                                                         const uint32_t *__restrict__ bigList, uint32_t *counters, TileShape shape,
                                                         uint64_t *__restrict__ bigBits, uint32_t bitCapacity,
@@ -326,11 +326,14 @@ __global__ __launch_bounds__(256) void big_count_kernel(const Splat *__restrict_
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t waves = gridDim.x * (blockDim.x / 64u), wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
   const uint32_t listed = counters[kBigSplats];
-  for (uint32_t base = wave * 64u; base < listed; base += waves * 64u) {
+  // Wave w takes listed splats w, w + waves, w + 2 waves, ..., 64 at a time (one
+  // per lane), so that a short list still spreads over every wave.
+  for (uint32_t base = 0; wave + base * waves < listed; base += 64u) {
+    const uint32_t mineAt = wave + (base + lane) * waves;
     BigSplat mine{};
     uint32_t rounds = 0;
-    if (base + lane < listed) {
-      const uint32_t i = bigList[base + lane];
+    if (mineAt < listed) {
+      const uint32_t i = bigList[mineAt];
       const Splat s = load_splat(splats, i);
       const TileRecord rec = records[i];
       mine = BigSplat{tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy), rec.origin & 0xFFFFu, rec.origin >> 16, rec.size & 0xFFFFu,
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(256) void big_count_kernel(const Splat *__restrict_
     const uint32_t bitStart = wordBase + (end - rounds);
     // Past the capacity a splat keeps no bits, and its list pass tests its tiles again.
     mine.first = bitStart + rounds <= bitCapacity ? bitStart : kNoBits;
-    const uint32_t inChunk = listed - base < 64u ? listed - base : 64u;
+    const uint32_t inChunk = static_cast<uint32_t>(__popcll(__ballot(mineAt < listed))); // a prefix of the lanes
     uint32_t myHits = 0;
     for (uint32_t j = 0; j < inChunk; ++j) {
       const BigSplat b = big_splat_of_lane(mine, j);
@@ -360,7 +363,7 @@ __global__ __launch_bounds__(256) void big_count_kernel(const Splat *__restrict_
       }
       if (lane == j) myHits = hits;
     }
-    if (base + lane < listed) {
+    if (mineAt < listed) {
       tileCount[mine.gaussian] = myHits;
       records[mine.gaussian].maskHi = mine.first; // where its bits are, for the list pass
     }

@@ -169,14 +169,14 @@ __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats
   }
 }
 
+// The forward pass of one 8x8 block (item) of a view.
 template <uint32_t TW, uint32_t TH>
-__global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
-                                                         const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
-                                                         uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
-                                                         uint32_t *__restrict__ stops) {
-  __shared__ Staged batch[kBlock];
+__device__ __forceinline__ void forward_block(uint32_t item, const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                              const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
+                                              float bgR, float bgG, float bgB, float4 *__restrict__ image, uint32_t *__restrict__ stops,
+                                              Staged *batch) {
   uint32_t tile;
-  const BlockGeometry geo = block_geometry<TW, TH>(blockIdx.x, gridW, width, height, tile);
+  const BlockGeometry geo = block_geometry<TW, TH>(item, gridW, width, height, tile);
   if (geo.x0 >= width || geo.y0 >= height) return; // a block past the image's edge (uniform)
   const uint32_t lx = threadIdx.x & 7u, ly = threadIdx.x >> 3;
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
@@ -190,6 +190,34 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict
     image[at] = make_float4(fma32(T, bgR, r), fma32(T, bgG, g), fma32(T, bgB, b), logT);
     if (stops != nullptr) stops[at] = stop; // none when the backward recomputes them
   }
+}
+
+template <uint32_t TW, uint32_t TH>
+__global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                                         const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
+                                                         uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
+                                                         uint32_t *__restrict__ stops) {
+  __shared__ Staged batch[kBlock];
+  forward_block<TW, TH>(blockIdx.x, splats, entryValues, ranges, width, height, gridW, bgR, bgG, bgB, image, stops, batch);
+}
+
+// The view of a batched launch that block `item` belongs to, and the block's
+// index within it (the views' blocks follow each other in view order).
+__device__ __forceinline__ uint32_t view_of(const RasterViews &views, uint32_t &item) {
+  uint32_t v = 0;
+  while (v + 1 < views.count && item >= views.view[v + 1].firstBlock) ++v;
+  item -= views.view[v].firstBlock;
+  return v;
+}
+
+// The forward pass of a batch's views in one launch: one workgroup per
+// (view, 8x8 block) pair, the views' tile lists all of shape TW x TH.
+template <uint32_t TW, uint32_t TH>
+__global__ __launch_bounds__(kBlock) void forward_views_kernel(RasterViews views, float bgR, float bgG, float bgB) {
+  __shared__ Staged batch[kBlock];
+  uint32_t item = blockIdx.x;
+  const RasterView &w = views.view[view_of(views, item)];
+  forward_block<TW, TH>(item, w.splats, w.entryValues, w.ranges, w.width, w.height, w.gridW, bgR, bgG, bgB, w.image, w.stops, batch);
 }
 
 // The fixed-point quantisation of a wave total (train_backward_fixed.comp:19-21),
@@ -296,16 +324,15 @@ __device__ __forceinline__ bool wave_emits(uint32_t lane) { return (lane & 9u) =
 // mark (the adjoint's fourth value) and its edge weight times the blending
 // weight, each over 3 pixels of the image.
 template <bool Fixed, bool Recompute, bool Score, uint32_t TW, uint32_t TH>
-__global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
-                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
-                                                          float bgR, float bgG, float bgB, const float4 *__restrict__ image,
-                                                          const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
-                                                          float fixedScale, unsigned long long *__restrict__ rawFixed,
-                                                          float *__restrict__ rawFloat, const float *__restrict__ edge) {
-  __shared__ Staged batch[kBlock];
-  __shared__ uint32_t batchIndex[kBlock];
+__device__ __forceinline__ void backward_block(uint32_t item, const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                               const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
+                                               float bgR, float bgG, float bgB, const float4 *__restrict__ image,
+                                               const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
+                                               float fixedScale, unsigned long long *__restrict__ rawFixed,
+                                               float *__restrict__ rawFloat, const float *__restrict__ edge, Staged *batch,
+                                               uint32_t *batchIndex) {
   uint32_t tile;
-  const BlockGeometry geo = block_geometry<TW, TH>(blockIdx.x, gridW, width, height, tile);
+  const BlockGeometry geo = block_geometry<TW, TH>(item, gridW, width, height, tile);
   if (geo.x0 >= width || geo.y0 >= height) return; // a block past the image's edge (uniform)
   const uint32_t lane = threadIdx.x, lx = lane & 7u, ly = lane >> 3;
   const uint32_t bits = lane_bits(lx, ly);
@@ -423,6 +450,33 @@ __global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restric
   }
 }
 
+
+template <bool Fixed, bool Recompute, bool Score, uint32_t TW, uint32_t TH>
+__global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
+                                                          float bgR, float bgG, float bgB, const float4 *__restrict__ image,
+                                                          const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
+                                                          float fixedScale, unsigned long long *__restrict__ rawFixed,
+                                                          float *__restrict__ rawFloat, const float *__restrict__ edge) {
+  __shared__ Staged batch[kBlock];
+  __shared__ uint32_t batchIndex[kBlock];
+  backward_block<Fixed, Recompute, Score, TW, TH>(blockIdx.x, splats, entryValues, ranges, width, height, gridW, bgR, bgG, bgB, image,
+                                                  adjoint, stops, fixedScale, rawFixed, rawFloat, edge, batch, batchIndex);
+}
+
+// The backward pass of a batch's views in one launch, one workgroup per
+// (view, 8x8 block) pair; each view's sums go to its own raw buffer.
+template <bool Fixed, bool Recompute, uint32_t TW, uint32_t TH>
+__global__ __launch_bounds__(kBlock) void backward_views_kernel(RasterViews views, float bgR, float bgG, float bgB) {
+  __shared__ Staged batch[kBlock];
+  __shared__ uint32_t batchIndex[kBlock];
+  uint32_t item = blockIdx.x;
+  const RasterView &w = views.view[view_of(views, item)];
+  backward_block<Fixed, Recompute, false, TW, TH>(item, w.splats, w.entryValues, w.ranges, w.width, w.height, w.gridW, bgR, bgG, bgB,
+                                                  w.image, w.adjoint, w.stops, w.fixedScale,
+                                                  reinterpret_cast<unsigned long long *>(w.rawFixed), w.rawFloat, nullptr, batch,
+                                                  batchIndex);
+}
 } // namespace
 
 // Calls LAUNCH(TW, TH) with the tile shape as compile-time constants.
@@ -483,6 +537,44 @@ void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const 
 #undef VSPLAT_BACKWARD_TTF
 #undef VSPLAT_BACKWARD_TTT
 #undef VSPLAT_BACKWARD_SHAPE
+}
+
+// The batched raster launches: one grid over every view's 8x8 blocks.
+void launch_forward_views(hipStream_t stream, const RasterViews &views, uint32_t tileW, uint32_t tileH, const float background[3]) {
+  if (views.count == 0) return;
+  const uint32_t blocks = views.view[views.count - 1].firstBlock + views.view[views.count - 1].blocks;
+#define VSPLAT_FORWARD_VIEWS(W, H)                                                                                                 \
+  hipLaunchKernelGGL((forward_views_kernel<W, H>), dim3(blocks), dim3(kBlock), 0, stream, views, background[0], background[1],      \
+                     background[2])
+  VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD_VIEWS)
+#undef VSPLAT_FORWARD_VIEWS
+}
+
+void launch_backward_views(hipStream_t stream, bool fixedPoint, bool recompute, const RasterViews &views, uint32_t tileW, uint32_t tileH,
+                           const float background[3]) {
+  if (views.count == 0) return;
+  const uint32_t blocks = views.view[views.count - 1].firstBlock + views.view[views.count - 1].blocks;
+#define VSPLAT_BACKWARD_VIEWS_SHAPE(F, R, W, H)                                                                                     \
+  hipLaunchKernelGGL((backward_views_kernel<F, R, W, H>), dim3(blocks), dim3(kBlock), 0, stream, views, background[0], background[1], \
+                     background[2])
+#define VSPLAT_BACKWARD_VIEWS_TT(W, H) VSPLAT_BACKWARD_VIEWS_SHAPE(true, true, W, H)
+#define VSPLAT_BACKWARD_VIEWS_TF(W, H) VSPLAT_BACKWARD_VIEWS_SHAPE(true, false, W, H)
+#define VSPLAT_BACKWARD_VIEWS_FT(W, H) VSPLAT_BACKWARD_VIEWS_SHAPE(false, true, W, H)
+#define VSPLAT_BACKWARD_VIEWS_FF(W, H) VSPLAT_BACKWARD_VIEWS_SHAPE(false, false, W, H)
+  if (fixedPoint && recompute) {
+    VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_VIEWS_TT)
+  } else if (fixedPoint) {
+    VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_VIEWS_TF)
+  } else if (recompute) {
+    VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_VIEWS_FT)
+  } else {
+    VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_BACKWARD_VIEWS_FF)
+  }
+#undef VSPLAT_BACKWARD_VIEWS_FF
+#undef VSPLAT_BACKWARD_VIEWS_FT
+#undef VSPLAT_BACKWARD_VIEWS_TF
+#undef VSPLAT_BACKWARD_VIEWS_TT
+#undef VSPLAT_BACKWARD_VIEWS_SHAPE
 }
 
 } // namespace vsplat

@@ -129,6 +129,9 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
     throw Error(VSPLAT_INVALID_ARGUMENT, "sh_storage must be VSPLAT_SH_HALF or VSPLAT_SH_BF16");
   }
   shFormat_ = options.sh_storage == VSPLAT_SH_BF16 ? kShBf16 : kShHalf;
+  if (options.batch_raster != VSPLAT_BATCH_VIEW_STREAMS && options.batch_raster != VSPLAT_BATCH_ONE_LAUNCH) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, "batch_raster must be VSPLAT_BATCH_VIEW_STREAMS or VSPLAT_BATCH_ONE_LAUNCH");
+  }
   if (options.tile_shape < VSPLAT_TILES_16X16 || options.tile_shape > VSPLAT_TILES_AUTO) {
     throw Error(VSPLAT_INVALID_ARGUMENT, "tile_shape must be one of VSPLAT_TILES_16X16, _8X8, _32X8 or _AUTO");
   }
@@ -154,6 +157,7 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   const uint32_t adamStart[4] = {0u, 0x3F800000u, 0x3F800000u, 0u}; // no steps; beta^0 = 1
   check(hipMemcpy(adamState_.as<uint32_t>(), adamStart, sizeof(adamStart), hipMemcpyHostToDevice), "adam state");
   check(hipEventCreateWithFlags(&batchStart_, hipEventDisableTiming), "hipEventCreate");
+  check(hipEventCreateWithFlags(&batchRaster_, hipEventDisableTiming), "hipEventCreate");
   view_buffers(0);
   check(hipStreamSynchronize(stream_), "trainer set-up");
 }
@@ -218,6 +222,7 @@ Trainer::~Trainer() {
   release_captured();
   release_views();
   if (batchStart_ != nullptr) (void)hipEventDestroy(batchStart_);
+  if (batchRaster_ != nullptr) (void)hipEventDestroy(batchRaster_);
   for (auto &slot : slots_)
     for (auto &e : slot.events)
       if (e != nullptr) (void)hipEventDestroy(e);
@@ -433,7 +438,7 @@ void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uin
     for (uint32_t k = 0; k < n; ++k) ensure_size(*views_[k], cameras[k].width, cameras[k].height);
     // One run sizes the tile-entry buffers for these shapes.
     for (int attempt = 0;; ++attempt) {
-      enqueue_view_passes(n, cameras, targets, params.data());
+      enqueue_batch_passes(n, cameras, targets, params.data());
       bool overflow = false;
       synchronize_views(stats.data(), n, overflow);
       if (!overflow) break;
@@ -447,7 +452,7 @@ void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uin
     float least = 0.0f;
     for (uint32_t r = 0; r < (repeats > 0 ? repeats : 1); ++r) {
       check(hipEventRecord(begin, stream_), "hipEventRecord");
-      enqueue_view_passes(n, cameras, targets, params.data());
+      enqueue_batch_passes(n, cameras, targets, params.data());
       check(hipEventRecord(end, stream_), "hipEventRecord");
       check(hipEventSynchronize(end), "hipEventSynchronize");
       float ms = 0.0f;
@@ -594,28 +599,59 @@ std::vector<std::pair<std::string, double>> Trainer::timings() const {
   return out;
 }
 
+// A view's passes in order, on its stream, with a stage event after each: the
+// tile lists, the forward pass, the loss and its adjoint, the backward pass,
+// and (chain) the chain rule.
 void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode,
                              const vsplat_learning_rates *rates, bool timed, bool chain) {
-  hipStream_t s = b.stream;
   current_ = nullptr;
   if (timed) begin_timing(true);
-  uint32_t *counters = b.counters.as<uint32_t>();
+  const ViewParams v = with_tiles(b, view);
+  enqueue_lists(b, v, mode, true);
+  launch_forward(b.stream, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH,
+                 options_.background, b.image.as<float4>(), options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>());
+  mark("composite");
+  if (mode == kModeRender) return;
+  enqueue_loss(b, v, deviceTarget);
+  mark("loss");
+  // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
+  // a resolution of 2^-20 (trainer.cpp:586-591 keeps 1/64 in 32 bits).
+  const bool scoring = scoreShare_ > 0.0f;
+  launch_backward(b.stream, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
+                  b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
+                  b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixed_scale(v), b.rawFixed.as<int64_t>(), b.rawFloat.as<float>(),
+                  scoring ? b.edge.as<float>() : nullptr);
+  mark("backward");
+  if (!chain) return;
+  enqueue_chain(b, v, mode, rates, b.stream, b.counters.as<uint32_t>());
+  mark(mode == kModeStep ? "chain rule and step" : "chain rule");
+}
+
+ViewParams Trainer::with_tiles(const ViewBuffers &b, const ViewParams &view) const {
   ViewParams v = view;
   v.tileW = b.tileW;
   v.tileH = b.tileH;
+  return v;
+}
+
+// The projection, the depth sort, the tile counts and the sorted tile lists
+// with their ranges (marks: with the stage events of a timed step).
+void Trainer::enqueue_lists(ViewBuffers &b, const ViewParams &v, uint32_t mode, bool marks) {
+  hipStream_t s = b.stream;
+  uint32_t *counters = b.counters.as<uint32_t>();
   launch_fill_u32(s, counters, kClearedPerPass, 0u); // not the sticky overflow word
   launch_project(s, positions_.as<float>(), scales_.as<float>(), rotations_.as<float>(), opacities_.as<float>(),
                  shWordsBuf_.as<uint32_t>(), filterSet_ ? filter3d_.as<float>() : nullptr, v, options_.mip_filter_2d != 0, dropoutNow_,
                  shRetained_, shSeed_, b.gridW, b.gridH, b.splats.as<Splat>(), b.keys.as<uint32_t>(), b.values.as<uint32_t>(),
                  b.tileCount.as<uint32_t>(), b.tileRecords.as<TileRecord>(), counters, b.bigList.as<uint32_t>(),
                  b.bigBits.as<uint64_t>(), big_bit_words(b));
-  mark("project");
+  if (marks) mark("project");
   launch_radix_sort(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.tmpKeys.as<uint32_t>(), b.tmpValues.as<uint32_t>(), nullptr, count_,
                     32, b.sortScratch.as<uint32_t>());
-  mark("depth sort");
+  if (marks) mark("depth sort");
   launch_coverage_scan(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.tileCount.as<uint32_t>(), count_, b.coverage.as<uint32_t>(),
                        counters, static_cast<uint32_t>(b.entryCapacity), mode == kModeStep, b.scanScratch.as<uint32_t>());
-  mark("count tiles");
+  if (marks) mark("count tiles");
   launch_fill_u32(s, b.ranges.as<uint32_t>(), static_cast<uint64_t>(b.tiles) * 2, 0u);
   if (b.entryCapacity > 0) {
     launch_bin_emit(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.splats.as<Splat>(), b.tileRecords.as<TileRecord>(), count_, b.gridW,
@@ -625,11 +661,14 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
                       counters + kEntriesUsed, static_cast<uint32_t>(b.entryCapacity), b.tileBits, b.sortScratch.as<uint32_t>());
     launch_tile_ranges(s, b.entryKeys.as<uint32_t>(), counters, static_cast<uint32_t>(b.entryCapacity), b.ranges.as<uint2>());
   }
-  mark("tile lists");
-  launch_forward(s, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH,
-                 options_.background, b.image.as<float4>(), options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>());
-  mark("composite");
-  if (mode == kModeRender) return;
+  if (marks) mark("tile lists");
+}
+
+// The loss and its adjoint from the view's image (as its exposure makes it),
+// with the scoring outputs when a view is scored; then the raw sums cleared
+// for the backward pass.
+void Trainer::enqueue_loss(ViewBuffers &b, const ViewParams &v, const uint8_t *deviceTarget) {
+  hipStream_t s = b.stream;
   // The loss sees the image as this view's exposure makes it (trainer.cpp:397-425).
   const uint32_t pixels = v.width * v.height;
   const float4 *seen = b.image.as<float4>();
@@ -661,17 +700,6 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   } else {
     launch_fill_u32(s, b.rawFloat.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat, 0u);
   }
-  mark("loss");
-  // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
-  // a resolution of 2^-20 (trainer.cpp:586-591 keeps 1/64 in 32 bits).
-  const float fixedScale = fixed_scale(v);
-  launch_backward(s, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
-                  b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(), b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixedScale, b.rawFixed.as<int64_t>(),
-                  b.rawFloat.as<float>(), scoring ? b.edge.as<float>() : nullptr);
-  mark("backward");
-  if (!chain) return;
-  enqueue_chain(b, v, mode, rates, s, counters);
-  mark(mode == kModeStep ? "chain rule and step" : "chain rule");
 }
 
 ChainParams Trainer::chain_params(const ViewParams &v, uint32_t mode, const vsplat_learning_rates *rates) const {
@@ -1130,6 +1158,99 @@ void Trainer::enqueue_view_passes(uint32_t n, const vsplat_camera *cameras, cons
   }
 }
 
+// A batch's passes up to each view's backward pass, by the schedule the options ask for.
+void Trainer::enqueue_batch_passes(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params) {
+  if (options_.batch_raster == VSPLAT_BATCH_ONE_LAUNCH) {
+    enqueue_view_passes_one_launch(n, cameras, targets, params);
+  } else {
+    enqueue_view_passes(n, cameras, targets, params);
+  }
+}
+
+// A batch with one forward launch and one backward launch over every view's
+// 8x8 pixel blocks (one per tile shape among the views, eight views a launch):
+// each view's lists on its own stream; once every view's lists are made, the
+// forward launch on the trainer's stream; each view's loss on its own stream;
+// then the backward launch. Each (view, block) pair runs the single-view
+// kernels' code, so the images, sums and gradients are the same bits.
+void Trainer::enqueue_view_passes_one_launch(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params) {
+  queuedViewSizes_.clear();
+  current_ = nullptr; // no stage events in a batch
+  check(hipEventRecord(batchStart_, stream_), "hipEventRecord");
+  for (uint32_t k = 0; k < n; ++k) {
+    ViewBuffers &b = *views_[k];
+    if (b.stream != stream_) check(hipStreamWaitEvent(b.stream, batchStart_, 0), "hipStreamWaitEvent");
+    params[k] = view_params(cameras[k]);
+    enqueue_lists(b, with_tiles(b, params[k]), kModeGradients, false);
+    check(hipEventRecord(b.done, b.stream), "hipEventRecord");
+    queuedViewSizes_.emplace_back(cameras[k].width, cameras[k].height);
+  }
+  for (uint32_t k = 1; k < n; ++k) check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
+  launch_batch_raster(n, params, false);
+  check(hipEventRecord(batchRaster_, stream_), "hipEventRecord");
+  for (uint32_t k = 0; k < n; ++k) {
+    ViewBuffers &b = *views_[k];
+    if (b.stream != stream_) check(hipStreamWaitEvent(b.stream, batchRaster_, 0), "hipStreamWaitEvent");
+    enqueue_loss(b, with_tiles(b, params[k]), targets[k]);
+    uint8_t *slot = viewStats_.as<uint8_t>() + static_cast<size_t>(k) * kViewStatsBytes;
+    check(hipMemcpyAsync(slot, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToDevice, b.stream), "view counters");
+    check(hipMemcpyAsync(slot + kCounterWords * 4, b.sums.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToDevice, b.stream),
+          "view sums");
+    check(hipEventRecord(b.done, b.stream), "hipEventRecord");
+  }
+  uint32_t *status = views_[0]->counters.as<uint32_t>();
+  for (uint32_t k = 1; k < n; ++k) {
+    check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
+    launch_merge_flags(stream_, status, views_[k]->counters.as<uint32_t>());
+  }
+  launch_batch_raster(n, params, true);
+}
+
+// The batch's forward (or backward) passes on the trainer's stream: the views
+// grouped by tile shape, up to kRasterViewsPerLaunch per launch.
+void Trainer::launch_batch_raster(uint32_t n, const ViewParams *params, bool backward) {
+  std::vector<bool> done(n, false);
+  for (uint32_t first = 0; first < n; ++first) {
+    if (done[first]) continue;
+    const uint32_t tileW = views_[first]->tileW, tileH = views_[first]->tileH;
+    RasterViews group{};
+    uint32_t blocks = 0;
+    const auto flush = [&]() {
+      if (group.count == 0) return;
+      if (backward) {
+        launch_backward_views(stream_, fixedPoint_, options_.recompute_transmittance != 0, group, tileW, tileH, options_.background);
+      } else {
+        launch_forward_views(stream_, group, tileW, tileH, options_.background);
+      }
+      group = RasterViews{};
+      blocks = 0;
+    };
+    for (uint32_t k = first; k < n; ++k) {
+      ViewBuffers &b = *views_[k];
+      if (done[k] || b.tileW != tileW || b.tileH != tileH) continue;
+      done[k] = true;
+      if (group.count == kRasterViewsPerLaunch) flush();
+      RasterView &w = group.view[group.count++];
+      w.splats = b.splats.as<Splat>();
+      w.entryValues = b.entryValues.as<uint32_t>();
+      w.ranges = b.ranges.as<uint2>();
+      w.image = b.image.as<float4>();
+      w.stops = options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>();
+      w.adjoint = b.adjoint.as<float4>();
+      w.rawFixed = fixedPoint_ ? b.rawFixed.as<int64_t>() : nullptr;
+      w.rawFloat = b.rawFloat.as<float>();
+      w.fixedScale = fixed_scale(params[k]);
+      w.width = b.width;
+      w.height = b.height;
+      w.gridW = b.gridW;
+      w.firstBlock = blocks;
+      w.blocks = b.tiles * (tileW / 8u) * (tileH / 8u);
+      blocks += w.blocks;
+    }
+    flush();
+  }
+}
+
 // The batch's chain rule on the trainer's stream, after every view's passes
 // (enqueue_view_passes): chain_views_kernel over the views in chunks of
 // kChainViewsPerLaunch, each chunk continuing the sums of the one before; the
@@ -1166,7 +1287,7 @@ void Trainer::enqueue_views(uint32_t n, const vsplat_camera *cameras, const uint
   dropoutNow_ = rates != nullptr ? shDropout_ : 0.0f;
   statisticsNext_ = true;
   std::vector<ViewParams> params(n);
-  enqueue_view_passes(n, cameras, targets, params.data());
+  enqueue_batch_passes(n, cameras, targets, params.data());
   uint32_t *status = views_[0]->counters.as<uint32_t>();
   enqueue_view_chains(n, params.data(), mode, rates);
   const ChainBuffers buffers = chain_buffers(*views_[0]);

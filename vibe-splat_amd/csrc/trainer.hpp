@@ -202,6 +202,13 @@ private:
   // The passes up to the backward pass on b's stream, then (chain) the chain rule.
   void enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8_t *deviceTarget, uint32_t mode,
                       const vsplat_learning_rates *rates, bool timed, bool chain = true);
+  ViewParams with_tiles(const ViewBuffers &b, const ViewParams &view) const;
+  void enqueue_lists(ViewBuffers &b, const ViewParams &view, uint32_t mode, bool marks);
+  void enqueue_loss(ViewBuffers &b, const ViewParams &view, const uint8_t *deviceTarget);
+  // A batch's passes with one forward and one backward launch over all views (VSPLAT_BATCH_ONE_LAUNCH).
+  void enqueue_view_passes_one_launch(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params);
+  void enqueue_batch_passes(uint32_t n, const vsplat_camera *cameras, const uint8_t *const *targets, ViewParams *params);
+  void launch_batch_raster(uint32_t n, const ViewParams *params, bool backward);
   void enqueue_chain(const ViewBuffers &b, const ViewParams &view, uint32_t mode, const vsplat_learning_rates *rates,
                      hipStream_t stream, uint32_t *counters);
   void enqueue_noise(const ViewBuffers &b, const vsplat_learning_rates *rates, hipStream_t stream, const uint32_t *counters);
@@ -254,7 +261,7 @@ private:
   bool filterSet_ = false;
   // The views' buffers; the first is the single-view calls' and runs on stream_.
   std::vector<std::unique_ptr<ViewBuffers>> views_;
-  hipEvent_t batchStart_ = nullptr;
+  hipEvent_t batchStart_ = nullptr, batchRaster_ = nullptr;
   // The captured batch: its executable graph, view count and sizes, and the
   // capture_key it was made under.
   hipGraphExec_t graph_ = nullptr;

@@ -26,6 +26,7 @@ BYTES_PER_TILE_ENTRY = 16
 MODE_STATISTICS, MODE_GRADIENTS, MODE_STEP = 0, 1, 2
 SH_HALF, SH_BF16 = 0, 1  # TrainerOptions.sh_storage
 TILES_16X16, TILES_8X8, TILES_32X8, TILES_AUTO = 0, 1, 2, 3  # TrainerOptions.tile_shape
+BATCH_VIEW_STREAMS, BATCH_ONE_LAUNCH = 0, 1  # TrainerOptions.batch_raster
 # vsplat_render_noalloc's limits (VSPLAT_NOALLOC_MAX_*, include/vsplat.h).
 NOALLOC_MAX_GAUSSIANS, NOALLOC_MAX_ENTRIES, NOALLOC_MAX_PIXELS, NOALLOC_MAX_TILES = 1 << 17, 1 << 21, 1 << 21, 1 << 16
 
@@ -80,7 +81,7 @@ class COptions(C.Structure):
                 ("force_fixed_point", C.c_int32), ("selective_adam", C.c_int32), ("opacity_reg", C.c_float),
                 ("scale_reg", C.c_float), ("position_noise", C.c_float), ("mip_filter_2d", C.c_int32),
                 ("pose_gradients", C.c_int32), ("memory_budget", C.c_uint64), ("sh_storage", C.c_int32),
-                ("recompute_transmittance", C.c_int32), ("tile_shape", C.c_int32)]
+                ("recompute_transmittance", C.c_int32), ("tile_shape", C.c_int32), ("batch_raster", C.c_int32)]
 
 
 class CRates(C.Structure):
@@ -217,6 +218,7 @@ class TrainerOptions:
     sh_storage: int = 0  # SH_HALF (the reference's layout) or SH_BF16 (this build only)
     recompute_transmittance: bool = False  # backward without the forward's stored log T and stops (this build only)
     tile_shape: int = 0  # TILES_16X16 (the reference's), TILES_8X8, TILES_32X8 or TILES_AUTO (this build only)
+    batch_raster: int = 0  # BATCH_VIEW_STREAMS or BATCH_ONE_LAUNCH: how a batch of views runs (this build only)
 
     def to_c(self) -> COptions:
         o = COptions()
@@ -232,6 +234,7 @@ class TrainerOptions:
         o.sh_storage = int(self.sh_storage)
         o.recompute_transmittance = int(self.recompute_transmittance)
         o.tile_shape = int(self.tile_shape)
+        o.batch_raster = int(self.batch_raster)
         return o
 
 
