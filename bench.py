@@ -445,13 +445,18 @@ def main():
     log(f"  {copy['gbs']} GB/s")
     # HBM bytes per launch of that kernel, from the FETCH_SIZE / WRITE_SIZE counter runs
     # on this workload (profiles/), when this run is that workload.
-    traffic = None
+    # The same runs' SQ counters give the kernel's share of the chip's VALU issue
+    # slots and of its LDS bandwidth.
+    traffic, issue = None, {}
     traffic_file = ROOT / "profiles" / "traffic_1M_1280x720_fixed_gradients.json"
     kernel_name = {"forward": f"forward_kernel<{tiles[0]}u, {tiles[1]}u>",
                    "backward": f"backward_kernel<true, false, false, {tiles[0]}u, {tiles[1]}u>"}[dominant[1]]
     if traffic_file.exists() and fixed and N == 1_000_000 and args.mode == "gradients":
         entry = json.loads(traffic_file.read_text())["kernels"].get(kernel_name)
-        traffic = None if entry is None else entry["hbm_bytes_per_launch"]
+        if entry is not None:
+            traffic = entry["hbm_bytes_per_launch"]
+            issue = {"valu_busy": entry.get("valu_busy"), "lds_frac": entry.get("lds_frac"),
+                     "counters": f"profiles/{traffic_file.name}"}
 
     result = {
         "metric": METRIC,
@@ -473,7 +478,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant[1], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": int(per_stage[dominant[1]]), "kernel_ms": round(kernel_ms, 4),
-                     "measured_copy_gbs": copy["gbs"], "frac_of_measured_copy": round(achieved / copy["gbs"], 5)},
+                     "measured_copy_gbs": copy["gbs"], "frac_of_measured_copy": round(achieved / copy["gbs"], 5), **issue},
         "iteration": {"algorithmic_bytes": int(total), "hbm_frac": round(total / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                       "frac_of_measured_copy": round(total / (ms * 1e-3) / 1e9 / copy["gbs"], 5),
                       "V": V, "E": E, "Ep": Ep, "P": P, "T": T, "Bp": Bp, "loss": round(float(stats.loss), 6)},

@@ -126,10 +126,19 @@ void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *v
                      const uint32_t *bigList, const uint64_t *bigBits, uint32_t *entryKeys, uint32_t *entryValues);
 void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges);
 
+// The raster launches' workgroups for a view of `tiles` list tiles of TW x TH
+// pixels: one per 8x8 block, the tile count rounded up to a multiple of the
+// chip's eight XCDs so that all of a tile's blocks run on one XCD.
+constexpr uint32_t kRasterXcds = 8;
+inline uint32_t raster_blocks(uint32_t tiles, uint32_t tileW, uint32_t tileH) {
+  return (tiles + kRasterXcds - 1) / kRasterXcds * kRasterXcds * (tileW / 8u) * (tileH / 8u);
+}
+
 // One view of a batched raster launch (its buffers and sizes), and up to
 // kRasterViewsPerLaunch of them, passed by value as the kernel's argument.
-// The views' 8x8 blocks follow each other: view v's are firstBlock ..
-// firstBlock + blocks - 1 of the grid.
+// The views' workgroups follow each other: view v's are firstBlock ..
+// firstBlock + blocks - 1 of the grid, blocks = raster_blocks(its tiles), so
+// every firstBlock is a multiple of kRasterXcds.
 struct RasterView {
   const Splat *splats;
   const uint32_t *entryValues;

@@ -47,15 +47,21 @@ struct BlockGeometry {
   uint32_t x0, y0, x1, y1;
 };
 
-// Workgroup `item` works 8x8 block q of list tile t, item = t * blocks + q,
-// the tile's blocks row-major (TW / 8 across): a tile's blocks are neighbours
-// in the grid, so its list is read from cache after the first.
+// Workgroups are dealt to the chip's eight XCDs in turn (workgroup h runs on
+// XCD h % 8), and each XCD has its own L2. Launch index h works 8x8 block q of
+// list tile t such that a tile's blocks all run on one XCD, one after another,
+// and read its list through one L2, while the tiles are still dealt over the
+// XCDs in turn (XCD x works tiles x, x + 8, x + 16, ...): with u = h / 8,
+// t = (u / blocks) * 8 + h % 8 and q = u % blocks, the tile's blocks row-major
+// (TW / 8 across). A grid of raster_blocks(tiles) covers every (t, q) once;
+// the blocks of the tiles past the last (t >= tiles) start below the image.
 template <uint32_t TW, uint32_t TH>
-__device__ __forceinline__ BlockGeometry block_geometry(uint32_t item, uint32_t gridW, uint32_t width, uint32_t height,
+__device__ __forceinline__ BlockGeometry block_geometry(uint32_t h, uint32_t gridW, uint32_t width, uint32_t height,
                                                         uint32_t &tile) {
   constexpr uint32_t kAcross = TW / 8u, kPerTile = (TW / 8u) * (TH / 8u);
-  tile = item / kPerTile;
-  const uint32_t q = item - tile * kPerTile;
+  const uint32_t u = h / kRasterXcds;
+  tile = (u / kPerTile) * kRasterXcds + h % kRasterXcds;
+  const uint32_t q = u % kPerTile;
   const uint32_t x0 = (tile % gridW) * TW + (q % kAcross) * 8u, y0 = (tile / gridW) * TH + (q / kAcross) * 8u;
   return {x0, y0, x0 + 8u < width ? x0 + 8u : width, y0 + 8u < height ? y0 + 8u : height};
 }
@@ -493,7 +499,7 @@ void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *ent
                     uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_FORWARD(W, H)                                                                                                       \
-  hipLaunchKernelGGL((forward_kernel<W, H>), dim3(gridW * gridH * (W / 8) * (H / 8)), dim3(kBlock), 0, stream, splats, entryValues, \
+  hipLaunchKernelGGL((forward_kernel<W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats, entryValues, \
                      ranges, width, height, gridW, background[0], background[1], background[2], image, stops)
   VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD)
 #undef VSPLAT_FORWARD
@@ -505,7 +511,7 @@ void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const 
                      float *rawFloat, const float *edge) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_BACKWARD_SHAPE(F, R, S, W, H)                                                                                       \
-  hipLaunchKernelGGL((backward_kernel<F, R, S, W, H>), dim3(gridW * gridH * (W / 8) * (H / 8)), dim3(kBlock), 0, stream, splats,   \
+  hipLaunchKernelGGL((backward_kernel<F, R, S, W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats,   \
                      entryValues, ranges, width, height, gridW, background[0], background[1], background[2], image, adjoint, stops, \
                      fixedScale, reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge)
 #define VSPLAT_BACKWARD_TTT(W, H) VSPLAT_BACKWARD_SHAPE(true, true, true, W, H)

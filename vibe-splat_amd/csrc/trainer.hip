@@ -1244,7 +1244,7 @@ void Trainer::launch_batch_raster(uint32_t n, const ViewParams *params, bool bac
       w.height = b.height;
       w.gridW = b.gridW;
       w.firstBlock = blocks;
-      w.blocks = b.tiles * (tileW / 8u) * (tileH / 8u);
+      w.blocks = raster_blocks(b.tiles, tileW, tileH);
       blocks += w.blocks;
     }
     flush();
