@@ -280,6 +280,48 @@ def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) ->
             "tile_shape": tile_shape, "tiles_chosen": {f"{k[0]}x{k[1]}": f"{v[0]}x{v[1]}" for k, v in shapes.items()}}
 
 
+def ring_1280x720(views: int):
+    """Eight (views) 1280x720 cameras on a ring of radius 3 about the cloud,
+    the first the single view's camera, and their targets."""
+    import math
+    cams = [F.look_at((3.0 * math.sin(2.0 * math.pi * k / views), 0.0, -3.0 * math.cos(2.0 * math.pi * k / views)), (0, 0, 0),
+                      (0.0, -1.0, 0.0), 60.0, 1280, 720) for k in range(views)]
+    targets = [F.smooth_target_explicit(1280, 720, 1016 + k) for k in range(views)]
+    return cams, targets
+
+
+def plain_ring_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str, fixed: bool, views: int = 8) -> dict:
+    """The plain form (fp32 SH, one view per pass, no graph) on each camera of
+    the batched 1280x720 ring in turn: ms per iteration of each view, gradients
+    mode, and their mean, the like-for-like baseline of the batched form."""
+    w = F.config(1, count=count)
+    cams, targets = ring_1280x720(views)
+    trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed,
+                                                            tile_shape=TILE_SHAPES[tile_shape]))
+    trainer.set_stage_timing(False)
+    per_view = []
+    try:
+        for cam, target in zip(cams, targets):
+            buffer = vsplat.DeviceBuffer(target)
+            try:
+                if tile_shape == "auto":
+                    trainer.autotune_tiles(cam, buffer.ptr, 2)
+                trainer.compute_gradients(cam, target)
+                for _ in range(warmup):
+                    trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
+                trainer.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
+                trainer.synchronize()
+                per_view.append(round((time.perf_counter() - t0) * 1000.0 / steps, 4))
+            finally:
+                buffer.free()
+    finally:
+        trainer.close()
+    return {"ms_per_view": per_view, "mean_ms_per_view": round(sum(per_view) / len(per_view), 4)}
+
+
 def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str, views: int = 8, fixed: bool = True,
                      schedule: int = vsplat.BATCH_VIEW_STREAMS) -> dict:
     """The target comparison at one size: the 1M-Gaussian cloud of `value`, seen
@@ -288,11 +330,8 @@ def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str
     SH stored as bf16, the sums in 64-bit fixed point (fixed) or float32
     atomics, captured as one HIP graph and replayed; per view against the plain
     fp32 single 1280x720 view."""
-    import math
     w = F.config(1, count=count)
-    cams = [F.look_at((3.0 * math.sin(2.0 * math.pi * k / views), 0.0, -3.0 * math.cos(2.0 * math.pi * k / views)), (0, 0, 0),
-                      (0.0, -1.0, 0.0), 60.0, 1280, 720) for k in range(views)]
-    targets = [F.smooth_target_explicit(1280, 720, 1016 + k) for k in range(views)]
+    cams, targets = ring_1280x720(views)
     trainer = vsplat.Trainer(w.cloud, vsplat.TrainerOptions(lambda_ssim=w.lambda_ssim, force_fixed_point=fixed,
                                                             sh_storage=vsplat.SH_BF16, tile_shape=TILE_SHAPES[tile_shape],
                                                             batch_raster=schedule))
@@ -495,26 +534,35 @@ def main():
         bt["plain_vs_graph_per_view"] = round(result["value"] / bt["graph_ms_per_view"], 4)
         log(f"  per view {bt['batched_ms_per_view']} ms batched (speedup {bt['speedup_per_view']}), "
             f"{bt['graph_ms_per_view']} ms as a graph (speedup {bt['graph_speedup_per_view']})")
+        log("plain fp32 fwd+bwd on each camera of the eight-view 1280x720 ring in turn")
+        plain_ring = plain_ring_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed)
+        result["plain_ring_1280x720"] = plain_ring
+        log(f"  {plain_ring['ms_per_view']} ms, mean {plain_ring['mean_ms_per_view']} ms per view")
+
+        def compare(leg):
+            # Per view against the plain single view of `value` and against the
+            # plain form on the same eight cameras.
+            leg["plain_single_view_ms"] = result["value"]
+            leg["plain_vs_batched_per_view"] = round(result["value"] / leg["graph_ms_per_view"], 4)
+            leg["plain_same_views_ms_per_view"] = plain_ring["mean_ms_per_view"]
+            leg["plain_same_views_vs_batched_per_view"] = round(plain_ring["mean_ms_per_view"] / leg["graph_ms_per_view"], 4)
+            return (f"{leg['graph_ms_per_view']} ms per view, {leg['plain_vs_batched_per_view']}x the plain single view, "
+                    f"{leg['plain_same_views_vs_batched_per_view']}x the plain form on the same views")
+
         log("batched bf16 fwd+bwd: eight 1280x720 views")
         same = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape)
-        same["plain_single_view_ms"] = result["value"]
-        same["plain_vs_batched_per_view"] = round(result["value"] / same["graph_ms_per_view"], 4)
         result["batched_1280x720"] = same
-        log(f"  {same['graph_ms_per_view']} ms per view, {same['plain_vs_batched_per_view']}x the plain single view")
+        log("  " + compare(same))
         log("batched bf16 fwd+bwd, sums accumulated in float32: eight 1280x720 views")
         same_float = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed=False)
-        same_float["plain_single_view_ms"] = result["value"]
-        same_float["plain_vs_batched_per_view"] = round(result["value"] / same_float["graph_ms_per_view"], 4)
         result["batched_1280x720_float_sums"] = same_float
-        log(f"  {same_float['graph_ms_per_view']} ms per view, {same_float['plain_vs_batched_per_view']}x the plain single view")
+        log("  " + compare(same_float))
         log("batched bf16 fwd+bwd, one forward and one backward launch over all views: eight 1280x720 views")
         for key, fixed_sums in (("batched_1280x720_one_launch", True), ("batched_1280x720_one_launch_float_sums", False)):
             one = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed=fixed_sums,
                                    schedule=vsplat.BATCH_ONE_LAUNCH)
-            one["plain_single_view_ms"] = result["value"]
-            one["plain_vs_batched_per_view"] = round(result["value"] / one["graph_ms_per_view"], 4)
             result[key] = one
-            log(f"  {key}: {one['graph_ms_per_view']} ms per view, {one['plain_vs_batched_per_view']}x the plain single view")
+            log(f"  {key}: " + compare(one))
     if not args.no_other_forms and args.count is None:
         result["other_forms"] = {}
         for key, index, fixed_sums, form_options in (

@@ -172,6 +172,45 @@ __device__ __forceinline__ bool load_raw(const int64_t *rawFixed, const float *r
   return convert_raw(reinterpret_cast<const uint32_t *>(rawFloat + static_cast<size_t>(i) * kRawPerSplat), false, fixedScale, g);
 }
 
+// One view's raw sums of Gaussian i, fetched into registers ahead of their
+// use (the loads are waited for where the words are first read), and with
+// `statistics` the splat's radius. Sums 2k and 2k + 1 sit in raw[4k .. 4k + 3]
+// as two int64 (low word first), or as two floats in raw[4k] and raw[4k + 1]:
+// the same places for both kinds, so that the registers stay registers.
+__device__ __forceinline__ void fetch_raw(const ChainView &w, uint32_t i, bool statistics, uint32_t raw[2 * kRawPerSplat], float &radius) {
+  if (w.rawFixed != nullptr) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(w.rawFixed + static_cast<size_t>(i) * kRawPerSplat);
+#pragma unroll
+    for (uint32_t k = 0; k < kRawPerSplat / 2; ++k) {
+      const uint4 v = src[k];
+      raw[4 * k] = v.x;
+      raw[4 * k + 1] = v.y;
+      raw[4 * k + 2] = v.z;
+      raw[4 * k + 3] = v.w;
+    }
+  } else {
+    const uint2 *src = reinterpret_cast<const uint2 *>(w.rawFloat + static_cast<size_t>(i) * kRawPerSplat);
+#pragma unroll
+    for (uint32_t k = 0; k < kRawPerSplat / 2; ++k) {
+      const uint2 v = src[k];
+      raw[4 * k] = v.x;
+      raw[4 * k + 1] = v.y;
+      raw[4 * k + 2] = 0u;
+      raw[4 * k + 3] = 0u;
+    }
+  }
+  if (statistics) radius = w.splats[i].radius;
+}
+
+// convert_raw for the words fetch_raw placed.
+__device__ __forceinline__ bool convert_fetched(const uint32_t raw[2 * kRawPerSplat], bool fixed, float fixedScale, float g[kRawPerSplat]) {
+  if (fixed) return convert_raw(raw, true, fixedScale, g);
+  uint32_t words[kRawPerSplat];
+#pragma unroll
+  for (uint32_t k = 0; k < kRawPerSplat; ++k) words[k] = raw[4 * (k / 2) + (k & 1u)];
+  return convert_raw(words, false, fixedScale, g);
+}
+
 // DensifyStat (train_gradient.comp:94-101), per half image rather than per pixel.
 __device__ __forceinline__ void add_statistics(float4 &a, float4 &b, const float g[kRawPerSplat], float W, float H, float radius) {
   const float ax = (g[1] * 0.5f) * W, ay = (g[2] * 0.5f) * H;
@@ -209,10 +248,9 @@ __device__ __forceinline__ void untouched_terms(const ChainBuffers &B, const Cha
 // Its SH part: the colour, clamped at zero, the SH terms handed to sh(...),
 // and dirTerm, the view direction's part of the position gradient.
 template <int Degree, uint32_t Format, typename ShSink>
-__device__ __forceinline__ void view_sh_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+__device__ __forceinline__ void view_sh_terms(const ViewParams &c, const ChainParams &o, uint32_t i, const float position[3],
                                               const uint32_t *words, const float g[kRawPerSplat], float dirTerm[3], ShSink &&sh) {
-  const float px = B.positions[i * 3], py = B.positions[i * 3 + 1], pz = B.positions[i * 3 + 2];
-  const float ddx = px - c.position[0], ddy = py - c.position[1], ddz = pz - c.position[2];
+  const float ddx = position[0] - c.position[0], ddy = position[1] - c.position[1], ddz = position[2] - c.position[2];
   const float dist = sqrtf(fma32(ddz, ddz, fma32(ddy, ddy, ddx * ddx)));
   const float idist = 1.0f / dist;
   const float dir[3] = {ddx * idist, ddy * idist, ddz * idist};
@@ -254,18 +292,31 @@ __device__ __forceinline__ void view_sh_terms(const ChainBuffers &B, const ViewP
   sh(basis, gcu, dir, used);
 }
 
-// Its geometric part: position (with dirTerm added when the colour depends on
-// the view), log-scale, rotation and opacity terms.
-__device__ __forceinline__ void view_geometry_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
-                                                    const float g[kRawPerSplat], const float dirTerm[3], bool withDir, ViewTerms &t) {
-  const float logScale[3] = {B.scales[i * 3], B.scales[i * 3 + 1], B.scales[i * 3 + 2]};
-  const float opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
-  const float s2[3] = {vs_exp(2.0f * logScale[0]), vs_exp(2.0f * logScale[1]), vs_exp(2.0f * logScale[2])};
-  const float px = B.positions[i * 3], py = B.positions[i * 3 + 1], pz = B.positions[i * 3 + 2];
+// What the chain rule needs of a Gaussian whatever the view: its position,
+// exp(log-scale) and its square, the rotation normalised (the identity for a
+// zero quaternion) with the quaternion's length, the rotation matrix R, the
+// 3D covariance Sigma = R diag(s2) R^T + f I, the opacity, the 3D filter f
+// and its opacity factor. Computed once per Gaussian however many views use it.
+struct GaussianFrame {
+  float p[3], e[3], s2[3];
+  float q[4], qlen;
+  float R[3][3], S[3][3];
+  float opacity, f, k3;
+};
+
+__device__ __forceinline__ void gaussian_frame(const ChainBuffers &B, uint32_t i, GaussianFrame &G) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    G.p[a] = B.positions[i * 3 + a];
+    const float logScale = B.scales[i * 3 + a];
+    G.e[a] = vs_exp(logScale);
+    G.s2[a] = vs_exp(2.0f * logScale);
+  }
+  G.opacity = 1.0f / (1.0f + vs_exp(-B.opacities[i]));
   float qw = B.rotations[i * 4], qx = B.rotations[i * 4 + 1], qy = B.rotations[i * 4 + 2], qz = B.rotations[i * 4 + 3];
-  const float qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
-  if (qlen > 0.0f) {
-    const float iq = 1.0f / qlen;
+  G.qlen = sqrtf(fma32(qz, qz, fma32(qy, qy, fma32(qx, qx, qw * qw))));
+  if (G.qlen > 0.0f) {
+    const float iq = 1.0f / G.qlen;
     qw = qw * iq;
     qx = qx * iq;
     qy = qy * iq;
@@ -274,24 +325,39 @@ __device__ __forceinline__ void view_geometry_terms(const ChainBuffers &B, const
     qw = 1.0f;
     qx = qy = qz = 0.0f;
   }
+  G.q[0] = qw;
+  G.q[1] = qx;
+  G.q[2] = qy;
+  G.q[3] = qz;
+  quat_to_mat(qw, qx, qy, qz, G.R);
+  G.f = B.filter3d != nullptr ? B.filter3d[i] : 0.0f;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      G.S[r][k] = dot3(G.R[r][0] * G.s2[0], G.R[r][1] * G.s2[1], G.R[r][2] * G.s2[2], G.R[k][0], G.R[k][1], G.R[k][2]);
+  G.S[0][0] = G.S[0][0] + G.f;
+  G.S[1][1] = G.S[1][1] + G.f;
+  G.S[2][2] = G.S[2][2] + G.f;
+  G.k3 = filter_factor(G.s2[0], G.s2[1], G.s2[2], G.f);
+}
+
+// Its geometric part: position (with dirTerm added when the colour depends on
+// the view), log-scale, rotation and opacity terms.
+__device__ __forceinline__ void view_geometry_terms(const GaussianFrame &G, const ViewParams &c, const ChainParams &o,
+                                                    const float g[kRawPerSplat], const float dirTerm[3], bool withDir, ViewTerms &t) {
+  const float opacity = G.opacity, f = G.f;
+  const float *s2 = G.s2;
+  const float qw = G.q[0], qx = G.q[1], qy = G.q[2], qz = G.q[3], qlen = G.qlen;
   float tx, ty, tz;
-  to_camera(c, px, py, pz, tx, ty, tz);
+  to_camera(c, G.p[0], G.p[1], G.p[2], tx, ty, tz);
   const float iz = 1.0f / tz, iz2 = iz * iz;
   const float gc0 = g[1], gc1 = g[2];
   float G00 = g[3], G01 = g[4], G11 = g[5];
   float k2 = 1.0f;
-  float R[3][3];
-  quat_to_mat(qw, qx, qy, qz, R);
-  const float f = B.filter3d != nullptr ? B.filter3d[i] : 0.0f;
-  // Sigma = R D R^T + f I; X = Wv Sigma Wv^T with Wv the view rotation.
-  float S[3][3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) S[r][k] = dot3(R[r][0] * s2[0], R[r][1] * s2[1], R[r][2] * s2[2], R[k][0], R[k][1], R[k][2]);
-  S[0][0] = S[0][0] + f;
-  S[1][1] = S[1][1] + f;
-  S[2][2] = S[2][2] + f;
+  const float (&R)[3][3] = G.R;
+  // X = Wv Sigma Wv^T with Wv the view rotation.
+  const float (&S)[3][3] = G.S;
   const float *V = c.view;
   float Y[3][3], X[3][3];
 #pragma unroll
@@ -302,7 +368,7 @@ __device__ __forceinline__ void view_geometry_terms(const ChainBuffers &B, const
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int k = 0; k < 3; ++k) X[r][k] = dot3(Y[r][0], Y[r][1], Y[r][2], V[0 * 4 + k], V[1 * 4 + k], V[2 * 4 + k]);
-  const float k3 = filter_factor(s2[0], s2[1], s2[2], f);
+  const float k3 = G.k3;
   const float opacity3 = opacity * k3;
   const float limX = 1.3f * c.tanHalfX, limY = 1.3f * c.tanHalfY;
   const float rx = tx * iz, ry = ty * iz;
@@ -376,7 +442,7 @@ __device__ __forceinline__ void view_geometry_terms(const ChainBuffers &B, const
     for (int r = 0; r < 3; ++r) v[r] = dot3(G3[r][0], G3[r][1], G3[r][2], R[0][a], R[1][a], R[2][a]);
     const float quad = dot3(R[0][a], R[1][a], R[2][a], v[0], v[1], v[2]);
     const float filterTerm = f != 0.0f ? ((((g[0] * k2) * opacity) * k3) * f) / (s2[a] + f) : 0.0f;
-    t.scale[a] = fma32(o.scaleReg, vs_exp(logScale[a]), fma32(2.0f * s2[a], quad, filterTerm));
+    t.scale[a] = fma32(o.scaleReg, G.e[a], fma32(2.0f * s2[a], quad, filterTerm));
   }
   // Rotation: P = G3 R D, dL/dq = 2 <2P, dR/dq> written out per component.
   float P[3][3];
@@ -407,21 +473,36 @@ __device__ __forceinline__ void view_geometry_terms(const ChainBuffers &B, const
 
 // Both parts in one thread (the batched chain rule).
 template <int Degree, uint32_t Format, typename ShSink>
-__device__ __forceinline__ void view_terms(const ChainBuffers &B, const ViewParams &c, const ChainParams &o, uint32_t i,
+__device__ __forceinline__ void view_terms(const GaussianFrame &G, const ViewParams &c, const ChainParams &o, uint32_t i,
                                            const uint32_t *words, const float g[kRawPerSplat], ViewTerms &t, ShSink &&sh) {
   float dirTerm[3];
-  view_sh_terms<Degree, Format>(B, c, o, i, words, g, dirTerm, sh);
+  view_sh_terms<Degree, Format>(c, o, i, G.p, words, g, dirTerm, sh);
   // Keep the scheduler from hoisting the geometry's work into the SH section.
   __builtin_amdgcn_sched_barrier(0);
-  view_geometry_terms(B, c, o, i, g, dirTerm, Degree > 0, t);
+  view_geometry_terms(G, c, o, g, dirTerm, Degree > 0, t);
+}
+
+// untouched_terms from a Gaussian's frame (the same values).
+__device__ __forceinline__ void untouched_terms(const GaussianFrame &G, const ChainParams &o, bool penalties, ViewTerms &t) {
+  const float op = penalties ? o.opacityReg * G.opacity * (1.0f - G.opacity) : 0.0f;
+#pragma unroll
+  for (uint32_t a = 0; a < 3; ++a) {
+    t.position[a] = 0.0f;
+    t.scale[a] = penalties ? o.scaleReg * G.e[a] : 0.0f;
+  }
+#pragma unroll
+  for (uint32_t a = 0; a < 4; ++a) t.rotation[a] = 0.0f;
+  t.opacity = op;
 }
 
 __device__ __forceinline__ void emit_terms(const ChainBuffers &B, const ChainParams &o, uint32_t i, const ViewTerms &t, const Radii &r) {
+#pragma unroll
   for (uint32_t a = 0; a < 3; ++a) {
     emit(o, B.positions, B.statePos, B.gradPos, i * 3 + a, t.position[a], o.rates[0], r.position, 0);
     emit(o, B.scales, B.stateScale, B.gradScale, i * 3 + a, t.scale[a], o.rates[1], r.scale, 1);
   }
   if (o.mode != kModeCurvature) {
+#pragma unroll
     for (uint32_t a = 0; a < 4; ++a) emit(o, B.rotations, B.stateRot, B.gradRot, i * 4 + a, t.rotation[a], o.rates[2], -1.0f, 2);
   }
   emit(o, B.opacities, B.stateOpacity, B.gradOpacity, i, t.opacity, o.rates[3], r.opacity, 3);
@@ -496,7 +577,8 @@ __global__ __launch_bounds__(kChainThreads) void chain_sh_kernel(ChainBuffers B,
   }
   constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
   float dirTerm[3];
-  view_sh_terms<Degree, Format>(B, c, o, i, B.shWords + static_cast<size_t>(i) * c.shWords, g, dirTerm,
+  const float position[3] = {B.positions[i * 3], B.positions[i * 3 + 1], B.positions[i * 3 + 2]};
+  view_sh_terms<Degree, Format>(c, o, i, position, B.shWords + static_cast<size_t>(i) * c.shWords, g, dirTerm,
                                 [&](const float basis[16], const float gcu[3], const float dir[3], uint32_t used) {
     if (o.mode == kModeGradients) {
       // Coefficients above the active degree get zero (they are stored, not trained).
@@ -539,7 +621,8 @@ __device__ __forceinline__ void chain_sh_gradients(const ChainBuffers &B, const 
     if (touched) {
       constexpr uint32_t kCoefficients = (Degree + 1) * (Degree + 1);
       float dirTerm[3];
-      view_sh_terms<Degree, Format>(B, c, o, i, B.shWords + static_cast<size_t>(i) * c.shWords, g, dirTerm,
+      const float position[3] = {B.positions[i * 3], B.positions[i * 3 + 1], B.positions[i * 3 + 2]};
+      view_sh_terms<Degree, Format>(c, o, i, position, B.shWords + static_cast<size_t>(i) * c.shWords, g, dirTerm,
                                     [&](const float basis[16], const float gcu[3], const float *, uint32_t used) {
         for (uint32_t k = 0; k < kCoefficients && k < used; ++k) {
 #pragma unroll
@@ -596,7 +679,9 @@ __global__ __launch_bounds__(kChainThreads) void chain_geometry_kernel(ChainBuff
   } else {
     const float4 d = B.dirTerms[i];
     const float dirTerm[3] = {d.x, d.y, d.z};
-    view_geometry_terms(B, c, o, i, g, dirTerm, c.shDegree > 0, t);
+    GaussianFrame G;
+    gaussian_frame(B, i, G);
+    view_geometry_terms(G, c, o, g, dirTerm, c.shDegree > 0, t);
     const float *gP = t.position;
     if ((o.flags & kFlagPose) != 0 && o.mode != kModeCurvature && (gP[0] != 0.0f || gP[1] != 0.0f || gP[2] != 0.0f)) {
       // How the loss moves with the camera through this Gaussian's position
@@ -626,12 +711,14 @@ __global__ __launch_bounds__(kChainThreads) void chain_geometry_kernel(ChainBuff
 // per-view chain rule's gradients added up in that order. Unlike the
 // single-view chain rule it stays one kernel: split, the second half would
 // read every view's raw sums and direction terms again (about 1 GB for eight
-// views at 1M Gaussians), which measured slower. The parameters are
-// read once per view from cache; the geometric sums stay in registers and the
-// SH sums in LDS, one row per coefficient so that a wave's lanes hit
-// consecutive banks. In gradients mode the sums are stored; in step mode the
-// geometric parameters take Adam's step from them here, and the SH sums are
-// stored for the SH step.
+// views at 1M Gaussians), which measured slower. What does not depend on the
+// view (the Gaussian's frame) is computed once, and each view's raw sums are
+// loaded while the view before is worked, which the kernel's low occupancy
+// (its LDS) needs to keep loads in flight. The geometric sums stay in
+// registers and the SH sums in LDS, one row per coefficient so that a wave's
+// lanes hit consecutive banks. In gradients mode the sums are stored; in step
+// mode the geometric parameters take Adam's step from them here, and the SH
+// sums are stored for the SH step.
 template <int Degree, uint32_t Format>
 __global__ __launch_bounds__(kChainThreads) void chain_views_kernel(ChainBuffers B, ChainViews views, ChainParams o,
                                                                    const uint32_t *counters) {
@@ -674,10 +761,17 @@ __global__ __launch_bounds__(kChainThreads) void chain_views_kernel(ChainBuffers
   o.flags &= ~kFlagAccumulate; // the sums below are complete: they are stored, not added
   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
   bool statsChanged = false;
+  GaussianFrame G;
+  gaussian_frame(B, i, G);
+  uint32_t raw[2 * kRawPerSplat];
+  float radius = 0.0f;
+  fetch_raw(views.view[0], i, statistics, raw, radius);
   for (uint32_t v = 0; v < views.count; ++v) {
     const ChainView &w = views.view[v];
     float g[kRawPerSplat];
-    const bool touched = load_raw(w.rawFixed, w.rawFloat, w.fixedScale, i, g);
+    const bool touched = convert_fetched(raw, w.rawFixed != nullptr, w.fixedScale, g);
+    const float viewRadius = radius;
+    if (v + 1 < views.count) fetch_raw(views.view[v + 1], i, statistics, raw, radius);
     if (touched && statistics) {
       if (!statsChanged) {
         const float4 *st = reinterpret_cast<const float4 *>(B.stats + static_cast<size_t>(i) * 8);
@@ -685,11 +779,11 @@ __global__ __launch_bounds__(kChainThreads) void chain_views_kernel(ChainBuffers
         b = st[1];
         statsChanged = true;
       }
-      add_statistics(a, b, g, static_cast<float>(w.c.width), static_cast<float>(w.c.height), w.splats[i].radius);
+      add_statistics(a, b, g, static_cast<float>(w.c.width), static_cast<float>(w.c.height), viewRadius);
     }
     ViewTerms t;
     if (touched) {
-      view_terms<Degree, Format>(B, w.c, o, i, words, g, t, [&](const float basis[16], const float gcu[3], const float[3], uint32_t used) {
+      view_terms<Degree, Format>(G, w.c, o, i, words, g, t, [&](const float basis[16], const float gcu[3], const float[3], uint32_t used) {
 #pragma unroll
         for (uint32_t k = 0; k < kCoefficients; ++k)
 #pragma unroll
@@ -699,7 +793,7 @@ __global__ __launch_bounds__(kChainThreads) void chain_views_kernel(ChainBuffers
           }
       });
     } else {
-      untouched_terms(B, o, i, penalties, t);
+      untouched_terms(G, o, penalties, t);
 #pragma unroll
       for (uint32_t j = 0; j < kCoefficients * 3; ++j) shSum[j][lane] = started ? shSum[j][lane] + 0.0f : 0.0f;
     }
