@@ -216,7 +216,8 @@ def copy_rate(nbytes: int = 1 << 30, repeats: int = 20) -> dict:
     return {"bytes": nbytes, "ms_per_copy": round(ms, 4), "gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
-TILE_SHAPES = {"auto": vsplat.TILES_AUTO, "16x16": vsplat.TILES_16X16, "8x8": vsplat.TILES_8X8, "32x8": vsplat.TILES_32X8}
+TILE_SHAPES = {"auto": vsplat.TILES_AUTO, "16x16": vsplat.TILES_16X16, "8x8": vsplat.TILES_8X8, "32x8": vsplat.TILES_32X8,
+               "32x32": vsplat.TILES_32X32}
 
 
 def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) -> dict:

@@ -78,9 +78,11 @@ typedef struct {
    * pass stored (no reference counterpart; the results are identical). */
   int32_t recompute_transmittance;
   /* The raster tile shape (no reference counterpart; the reference's is 16x16):
-   * VSPLAT_TILES_16X16, _8X8, _32X8, or _AUTO, which times the three on the
-   * first synchronous call for each view size (vsplat_step, compute_gradients,
-   * step_views, capture_views, or vsplat_autotune_tiles) and keeps the fastest.
+   * VSPLAT_TILES_16X16, _8X8, _32X8, _32X32, or _AUTO, which times the four
+   * on the first synchronous call for each view size (vsplat_step,
+   * compute_gradients, step_views, capture_views, or vsplat_autotune_tiles) and
+   * keeps the fastest. Large tiles list far fewer entries for scenes of large
+   * splats, small ones make shorter lists per pixel block.
    * Every shape gives the same images and sums; only the tile lists differ. */
   int32_t tile_shape;
   /* How a batch of views runs (no reference counterpart; the results are the
@@ -93,7 +95,7 @@ typedef struct {
 
 enum { VSPLAT_BATCH_VIEW_STREAMS = 0, VSPLAT_BATCH_ONE_LAUNCH = 1 };
 
-enum { VSPLAT_TILES_16X16 = 0, VSPLAT_TILES_8X8 = 1, VSPLAT_TILES_32X8 = 2, VSPLAT_TILES_AUTO = 3 };
+enum { VSPLAT_TILES_16X16 = 0, VSPLAT_TILES_8X8 = 1, VSPLAT_TILES_32X8 = 2, VSPLAT_TILES_AUTO = 3, VSPLAT_TILES_32X32 = 4 };
 
 enum { VSPLAT_SH_HALF = 0, VSPLAT_SH_BF16 = 1 };
 

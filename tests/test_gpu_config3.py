@@ -152,7 +152,8 @@ def test_config3_full_size_captured_properties():
         # Each view's counts in the batch equal a single-view pass of the same
         # view with the tile shape the batch kept for it.
         for k in (0, 6):
-            shape = {(16, 16): vsplat.TILES_16X16, (8, 8): vsplat.TILES_8X8, (32, 8): vsplat.TILES_32X8}[tuple(shapes[k])]
+            shape = {(16, 16): vsplat.TILES_16X16, (8, 8): vsplat.TILES_8X8, (32, 8): vsplat.TILES_32X8,
+                     (32, 32): vsplat.TILES_32X32}[tuple(shapes[k])]
             with vsplat.Trainer(cloud, dataclasses.replace(OPTIONS, tile_shape=shape)) as single:
                 s = single.compute_gradients(cams[k], targets[k])
             assert (s.visible_splats, s.tile_entries) == (first[k].visible_splats, first[k].tile_entries)

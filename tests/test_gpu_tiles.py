@@ -1,5 +1,5 @@
 """Raster tile shapes (no reference counterpart; the reference's is 16x16):
-8x8, 32x8 and the autotuner's choice, on the GPU.
+8x8, 32x8, 32x32 and the autotuner's choice, on the GPU.
 
 Each wave holds the same 8x8 block of pixels whatever the tile, and a tile's
 list holds its splats in depth order, so the image, the fixed-point sums, the
@@ -19,7 +19,7 @@ import vsplat
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = {vsplat.TILES_8X8: (8, 8), vsplat.TILES_32X8: (32, 8)}
+SHAPES = {vsplat.TILES_8X8: (8, 8), vsplat.TILES_32X8: (32, 8), vsplat.TILES_32X32: (32, 32)}
 
 
 def _bits(a):
@@ -48,7 +48,7 @@ def _run(cloud, cam, target, tile_shape):
 
 
 @pytest.mark.parametrize("name", ["ref300", "deg3", "config1_small"])
-@pytest.mark.parametrize("shape", [vsplat.TILES_8X8, vsplat.TILES_32X8])
+@pytest.mark.parametrize("shape", [vsplat.TILES_8X8, vsplat.TILES_32X8, vsplat.TILES_32X32])
 def test_tile_shape_gives_the_same_results(name, shape):
     cloud, cam, target = _case(name)
     base_stats, base = _run(cloud, cam, target, vsplat.TILES_16X16)
@@ -110,7 +110,7 @@ def test_autotuned_shape_steps_as_16x16():
          vsplat.DeviceBuffer(target) as d:
         shape = b.autotune_tiles(cam, d.ptr, 2)
         print(f"autotuned {shape} for {cam.width}x{cam.height}", flush=True)
-        assert shape in ((16, 16), (8, 8), (32, 8))
+        assert shape in ((16, 16), (8, 8), (32, 8), (32, 32))
         for _ in range(2):
             a.step(cam, target, rates)
             b.step(cam, target, rates)
@@ -132,7 +132,7 @@ def test_batch_autotuner_keeps_the_batch_results():
              vsplat.Trainer(w.cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=vsplat.TILES_AUTO)) as b:
             shapes = b.autotune_views(cams, [x.ptr for x in buffers], 2)
             print(f"batch shapes {shapes}", flush=True)
-            assert len(shapes) == len(cams) and all(s in ((16, 16), (8, 8), (32, 8)) for s in shapes)
+            assert len(shapes) == len(cams) and all(s in ((16, 16), (8, 8), (32, 8), (32, 32)) for s in shapes)
             for cam, shape in zip(cams, shapes):  # one shape per size
                 assert all(t == shape for c, t in zip(cams, shapes) if (c.width, c.height) == (cam.width, cam.height))
             assert not np.any(b.densify_stats())

@@ -161,7 +161,7 @@ void launch_forward_views(hipStream_t stream, const RasterViews &views, uint32_t
 void launch_backward_views(hipStream_t stream, bool fixedPoint, bool recompute, const RasterViews &views, uint32_t tileW, uint32_t tileH,
                            const float background[3]);
 
-// The raster kernels for one tile shape: 8x8, 16x16 or 32x8 (tileW x tileH).
+// The raster kernels for one tile shape: 8x8, 16x16, 32x8 or 32x32 (tileW x tileH).
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
                     uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops);
 // (stops may be null: not kept when the backward recomputes them.)

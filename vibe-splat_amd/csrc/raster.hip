@@ -1,6 +1,6 @@
 // Per-tile compositing and its derivative (train_forward.comp:38-100,
 // train_backward.glsl:78-158). The lists are per tile (16x16 by default, or
-// 8x8 or 32x8); the pixels are worked in 8x8 blocks, one workgroup of one
+// 8x8, 32x8 or 32x32); the pixels are worked in 8x8 blocks, one workgroup of one
 // wave64 per block, lane l holding pixel (l & 7, l >> 3). A block reads its
 // tile's list in batches of 64 entries: each lane stages one splat with the
 // block's pixels it can touch already worked out, and only the splats that
@@ -491,6 +491,8 @@ __global__ __launch_bounds__(kBlock) void backward_views_kernel(RasterViews view
     LAUNCH(8, 8);                                                                                                                  \
   } else if ((tileW) == 32 && (tileH) == 8) {                                                                                      \
     LAUNCH(32, 8);                                                                                                                 \
+  } else if ((tileW) == 32 && (tileH) == 32) {                                                                                     \
+    LAUNCH(32, 32);                                                                                                                \
   } else {                                                                                                                         \
     LAUNCH(16, 16);                                                                                                                \
   }

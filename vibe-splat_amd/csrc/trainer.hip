@@ -132,8 +132,8 @@ Trainer::Trainer(const vsplat_cloud &c, const vsplat_trainer_options &options) :
   if (options.batch_raster != VSPLAT_BATCH_VIEW_STREAMS && options.batch_raster != VSPLAT_BATCH_ONE_LAUNCH) {
     throw Error(VSPLAT_INVALID_ARGUMENT, "batch_raster must be VSPLAT_BATCH_VIEW_STREAMS or VSPLAT_BATCH_ONE_LAUNCH");
   }
-  if (options.tile_shape < VSPLAT_TILES_16X16 || options.tile_shape > VSPLAT_TILES_AUTO) {
-    throw Error(VSPLAT_INVALID_ARGUMENT, "tile_shape must be one of VSPLAT_TILES_16X16, _8X8, _32X8 or _AUTO");
+  if (options.tile_shape < VSPLAT_TILES_16X16 || options.tile_shape > VSPLAT_TILES_32X32) {
+    throw Error(VSPLAT_INVALID_ARGUMENT, "tile_shape must be one of VSPLAT_TILES_16X16, _8X8, _32X8, _32X32 or _AUTO");
   }
   shWords_ = shFormat_ == kShBf16 ? bf16_words(coefficients) : half_words(coefficients);
   // Float atomics are the default on this GPU, as on any with them (trainer.cpp:181).
@@ -302,6 +302,7 @@ std::pair<uint32_t, uint32_t> Trainer::tile_shape_for(int width, int height) con
   switch (options_.tile_shape) {
   case VSPLAT_TILES_8X8: return {8u, 8u};
   case VSPLAT_TILES_32X8: return {32u, 8u};
+  case VSPLAT_TILES_32X32: return {32u, 32u};
   case VSPLAT_TILES_AUTO: {
     const auto found = tuned_.find({width, height});
     return found != tuned_.end() ? found->second : std::make_pair(16u, 16u);
@@ -366,7 +367,7 @@ std::pair<uint32_t, uint32_t> Trainer::autotune_tiles(const vsplat_camera &camer
   hipEvent_t begin = nullptr, end = nullptr;
   check(hipEventCreate(&begin), "hipEventCreate");
   check(hipEventCreate(&end), "hipEventCreate");
-  const std::pair<uint32_t, uint32_t> shapes[3] = {{16u, 16u}, {8u, 8u}, {32u, 8u}};
+  const std::pair<uint32_t, uint32_t> shapes[] = {{16u, 16u}, {8u, 8u}, {32u, 8u}, {32u, 32u}};
   std::pair<uint32_t, uint32_t> best = shapes[0];
   float bestMs = 0.0f;
   try {
@@ -462,7 +463,7 @@ void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uin
     queuedViewSizes_.clear();
     return least;
   };
-  const std::pair<uint32_t, uint32_t> shapes[3] = {{16u, 16u}, {8u, 8u}, {32u, 8u}};
+  const std::pair<uint32_t, uint32_t> shapes[] = {{16u, 16u}, {8u, 8u}, {32u, 8u}, {32u, 32u}};
   try {
     check(hipEventCreate(&begin), "hipEventCreate");
     check(hipEventCreate(&end), "hipEventCreate");

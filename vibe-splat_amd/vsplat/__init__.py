@@ -25,7 +25,7 @@ OK, RUNTIME_ERROR, BUDGET_EXCEEDED, INVALID_ARGUMENT, LOGIC_ERROR, DEVICE_ERROR,
 BYTES_PER_TILE_ENTRY = 16
 MODE_STATISTICS, MODE_GRADIENTS, MODE_STEP = 0, 1, 2
 SH_HALF, SH_BF16 = 0, 1  # TrainerOptions.sh_storage
-TILES_16X16, TILES_8X8, TILES_32X8, TILES_AUTO = 0, 1, 2, 3  # TrainerOptions.tile_shape
+TILES_16X16, TILES_8X8, TILES_32X8, TILES_AUTO, TILES_32X32 = 0, 1, 2, 3, 4  # TrainerOptions.tile_shape
 BATCH_VIEW_STREAMS, BATCH_ONE_LAUNCH = 0, 1  # TrainerOptions.batch_raster
 # vsplat_render_noalloc's limits (VSPLAT_NOALLOC_MAX_*, include/vsplat.h).
 NOALLOC_MAX_GAUSSIANS, NOALLOC_MAX_ENTRIES, NOALLOC_MAX_PIXELS, NOALLOC_MAX_TILES = 1 << 17, 1 << 21, 1 << 21, 1 << 16
@@ -217,7 +217,7 @@ class TrainerOptions:
     memory_budget: int = 2 ** 64 - 1
     sh_storage: int = 0  # SH_HALF (the reference's layout) or SH_BF16 (this build only)
     recompute_transmittance: bool = False  # backward without the forward's stored log T and stops (this build only)
-    tile_shape: int = 0  # TILES_16X16 (the reference's), TILES_8X8, TILES_32X8 or TILES_AUTO (this build only)
+    tile_shape: int = 0  # TILES_16X16 (the reference's), TILES_8X8, TILES_32X8, TILES_32X32 or TILES_AUTO (this build only)
     batch_raster: int = 0  # BATCH_VIEW_STREAMS or BATCH_ONE_LAUNCH: how a batch of views runs (this build only)
 
     def to_c(self) -> COptions:
