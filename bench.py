@@ -438,7 +438,7 @@ def main():
     device_target = vsplat.DeviceBuffer(target)
     d_target = device_target.ptr
     if args.tile_shape == "auto":
-        log("tile shape: timing 16x16, 8x8 and 32x8")
+        log("tile shape: timing 16x16, 8x8, 32x8 and 32x32")
         tiles = trainer.autotune_tiles(cam, d_target, 3)
         log(f"  chose {tiles[0]}x{tiles[1]}")
     else:

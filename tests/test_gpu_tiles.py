@@ -120,6 +120,29 @@ def test_autotuned_shape_steps_as_16x16():
         np.testing.assert_array_equal(_bits(x), _bits(y))
 
 
+def test_tuner_leaves_the_entry_buffers_fit_for_its_choice():
+    """The tuner tries 8x8 tiles, which list several times the entries of the
+    others; after it, the tile-entry buffers are at most twice what the chosen
+    shape's entries take by the growth rule (the sort's grids are sized by
+    them), and the trainer's memory is that of a trainer given the shape."""
+    cloud, cam, target = _case("config1_small")
+    named = {(16, 16): vsplat.TILES_16X16, (8, 8): vsplat.TILES_8X8, (32, 8): vsplat.TILES_32X8, (32, 32): vsplat.TILES_32X32}
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=vsplat.TILES_AUTO)) as b, \
+         vsplat.DeviceBuffer(target) as d:
+        shape = b.autotune_tiles(cam, d.ptr, 1)
+        b.compute_gradients(cam, target)
+        tuned = b.gpu_bytes()
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=named[tuple(shape)])) as a:
+        stats = a.compute_gradients(cam, target)
+        given = a.gpu_bytes()
+        with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=vsplat.TILES_8X8)) as e:
+            eight = e.compute_gradients(cam, target).tile_entries
+    wanted = stats.tile_entries + stats.tile_entries // 2 + 1024
+    print(f"chose {shape}: {stats.tile_entries} entries (8x8: {eight}); tuned {tuned} B, given {given} B", flush=True)
+    assert eight > 2 * stats.tile_entries
+    assert tuned <= given + 2 * wanted * vsplat.BYTES_PER_TILE_ENTRY
+
+
 def test_batch_autotuner_keeps_the_batch_results():
     """The batch tuner (vsplat_autotune_views) picks a shape per view size by
     timing the whole batch, changes no parameter, gradient or statistic, and

@@ -370,19 +370,24 @@ std::pair<uint32_t, uint32_t> Trainer::autotune_tiles(const vsplat_camera &camer
   const std::pair<uint32_t, uint32_t> shapes[] = {{16u, 16u}, {8u, 8u}, {32u, 8u}, {32u, 32u}};
   std::pair<uint32_t, uint32_t> best = shapes[0];
   float bestMs = 0.0f;
+  uint64_t bestEntries = 0;
   try {
     for (const auto &shape : shapes) {
       ensure_layout(b, camera.width, camera.height, shape.first, shape.second);
-      // One run sizes the tile-entry buffers for this shape.
+      // One run sizes the tile-entry buffers for this shape (also down, after
+      // a shape that needed more).
+      uint64_t entries = 0;
       for (int attempt = 0;; ++attempt) {
         enqueue_passes(b, v, deviceTarget, kModeGradients, nullptr, false, false);
         bool overflow = false;
         const vsplat_step_stats stats = read_stats(overflow);
+        entries = stats.tile_entries;
         if (!overflow) break;
         launch_fill_u32(stream_, b.counters.as<uint32_t>() + kOverflow, 2, 0u);
         if (attempt > 0 && stats.tile_entries <= b.entryCapacity) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
         grow_entries(b, stats.tile_entries);
       }
+      fit_entries(b, entries);
       float least = 0.0f;
       for (uint32_t r = 0; r < (repeats > 0 ? repeats : 1); ++r) {
         check(hipEventRecord(begin, stream_), "hipEventRecord");
@@ -396,6 +401,7 @@ std::pair<uint32_t, uint32_t> Trainer::autotune_tiles(const vsplat_camera &camer
       if (bestMs == 0.0f || least < bestMs) {
         bestMs = least;
         best = shape;
+        bestEntries = entries;
       }
     }
   } catch (...) {
@@ -409,6 +415,7 @@ std::pair<uint32_t, uint32_t> Trainer::autotune_tiles(const vsplat_camera &camer
   dropoutNow_ = savedDropout;
   tuned_[{camera.width, camera.height}] = best;
   ensure_size(b, camera.width, camera.height);
+  fit_entries(b, bestEntries);
   lastView_.width = 0; // the buffers hold the timed passes of some shape, not a step
   return best;
 }
@@ -435,9 +442,11 @@ void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uin
   hipEvent_t begin = nullptr, end = nullptr;
   std::vector<ViewParams> params(n);
   std::vector<vsplat_step_stats> stats(n);
+  // Each view's tile entries under each shape it was run with.
+  std::vector<std::map<std::pair<uint32_t, uint32_t>, uint64_t>> entriesOf(n);
   const auto time_batch = [&]() -> float {
     for (uint32_t k = 0; k < n; ++k) ensure_size(*views_[k], cameras[k].width, cameras[k].height);
-    // One run sizes the tile-entry buffers for these shapes.
+    // One run sizes the tile-entry buffers for these shapes (also down).
     for (int attempt = 0;; ++attempt) {
       enqueue_batch_passes(n, cameras, targets, params.data());
       bool overflow = false;
@@ -449,6 +458,10 @@ void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uin
         grow_entries(*views_[k], stats[k].tile_entries);
       }
       if (attempt > 0 && !grew) throw Error(VSPLAT_DEVICE_ERROR, "tile entries overflowed twice");
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+      entriesOf[k][tuned_[{cameras[k].width, cameras[k].height}]] = stats[k].tile_entries;
+      fit_entries(*views_[k], stats[k].tile_entries);
     }
     float least = 0.0f;
     for (uint32_t r = 0; r < (repeats > 0 ? repeats : 1); ++r) {
@@ -493,6 +506,7 @@ void Trainer::autotune_views(uint32_t n, const vsplat_camera *cameras, const uin
   for (uint32_t k = 0; k < n; ++k) {
     ensure_size(*views_[k], cameras[k].width, cameras[k].height);
     const auto shape = tuned_[{cameras[k].width, cameras[k].height}];
+    fit_entries(*views_[k], entriesOf[k][shape]);
     if (tileW != nullptr) tileW[k] = shape.first;
     if (tileH != nullptr) tileH[k] = shape.second;
   }
@@ -531,6 +545,32 @@ void Trainer::grow_entries(ViewBuffers &b, uint64_t entries) {
   b.sortScratch.ensure(std::max(radix_sort_scratch_words(capacity), radix_sort_scratch_words(count_)) * 4);
   b.bigBits.ensure(big_bit_words_for(capacity) * 8);
   b.entryCapacity = capacity;
+}
+
+// A view's tile-entry buffers sized for `entries` by grow_entries' rule (half
+// as many again), never below the reserved count: grown when too small, and
+// reallocated smaller when more than twice that, as after the tile-shape
+// tuner has tried a shape that lists many more entries (the sort's grids and
+// histograms are sized by the capacity, so an oversized buffer costs time on
+// every pass, not only memory).
+void Trainer::fit_entries(ViewBuffers &b, uint64_t entries) {
+  entries = std::max(entries, reservedEntries_);
+  if (entries > b.entryCapacity) {
+    grow_entries(b, entries);
+    return;
+  }
+  const uint64_t wanted = std::min<uint64_t>(entries + entries / 2 + 1024, 0xFFFFFFFFull);
+  if (b.entryCapacity <= 2 * wanted) return;
+  check(hipStreamSynchronize(b.stream), "fit entries");
+  for (DeviceArray *a : {&b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues}) {
+    a->release();
+    a->ensure(wanted * 4);
+  }
+  b.sortScratch.release();
+  b.sortScratch.ensure(std::max(radix_sort_scratch_words(wanted), radix_sort_scratch_words(count_)) * 4);
+  b.bigBits.release();
+  b.bigBits.ensure(big_bit_words_for(wanted) * 8);
+  b.entryCapacity = wanted;
 }
 
 void Trainer::reserve_entries(uint64_t entries) {

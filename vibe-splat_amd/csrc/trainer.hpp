@@ -197,6 +197,7 @@ private:
   std::pair<uint32_t, uint32_t> tile_shape_for(int width, int height) const;
   bool needs_tuning(int width, int height) const;
   void grow_entries(ViewBuffers &b, uint64_t entries);
+  void fit_entries(ViewBuffers &b, uint64_t entries);
   uint64_t big_bit_words_for(uint64_t entries) const;
   void stage_target(const uint8_t *target, size_t bytes);
   // The passes up to the backward pass on b's stream, then (chain) the chain rule.
