@@ -364,8 +364,10 @@ int vsplat_autotune_views(vsplat_trainer *trainer, uint32_t n, const vsplat_came
 
 /* Copies one of the trainer's intermediate device buffers, as the last call
  * left it, to host memory: "splats" (48 bytes per Gaussian), "keys", "values"
- * (depth-sorted, 4 bytes per Gaussian), "coverage", "entry_keys",
- * "entry_values" (4 bytes per tile entry), "ranges" (8 bytes per tile),
+ * (depth-sorted, 4 bytes per Gaussian), "coverage", "entry_keys" (each
+ * entry's tile), "entry_blocks" (the 8x8 pixel blocks of the tile its splat
+ * reaches, bit q for block q, row-major; every block but for 32x32 tiles), "entry_values" (4 bytes per tile
+ * entry each), "ranges" (8 bytes per tile),
  * "image" (16 bytes per pixel: rgb, log T), "stops", "adjoint" (16 per pixel),
  * "fields" (36 per pixel), "tile_totals" (8 per tile), "raw" (14 int64 per
  * Gaussian with fixed point, else 14 float), "counters" (4 words),

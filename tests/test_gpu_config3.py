@@ -129,7 +129,7 @@ def test_config3_full_size_captured_properties():
         for run in range(2):
             with vsplat.Trainer(cloud, OPTIONS) as t:
                 tuned = t.autotune_views(cams, ptrs)
-                shapes = tuned if run == 0 else shapes
+                shapes = tuned  # the last run's, which `first` below belongs to
                 _size_entries(t, cams, targets)
                 t.capture_views(cams, ptrs, vsplat.MODE_STEP, RATES)
                 t.launch_captured(1)
@@ -149,8 +149,8 @@ def test_config3_full_size_captured_properties():
             np.testing.assert_array_equal(_bits(x), _bits(y))
         assert stats_seen[0] == stats_seen[1]
         assert np.any(_bits(a.positions) != _bits(cloud.positions))
-        # Each view's counts in the batch equal a single-view pass of the same
-        # view with the tile shape the batch kept for it.
+        # Each view's counts in the batch (the last run's) equal a single-view
+        # pass of the same view with the tile shape that run kept for it.
         for k in (0, 6):
             shape = {(16, 16): vsplat.TILES_16X16, (8, 8): vsplat.TILES_8X8, (32, 8): vsplat.TILES_32X8,
                      (32, 32): vsplat.TILES_32X32}[tuple(shapes[k])]

@@ -40,7 +40,8 @@ def _case(name):
 def _run(cloud, cam, target, tile_shape):
     with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=tile_shape)) as t:
         stats = t.compute_gradients(cam, target)
-        out = {name: t.inspect(name) for name in ("image", "raw", "keys", "values", "entry_keys", "entry_values", "ranges", "splats")}
+        out = {name: t.inspect(name) for name in ("image", "raw", "keys", "values", "entry_keys", "entry_blocks", "entry_values", "ranges",
+                                                  "splats")}
         g = t.gradients()
         out["gradients"] = [g.positions, g.scales, g.rotations, g.opacities, g.sh]
         out["stats"] = t.densify_stats()
@@ -62,11 +63,18 @@ def test_tile_shape_gives_the_same_results(name, shape):
         np.testing.assert_array_equal(_bits(a), _bits(b))
     np.testing.assert_array_equal(_bits(got["stats"]), _bits(base["stats"]))
 
+    _check_lists(cloud, cam, got, stats, SHAPES[shape])
+    _check_lists(cloud, cam, base, base_stats, (16, 16))
+
+
+def _check_lists(cloud, cam, got, stats, shape):
     # The lists: sorted by tile, depth order within a tile, ranges partitioning them.
-    tw, th = SHAPES[shape]
+    tw, th = shape
     grid_w, grid_h = (cam.width + tw - 1) // tw, (cam.height + th - 1) // th
     keys, values = got["entry_keys"][: stats.tile_entries], got["entry_values"][: stats.tile_entries]
+    blocks = got["entry_blocks"][: stats.tile_entries]
     assert (np.diff(keys.astype(np.int64)) >= 0).all() and keys.max() < grid_w * grid_h
+    assert blocks.max() < (1 << ((tw // 8) * (th // 8)))
     rank = np.empty(cloud.count, np.int64)
     rank[got["values"]] = np.arange(cloud.count)
     same_tile = np.diff(keys.astype(np.int64)) == 0
@@ -80,10 +88,11 @@ def test_tile_shape_gives_the_same_results(name, shape):
             total += b - a
     assert total == stats.tile_entries
 
-    # Brute force: every pixel where a splat's alpha reaches 1/255 lies in a tile listed for it.
+    # Brute force: every pixel where a splat's alpha reaches 1/255 lies in a
+    # tile listed for it, and in a block of that tile its entry marks.
     listed = {}
-    for key, value in zip(keys, values):
-        listed.setdefault(int(value), set()).add(int(key))
+    for key, value, mask in zip(keys, values, blocks):
+        listed.setdefault(int(value), {})[int(key)] = int(mask)
     splats = got["splats"]
     visible = got["values"][got["keys"] != 0xFFFFFFFF]
     rng = np.random.default_rng(0)
@@ -99,7 +108,10 @@ def test_tile_shape_gives_the_same_results(name, shape):
                 power = -0.5 * (s[0] * dx * dx + s[2] * dy * dy) - s[1] * dx * dy
                 alpha = min(0.99, s[3] * math.exp(power)) if power <= 0 else 0.0
                 if alpha >= (1 / 255) * (1 + 1e-3):
-                    assert (y // th) * grid_w + x // tw in listed.get(int(i), set()), (i, x, y)
+                    tile = (y // th) * grid_w + x // tw
+                    assert tile in listed.get(int(i), {}), (i, x, y)
+                    block = ((y % th) // 8) * (tw // 8) + (x % tw) // 8
+                    assert (listed[int(i)][tile] >> block) & 1, (i, x, y, tile, block)
 
 
 def test_autotuned_shape_steps_as_16x16():

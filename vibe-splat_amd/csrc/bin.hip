@@ -110,11 +110,16 @@ __global__ void merge_flags_kernel(uint32_t *status, const uint32_t *counters) {
 // scan's offset) plus the number of its splat's reached tiles before it, so a
 // round's stores are contiguous except where a big splat's run lies between
 // two small ones. A big splat's first entry is left in its record for
-// big_emit_kernel.
+// big_emit_kernel. An entry's key is its tile in the low tileBits bits and,
+// above them, the tile's 8x8 pixel blocks its splat can reach (block_mask;
+// with blockMasks off, all of them): the sort orders by the tile alone, and
+// the raster pass stages an entry only in the blocks whose bit is set.
 __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
-                                                       TileRecord *__restrict__ records, uint32_t count, uint32_t gridW,
-                                                       const uint32_t *__restrict__ offsets, const uint32_t *counters,
-                                                       uint32_t *__restrict__ entryKeys, uint32_t *__restrict__ entryValues) {
+                                                       const Splat *__restrict__ splats, TileRecord *__restrict__ records,
+                                                       uint32_t count, uint32_t gridW, TileShape shape, uint32_t width, uint32_t height,
+                                                       uint32_t tileBits, bool blockMasks, const uint32_t *__restrict__ offsets,
+                                                       const uint32_t *counters, uint32_t *__restrict__ entryKeys,
+                                                       uint32_t *__restrict__ entryValues) {
   __shared__ EmitOwner owners[256]; // per wave: its small splats with candidates, packed
   __shared__ uint32_t flags[256];   // per wave: the walk's round marks
   if (counters[kOverflow] != 0) return; // the same for every thread of the launch
@@ -138,11 +143,18 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
   if (total == 0) return; // nothing in this wave; no barrier follows
   const uint32_t start = end - candidates;
   if (owner) {
-    const uint32_t width = rec.size & 0xFFFFu;
+    const uint32_t boxWidth = rec.size & 0xFFFFu, tx0 = rec.origin & 0xFFFFu, ty0 = rec.origin >> 16;
+    const uint32_t tx1 = tx0 + boxWidth - 1u, ty1 = ty0 + (rec.size >> 16) - 1u;
     const uint32_t k = static_cast<uint32_t>(__popcll(owning & ((1ull << lane) - 1ull)));
-    owners[wave + k] = EmitOwner{rec.origin & 0xFFFFu, rec.origin >> 16, width, start, 1.0f / static_cast<float>(width), i,
-                                 rec.maskLo, rec.maskHi, offsets[r], {0u, 0u, 0u}};
+    uint32_t edges = 0;
+    if (blockMasks) {
+      const float4 c = reinterpret_cast<const float4 *>(splats + i)[2]; // cx, cy, ex, ey
+      edges = box_block_edges(splat_pixel_box(c.x, c.y, c.z, c.w), tx0, ty0, tx1, ty1, shape, width, height);
+    }
+    owners[wave + k] = EmitOwner{tx0, ty0, boxWidth, start, 1.0f / static_cast<float>(boxWidth), i, rec.maskLo, rec.maskHi, offsets[r],
+                                 tx1 | (ty1 << 16), edges, 0u};
   }
+  const uint32_t allBlocks = (1u << ((shape.w >> 3) * (shape.h >> 3))) - 1u;
   __builtin_amdgcn_wave_barrier();
   const uint32_t limit = counters[kEntriesUsed];
   uint32_t before = 0;
@@ -159,7 +171,8 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
         const uint32_t at = o.offset + static_cast<uint32_t>(__popcll(mask & ((1ull << local) - 1ull)));
         // The hits are the ones the projection counted, so at < limit; the test only guards the buffer.
         if (at < limit) {
-          entryKeys[at] = ty * gridW + tx;
+          const uint32_t blocks = blockMasks ? block_mask(o.edges, tx, ty, o.tx0, o.ty0, o.last & 0xFFFFu, o.last >> 16, shape) : allBlocks;
+          entryKeys[at] = (ty * gridW + tx) | (blocks << tileBits);
           entryValues[at] = o.gaussian;
         }
       }
@@ -172,15 +185,17 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const uint32_t *__restric
 // round, row-major, each round's outcome read from the bits the counting pass
 // kept (or tested again where it kept none), the reached ones numbered by a
 // prefix count from the splat's first entry. The listed splats are dealt out
-// over all the waves.
+// over all the waves. Their entries mark every 8x8 block of the tile (the
+// tiles inside a big box are covered whole).
 __global__ __launch_bounds__(256) void big_emit_kernel(const Splat *__restrict__ splats, const TileRecord *__restrict__ records,
                                                        const uint32_t *__restrict__ bigList, const uint64_t *__restrict__ bigBits,
-                                                       uint32_t gridW, TileShape shape, const uint32_t *counters,
+                                                       uint32_t gridW, TileShape shape, uint32_t tileBits, const uint32_t *counters,
                                                        uint32_t *__restrict__ entryKeys, uint32_t *__restrict__ entryValues) {
   if (counters[kOverflow] != 0) return; // the same for every thread of the launch
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t waves = gridDim.x * (blockDim.x / 64u), wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
   const uint32_t listed = counters[kBigSplats], limit = counters[kEntriesUsed];
+  const uint32_t allBlocks = ((1u << ((shape.w >> 3) * (shape.h >> 3))) - 1u) << tileBits;
   // Wave w takes listed splats w, w + waves, w + 2 waves, ..., 64 at a time (one
   // per lane), so that a short list still spreads over every wave.
   for (uint32_t base = 0; wave + base * waves < listed; base += 64u) {
@@ -213,7 +228,7 @@ __global__ __launch_bounds__(256) void big_emit_kernel(const Splat *__restrict__
         const bool hit = ((mask >> lane) & 1ull) != 0;
         const uint32_t at = next + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
         if (hit && at < limit) {
-          entryKeys[at] = ty * gridW + tx;
+          entryKeys[at] = (ty * gridW + tx) | allBlocks;
           entryValues[at] = b.gaussian;
         }
         next += static_cast<uint32_t>(__popcll(mask));
@@ -225,9 +240,10 @@ __global__ __launch_bounds__(256) void big_emit_kernel(const Splat *__restrict__
 
 // Start and end of each tile's run; every write belongs to one thread
 // (train_tile_ranges.comp:11-22). A thread takes four consecutive keys with one
-// 16-byte load, and the keys either side of them.
+// 16-byte load, and the keys either side of them; a key's tile is its low bits
+// (tileMask), its block bits above them.
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *__restrict__ entryKeys, const uint32_t *counters,
-                                                          uint2 *__restrict__ ranges) {
+                                                          uint32_t tileMask, uint2 *__restrict__ ranges) {
   const uint32_t count = counters[kEntriesUsed];
   const uint32_t quads = (count + 3u) / 4u;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
@@ -243,8 +259,10 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *__rest
 #pragma unroll
       for (uint32_t j = 0; j < 4u; ++j) key[j] = k0 + j < count ? entryKeys[k0 + j] : ~0u;
     }
-    const uint32_t before = k0 == 0 ? ~0u : entryKeys[k0 - 1u];
-    const uint32_t after = k0 + 4u < count ? entryKeys[k0 + 4u] : ~0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) key[j] &= tileMask;
+    const uint32_t before = k0 == 0 ? ~0u : entryKeys[k0 - 1u] & tileMask;
+    const uint32_t after = k0 + 4u < count ? entryKeys[k0 + 4u] & tileMask : ~0u;
 #pragma unroll
     for (uint32_t j = 0; j < 4u; ++j) {
       const uint32_t k = k0 + j;
@@ -273,20 +291,26 @@ void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *co
 }
 
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, TileRecord *records,
-                     uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, const uint32_t *offsets, const uint32_t *counters,
-                     const uint32_t *bigList, const uint64_t *bigBits, uint32_t *entryKeys, uint32_t *entryValues) {
+                     uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, uint32_t width, uint32_t height, uint32_t tileBits,
+                     const uint32_t *offsets, const uint32_t *counters, const uint32_t *bigList, const uint64_t *bigBits,
+                     uint32_t *entryKeys, uint32_t *entryValues) {
   if (count == 0) return;
-  hipLaunchKernelGGL(bin_emit_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, records, count, gridW, offsets,
-                     counters, entryKeys, entryValues);
+  // Blocks are marked per entry for tiles of 16 blocks (32x32), where a block
+  // skipping the entries that miss it saves more raster time than marking
+  // costs the list pass; smaller tiles mark every block.
+  const bool blockMasks = (tileW / 8u) * (tileH / 8u) >= 16u;
+  hipLaunchKernelGGL(bin_emit_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, keys, values, splats, records, count, gridW,
+                     TileShape{tileW, tileH}, width, height, tileBits, blockMasks, offsets, counters, entryKeys, entryValues);
   hipLaunchKernelGGL(big_emit_kernel, dim3(kBigSplatBlocks), dim3(256), 0, stream, splats, records, bigList, bigBits, gridW,
-                     TileShape{tileW, tileH}, counters, entryKeys, entryValues);
+                     TileShape{tileW, tileH}, tileBits, counters, entryKeys, entryValues);
 }
 
-void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges) {
+void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint32_t tileBits,
+                        uint2 *ranges) {
   if (capacity == 0) return;
   const uint32_t quads = (capacity + 3) / 4;
   const uint32_t blocks = (quads + 255) / 256 < 16384 ? (quads + 255) / 256 : 16384;
-  hipLaunchKernelGGL(tile_ranges_kernel, dim3(blocks), dim3(256), 0, stream, entryKeys, counters, ranges);
+  hipLaunchKernelGGL(tile_ranges_kernel, dim3(blocks), dim3(256), 0, stream, entryKeys, counters, (1u << tileBits) - 1u, ranges);
 }
 
 } // namespace vsplat

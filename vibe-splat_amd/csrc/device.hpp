@@ -314,13 +314,23 @@ __device__ __forceinline__ Splat load_splat(const Splat *splats, uint32_t i) {
   return Splat{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
 }
 
+// The pixels a splat can reach before any clipping, half-open: columns
+// [ax, bx), rows [ay, by), from the extent box grown by half a pixel
+// (train_types.glsl:43-48).
+struct PixelBox {
+  uint32_t ax, ay, bx, by;
+};
+__device__ __forceinline__ PixelBox splat_pixel_box(float cx, float cy, float ex, float ey) {
+  const float lx = clampf(cx - ex - 0.5f, 0.0f, 16777216.0f), ly = clampf(cy - ey - 0.5f, 0.0f, 16777216.0f);
+  const float hx = clampf(cx + ex + 0.5f, 0.0f, 16777216.0f), hy = clampf(cy + ey + 0.5f, 0.0f, 16777216.0f);
+  return PixelBox{static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), static_cast<uint32_t>(hx) + 1u, static_cast<uint32_t>(hy) + 1u};
+}
+
 // The pixels of the tile at (x0, y0, x1, y1) that a splat can reach, half-open,
 // tile-local, packed as four bytes x0 | y0 << 8 | x1 << 16 | y1 << 24 (train_types.glsl:43-48).
 __device__ __forceinline__ uint32_t splat_box(const Splat &s, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1) {
-  const float lx = clampf(s.cx - s.ex - 0.5f, 0.0f, 16777216.0f), ly = clampf(s.cy - s.ey - 0.5f, 0.0f, 16777216.0f);
-  const float hx = clampf(s.cx + s.ex + 0.5f, 0.0f, 16777216.0f), hy = clampf(s.cy + s.ey + 0.5f, 0.0f, 16777216.0f);
-  uint32_t ax = static_cast<uint32_t>(lx), ay = static_cast<uint32_t>(ly);
-  uint32_t bx = static_cast<uint32_t>(hx) + 1u, by = static_cast<uint32_t>(hy) + 1u;
+  const PixelBox p = splat_pixel_box(s.cx, s.cy, s.ex, s.ey);
+  uint32_t ax = p.ax, ay = p.ay, bx = p.bx, by = p.by;
   ax = ax > x0 ? ax : x0;
   ay = ay > y0 ? ay : y0;
   bx = bx < x1 ? bx : x1;

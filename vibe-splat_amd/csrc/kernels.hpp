@@ -121,10 +121,15 @@ void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *co
 // The tile lists in depth order: the small splats' entries from their bits,
 // then the big splats' (bigList, with the bits launch_project kept). records is written: a
 // big splat's first entry is kept in its maskLo for the second kernel.
+// The tile entries (keys: the tile in the low tileBits bits, above them the
+// tile's 8x8 blocks the splat reaches, marked for 32x32 tiles and all set for
+// the others; values: the Gaussian), for a view of width x height pixels.
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, TileRecord *records,
-                     uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, const uint32_t *offsets, const uint32_t *counters,
-                     const uint32_t *bigList, const uint64_t *bigBits, uint32_t *entryKeys, uint32_t *entryValues);
-void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint2 *ranges);
+                     uint32_t count, uint32_t gridW, uint32_t tileW, uint32_t tileH, uint32_t width, uint32_t height, uint32_t tileBits,
+                     const uint32_t *offsets, const uint32_t *counters, const uint32_t *bigList, const uint64_t *bigBits,
+                     uint32_t *entryKeys, uint32_t *entryValues);
+void launch_tile_ranges(hipStream_t stream, const uint32_t *entryKeys, const uint32_t *counters, uint32_t capacity, uint32_t tileBits,
+                        uint2 *ranges);
 
 // The raster launches' workgroups for a view of `tiles` list tiles of TW x TH
 // pixels: one per 8x8 block, the tile count rounded up to a multiple of the
@@ -141,7 +146,7 @@ inline uint32_t raster_blocks(uint32_t tiles, uint32_t tileW, uint32_t tileH) {
 // every firstBlock is a multiple of kRasterXcds.
 struct RasterView {
   const Splat *splats;
-  const uint32_t *entryValues;
+  const uint32_t *entryKeys, *entryValues;
   const uint2 *ranges;
   float4 *image;
   uint32_t *stops;
@@ -150,6 +155,7 @@ struct RasterView {
   float *rawFloat;
   float fixedScale;
   uint32_t width, height, gridW, firstBlock, blocks;
+  uint32_t tileBits;
 };
 constexpr uint32_t kRasterViewsPerLaunch = 8;
 struct RasterViews {
@@ -162,17 +168,20 @@ void launch_backward_views(hipStream_t stream, bool fixedPoint, bool recompute, 
                            const float background[3]);
 
 // The raster kernels for one tile shape: 8x8, 16x16, 32x8 or 32x32 (tileW x tileH).
-void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
-                    uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops);
+// The lists are read as launch_bin_emit left them: entryKeys carry each
+// entry's tile (low tileBits bits) and blocks, entryValues its Gaussian.
+void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryKeys, const uint32_t *entryValues, uint32_t tileBits,
+                    const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
+                    float4 *image, uint32_t *stops);
 // (stops may be null: not kept when the backward recomputes them.)
 // recompute: the transmittance and stops are recomputed per tile rather than
 // read from the forward pass (image and stops may then be null). edge: a
 // scored view's edge weights (the adjoint's fourth value its marks), which
 // fill the last two raw sums; null otherwise.
-void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
-                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
-                     const float4 *image, const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed,
-                     float *rawFloat, const float *edge = nullptr);
+void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryKeys,
+                     const uint32_t *entryValues, uint32_t tileBits, const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW,
+                     uint32_t tileH, const float background[3], const float4 *image, const float4 *adjoint, const uint32_t *stops,
+                     float fixedScale, int64_t *rawFixed, float *rawFloat, const float *edge = nullptr);
 
 // errorMax and edge: when scoring, the view's worst error (cleared before)
 // and the target's edge weights, and the adjoint's fourth value the pixel's

@@ -45,6 +45,7 @@ constexpr uint32_t kBlock = 64; // pixels of an 8x8 block, threads of its workgr
 
 struct BlockGeometry {
   uint32_t x0, y0, x1, y1;
+  uint32_t block; // the block's number in its tile (its bit in the entries' block masks)
 };
 
 // Workgroups are dealt to the chip's eight XCDs in turn (workgroup h runs on
@@ -63,7 +64,7 @@ __device__ __forceinline__ BlockGeometry block_geometry(uint32_t h, uint32_t gri
   tile = (u / kPerTile) * kRasterXcds + h % kRasterXcds;
   const uint32_t q = u % kPerTile;
   const uint32_t x0 = (tile % gridW) * TW + (q % kAcross) * 8u, y0 = (tile / gridW) * TH + (q / kAcross) * 8u;
-  return {x0, y0, x0 + 8u < width ? x0 + 8u : width, y0 + 8u < height ? y0 + 8u : height};
+  return {x0, y0, x0 + 8u < width ? x0 + 8u : width, y0 + 8u < height ? y0 + 8u : height, q};
 }
 
 // The block's pixels a splat can touch: inside the extent box and within the
@@ -95,20 +96,62 @@ __device__ __forceinline__ bool covers(const Staged &s, uint32_t bits) { return 
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0u; }
 __device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0u; }
 
-// Entries first .. first + count - 1 of the list: each lane stages one, and the
-// splats that can touch a pixel of the block are packed at the front of
-// `batch` in list order (with their Gaussians in `index`, when given). Returns
-// how many were kept. The caller has made sure the previous batch is read.
+// The position of set bit number k (from 0) of m, which has more than k.
+__device__ __forceinline__ uint32_t kth_set_bit(uint64_t m, uint32_t k) {
+  uint32_t at = 0;
+#pragma unroll
+  for (uint32_t w = 32; w >= 1; w >>= 1) {
+    const uint32_t low = static_cast<uint32_t>(__popcll(m & ((1ull << w) - 1ull)));
+    if (k >= low) {
+      k -= low;
+      m >>= w;
+      at += w;
+    }
+  }
+  return at;
+}
+
+// The next entries of the block's walk over its tile's list, from `pos` on
+// towards `limit` (up: pos, pos + 1, ...; down: pos - 1, pos - 2, ...), that
+// have the block's bit (blockBit) in their key: up to kBlock of them, in walk
+// order, into `queue`; `pos` is left at the first entry not yet looked at (up)
+// or one past it (down). The keys are read a window of kBlock at a time; a
+// window is taken up to the first wanted entry that does not fit. Returns how
+// many were queued. The caller has made sure the previous queue is read.
+template <bool Down>
+__device__ __forceinline__ uint32_t gather_entries(const uint32_t *__restrict__ entryKeys, uint32_t blockBit, uint32_t &pos,
+                                                   uint32_t limit, uint32_t *queue) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t queued = 0;
+  while (queued < kBlock && pos != limit) {
+    const uint32_t left = Down ? pos - limit : limit - pos;
+    const uint32_t n = left < kBlock ? left : kBlock;
+    const uint32_t e = Down ? pos - 1u - lane : pos + lane;
+    const bool wanted = lane < n && (entryKeys[e] & blockBit) != 0u;
+    const uint64_t m = __ballot(wanted);
+    const uint32_t found = static_cast<uint32_t>(__popcll(m)), room = kBlock - queued;
+    const uint32_t taken = found > room ? kth_set_bit(m, room) : n; // window entries used up
+    if (wanted && lane < taken) queue[queued + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)))] = e;
+    queued += found < room ? found : room;
+    pos = Down ? pos - taken : pos + taken;
+  }
+  return queued;
+}
+
+// The queued entries: each lane stages one, and the splats that can touch a
+// pixel of the block are packed at the front of `batch` in queue order (with
+// their Gaussians in `index`, when given). Returns how many were kept.
 __device__ __forceinline__ uint32_t stage_batch(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
-                                                uint32_t first, uint32_t count, const BlockGeometry &geo, Staged *batch,
+                                                const uint32_t *queue, uint32_t count, const BlockGeometry &geo, Staged *batch,
                                                 uint32_t *index) {
   const uint32_t lane = threadIdx.x;
   Staged st{};
   uint32_t gaussian = 0;
   bool keep = false;
   if (lane < count) {
-    gaussian = entryValues[first + lane];
-    st = stage(load_splat(splats, gaussian), geo, first + lane);
+    const uint32_t entry = queue[lane];
+    gaussian = entryValues[entry];
+    st = stage(load_splat(splats, gaussian), geo, entry);
     keep = (st.cols & 0xFFu) != 0u && (st.cols >> 16) != 0u;
   }
   const uint64_t kept = __ballot(keep);
@@ -131,18 +174,20 @@ __device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
 // calling lane's pixel. The forward pass and the backward pass that
 // recomputes the transmittance both run this, so they agree bit for bit.
 template <bool Colour>
-__device__ __forceinline__ void composite_block(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues, uint2 range,
-                                                const BlockGeometry &geo, Staged *batch, bool inside, float px, float py, uint32_t bits,
-                                                float &r, float &g, float &b, float &T, float &logT, uint32_t &stop) {
+__device__ __forceinline__ void composite_block(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryKeys,
+                                                const uint32_t *__restrict__ entryValues, uint32_t blockBit, uint2 range,
+                                                const BlockGeometry &geo, Staged *batch, uint32_t *queue, bool inside, float px, float py,
+                                                uint32_t bits, float &r, float &g, float &b, float &T, float &logT, uint32_t &stop) {
   stop = range.x;
   bool done = !inside;
-  for (uint32_t end = range.y; end > range.x;) {
+  for (uint32_t pos = range.y; pos > range.x;) {
     if (wave_all(done)) break;
-    const uint32_t count = end - range.x < kBlock ? end - range.x : kBlock;
-    end -= count;
-    __syncthreads(); // the previous batch is read
-    const uint32_t kept = stage_batch(splats, entryValues, end, count, geo, batch, nullptr);
-    for (uint32_t k = kept; k-- > 0;) {
+    __syncthreads(); // the previous batch and queue are read
+    const uint32_t queued = gather_entries<true>(entryKeys, blockBit, pos, range.x, queue);
+    __syncthreads();
+    const uint32_t kept = stage_batch(splats, entryValues, queue, queued, geo, batch, nullptr);
+    // The batch runs front to back already (the queue was filled walking down).
+    for (uint32_t k = 0; k < kept; ++k) {
       const Staged s = batch[k];
       const bool candidate = !done && covers(s, bits);
       if (!wave_any(candidate)) continue;
@@ -177,10 +222,11 @@ __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats
 
 // The forward pass of one 8x8 block (item) of a view.
 template <uint32_t TW, uint32_t TH>
-__device__ __forceinline__ void forward_block(uint32_t item, const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+__device__ __forceinline__ void forward_block(uint32_t item, const Splat *__restrict__ splats, const uint32_t *__restrict__ entryKeys,
+                                              const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                               const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                               float bgR, float bgG, float bgB, float4 *__restrict__ image, uint32_t *__restrict__ stops,
-                                              Staged *batch) {
+                                              Staged *batch, uint32_t *queue) {
   uint32_t tile;
   const BlockGeometry geo = block_geometry<TW, TH>(item, gridW, width, height, tile);
   if (geo.x0 >= width || geo.y0 >= height) return; // a block past the image's edge (uniform)
@@ -190,7 +236,8 @@ __device__ __forceinline__ void forward_block(uint32_t item, const Splat *__rest
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, logT = 0.0f;
   uint32_t stop;
-  composite_block<true>(splats, entryValues, ranges[tile], geo, batch, inside, px, py, lane_bits(lx, ly), r, g, b, T, logT, stop);
+  composite_block<true>(splats, entryKeys, entryValues, 1u << (tileBits + geo.block), ranges[tile], geo, batch, queue, inside, px, py,
+                        lane_bits(lx, ly), r, g, b, T, logT, stop);
   if (inside) {
     const uint32_t at = y * width + x;
     image[at] = make_float4(fma32(T, bgR, r), fma32(T, bgG, g), fma32(T, bgB, b), logT);
@@ -199,12 +246,15 @@ __device__ __forceinline__ void forward_block(uint32_t item, const Splat *__rest
 }
 
 template <uint32_t TW, uint32_t TH>
-__global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+__global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryKeys,
+                                                         const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
                                                          uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
                                                          uint32_t *__restrict__ stops) {
   __shared__ Staged batch[kBlock];
-  forward_block<TW, TH>(blockIdx.x, splats, entryValues, ranges, width, height, gridW, bgR, bgG, bgB, image, stops, batch);
+  __shared__ uint32_t queue[kBlock];
+  forward_block<TW, TH>(blockIdx.x, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR, bgG, bgB, image, stops,
+                        batch, queue);
 }
 
 // The view of a batched launch that block `item` belongs to, and the block's
@@ -221,9 +271,11 @@ __device__ __forceinline__ uint32_t view_of(const RasterViews &views, uint32_t &
 template <uint32_t TW, uint32_t TH>
 __global__ __launch_bounds__(kBlock) void forward_views_kernel(RasterViews views, float bgR, float bgG, float bgB) {
   __shared__ Staged batch[kBlock];
+  __shared__ uint32_t queue[kBlock];
   uint32_t item = blockIdx.x;
   const RasterView &w = views.view[view_of(views, item)];
-  forward_block<TW, TH>(item, w.splats, w.entryValues, w.ranges, w.width, w.height, w.gridW, bgR, bgG, bgB, w.image, w.stops, batch);
+  forward_block<TW, TH>(item, w.splats, w.entryKeys, w.entryValues, w.tileBits, w.ranges, w.width, w.height, w.gridW, bgR, bgG, bgB,
+                        w.image, w.stops, batch, queue);
 }
 
 // The fixed-point quantisation of a wave total (train_backward_fixed.comp:19-21),
@@ -330,13 +382,14 @@ __device__ __forceinline__ bool wave_emits(uint32_t lane) { return (lane & 9u) =
 // mark (the adjoint's fourth value) and its edge weight times the blending
 // weight, each over 3 pixels of the image.
 template <bool Fixed, bool Recompute, bool Score, uint32_t TW, uint32_t TH>
-__device__ __forceinline__ void backward_block(uint32_t item, const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+__device__ __forceinline__ void backward_block(uint32_t item, const Splat *__restrict__ splats, const uint32_t *__restrict__ entryKeys,
+                                               const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                                const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                                float bgR, float bgG, float bgB, const float4 *__restrict__ image,
                                                const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
                                                float fixedScale, unsigned long long *__restrict__ rawFixed,
                                                float *__restrict__ rawFloat, const float *__restrict__ edge, Staged *batch,
-                                               uint32_t *batchIndex) {
+                                               uint32_t *batchIndex, uint32_t *queue) {
   uint32_t tile;
   const BlockGeometry geo = block_geometry<TW, TH>(item, gridW, width, height, tile);
   if (geo.x0 >= width || geo.y0 >= height) return; // a block past the image's edge (uniform)
@@ -355,7 +408,8 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
   if constexpr (Recompute) {
     float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f;
     logTotal = 0.0f;
-    composite_block<false>(splats, entryValues, range, geo, batch, inside, px, py, bits, r, g, b, T, logTotal, stop);
+    composite_block<false>(splats, entryKeys, entryValues, 1u << (tileBits + geo.block), range, geo, batch, queue, inside, px, py, bits, r,
+                           g, b, T, logTotal, stop);
     if (!inside) stop = range.y;
   } else {
     logTotal = inside ? image[at].w : 0.0f;
@@ -370,10 +424,12 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
   }
   float Br = bgR, Bg = bgG, Bb = bgB, logBehind = 0.0f;
 
-  for (uint32_t start = first; start < range.y; start += kBlock) {
-    const uint32_t count = range.y - start < kBlock ? range.y - start : kBlock;
-    __syncthreads(); // the previous batch is read
-    const uint32_t kept = stage_batch(splats, entryValues, start, count, geo, batch, batchIndex);
+  const uint32_t blockBit = 1u << (tileBits + geo.block);
+  for (uint32_t pos = first < range.y ? first : range.y; pos < range.y;) {
+    __syncthreads(); // the previous batch and queue are read
+    const uint32_t queued = gather_entries<false>(entryKeys, blockBit, pos, range.y, queue);
+    __syncthreads();
+    const uint32_t kept = stage_batch(splats, entryValues, queue, queued, geo, batch, batchIndex);
     for (uint32_t k = 0; k < kept; ++k) {
       const Staged s = batch[k];
       const bool candidate = inside && s.entry >= stop && covers(s, bits);
@@ -458,7 +514,8 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
 
 
 template <bool Fixed, bool Recompute, bool Score, uint32_t TW, uint32_t TH>
-__global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryValues,
+__global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryKeys,
+                                                          const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                                           const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                                           float bgR, float bgG, float bgB, const float4 *__restrict__ image,
                                                           const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
@@ -466,8 +523,10 @@ __global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restric
                                                           float *__restrict__ rawFloat, const float *__restrict__ edge) {
   __shared__ Staged batch[kBlock];
   __shared__ uint32_t batchIndex[kBlock];
-  backward_block<Fixed, Recompute, Score, TW, TH>(blockIdx.x, splats, entryValues, ranges, width, height, gridW, bgR, bgG, bgB, image,
-                                                  adjoint, stops, fixedScale, rawFixed, rawFloat, edge, batch, batchIndex);
+  __shared__ uint32_t queue[kBlock];
+  backward_block<Fixed, Recompute, Score, TW, TH>(blockIdx.x, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR,
+                                                  bgG, bgB, image, adjoint, stops, fixedScale, rawFixed, rawFloat, edge, batch, batchIndex,
+                                                  queue);
 }
 
 // The backward pass of a batch's views in one launch, one workgroup per
@@ -476,12 +535,13 @@ template <bool Fixed, bool Recompute, uint32_t TW, uint32_t TH>
 __global__ __launch_bounds__(kBlock) void backward_views_kernel(RasterViews views, float bgR, float bgG, float bgB) {
   __shared__ Staged batch[kBlock];
   __shared__ uint32_t batchIndex[kBlock];
+  __shared__ uint32_t queue[kBlock];
   uint32_t item = blockIdx.x;
   const RasterView &w = views.view[view_of(views, item)];
-  backward_block<Fixed, Recompute, false, TW, TH>(item, w.splats, w.entryValues, w.ranges, w.width, w.height, w.gridW, bgR, bgG, bgB,
-                                                  w.image, w.adjoint, w.stops, w.fixedScale,
+  backward_block<Fixed, Recompute, false, TW, TH>(item, w.splats, w.entryKeys, w.entryValues, w.tileBits, w.ranges, w.width, w.height,
+                                                  w.gridW, bgR, bgG, bgB, w.image, w.adjoint, w.stops, w.fixedScale,
                                                   reinterpret_cast<unsigned long long *>(w.rawFixed), w.rawFloat, nullptr, batch,
-                                                  batchIndex);
+                                                  batchIndex, queue);
 }
 } // namespace
 
@@ -497,25 +557,26 @@ __global__ __launch_bounds__(kBlock) void backward_views_kernel(RasterViews view
     LAUNCH(16, 16);                                                                                                                \
   }
 
-void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryValues, const uint2 *ranges, uint32_t width,
-                    uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3], float4 *image, uint32_t *stops) {
+void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryKeys, const uint32_t *entryValues, uint32_t tileBits,
+                    const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
+                    float4 *image, uint32_t *stops) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_FORWARD(W, H)                                                                                                       \
-  hipLaunchKernelGGL((forward_kernel<W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats, entryValues, \
-                     ranges, width, height, gridW, background[0], background[1], background[2], image, stops)
+  hipLaunchKernelGGL((forward_kernel<W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats, entryKeys,   \
+                     entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, stops)
   VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD)
 #undef VSPLAT_FORWARD
 }
 
-void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryValues,
-                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
-                     const float4 *image, const float4 *adjoint, const uint32_t *stops, float fixedScale, int64_t *rawFixed,
-                     float *rawFloat, const float *edge) {
+void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryKeys,
+                     const uint32_t *entryValues, uint32_t tileBits, const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW,
+                     uint32_t tileH, const float background[3], const float4 *image, const float4 *adjoint, const uint32_t *stops,
+                     float fixedScale, int64_t *rawFixed, float *rawFloat, const float *edge) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_BACKWARD_SHAPE(F, R, S, W, H)                                                                                       \
   hipLaunchKernelGGL((backward_kernel<F, R, S, W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats,   \
-                     entryValues, ranges, width, height, gridW, background[0], background[1], background[2], image, adjoint, stops, \
-                     fixedScale, reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge)
+                     entryKeys, entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, \
+                     adjoint, stops, fixedScale, reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge)
 #define VSPLAT_BACKWARD_TTT(W, H) VSPLAT_BACKWARD_SHAPE(true, true, true, W, H)
 #define VSPLAT_BACKWARD_TTF(W, H) VSPLAT_BACKWARD_SHAPE(true, true, false, W, H)
 #define VSPLAT_BACKWARD_TFT(W, H) VSPLAT_BACKWARD_SHAPE(true, false, true, W, H)

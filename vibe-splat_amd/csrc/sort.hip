@@ -227,8 +227,10 @@ void launch_radix_sort(hipStream_t stream, uint32_t *keys, uint32_t *values, uin
   const uint32_t digit = div_up(bits, passes); // even split, at most 8 bits a pass
   uint32_t *srcK = keys, *srcV = values, *dstK = tmpKeys, *dstV = tmpValues;
   for (uint32_t p = 0; p < passes; ++p) {
+    // The last pass takes the bits that are left, so that no bit above `bits`
+    // (which may carry other data) takes part in the order.
     const uint32_t shift = p * digit;
-    switch (digit) {
+    switch (digit < bits - shift ? digit : bits - shift) {
     case 1: sort_pass<1>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
     case 2: sort_pass<2>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;
     case 3: sort_pass<3>(stream, srcK, srcV, dstK, dstV, countDev, capacity, shift, scratch); break;

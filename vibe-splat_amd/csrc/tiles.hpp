@@ -100,7 +100,9 @@ struct EmitOwner {
   uint32_t gaussian;
   uint32_t maskLo, maskHi;
   uint32_t offset; // its first entry
-  uint32_t pad[3];
+  uint32_t last;   // the box's last tile column and row, tx1 | ty1 << 16
+  uint32_t edges;  // box_block_edges
+  uint32_t pad;
 };
 static_assert(sizeof(EmitOwner) == 48, "48-byte emit owner");
 
@@ -193,6 +195,42 @@ __device__ __forceinline__ BigSplat big_splat_of_lane(const BigSplat &b, uint32_
                            lane_f32(b.test.cy, j), lane_f32(b.test.limit, j), lane_f32(b.test.iqx, j), lane_f32(b.test.iqz, j)},
                   lane_u32(b.tx0, j), lane_u32(b.ty0, j), lane_u32(b.width, j), lane_u32(b.candidates, j), lane_u32(b.gaussian, j),
                   lane_u32(b.first, j)};
+}
+
+// Where a splat's pixel box starts and ends in the 8x8 pixel blocks of the
+// first and last tiles of its tile box (tx0..tx1, ty0..ty1), packed as
+// c0 | c1 << 8 | r0 << 16 | r1 << 24: in the first tile column the box's
+// block columns are c0 and up, in the last those below c1 (and the same for
+// rows); a column or row the box misses has c0 = blocks across (r0 = blocks
+// down), c1 = 0 (r1 = 0). Tiles clip at the image's edge.
+__device__ __forceinline__ uint32_t box_block_edges(const PixelBox &p, uint32_t tx0, uint32_t ty0, uint32_t tx1, uint32_t ty1,
+                                                    TileShape shape, uint32_t width, uint32_t height) {
+  const uint32_t across = shape.w >> 3, down = shape.h >> 3;
+  const uint32_t x0 = tx0 * shape.w, y0 = ty0 * shape.h, x1 = tx1 * shape.w, y1 = ty1 * shape.h;
+  const uint32_t lx = p.ax > x0 ? p.ax : x0, ly = p.ay > y0 ? p.ay : y0;
+  const uint32_t endX0 = x0 + shape.w < width ? x0 + shape.w : width, endY0 = y0 + shape.h < height ? y0 + shape.h : height;
+  const uint32_t c0 = lx >= endX0 ? across : (lx - x0) >> 3, r0 = ly >= endY0 ? down : (ly - y0) >> 3;
+  const uint32_t endX1 = x1 + shape.w < width ? x1 + shape.w : width, endY1 = y1 + shape.h < height ? y1 + shape.h : height;
+  const uint32_t hx = p.bx < endX1 ? p.bx : endX1, hy = p.by < endY1 ? p.by : endY1;
+  const uint32_t c1 = hx > x1 ? (((hx - 1u - x1) >> 3) + 1u < across ? ((hx - 1u - x1) >> 3) + 1u : across) : 0u;
+  const uint32_t r1 = hy > y1 ? (((hy - 1u - y1) >> 3) + 1u < down ? ((hy - 1u - y1) >> 3) + 1u : down) : 0u;
+  return c0 | (c1 << 8) | (r0 << 16) | (r1 << 24);
+}
+
+// The 8x8 pixel blocks of tile (tx, ty) of a splat's tile box that its pixel
+// box reaches, bit q for the tile's block q (row-major, shape.w / 8 across,
+// as the raster numbers them): exactly the blocks for which splat_box, given
+// the block, is not empty, so the raster pass stages the entry there only.
+__device__ __forceinline__ uint32_t block_mask(uint32_t edges, uint32_t tx, uint32_t ty, uint32_t tx0, uint32_t ty0, uint32_t tx1,
+                                               uint32_t ty1, TileShape shape) {
+  const uint32_t across = shape.w >> 3, down = shape.h >> 3;
+  const uint32_t c0 = tx == tx0 ? edges & 0xFFu : 0u, c1 = tx == tx1 ? (edges >> 8) & 0xFFu : across;
+  const uint32_t r0 = ty == ty0 ? (edges >> 16) & 0xFFu : 0u, r1 = ty == ty1 ? edges >> 24 : down;
+  const uint32_t row = (1u << across) - 1u;                  // one row of blocks
+  const uint32_t column = 0xFFFFFFFFu / row & ((1u << (across * down)) - 1u); // bit 0 of every row
+  const uint32_t cols = (row << c0) & (row >> (across - c1)) & row;
+  const uint32_t rows = (column << (r0 * across)) & (column >> ((down - r1) * across));
+  return cols * rows;
 }
 
 // Splats whose bounding box has at most this many tiles keep the outcome of

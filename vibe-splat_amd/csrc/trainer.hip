@@ -350,6 +350,12 @@ void Trainer::set_geometry(ViewBuffers &b, int width, int height, uint32_t tileW
   b.tiles = b.gridW * b.gridH;
   b.tileBits = 1;
   while ((uint64_t{1} << b.tileBits) < b.tiles) ++b.tileBits;
+  // An entry's key holds its tile and, above it, one bit per 8x8 block of the
+  // tile; the list pass packs pixel coordinates in 16 bits.
+  if (b.tileBits + (tileW / 8u) * (tileH / 8u) > 32u) {
+    throw Error(VSPLAT_RUNTIME_ERROR, "the image has too many tiles of this shape for the tile keys");
+  }
+  if (width > 0xFFFF || height > 0xFFFF) throw Error(VSPLAT_RUNTIME_ERROR, "the image is too wide or too tall");
   // The loss works on 16x16 tiles of its own, whatever the raster's shape.
   b.lossTiles = static_cast<uint32_t>(div_up(static_cast<uint64_t>(width), 16) * div_up(static_cast<uint64_t>(height), 16));
 }
@@ -649,8 +655,9 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   if (timed) begin_timing(true);
   const ViewParams v = with_tiles(b, view);
   enqueue_lists(b, v, mode, true);
-  launch_forward(b.stream, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(), b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH,
-                 options_.background, b.image.as<float4>(), options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>());
+  launch_forward(b.stream, b.splats.as<Splat>(), b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tileBits, b.ranges.as<uint2>(),
+                 v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
+                 options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>());
   mark("composite");
   if (mode == kModeRender) return;
   enqueue_loss(b, v, deviceTarget);
@@ -658,8 +665,9 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
   // a resolution of 2^-20 (trainer.cpp:586-591 keeps 1/64 in 32 bits).
   const bool scoring = scoreShare_ > 0.0f;
-  launch_backward(b.stream, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
-                  b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
+  launch_backward(b.stream, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryKeys.as<uint32_t>(),
+                  b.entryValues.as<uint32_t>(), b.tileBits, b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background,
+                  b.image.as<float4>(),
                   b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixed_scale(v), b.rawFixed.as<int64_t>(), b.rawFloat.as<float>(),
                   scoring ? b.edge.as<float>() : nullptr);
   mark("backward");
@@ -696,11 +704,12 @@ void Trainer::enqueue_lists(ViewBuffers &b, const ViewParams &v, uint32_t mode, 
   launch_fill_u32(s, b.ranges.as<uint32_t>(), static_cast<uint64_t>(b.tiles) * 2, 0u);
   if (b.entryCapacity > 0) {
     launch_bin_emit(s, b.keys.as<uint32_t>(), b.values.as<uint32_t>(), b.splats.as<Splat>(), b.tileRecords.as<TileRecord>(), count_, b.gridW,
-                    b.tileW, b.tileH, b.coverage.as<uint32_t>(), counters, b.bigList.as<uint32_t>(), b.bigBits.as<uint64_t>(),
-                    b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>());
+                    b.tileW, b.tileH, static_cast<uint32_t>(b.width), static_cast<uint32_t>(b.height), b.tileBits,
+                    b.coverage.as<uint32_t>(), counters, b.bigList.as<uint32_t>(), b.bigBits.as<uint64_t>(), b.entryKeys.as<uint32_t>(),
+                    b.entryValues.as<uint32_t>());
     launch_radix_sort(s, b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tmpEntryKeys.as<uint32_t>(), b.tmpEntryValues.as<uint32_t>(),
                       counters + kEntriesUsed, static_cast<uint32_t>(b.entryCapacity), b.tileBits, b.sortScratch.as<uint32_t>());
-    launch_tile_ranges(s, b.entryKeys.as<uint32_t>(), counters, static_cast<uint32_t>(b.entryCapacity), b.ranges.as<uint2>());
+    launch_tile_ranges(s, b.entryKeys.as<uint32_t>(), counters, static_cast<uint32_t>(b.entryCapacity), b.tileBits, b.ranges.as<uint2>());
   }
   if (marks) mark("tile lists");
 }
@@ -1273,7 +1282,9 @@ void Trainer::launch_batch_raster(uint32_t n, const ViewParams *params, bool bac
       if (group.count == kRasterViewsPerLaunch) flush();
       RasterView &w = group.view[group.count++];
       w.splats = b.splats.as<Splat>();
+      w.entryKeys = b.entryKeys.as<uint32_t>();
       w.entryValues = b.entryValues.as<uint32_t>();
+      w.tileBits = b.tileBits;
       w.ranges = b.ranges.as<uint2>();
       w.image = b.image.as<float4>();
       w.stops = options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>();
@@ -1648,8 +1659,9 @@ void Trainer::update_curvature() {
   } else {
     launch_fill_u32(stream_, b.rawFloat.as<uint32_t>(), static_cast<uint64_t>(count_) * kRawPerSplat, 0u);
   }
-  launch_backward(stream_, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryValues.as<uint32_t>(),
-                  b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
+  launch_backward(stream_, fixedPoint_, options_.recompute_transmittance != 0, b.splats.as<Splat>(), b.entryKeys.as<uint32_t>(),
+                  b.entryValues.as<uint32_t>(), b.tileBits, b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background,
+                  b.image.as<float4>(),
                   b.adjoint.as<float4>(), b.stops.as<uint32_t>(), curvature_fixed_scale(), b.rawFixed.as<int64_t>(),
                   b.rawFloat.as<float>());
   const bool statistics = statisticsNext_;
@@ -1727,7 +1739,7 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
   const ViewBuffers &b = *views_[view];
   const uint64_t n = count_, pixels = static_cast<uint64_t>(b.width) * static_cast<uint64_t>(b.height);
   uint64_t entries = 0;
-  if (name == "entry_keys" || name == "entry_values") {
+  if (name == "entry_keys" || name == "entry_values" || name == "entry_blocks") {
     uint32_t counters[kCounterWords];
     check(hipStreamSynchronize(stream_), "inspect");
     check(hipMemcpy(counters, b.counters.as<uint32_t>(), sizeof(counters), hipMemcpyDeviceToHost), "inspect counters");
@@ -1743,7 +1755,8 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
       {"keys", &b.keys, n * 4},
       {"values", &b.values, n * 4},
       {"coverage", &b.coverage, n * 4},
-      {"entry_keys", &b.entryKeys, entries * 4},
+      {"entry_keys", &b.entryKeys, entries * 4},   // each entry's tile
+      {"entry_blocks", &b.entryKeys, entries * 4}, // the tile's 8x8 blocks (bit q: block q) the entry's splat reaches
       {"entry_values", &b.entryValues, entries * 4},
       {"ranges", &b.ranges, static_cast<uint64_t>(b.tiles) * 8},
       {"image", &b.image, pixels * 16},
@@ -1770,6 +1783,13 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
     if (out != nullptr && copy > 0) {
       check(hipStreamSynchronize(stream_), "inspect");
       check(hipMemcpy(out, item.array->as<void>(), copy, hipMemcpyDeviceToHost), "inspect copy");
+      // The keys hold both: the tile in the low tileBits bits, the blocks above.
+      uint32_t *words = static_cast<uint32_t *>(out);
+      if (name == "entry_keys") {
+        for (uint64_t k = 0; k < copy / 4; ++k) words[k] &= (1u << b.tileBits) - 1u;
+      } else if (name == "entry_blocks") {
+        for (uint64_t k = 0; k < copy / 4; ++k) words[k] >>= b.tileBits;
+      }
     }
     return;
   }
