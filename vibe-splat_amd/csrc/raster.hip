@@ -91,10 +91,11 @@ __device__ __forceinline__ Staged stage(const Splat &s, const BlockGeometry &geo
 __device__ __forceinline__ uint32_t lane_bits(uint32_t lx, uint32_t ly) { return (1u << lx) | (1u << (16u + ly)); }
 __device__ __forceinline__ bool covers(const Staged &s, uint32_t bits) { return (s.cols & bits) == bits; }
 
-// The wave votes as scalar masks: a ballot of a lane predicate is a scalar
-// comparison away from a uniform branch.
-__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0u; }
-__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0u; }
+// The wave votes as scalar masks: a ballot of one comparison is that
+// comparison's lane mask, so the loops vote with the AND of the ballots of
+// single comparisons (scalar work) rather than the ballot of a combined
+// predicate (which costs vector instructions to form).
+__device__ __forceinline__ uint64_t vote(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // The position of set bit number k (from 0) of m, which has more than k.
 __device__ __forceinline__ uint32_t kth_set_bit(uint64_t m, uint32_t k) {
@@ -180,8 +181,9 @@ __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats
                                                 uint32_t bits, float &r, float &g, float &b, float &T, float &logT, uint32_t &stop) {
   stop = range.x;
   bool done = !inside;
+  uint64_t doneLanes = ~vote(inside); // lanes whose pixel is finished (or outside the image)
   for (uint32_t pos = range.y; pos > range.x;) {
-    if (wave_all(done)) break;
+    if (doneLanes == ~0ull) break;
     __syncthreads(); // the previous batch and queue are read
     const uint32_t queued = gather_entries<true>(entryKeys, blockBit, pos, range.x, queue);
     __syncthreads();
@@ -189,21 +191,26 @@ __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats
     // The batch runs front to back already (the queue was filled walking down).
     for (uint32_t k = 0; k < kept; ++k) {
       const Staged s = batch[k];
-      const bool candidate = !done && covers(s, bits);
-      if (!wave_any(candidate)) continue;
+      const bool covered = covers(s, bits);
+      const uint64_t candidates = vote(covered) & ~doneLanes;
+      if (candidates == 0) continue;
+      const bool candidate = !done && covered;
       const float dx = px - s.cx, dy = py - s.cy;
       const float power = power_at(s, dx, dy);
       // Below -87 alpha is far under 1/255 either way, so the clamp in exp_finite decides nothing.
       const float e = exp_finite(power);
       const float a = s.o * e;
       const float alpha = a < 0.99f ? a : 0.99f;
-      const bool ok = candidate && !(power > 0.0f) && alpha >= kMinAlpha;
-      if (!wave_any(ok)) continue;
+      const bool front = !(power > 0.0f), visible = alpha >= kMinAlpha;
+      const uint64_t oks = candidates & vote(front) & vote(visible);
+      if (oks == 0) continue;
+      const bool ok = candidate && front && visible;
       // Without branches: a lane that takes nothing adds a zero weight
       // (fma(0, c, r) is r) and keeps its transmittance by selection.
       const float logRest = log_normal(1.0f - alpha);
       const float logNext = logT + logRest;
-      const bool stopHere = ok && logNext < kLogMinTransmittance;
+      const bool below = logNext < kLogMinTransmittance;
+      const bool stopHere = ok && below;
       const bool take = ok && !stopHere;
       if constexpr (Colour) {
         const float w = take ? T * alpha : 0.0f;
@@ -215,7 +222,8 @@ __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats
       logT = take ? logNext : logT;
       stop = stopHere ? s.entry + 1u : stop;
       done = done || stopHere;
-      if (wave_all(done)) break;
+      doneLanes |= oks & vote(below);
+      if (doneLanes == ~0ull) break;
     }
   }
 }
@@ -425,6 +433,7 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
   float Br = bgR, Bg = bgG, Bb = bgB, logBehind = 0.0f;
 
   const uint32_t blockBit = 1u << (tileBits + geo.block);
+  const uint64_t insideLanes = vote(inside);
   for (uint32_t pos = first < range.y ? first : range.y; pos < range.y;) {
     __syncthreads(); // the previous batch and queue are read
     const uint32_t queued = gather_entries<false>(entryKeys, blockBit, pos, range.y, queue);
@@ -432,15 +441,18 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
     const uint32_t kept = stage_batch(splats, entryValues, queue, queued, geo, batch, batchIndex);
     for (uint32_t k = 0; k < kept; ++k) {
       const Staged s = batch[k];
-      const bool candidate = inside && s.entry >= stop && covers(s, bits);
-      if (!wave_any(candidate)) continue;
+      const bool live = s.entry >= stop, covered = covers(s, bits);
+      const uint64_t candidates = insideLanes & vote(live) & vote(covered);
+      if (candidates == 0) continue;
+      const bool candidate = inside && live && covered;
       const float dx = px - s.cx, dy = py - s.cy;
       const float power = power_at(s, dx, dy);
       const float e = exp_finite(power);
       const float a = s.o * e;
       const float alpha = a < 0.99f ? a : 0.99f;
-      const bool ok = candidate && !(power > 0.0f) && alpha >= kMinAlpha;
-      if (!wave_any(ok)) continue;
+      const bool front = !(power > 0.0f), visible = alpha >= kMinAlpha;
+      if ((candidates & vote(front) & vote(visible)) == 0) continue;
+      const bool ok = candidate && front && visible;
       const float logRest = log_normal(1.0f - alpha);
       const float T = exp_finite((logTotal - logBehind) - logRest);
       // A lane that takes nothing from the splat carries zero factors, so its
