@@ -69,10 +69,14 @@ SCHEDULES = [vsplat.BATCH_VIEW_STREAMS, vsplat.BATCH_ONE_LAUNCH]
 
 
 @pytest.mark.parametrize("schedule", SCHEDULES)
-def test_gradients_summed_over_views_bit_exact(schedule):
+@pytest.mark.parametrize("tile_shape", [vsplat.TILES_16X16, vsplat.TILES_32X8, vsplat.TILES_32X32])
+def test_gradients_summed_over_views_bit_exact(schedule, tile_shape):
+    # Every tile shape gives the 16x16 sums (the batch's raster kernels read
+    # the lists' block marks with 32x32 tiles); Oracle B lists 16x16 tiles, so
+    # only those entry counts are compared.
     cloud, cams, targets = _ring()
     want, want_stats, per_view, _ = _model_sum(cloud, cams, targets)
-    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, batch_raster=schedule)) as t:
+    with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, batch_raster=schedule, tile_shape=tile_shape)) as t:
         stats = t.step_views(cams, targets)
         g = t.gradients()
         dens = t.densify_stats()
@@ -81,7 +85,8 @@ def test_gradients_summed_over_views_bit_exact(schedule):
     np.testing.assert_array_equal(_bits(dens), _bits(want_stats))
     assert len(stats) == len(cams)
     for s, (visible, entries, l1, ssim) in zip(stats, per_view):
-        assert s.visible_splats == visible and s.tile_entries == entries
+        assert s.visible_splats == visible
+        assert tile_shape != vsplat.TILES_16X16 or s.tile_entries == entries
         assert abs(s.l1 - l1) < 1e-6 and abs(s.ssim - ssim) < 1e-6
 
 
