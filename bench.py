@@ -216,6 +216,9 @@ def copy_rate(nbytes: int = 1 << 30, repeats: int = 20) -> dict:
     return {"bytes": nbytes, "ms_per_copy": round(ms, 4), "gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
+# Timed passes per tile shape when tuning (the least kept; the tuning is not
+# timed): shapes within a few per cent of each other need several to be told apart.
+TUNE_REPEATS = 6
 TILE_SHAPES = {"auto": vsplat.TILES_AUTO, "16x16": vsplat.TILES_16X16, "8x8": vsplat.TILES_8X8, "32x8": vsplat.TILES_32X8,
                "32x32": vsplat.TILES_32X32}
 
@@ -236,7 +239,7 @@ def batched_ring(steps: int, warmup: int, count: int | None, tile_shape: str) ->
     shapes = {}
     try:
         if tile_shape == "auto":  # the sizes' shapes tuned together, on the batch
-            for cam, shape in zip(w.cameras, trainer.autotune_views(w.cameras, ptrs, 2)):
+            for cam, shape in zip(w.cameras, trainer.autotune_views(w.cameras, ptrs, TUNE_REPEATS)):
                 shapes[(cam.width, cam.height)] = shape
         trainer.step_views(w.cameras, w.targets, rates)  # sizes every buffer and the entry reservation
         single = []
@@ -306,7 +309,7 @@ def plain_ring_1280x720(steps: int, warmup: int, count: int | None, tile_shape: 
             buffer = vsplat.DeviceBuffer(target)
             try:
                 if tile_shape == "auto":
-                    trainer.autotune_tiles(cam, buffer.ptr, 2)
+                    trainer.autotune_tiles(cam, buffer.ptr, TUNE_REPEATS)
                 trainer.compute_gradients(cam, target)
                 for _ in range(warmup):
                     trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
@@ -342,7 +345,7 @@ def batched_1280x720(steps: int, warmup: int, count: int | None, tile_shape: str
     chosen = None
     try:
         if tile_shape == "auto":
-            chosen = trainer.autotune_views(cams, ptrs, 2)[0]
+            chosen = trainer.autotune_views(cams, ptrs, TUNE_REPEATS)[0]
         trainer.step_views(cams, targets)  # gradients mode; sizes every view's buffers
         trainer.capture_views(cams, ptrs, vsplat.MODE_GRADIENTS)
         trainer.launch_captured(warmup)
@@ -379,7 +382,7 @@ def other_form(index: int, steps: int, warmup: int, tile_shape: str, fixed: bool
     chosen = None
     try:
         if tile_shape == "auto":
-            chosen = trainer.autotune_tiles(cam, buffer.ptr, 2)
+            chosen = trainer.autotune_tiles(cam, buffer.ptr, TUNE_REPEATS)
         first = trainer.compute_gradients(cam, target)
         for _ in range(warmup):
             trainer.enqueue(cam, buffer.ptr, vsplat.MODE_GRADIENTS)
@@ -439,7 +442,7 @@ def main():
     d_target = device_target.ptr
     if args.tile_shape == "auto":
         log("tile shape: timing 16x16, 8x8, 32x8 and 32x32")
-        tiles = trainer.autotune_tiles(cam, d_target, 3)
+        tiles = trainer.autotune_tiles(cam, d_target, TUNE_REPEATS)
         log(f"  chose {tiles[0]}x{tiles[1]}")
     else:
         tiles = tuple(int(v) for v in args.tile_shape.split("x"))
