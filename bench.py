@@ -542,16 +542,21 @@ def main():
         plain_ring = plain_ring_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed)
         result["plain_ring_1280x720"] = plain_ring
         log(f"  {plain_ring['ms_per_view']} ms, mean {plain_ring['mean_ms_per_view']} ms per view")
+        log("the same with the sums as float atomics")
+        plain_ring_float = plain_ring_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, False)
+        result["plain_ring_1280x720_float_sums"] = plain_ring_float
+        log(f"  {plain_ring_float['ms_per_view']} ms, mean {plain_ring_float['mean_ms_per_view']} ms per view")
 
-        def compare(leg):
-            # Per view against the plain single view of `value` and against the
-            # plain form on the same eight cameras.
+        def compare(leg, float_sums=False):
+            # Per view against the plain single view of `value`, and against the
+            # plain form on the same eight cameras with the same kind of sums.
+            same = plain_ring_float if float_sums else plain_ring
             leg["plain_single_view_ms"] = result["value"]
             leg["plain_vs_batched_per_view"] = round(result["value"] / leg["graph_ms_per_view"], 4)
-            leg["plain_same_views_ms_per_view"] = plain_ring["mean_ms_per_view"]
-            leg["plain_same_views_vs_batched_per_view"] = round(plain_ring["mean_ms_per_view"] / leg["graph_ms_per_view"], 4)
+            leg["plain_same_views_ms_per_view"] = same["mean_ms_per_view"]
+            leg["plain_same_views_vs_batched_per_view"] = round(same["mean_ms_per_view"] / leg["graph_ms_per_view"], 4)
             return (f"{leg['graph_ms_per_view']} ms per view, {leg['plain_vs_batched_per_view']}x the plain single view, "
-                    f"{leg['plain_same_views_vs_batched_per_view']}x the plain form on the same views")
+                    f"{leg['plain_same_views_vs_batched_per_view']}x the plain form on the same views with the same sums")
 
         log("batched bf16 fwd+bwd: eight 1280x720 views")
         same = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape)
@@ -560,13 +565,13 @@ def main():
         log("batched bf16 fwd+bwd, sums accumulated in float32: eight 1280x720 views")
         same_float = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed=False)
         result["batched_1280x720_float_sums"] = same_float
-        log("  " + compare(same_float))
+        log("  " + compare(same_float, True))
         log("batched bf16 fwd+bwd, one forward and one backward launch over all views: eight 1280x720 views")
         for key, fixed_sums in (("batched_1280x720_one_launch", True), ("batched_1280x720_one_launch_float_sums", False)):
             one = batched_1280x720(max(5, args.steps // 2), 2, args.count, args.tile_shape, fixed=fixed_sums,
                                    schedule=vsplat.BATCH_ONE_LAUNCH)
             result[key] = one
-            log(f"  {key}: " + compare(one))
+            log(f"  {key}: " + compare(one, not fixed_sums))
     if not args.no_other_forms and args.count is None:
         result["other_forms"] = {}
         for key, index, fixed_sums, form_options in (
