@@ -815,9 +815,12 @@ extern "C" int ob_loss(const float *image, const uint8_t *target, uint32_t width
 namespace {
 
 // train_backward.glsl:78-158 for one pixel and one entry: the 14 values, or
-// false when the pixel takes nothing from the splat. B, logBehind advance.
+// false when the pixel takes nothing from the splat. B and behind advance:
+// behind is the transmittance in front of the entries visited so far (at the
+// start exp(logTotal)), and an entry's own transmittance is behind / (1 - alpha),
+// where train_backward.glsl:122-123 takes exp(logTotal - logBehind - logRest).
 struct PixelState {
-  float logTotal, A[3], B[3], logBehind;
+  float logTotal, A[3], B[3], behind;
   float marked, edge; // a scored view: the adjoint's fourth value and the target's edge weight
   std::uint32_t stop;
   bool inside;
@@ -830,8 +833,7 @@ bool backward_pair(const Splat &s, const std::uint32_t box[4], std::uint32_t x, 
   if (x < box[0] || x >= box[2] || y < box[1] || y >= box[3]) return false;
   float dx, dy, power, ep, alpha;
   if (!splat_at(s, x, y, dx, dy, power, ep, alpha)) return false;
-  const float logRest = flog(1.0f - alpha);
-  const float T = fexp((p.logTotal - p.logBehind) - logRest);
+  const float T = p.behind / (1.0f - alpha);
   const float Ta = T * alpha;
   g[6] = Ta * p.A[0];
   g[7] = Ta * p.A[1];
@@ -857,7 +859,7 @@ bool backward_pair(const Splat &s, const std::uint32_t box[4], std::uint32_t x, 
   p.B[0] = fma(alpha, s.r - p.B[0], p.B[0]);
   p.B[1] = fma(alpha, s.g - p.B[1], p.B[1]);
   p.B[2] = fma(alpha, s.b - p.B[2], p.B[2]);
-  p.logBehind = p.logBehind + logRest;
+  p.behind = T;
   return true;
 }
 
@@ -893,7 +895,7 @@ void backward_waves(const float *splats, const uint32_t *entryValues, const uint
           p.A[ch] = p.inside ? adjoint[at * 4 + ch] : 0.0f;
           p.B[ch] = background[ch];
         }
-        p.logBehind = 0.0f;
+        p.behind = fexp(p.logTotal);
         p.marked = p.inside && edge != nullptr ? adjoint[at * 4 + 3] : 0.0f;
         p.edge = p.inside && edge != nullptr ? edge[at] : 0.0f;
         p.stop = p.inside ? stops[at] : end;

@@ -430,7 +430,13 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
     const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(first), static_cast<int>(d), 64));
     first = other < first ? other : first;
   }
-  float Br = bgR, Bg = bgG, Bb = bgB, logBehind = 0.0f;
+  // The colour behind the splats visited so far, and the transmittance in
+  // front of them: before any, the background and the pixel's final
+  // transmittance. Walking back, a splat's own transmittance is the one in
+  // front of those behind it over 1 - alpha, one rounding a step, where
+  // train_backward.glsl:119 takes the exp of the difference of two log sums.
+  float Br = bgR, Bg = bgG, Bb = bgB;
+  float Tbehind = exp_finite(logTotal);
 
   const uint32_t blockBit = 1u << (tileBits + geo.block);
   const uint64_t insideLanes = vote(inside);
@@ -453,8 +459,7 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
       const bool front = !(power > 0.0f), visible = alpha >= kMinAlpha;
       if ((candidates & vote(front) & vote(visible)) == 0) continue;
       const bool ok = candidate && front && visible;
-      const float logRest = log_normal(1.0f - alpha);
-      const float T = exp_finite((logTotal - logBehind) - logRest);
+      const float T = Tbehind / (1.0f - alpha); // alpha <= 0.99: never a division by zero
       // A lane that takes nothing from the splat carries zero factors, so its
       // values below are all zero (every other factor is finite); the sign of
       // a zero cannot change a total that is not zero.
@@ -485,12 +490,12 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
         v[13] = (edgeWeight * Ta) * perPixel;
         v[14] = v[15] = 0.0f;
       }
-      // The colour and log-transmittance behind: unchanged where alpha is taken as 0.
+      // The colour behind and the transmittance in front: unchanged where alpha is taken as 0.
       const float alphaBehind = ok ? alpha : 0.0f;
       Br = fma32(alphaBehind, s.r - Br, Br);
       Bg = fma32(alphaBehind, s.g - Bg, Bg);
       Bb = fma32(alphaBehind, s.b - Bb, Bb);
-      logBehind = logBehind + (ok ? logRest : 0.0f);
+      Tbehind = ok ? T : Tbehind;
       uint32_t value;
       bool emits;
       if constexpr (Score) {
