@@ -369,7 +369,10 @@ int vsplat_autotune_views(vsplat_trainer *trainer, uint32_t n, const vsplat_came
  * reaches, bit q for block q, row-major; every block but for 32x32 tiles), "entry_values" (4 bytes per tile
  * entry each), "ranges" (8 bytes per tile),
  * "image" (16 bytes per pixel: rgb, log T), "stops", "adjoint" (16 per pixel),
- * "fields" (36 per pixel), "tile_totals" (8 per tile), "raw" (14 int64 per
+ * "fields" (36 per pixel), "tile_totals" (8 per tile), "block_work" (per
+ * raster launch index, 4 bytes: the entries the backward walks for that 8x8
+ * block), "block_order" (the backward's launch order: a permutation of those
+ * indices, heaviest first on each XCD), "raw" (14 int64 per
  * Gaussian with fixed point, else 14 float), "counters" (4 words),
  * "state_positions", "state_scales", "state_rotations", "state_opacities",
  * "state_sh" (Adam's state, one word per parameter: bf16 m low, bf16 v high),

@@ -67,6 +67,48 @@ def test_tile_shape_gives_the_same_results(name, shape):
     _check_lists(cloud, cam, base, base_stats, (16, 16))
 
 
+
+def _heavy_first(work):
+    v = work.astype(np.int64) + 1
+    e = np.floor(np.log2(v)).astype(np.int64)
+    fraction = np.where(e >= 3, v >> np.maximum(e - 3, 0), v << np.maximum(3 - e, 0)) & 7
+    return 255 - (8 * e + fraction)
+
+
+@pytest.mark.parametrize("shape", [vsplat.TILES_16X16, vsplat.TILES_8X8, vsplat.TILES_32X8, vsplat.TILES_32X32])
+def test_backward_runs_each_xcds_blocks_heaviest_first(shape):
+    """The backward's launch order: a permutation of the raster launch indices
+    that keeps each index on its XCD (index mod 8) and runs each XCD's blocks
+    by non-increasing work, the work of a block being the entries from its
+    pixels' lowest stop to its tile list's end, recomputed here from the
+    ranges and stops."""
+    w = F.config(1, count=20_000)
+    cam = w.cameras[0]
+    tw, th = SHAPES.get(shape, (16, 16))
+    with vsplat.Trainer(w.cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=shape)) as t:
+        t.compute_gradients(cam, w.targets[0])
+        work, order = t.inspect("block_work"), t.inspect("block_order")
+        ranges, stops = t.inspect("ranges"), t.inspect("stops").reshape(cam.height, cam.width)
+    grid_w, grid_h = (cam.width + tw - 1) // tw, (cam.height + th - 1) // th
+    tiles, across, per_tile = grid_w * grid_h, tw // 8, (tw // 8) * (th // 8)
+    items = (tiles + 7) // 8 * 8 * per_tile
+    assert len(work) == items and len(order) == items
+    expected = np.zeros(items, np.int64)
+    for h in range(items):
+        u = h // 8
+        tile, q = (u // per_tile) * 8 + h % 8, u % per_tile
+        x0, y0 = (tile % grid_w) * tw + (q % across) * 8, (tile // grid_w) * th + (q // across) * 8
+        if tile >= tiles or x0 >= cam.width or y0 >= cam.height:
+            continue
+        end = int(ranges[tile][1])
+        expected[h] = end - min(int(stops[y0:y0 + 8, x0:x0 + 8].min()), end)
+    np.testing.assert_array_equal(work, expected)
+    assert sorted(order.tolist()) == list(range(items))
+    assert (order % 8 == np.arange(items) % 8).all()
+    keys = _heavy_first(work[order]).reshape(-1, 8)
+    assert (np.diff(keys, axis=0) >= 0).all()
+    print(f"{tw}x{th}: {items} blocks, heaviest {work.max()} entries, median {int(np.median(work))}", flush=True)
+
 def _check_lists(cloud, cam, got, stats, shape):
     # The lists: sorted by tile, depth order within a tile, ranges partitioning them.
     tw, th = shape

@@ -27,6 +27,7 @@
 // any order) or float.
 #include "device.hpp"
 #include "kernels.hpp"
+#include "lookback.hpp"
 
 namespace vsplat {
 
@@ -65,6 +66,22 @@ __device__ __forceinline__ BlockGeometry block_geometry(uint32_t h, uint32_t gri
   const uint32_t q = u % kPerTile;
   const uint32_t x0 = (tile % gridW) * TW + (q % kAcross) * 8u, y0 = (tile / gridW) * TH + (q / kAcross) * 8u;
   return {x0, y0, x0 + 8u < width ? x0 + 8u : width, y0 + 8u < height ? y0 + 8u : height, q};
+}
+
+// Heaviest first: a block's work (entries) as one of kOrderKeys keys, 8 per
+// octave (the exponent and the next 3 bits of work + 1), the largest work the
+// smallest key.
+__device__ __forceinline__ uint32_t heavy_first(uint32_t work) {
+  const uint32_t v = work < 0xFFFFFFFFu ? work + 1u : work;
+  const uint32_t e = 31u - static_cast<uint32_t>(__builtin_clz(v));
+  const uint32_t fraction = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
+  return kOrderKeys - 1u - (e * 8u + fraction);
+}
+
+// One lane of the forward's block: its work, and its place in its (XCD, key) slot.
+__device__ __forceinline__ void publish_work(const BlockOrder &order, uint32_t item, uint32_t work) {
+  order.work[item] = work;
+  order.place[item] = atomicAdd(&order.slots[(item % kRasterXcds) * kOrderKeys + heavy_first(work)], 1u);
 }
 
 // The block's pixels a splat can touch: inside the extent box and within the
@@ -234,22 +251,36 @@ __device__ __forceinline__ void forward_block(uint32_t item, const Splat *__rest
                                               const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                               const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                               float bgR, float bgG, float bgB, float4 *__restrict__ image, uint32_t *__restrict__ stops,
-                                              Staged *batch, uint32_t *queue) {
+                                              const BlockOrder &order, Staged *batch, uint32_t *queue) {
   uint32_t tile;
   const BlockGeometry geo = block_geometry<TW, TH>(item, gridW, width, height, tile);
-  if (geo.x0 >= width || geo.y0 >= height) return; // a block past the image's edge (uniform)
+  if (geo.x0 >= width || geo.y0 >= height) { // a block past the image's edge (uniform)
+    if (order.work != nullptr && threadIdx.x == 0) publish_work(order, item, 0u);
+    return;
+  }
   const uint32_t lx = threadIdx.x & 7u, ly = threadIdx.x >> 3;
   const uint32_t x = geo.x0 + lx, y = geo.y0 + ly;
   const bool inside = x < width && y < height;
   const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
   float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, logT = 0.0f;
   uint32_t stop;
-  composite_block<true>(splats, entryKeys, entryValues, 1u << (tileBits + geo.block), ranges[tile], geo, batch, queue, inside, px, py,
+  const uint2 range = ranges[tile];
+  composite_block<true>(splats, entryKeys, entryValues, 1u << (tileBits + geo.block), range, geo, batch, queue, inside, px, py,
                         lane_bits(lx, ly), r, g, b, T, logT, stop);
   if (inside) {
     const uint32_t at = y * width + x;
     image[at] = make_float4(fma32(T, bgR, r), fma32(T, bgG, g), fma32(T, bgB, b), logT);
     if (stops != nullptr) stops[at] = stop; // none when the backward recomputes them
+  }
+  if (order.work != nullptr) {
+    // The backward's walk over this block: the entries from its lowest stop to the list's end.
+    uint32_t low = inside ? stop : range.y;
+#pragma unroll
+    for (uint32_t d = 32; d >= 1; d >>= 1) {
+      const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(low), static_cast<int>(d), 64));
+      low = other < low ? other : low;
+    }
+    if (threadIdx.x == 0) publish_work(order, item, range.y - low);
   }
 }
 
@@ -258,11 +289,11 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict
                                                          const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
                                                          uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
-                                                         uint32_t *__restrict__ stops) {
+                                                         uint32_t *__restrict__ stops, BlockOrder order) {
   __shared__ Staged batch[kBlock];
   __shared__ uint32_t queue[kBlock];
   forward_block<TW, TH>(blockIdx.x, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR, bgG, bgB, image, stops,
-                        batch, queue);
+                        order, batch, queue);
 }
 
 // The view of a batched launch that block `item` belongs to, and the block's
@@ -283,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void forward_views_kernel(RasterViews views
   uint32_t item = blockIdx.x;
   const RasterView &w = views.view[view_of(views, item)];
   forward_block<TW, TH>(item, w.splats, w.entryKeys, w.entryValues, w.tileBits, w.ranges, w.width, w.height, w.gridW, bgR, bgG, bgB,
-                        w.image, w.stops, batch, queue);
+                        w.image, w.stops, BlockOrder{}, batch, queue);
 }
 
 // The fixed-point quantisation of a wave total (train_backward_fixed.comp:19-21),
@@ -537,11 +568,13 @@ __global__ __launch_bounds__(kBlock) void backward_kernel(const Splat *__restric
                                                           float bgR, float bgG, float bgB, const float4 *__restrict__ image,
                                                           const float4 *__restrict__ adjoint, const uint32_t *__restrict__ stops,
                                                           float fixedScale, unsigned long long *__restrict__ rawFixed,
-                                                          float *__restrict__ rawFloat, const float *__restrict__ edge) {
+                                                          float *__restrict__ rawFloat, const float *__restrict__ edge,
+                                                          const uint32_t *__restrict__ order) {
   __shared__ Staged batch[kBlock];
   __shared__ uint32_t batchIndex[kBlock];
   __shared__ uint32_t queue[kBlock];
-  backward_block<Fixed, Recompute, Score, TW, TH>(blockIdx.x, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR,
+  const uint32_t item = order != nullptr ? order[blockIdx.x] : blockIdx.x;
+  backward_block<Fixed, Recompute, Score, TW, TH>(item, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR,
                                                   bgG, bgB, image, adjoint, stops, fixedScale, rawFixed, rawFloat, edge, batch, batchIndex,
                                                   queue);
 }
@@ -560,7 +593,62 @@ __global__ __launch_bounds__(kBlock) void backward_views_kernel(RasterViews view
                                                   reinterpret_cast<unsigned long long *>(w.rawFixed), w.rawFloat, nullptr, batch,
                                                   batchIndex, queue);
 }
+
+constexpr uint32_t kOrderThreads = 256;
+static_assert(kOrderSlots == 8 * kOrderThreads, "eight slots a thread in the scan");
+
+// The backward's launch order: launch index h' = 8 j + x runs on XCD x, as
+// launch index h = 8 j + x would (every item keeps its XCD, so a tile's blocks
+// still share one L2), and takes the j-th heaviest of XCD x's items. The
+// forward counted its blocks per (XCD, key) slot and gave each a place in its
+// slot (publish_work); each workgroup here scans the 2048 counts and puts its
+// launch indices at their slot's start plus their place. Places within a slot
+// follow the forward's finishing order, which changes nothing: each item's
+// sums are added with atomics. The last workgroup to read the counts clears
+// them (and the ticket) for the next forward. `items` is a multiple of 8.
+__global__ __launch_bounds__(kOrderThreads) void block_order_kernel(BlockOrder order, uint32_t items, uint32_t *__restrict__ launchOrder) {
+  __shared__ uint32_t start[kOrderSlots];
+  __shared__ uint32_t waveTotals[kOrderThreads / 64];
+  __shared__ uint32_t lastGroup;
+  uint32_t count[8], sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    count[k] = order.slots[threadIdx.x * 8 + k];
+    sum += count[k];
+  }
+  uint32_t before;
+  block_exclusive_sum(sum, before, waveTotals);
+  // XCD x's items start at x * items / 8 of the flattened (XCD, key) order.
+  const uint32_t share = items / kRasterXcds;
+  before -= (threadIdx.x * 8 / kOrderKeys) * share;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    start[threadIdx.x * 8 + k] = before;
+    before += count[k];
+  }
+  __syncthreads(); // the counts are read and the starts in LDS
+  if (threadIdx.x == 0) {
+    __threadfence();
+    lastGroup = atomicAdd(&order.slots[kOrderSlots], 1u) == gridDim.x - 1 ? 1u : 0u;
+  }
+  const uint32_t h = blockIdx.x * kOrderThreads + threadIdx.x;
+  if (h < items) {
+    const uint32_t x = h % kRasterXcds;
+    const uint32_t j = start[x * kOrderKeys + heavy_first(order.work[h])] + order.place[h];
+    if (j < share) launchOrder[j * kRasterXcds + x] = h; // always, when the forward ran over the same items
+  }
+  __syncthreads();
+  if (lastGroup != 0) {
+    for (uint32_t k = threadIdx.x; k <= kOrderSlots; k += kOrderThreads) order.slots[k] = 0;
+  }
+}
 } // namespace
+
+void launch_block_order(hipStream_t stream, const BlockOrder &order, uint32_t items, uint32_t *launchOrder) {
+  if (items == 0) return;
+  hipLaunchKernelGGL(block_order_kernel, dim3((items + kOrderThreads - 1) / kOrderThreads), dim3(kOrderThreads), 0, stream, order, items,
+                     launchOrder);
+}
 
 // Calls LAUNCH(TW, TH) with the tile shape as compile-time constants.
 #define VSPLAT_TILE_DISPATCH(tileW, tileH, LAUNCH)                                                                                 \
@@ -576,11 +664,11 @@ __global__ __launch_bounds__(kBlock) void backward_views_kernel(RasterViews view
 
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryKeys, const uint32_t *entryValues, uint32_t tileBits,
                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
-                    float4 *image, uint32_t *stops) {
+                    float4 *image, uint32_t *stops, const BlockOrder &order) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_FORWARD(W, H)                                                                                                       \
   hipLaunchKernelGGL((forward_kernel<W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats, entryKeys,   \
-                     entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, stops)
+                     entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, stops, order)
   VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD)
 #undef VSPLAT_FORWARD
 }
@@ -588,12 +676,12 @@ void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *ent
 void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryKeys,
                      const uint32_t *entryValues, uint32_t tileBits, const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW,
                      uint32_t tileH, const float background[3], const float4 *image, const float4 *adjoint, const uint32_t *stops,
-                     float fixedScale, int64_t *rawFixed, float *rawFloat, const float *edge) {
+                     float fixedScale, int64_t *rawFixed, float *rawFloat, const float *edge, const uint32_t *order) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_BACKWARD_SHAPE(F, R, S, W, H)                                                                                       \
   hipLaunchKernelGGL((backward_kernel<F, R, S, W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats,   \
                      entryKeys, entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, \
-                     adjoint, stops, fixedScale, reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge)
+                     adjoint, stops, fixedScale, reinterpret_cast<unsigned long long *>(rawFixed), rawFloat, edge, order)
 #define VSPLAT_BACKWARD_TTT(W, H) VSPLAT_BACKWARD_SHAPE(true, true, true, W, H)
 #define VSPLAT_BACKWARD_TTF(W, H) VSPLAT_BACKWARD_SHAPE(true, true, false, W, H)
 #define VSPLAT_BACKWARD_TFT(W, H) VSPLAT_BACKWARD_SHAPE(true, false, true, W, H)

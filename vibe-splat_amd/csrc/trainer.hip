@@ -332,6 +332,13 @@ void Trainer::ensure_layout(ViewBuffers &b, int width, int height, uint32_t tile
   b.fields.ensure(pixels * 36);
   b.ranges.ensure(static_cast<size_t>(b.tiles) * 8);
   b.tileTotals.ensure(static_cast<size_t>(b.lossTiles) * 8);
+  const size_t blocks = raster_blocks(b.tiles, tileW, tileH);
+  b.blockWork.ensure(blocks * 4);
+  b.blockPlace.ensure(blocks * 4);
+  b.blockOrder.ensure(blocks * 4);
+  // The slots' counts and ticket are zero between passes (launch_block_order clears them).
+  b.blockSlots.ensure((kOrderSlots + 1) * 4);
+  launch_fill_u32(b.stream, b.blockSlots.as<uint32_t>(), kOrderSlots + 1, 0u);
 }
 
 // The tile grid of a view size and tile shape (no buffers).
@@ -655,11 +662,15 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   if (timed) begin_timing(true);
   const ViewParams v = with_tiles(b, view);
   enqueue_lists(b, v, mode, true);
+  const bool backward = mode != kModeRender;
+  BlockOrder order;
+  if (backward) order = BlockOrder{b.blockWork.as<uint32_t>(), b.blockPlace.as<uint32_t>(), b.blockSlots.as<uint32_t>()};
   launch_forward(b.stream, b.splats.as<Splat>(), b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tileBits, b.ranges.as<uint2>(),
                  v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
-                 options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>());
+                 options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>(), order);
+  if (backward) launch_block_order(b.stream, order, raster_blocks(b.tiles, b.tileW, b.tileH), b.blockOrder.as<uint32_t>());
   mark("composite");
-  if (mode == kModeRender) return;
+  if (!backward) return;
   enqueue_loss(b, v, deviceTarget);
   mark("loss");
   // Per-pixel derivatives carry 1 / (3 pixels); the scale undoes it and leaves
@@ -669,7 +680,7 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
                   b.entryValues.as<uint32_t>(), b.tileBits, b.ranges.as<uint2>(), v.width, v.height, b.tileW, b.tileH, options_.background,
                   b.image.as<float4>(),
                   b.adjoint.as<float4>(), b.stops.as<uint32_t>(), fixed_scale(v), b.rawFixed.as<int64_t>(), b.rawFloat.as<float>(),
-                  scoring ? b.edge.as<float>() : nullptr);
+                  scoring ? b.edge.as<float>() : nullptr, b.blockOrder.as<uint32_t>());
   mark("backward");
   if (!chain) return;
   enqueue_chain(b, v, mode, rates, b.stream, b.counters.as<uint32_t>());
@@ -1379,8 +1390,8 @@ uint64_t Trainer::capture_key(uint32_t n) const {
     const ViewBuffers &b = *views_[k];
     for (const DeviceArray *a : {&b.splats, &b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount, &b.tileRecords,
                                  &b.bigList, &b.bigBits, &b.rawFixed, &b.rawFloat, &b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues, &b.image,
-                                 &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.sortScratch, &b.scanScratch, &b.counters,
-                                 &b.sums})
+                                 &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.blockWork, &b.blockPlace, &b.blockSlots, &b.blockOrder, &b.sortScratch,
+                                 &b.scanScratch, &b.counters, &b.sums})
       array(*a);
     mix(b.entryCapacity);
     mix(static_cast<uint64_t>(b.width) << 32 | static_cast<uint32_t>(b.height));
@@ -1764,6 +1775,8 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
       {"adjoint", &b.adjoint, pixels * 16},
       {"fields", &b.fields, pixels * 36},
       {"tile_totals", &b.tileTotals, static_cast<uint64_t>(b.lossTiles) * 8},
+      {"block_work", &b.blockWork, static_cast<uint64_t>(raster_blocks(b.tiles, b.tileW, b.tileH)) * 4},
+      {"block_order", &b.blockOrder, static_cast<uint64_t>(raster_blocks(b.tiles, b.tileW, b.tileH)) * 4},
       {"raw", fixedPoint_ ? &b.rawFixed : &b.rawFloat, n * kRawPerSplat * (fixedPoint_ ? 8u : 4u)},
       {"counters", &b.counters, kCounterWords * 4},
       {"state_positions", &state_[0], n * 12},

@@ -101,8 +101,9 @@ struct ViewBuffers {
   DeviceArray splats, keys, values, tmpKeys, tmpValues, coverage, tileCount, tileRecords, bigList, bigBits, rawFixed, rawFloat;
   // Per tile entry.
   DeviceArray entryKeys, entryValues, tmpEntryKeys, tmpEntryValues;
-  // Per pixel and per tile.
-  DeviceArray image, stops, adjoint, fields, ranges, tileTotals;
+  // Per pixel and per tile; per 8x8 block, the forward's record (work, place,
+  // and the slots' counts) and the backward's launch order.
+  DeviceArray image, stops, adjoint, fields, ranges, tileTotals, blockWork, blockPlace, blockSlots, blockOrder;
   // Exposure and scoring (used only when set): the exposed image, the target's
   // edge weights, the exposure backward's workgroup totals, the worst error.
   DeviceArray exposed, edge, exposureTotals, errorMax;
