@@ -75,13 +75,21 @@ def _heavy_first(work):
     return 255 - (8 * e + fraction)
 
 
+def _check_order(order, keys):
+    """A permutation of the launch indices that keeps each on its XCD (index
+    mod 8) and runs each XCD's by non-decreasing key."""
+    items = len(keys)
+    assert len(order) == items
+    assert sorted(order.tolist()) == list(range(items))
+    assert (order % 8 == np.arange(items) % 8).all()
+    assert (np.diff(keys[order].reshape(-1, 8), axis=0) >= 0).all()
+
+
 @pytest.mark.parametrize("shape", [vsplat.TILES_16X16, vsplat.TILES_8X8, vsplat.TILES_32X8, vsplat.TILES_32X32])
 def test_backward_runs_each_xcds_blocks_heaviest_first(shape):
-    """The backward's launch order: a permutation of the raster launch indices
-    that keeps each index on its XCD (index mod 8) and runs each XCD's blocks
-    by non-increasing work, the work of a block being the entries from its
-    pixels' lowest stop to its tile list's end, recomputed here from the
-    ranges and stops."""
+    """The backward's launch order runs each XCD's blocks by non-increasing
+    work, the work of a block being the entries from its pixels' lowest stop
+    to its tile list's end, recomputed here from the ranges and stops."""
     w = F.config(1, count=20_000)
     cam = w.cameras[0]
     tw, th = SHAPES.get(shape, (16, 16))
@@ -92,7 +100,7 @@ def test_backward_runs_each_xcds_blocks_heaviest_first(shape):
     grid_w, grid_h = (cam.width + tw - 1) // tw, (cam.height + th - 1) // th
     tiles, across, per_tile = grid_w * grid_h, tw // 8, (tw // 8) * (th // 8)
     items = (tiles + 7) // 8 * 8 * per_tile
-    assert len(work) == items and len(order) == items
+    assert len(work) == items
     expected = np.zeros(items, np.int64)
     for h in range(items):
         u = h // 8
@@ -103,10 +111,7 @@ def test_backward_runs_each_xcds_blocks_heaviest_first(shape):
         end = int(ranges[tile][1])
         expected[h] = end - min(int(stops[y0:y0 + 8, x0:x0 + 8].min()), end)
     np.testing.assert_array_equal(work, expected)
-    assert sorted(order.tolist()) == list(range(items))
-    assert (order % 8 == np.arange(items) % 8).all()
-    keys = _heavy_first(work[order]).reshape(-1, 8)
-    assert (np.diff(keys, axis=0) >= 0).all()
+    _check_order(order, _heavy_first(work))
     print(f"{tw}x{th}: {items} blocks, heaviest {work.max()} entries, median {int(np.median(work))}", flush=True)
 
 def _check_lists(cloud, cam, got, stats, shape):
@@ -178,7 +183,8 @@ def test_tuner_leaves_the_entry_buffers_fit_for_its_choice():
     """The tuner tries 8x8 tiles, which list several times the entries of the
     others; after it, the tile-entry buffers are at most twice what the chosen
     shape's entries take by the growth rule (the sort's grids are sized by
-    them), and the trainer's memory is that of a trainer given the shape."""
+    them), and the trainer's memory is that of a trainer given the shape
+    (whichever it chose)."""
     cloud, cam, target = _case("config1_small")
     named = {(16, 16): vsplat.TILES_16X16, (8, 8): vsplat.TILES_8X8, (32, 8): vsplat.TILES_32X8, (32, 32): vsplat.TILES_32X32}
     with vsplat.Trainer(cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=vsplat.TILES_AUTO)) as b, \
@@ -193,7 +199,8 @@ def test_tuner_leaves_the_entry_buffers_fit_for_its_choice():
             eight = e.compute_gradients(cam, target).tile_entries
     wanted = stats.tile_entries + stats.tile_entries // 2 + 1024
     print(f"chose {shape}: {stats.tile_entries} entries (8x8: {eight}); tuned {tuned} B, given {given} B", flush=True)
-    assert eight > 2 * stats.tile_entries
+    if tuple(shape) != (8, 8):  # the tuner may pick 8x8 itself on this small cloud
+        assert eight > 2 * stats.tile_entries
     assert tuned <= given + 2 * wanted * vsplat.BYTES_PER_TILE_ENTRY
 
 

@@ -167,37 +167,28 @@ void launch_forward_views(hipStream_t stream, const RasterViews &views, uint32_t
 void launch_backward_views(hipStream_t stream, bool fixedPoint, bool recompute, const RasterViews &views, uint32_t tileW, uint32_t tileH,
                            const float background[3]);
 
-// The forward's record of its 8x8 blocks for the backward's launch order, per
-// raster launch index: the entries the backward will walk (work), and the
-// block's place among those of its (XCD, key) slot (place); slots holds the
-// slots' counts (kOrderSlots words, then a ticket), zero between a forward and
-// the launch_block_order after it. All null: nothing recorded.
-constexpr uint32_t kOrderKeys = 256, kOrderSlots = kRasterXcds * kOrderKeys;
-struct BlockOrder {
-  uint32_t *work = nullptr, *place = nullptr, *slots = nullptr;
-};
-
 // The raster kernels for one tile shape: 8x8, 16x16, 32x8 or 32x32 (tileW x tileH).
 // The lists are read as launch_bin_emit left them: entryKeys carry each
 // entry's tile (low tileBits bits) and blocks, entryValues its Gaussian.
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryKeys, const uint32_t *entryValues, uint32_t tileBits,
                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
-                    float4 *image, uint32_t *stops, const BlockOrder &order = BlockOrder{});
-// (stops may be null: not kept when the backward recomputes them. order, when
-// given, records the blocks for launch_block_order: raster_blocks(tiles) of them.)
+                    float4 *image, uint32_t *stops, uint32_t *work = nullptr);
+// (stops may be null: not kept when the backward recomputes them. work, when
+// given, receives per launch index the entries the backward will walk for
+// that 8x8 block, raster_blocks(tiles) of them.)
 // recompute: the transmittance and stops are recomputed per tile rather than
 // read from the forward pass (image and stops may then be null). edge: a
 // scored view's edge weights (the adjoint's fourth value its marks), which
 // fill the last two raw sums; null otherwise. order: the blocks' launch order
-// from launch_block_order, or null for the forward's.
+// from launch_work_order, or null for raster_blocks' own.
 void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const Splat *splats, const uint32_t *entryKeys,
                      const uint32_t *entryValues, uint32_t tileBits, const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW,
                      uint32_t tileH, const float background[3], const float4 *image, const float4 *adjoint, const uint32_t *stops,
                      float fixedScale, int64_t *rawFixed, float *rawFloat, const float *edge = nullptr, const uint32_t *order = nullptr);
-// The backward's blocks heaviest first by the forward's work, each kept on its
-// XCD: launchOrder becomes a permutation of the items = raster_blocks(tiles)
-// launch indices; the slots' counts are left zero.
-void launch_block_order(hipStream_t stream, const BlockOrder &order, uint32_t items, uint32_t *launchOrder);
+// The backward's launch order, heaviest first on each XCD by the forward's work
+// (each block keeps its XCD): a permutation of the items = raster_blocks(tiles)
+// launch indices.
+void launch_work_order(hipStream_t stream, const uint32_t *work, uint32_t items, uint32_t *order);
 
 // errorMax and edge: when scoring, the view's worst error (cleared before)
 // and the target's edge weights, and the adjoint's fourth value the pixel's

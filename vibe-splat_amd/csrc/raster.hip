@@ -71,17 +71,12 @@ __device__ __forceinline__ BlockGeometry block_geometry(uint32_t h, uint32_t gri
 // Heaviest first: a block's work (entries) as one of kOrderKeys keys, 8 per
 // octave (the exponent and the next 3 bits of work + 1), the largest work the
 // smallest key.
+constexpr uint32_t kOrderKeys = 256;
 __device__ __forceinline__ uint32_t heavy_first(uint32_t work) {
   const uint32_t v = work < 0xFFFFFFFFu ? work + 1u : work;
   const uint32_t e = 31u - static_cast<uint32_t>(__builtin_clz(v));
   const uint32_t fraction = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
   return kOrderKeys - 1u - (e * 8u + fraction);
-}
-
-// One lane of the forward's block: its work, and its place in its (XCD, key) slot.
-__device__ __forceinline__ void publish_work(const BlockOrder &order, uint32_t item, uint32_t work) {
-  order.work[item] = work;
-  order.place[item] = atomicAdd(&order.slots[(item % kRasterXcds) * kOrderKeys + heavy_first(work)], 1u);
 }
 
 // The block's pixels a splat can touch: inside the extent box and within the
@@ -251,11 +246,11 @@ __device__ __forceinline__ void forward_block(uint32_t item, const Splat *__rest
                                               const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                               const uint2 *__restrict__ ranges, uint32_t width, uint32_t height, uint32_t gridW,
                                               float bgR, float bgG, float bgB, float4 *__restrict__ image, uint32_t *__restrict__ stops,
-                                              const BlockOrder &order, Staged *batch, uint32_t *queue) {
+                                              uint32_t *__restrict__ work, Staged *batch, uint32_t *queue) {
   uint32_t tile;
   const BlockGeometry geo = block_geometry<TW, TH>(item, gridW, width, height, tile);
   if (geo.x0 >= width || geo.y0 >= height) { // a block past the image's edge (uniform)
-    if (order.work != nullptr && threadIdx.x == 0) publish_work(order, item, 0u);
+    if (work != nullptr && threadIdx.x == 0) work[item] = 0;
     return;
   }
   const uint32_t lx = threadIdx.x & 7u, ly = threadIdx.x >> 3;
@@ -272,7 +267,7 @@ __device__ __forceinline__ void forward_block(uint32_t item, const Splat *__rest
     image[at] = make_float4(fma32(T, bgR, r), fma32(T, bgG, g), fma32(T, bgB, b), logT);
     if (stops != nullptr) stops[at] = stop; // none when the backward recomputes them
   }
-  if (order.work != nullptr) {
+  if (work != nullptr) {
     // The backward's walk over this block: the entries from its lowest stop to the list's end.
     uint32_t low = inside ? stop : range.y;
 #pragma unroll
@@ -280,7 +275,7 @@ __device__ __forceinline__ void forward_block(uint32_t item, const Splat *__rest
       const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(low), static_cast<int>(d), 64));
       low = other < low ? other : low;
     }
-    if (threadIdx.x == 0) publish_work(order, item, range.y - low);
+    if (threadIdx.x == 0) work[item] = range.y - low;
   }
 }
 
@@ -289,11 +284,11 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict
                                                          const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
                                                          uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
-                                                         uint32_t *__restrict__ stops, BlockOrder order) {
+                                                         uint32_t *__restrict__ stops, uint32_t *__restrict__ work) {
   __shared__ Staged batch[kBlock];
   __shared__ uint32_t queue[kBlock];
   forward_block<TW, TH>(blockIdx.x, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR, bgG, bgB, image, stops,
-                        order, batch, queue);
+                        work, batch, queue);
 }
 
 // The view of a batched launch that block `item` belongs to, and the block's
@@ -314,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void forward_views_kernel(RasterViews views
   uint32_t item = blockIdx.x;
   const RasterView &w = views.view[view_of(views, item)];
   forward_block<TW, TH>(item, w.splats, w.entryKeys, w.entryValues, w.tileBits, w.ranges, w.width, w.height, w.gridW, bgR, bgG, bgB,
-                        w.image, w.stops, BlockOrder{}, batch, queue);
+                        w.image, w.stops, nullptr, batch, queue);
 }
 
 // The fixed-point quantisation of a wave total (train_backward_fixed.comp:19-21),
@@ -594,61 +589,72 @@ __global__ __launch_bounds__(kBlock) void backward_views_kernel(RasterViews view
                                                   batchIndex, queue);
 }
 
-constexpr uint32_t kOrderThreads = 256;
-static_assert(kOrderSlots == 8 * kOrderThreads, "eight slots a thread in the scan");
+constexpr uint32_t kOrderThreads = 1024;
+// Keys kept in LDS between the two passes, up to this many items per XCD
+// (a 2880x2880 view of 8x8 blocks); past it the second pass reads the work again.
+constexpr uint32_t kOrderStaged = 16384;
 
-// The backward's launch order: launch index h' = 8 j + x runs on XCD x, as
-// launch index h = 8 j + x would (every item keeps its XCD, so a tile's blocks
-// still share one L2), and takes the j-th heaviest of XCD x's items. The
-// forward counted its blocks per (XCD, key) slot and gave each a place in its
-// slot (publish_work); each workgroup here scans the 2048 counts and puts its
-// launch indices at their slot's start plus their place. Places within a slot
-// follow the forward's finishing order, which changes nothing: each item's
-// sums are added with atomics. The last workgroup to read the counts clears
-// them (and the ticket) for the next forward. `items` is a multiple of 8.
-__global__ __launch_bounds__(kOrderThreads) void block_order_kernel(BlockOrder order, uint32_t items, uint32_t *__restrict__ launchOrder) {
-  __shared__ uint32_t start[kOrderSlots];
+// Adds each lane's item to its key's count, one atomic per distinct key of the
+// wave (most blocks share a few small keys), and returns the lane's place
+// among its key's items.
+__device__ __forceinline__ uint32_t key_place(uint32_t *counts, uint32_t key, bool valid) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t same = __ballot(valid);
+#pragma unroll
+  for (uint32_t b = 0; b < 8; ++b) { // key < kOrderKeys
+    const uint64_t ones = __ballot((key >> b) & 1u);
+    same &= ((key >> b) & 1u) ? ones : ~ones;
+  }
+  const uint32_t lower = static_cast<uint32_t>(__popcll(same & ((uint64_t{1} << lane) - 1u)));
+  uint32_t base = 0;
+  if (valid && lower == 0) base = atomicAdd(&counts[key], static_cast<uint32_t>(__popcll(same)));
+  const int leader = valid ? __builtin_ctzll(same) : static_cast<int>(lane);
+  return static_cast<uint32_t>(__shfl(static_cast<int>(base), leader, 64)) + lower;
+}
+
+// The backward's launch order, heaviest first: one workgroup per XCD x sorts
+// the launch indices h = 8 i + x of XCD x by the forward's work (a counting
+// sort over kOrderKeys keys in LDS) and gives the j-th to launch index 8 j + x,
+// which runs on XCD x too (so a tile's blocks still share one L2). Items of one
+// key land in any order, which changes nothing: each block's sums are added
+// with atomics. `items` is a multiple of 8 (raster_blocks).
+__global__ __launch_bounds__(kOrderThreads) void work_order_kernel(const uint32_t *__restrict__ work, uint32_t items,
+                                                                    uint32_t *__restrict__ order) {
+  __shared__ uint32_t counts[kOrderKeys];
   __shared__ uint32_t waveTotals[kOrderThreads / 64];
-  __shared__ uint32_t lastGroup;
-  uint32_t count[8], sum = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < 8; ++k) {
-    count[k] = order.slots[threadIdx.x * 8 + k];
-    sum += count[k];
-  }
-  uint32_t before;
-  block_exclusive_sum(sum, before, waveTotals);
-  // XCD x's items start at x * items / 8 of the flattened (XCD, key) order.
-  const uint32_t share = items / kRasterXcds;
-  before -= (threadIdx.x * 8 / kOrderKeys) * share;
-#pragma unroll
-  for (uint32_t k = 0; k < 8; ++k) {
-    start[threadIdx.x * 8 + k] = before;
-    before += count[k];
-  }
-  __syncthreads(); // the counts are read and the starts in LDS
-  if (threadIdx.x == 0) {
-    __threadfence();
-    lastGroup = atomicAdd(&order.slots[kOrderSlots], 1u) == gridDim.x - 1 ? 1u : 0u;
-  }
-  const uint32_t h = blockIdx.x * kOrderThreads + threadIdx.x;
-  if (h < items) {
-    const uint32_t x = h % kRasterXcds;
-    const uint32_t j = start[x * kOrderKeys + heavy_first(order.work[h])] + order.place[h];
-    if (j < share) launchOrder[j * kRasterXcds + x] = h; // always, when the forward ran over the same items
+  __shared__ uint8_t staged[kOrderStaged];
+  const uint32_t x = blockIdx.x, n = items / kRasterXcds;
+  for (uint32_t k = threadIdx.x; k < kOrderKeys; k += kOrderThreads) counts[k] = 0;
+  __syncthreads();
+  // Every lane runs every trip, so that the ballots see whole waves.
+  for (uint32_t start = 0; start < n; start += kOrderThreads) {
+    const uint32_t i = start + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = valid ? heavy_first(work[i * kRasterXcds + x]) : 0u;
+    if (valid && i < kOrderStaged) staged[i] = static_cast<uint8_t>(key);
+    (void)key_place(counts, key, valid);
   }
   __syncthreads();
-  if (lastGroup != 0) {
-    for (uint32_t k = threadIdx.x; k <= kOrderSlots; k += kOrderThreads) order.slots[k] = 0;
+  const uint32_t c = threadIdx.x < kOrderKeys ? counts[threadIdx.x] : 0u;
+  uint32_t before;
+  block_exclusive_sum(c, before, waveTotals);
+  if (threadIdx.x < kOrderKeys) counts[threadIdx.x] = before;
+  __syncthreads();
+  for (uint32_t start = 0; start < n; start += kOrderThreads) {
+    const uint32_t i = start + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = !valid ? 0u : i < kOrderStaged ? staged[i] : heavy_first(work[i * kRasterXcds + x]);
+    const uint32_t j = key_place(counts, key, valid);
+    if (valid) order[j * kRasterXcds + x] = i * kRasterXcds + x;
   }
 }
 } // namespace
 
-void launch_block_order(hipStream_t stream, const BlockOrder &order, uint32_t items, uint32_t *launchOrder) {
+void launch_work_order(hipStream_t stream, const uint32_t *work, uint32_t items, uint32_t *order) {
   if (items == 0) return;
-  hipLaunchKernelGGL(block_order_kernel, dim3((items + kOrderThreads - 1) / kOrderThreads), dim3(kOrderThreads), 0, stream, order, items,
-                     launchOrder);
+  hipLaunchKernelGGL(work_order_kernel, dim3(kRasterXcds), dim3(kOrderThreads), 0, stream, work, items, order);
 }
+
 
 // Calls LAUNCH(TW, TH) with the tile shape as compile-time constants.
 #define VSPLAT_TILE_DISPATCH(tileW, tileH, LAUNCH)                                                                                 \
@@ -664,11 +670,11 @@ void launch_block_order(hipStream_t stream, const BlockOrder &order, uint32_t it
 
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryKeys, const uint32_t *entryValues, uint32_t tileBits,
                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
-                    float4 *image, uint32_t *stops, const BlockOrder &order) {
+                    float4 *image, uint32_t *stops, uint32_t *work) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_FORWARD(W, H)                                                                                                       \
   hipLaunchKernelGGL((forward_kernel<W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats, entryKeys,   \
-                     entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, stops, order)
+                     entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, stops, work)
   VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD)
 #undef VSPLAT_FORWARD
 }

@@ -334,11 +334,7 @@ void Trainer::ensure_layout(ViewBuffers &b, int width, int height, uint32_t tile
   b.tileTotals.ensure(static_cast<size_t>(b.lossTiles) * 8);
   const size_t blocks = raster_blocks(b.tiles, tileW, tileH);
   b.blockWork.ensure(blocks * 4);
-  b.blockPlace.ensure(blocks * 4);
   b.blockOrder.ensure(blocks * 4);
-  // The slots' counts and ticket are zero between passes (launch_block_order clears them).
-  b.blockSlots.ensure((kOrderSlots + 1) * 4);
-  launch_fill_u32(b.stream, b.blockSlots.as<uint32_t>(), kOrderSlots + 1, 0u);
 }
 
 // The tile grid of a view size and tile shape (no buffers).
@@ -662,13 +658,13 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   if (timed) begin_timing(true);
   const ViewParams v = with_tiles(b, view);
   enqueue_lists(b, v, mode, true);
+  // The backward's 8x8 blocks launch heaviest first on their XCD, by the work
+  // the forward found for each.
   const bool backward = mode != kModeRender;
-  BlockOrder order;
-  if (backward) order = BlockOrder{b.blockWork.as<uint32_t>(), b.blockPlace.as<uint32_t>(), b.blockSlots.as<uint32_t>()};
   launch_forward(b.stream, b.splats.as<Splat>(), b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tileBits, b.ranges.as<uint2>(),
                  v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
-                 options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>(), order);
-  if (backward) launch_block_order(b.stream, order, raster_blocks(b.tiles, b.tileW, b.tileH), b.blockOrder.as<uint32_t>());
+                 options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>(), backward ? b.blockWork.as<uint32_t>() : nullptr);
+  if (backward) launch_work_order(b.stream, b.blockWork.as<uint32_t>(), raster_blocks(b.tiles, b.tileW, b.tileH), b.blockOrder.as<uint32_t>());
   mark("composite");
   if (!backward) return;
   enqueue_loss(b, v, deviceTarget);
@@ -1390,7 +1386,7 @@ uint64_t Trainer::capture_key(uint32_t n) const {
     const ViewBuffers &b = *views_[k];
     for (const DeviceArray *a : {&b.splats, &b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount, &b.tileRecords,
                                  &b.bigList, &b.bigBits, &b.rawFixed, &b.rawFloat, &b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues, &b.image,
-                                 &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.blockWork, &b.blockPlace, &b.blockSlots, &b.blockOrder, &b.sortScratch,
+                                 &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.blockWork, &b.blockOrder, &b.sortScratch,
                                  &b.scanScratch, &b.counters, &b.sums})
       array(*a);
     mix(b.entryCapacity);

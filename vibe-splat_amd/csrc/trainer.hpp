@@ -101,9 +101,9 @@ struct ViewBuffers {
   DeviceArray splats, keys, values, tmpKeys, tmpValues, coverage, tileCount, tileRecords, bigList, bigBits, rawFixed, rawFloat;
   // Per tile entry.
   DeviceArray entryKeys, entryValues, tmpEntryKeys, tmpEntryValues;
-  // Per pixel and per tile; per 8x8 block, the forward's record (work, place,
-  // and the slots' counts) and the backward's launch order.
-  DeviceArray image, stops, adjoint, fields, ranges, tileTotals, blockWork, blockPlace, blockSlots, blockOrder;
+  // Per pixel and per tile; per 8x8 block (raster launch index), the backward's
+  // work found by the forward and the backward's launch order.
+  DeviceArray image, stops, adjoint, fields, ranges, tileTotals, blockWork, blockOrder;
   // Exposure and scoring (used only when set): the exposed image, the target's
   // edge weights, the exposure backward's workgroup totals, the worst error.
   DeviceArray exposed, edge, exposureTotals, errorMax;
