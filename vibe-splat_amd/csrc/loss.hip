@@ -18,7 +18,20 @@ namespace {
 
 __constant__ float kWeights[11] = {0.00102838f, 0.00759876f, 0.03600077f, 0.10936069f, 0.21300554f, 0.26601172f,
                                    0.21300554f, 0.10936069f, 0.03600077f, 0.00759876f, 0.00102838f};
+// The same weights as compile-time constants, for the windows that lie inside
+// the image: their loops unroll with the weights as immediates and the taps'
+// loads at fixed offsets, adding in the same order as the general loops.
+constexpr float kTaps[11] = {0.00102838f, 0.00759876f, 0.03600077f, 0.10936069f, 0.21300554f, 0.26601172f,
+                             0.21300554f, 0.10936069f, 0.03600077f, 0.00759876f, 0.00102838f};
 constexpr int kHalo = 5;
+
+// The sum of the 11 weights, added in tap order (window_norm of a window inside the image).
+__device__ __forceinline__ float full_norm() {
+  float norm = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 11; ++k) norm = norm + kTaps[k];
+  return norm;
+}
 constexpr int kRows = 16 + 2 * kHalo;
 constexpr int kHalfPixels = kRows * 16;
 
@@ -58,13 +71,8 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
 #pragma unroll
     for (int q = 0; q < 15; ++q) acc[q] = 0.0f;
     if (y >= 0 && y < H && x < W) {
-      float norm = 0.0f;
-      const int u0 = x - 5 > 0 ? x - 5 : 0, u1 = x + 5 < W - 1 ? x + 5 : W - 1;
-      for (int u = u0; u <= u1; ++u) {
-        const float w = kWeights[u - x + 5];
-        const size_t at = static_cast<size_t>(y) * W + u;
-        const float4 a4 = image[at];
-        const uchar4 b4 = target[at];
+      // One tap's five sums per channel: mean, squares and product.
+      const auto tap = [&acc](float w, const float4 a4, const uchar4 b4) {
         const float a[3] = {a4.x, a4.y, a4.z};
         const float b[3] = {unorm8(b4.x), unorm8(b4.y), unorm8(b4.z)};
 #pragma unroll
@@ -76,7 +84,21 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
           acc[9 + ch] = fma32(wb, b[ch], acc[9 + ch]);
           acc[12 + ch] = fma32(wa, b[ch], acc[12 + ch]);
         }
-        norm = norm + w;
+      };
+      float norm = 0.0f;
+      if (x >= 5 && x + 5 < W) {
+        const size_t row = static_cast<size_t>(y) * W + (x - 5);
+#pragma unroll
+        for (int k = 0; k < 11; ++k) tap(kTaps[k], image[row + k], target[row + k]);
+        norm = full_norm();
+      } else {
+        const int u0 = x - 5 > 0 ? x - 5 : 0, u1 = x + 5 < W - 1 ? x + 5 : W - 1;
+        for (int u = u0; u <= u1; ++u) {
+          const float w = kWeights[u - x + 5];
+          const size_t at = static_cast<size_t>(y) * W + u;
+          tap(w, image[at], target[at]);
+          norm = norm + w;
+        }
       }
       const float inorm = 1.0f / norm;
 #pragma unroll
@@ -95,12 +117,20 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
     float acc[15];
 #pragma unroll
     for (int q = 0; q < 15; ++q) acc[q] = 0.0f;
-    const int v0 = py - 5 > 0 ? py - 5 : 0, v1 = py + 5 < H - 1 ? py + 5 : H - 1;
-    for (int v = v0; v <= v1; ++v) {
-      const float w = kWeights[v - py + 5];
-      const int idx = (v - y0 + kHalo) * 16 + lx;
+    if (py >= 5 && py + 5 < H) {
+      const int idx = (py - y0) * 16 + lx; // the window's top row (v = py - 5) in the stage
 #pragma unroll
-      for (int q = 0; q < 15; ++q) acc[q] = fma32(w, h[q][idx], acc[q]);
+      for (int k = 0; k < 11; ++k)
+#pragma unroll
+        for (int q = 0; q < 15; ++q) acc[q] = fma32(kTaps[k], h[q][idx + k * 16], acc[q]);
+    } else {
+      const int v0 = py - 5 > 0 ? py - 5 : 0, v1 = py + 5 < H - 1 ? py + 5 : H - 1;
+      for (int v = v0; v <= v1; ++v) {
+        const float w = kWeights[v - py + 5];
+        const int idx = (v - y0 + kHalo) * 16 + lx;
+#pragma unroll
+        for (int q = 0; q < 15; ++q) acc[q] = fma32(w, h[q][idx], acc[q]);
+      }
     }
     const float inorm = 1.0f / (rowNorm * window_norm(px, W));
     const size_t at = static_cast<size_t>(py) * W + px;
@@ -171,12 +201,20 @@ __global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ im
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
     if (y >= 0 && y < H && x < W) {
-      const int u0 = x - 5 > 0 ? x - 5 : 0, u1 = x + 5 < W - 1 ? x + 5 : W - 1;
-      for (int u = u0; u <= u1; ++u) {
-        const float w = kWeights[u - x + 5];
-        const float *f = fields + (static_cast<size_t>(y) * W + u) * 9;
+      if (x >= 5 && x + 5 < W) {
+        const float *f = fields + (static_cast<size_t>(y) * W + (x - 5)) * 9;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] = fma32(w, f[q], acc[q]);
+        for (int k = 0; k < 11; ++k)
+#pragma unroll
+          for (int q = 0; q < 9; ++q) acc[q] = fma32(kTaps[k], f[k * 9 + q], acc[q]);
+      } else {
+        const int u0 = x - 5 > 0 ? x - 5 : 0, u1 = x + 5 < W - 1 ? x + 5 : W - 1;
+        for (int u = u0; u <= u1; ++u) {
+          const float w = kWeights[u - x + 5];
+          const float *f = fields + (static_cast<size_t>(y) * W + u) * 9;
+#pragma unroll
+          for (int q = 0; q < 9; ++q) acc[q] = fma32(w, f[q], acc[q]);
+        }
       }
     }
 #pragma unroll
@@ -191,12 +229,20 @@ __global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ im
     float acc[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
-    const int v0 = py - 5 > 0 ? py - 5 : 0, v1 = py + 5 < H - 1 ? py + 5 : H - 1;
-    for (int v = v0; v <= v1; ++v) {
-      const float w = kWeights[v - py + 5];
-      const int idx = (v - y0 + kHalo) * 16 + lx;
+    if (py >= 5 && py + 5 < H) {
+      const int idx = (py - y0) * 16 + lx; // the window's top row (v = py - 5) in the stage
 #pragma unroll
-      for (int q = 0; q < 9; ++q) acc[q] = fma32(w, h[q][idx], acc[q]);
+      for (int k = 0; k < 11; ++k)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] = fma32(kTaps[k], h[q][idx + k * 16], acc[q]);
+    } else {
+      const int v0 = py - 5 > 0 ? py - 5 : 0, v1 = py + 5 < H - 1 ? py + 5 : H - 1;
+      for (int v = v0; v <= v1; ++v) {
+        const float w = kWeights[v - py + 5];
+        const int idx = (v - y0 + kHalo) * 16 + lx;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] = fma32(w, h[q][idx], acc[q]);
+      }
     }
     const size_t at = static_cast<size_t>(py) * W + px;
     const float perValue = 1.0f / (3.0f * static_cast<float>(W) * static_cast<float>(H));
