@@ -36,7 +36,7 @@ using std::fma;
 constexpr std::uint32_t kTile = 16;
 constexpr std::uint32_t kInvisible = 0xFFFFFFFFu;
 constexpr float kMinAlpha = 1.0f / 255.0f;            // train_types.glsl:92-93
-constexpr float kLogMinTransmittance = -9.210340372f; // train_types.glsl:11
+constexpr float kMinTransmittance = 1e-4f; // train_types.glsl:11 holds its log
 constexpr int kRaw = 14;                              // train_types.glsl:15
 
 // The splat record, 12 floats: conic xx xy yy, opacity, rgb, radius, centre xy, extent xy
@@ -618,6 +618,9 @@ extern "C" int ob_tile_ranges(const uint32_t *entryKeys, uint64_t entries, uint3
 }
 
 // train_forward.comp:38-100, pixel by pixel (a pixel's result does not depend on the batching).
+// A pixel stops where the product of 1 - alpha would fall under 1e-4 (the
+// shader tests the sum of the logs against log 1e-4, the same bound), and
+// its log transmittance is the log of that product.
 extern "C" int ob_forward(const float *splats, const uint32_t *entryValues, const uint32_t *ranges, uint32_t width, uint32_t height,
                           const float *background, float *image, uint32_t *stops) {
   const std::uint32_t gridW = (width + kTile - 1) / kTile, gridH = (height + kTile - 1) / kTile;
@@ -626,7 +629,7 @@ extern "C" int ob_forward(const float *splats, const uint32_t *entryValues, cons
     const std::uint32_t begin = ranges[tile * 2], end = ranges[tile * 2 + 1];
     for (std::uint32_t y = g.y0; y < g.y1; ++y) {
       for (std::uint32_t x = g.x0; x < g.x1; ++x) {
-        float rgb[3] = {0.0f, 0.0f, 0.0f}, T = 1.0f, logT = 0.0f;
+        float rgb[3] = {0.0f, 0.0f, 0.0f}, T = 1.0f;
         std::uint32_t stop = begin;
         for (std::uint32_t e = end; e-- > begin;) {
           const Splat s = load_splat(splats, entryValues[e]);
@@ -635,8 +638,8 @@ extern "C" int ob_forward(const float *splats, const uint32_t *entryValues, cons
           if (x < box[0] || x >= box[2] || y < box[1] || y >= box[3]) continue;
           float dx, dy, power, ep, alpha;
           if (!splat_at(s, x, y, dx, dy, power, ep, alpha)) continue;
-          const float logRest = flog(1.0f - alpha);
-          if (logT + logRest < kLogMinTransmittance) {
+          const float Tnext = T * (1.0f - alpha);
+          if (Tnext < kMinTransmittance) {
             stop = e + 1;
             break;
           }
@@ -644,9 +647,9 @@ extern "C" int ob_forward(const float *splats, const uint32_t *entryValues, cons
           rgb[0] = fma(w, s.r, rgb[0]);
           rgb[1] = fma(w, s.g, rgb[1]);
           rgb[2] = fma(w, s.b, rgb[2]);
-          T = T * (1.0f - alpha);
-          logT = logT + logRest;
+          T = Tnext;
         }
+        const float logT = flog(T);
         const std::size_t at = static_cast<std::size_t>(y) * width + x;
         image[at * 4] = fma(T, background[0], rgb[0]);
         image[at * 4 + 1] = fma(T, background[1], rgb[1]);

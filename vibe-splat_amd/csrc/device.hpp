@@ -345,6 +345,6 @@ __device__ __forceinline__ bool in_box(uint32_t box, uint32_t lx, uint32_t ly) {
 }
 
 constexpr float kMinAlpha = 1.0f / 255.0f;
-constexpr float kLogMinTransmittance = -9.210340372f; // log(1e-4), train_types.glsl:11
+constexpr float kMinTransmittance = 1e-4f; // train_types.glsl:11 holds its log, LOG_MIN_TRANSMITTANCE
 
 } // namespace vsplat

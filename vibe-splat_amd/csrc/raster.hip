@@ -186,6 +186,11 @@ __device__ __forceinline__ float power_at(const Staged &s, float dx, float dy) {
 // (with Colour), the transmittance and its log, and the stop index of the
 // calling lane's pixel. The forward pass and the backward pass that
 // recomputes the transmittance both run this, so they agree bit for bit.
+// A pixel stops at the splat that would take its transmittance, the product
+// of 1 - alpha, under 1e-4; train_forward.comp:84-88 asks the same of the sum
+// of the logs (log T < log 1e-4), which costs a log per pair, and rounds
+// further from the exact product than the product does. The log of the
+// final transmittance is taken once per pixel.
 template <bool Colour>
 __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats, const uint32_t *__restrict__ entryKeys,
                                                 const uint32_t *__restrict__ entryValues, uint32_t blockBit, uint2 range,
@@ -219,9 +224,8 @@ __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats
       const bool ok = candidate && front && visible;
       // Without branches: a lane that takes nothing adds a zero weight
       // (fma(0, c, r) is r) and keeps its transmittance by selection.
-      const float logRest = log_normal(1.0f - alpha);
-      const float logNext = logT + logRest;
-      const bool below = logNext < kLogMinTransmittance;
+      const float Tnext = T * (1.0f - alpha);
+      const bool below = Tnext < kMinTransmittance;
       const bool stopHere = ok && below;
       const bool take = ok && !stopHere;
       if constexpr (Colour) {
@@ -229,15 +233,15 @@ __device__ __forceinline__ void composite_block(const Splat *__restrict__ splats
         r = fma32(w, s.r, r);
         g = fma32(w, s.g, g);
         b = fma32(w, s.b, b);
-        T = take ? T * (1.0f - alpha) : T;
       }
-      logT = take ? logNext : logT;
+      T = take ? Tnext : T;
       stop = stopHere ? s.entry + 1u : stop;
       done = done || stopHere;
       doneLanes |= oks & vote(below);
       if (doneLanes == ~0ull) break;
     }
   }
+  logT = log_normal(T); // T in [1e-4, 1]: a normal float
 }
 
 // The forward pass of one 8x8 block (item) of a view.
