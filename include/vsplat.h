@@ -373,7 +373,7 @@ int vsplat_autotune_views(vsplat_trainer *trainer, uint32_t n, const vsplat_came
  * raster launch index, 4 bytes: the entries the backward walks for that 8x8
  * block), "block_order" (the backward's launch order: a permutation of those
  * indices, heaviest first on each XCD), "raw" (14 int64 per
- * Gaussian with fixed point, else 14 float), "counters" (4 words),
+ * Gaussian with fixed point, else 14 float), "counters" (8 words: visible splats, entries, entries listed, big splats, big-splat bit words, overflow, skipped steps, unused),
  * "state_positions", "state_scales", "state_rotations", "state_opacities",
  * "state_sh" (Adam's state, one word per parameter: bf16 m low, bf16 v high),
  * "stats" (8 floats per Gaussian). *bytes

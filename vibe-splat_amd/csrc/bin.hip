@@ -4,7 +4,7 @@
 // counts themselves come from the projection (project.hip).
 #include "device.hpp"
 #include "kernels.hpp"
-#include "lookback.hpp"
+#include "blockscan.hpp"
 #include "tiles.hpp"
 
 namespace vsplat {
@@ -12,97 +12,101 @@ namespace vsplat {
 namespace {
 
 constexpr uint32_t kScanThreads = 256;
-constexpr uint32_t kScanPerThread = 16;
-constexpr uint32_t kScanTile = kScanThreads * kScanPerThread; // 4096 splats per workgroup
+constexpr uint32_t kScanPerThread = 4;
+constexpr uint32_t kScanTile = kScanThreads * kScanPerThread; // 1024 splats per workgroup
 
 // The list offsets: the exclusive sum, in depth order, of the tile counts the
 // projection left in Gaussian order (train_bin_count.comp's counts, scanned as
-// scan_blocks.comp does), in one launch by decoupled look-back
-// (lookback.hpp). The last tile also stores the total and decides whether the
-// entries fit: the later kernels only list and sort what fits. A view that does
-// not fit sets the sticky overflow word; from then on every queued pass lists
-// nothing and its chain rule does nothing, so the passes that ran are exactly
-// those before the first overflow.
+// scan_blocks.comp does), in two launches over chunks of kScanTile splats.
+// coverage_count_kernel gathers each chunk's counts in depth order into the
+// offsets buffer (an invisible splat counts 0) and stores the chunk's total and
+// its visible splats; coverage_offsets_kernel adds up the totals of the chunks
+// before its own and turns its counts into offsets in place. The last chunk
+// also stores the visible count and the entries and decides whether they fit:
+// the later kernels only list and sort what fits. A view that does not fit sets
+// the sticky overflow word; from then on every queued pass lists nothing and
+// its chain rule does nothing, so the passes that ran are exactly those before
+// the first overflow.
 //
-// control: [0] the ticket, [1] the launch number, which tags this launch's
-// status words (one 64-bit word per tile, after the first two 64-bit words),
-// so they never need clearing: each launch rewrites every tile's word, and a
-// word left by the previous launch carries the previous number.
-__global__ __launch_bounds__(kScanThreads) void coverage_scan_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
-                                                                     const uint32_t *__restrict__ tileCount, uint32_t count,
-                                                                     uint32_t *__restrict__ offsets, uint32_t *counters, uint32_t capacity,
-                                                                     uint32_t isStep, uint32_t *control) {
+// totals: per chunk, the entries (totals[c]) and the visible splats
+// (totals[chunks + c]), each written before it is read.
+__global__ __launch_bounds__(kScanThreads) void coverage_count_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ values,
+                                                                      const uint32_t *__restrict__ tileCount, uint32_t count,
+                                                                      uint32_t *__restrict__ offsets, uint32_t *__restrict__ totals) {
   __shared__ uint32_t waveTotals[kScanThreads / 64];
-  __shared__ uint32_t ticketShared, launchShared, earlierShared;
-  uint64_t *status = reinterpret_cast<uint64_t *>(control) + 2;
-  if (threadIdx.x == 0) {
-    launchShared = __hip_atomic_load(&control[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // The launch number is read before the ticket is taken (release), so the
-    // last ticket can advance it for the next launch.
-    ticketShared = __hip_atomic_fetch_add(&control[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  const uint32_t tile = ticketShared;
-  const uint32_t tag = (launchShared & 0x3FFFFFFFu) << 2;
-  const uint32_t first = tile * kScanTile + threadIdx.x * kScanPerThread;
-  uint32_t v[kScanPerThread];
+  const uint32_t chunk = blockIdx.x;
   uint32_t sum = 0, seen = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kScanPerThread; ++k) {
-    const uint32_t r = first + k;
+    const uint32_t r = chunk * kScanTile + k * kScanThreads + threadIdx.x;
     const bool visible = r < count && keys[r] != kInvisible;
-    v[k] = visible ? tileCount[values[r]] : 0u;
-    sum += v[k];
+    const uint32_t v = visible ? tileCount[values[r]] : 0u;
+    if (r < count) offsets[r] = v;
+    sum += v;
     seen += visible ? 1u : 0u;
   }
-  // The visible count, one atomic per workgroup (a same-address atomic per
-  // wave, over a million splats, would serialise).
   uint32_t unused;
-  const uint32_t visibleTotal = block_exclusive_sum(seen, unused, waveTotals);
-  if (threadIdx.x == 0 && visibleTotal != 0) atomicAdd(&counters[kVisible], visibleTotal);
-  uint32_t exclusive;
-  const uint32_t tileTotal = block_exclusive_sum(sum, exclusive, waveTotals);
-  if (threadIdx.x < 64) {
-    const uint32_t lane = threadIdx.x;
-    uint32_t earlier = 0;
-    if (tile == 0) {
-      if (lane == 0) status_publish(status, tag | kTagInclusive, tileTotal);
-    } else {
-      if (lane == 0) status_publish(status + tile, tag | kTagAggregate, tileTotal);
-      earlier = look_back_window(status, tile, tag, lane, &counters[kFault]);
-      if (lane == 0) status_publish(status + tile, tag | kTagInclusive, earlier + tileTotal);
-    }
-    if (lane == 0) earlierShared = earlier;
+  const uint32_t chunkTotal = block_exclusive_sum(sum, unused, waveTotals);
+  const uint32_t chunkSeen = block_exclusive_sum(seen, unused, waveTotals);
+  if (threadIdx.x == 0) {
+    totals[chunk] = chunkTotal;
+    totals[gridDim.x + chunk] = chunkSeen;
   }
-  __syncthreads();
-  uint32_t running = earlierShared + exclusive;
+}
+
+__global__ __launch_bounds__(kScanThreads) void coverage_offsets_kernel(uint32_t count, uint32_t *__restrict__ offsets,
+                                                                        const uint32_t *__restrict__ totals, uint32_t *counters,
+                                                                        uint32_t capacity, uint32_t isStep) {
+  __shared__ uint32_t waveTotals[kScanThreads / 64];
+  const uint32_t chunk = blockIdx.x;
+  const bool last = chunk == gridDim.x - 1;
+  // The entries of the chunks before this one (and, for the last, every
+  // chunk's visible splats).
+  uint32_t earlier = 0, seen = 0;
+  for (uint32_t c = threadIdx.x; c < chunk; c += kScanThreads) earlier += totals[c];
+  if (last) {
+    for (uint32_t c = threadIdx.x; c < gridDim.x; c += kScanThreads) seen += totals[gridDim.x + c];
+  }
+  uint32_t unused;
+  earlier = block_exclusive_sum(earlier, unused, waveTotals);
+  // Each thread its kScanPerThread consecutive splats: their counts, their sum,
+  // the exclusive sum over the chunk's threads, then the running offsets.
+  const uint32_t first = chunk * kScanTile + threadIdx.x * kScanPerThread;
+  uint32_t v[kScanPerThread];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPerThread; ++k) {
+    v[k] = first + k < count ? offsets[first + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t exclusive;
+  const uint32_t chunkTotal = block_exclusive_sum(sum, exclusive, waveTotals);
+  uint32_t running = earlier + exclusive;
 #pragma unroll
   for (uint32_t k = 0; k < kScanPerThread; ++k) {
     if (first + k < count) offsets[first + k] = running;
     running += v[k];
   }
-  if (tile == gridDim.x - 1 && threadIdx.x == 0) {
-    const uint32_t entries = earlierShared + tileTotal;
-    counters[kEntries] = entries;
-    if (counters[kOverflow] != 0 || entries > capacity) {
-      counters[kEntriesUsed] = 0;
-      counters[kOverflow] = 1;
-      counters[kSkipped] += isStep;
-    } else {
-      counters[kEntriesUsed] = entries;
+  if (last) {
+    const uint32_t visible = block_exclusive_sum(seen, unused, waveTotals);
+    if (threadIdx.x == 0) {
+      const uint32_t entries = earlier + chunkTotal;
+      counters[kVisible] = visible;
+      counters[kEntries] = entries;
+      if (counters[kOverflow] != 0 || entries > capacity) {
+        counters[kEntriesUsed] = 0;
+        counters[kOverflow] = 1;
+        counters[kSkipped] += isStep;
+      } else {
+        counters[kEntriesUsed] = entries;
+      }
     }
-    // Every workgroup has its ticket and its launch number: ready the next launch.
-    control[0] = 0;
-    __hip_atomic_store(&control[1], launchShared + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// A batch's views each have their own counters; the first view's gather the
-// sticky words of all (a view that did not fit, a wait that gave up).
-__global__ void merge_flags_kernel(uint32_t *status, const uint32_t *counters) {
-  status[kOverflow] |= counters[kOverflow];
-  status[kFault] |= counters[kFault];
-}
+// A batch's views each have their own counters; the first view's gathers the
+// sticky overflow word of all (a view that did not fit).
+__global__ void merge_flags_kernel(uint32_t *status, const uint32_t *counters) { status[kOverflow] |= counters[kOverflow]; }
 
 // A wave takes 64 consecutive depth-sorted splats and walks their candidate
 // tiles 64 at a time (tiles.hpp), reading each small splat's outcome from the
@@ -277,13 +281,14 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *__rest
 
 } // namespace
 
-size_t coverage_scan_words(uint32_t count) { return 4 + 2 * static_cast<size_t>((count + kScanTile - 1) / kScanTile); }
+size_t coverage_scan_words(uint32_t count) { return 2 * static_cast<size_t>(count == 0 ? 1 : (count + kScanTile - 1) / kScanTile); }
 
 void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
-                          uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *control) {
-  const uint32_t tiles = count == 0 ? 1 : (count + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(coverage_scan_kernel, dim3(tiles), dim3(kScanThreads), 0, stream, keys, values, tileCount, count, offsets,
-                     counters, capacity, isStep ? 1u : 0u, control);
+                          uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *totals) {
+  const uint32_t chunks = count == 0 ? 1 : (count + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(coverage_count_kernel, dim3(chunks), dim3(kScanThreads), 0, stream, keys, values, tileCount, count, offsets, totals);
+  hipLaunchKernelGGL(coverage_offsets_kernel, dim3(chunks), dim3(kScanThreads), 0, stream, count, offsets, totals, counters, capacity,
+                     isStep ? 1u : 0u);
 }
 
 void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters) {

@@ -70,8 +70,7 @@ enum Counter : uint32_t {
   kBigBitWords = 4,  // 64-bit words of the big splats' reached-tile bits handed out so far
   kOverflow = 5,     // sticky: a view needed more entries than the buffers hold; every later pass is skipped until the host clears it
   kSkipped = 6,      // step-mode passes skipped that way since the host last cleared it
-  kFault = 7,        // a kernel gave up a wait that should have ended (see lookback.hpp)
-  kCounterWords = 8,
+  kCounterWords = 8,  // word 7 unused (the readback keeps 8 words, so the sums after them stay 8-byte aligned)
   kClearedPerPass = 5, // the first five are cleared at the start of every pass
 };
 

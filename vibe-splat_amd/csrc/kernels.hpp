@@ -111,11 +111,13 @@ constexpr uint32_t kMaxGatherArrays = 10;
 // new Gaussians g < gaussians, for each of `count` (at most kMaxGatherArrays) arrays.
 void launch_gather(hipStream_t stream, const int32_t *source, uint32_t gaussians, const GatherArray *arrays, uint32_t count);
 
-// Scratch (zeroed once when allocated) for launch_coverage_scan over count splats.
+// Scratch for launch_coverage_scan over count splats (per chunk of splats, its
+// entries and its visible splats; written before it is read).
 size_t coverage_scan_words(uint32_t count);
-// Offsets of each depth-sorted splat's entries, the entry count and the capacity check.
+// Offsets of each depth-sorted splat's entries, the visible and entry counts
+// and the capacity check.
 void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
-                          uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *control);
+                          uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *totals);
 // status[overflow] and status[fault] |= counters' (a batch's views).
 void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters);
 // The tile lists in depth order: the small splats' entries from their bits,

@@ -27,7 +27,7 @@
 // any order) or float.
 #include "device.hpp"
 #include "kernels.hpp"
-#include "lookback.hpp"
+#include "blockscan.hpp"
 
 namespace vsplat {
 

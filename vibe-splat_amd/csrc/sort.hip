@@ -23,7 +23,7 @@
 // device): grids are sized for the capacity and tiles past the count exit.
 #include "device.hpp"
 #include "kernels.hpp"
-#include "lookback.hpp"
+#include "blockscan.hpp"
 
 namespace vsplat {
 

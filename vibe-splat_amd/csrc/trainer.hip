@@ -260,7 +260,6 @@ ViewBuffers &Trainer::view_buffers(size_t k) {
       launch_fill_u32(v->stream, v->rawFloat.as<uint32_t>(), n * kRawPerSplat, 0u);
     }
     v->scanScratch.ensure(coverage_scan_words(static_cast<uint32_t>(n)) * 4);
-    launch_fill_u32(v->stream, v->scanScratch.as<uint32_t>(), v->scanScratch.bytes() / 4, 0u);
     v->sortScratch.ensure(radix_sort_scratch_words(n) * 4);
     v->counters.ensure(kCounterWords * 4);
     launch_fill_u32(v->stream, v->counters.as<uint32_t>(), kCounterWords, 0u);
@@ -882,10 +881,6 @@ vsplat_step_stats Trainer::read_stats(bool &overflow) {
   double sums[2];
   std::memcpy(counters, host, sizeof(counters));
   std::memcpy(sums, host + kCounterWords * 4, sizeof(sums));
-  if (counters[kFault] != 0) {
-    launch_fill_u32(stream_, b.counters.as<uint32_t>() + kFault, 1, 0u);
-    throw Error(VSPLAT_DEVICE_ERROR, "a prefix-sum wait on the GPU timed out; the step's results are not valid");
-  }
   overflow = counters[kOverflow] != 0;
   return make_stats(counters, sums, b.width, b.height);
 }
@@ -1050,9 +1045,9 @@ void Trainer::render(const vsplat_camera &camera, float *out) {
 // render() without allocating (vsplat_render_noalloc). The per-Gaussian,
 // per-entry and per-pixel buffers are carved out of `scratch` at the sizes of
 // the VSPLAT_NOALLOC_MAX_* limits, the same layout on every call; the image is
-// written straight into the caller's buffer. The counters and the tile-count
-// scan's control and status words are cleared first (the scratch keeps the last
-// call's contents); every other carved buffer is written before it is read. The
+// written straight into the caller's buffer. The counters are cleared first
+// (the scratch keeps the last call's contents); every other carved buffer is
+// written before it is read. The
 // trainer's own view buffers, its last view and its statistics are not touched.
 void Trainer::render_into(const vsplat_camera &camera, float *deviceOut, uint64_t capacity, void *scratch, uint64_t scratchBytes) {
   if (count_ > VSPLAT_NOALLOC_MAX_GAUSSIANS) {
@@ -1098,13 +1093,11 @@ void Trainer::render_into(const vsplat_camera &camera, float *deviceOut, uint64_
   b.entryCapacity = entries;
   // Fill kernels rather than memsets, as everywhere in a step.
   launch_fill_u32(stream_, b.counters.as<uint32_t>(), kCounterWords, 0u);
-  launch_fill_u32(stream_, b.scanScratch.as<uint32_t>(), b.scanScratch.bytes() / 4, 0u);
   dropoutNow_ = 0.0f; // as render(): the whole model
   enqueue_passes(b, view_params(camera), nullptr, kModeRender, nullptr, false);
   uint32_t *host = readback_.as<uint32_t>(); // pinned at construction
   check(hipMemcpyAsync(host, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToHost, stream_), "read counters");
   check(hipStreamSynchronize(stream_), "render");
-  if (host[kFault] != 0) throw Error(VSPLAT_DEVICE_ERROR, "a prefix-sum wait on the GPU timed out; the image is not valid");
   if (host[kOverflow] != 0) {
     throw Error(VSPLAT_BUDGET_EXCEEDED, std::to_string(host[kEntries]) + " tile entries: the allocation-free render lists at most " +
                                             std::to_string(VSPLAT_NOALLOC_MAX_ENTRIES));
