@@ -35,6 +35,8 @@ def _case(name):
     if name == "config1_small":  # the 1280x720 workload's distributions at 20k Gaussians
         w = F.config(1, count=20_000)
         return w.cloud, w.cameras[0], w.targets[0]
+    if name == "unseen":  # the camera looks away from every Gaussian: empty lists, the background, untouched Gaussians
+        return F.random_cloud(60, 1, 13), F.look_at((0, 0, -3), (0, 0, -6), (0.0, -1.0, 0.0), 60.0, 72, 56), F.smooth_target(72, 56, 11)
     if name == "heavy_tails":  # the stress workload's distributions at 3000 Gaussians, 640x480: many boxes of more than 64 tiles
         w = F.config(4, count=3000)
         cam = F.look_at((0, 0, -3), (0, 0, 0), (0.0, -1.0, 0.0), 60.0, 640, 480)
@@ -87,10 +89,12 @@ def _run_both(name, fixed=True, lam=0.2, sh_storage=0):
     return cloud, cam, target, model, r, g_model, st_model, t, stats
 
 
-@pytest.mark.parametrize("name", ["ref60", "ref300", "stop400", "deg3", "config0", "config1_small", "heavy_tails"])
+@pytest.mark.parametrize("name", ["ref60", "ref300", "stop400", "deg3", "config0", "config1_small", "heavy_tails", "unseen"])
 def test_pipeline_bit_exact_fixed_point(name):
     cloud, cam, target, model, r, g_model, st_model, t, stats = _run_both(name, fixed=True)
     print(f"{name}: visible {r.visible} entries {r.entries}")
+    if name == "unseen":
+        assert r.visible == 0 and r.entries == 0
     if name == "heavy_tails":  # splats listed by the big-splat kernels are in the case
         s = r.splats[r.values[r.keys != 0xFFFFFFFF]]
         tiles_x = np.floor((s[:, 8] + s[:, 10] + 1) / 16) - np.floor((s[:, 8] - s[:, 10] - 1) / 16) + 1
