@@ -763,15 +763,27 @@ __global__ __launch_bounds__(kChainThreads) void chain_views_kernel(ChainBuffers
   bool statsChanged = false;
   GaussianFrame G;
   gaussian_frame(B, i, G);
-  uint32_t raw[2 * kRawPerSplat];
-  float radius = 0.0f;
-  fetch_raw(views.view[0], i, statistics, raw, radius);
-  for (uint32_t v = 0; v < views.count; ++v) {
+  // Each view's raw sums are loaded kAhead views ahead of the view being worked,
+  // in kAhead register sets that the unrolled loop uses in turn: two where the
+  // kernel stays within 256 VGPRs with them (two waves per SIMD), one for
+  // degree 3 with 16-bit SH, whose larger SH terms leave no room.
+  constexpr uint32_t kAhead = Degree == 3 && Format != kShBf16 ? 1u : 2u;
+  uint32_t raw[kAhead][2 * kRawPerSplat];
+  float radius[kAhead];
+#pragma unroll
+  for (uint32_t k = 0; k < kAhead; ++k) {
+    radius[k] = 0.0f;
+    if (k < views.count) fetch_raw(views.view[k], i, statistics, raw[k], radius[k]);
+  }
+#pragma unroll
+  for (uint32_t v = 0; v < kChainViewsPerLaunch; ++v) {
+    if (v >= views.count) break;
     const ChainView &w = views.view[v];
+    const uint32_t set = v % kAhead;
     float g[kRawPerSplat];
-    const bool touched = convert_fetched(raw, w.rawFixed != nullptr, w.fixedScale, g);
-    const float viewRadius = radius;
-    if (v + 1 < views.count) fetch_raw(views.view[v + 1], i, statistics, raw, radius);
+    const bool touched = convert_fetched(raw[set], w.rawFixed != nullptr, w.fixedScale, g);
+    const float viewRadius = radius[set];
+    if (v + kAhead < views.count) fetch_raw(views.view[v + kAhead], i, statistics, raw[set], radius[set]);
     if (touched && statistics) {
       if (!statsChanged) {
         const float4 *st = reinterpret_cast<const float4 *>(B.stats + static_cast<size_t>(i) * 8);
