@@ -176,13 +176,16 @@ __device__ __forceinline__ bool load_raw(const int64_t *rawFixed, const float *r
 // use (the loads are waited for where the words are first read), and with
 // `statistics` the splat's radius. Sums 2k and 2k + 1 sit in raw[4k .. 4k + 3]
 // as two int64 (low word first), or as two floats in raw[4k] and raw[4k + 1]:
-// the same places for both kinds, so that the registers stay registers.
+// the same places for both kinds, so that the registers stay registers. The
+// last two sums are a scored view's; a view that is not scored leaves them zero
+// and they are not loaded.
+constexpr uint32_t kUnscoredSums = 12;
 __device__ __forceinline__ void fetch_raw(const ChainView &w, uint32_t i, bool statistics, uint32_t raw[2 * kRawPerSplat], float &radius) {
   if (w.rawFixed != nullptr) {
     const uint4 *src = reinterpret_cast<const uint4 *>(w.rawFixed + static_cast<size_t>(i) * kRawPerSplat);
 #pragma unroll
     for (uint32_t k = 0; k < kRawPerSplat / 2; ++k) {
-      const uint4 v = src[k];
+      const uint4 v = 2 * k < kUnscoredSums ? src[k] : make_uint4(0u, 0u, 0u, 0u);
       raw[4 * k] = v.x;
       raw[4 * k + 1] = v.y;
       raw[4 * k + 2] = v.z;
@@ -192,7 +195,7 @@ __device__ __forceinline__ void fetch_raw(const ChainView &w, uint32_t i, bool s
     const uint2 *src = reinterpret_cast<const uint2 *>(w.rawFloat + static_cast<size_t>(i) * kRawPerSplat);
 #pragma unroll
     for (uint32_t k = 0; k < kRawPerSplat / 2; ++k) {
-      const uint2 v = src[k];
+      const uint2 v = 2 * k < kUnscoredSums ? src[k] : make_uint2(0u, 0u);
       raw[4 * k] = v.x;
       raw[4 * k + 1] = v.y;
       raw[4 * k + 2] = 0u;
