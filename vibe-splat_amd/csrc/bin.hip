@@ -104,9 +104,27 @@ __global__ __launch_bounds__(kScanThreads) void coverage_offsets_kernel(uint32_t
   }
 }
 
-// A batch's views each have their own counters; the first view's gathers the
-// sticky overflow word of all (a view that did not fit).
-__global__ void merge_flags_kernel(uint32_t *status, const uint32_t *counters) { status[kOverflow] |= counters[kOverflow]; }
+// A batch's views each have their own counters and loss sums: each view's go
+// to its slot as they are, and the first view's counters gather the sticky
+// overflow word of all (a view that did not fit). One workgroup; the slots are
+// written before the first view's counters change.
+static_assert(kViewStatusBytes == kCounterWords * 4 + 2 * sizeof(double), "a view's slot: its counters, then its two sums");
+__global__ __launch_bounds__(128) void gather_views_kernel(ViewStatus v, uint8_t *__restrict__ slots, uint32_t *status) {
+  constexpr uint32_t kWords = kViewStatusBytes / 4;
+  for (uint32_t k = threadIdx.x; k < v.count * kWords; k += blockDim.x) {
+    const uint32_t view = k / kWords, w = k - view * kWords;
+    uint32_t *slot = reinterpret_cast<uint32_t *>(slots + static_cast<size_t>(v.first + view) * kViewStatusBytes);
+    slot[w] = w < kCounterWords ? v.counters[view][w] : reinterpret_cast<const uint32_t *>(v.sums[view])[w - kCounterWords];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t overflow = 0;
+    for (uint32_t view = 0; view < v.count; ++view) {
+      if (v.first + view != 0) overflow |= v.counters[view][kOverflow];
+    }
+    status[kOverflow] |= overflow;
+  }
+}
 
 // A wave takes 64 consecutive depth-sorted splats and walks their candidate
 // tiles 64 at a time (tiles.hpp), reading each small splat's outcome from the
@@ -291,8 +309,9 @@ void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32
                      isStep ? 1u : 0u);
 }
 
-void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters) {
-  hipLaunchKernelGGL(merge_flags_kernel, dim3(1), dim3(1), 0, stream, status, counters);
+void launch_gather_views(hipStream_t stream, const ViewStatus &views, uint8_t *slots, uint32_t *status) {
+  if (views.count == 0) return;
+  hipLaunchKernelGGL(gather_views_kernel, dim3(1), dim3(128), 0, stream, views, slots, status);
 }
 
 void launch_bin_emit(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const Splat *splats, TileRecord *records,

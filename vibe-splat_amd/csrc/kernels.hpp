@@ -118,8 +118,17 @@ size_t coverage_scan_words(uint32_t count);
 // and the capacity check.
 void launch_coverage_scan(hipStream_t stream, const uint32_t *keys, const uint32_t *values, const uint32_t *tileCount, uint32_t count,
                           uint32_t *offsets, uint32_t *counters, uint32_t capacity, bool isStep, uint32_t *totals);
-// status[overflow] and status[fault] |= counters' (a batch's views).
-void launch_merge_flags(hipStream_t stream, uint32_t *status, const uint32_t *counters);
+// A batch's views' status, up to kStatusViewsPerLaunch views (views first ..
+// first + count - 1) a launch: each view's counters and loss sums copied to its
+// kViewStatusBytes slot of `slots`, then status[overflow] (the first view's
+// counters) |= every other view's.
+constexpr uint32_t kStatusViewsPerLaunch = 8, kViewStatusBytes = 48;
+struct ViewStatus {
+  const uint32_t *counters[kStatusViewsPerLaunch];
+  const double *sums[kStatusViewsPerLaunch];
+  uint32_t first, count;
+};
+void launch_gather_views(hipStream_t stream, const ViewStatus &views, uint8_t *slots, uint32_t *status);
 // The tile lists in depth order: the small splats' entries from their bits,
 // then the big splats' (bigList, with the bits launch_project kept). records is written: a
 // big splat's first entry is kept in its maskLo for the second kernel.

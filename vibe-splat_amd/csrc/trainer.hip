@@ -1194,17 +1194,27 @@ void Trainer::enqueue_view_passes(uint32_t n, const vsplat_camera *cameras, cons
     if (b.stream != stream_) check(hipStreamWaitEvent(b.stream, batchStart_, 0), "hipStreamWaitEvent");
     params[k] = view_params(cameras[k]);
     enqueue_passes(b, params[k], targets[k], kModeGradients, nullptr, false, false);
-    uint8_t *slot = viewStats_.as<uint8_t>() + static_cast<size_t>(k) * kViewStatsBytes;
-    check(hipMemcpyAsync(slot, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToDevice, b.stream), "view counters");
-    check(hipMemcpyAsync(slot + kCounterWords * 4, b.sums.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToDevice, b.stream),
-          "view sums");
     check(hipEventRecord(b.done, b.stream), "hipEventRecord");
     queuedViewSizes_.emplace_back(cameras[k].width, cameras[k].height);
   }
+  for (uint32_t k = 1; k < n; ++k) check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
+  gather_view_status(n);
+}
+
+// Once every view's passes are through (on the trainer's stream): each view's
+// counters and loss sums into its stats slot, and the views' sticky flags into
+// the first view's counters, the batch's status.
+void Trainer::gather_view_status(uint32_t n) {
   uint32_t *status = views_[0]->counters.as<uint32_t>();
-  for (uint32_t k = 1; k < n; ++k) {
-    check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
-    launch_merge_flags(stream_, status, views_[k]->counters.as<uint32_t>());
+  for (uint32_t first = 0; first < n; first += kStatusViewsPerLaunch) {
+    ViewStatus v{};
+    v.first = first;
+    v.count = std::min(n - first, kStatusViewsPerLaunch);
+    for (uint32_t k = 0; k < v.count; ++k) {
+      v.counters[k] = views_[first + k]->counters.as<uint32_t>();
+      v.sums[k] = views_[first + k]->sums.as<double>();
+    }
+    launch_gather_views(stream_, v, viewStats_.as<uint8_t>(), status);
   }
 }
 
@@ -1242,17 +1252,10 @@ void Trainer::enqueue_view_passes_one_launch(uint32_t n, const vsplat_camera *ca
     ViewBuffers &b = *views_[k];
     if (b.stream != stream_) check(hipStreamWaitEvent(b.stream, batchRaster_, 0), "hipStreamWaitEvent");
     enqueue_loss(b, with_tiles(b, params[k]), targets[k]);
-    uint8_t *slot = viewStats_.as<uint8_t>() + static_cast<size_t>(k) * kViewStatsBytes;
-    check(hipMemcpyAsync(slot, b.counters.as<uint32_t>(), kCounterWords * 4, hipMemcpyDeviceToDevice, b.stream), "view counters");
-    check(hipMemcpyAsync(slot + kCounterWords * 4, b.sums.as<double>(), 2 * sizeof(double), hipMemcpyDeviceToDevice, b.stream),
-          "view sums");
     check(hipEventRecord(b.done, b.stream), "hipEventRecord");
   }
-  uint32_t *status = views_[0]->counters.as<uint32_t>();
-  for (uint32_t k = 1; k < n; ++k) {
-    check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
-    launch_merge_flags(stream_, status, views_[k]->counters.as<uint32_t>());
-  }
+  for (uint32_t k = 1; k < n; ++k) check(hipStreamWaitEvent(stream_, views_[k]->done, 0), "hipStreamWaitEvent");
+  gather_view_status(n);
   launch_batch_raster(n, params, true);
 }
 

@@ -193,6 +193,7 @@ private:
   void upload_cloud(const vsplat_cloud &cloud, DeviceArray *const values[5]) const;
   void allocate_per_gaussian();
   void ensure_size(ViewBuffers &b, int width, int height);
+  void gather_view_status(uint32_t n);
   void ensure_layout(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH);
   static void set_geometry(ViewBuffers &b, int width, int height, uint32_t tileW, uint32_t tileH);
   std::pair<uint32_t, uint32_t> tile_shape_for(int width, int height) const;
@@ -248,7 +249,7 @@ private:
   DeviceArray poseTerms_, poseTotals_; // 6 floats per Gaussian, and per group of 256
   bool accumulateNext_ = false; // the next chain pass (gradients mode) adds to the gradient sums
   // Per queued view of a batch: its counters and loss sums, copied on the device.
-  static constexpr size_t kViewStatsBytes = 48;
+  static constexpr size_t kViewStatsBytes = kViewStatusBytes;
   std::vector<std::pair<int, int>> queuedViewSizes_;
   ViewParams lastView_{};   // the camera of the last view that ran (width 0: none)
   size_t lastViewIndex_ = 0; // whose buffers hold its lists, image and sums
