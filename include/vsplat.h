@@ -371,8 +371,9 @@ int vsplat_autotune_views(vsplat_trainer *trainer, uint32_t n, const vsplat_came
  * "image" (16 bytes per pixel: rgb, log T), "stops", "adjoint" (16 per pixel),
  * "fields" (36 per pixel), "tile_totals" (8 per tile), "block_work" (per
  * raster launch index, 4 bytes: the entries the backward walks for that 8x8
- * block), "block_order" (the backward's launch order: a permutation of those
- * indices, heaviest first on each XCD), "raw" (14 int64 per
+ * block), "block_order" and "list_order" (the backward's and the forward's
+ * launch orders: permutations of those indices, heaviest first on each XCD,
+ * by block work and by tile list length), "raw" (14 int64 per
  * Gaussian with fixed point, else 14 float), "counters" (8 words: visible splats, entries, entries listed, big splats, big-splat bit words, overflow, skipped steps, unused),
  * "state_positions", "state_scales", "state_rotations", "state_opacities",
  * "state_sh" (Adam's state, one word per parameter: bf16 m low, bf16 v high),

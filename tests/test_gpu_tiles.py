@@ -86,25 +86,29 @@ def _check_order(order, keys):
 
 
 @pytest.mark.parametrize("shape", [vsplat.TILES_16X16, vsplat.TILES_8X8, vsplat.TILES_32X8, vsplat.TILES_32X32])
-def test_backward_runs_each_xcds_blocks_heaviest_first(shape):
-    """The backward's launch order runs each XCD's blocks by non-increasing
-    work, the work of a block being the entries from its pixels' lowest stop
-    to its tile list's end, recomputed here from the ranges and stops."""
+def test_raster_passes_run_each_xcds_blocks_heaviest_first(shape):
+    """The raster passes' launch orders. The backward's runs each XCD's blocks
+    by non-increasing work, the work of a block being the entries from its
+    pixels' lowest stop to its tile list's end, recomputed here from the ranges
+    and stops; the forward's by non-increasing length of the block's tile list."""
     w = F.config(1, count=20_000)
     cam = w.cameras[0]
     tw, th = SHAPES.get(shape, (16, 16))
     with vsplat.Trainer(w.cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=shape)) as t:
         t.compute_gradients(cam, w.targets[0])
-        work, order = t.inspect("block_work"), t.inspect("block_order")
+        work, order, list_order = t.inspect("block_work"), t.inspect("block_order"), t.inspect("list_order")
         ranges, stops = t.inspect("ranges"), t.inspect("stops").reshape(cam.height, cam.width)
     grid_w, grid_h = (cam.width + tw - 1) // tw, (cam.height + th - 1) // th
     tiles, across, per_tile = grid_w * grid_h, tw // 8, (tw // 8) * (th // 8)
     items = (tiles + 7) // 8 * 8 * per_tile
     assert len(work) == items
     expected = np.zeros(items, np.int64)
+    length = np.zeros(items, np.int64)
     for h in range(items):
         u = h // 8
         tile, q = (u // per_tile) * 8 + h % 8, u % per_tile
+        if tile < tiles:
+            length[h] = int(ranges[tile][1]) - int(ranges[tile][0])
         x0, y0 = (tile % grid_w) * tw + (q % across) * 8, (tile // grid_w) * th + (q // across) * 8
         if tile >= tiles or x0 >= cam.width or y0 >= cam.height:
             continue
@@ -112,6 +116,7 @@ def test_backward_runs_each_xcds_blocks_heaviest_first(shape):
         expected[h] = end - min(int(stops[y0:y0 + 8, x0:x0 + 8].min()), end)
     np.testing.assert_array_equal(work, expected)
     _check_order(order, _heavy_first(work))
+    _check_order(list_order, _heavy_first(length))
     print(f"{tw}x{th}: {items} blocks, heaviest {work.max()} entries, median {int(np.median(work))}", flush=True)
 
 def _check_lists(cloud, cam, got, stats, shape):

@@ -183,10 +183,11 @@ void launch_backward_views(hipStream_t stream, bool fixedPoint, bool recompute, 
 // entry's tile (low tileBits bits) and blocks, entryValues its Gaussian.
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryKeys, const uint32_t *entryValues, uint32_t tileBits,
                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
-                    float4 *image, uint32_t *stops, uint32_t *work = nullptr);
+                    float4 *image, uint32_t *stops, uint32_t *work = nullptr, const uint32_t *order = nullptr);
 // (stops may be null: not kept when the backward recomputes them. work, when
 // given, receives per launch index the entries the backward will walk for
-// that 8x8 block, raster_blocks(tiles) of them.)
+// that 8x8 block, raster_blocks(tiles) of them. order: the blocks' launch
+// order from launch_list_order, or null for raster_blocks' own.)
 // recompute: the transmittance and stops are recomputed per tile rather than
 // read from the forward pass (image and stops may then be null). edge: a
 // scored view's edge weights (the adjoint's fourth value its marks), which
@@ -200,6 +201,9 @@ void launch_backward(hipStream_t stream, bool fixedPoint, bool recompute, const 
 // (each block keeps its XCD): a permutation of the items = raster_blocks(tiles)
 // launch indices.
 void launch_work_order(hipStream_t stream, const uint32_t *work, uint32_t items, uint32_t *order);
+// The forward's launch order, heaviest first on each XCD by the length of each
+// block's tile list.
+void launch_list_order(hipStream_t stream, const uint2 *ranges, uint32_t tiles, uint32_t tileW, uint32_t tileH, uint32_t *order);
 
 // errorMax and edge: when scoring, the view's worst error (cleared before)
 // and the target's edge weights, and the adjoint's fourth value the pixel's

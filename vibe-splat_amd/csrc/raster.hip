@@ -288,11 +288,13 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const Splat *__restrict
                                                          const uint32_t *__restrict__ entryValues, uint32_t tileBits,
                                                          const uint2 *__restrict__ ranges, uint32_t width, uint32_t height,
                                                          uint32_t gridW, float bgR, float bgG, float bgB, float4 *__restrict__ image,
-                                                         uint32_t *__restrict__ stops, uint32_t *__restrict__ work) {
+                                                         uint32_t *__restrict__ stops, uint32_t *__restrict__ work,
+                                                         const uint32_t *__restrict__ order) {
   __shared__ Staged batch[kBlock];
   __shared__ uint32_t queue[kBlock];
-  forward_block<TW, TH>(blockIdx.x, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR, bgG, bgB, image, stops,
-                        work, batch, queue);
+  const uint32_t item = order != nullptr ? order[blockIdx.x] : blockIdx.x;
+  forward_block<TW, TH>(item, splats, entryKeys, entryValues, tileBits, ranges, width, height, gridW, bgR, bgG, bgB, image, stops, work,
+                        batch, queue);
 }
 
 // The view of a batched launch that block `item` belongs to, and the block's
@@ -616,14 +618,14 @@ __device__ __forceinline__ uint32_t key_place(uint32_t *counts, uint32_t key, bo
   return static_cast<uint32_t>(__shfl(static_cast<int>(base), leader, 64)) + lower;
 }
 
-// The backward's launch order, heaviest first: one workgroup per XCD x sorts
-// the launch indices h = 8 i + x of XCD x by the forward's work (a counting
-// sort over kOrderKeys keys in LDS) and gives the j-th to launch index 8 j + x,
-// which runs on XCD x too (so a tile's blocks still share one L2). Items of one
-// key land in any order, which changes nothing: each block's sums are added
-// with atomics. `items` is a multiple of 8 (raster_blocks).
-__global__ __launch_bounds__(kOrderThreads) void work_order_kernel(const uint32_t *__restrict__ work, uint32_t items,
-                                                                    uint32_t *__restrict__ order) {
+// A raster launch order, heaviest first: one workgroup per XCD x sorts the
+// launch indices h = 8 i + x of XCD x by key(h) (a counting sort over
+// kOrderKeys keys in LDS) and gives the j-th to launch index 8 j + x, which
+// runs on XCD x too (so a tile's blocks still share one L2). Items of one key
+// land in any order, which changes nothing: each block's pixels are its own
+// and its sums are added with atomics. `items` is a multiple of 8 (raster_blocks).
+template <class Key>
+__device__ __forceinline__ void xcd_order(const Key &key, uint32_t items, uint32_t *__restrict__ order) {
   __shared__ uint32_t counts[kOrderKeys];
   __shared__ uint32_t waveTotals[kOrderThreads / 64];
   __shared__ uint8_t staged[kOrderStaged];
@@ -634,9 +636,9 @@ __global__ __launch_bounds__(kOrderThreads) void work_order_kernel(const uint32_
   for (uint32_t start = 0; start < n; start += kOrderThreads) {
     const uint32_t i = start + threadIdx.x;
     const bool valid = i < n;
-    const uint32_t key = valid ? heavy_first(work[i * kRasterXcds + x]) : 0u;
-    if (valid && i < kOrderStaged) staged[i] = static_cast<uint8_t>(key);
-    (void)key_place(counts, key, valid);
+    const uint32_t k = valid ? key(i * kRasterXcds + x) : 0u;
+    if (valid && i < kOrderStaged) staged[i] = static_cast<uint8_t>(k);
+    (void)key_place(counts, k, valid);
   }
   __syncthreads();
   const uint32_t c = threadIdx.x < kOrderKeys ? counts[threadIdx.x] : 0u;
@@ -647,16 +649,52 @@ __global__ __launch_bounds__(kOrderThreads) void work_order_kernel(const uint32_
   for (uint32_t start = 0; start < n; start += kOrderThreads) {
     const uint32_t i = start + threadIdx.x;
     const bool valid = i < n;
-    const uint32_t key = !valid ? 0u : i < kOrderStaged ? staged[i] : heavy_first(work[i * kRasterXcds + x]);
-    const uint32_t j = key_place(counts, key, valid);
+    const uint32_t k = !valid ? 0u : i < kOrderStaged ? staged[i] : key(i * kRasterXcds + x);
+    const uint32_t j = key_place(counts, k, valid);
     if (valid) order[j * kRasterXcds + x] = i * kRasterXcds + x;
   }
+}
+
+// The backward's key: the entries the forward found that block walks.
+struct WorkKey {
+  const uint32_t *work;
+  __device__ uint32_t operator()(uint32_t h) const { return heavy_first(work[h]); }
+};
+
+// The forward's key: the length of the block's tile list (block_geometry's
+// mapping from launch index to tile), which the forward's walk follows.
+struct ListKey {
+  const uint2 *ranges;
+  uint32_t tiles, perTile;
+  __device__ uint32_t operator()(uint32_t h) const {
+    const uint32_t tile = (h / kRasterXcds / perTile) * kRasterXcds + h % kRasterXcds;
+    if (tile >= tiles) return kOrderKeys - 1u;
+    const uint2 r = ranges[tile];
+    return heavy_first(r.y - r.x);
+  }
+};
+
+__global__ __launch_bounds__(kOrderThreads) void work_order_kernel(const uint32_t *__restrict__ work, uint32_t items,
+                                                                    uint32_t *__restrict__ order) {
+  xcd_order(WorkKey{work}, items, order);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void list_order_kernel(const uint2 *__restrict__ ranges, uint32_t tiles, uint32_t perTile,
+                                                                    uint32_t items, uint32_t *__restrict__ order) {
+  xcd_order(ListKey{ranges, tiles, perTile}, items, order);
 }
 } // namespace
 
 void launch_work_order(hipStream_t stream, const uint32_t *work, uint32_t items, uint32_t *order) {
   if (items == 0) return;
   hipLaunchKernelGGL(work_order_kernel, dim3(kRasterXcds), dim3(kOrderThreads), 0, stream, work, items, order);
+}
+
+void launch_list_order(hipStream_t stream, const uint2 *ranges, uint32_t tiles, uint32_t tileW, uint32_t tileH, uint32_t *order) {
+  const uint32_t items = raster_blocks(tiles, tileW, tileH);
+  if (items == 0) return;
+  hipLaunchKernelGGL(list_order_kernel, dim3(kRasterXcds), dim3(kOrderThreads), 0, stream, ranges, tiles, (tileW / 8u) * (tileH / 8u), items,
+                     order);
 }
 
 
@@ -674,11 +712,12 @@ void launch_work_order(hipStream_t stream, const uint32_t *work, uint32_t items,
 
 void launch_forward(hipStream_t stream, const Splat *splats, const uint32_t *entryKeys, const uint32_t *entryValues, uint32_t tileBits,
                     const uint2 *ranges, uint32_t width, uint32_t height, uint32_t tileW, uint32_t tileH, const float background[3],
-                    float4 *image, uint32_t *stops, uint32_t *work) {
+                    float4 *image, uint32_t *stops, uint32_t *work, const uint32_t *order) {
   const uint32_t gridW = (width + tileW - 1) / tileW, gridH = (height + tileH - 1) / tileH;
 #define VSPLAT_FORWARD(W, H)                                                                                                       \
   hipLaunchKernelGGL((forward_kernel<W, H>), dim3(raster_blocks(gridW * gridH, W, H)), dim3(kBlock), 0, stream, splats, entryKeys,   \
-                     entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, stops, work)
+                     entryValues, tileBits, ranges, width, height, gridW, background[0], background[1], background[2], image, stops, work,  \
+                     order)
   VSPLAT_TILE_DISPATCH(tileW, tileH, VSPLAT_FORWARD)
 #undef VSPLAT_FORWARD
 }

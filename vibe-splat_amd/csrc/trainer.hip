@@ -332,6 +332,7 @@ void Trainer::ensure_layout(ViewBuffers &b, int width, int height, uint32_t tile
   b.ranges.ensure(static_cast<size_t>(b.tiles) * 8);
   b.tileTotals.ensure(static_cast<size_t>(b.lossTiles) * 8);
   const size_t blocks = raster_blocks(b.tiles, tileW, tileH);
+  b.listOrder.ensure(blocks * 4);
   b.blockWork.ensure(blocks * 4);
   b.blockOrder.ensure(blocks * 4);
 }
@@ -657,12 +658,15 @@ void Trainer::enqueue_passes(ViewBuffers &b, const ViewParams &view, const uint8
   if (timed) begin_timing(true);
   const ViewParams v = with_tiles(b, view);
   enqueue_lists(b, v, mode, true);
-  // The backward's 8x8 blocks launch heaviest first on their XCD, by the work
-  // the forward found for each.
+  // Each pass's 8x8 blocks launch heaviest first on their XCD: the forward's by
+  // their tiles' list lengths, the backward's by the work the forward found.
   const bool backward = mode != kModeRender;
+  uint32_t *listOrder = b.listOrder.as<uint32_t>(); // none in the allocation-free render
+  if (listOrder != nullptr) launch_list_order(b.stream, b.ranges.as<uint2>(), b.tiles, b.tileW, b.tileH, listOrder);
   launch_forward(b.stream, b.splats.as<Splat>(), b.entryKeys.as<uint32_t>(), b.entryValues.as<uint32_t>(), b.tileBits, b.ranges.as<uint2>(),
                  v.width, v.height, b.tileW, b.tileH, options_.background, b.image.as<float4>(),
-                 options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>(), backward ? b.blockWork.as<uint32_t>() : nullptr);
+                 options_.recompute_transmittance != 0 ? nullptr : b.stops.as<uint32_t>(), backward ? b.blockWork.as<uint32_t>() : nullptr,
+                 listOrder);
   if (backward) launch_work_order(b.stream, b.blockWork.as<uint32_t>(), raster_blocks(b.tiles, b.tileW, b.tileH), b.blockOrder.as<uint32_t>());
   mark("composite");
   if (!backward) return;
@@ -1382,7 +1386,7 @@ uint64_t Trainer::capture_key(uint32_t n) const {
     const ViewBuffers &b = *views_[k];
     for (const DeviceArray *a : {&b.splats, &b.keys, &b.values, &b.tmpKeys, &b.tmpValues, &b.coverage, &b.tileCount, &b.tileRecords,
                                  &b.bigList, &b.bigBits, &b.rawFixed, &b.rawFloat, &b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues, &b.image,
-                                 &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.blockWork, &b.blockOrder, &b.sortScratch,
+                                 &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.listOrder, &b.blockWork, &b.blockOrder, &b.sortScratch,
                                  &b.scanScratch, &b.counters, &b.sums})
       array(*a);
     mix(b.entryCapacity);
@@ -1769,6 +1773,7 @@ void Trainer::inspect(const std::string &name, void *out, uint64_t capacity, uin
       {"tile_totals", &b.tileTotals, static_cast<uint64_t>(b.lossTiles) * 8},
       {"block_work", &b.blockWork, static_cast<uint64_t>(raster_blocks(b.tiles, b.tileW, b.tileH)) * 4},
       {"block_order", &b.blockOrder, static_cast<uint64_t>(raster_blocks(b.tiles, b.tileW, b.tileH)) * 4},
+      {"list_order", &b.listOrder, static_cast<uint64_t>(raster_blocks(b.tiles, b.tileW, b.tileH)) * 4},
       {"raw", fixedPoint_ ? &b.rawFixed : &b.rawFloat, n * kRawPerSplat * (fixedPoint_ ? 8u : 4u)},
       {"counters", &b.counters, kCounterWords * 4},
       {"state_positions", &state_[0], n * 12},

@@ -103,7 +103,7 @@ struct ViewBuffers {
   DeviceArray entryKeys, entryValues, tmpEntryKeys, tmpEntryValues;
   // Per pixel and per tile; per 8x8 block (raster launch index), the backward's
   // work found by the forward and the backward's launch order.
-  DeviceArray image, stops, adjoint, fields, ranges, tileTotals, blockWork, blockOrder;
+  DeviceArray image, stops, adjoint, fields, ranges, tileTotals, listOrder, blockWork, blockOrder;
   // Exposure and scoring (used only when set): the exposed image, the target's
   // edge weights, the exposure backward's workgroup totals, the worst error.
   DeviceArray exposed, edge, exposureTotals, errorMax;

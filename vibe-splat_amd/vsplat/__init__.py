@@ -292,7 +292,7 @@ def camera_to_c(cam: Camera) -> CCamera:
 INSPECT_TYPES = {"splats": (np.float32, 12), "keys": (np.uint32, 1), "values": (np.uint32, 1), "coverage": (np.uint32, 1),
                  "entry_keys": (np.uint32, 1), "entry_blocks": (np.uint32, 1), "entry_values": (np.uint32, 1), "ranges": (np.uint32, 2),
                  "image": (np.float32, 4), "stops": (np.uint32, 1), "adjoint": (np.float32, 4), "fields": (np.float32, 9),
-                 "tile_totals": (np.float32, 2), "block_work": (np.uint32, 1), "block_order": (np.uint32, 1), "counters": (np.uint32, 1),
+                 "tile_totals": (np.float32, 2), "block_work": (np.uint32, 1), "block_order": (np.uint32, 1), "list_order": (np.uint32, 1), "counters": (np.uint32, 1),
                  "state_positions": (np.uint32, 3), "state_scales": (np.uint32, 3), "state_rotations": (np.uint32, 4),
                  "state_opacities": (np.uint32, 1), "state_sh": (np.uint32, 1), "stats": (np.float32, 8)}
 
