@@ -1722,8 +1722,9 @@ uint64_t Trainer::gpu_bytes() const {
   uint64_t total = bytes_per_gaussian() * count_ + target_.bytes() + viewStats_.bytes();
   for (size_t k = 0; k < views_.size(); ++k) {
     const ViewBuffers &b = *views_[k];
-    for (const DeviceArray *a : {&b.image, &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.entryKeys, &b.entryValues,
-                                 &b.tmpEntryKeys, &b.tmpEntryValues, &b.bigBits, &b.exposed, &b.edge, &b.exposureTotals, &b.errorMax}) {
+    for (const DeviceArray *a : {&b.image, &b.stops, &b.adjoint, &b.fields, &b.ranges, &b.tileTotals, &b.listOrder, &b.blockWork,
+                                 &b.blockOrder, &b.entryKeys, &b.entryValues, &b.tmpEntryKeys, &b.tmpEntryValues, &b.bigBits, &b.exposed,
+                                 &b.edge, &b.exposureTotals, &b.errorMax}) {
       total += a->bytes();
     }
     if (k == 0) continue;
