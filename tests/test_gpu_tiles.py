@@ -85,17 +85,23 @@ def _check_order(order, keys):
     assert (np.diff(keys[order].reshape(-1, 8), axis=0) >= 0).all()
 
 
-@pytest.mark.parametrize("shape", [vsplat.TILES_16X16, vsplat.TILES_8X8, vsplat.TILES_32X8, vsplat.TILES_32X32])
-def test_raster_passes_run_each_xcds_blocks_heaviest_first(shape):
+@pytest.mark.parametrize("shape,large", [(vsplat.TILES_16X16, False), (vsplat.TILES_8X8, False), (vsplat.TILES_32X8, False),
+                                         (vsplat.TILES_32X32, False), (vsplat.TILES_8X8, True)])
+def test_raster_passes_run_each_xcds_blocks_heaviest_first(shape, large):
     """The raster passes' launch orders. The backward's runs each XCD's blocks
     by non-increasing work, the work of a block being the entries from its
     pixels' lowest stop to its tile list's end, recomputed here from the ranges
-    and stops; the forward's by non-increasing length of the block's tile list."""
+    and stops; the forward's by non-increasing length of the block's tile list.
+    The large case (a 3072x3072 view of 8x8 tiles: 18432 blocks per XCD) has
+    more blocks per XCD than the order kernels keep in LDS between their passes."""
     w = F.config(1, count=20_000)
-    cam = w.cameras[0]
+    cam, target = w.cameras[0], w.targets[0]
+    if large:
+        cam = F.look_at((0, 0, -3), (0, 0, 0), (0.0, -1.0, 0.0), 60.0, 3072, 3072)
+        target = F.smooth_target(3072, 3072, 29)
     tw, th = SHAPES.get(shape, (16, 16))
     with vsplat.Trainer(w.cloud, vsplat.TrainerOptions(force_fixed_point=True, tile_shape=shape)) as t:
-        t.compute_gradients(cam, w.targets[0])
+        t.compute_gradients(cam, target)
         work, order, list_order = t.inspect("block_work"), t.inspect("block_order"), t.inspect("list_order")
         ranges, stops = t.inspect("ranges"), t.inspect("stops").reshape(cam.height, cam.width)
     grid_w, grid_h = (cam.width + tw - 1) // tw, (cam.height + th - 1) // th
