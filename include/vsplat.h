@@ -401,8 +401,10 @@ void vsplat_device_free(void *device);
 
 /* Runs one of the device's elementwise functions over n floats (device
  * arithmetic check): 0 exp, 1 log, 2 sqrt, 3 reciprocal (1/x), 4 float to half
- * and back, 5 the half in the low 16 bits of the input's bits to float. in and
- * out are host arrays. */
+ * and back, 5 the half in the low 16 bits of the input's bits to float, 6 the
+ * backward's short division of pairs: out[2k] = in[2k] / in[2k + 1] for
+ * 2^-100 <= in[2k] <= 2^60 and 2^-20 <= in[2k + 1] <= 1 (outside that range
+ * the result is undefined), out[2k + 1] = 0. in and out are host arrays. */
 int vsplat_math_probe(int32_t function, const float *in, float *out, uint64_t n);
 
 #ifdef __cplusplus

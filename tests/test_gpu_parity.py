@@ -74,6 +74,28 @@ def test_device_math_is_bit_exact():
     assert ((_bits(got) == _bits(want)) | (np.isnan(got) & np.isnan(want))).all()
 
 
+def test_backward_division_is_ieee_division():
+    """The backward's division without the hardware's range scaling equals float32 a / b over the
+    range it is used in (IEEE division rounds correctly, numpy's too)."""
+    rng = np.random.default_rng(2)
+    m = 1 << 20
+    # The backward's own operands (transmittance in [1e-4, 1], 1 - alpha in [0.01, 1]), then the
+    # whole stated range, then the ends and mantissas of all ones next to ones.
+    a = [rng.uniform(1e-4, 1.0, m), 2.0 ** rng.uniform(-100, 60, m)]
+    b = [rng.uniform(0.01, 1.0, m), 2.0 ** rng.uniform(-20, 0, m)]
+    ends_a = np.array([2.0 ** -100, 2.0 ** 60, 1e-4, 0.5, 1.0 - 2.0 ** -24, 1.0, 1.0 + 2.0 ** -23, 3.0, 0.1], np.float32)
+    ends_b = np.array([2.0 ** -20, 0.01, np.nextafter(np.float32(0.01), np.float32(1)), 1.0 - 1.0 / 255.0, 0.75,
+                       1.0 - 2.0 ** -23, 1.0 - 2.0 ** -24, 1.0], np.float32)
+    ea, eb = np.meshgrid(ends_a, ends_b)
+    a = np.concatenate(a + [ea.ravel()]).astype(np.float32)
+    b = np.concatenate(b + [eb.ravel()]).astype(np.float32)
+    got = vsplat.math_probe(6, np.stack([a, b], 1).ravel())
+    want = a / b
+    same = _bits(got[0::2]) == _bits(want)
+    assert same.all(), f"{np.count_nonzero(~same)} differ, first at a={a[~same][:4]} b={b[~same][:4]}"
+    assert (got[1::2] == 0).all()
+
+
 def _run_both(name, fixed=True, lam=0.2, sh_storage=0):
     cloud, cam, target = _case(name)
     model = O.Model(cloud, cam, sh_storage=sh_storage)

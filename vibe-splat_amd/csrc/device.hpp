@@ -81,6 +81,22 @@ __device__ __forceinline__ float minf(float a, float b) { return b < a ? b : a; 
 __device__ __forceinline__ float maxf(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// a / b correctly rounded, for 2^-100 <= a <= 2^60 and 2^-20 <= b <= 1: the
+// hardware's IEEE division sequence (reciprocal, one Newton step on it, the
+// quotient and two residual corrections by fma) without V_DIV_SCALE and
+// V_DIV_FMAS's range scaling and V_DIV_FIXUP's special cases. V_DIV_SCALE
+// scales only a denominator that is denormal or has a denormal reciprocal, a
+// numerator at or below 2^-103, or a quotient at or above 2^96 or denormal;
+// V_DIV_FIXUP changes only NaN, infinite, zero or out-of-range results. In the
+// range above none of these occur, so the result is `a / b` bit for bit.
+__device__ __forceinline__ float div_in_range(float a, float b) {
+  const float r0 = __builtin_amdgcn_rcpf(b);
+  const float r = fma32(fma32(-b, r0, 1.0f), r0, r0);
+  const float q0 = a * r;
+  const float q1 = fma32(fma32(-b, q0, a), r, q0);
+  return fma32(fma32(-b, q1, a), r, q1);
+}
+
 // e^x: x = n ln2 + r, |r| <= ln2 / 2 (ln2 split so that n ln2_hi is exact),
 // e^r = 1 + r + r^2 q(r) with q a degree-4 near-minimax fit of (e^r - 1 - r) / r^2,
 // then 2^n applied as two multiplications so that overflow to +inf and gradual

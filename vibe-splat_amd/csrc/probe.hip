@@ -11,6 +11,11 @@ __global__ void math_probe_kernel(int function, const float *__restrict__ in, fl
   if (i >= n) return;
   const float x = in[i];
   float y = 0.0f;
+  if (function == 6) {
+    // Pairs: the quotient of (in[2k], in[2k + 1]) goes to out[2k], out[2k + 1] is 0.
+    out[i] = (i & 1u) != 0 || i + 1 >= n ? 0.0f : div_in_range(x, in[i + 1]);
+    return;
+  }
   switch (function) {
   case 0: y = vs_exp(x); break;
   case 1: y = vs_log(x); break;
@@ -29,7 +34,7 @@ void check(hipError_t e, const char *what) {
 } // namespace
 
 void math_probe(int function, const float *in, float *out, uint64_t n) {
-  if (function < 0 || function > 5) throw Error(VSPLAT_INVALID_ARGUMENT, "math probe functions are 0 to 5");
+  if (function < 0 || function > 6) throw Error(VSPLAT_INVALID_ARGUMENT, "math probe functions are 0 to 6");
   if (n == 0) return;
   DeviceArray a, b;
   a.ensure(n * 4);

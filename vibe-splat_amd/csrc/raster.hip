@@ -491,7 +491,10 @@ __device__ __forceinline__ void backward_block(uint32_t item, const Splat *__res
       const bool front = !(power > 0.0f), visible = alpha >= kMinAlpha;
       if ((candidates & vote(front) & vote(visible)) == 0) continue;
       const bool ok = candidate && front && visible;
-      const float T = Tbehind / (1.0f - alpha); // alpha <= 0.99: never a division by zero
+      // Used where ok only: there 0.01 <= 1 - alpha <= 1 and the transmittance
+      // is at least the pixel's final one, which the forward keeps at or above
+      // 1e-4, so the short division is exact `Tbehind / (1 - alpha)`.
+      const float T = div_in_range(Tbehind, 1.0f - alpha);
       // A lane that takes nothing from the splat carries zero factors, so its
       // values below are all zero (every other factor is finite); the sign of
       // a zero cannot change a total that is not zero.

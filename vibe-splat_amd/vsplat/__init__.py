@@ -678,7 +678,8 @@ class DeviceBuffer:
 
 
 def math_probe(function: int, x: np.ndarray) -> np.ndarray:
-    """The device's exp (0), log (1), sqrt (2), reciprocal (3), half round trip (4) or half bits to float (5)."""
+    """The device's exp (0), log (1), sqrt (2), reciprocal (3), half round trip (4), half bits to float (5) or
+    the backward's division of pairs (6: out[2k] = x[2k] / x[2k + 1] in its stated range, out[2k + 1] = 0)."""
     a = np.ascontiguousarray(x, np.float32)
     out = np.empty_like(a)
     _raise(lib().vsplat_math_probe(function, _p(a), _p(out), a.size), "math_probe")
