@@ -1,14 +1,17 @@
 // Loss and its derivative per pixel, L = (1 - lambda) L1 + lambda (1 - SSIM)
 // (train_ssim.comp:42-123, train_loss.comp:52-113). The 11x11 Gaussian window
-// (sigma 1.5) is applied separably: a horizontal pass over the tile's 16
-// columns and 26 rows (5 rows of halo above and below) into LDS, then a
-// vertical pass out of it. The window is cut at the image border and
+// (sigma 1.5) is applied separably: a workgroup takes a strip 16 columns wide
+// and TH rows tall, a horizontal pass over its 16 columns and TH + 10 rows (5
+// rows of halo above and below) into LDS, then a vertical pass out of it; the
+// taller the strip, the fewer halo rows are worked twice (26 per 16 output rows
+// at TH = 16, 74 per 64 at TH = 64). The window is cut at the image border and
 // renormalised there, as in the reference.
 //
 // Pass 1 writes per pixel the three fields P = dS/dmu - 2 mu dS/dvar - mu_y dS/dcov,
 // Q = dS/dvar, R = dS/dcov (each over the window's weight), 9 floats; pass 2
 // applies the same window to them to get dSSIM/dimage, and adds the L1 term.
-// Each pass also leaves its tile's total (SSIM, L1), summed in a fixed tree.
+// Each pass also leaves the total (SSIM, L1) of every 16x16 tile of its strip,
+// summed in a fixed tree over the tile's 256 pixels whatever the strip height.
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -32,8 +35,11 @@ __device__ __forceinline__ float full_norm() {
   for (int k = 0; k < 11; ++k) norm = norm + kTaps[k];
   return norm;
 }
-constexpr int kRows = 16 + 2 * kHalo;
-constexpr int kHalfPixels = kRows * 16;
+// Strip heights: the SSIM pass's 73 registers allow six waves per SIMD, which
+// 32 rows keep (8 waves and 40 KB of LDS a workgroup); the loss pass's 44 allow
+// eight at 64 rows (16 waves, 43 KB).
+constexpr int kSsimRows = 32, kLossRows = 64;
+template <int TH> constexpr int staged_pixels() { return (TH + 2 * kHalo) * 16; }
 
 __device__ __forceinline__ float window_norm(int at, int size) {
   float norm = 0.0f;
@@ -44,27 +50,38 @@ __device__ __forceinline__ float window_norm(int at, int size) {
 
 __device__ __forceinline__ float unorm8(uint32_t v) { return static_cast<float>(v) * (1.0f / 255.0f); }
 
-// The tile's total of one value per thread: xor butterfly within each wave
-// (lane l plus lane l ^ m, m = 32 ... 1), then the four waves in order.
+// The total of one value per thread over each group of 256 threads (a 16x16
+// tile): xor butterfly within each wave (lane l plus lane l ^ m, m = 32 ... 1),
+// then the group's four waves in order.
 __device__ __forceinline__ float tile_sum(float v, float *waveTotals) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, 64);
   if ((threadIdx.x & 63u) == 0) waveTotals[threadIdx.x >> 6] = v;
   __syncthreads();
-  const float total = ((waveTotals[0] + waveTotals[1]) + waveTotals[2]) + waveTotals[3];
+  const float *w = waveTotals + (threadIdx.x >> 8) * 4u;
+  const float total = ((w[0] + w[1]) + w[2]) + w[3];
   __syncthreads();
   return total;
 }
 
-__global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ image, const uchar4 *__restrict__ target, uint32_t width,
-                                                   uint32_t height, uint32_t gridW, float *__restrict__ fields,
-                                                   float2 *__restrict__ tileTotals) {
-  __shared__ float h[15][kHalfPixels];
-  __shared__ float waveTotals[4];
+// The 16x16 tile that thread t of the strip at (x0, y0) sums into, when that
+// tile lies in the image (its thread t & 255 == 0 stores it); else -1. Thread t
+// works pixel (x0 + (t & 15), y0 + (t >> 4)).
+__device__ __forceinline__ int strip_tile(int x0, int y0, int H, uint32_t gridW) {
+  const int ty = y0 / 16 + static_cast<int>(threadIdx.x >> 8);
+  return ty * 16 < H ? ty * static_cast<int>(gridW) + x0 / 16 : -1;
+}
+
+template <int TH>
+__global__ __launch_bounds__(16 * TH) void ssim_kernel(const float4 *__restrict__ image, const uchar4 *__restrict__ target,
+                                                       uint32_t width, uint32_t height, uint32_t gridW, float *__restrict__ fields,
+                                                       float2 *__restrict__ tileTotals) {
+  constexpr int kThreads = 16 * TH, kStaged = staged_pixels<TH>();
+  __shared__ float h[15][kStaged];
+  __shared__ float waveTotals[kThreads / 64];
   const int W = static_cast<int>(width), H = static_cast<int>(height);
-  const uint32_t tile = blockIdx.x;
-  const int x0 = static_cast<int>(tile % gridW) * 16, y0 = static_cast<int>(tile / gridW) * 16;
-  for (int i = static_cast<int>(threadIdx.x); i < kHalfPixels; i += 256) {
+  const int x0 = static_cast<int>(blockIdx.x % gridW) * 16, y0 = static_cast<int>(blockIdx.x / gridW) * TH;
+  for (int i = static_cast<int>(threadIdx.x); i < kStaged; i += kThreads) {
     const int row = i / 16, col = i % 16;
     const int y = y0 + row - kHalo, x = x0 + col;
     float acc[15];
@@ -157,7 +174,8 @@ __global__ __launch_bounds__(256) void ssim_kernel(const float4 *__restrict__ im
     }
   }
   const float total = tile_sum(own, waveTotals);
-  if (threadIdx.x == 0) tileTotals[tile].x = total;
+  const int tile = strip_tile(x0, y0, H, gridW);
+  if ((threadIdx.x & 255u) == 0 && tile >= 0) tileTotals[tile].x = total;
 }
 
 // The target's luminance at a pixel, clamped to the image (train_loss.comp:34-40):
@@ -185,16 +203,18 @@ __device__ __forceinline__ float target_edge(const uchar4 *__restrict__ target, 
 // mean error, errorMax the worst over the view in 1/65535 (one atomic per
 // wave; a maximum does not depend on the order), and edge the target's edge
 // weight (train_loss.comp:106-110). Otherwise the fourth value is 0.
-__global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ image, const uchar4 *__restrict__ target, uint32_t width,
-                                                   uint32_t height, uint32_t gridW, float lambda, const float *__restrict__ fields,
-                                                   float4 *__restrict__ adjoint, float2 *__restrict__ tileTotals,
-                                                   uint32_t *__restrict__ errorMax, float *__restrict__ edge) {
-  __shared__ float h[9][kHalfPixels];
-  __shared__ float waveTotals[4];
+template <int TH>
+__global__ __launch_bounds__(16 * TH) void loss_kernel(const float4 *__restrict__ image, const uchar4 *__restrict__ target,
+                                                       uint32_t width, uint32_t height, uint32_t gridW, float lambda,
+                                                       const float *__restrict__ fields, float4 *__restrict__ adjoint,
+                                                       float2 *__restrict__ tileTotals, uint32_t *__restrict__ errorMax,
+                                                       float *__restrict__ edge) {
+  constexpr int kThreads = 16 * TH, kStaged = staged_pixels<TH>();
+  __shared__ float h[9][kStaged];
+  __shared__ float waveTotals[kThreads / 64];
   const int W = static_cast<int>(width), H = static_cast<int>(height);
-  const uint32_t tile = blockIdx.x;
-  const int x0 = static_cast<int>(tile % gridW) * 16, y0 = static_cast<int>(tile / gridW) * 16;
-  for (int i = static_cast<int>(threadIdx.x); i < kHalfPixels; i += 256) {
+  const int x0 = static_cast<int>(blockIdx.x % gridW) * 16, y0 = static_cast<int>(blockIdx.x / gridW) * TH;
+  for (int i = static_cast<int>(threadIdx.x); i < kStaged; i += kThreads) {
     const int row = i / 16, col = i % 16;
     const int y = y0 + row - kHalo, x = x0 + col;
     float acc[9];
@@ -277,7 +297,8 @@ __global__ __launch_bounds__(256) void loss_kernel(const float4 *__restrict__ im
     if ((threadIdx.x & 63u) == 0 && worst != 0) atomicMax(errorMax, worst);
   }
   const float total = tile_sum(own, waveTotals);
-  if (threadIdx.x == 0) tileTotals[tile].y = total;
+  const int tile = strip_tile(x0, y0, H, gridW);
+  if ((threadIdx.x & 255u) == 0 && tile >= 0) tileTotals[tile].y = total;
 }
 
 // The exposed image (train_exposure.comp:17-24): rgb' = T (rgb, 1), each row's
@@ -371,10 +392,12 @@ __global__ __launch_bounds__(256) void total_kernel(const float2 *__restrict__ t
 void launch_loss(hipStream_t stream, const float4 *image, const uint8_t *target, uint32_t width, uint32_t height, float lambda,
                  float *fields, float4 *adjoint, float2 *tileTotals, double *sums, uint32_t *errorMax, float *edge) {
   const uint32_t gridW = (width + 15) / 16, tiles = gridW * ((height + 15) / 16);
+  const uint32_t ssimStrips = gridW * ((height + kSsimRows - 1) / kSsimRows), lossStrips = gridW * ((height + kLossRows - 1) / kLossRows);
   const uchar4 *t = reinterpret_cast<const uchar4 *>(target);
-  hipLaunchKernelGGL(ssim_kernel, dim3(tiles), dim3(256), 0, stream, image, t, width, height, gridW, fields, tileTotals);
-  hipLaunchKernelGGL(loss_kernel, dim3(tiles), dim3(256), 0, stream, image, t, width, height, gridW, lambda, fields, adjoint,
-                     tileTotals, errorMax, edge);
+  hipLaunchKernelGGL(ssim_kernel<kSsimRows>, dim3(ssimStrips), dim3(16 * kSsimRows), 0, stream, image, t, width, height, gridW, fields,
+                     tileTotals);
+  hipLaunchKernelGGL(loss_kernel<kLossRows>, dim3(lossStrips), dim3(16 * kLossRows), 0, stream, image, t, width, height, gridW, lambda,
+                     fields, adjoint, tileTotals, errorMax, edge);
   hipLaunchKernelGGL(total_kernel, dim3(1), dim3(256), 0, stream, tileTotals, tiles, sums);
 }
 
