@@ -28,6 +28,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "blockscan.hpp"
+#include "tiles.hpp"
 
 namespace vsplat {
 
@@ -47,6 +48,7 @@ constexpr uint32_t kBlock = 64; // pixels of an 8x8 block, threads of its workgr
 struct BlockGeometry {
   uint32_t x0, y0, x1, y1;
   uint32_t block; // the block's number in its tile (its bit in the entries' block masks)
+  bool reach;     // lists of tiles larger than the block: stage only the splats whose ellipse reaches it
 };
 
 // Workgroups are dealt to the chip's eight XCDs in turn (workgroup h runs on
@@ -65,7 +67,7 @@ __device__ __forceinline__ BlockGeometry block_geometry(uint32_t h, uint32_t gri
   tile = (u / kPerTile) * kRasterXcds + h % kRasterXcds;
   const uint32_t q = u % kPerTile;
   const uint32_t x0 = (tile % gridW) * TW + (q % kAcross) * 8u, y0 = (tile / gridW) * TH + (q / kAcross) * 8u;
-  return {x0, y0, x0 + 8u < width ? x0 + 8u : width, y0 + 8u < height ? y0 + 8u : height, q};
+  return {x0, y0, x0 + 8u < width ? x0 + 8u : width, y0 + 8u < height ? y0 + 8u : height, q, TW * TH > 64u};
 }
 
 // Heaviest first: a block's work (entries) as one of kOrderKeys keys, 8 per
@@ -80,9 +82,19 @@ __device__ __forceinline__ uint32_t heavy_first(uint32_t work) {
 }
 
 // The block's pixels a splat can touch: inside the extent box and within the
-// radius (|d| <= r on both axes, d from the pixel centre).
+// radius (|d| <= r on both axes, d from the pixel centre). A larger tile's list
+// holds every splat whose extent box meets the block (the block marks), and
+// about a quarter of those (elongated, turned splats) have boxes that meet the
+// block where their 1/255 ellipse does not: those are dropped here by the test
+// that makes the lists of 8x8 tiles (reaches_tile, with its slack against
+// rounding), on the same values, so that every tile shape stages the same splats
+// and none that could take a pixel of the block.
 __device__ __forceinline__ Staged stage(const Splat &s, const BlockGeometry &geo, uint32_t entry) {
-  const uint32_t box = splat_box(s, geo.x0, geo.y0, geo.x1, geo.y1);
+  uint32_t box = splat_box(s, geo.x0, geo.y0, geo.x1, geo.y1);
+  if (box != 0u && geo.reach &&
+      !reaches_tile(tile_test(s.qx, s.qy, s.qz, s.o, s.cx, s.cy), geo.x0 >> 3, geo.y0 >> 3, TileShape{8u, 8u})) {
+    box = 0u;
+  }
   uint32_t mask = 0;
   if (box != 0u) {
     const uint32_t bx0 = box & 0xFFu, by0 = (box >> 8) & 0xFFu, bx1 = (box >> 16) & 0xFFu, by1 = box >> 24;
